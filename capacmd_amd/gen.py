@@ -1,0 +1,443 @@
+"""Seeded generator of CapacMD input decks (init.gro, param.txt, mdset.txt).
+
+Systems are gold Mackay icosahedra solvated in rigid SPC/E water.  The files
+follow the grammar the reference reads:
+  * mdset.txt  key/value lines             (reference file.c:74-130)
+  * param.txt  sectioned force field       (reference file.c:286-562, 569-1023)
+  * init.gro   title, count, %5d%-5s%5s%5d + free-format numbers from column 20,
+               box line                    (reference file.c:138-203)
+Metal atoms come first (the reference's metal-metal CPIM force loop assumes
+the metal block starts at atom 0, cpff.c:762-764).  Atom and residue numbers
+wrap modulo 100000 so that the fixed-width columns stay aligned for systems
+of more than 99,999 atoms (the %5d fields of file.c:188-191,221).
+
+Units are the reference's MD units (nm, ps, kJ/mol, e, amu); CPIM parameters
+are in atomic units (bohr^3 and bohr), as param.txt expects.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+EV2KJMOL = 96.4853363969351          # kJ/mol per eV (reference typedef.h:70)
+NM2BOHR = 18.8972612456506           # bohr per nm (reference typedef.h:73)
+K_BOLTZ = 0.00831446214546895        # kJ/mol/K (reference typedef.h:77)
+
+# SPC/E water (Berendsen et al. 1987; GROMACS spce.itp values)
+SPCE_QO = -0.8476
+SPCE_QH = 0.4238
+SPCE_ROH = 0.1
+SPCE_HOH_DEG = 109.47
+SPCE_C6 = 2.6171e-3
+SPCE_C12 = 2.6331e-6
+M_O = 15.9994
+M_H = 1.008
+M_AU = 196.97
+
+# Au-O / Au-H erf-damped vdW (funct 4), the only metal-water pair terms that
+# ship with the reference (example_01/param.txt:17-18).
+AU_O_ERFVDW = (0.002522219544, 3.624859037184e-06, 2.25)
+AU_H_ERFVDW = (0.001028707002, 1.45541995368e-06, 2.25)
+
+# Sutton-Chen gold: n=10, m=8, eps=1.2793e-2 eV, c=34.408, a=4.08 A
+SC_N, SC_M = 10, 8
+SC_EPS = 1.2793e-2 * EV2KJMOL
+SC_C = 34.408
+SC_A = 0.408
+
+# CPIM gold: polarizability 1.5 A^3, capacitance 2.0 A, in atomic units
+CPIM_POLAR = 1.5 * (NM2BOHR / 10.0) ** 3
+CPIM_CAPAC = 2.0 * (NM2BOHR / 10.0)
+
+AU_NN = 0.288          # nm, nearest-neighbour distance along icosahedron edges
+WATER_DENSITY = 1.0    # g/cm^3
+AVOGADRO = 6.02214129e23
+
+
+def icosahedron_vertices() -> np.ndarray:
+    """Unit-edge icosahedron vertices (12 x 3) centred at the origin."""
+    phi = (1.0 + math.sqrt(5.0)) / 2.0
+    v = []
+    for s1 in (-1.0, 1.0):
+        for s2 in (-1.0, 1.0):
+            v.append((0.0, s1, s2 * phi))
+            v.append((s1, s2 * phi, 0.0))
+            v.append((s2 * phi, 0.0, s1))
+    v = np.array(v) / 2.0   # edge length 1
+    return v
+
+
+def icosahedron_faces(v: np.ndarray) -> List[Tuple[int, int, int]]:
+    n = len(v)
+    d = np.linalg.norm(v[:, None, :] - v[None, :, :], axis=-1)
+    faces = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            if abs(d[i, j] - 1.0) > 1e-9:
+                continue
+            for k in range(j + 1, n):
+                if abs(d[i, k] - 1.0) < 1e-9 and abs(d[j, k] - 1.0) < 1e-9:
+                    faces.append((i, j, k))
+    assert len(faces) == 20
+    return faces
+
+
+def mackay_icosahedron(shells: int, nn: float = AU_NN) -> np.ndarray:
+    """Mackay icosahedron with `shells` shells counting the central atom.
+
+    Shell k (k >= 1) is the surface of an icosahedron of edge k*nn, with its
+    edges and faces filled on a triangular grid of spacing nn: 10k^2+2 atoms.
+    Totals: 1, 13, 55, 147, 309, 561, 923, 1415, 2057, ...
+    """
+    v = icosahedron_vertices()
+    faces = icosahedron_faces(v)
+    pts = [np.zeros(3)]
+    for k in range(1, shells):
+        shell = {}
+        for (a, b, c) in faces:
+            A, B, C = v[a] * k * nn, v[b] * k * nn, v[c] * k * nn
+            for i in range(k + 1):
+                for j in range(k + 1 - i):
+                    p = A + (B - A) * (i / k) + (C - A) * (j / k)
+                    key = tuple(np.round(p / nn * 1e6).astype(np.int64))
+                    shell[key] = p
+        assert len(shell) == 10 * k * k + 2, (k, len(shell))
+        pts.extend(shell[key] for key in sorted(shell))
+    return np.array(pts)
+
+
+def random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random rotation matrices from random unit quaternions."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def water_template() -> np.ndarray:
+    """O, H1, H2 positions of a rigid SPC/E water with O at the origin."""
+    half = math.radians(SPCE_HOH_DEG) / 2.0
+    h1 = np.array([SPCE_ROH * math.sin(half), 0.0, SPCE_ROH * math.cos(half)])
+    h2 = np.array([-SPCE_ROH * math.sin(half), 0.0, SPCE_ROH * math.cos(half)])
+    return np.array([np.zeros(3), h1, h2])
+
+
+def spce_hh() -> float:
+    return 2.0 * SPCE_ROH * math.sin(math.radians(SPCE_HOH_DEG) / 2.0)
+
+
+@dataclasses.dataclass
+class Deck:
+    """A generated system: coordinates/velocities plus everything the files carry."""
+    name: str
+    box: np.ndarray                 # (3,) nm
+    metal: np.ndarray               # (n_metal, 3)
+    np_ranges: List[Tuple[int, int, float]]   # (first, last, total charge), metal indices
+    water: np.ndarray               # (n_water, 3, 3)
+    vel: np.ndarray                 # (n_atoms, 3)
+    n_steps: int
+    dt: float = 0.001
+    save_step: int = 10
+    efield: Tuple[float, float, float] = (0.0, 0.0, 0.0)   # V/nm
+    ref_temp: float = 300.0
+    tau_t: float = 0.1
+    use_cpff: int = 1
+    fix_metal: int = 0
+    ensemble: str = "nvt"
+    coulomb_alpha: float = 2.0
+    rcut: float = 1.2
+
+    @property
+    def n_metal(self) -> int:
+        return len(self.metal)
+
+    @property
+    def n_water(self) -> int:
+        return len(self.water)
+
+    @property
+    def n_atoms(self) -> int:
+        return self.n_metal + 3 * self.n_water
+
+    def positions(self) -> np.ndarray:
+        return np.concatenate([self.metal, self.water.reshape(-1, 3)], axis=0)
+
+    def masses(self) -> np.ndarray:
+        return np.concatenate([np.full(self.n_metal, M_AU),
+                               np.tile([M_O, M_H, M_H], self.n_water)])
+
+
+def fill_water(box: Sequence[float], metal: np.ndarray, rng: np.random.Generator,
+               exclusion: float = 0.25) -> np.ndarray:
+    """SPC/E waters on a cubic lattice at 1.0 g/cm^3 with random orientations.
+
+    Molecules with any atom within `exclusion` nm of a metal atom are removed.
+    """
+    box = np.asarray(box, dtype=float)
+    mw = M_O + 2 * M_H
+    n_density = WATER_DENSITY / mw * AVOGADRO * 1e-21      # molecules per nm^3
+    spacing = n_density ** (-1.0 / 3.0)
+    # a lattice at least as dense as the target, thinned by seeded vacancies
+    # so that the box holds the 1.0 g/cm^3 count before the metal is carved out
+    counts = np.maximum(1, np.ceil(box / spacing).astype(int))
+    grid = [(np.arange(c) + 0.5) * (b / c) for c, b in zip(counts, box)]
+    gx, gy, gz = np.meshgrid(*grid, indexing="ij")
+    centres = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+    target = int(round(n_density * float(np.prod(box))))
+    if target < len(centres):
+        pick = np.sort(rng.choice(len(centres), size=target, replace=False))
+        centres = centres[pick]
+    tmpl = water_template()
+    com = (tmpl * np.array([M_O, M_H, M_H])[:, None]).sum(0) / mw
+    tmpl = tmpl - com
+    R = random_rotations(rng, len(centres))
+    mols = centres[:, None, :] + np.einsum("nij,aj->nai", R, tmpl)
+    if len(metal):
+        keep = np.ones(len(mols), dtype=bool)
+        flat = mols.reshape(-1, 3)
+        for start in range(0, len(flat), 4096):
+            blk = flat[start:start + 4096]
+            d = blk[:, None, :] - metal[None, :, :]
+            d -= box * np.round(d / box)
+            close = (np.einsum("ijk,ijk->ij", d, d) < exclusion ** 2).any(axis=1)
+            idx = np.nonzero(close)[0] + start
+            keep[idx // 3] = False
+        mols = mols[keep]
+    return mols
+
+
+def maxwell_boltzmann(masses: np.ndarray, temp: float, rng: np.random.Generator) -> np.ndarray:
+    sigma = np.sqrt(K_BOLTZ * temp / masses)
+    v = rng.normal(size=(len(masses), 3)) * sigma[:, None]
+    vcom = (v * masses[:, None]).sum(0) / masses.sum()
+    return v - vcom
+
+
+def make_deck(name: str, box, clusters: Sequence[Tuple[int, Sequence[float]]],
+              n_steps: int, seed: int, save_step: int = 10,
+              efield=(0.0, 0.0, 0.0), with_water: bool = True, **kw) -> Deck:
+    """clusters: list of (shells, centre) for each gold nanoparticle."""
+    box = np.asarray(box, dtype=float)
+    metal_blocks, ranges = [], []
+    first = 0
+    for shells, centre in clusters:
+        pts = mackay_icosahedron(shells) + np.asarray(centre, dtype=float)
+        metal_blocks.append(pts)
+        ranges.append((first, first + len(pts) - 1, 0.0))
+        first += len(pts)
+    metal = np.concatenate(metal_blocks) if metal_blocks else np.zeros((0, 3))
+    rng = np.random.default_rng(1000 + seed)
+    water = fill_water(box, metal, rng) if with_water else np.zeros((0, 3, 3))
+    deck = Deck(name=name, box=box, metal=metal, np_ranges=ranges, water=water,
+                vel=np.zeros((0, 3)), n_steps=n_steps, save_step=save_step,
+                efield=tuple(efield), **kw)
+    vrng = np.random.default_rng(2000 + seed)
+    deck.vel = maxwell_boltzmann(deck.masses(), deck.ref_temp, vrng)
+    return deck
+
+
+def config(index: int) -> Deck:
+    """The five benchmark systems.
+
+    0: Au147 in a 3.6 nm box, ~1,500 waters, 100 steps
+    1: Au561 in a 6 nm box, ~7,000 waters, 500 steps
+    2: two Au309 on x with a 1.0 nm surface gap, 8x5x5 nm box, ~6,500 waters,
+       0.5 V/nm field along x, 500 steps
+    3: Au2057 in a 10 nm box, ~33,000 waters, 1,000 steps
+    4: Au309 in a 4.8 nm box, ~4,000 waters, 3,200 steps (frames every 50)
+    """
+    if index == 0:
+        return make_deck("cfg0_au147", [3.6] * 3, [(4, [1.8] * 3)], 100, 0)
+    if index == 1:
+        return make_deck("cfg1_au561", [6.0] * 3, [(6, [3.0] * 3)], 500, 1)
+    if index == 2:
+        # Au309 vertex radius: 4 * 0.288 * 0.9511 nm; surface gap 1.0 nm
+        rad = 4 * AU_NN * 0.9510565162951535
+        half = rad + 0.5
+        return make_deck("cfg2_2au309_field", [8.0, 5.0, 5.0],
+                         [(5, [4.0 - half, 2.5, 2.5]), (5, [4.0 + half, 2.5, 2.5])],
+                         500, 2, efield=(0.5, 0.0, 0.0))
+    if index == 3:
+        return make_deck("cfg3_au2057", [10.0] * 3, [(9, [5.0] * 3)], 1000, 3,
+                         save_step=100)
+    if index == 4:
+        return make_deck("cfg4_au309", [4.8] * 3, [(5, [2.4] * 3)], 3200, 4,
+                         save_step=50)
+    raise ValueError(f"unknown config {index}")
+
+
+def small(kind: str) -> Deck:
+    """Small systems that the CPU oracle finishes in seconds."""
+    if kind == "au13_w":
+        return make_deck("au13_w", [2.6] * 3, [(2, [1.3] * 3)], 10, 101, save_step=1)
+    if kind == "au55_w":
+        return make_deck("au55_w", [3.0] * 3, [(3, [1.5] * 3)], 10, 102, save_step=1)
+    if kind == "2au13_field":
+        return make_deck("2au13_field", [4.4, 2.6, 2.6],
+                         [(2, [1.3, 1.3, 1.3]), (2, [3.1, 1.3, 1.3])], 10, 103,
+                         save_step=1, efield=(0.5, 0.0, 0.0))
+    if kind == "au147_vac":
+        return make_deck("au147_vac", [3.6] * 3, [(4, [1.8] * 3)], 10, 104,
+                         save_step=1, with_water=False)
+    if kind == "water_only":
+        return make_deck("water_only", [2.5] * 3, [], 10, 105, save_step=1)
+    raise ValueError(kind)
+
+
+# ---------------------------------------------------------------- writers
+
+def _wrap5(n: int) -> int:
+    return n % 100000
+
+
+def write_gro(deck: Deck, path: str) -> None:
+    pos = deck.positions()
+    vel = deck.vel
+    lines = ["step  0  vQ  0.000000  vP  0.000000", "%5d" % deck.n_atoms]
+    res = 0
+    for i in range(deck.n_metal):
+        res += 1
+        lines.append("%5d%-5s%5s%5d%14.9f%14.9f%14.9f%14.9f%14.9f%14.9f" % (
+            _wrap5(res), "AU", "Au", _wrap5(i + 1), *pos[i], *vel[i]))
+    names = ("OW", "HW1", "HW2")
+    for w in range(deck.n_water):
+        res += 1
+        for a in range(3):
+            i = deck.n_metal + 3 * w + a
+            lines.append("%5d%-5s%5s%5d%14.9f%14.9f%14.9f%14.9f%14.9f%14.9f" % (
+                _wrap5(res), "SOL", names[a], _wrap5(i + 1), *pos[i], *vel[i]))
+    lines.append("%10.5f%10.5f%10.5f" % tuple(deck.box))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def write_param(deck: Deck, path: str) -> None:
+    out = ["SCALE_LJ_QQ    0.50000000     0.83333333", ""]
+    mols = []
+    if deck.n_metal:
+        mols.append(("gold", deck.n_metal))
+    if deck.n_water:
+        mols.append(("SOL", deck.n_water))
+    out.append("MOLECULES  %d" % len(mols))
+    for name, n in mols:
+        out.append("%-10s %d" % (name, n))
+    out.append("")
+    for name, _ in mols:
+        if name == "gold":
+            out += ["gold_ATOMS  1", "  0.000000  %.6f  0" % M_AU, ""]
+        else:
+            out += ["SOL_ATOMS  3",
+                    "  %.6f  %.6f  1" % (SPCE_QO, M_O),
+                    "  %.6f  %.6f  2" % (SPCE_QH, M_H),
+                    "  %.6f  %.6f  2" % (SPCE_QH, M_H), ""]
+    out.append("NONBONDED  3  6")
+    out.append(" 1  0.0  0.0")
+    out.append(" 4  %.12g  %.12g  %.12g" % AU_O_ERFVDW)
+    out.append(" 4  %.12g  %.12g  %.12g" % AU_H_ERFVDW)
+    out.append(" 1  %.8e  %.8e" % (SPCE_C6, SPCE_C12))
+    out.append(" 1  0.0  0.0")
+    out.append(" 1  0.0  0.0")
+    out.append("")
+    for name, _ in mols:
+        for sec in ("BONDS", "PAIRS", "ANGLES", "DIHEDRALS", "VSITES"):
+            out.append("%s_%s  0" % (name, sec))
+        if name == "gold":
+            out.append("gold_CSTRS  0")
+        else:
+            out += ["SOL_CSTRS  3", "0 1  %.10f" % SPCE_ROH, "0 2  %.10f" % SPCE_ROH,
+                    "1 2  %.10f" % spce_hh()]
+        out.append("")
+    if deck.n_metal:
+        out.append("MM_METAL  0  %d" % (deck.n_metal - 1))
+    else:
+        out.append("MM_METAL  -1  -1")
+    out.append("QSC_METAL  %d  %d  %.10f  %.6f  %.6f" % (SC_N, SC_M, SC_EPS, SC_C, SC_A))
+    out.append("CPFF_METAL  %.10f  %.10f   ;  atomic units" % (CPIM_POLAR, CPIM_CAPAC))
+    out.append("")
+    out.append("NANOPARTICLES  %d" % len(deck.np_ranges))
+    for first, last, q in deck.np_ranges:
+        out.append("START_END_CHG  %d  %d  %.6f" % (first, last, q))
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def write_mdset(deck: Deck, path: str) -> None:
+    out = [
+        "run_type          md",
+        "ensemble          %s" % deck.ensemble,
+        "",
+        "em_steps          100",
+        "em_length_in_nm   0.0001",
+        "em_tolerance      1000.0",
+        "",
+        "number_of_steps   %d" % deck.n_steps,
+        "time_step_in_ps   %g" % deck.dt,
+        "save_step         %d" % deck.save_step,
+        "",
+        "vdw_type          shifted",
+        "coulomb_type      wolf_sum",
+        "cut_off_radius    %g" % deck.rcut,
+        "coulomb_alpha     %g" % deck.coulomb_alpha,
+        "",
+        "heating_steps     0",
+        "ref_T_in_K        %g" % deck.ref_temp,
+        "tau_T_in_ps       %g" % deck.tau_t,
+        "",
+        "fix_metal         %d" % deck.fix_metal,
+        "use_cpff          %d" % deck.use_cpff,
+        "",
+        "external_Ex       %g" % deck.efield[0],
+        "external_Ey       %g" % deck.efield[1],
+        "external_Ez       %g" % deck.efield[2],
+    ]
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def write_deck(deck: Deck, directory: str) -> Tuple[str, str, str]:
+    os.makedirs(directory, exist_ok=True)
+    gro = os.path.join(directory, "init.gro")
+    par = os.path.join(directory, "param.txt")
+    mds = os.path.join(directory, "mdset.txt")
+    write_gro(deck, gro)
+    write_param(deck, par)
+    write_mdset(deck, mds)
+    return gro, par, mds
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("which", help="config index 0-4 or a small-system name")
+    ap.add_argument("outdir")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--save", type=int, default=None)
+    a = ap.parse_args(argv)
+    deck = config(int(a.which)) if a.which.isdigit() else small(a.which)
+    if a.steps is not None:
+        deck.n_steps = a.steps
+    if a.save is not None:
+        deck.save_step = a.save
+    write_deck(deck, a.outdir)
+    print("%s: %d atoms, %d metal, %d waters, n_mat %d" % (
+        deck.name, deck.n_atoms, deck.n_metal, deck.n_water,
+        4 * deck.n_metal + len(deck.np_ranges)))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,538 @@
+/* Input decks and output files of CapacMD (see include/capac_host.h).
+ *
+ * The grammar follows the reference's readers; the code is this project's
+ * own: one pass over param.txt into growable tables, then flattening into
+ * the capac_params arrays the device library and the oracle consume.
+ */
+#include "capac_host.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EV2KJMOL_H 96.4853363969351
+
+typedef struct {
+    FILE* fh;
+    const char* path;
+    char line[CAPAC_STR];
+    char word[CAPAC_STR];
+    char* err;
+    int errlen;
+} reader;
+
+static int fail(reader* rd, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    if (rd->err && rd->errlen > 0) vsnprintf(rd->err, rd->errlen, fmt, ap);
+    va_end(ap);
+    return -1;
+}
+
+/* next non-blank line; its first word goes to rd->word (file.c:49-67) */
+static int next_line(reader* rd) {
+    for (;;) {
+        if (!fgets(rd->line, CAPAC_STR, rd->fh))
+            return fail(rd, "Error while reading file_par!\n");
+        rd->word[0] = '\0';
+        sscanf(rd->line, "%255s", rd->word);
+        if (rd->word[0]) return 0;
+    }
+}
+
+/* "<NAME> <int>" section header with an expected name */
+static int section(reader* rd, const char* name, int* count) {
+    if (next_line(rd)) return -1;
+    if (strcmp(rd->word, name) != 0)
+        return fail(rd, "Error in %s: expected %s in line\n%s", rd->path, name, rd->line);
+    if (sscanf(rd->line, "%*s%d", count) != 1) *count = 0;
+    return 0;
+}
+
+static void* xcalloc(size_t n, size_t sz) {
+    void* p = calloc(n ? n : 1, sz);
+    if (!p) { fprintf(stderr, "Error: out of host memory\n"); exit(1); }
+    return p;
+}
+
+/* ------------------------------------------------------------ mdset.txt */
+
+static int read_mdset(capac_deck* d, const char* path, char* err, int errlen) {
+    FILE* fh = fopen(path, "r");
+    if (!fh) {
+        snprintf(err, errlen, "Cannot open %s for input!\n", path);
+        return -1;
+    }
+    char line[CAPAC_STR], key[CAPAC_STR], val[CAPAC_STR];
+    capac_params* p = &d->p;
+    while (fgets(line, CAPAC_STR, fh)) {
+        key[0] = val[0] = '\0';
+        if (sscanf(line, "%255s%255s", key, val) < 1) continue;
+        if (!strcmp(key, "run_type")) strcpy(d->run_type, val);
+        else if (!strcmp(key, "ensemble")) strcpy(d->ensemble, val);
+        else if (!strcmp(key, "em_steps")) d->em_steps = atoi(val);
+        else if (!strcmp(key, "em_length_in_nm")) d->em_length = atof(val);
+        else if (!strcmp(key, "em_tolerance")) d->em_tol = atof(val);
+        else if (!strcmp(key, "number_of_steps")) d->n_steps = atoi(val);
+        else if (!strcmp(key, "time_step_in_ps")) p->dt = atof(val);
+        else if (!strcmp(key, "save_step")) d->n_save = atoi(val);
+        else if (!strcmp(key, "vdw_type")) strcpy(d->vdw_type, val);
+        else if (!strcmp(key, "coulomb_type")) strcpy(d->coulomb_type, val);
+        else if (!strcmp(key, "cut_off_radius")) p->rcut = atof(val);
+        else if (!strcmp(key, "coulomb_alpha")) p->dsf_alpha = atof(val);
+        else if (!strcmp(key, "heating_steps")) p->n_heating = atoi(val);
+        else if (!strcmp(key, "ref_T_in_K")) p->ref_temp = atof(val);
+        else if (!strcmp(key, "tau_T_in_ps")) p->tau_t = atof(val);
+        else if (!strcmp(key, "fix_metal")) p->fix_metal = atoi(val);
+        else if (!strcmp(key, "use_cpff")) p->use_cpff = atoi(val);
+        /* V/nm -> kJ/mol/(e nm) (file.c:119-123) */
+        else if (!strcmp(key, "external_Ex")) p->efield[0] = atof(val) * EV2KJMOL_H;
+        else if (!strcmp(key, "external_Ey")) p->efield[1] = atof(val) * EV2KJMOL_H;
+        else if (!strcmp(key, "external_Ez")) p->efield[2] = atof(val) * EV2KJMOL_H;
+        else printf("Warning: unkonwn option %s in %s\n", key, path);
+    }
+    fclose(fh);
+    if (!strcmp(d->coulomb_type, "cut_off")) p->use_coulomb = 0;
+    else if (!strcmp(d->coulomb_type, "wolf_sum")) p->use_coulomb = 1;
+    else { snprintf(err, errlen, "Error: unknown coulomb_type %s!\n", d->coulomb_type); return -1; }
+    if (!strcmp(d->vdw_type, "cut_off")) p->use_vdw = 0;
+    else if (!strcmp(d->vdw_type, "shifted")) p->use_vdw = 1;
+    else { snprintf(err, errlen, "Error: unknown vdw_type %s!\n", d->vdw_type); return -1; }
+    p->nvt = !strcmp(d->ensemble, "nvt");
+    return 0;
+}
+
+/* ------------------------------------------------------------ param.txt */
+
+typedef struct { int i, j; double r; } cstr_t;
+
+typedef struct {
+    int n_atoms;
+    double* q; double* m; int* type;
+    unsigned excl[32];          /* excl[a] bit b: a and b excluded */
+    int n_cstr; cstr_t* cstr;
+    int n_bonded_terms;         /* bonds (not funct 5), pairs, angles, dihedrals */
+    int n_vsites;
+} moltype;
+
+static void excl_pair(moltype* mt, int a, int b) {
+    if (a >= 0 && b >= 0 && a < 32 && b < 32) {
+        mt->excl[a] |= 1u << b;
+        mt->excl[b] |= 1u << a;
+    }
+}
+
+static int read_param(capac_deck* d, const char* path, moltype** out_mt, char* err, int errlen) {
+    reader rd = {0};
+    rd.path = path; rd.err = err; rd.errlen = errlen;
+    rd.fh = fopen(path, "r");
+    if (!rd.fh) { snprintf(err, errlen, "Error: Cannot open param file %s!\n", path); return -1; }
+    capac_params* p = &d->p;
+    int rc = -1, n;
+
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "SCALE_LJ_QQ")) {
+        fail(&rd, "Error in %s: expected SCALE_LJ_QQ in line\n%s", path, rd.line); goto done;
+    }
+    sscanf(rd.line, "%*s%lf%lf", &d->scale_lj, &d->scale_qq);
+    if (section(&rd, "MOLECULES", &d->mol_types)) goto done;
+    d->mol_name = xcalloc(d->mol_types, sizeof *d->mol_name);
+    d->mol_count = xcalloc(d->mol_types, sizeof(int));
+    d->mol_atoms = xcalloc(d->mol_types, sizeof(int));
+    moltype* mt = xcalloc(d->mol_types, sizeof(moltype));
+    *out_mt = mt;
+    for (int k = 0; k < d->mol_types; ++k) {
+        if (next_line(&rd)) goto done;
+        sscanf(rd.line, "%255s%d", d->mol_name[k], &d->mol_count[k]);
+    }
+    for (int k = 0; k < d->mol_types; ++k) {
+        char name[CAPAC_STR + 16];
+        snprintf(name, sizeof name, "%s_ATOMS", d->mol_name[k]);
+        if (section(&rd, name, &n)) goto done;
+        if (n > 32) { fail(&rd, "Error: molecule %s has %d atoms; at most 32 are supported\n",
+                           d->mol_name[k], n); goto done; }
+        mt[k].n_atoms = n;
+        d->mol_atoms[k] = n;
+        mt[k].q = xcalloc(n, sizeof(double));
+        mt[k].m = xcalloc(n, sizeof(double));
+        mt[k].type = xcalloc(n, sizeof(int));
+        for (int a = 0; a < n; ++a) {
+            if (next_line(&rd)) goto done;
+            sscanf(rd.line, "%lf%lf%d", &mt[k].q[a], &mt[k].m[a], &mt[k].type[a]);
+        }
+    }
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "NONBONDED")) {
+        fail(&rd, "Error in %s: expected NONBONDED in line\n%s", path, rd.line); goto done;
+    }
+    sscanf(rd.line, "%*s%d", &p->n_types);
+    {
+        int nt = p->n_types;
+        d->nb_funct = xcalloc((size_t)nt * nt, sizeof(int));
+        d->nb_c6 = xcalloc((size_t)nt * nt, sizeof(double));
+        d->nb_c12 = xcalloc((size_t)nt * nt, sizeof(double));
+        d->nb_a = xcalloc((size_t)nt * nt, sizeof(double));
+        d->nb_b = xcalloc((size_t)nt * nt, sizeof(double));
+        for (int i = 0; i < nt; ++i)
+            for (int j = i; j < nt; ++j) {
+                int f = 0;
+                double a = 0, b = 0, c = 0;
+                if (next_line(&rd)) goto done;
+                sscanf(rd.line, "%d%le%le%le", &f, &a, &b, &c);
+                int ij = i * nt + j, ji = j * nt + i;
+                d->nb_funct[ij] = f;
+                if (f == 1) { d->nb_c6[ij] = a; d->nb_c12[ij] = b; }
+                else if (f == 2 || f == 3) { d->nb_a[ij] = a; d->nb_b[ij] = b; d->nb_c6[ij] = c; }
+                else if (f == 4) { d->nb_c6[ij] = a; d->nb_c12[ij] = b; d->nb_a[ij] = c; }
+                else { fail(&rd, "Error: unsupported vdW type: %d in line:\n%s\n", f, rd.line); goto done; }
+                d->nb_funct[ji] = f;
+                d->nb_c6[ji] = d->nb_c6[ij]; d->nb_c12[ji] = d->nb_c12[ij];
+                d->nb_a[ji] = d->nb_a[ij]; d->nb_b[ji] = d->nb_b[ij];
+            }
+    }
+    for (int k = 0; k < d->mol_types; ++k) {
+        static const char* secs[6] = {"BONDS", "PAIRS", "ANGLES", "DIHEDRALS", "VSITES", "CSTRS"};
+        for (int s = 0; s < 6; ++s) {
+            char name[CAPAC_STR + 16];
+            snprintf(name, sizeof name, "%s_%s", d->mol_name[k], secs[s]);
+            if (section(&rd, name, &n)) goto done;
+            if (s == 5) { mt[k].n_cstr = n; mt[k].cstr = xcalloc(n, sizeof(cstr_t)); }
+            for (int t = 0; t < n; ++t) {
+                int ai = 0, aj = 0, ak = 0, al = 0, f = 0;
+                if (next_line(&rd)) goto done;
+                if (s == 0) {               /* bonds: i j funct ... (file.c:721-765) */
+                    sscanf(rd.line, "%d%d%d", &ai, &aj, &f);
+                    excl_pair(&mt[k], ai, aj);
+                    if (f != 5) mt[k].n_bonded_terms++;
+                } else if (s == 1) {        /* 1-4 pairs */
+                    sscanf(rd.line, "%d%d", &ai, &aj);
+                    excl_pair(&mt[k], ai, aj);
+                    mt[k].n_bonded_terms++;
+                } else if (s == 2) {        /* angles exclude all three pairs */
+                    sscanf(rd.line, "%d%d%d", &ai, &aj, &ak);
+                    excl_pair(&mt[k], ai, aj); excl_pair(&mt[k], ai, ak); excl_pair(&mt[k], aj, ak);
+                    mt[k].n_bonded_terms++;
+                } else if (s == 3) {        /* dihedrals exclude all six pairs */
+                    sscanf(rd.line, "%d%d%d%d", &ai, &aj, &ak, &al);
+                    excl_pair(&mt[k], ai, aj); excl_pair(&mt[k], ai, ak); excl_pair(&mt[k], ai, al);
+                    excl_pair(&mt[k], aj, ak); excl_pair(&mt[k], aj, al); excl_pair(&mt[k], ak, al);
+                    mt[k].n_bonded_terms++;
+                } else if (s == 4) {        /* TIP5P-type virtual site s excluded from i, j, k */
+                    int as = 0;
+                    sscanf(rd.line, "%d%d%d%d", &ai, &aj, &ak, &as);
+                    excl_pair(&mt[k], as, ai); excl_pair(&mt[k], as, aj); excl_pair(&mt[k], as, ak);
+                    mt[k].n_vsites++;
+                } else {                    /* constraints i j r (file.c:977-989) */
+                    double r = 0;
+                    sscanf(rd.line, "%d%d%lf", &ai, &aj, &r);
+                    mt[k].cstr[t].i = ai; mt[k].cstr[t].j = aj; mt[k].cstr[t].r = r;
+                    excl_pair(&mt[k], ai, aj);
+                }
+            }
+        }
+    }
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "MM_METAL")) {
+        fail(&rd, "Error in %s: expected MM_METAL in line\n%s", path, rd.line); goto done;
+    }
+    d->metal_min = -1; d->metal_max = -1;
+    sscanf(rd.line, "%*s%d%d", &d->metal_min, &d->metal_max);
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "QSC_METAL")) {
+        fail(&rd, "Error in %s: expected QSC_METAL in line\n%s", path, rd.line); goto done;
+    }
+    sscanf(rd.line, "%*s%d%d%lf%lf%lf", &p->qsc_n, &p->qsc_m, &p->qsc_eps, &p->qsc_c, &p->qsc_a);
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "CPFF_METAL")) {
+        fail(&rd, "Error in %s: expected CPFF_METAL in line\n%s", path, rd.line); goto done;
+    }
+    sscanf(rd.line, "%*s%lf%lf", &p->cpff_polar, &p->cpff_capac);
+    if (next_line(&rd)) goto done;
+    if (strcmp(rd.word, "NANOPARTICLES")) {
+        fail(&rd, "Error in %s: expected NANOPARTICLES in line\n%s", path, rd.line); goto done;
+    }
+    sscanf(rd.line, "%*s%d", &p->n_nps);
+    d->np_first = xcalloc(p->n_nps, sizeof(int));
+    d->np_last = xcalloc(p->n_nps, sizeof(int));
+    d->np_charge = xcalloc(p->n_nps, sizeof(double));
+    for (int k = 0; k < p->n_nps; ++k) {
+        if (next_line(&rd)) goto done;
+        sscanf(rd.line, "%*s%d%d%lf", &d->np_first[k], &d->np_last[k], &d->np_charge[k]);
+    }
+    rc = 0;
+done:
+    fclose(rd.fh);
+    return rc;
+}
+
+/* ------------------------------------------------------------ init.gro */
+
+static int read_gro(capac_deck* d, const char* path, char* err, int errlen) {
+    FILE* fh = fopen(path, "r");
+    if (!fh) { snprintf(err, errlen, "Cannot open %s for input!\n", path); return -1; }
+    char line[CAPAC_STR], w1[CAPAC_STR], w2[CAPAC_STR], w3[CAPAC_STR];
+    int step = 0, n = 0, N = d->p.n_atoms;
+    d->vq0 = 0.0; d->vp = 0.0;
+    if (fgets(line, CAPAC_STR, fh))
+        sscanf(line, "%255s%d%255s%lf%255s%lf", w1, &step, w2, &d->vq0, w3, &d->vp);
+    if (fgets(line, CAPAC_STR, fh)) sscanf(line, "%d", &n);
+    if (n != N) {
+        fclose(fh);
+        snprintf(err, errlen, "Error: groNAtoms(%d) not equal to nAtoms(%d)!\n", n, N);
+        return -1;
+    }
+    for (int i = 0; i < N; ++i) {
+        if (!fgets(line, CAPAC_STR, fh)) break;
+        char field[CAPAC_STR];
+        size_t len = strlen(line);
+        memcpy(field, line, 5); field[5] = 0;
+        d->res_id[i] = 0;
+        sscanf(field, "%d", &d->res_id[i]);
+        memcpy(field, line + 5, 5); field[5] = 0;
+        d->res_name[i][0] = 0;
+        sscanf(field, "%7s", d->res_name[i]);
+        memcpy(field, line + 10, 5); field[5] = 0;
+        d->atom_name[i][0] = 0;
+        sscanf(field, "%7s", d->atom_name[i]);
+        double x[6] = {0, 0, 0, 0, 0, 0};
+        if (len > 20) sscanf(line + 20, "%lf%lf%lf%lf%lf%lf", &x[0], &x[1], &x[2], &x[3], &x[4], &x[5]);
+        for (int c = 0; c < 3; ++c) {
+            d->r[c * N + i] = x[c];
+            d->v[c * N + i] = x[3 + c];
+        }
+    }
+    if (fgets(line, CAPAC_STR, fh))
+        sscanf(line, "%lf%lf%lf", &d->p.box[0], &d->p.box[1], &d->p.box[2]);
+    fclose(fh);
+    return 0;
+}
+
+/* ------------------------------------------------------------ assembly */
+
+int capac_deck_load(const char* gro, const char* par, const char* mds, capac_deck** out,
+                    char* err, int errlen) {
+    char dummy[8];
+    if (!err) { err = dummy; errlen = sizeof dummy; }
+    err[0] = 0;
+    *out = NULL;
+    capac_deck* d = xcalloc(1, sizeof(capac_deck));
+    capac_params* p = &d->p;
+    moltype* mt = NULL;
+    int rc = read_mdset(d, mds, err, errlen);
+    if (!rc) rc = read_param(d, par, &mt, err, errlen);
+    if (rc) goto fail_free;
+
+    /* atoms and molecules in param order (file.c:1031-1099) */
+    int N = 0, M = 0, C = 0, nvs = 0, nbonded = 0;
+    for (int k = 0; k < d->mol_types; ++k) {
+        N += d->mol_count[k] * mt[k].n_atoms;
+        M += d->mol_count[k];
+        C += d->mol_count[k] * mt[k].n_cstr;
+        nvs += d->mol_count[k] * mt[k].n_vsites;
+        nbonded += d->mol_count[k] * mt[k].n_bonded_terms;
+    }
+    p->n_atoms = N; p->n_mols = M; p->n_cstr = C;
+    d->n_vsites = nvs;
+    d->charge = xcalloc(N, sizeof(double));
+    d->mass = xcalloc(N, sizeof(double));
+    d->atomtype = xcalloc(N, sizeof(int));
+    d->mol_id = xcalloc(N, sizeof(int));
+    d->local_idx = xcalloc(N, sizeof(int));
+    d->excl_mask = xcalloc(N, sizeof(unsigned));
+    d->mol_first = xcalloc(M, sizeof(int));
+    d->mol_last = xcalloc(M, sizeof(int));
+    d->mol_cstr = xcalloc(M + 1, sizeof(int));
+    d->cstr_i = xcalloc(C, sizeof(int));
+    d->cstr_j = xcalloc(C, sizeof(int));
+    d->cstr_r = xcalloc(C, sizeof(double));
+    d->r = xcalloc(3 * (size_t)N, sizeof(double));
+    d->v = xcalloc(3 * (size_t)N, sizeof(double));
+    d->res_id = xcalloc(N, sizeof(int));
+    d->res_name = xcalloc(N, sizeof *d->res_name);
+    d->atom_name = xcalloc(N, sizeof *d->atom_name);
+    {
+        int i = 0, im = 0, ic = 0;
+        for (int k = 0; k < d->mol_types; ++k)
+            for (int c = 0; c < d->mol_count[k]; ++c, ++im) {
+                d->mol_first[im] = i;
+                d->mol_cstr[im] = ic;
+                for (int t = 0; t < mt[k].n_cstr; ++t, ++ic) {
+                    d->cstr_i[ic] = i + mt[k].cstr[t].i;
+                    d->cstr_j[ic] = i + mt[k].cstr[t].j;
+                    d->cstr_r[ic] = mt[k].cstr[t].r;
+                }
+                for (int a = 0; a < mt[k].n_atoms; ++a, ++i) {
+                    d->charge[i] = mt[k].q[a];
+                    d->mass[i] = mt[k].m[a];
+                    d->atomtype[i] = mt[k].type[a];
+                    d->mol_id[i] = im;
+                    d->local_idx[i] = a;
+                    d->excl_mask[i] = mt[k].excl[a];
+                }
+                d->mol_last[im] = i - 1;
+            }
+        d->mol_cstr[M] = ic;
+    }
+    for (int i = 0; i < N; ++i)
+        if (d->atomtype[i] < 0 || d->atomtype[i] >= p->n_types) {
+            snprintf(err, errlen, "Error: atom %d has atomtype %d outside the %d NONBONDED types\n",
+                     i, d->atomtype[i], p->n_types);
+            rc = -1; goto fail_free;
+        }
+    if (d->metal_min >= 0 && d->metal_max >= d->metal_min) {
+        if (d->metal_min != 0) {
+            snprintf(err, errlen, "Error: metal atoms must be the first atoms (MM_METAL starts at %d)\n",
+                     d->metal_min);
+            rc = -1; goto fail_free;
+        }
+        if (d->metal_max >= N) {
+            snprintf(err, errlen, "Error: MM_METAL range ends at %d but the system has %d atoms\n",
+                     d->metal_max, N);
+            rc = -1; goto fail_free;
+        }
+        p->n_metal = d->metal_max - d->metal_min + 1;
+    } else {
+        p->n_metal = 0;
+    }
+    for (int k = 0; k < p->n_nps; ++k)
+        if (d->np_first[k] < 0 || d->np_last[k] >= p->n_metal || d->np_first[k] > d->np_last[k]) {
+            snprintf(err, errlen, "Error: nanoparticle %d range %d..%d lies outside the metal atoms\n",
+                     k, d->np_first[k], d->np_last[k]);
+            rc = -1; goto fail_free;
+        }
+    if (nbonded > 0 || nvs > 0) {
+        snprintf(err, errlen, "Error: bonded terms and virtual sites are not supported by this "
+                 "build (rigid molecules and metal only)\n");
+        rc = CAPAC_ERR_UNSUPPORTED; goto fail_free;
+    }
+    p->ndf = 3 * (N - nvs) - 3 - C;            /* mpi-main.c:548 */
+    rc = read_gro(d, gro, err, errlen);
+    if (rc) goto fail_free;
+
+    p->charge = d->charge; p->mass = d->mass; p->atomtype = d->atomtype;
+    p->mol_id = d->mol_id; p->local_idx = d->local_idx; p->excl_mask = d->excl_mask;
+    p->mol_first = d->mol_first; p->mol_last = d->mol_last; p->mol_cstr = d->mol_cstr;
+    p->cstr_i = d->cstr_i; p->cstr_j = d->cstr_j; p->cstr_r = d->cstr_r;
+    p->np_first = d->np_first; p->np_last = d->np_last; p->np_charge = d->np_charge;
+    p->nb_funct = d->nb_funct; p->nb_c6 = d->nb_c6; p->nb_c12 = d->nb_c12;
+    p->nb_a = d->nb_a; p->nb_b = d->nb_b;
+    for (int k = 0; mt && k < d->mol_types; ++k) {
+        free(mt[k].q); free(mt[k].m); free(mt[k].type); free(mt[k].cstr);
+    }
+    free(mt);
+    *out = d;
+    return 0;
+fail_free:
+    for (int k = 0; mt && k < d->mol_types; ++k) {
+        free(mt[k].q); free(mt[k].m); free(mt[k].type); free(mt[k].cstr);
+    }
+    free(mt);
+    capac_deck_free(d);
+    return rc ? rc : -1;
+}
+
+void capac_deck_free(capac_deck* d) {
+    if (!d) return;
+    free(d->mol_name); free(d->mol_count); free(d->mol_atoms);
+    free(d->charge); free(d->mass); free(d->atomtype); free(d->mol_id); free(d->local_idx);
+    free(d->excl_mask); free(d->mol_first); free(d->mol_last); free(d->mol_cstr);
+    free(d->cstr_i); free(d->cstr_j); free(d->cstr_r);
+    free(d->np_first); free(d->np_last); free(d->np_charge);
+    free(d->nb_funct); free(d->nb_c6); free(d->nb_c12); free(d->nb_a); free(d->nb_b);
+    free(d->r); free(d->v); free(d->res_id); free(d->res_name); free(d->atom_name);
+    free(d);
+}
+
+int capac_deck_n_mat(const capac_deck* d) {
+    return d->p.n_metal > 0 ? 4 * d->p.n_metal + d->p.n_nps : 0;
+}
+
+const capac_params* capac_deck_params(const capac_deck* d) { return &d->p; }
+const double* capac_deck_positions(const capac_deck* d) { return d->r; }
+const double* capac_deck_velocities(const capac_deck* d) { return d->v; }
+void capac_deck_run(const capac_deck* d, int* n_steps, int* n_save, double* vq0, double* vp) {
+    if (n_steps) *n_steps = d->n_steps;
+    if (n_save) *n_save = d->n_save;
+    if (vq0) *vq0 = d->vq0;
+    if (vp) *vp = d->vp;
+}
+int capac_params_size(void) { return (int)sizeof(capac_params); }
+
+/* ------------------------------------------------------------ writers */
+
+void capac_write_gro(FILE* fh, const capac_deck* d, const double* r, const double* v,
+                     double vq0, double vp, int step) {
+    int N = d->p.n_atoms;
+    fprintf(fh, "step  %d  vQ  %lf  vP  %lf\n", step, vq0, vp);
+    fprintf(fh, "%5d\n", N);
+    for (int i = 0; i < N; ++i)
+        fprintf(fh, "%5d%-5s%5s%5d%10.5f%10.5f%10.5f%10.6f%10.6f%10.6f\n",
+                d->res_id[i], d->res_name[i], d->atom_name[i], i + 1,
+                r[i], r[N + i], r[2 * N + i], v[i], v[N + i], v[2 * N + i]);
+    fprintf(fh, "%10.5f%10.5f%10.5f\n", d->p.box[0], d->p.box[1], d->p.box[2]);
+}
+
+void capac_write_vec_pq(FILE* fh, const capac_deck* d, const double* x, int step) {
+    int nm = d->p.n_metal;
+    fprintf(fh, "step %d", step);
+    fprintf(fh, "  lambda");
+    for (int k = 0; k < d->p.n_nps; ++k) fprintf(fh, " %.10f", x[4 * nm + k]);
+    fprintf(fh, "\n");
+    fprintf(fh, "%5d\n", nm);
+    for (int i = 0; i < nm; ++i)
+        fprintf(fh, "%15.8f%15.8f%15.8f%15.8f\n", x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * nm + i]);
+}
+
+void capac_write_binary(FILE* fh, const capac_deck* d, const double* r, int step) {
+    int N = d->p.n_atoms;
+    fwrite(&N, sizeof(int), 1, fh);
+    fwrite(&step, sizeof(int), 1, fh);
+    fwrite(d->p.box, sizeof(double), 3, fh);
+    fwrite(r, sizeof(double), 3 * (size_t)N, fh);   /* rx | ry | rz */
+}
+
+int capac_write_gro_path(const char* path, const char* mode, const capac_deck* d,
+                         const double* r, const double* v, double vq0, double vp, int step) {
+    FILE* fh = fopen(path, mode);
+    if (!fh) return -1;
+    capac_write_gro(fh, d, r, v, vq0, vp, step);
+    fclose(fh);
+    return 0;
+}
+
+int capac_write_vec_pq_path(const char* path, const char* mode, const capac_deck* d,
+                            const double* x, int step) {
+    FILE* fh = fopen(path, mode);
+    if (!fh) return -1;
+    capac_write_vec_pq(fh, d, x, step);
+    fclose(fh);
+    return 0;
+}
+
+int capac_write_binary_path(const char* path, const char* mode, const capac_deck* d,
+                            const double* r, int step) {
+    FILE* fh = fopen(path, mode);
+    if (!fh) return -1;
+    capac_write_binary(fh, d, r, step);
+    fclose(fh);
+    return 0;
+}
+
+void capac_print_potential(FILE* fh, const double* pot) {
+    fprintf(fh, "    Potential energies: \n");
+    fprintf(fh, " %15.6e   total energy                         \n", pot[0]);
+    fprintf(fh, " %15.6e   metal quantum Sutton-Chen energy     \n", pot[1]);
+    fprintf(fh, " %15.6e   non-metal bond stretching energy     \n", pot[2]);
+    fprintf(fh, " %15.6e   non-metal angle bending energy       \n", pot[3]);
+    fprintf(fh, " %15.6e   non-metal torsional energy           \n", pot[4]);
+    fprintf(fh, " %15.6e   Long range Coulomb                   \n", pot[6]);
+    fprintf(fh, " %15.6e   Coulomb energy (including 1-4)       \n", pot[7]);
+    fprintf(fh, " %15.6e   vdW energy (including 1-4)           \n", pot[8]);
+    fprintf(fh, " %15.6e   CPIM metal charge - non-metal charge \n", pot[10]);
+    fprintf(fh, " %15.6e   CPIM metal dipole - non-metal charge \n", pot[11]);
+    fprintf(fh, " %15.6e   CPIM metal charge - metal charge     \n", pot[12]);
+    fprintf(fh, " %15.6e   CPIM metal charge - metal dipole     \n", pot[13]);
+    fprintf(fh, " %15.6e   CPIM metal dipole - metal dipole     \n", pot[14]);
+    fprintf(fh, "\n");
+}

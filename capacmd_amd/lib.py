@@ -1,0 +1,347 @@
+"""ctypes bindings of the two C ABIs (include/capac_host.h, include/capac_gpu.h).
+
+The shared libraries are built in-tree by capacmd_amd/build.py.  Loading is
+strict: a missing libcapac_gpu.so raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_SO = os.path.join(HERE, "libcapac_host.so")
+GPU_SO = os.path.join(HERE, "libcapac_gpu.so")
+
+NPOT = 15
+NHC = 10
+
+PREC_FP64 = 0
+PREC_FP32_REFINED = 1
+
+SOLVE_KINDS = {0: "converged", 1: "loosened", 2: "failed", 3: "zero_rhs", 4: "zero_rho"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_uint)
+_lp = C.POINTER(C.c_long)
+
+
+class CapacParams(C.Structure):
+    """Mirror of capac_params (include/capac_gpu.h)."""
+    _fields_ = [
+        ("n_atoms", C.c_int), ("n_metal", C.c_int), ("n_nps", C.c_int),
+        ("n_types", C.c_int), ("n_mols", C.c_int), ("n_cstr", C.c_int),
+        ("charge", _dp), ("mass", _dp), ("atomtype", _ip), ("mol_id", _ip),
+        ("local_idx", _ip), ("excl_mask", _up),
+        ("mol_first", _ip), ("mol_last", _ip), ("mol_cstr", _ip),
+        ("cstr_i", _ip), ("cstr_j", _ip), ("cstr_r", _dp),
+        ("np_first", _ip), ("np_last", _ip), ("np_charge", _dp),
+        ("nb_funct", _ip), ("nb_c6", _dp), ("nb_c12", _dp), ("nb_a", _dp), ("nb_b", _dp),
+        ("box", C.c_double * 3), ("rcut", C.c_double), ("dsf_alpha", C.c_double),
+        ("use_coulomb", C.c_int), ("use_vdw", C.c_int),
+        ("qsc_n", C.c_int), ("qsc_m", C.c_int),
+        ("qsc_eps", C.c_double), ("qsc_c", C.c_double), ("qsc_a", C.c_double),
+        ("use_cpff", C.c_int), ("cpff_polar", C.c_double), ("cpff_capac", C.c_double),
+        ("efield", C.c_double * 3),
+        ("dt", C.c_double), ("fix_metal", C.c_int), ("nvt", C.c_int),
+        ("ref_temp", C.c_double), ("tau_t", C.c_double), ("n_heating", C.c_int),
+        ("ndf", C.c_int),
+    ]
+
+
+class CapacTiming(C.Structure):
+    _fields_ = [("t", C.c_double * NPOT), ("n_force_calls", C.c_longlong),
+                ("solve_iters", C.c_longlong)]
+
+
+def dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+_host = None
+_gpu = None
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_SO):
+            raise RuntimeError(f"{HOST_SO} is not built; run capacmd_amd/build.py")
+        lib = C.CDLL(HOST_SO)
+        lib.capac_deck_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p,
+                                        C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        lib.capac_deck_load.restype = C.c_int
+        lib.capac_deck_free.argtypes = [C.c_void_p]
+        lib.capac_deck_n_mat.argtypes = [C.c_void_p]
+        lib.capac_deck_params.argtypes = [C.c_void_p]
+        lib.capac_deck_params.restype = C.POINTER(CapacParams)
+        lib.capac_deck_positions.argtypes = [C.c_void_p]
+        lib.capac_deck_positions.restype = _dp
+        lib.capac_deck_velocities.argtypes = [C.c_void_p]
+        lib.capac_deck_velocities.restype = _dp
+        lib.capac_deck_run.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
+        lib.capac_params_size.restype = C.c_int
+        for fn in ("capac_write_gro_path",):
+            getattr(lib, fn).argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, _dp, _dp,
+                                         C.c_double, C.c_double, C.c_int]
+        lib.capac_write_vec_pq_path.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, _dp, C.c_int]
+        lib.capac_write_binary_path.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, _dp, C.c_int]
+        if lib.capac_params_size() != C.sizeof(CapacParams):
+            raise RuntimeError("capac_params layout mismatch between C and ctypes")
+        _host = lib
+    return _host
+
+
+class Deck:
+    """An input deck read by the C host library (init.gro, param.txt, mdset.txt)."""
+
+    def __init__(self, gro: str, par: str, mds: str):
+        lib = host_lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib.capac_deck_load(gro.encode(), par.encode(), mds.encode(), C.byref(h), err, 1024)
+        if rc != 0:
+            raise ValueError(err.value.decode(errors="replace").strip())
+        self._h = h
+        self._lib = lib
+        self.params_ptr = lib.capac_deck_params(h)
+        self.p = self.params_ptr.contents
+        self.n_atoms = self.p.n_atoms
+        self.n_metal = self.p.n_metal
+        self.n_mat = lib.capac_deck_n_mat(h)
+        n3 = 3 * self.n_atoms
+        self.r0 = np.ctypeslib.as_array(lib.capac_deck_positions(h), shape=(n3,)).copy()
+        self.v0 = np.ctypeslib.as_array(lib.capac_deck_velocities(h), shape=(n3,)).copy()
+        ns, nsave = C.c_int(), C.c_int()
+        vq0, vp = C.c_double(), C.c_double()
+        lib.capac_deck_run(h, C.byref(ns), C.byref(nsave), C.byref(vq0), C.byref(vp))
+        self.n_steps, self.n_save, self.vq0, self.vp = ns.value, nsave.value, vq0.value, vp.value
+
+    @classmethod
+    def from_dir(cls, d: str) -> "Deck":
+        return cls(os.path.join(d, "init.gro"), os.path.join(d, "param.txt"),
+                   os.path.join(d, "mdset.txt"))
+
+    def array(self, name: str, n: int, dtype=np.float64) -> np.ndarray:
+        ptr = getattr(self.p, name)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+    @property
+    def box(self) -> np.ndarray:
+        return np.array(self.p.box[:])
+
+    def close(self):
+        if self._h:
+            self._lib.capac_deck_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- GPU ABI
+
+GPU_EXPORTS = [
+    "capac_last_error", "capac_version", "capac_create", "capac_destroy",
+    "capac_set_state", "capac_get_state", "capac_get_forces", "capac_set_box",
+    "capac_force", "capac_neighbours", "capac_nonbonded", "capac_sutton_chen",
+    "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_force",
+    "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
+    "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
+]
+
+
+def gpu_lib() -> C.CDLL:
+    """Loads libcapac_gpu.so; raises if it is missing (no fallback path)."""
+    global _gpu
+    if _gpu is None:
+        if not os.path.exists(GPU_SO):
+            raise RuntimeError(f"{GPU_SO} is not built; run capacmd_amd/build.py")
+        lib = C.CDLL(GPU_SO)
+        lib.capac_last_error.restype = C.c_char_p
+        lib.capac_version.restype = C.c_char_p
+        lib.capac_create.argtypes = [C.POINTER(CapacParams), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        lib.capac_destroy.argtypes = [C.c_void_p]
+        lib.capac_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_get_forces.argtypes = [C.c_void_p, _dp]
+        lib.capac_set_box.argtypes = [C.c_void_p, _dp]
+        lib.capac_force.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_neighbours.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
+        lib.capac_nonbonded.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_sutton_chen.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        lib.capac_cpim_rhs.argtypes = [C.c_void_p, _dp]
+        lib.capac_cpim_assemble.argtypes = [C.c_void_p, _dp, _dp]
+        lib.capac_cpim_solve.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, _ip]
+        lib.capac_cpim_force.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        lib.capac_solve_stats.argtypes = [C.c_void_p, _ip, _dp, _dp, _ip]
+        lib.capac_md_begin.argtypes = [C.c_void_p]
+        lib.capac_md_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        lib.capac_md_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.capac_set_thermostat.argtypes = [C.c_void_p, _dp, C.c_double]
+        lib.capac_get_timing.argtypes = [C.c_void_p, C.POINTER(CapacTiming)]
+        lib.capac_reset_timing.argtypes = [C.c_void_p]
+        lib.capac_enable_timing.argtypes = [C.c_void_p, C.c_int]
+        _gpu = lib
+    return _gpu
+
+
+class CapacError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = gpu_lib().capac_last_error()
+        raise CapacError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+class Context:
+    """One device-resident box (capac_create / capac_destroy)."""
+
+    def __init__(self, deck: Deck, precision: int = PREC_FP64, device: int = 0):
+        self.lib = gpu_lib()
+        self.deck = deck
+        self.N = deck.n_atoms
+        self.n_mat = deck.n_mat
+        self.n_metal = deck.n_metal
+        h = C.c_void_p()
+        _check(self.lib.capac_create(deck.params_ptr, precision, device, C.byref(h)), "capac_create")
+        self.h = h
+        self.set_state(deck.r0, deck.v0, np.zeros(max(self.n_mat, 1)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.capac_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state
+    def set_state(self, r=None, v=None, x=None):
+        r = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        x = None if x is None or self.n_mat == 0 else np.ascontiguousarray(x, dtype=np.float64)
+        _check(self.lib.capac_set_state(self.h, dptr(r), dptr(v), dptr(x)), "capac_set_state")
+
+    def get_state(self):
+        r = np.empty(3 * self.N); v = np.empty(3 * self.N); x = np.empty(max(self.n_mat, 1))
+        _check(self.lib.capac_get_state(self.h, dptr(r), dptr(v), dptr(x)), "capac_get_state")
+        return r, v, x[: self.n_mat]
+
+    def forces(self):
+        f = np.empty(3 * self.N)
+        _check(self.lib.capac_get_forces(self.h, dptr(f)), "capac_get_forces")
+        return f
+
+    # operators
+    def force(self):
+        pot = np.zeros(NPOT); vir = np.zeros(9); f = np.empty(3 * self.N)
+        _check(self.lib.capac_force(self.h, dptr(pot), dptr(vir), dptr(f)), "capac_force")
+        return f, pot, vir
+
+    def neighbours(self, max_per_atom: int = 2048):
+        cnt = np.zeros(self.N, dtype=np.int32)
+        lst = np.zeros(self.N * max_per_atom, dtype=np.int32)
+        m = _check(self.lib.capac_neighbours(self.h, iptr(cnt), iptr(lst), max_per_atom),
+                   "capac_neighbours")
+        return cnt, lst.reshape(self.N, max_per_atom), m
+
+    def nonbonded(self):
+        f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
+        _check(self.lib.capac_nonbonded(self.h, dptr(f), dptr(pot), dptr(vir)), "capac_nonbonded")
+        return f, pot, vir
+
+    def sutton_chen(self):
+        f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
+        isd = np.empty(max(self.n_metal, 1))
+        _check(self.lib.capac_sutton_chen(self.h, dptr(f), dptr(pot), dptr(vir), dptr(isd)),
+               "capac_sutton_chen")
+        return f, pot, vir, isd[: self.n_metal]
+
+    def cpim_rhs(self):
+        b = np.empty(self.n_mat)
+        _check(self.lib.capac_cpim_rhs(self.h, dptr(b)), "capac_cpim_rhs")
+        return b
+
+    def cpim_assemble(self, dense: bool = True):
+        n4 = 4 * self.n_metal
+        a = np.empty(n4 * n4) if dense else None
+        diag = np.empty(self.n_mat)
+        _check(self.lib.capac_cpim_assemble(self.h, dptr(a), dptr(diag)), "capac_cpim_assemble")
+        return (a.reshape(n4, n4) if dense else None), diag
+
+    def cpim_solve(self, b=None, x=None):
+        it = C.c_int(); rtr = C.c_double(); btb = C.c_double(); kind = C.c_int()
+        b = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        x = None if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        _check(self.lib.capac_cpim_solve(self.h, dptr(b), dptr(x), C.byref(it), C.byref(rtr),
+                                         C.byref(btb), C.byref(kind)), "capac_cpim_solve")
+        return x, it.value, rtr.value, btb.value, kind.value
+
+    def cpim_force(self, x=None):
+        f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
+        x = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        _check(self.lib.capac_cpim_force(self.h, dptr(x), dptr(f), dptr(pot), dptr(vir)),
+               "capac_cpim_force")
+        return f, pot, vir
+
+    def solve_stats(self):
+        it = C.c_int(); rtr = C.c_double(); btb = C.c_double(); kind = C.c_int()
+        _check(self.lib.capac_solve_stats(self.h, C.byref(it), C.byref(rtr), C.byref(btb),
+                                          C.byref(kind)), "capac_solve_stats")
+        return it.value, rtr.value, btb.value, kind.value
+
+    # MD
+    def md_begin(self):
+        _check(self.lib.capac_md_begin(self.h), "capac_md_begin")
+
+    def md_run(self, first_step: int, n_steps: int, n_save: int, stop_at_save: bool = False) -> int:
+        return _check(self.lib.capac_md_run(self.h, first_step, n_steps, n_save, int(stop_at_save)),
+                      "capac_md_run")
+
+    def observables(self):
+        ek = C.c_double(); t = C.c_double(); rt = C.c_double()
+        vq = np.zeros(NHC); fmax = C.c_double(); frms = C.c_double()
+        pot = np.zeros(NPOT); vir = np.zeros(9)
+        _check(self.lib.capac_md_observables(self.h, C.byref(ek), C.byref(t), C.byref(rt), dptr(vq),
+                                             C.byref(fmax), C.byref(frms), dptr(pot), dptr(vir)),
+               "capac_md_observables")
+        return dict(eksum=ek.value, temp=t.value, ref_temp=rt.value, vq=vq, fmax=fmax.value,
+                    frms=frms.value, pot=pot, vir=vir)
+
+    def set_thermostat(self, vq, vp: float = 0.0):
+        vq = np.ascontiguousarray(vq, dtype=np.float64)
+        _check(self.lib.capac_set_thermostat(self.h, dptr(vq), vp), "capac_set_thermostat")
+
+    def timing(self):
+        t = CapacTiming()
+        _check(self.lib.capac_get_timing(self.h, C.byref(t)), "capac_get_timing")
+        return np.array(t.t[:]), t.n_force_calls, t.solve_iters
+
+    def reset_timing(self):
+        _check(self.lib.capac_reset_timing(self.h), "capac_reset_timing")
+
+    def enable_timing(self, on: bool = True):
+        _check(self.lib.capac_enable_timing(self.h, int(on)), "capac_enable_timing")

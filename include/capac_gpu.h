@@ -1,0 +1,221 @@
+/* capac_gpu.h -- C ABI of the MI355X force + CPIM path (libcapac_gpu.so).
+ *
+ * One context owns one simulation box resident in HBM.  The entry points
+ * replace the reference's per-step operator
+ *
+ *   void mpi_force(Task*, Topol*, Atom_Info*, Mol_Info*, RunSet*, Metal*,
+ *                  System*, Bicgstab*, int my_id, int num_procs,
+ *                  double** time_used)            (reference force.h:100,
+ *                                                   force.c:2097-2320)
+ *
+ * and its sub-operators (cpff.h:30,38,45, bicg_stab.h:58, force.c:385-542,
+ * 1402-1540), plus the MD step body that the reference runs on rank 0
+ * (mpi-main.c:874-1013).  Plain pointers and sizes only.
+ *
+ * Conventions, as the reference:
+ *   - MD units: nm, ps, kJ/mol, e, amu.  CPIM vectors in atomic units.
+ *   - Per-atom 3-vectors are SoA blocks x[N] | y[N] | z[N] (the reference's
+ *     rx/ry/rz and fx/fy/fz layout, mpi-main.c:476-488).
+ *   - pot[15] uses the reference's index map (mpi-main.c:431-446):
+ *     1 Sutton-Chen, 7 Coulomb, 8 vdW, 9 external field, 10-14 CPIM terms,
+ *     0 = sum of 1..14 (force.c:322-331).
+ *   - vir[9] is the reference's virial[3][3], row-major.
+ *   - The CPIM vector x = vec_pq is [p (3 N_m, interleaved xyz) | q (N_m) |
+ *     lambda (n_NP)] in atomic units (cpff.c:167-193).
+ *   - Metal atoms must be atoms [0, n_metal) (cpff.c:762-764).
+ * Errors are returned, never exit()ed: 0 = success, < 0 = failure; the
+ * message is in capac_last_error().  One host thread per context; calls are
+ * synchronous at the API boundary.
+ */
+#ifndef CAPAC_GPU_H
+#define CAPAC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAPAC_NPOT 15
+#define CAPAC_NHC 10              /* Nose-Hoover chain length (mpi-main.c:514) */
+
+/* return / status codes */
+#define CAPAC_OK               0
+#define CAPAC_ERR_ARG        (-1)
+#define CAPAC_ERR_HIP        (-2)
+#define CAPAC_ERR_SOLVER     (-3)  /* BiCGSTAB not converged (bicg_stab.c:563-569) */
+#define CAPAC_ERR_RATTLE1    (-4)  /* RATTLE_1 not converged (force.c:1990-1994) */
+#define CAPAC_ERR_RATTLE2    (-5)  /* RATTLE_2 not converged (force.c:2083-2087) */
+#define CAPAC_ERR_UNSUPPORTED (-6)
+
+/* precision of the CPIM relay matrix and inner solve */
+#define CAPAC_PREC_FP64          0  /* FP64 matrix, FP64 BiCGSTAB (the reference's arithmetic) */
+#define CAPAC_PREC_FP32_REFINED  1  /* FP32 matrix + FP32 inner solve in FP64 refinement */
+
+/* solver exit kinds (bicg_stab.c:300,368,539-571) */
+#define CAPAC_SOLVE_CONVERGED   0   /* rTr < 1e-8 bTb */
+#define CAPAC_SOLVE_LOOSENED    1   /* iter > 1000 and rTr < 1e-6 bTb */
+#define CAPAC_SOLVE_FAILED      2   /* iter > 1000 and rTr >= 1e-6 bTb */
+#define CAPAC_SOLVE_ZERO_RHS    3   /* bTb < 1e-10: x left unchanged */
+#define CAPAC_SOLVE_ZERO_RHO    4   /* (r0,r) < 1e-10 at the first iteration */
+
+/* Everything the path reads from param.txt/mdset.txt, flattened.  The
+ * pointers must stay valid only for the duration of capac_create(). */
+typedef struct capac_params {
+    int n_atoms;
+    int n_metal;                /* metal atoms are [0, n_metal)              */
+    int n_nps;                  /* nanoparticles                             */
+    int n_types;                /* vdW types                                 */
+    int n_mols;                 /* molecules                                 */
+    int n_cstr;                 /* constraints, all molecules                */
+
+    /* per atom */
+    const double*   charge;     /* e                                         */
+    const double*   mass;       /* amu (0 for virtual sites)                 */
+    const int*      atomtype;   /* row/column of the pair table              */
+    const int*      mol_id;     /* molecule index in the system              */
+    const int*      local_idx;  /* index inside its molecule (< 32)          */
+    const unsigned* excl_mask;  /* bit k: excluded from the atom with local
+                                   index k of the same molecule
+                                   (force.c:1457-1461)                       */
+    /* per molecule */
+    const int*      mol_first;  /* first atom                                */
+    const int*      mol_last;   /* last atom                                 */
+    const int*      mol_cstr;   /* constraint range [mol_cstr[m], mol_cstr[m+1]) */
+    /* constraints, absolute atom indices, in param.txt order per molecule */
+    const int*      cstr_i;
+    const int*      cstr_j;
+    const double*   cstr_r;     /* nm                                        */
+    /* nanoparticles: metal index ranges and total charges */
+    const int*      np_first;
+    const int*      np_last;
+    const double*   np_charge;
+    /* pair table, n_types x n_types row-major, symmetric (file.c:618-695);
+       funct 1 = LJ (C6, C12), 4 = erf-damped vdW (C6, C12, width in nb_a) */
+    const int*      nb_funct;
+    const double*   nb_c6;
+    const double*   nb_c12;
+    const double*   nb_a;
+    const double*   nb_b;
+
+    double box[3];              /* nm, rectangular                           */
+    double rcut;                /* nm                                        */
+    double dsf_alpha;           /* nm^-1                                     */
+    int    use_coulomb;         /* 1 = damped shifted force (wolf_sum), 0 = plain cut-off */
+    int    use_vdw;             /* 1 = shifted-force LJ, 0 = plain cut-off   */
+    int    qsc_n, qsc_m;
+    double qsc_eps, qsc_c, qsc_a;   /* kJ/mol, -, nm                         */
+    int    use_cpff;
+    double cpff_polar;          /* bohr^3                                    */
+    double cpff_capac;          /* bohr                                      */
+    double efield[3];           /* kJ/mol/(e nm)                             */
+
+    /* integrator */
+    double dt;                  /* ps                                        */
+    int    fix_metal;
+    int    nvt;                 /* 1: Nose-Hoover chain (ensemble nvt)       */
+    double ref_temp;            /* K                                         */
+    double tau_t;               /* ps                                        */
+    int    n_heating;
+    int    ndf;                 /* 3(N - n_vsites) - 3 - n_cstr (mpi-main.c:548) */
+} capac_params;
+
+/* per-phase wall times in seconds, the reference's timer categories
+ * (mpi-main.c:119-128): [1] SC density [2] SC force [4] nonbonded
+ * [5] CPIM matrix [6] CPIM vector [7] CPIM solve [8] CPIM force
+ * [10] neighbour build [12] integrator [0] total force */
+typedef struct capac_timing {
+    double t[CAPAC_NPOT];
+    long long n_force_calls;
+    long long solve_iters;      /* BiCGSTAB iterations, summed               */
+} capac_timing;
+
+typedef struct capac_ctx capac_ctx;
+
+const char* capac_last_error(void);
+const char* capac_version(void);
+
+/* Device set-up: copies the topology and parameters to HBM, sizes the
+ * neighbour list, relay matrix and solver work space.  Replaces the
+ * reference's allocation block (mpi-main.c:448-609). */
+int  capac_create(const capac_params* p, int precision, int device, capac_ctx** out);
+void capac_destroy(capac_ctx* ctx);
+
+/* Positions/velocities (3N SoA, nullable) and the CPIM vector (n_mat, nullable,
+ * the solver's warm start, mpi-main.c:585-590). */
+int capac_set_state(capac_ctx* ctx, const double* r, const double* v, const double* vec_pq);
+int capac_get_state(capac_ctx* ctx, double* r, double* v, double* vec_pq);
+int capac_get_forces(capac_ctx* ctx, double* f);
+int capac_set_box(capac_ctx* ctx, const double box[3]);
+
+/* One force evaluation = mpi_force (force.c:2097-2320): zeroes, pair forces,
+ * Sutton-Chen, CPIM right-hand side, relay assembly, BiCGSTAB (warm start
+ * from the resident vec_pq), CPIM forces and the external field.  pot[0] is
+ * left as the reference leaves it (the caller sums); f (3N) and the outputs
+ * are nullable.  Returns CAPAC_ERR_SOLVER where the reference exits. */
+int capac_force(capac_ctx* ctx, double pot[CAPAC_NPOT], double vir[9], double* f);
+
+/* ---- sub-operators, each one kernel family, for parity tests ---- */
+
+/* Neighbour list (cell list + the reference's exact cut-off test):
+ * per atom i, every j != i not excluded with |pbc_12(r_j - r_i)|^2 < rc^2.
+ * count[N]; list[N * max_per_atom] (j indices, unordered); returns the
+ * largest row length, or < 0. */
+int capac_neighbours(capac_ctx* ctx, int* count, int* list, int max_per_atom);
+
+/* mpi_nonb_pair (force.c:1402-1540): f, pot[7], pot[8], vir (each nullable;
+ * values are written, not accumulated) */
+int capac_nonbonded(capac_ctx* ctx, double* f, double pot[CAPAC_NPOT], double vir[9]);
+
+/* mpi_qsc_dens + mpi_qsc_force (force.c:385-542) */
+int capac_sutton_chen(capac_ctx* ctx, double* f, double pot[CAPAC_NPOT], double vir[9],
+                      double* inv_sqrt_dens);
+
+/* mpi_cpff_vec_ext (cpff.c:43-198): b[n_mat] */
+int capac_cpim_rhs(capac_ctx* ctx, double* b);
+
+/* mpi_cpff_mat_relay_COO (cpff.c:205-519), assembled into the device layout;
+ * optionally expanded to the dense (4 N_m) x (4 N_m) block without the
+ * Lagrange rows (row-major, nullable) and the Jacobi diagonal diag[n_mat]. */
+int capac_cpim_assemble(capac_ctx* ctx, double* dense, double* diag);
+
+/* mpi_precon_bicg_stab_COO (bicg_stab.c:253-573) on the assembled matrix.
+ * b (nullable: the resident right-hand side), x in/out (nullable: the
+ * resident vec_pq).  kind = CAPAC_SOLVE_*. */
+int capac_cpim_solve(capac_ctx* ctx, const double* b, double* x, int* iters,
+                     double* rTr, double* bTb, int* kind);
+
+/* mpi_cpff_force (cpff.c:526-950) for the given x (nullable: resident) */
+int capac_cpim_force(capac_ctx* ctx, const double* x, double* f, double pot[CAPAC_NPOT],
+                     double vir[9]);
+
+/* statistics of the last solve */
+int capac_solve_stats(capac_ctx* ctx, int* iters, double* rTr, double* bTb, int* kind);
+
+/* ---- MD on the device (mpi-main.c:629-640, 874-1013) ---- */
+
+/* After the first force call: remove the centre-of-mass motion and compute
+ * the kinetic energy; records the starting temperature. */
+int capac_md_begin(capac_ctx* ctx);
+
+/* Runs steps first_step .. first_step + n_steps - 1 of the MD loop: NHC half
+ * step, velocity Verlet with RATTLE, force, NHC, centre-of-mass removal, and
+ * the molecule wrap every n_save steps (force.c:127-165).  Stops early after
+ * a step that is a multiple of n_save when stop_at_save is set, so the host
+ * can write output; returns the number of steps done, or < 0. */
+int capac_md_run(capac_ctx* ctx, int first_step, int n_steps, int n_save, int stop_at_save);
+
+/* Thermostat and step observables: eKsum, instantaneous T, the reference
+ * temperature used in the last step, vQ[10], F_max, F_rms (force.c:361-378),
+ * pot[15] and vir[9] of the last step (all nullable). */
+int capac_md_observables(capac_ctx* ctx, double* eksum, double* temp, double* ref_temp,
+                         double* vq, double* fmax, double* frms, double pot[CAPAC_NPOT],
+                         double vir[9]);
+int capac_set_thermostat(capac_ctx* ctx, const double* vq, double vp);
+
+int capac_get_timing(capac_ctx* ctx, capac_timing* t);
+int capac_reset_timing(capac_ctx* ctx);
+int capac_enable_timing(capac_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,344 @@
+// Golden-vector driver for the reference (test infrastructure; never shipped).
+//
+// Links the reference's own compiled units (file, force, thermostat, cpff,
+// bicg_stab, my_malloc) and calls its phase functions on one input deck,
+// writing every intermediate array at full FP64 precision as .npy files:
+//
+//   ref_dump init.gro param.txt mdset.txt OUTDIR [nsteps]
+//
+// The set-up below restates what the reference's main() does before its
+// first force call (mpi-main.c:99-609) with the structs constructed properly;
+// the phase order restates mpi_force (force.c:2097-2320).  With nsteps > 0
+// it then runs the MD loop body (mpi-main.c:874-1013) calling the
+// reference's own integrator pieces and dumps the state after every step.
+#include <mpi.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "typedef.h"
+#include "my_malloc.h"
+#include "file.h"
+#include "force.h"
+#include "cpff.h"
+#include "bicg_stab.h"
+#include "thermostat.h"
+
+// phase functions that force.c defines but its header does not declare
+void mpi_qsc_dens(Task* p_task, const double rCut2, Metal* p_metal, System* p_system,
+                  const int my_id, const int num_procs);
+void mpi_qsc_force(Task* p_task, const double rCut2, Metal* p_metal, System* p_system,
+                   const int my_id);
+void mpi_nonb_pair(Topol* p_topol, Task* p_task, Atom_Info* atom_info, RunSet* p_runset,
+                   System* p_system, int my_id);
+void zero_force_pot_vir(int nAtoms, double* fx, double* fy, double* fz, double* potential,
+                        double** virial);
+
+static std::string g_out;
+
+static void npy(const char* name, const void* data, const char* descr, size_t n,
+                size_t elem) {
+  std::string path = g_out + "/" + name + ".npy";
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) { perror(path.c_str()); exit(2); }
+  char hdr[256];
+  int len = snprintf(hdr, sizeof hdr, "{'descr': '%s', 'fortran_order': False, 'shape': (%zu,), }",
+                     descr, n);
+  int total = 10 + len + 1;
+  int pad = (64 - total % 64) % 64;
+  std::string h(hdr);
+  h.append(pad, ' ');
+  h.push_back('\n');
+  unsigned short hl = (unsigned short)h.size();
+  fwrite("\x93NUMPY\x01\x00", 1, 8, f);
+  fwrite(&hl, 2, 1, f);
+  fwrite(h.data(), 1, h.size(), f);
+  if (n) fwrite(data, elem, n, f);
+  fclose(f);
+}
+static void npy_d(const char* name, const double* d, size_t n) { npy(name, d, "<f8", n, 8); }
+
+struct Ref {
+  RunSet* rs;
+  Metal* metal;
+  Topol* top;
+  Task* task;
+  Atom_Info* ai;
+  Mol_Info* mi;
+  System* sys;
+  Bicgstab* bcg;
+  double** time_used;
+  int nAtoms, nMols, n_mat;
+};
+
+static void setup(Ref& R, char* gro, char* par, char* mds) {
+  R.rs = (RunSet*)calloc(1, sizeof(RunSet));
+  R.metal = new Metal();          // value-initialised: vectors constructed, scalars zero
+  R.rs->external_efield = (double*)calloc(3, sizeof(double));
+  read_settings(mds, R.rs, R.metal);
+
+  R.time_used = (double**)malloc(sizeof(double*));
+  R.time_used[0] = (double*)calloc(15, sizeof(double));
+
+  RunSet* rs = R.rs;
+  if (!strcmp(rs->coulomb_type, "cut_off")) rs->use_coulomb = 0;
+  else if (!strcmp(rs->coulomb_type, "wolf_sum")) rs->use_coulomb = 1;
+  else { printf("unknown coulomb_type\n"); exit(1); }
+  rs->rCut2 = rs->rCut * rs->rCut;
+  rs->w_a2_sqrtPI = rs->w_alpha * 2.0 * INV_SQRT_PI;
+  rs->w_Const = erfc(rs->w_alpha * rs->rCut) / rs->rCut2 +
+                rs->w_a2_sqrtPI * exp(-rs->w_alpha * rs->w_alpha * rs->rCut2) / rs->rCut;
+  rs->w_erfc_arCut = erfc(rs->w_alpha * rs->rCut) / rs->rCut;
+  if (!strcmp(rs->vdw_type, "cut_off")) rs->use_vdw = 0;
+  else if (!strcmp(rs->vdw_type, "shifted")) rs->use_vdw = 1;
+  else { printf("unknown vdw_type\n"); exit(1); }
+  rs->inv_rc12 = 1.0 / pow(rs->rCut, 12.0);
+  rs->inv_rc6 = 1.0 / pow(rs->rCut, 6.0);
+
+  Topol* t = R.top = (Topol*)calloc(1, sizeof(Topol));
+  Metal* m = R.metal;
+  read_param_1(par, t, m);
+  m->cpff_chg = (double*)calloc(m->n_NPs + 1, sizeof(double));
+  m->start_NP = (int*)calloc(m->n_NPs + 1, sizeof(int));
+  m->end_NP = (int*)calloc(m->n_NPs + 1, sizeof(int));
+  t->atom_num = (int*)calloc(t->mol_types, sizeof(int));
+  t->mol_num = (int*)calloc(t->mol_types, sizeof(int));
+  t->atom_param = (AtomParam**)calloc(t->mol_types, sizeof(AtomParam*));
+  NonbondedParam* nb = (NonbondedParam*)calloc(t->n_types * t->n_types, sizeof(NonbondedParam));
+  t->nonbonded_param = (NonbondedParam**)calloc(t->n_types, sizeof(NonbondedParam*));
+  for (int i = 0; i < t->n_types; ++i) t->nonbonded_param[i] = nb + t->n_types * i;
+  int* bonded = (int*)calloc(t->mol_types * 6, sizeof(int));
+  t->n_bonds = bonded;
+  t->n_pairs = bonded + t->mol_types;
+  t->n_angles = bonded + 2 * t->mol_types;
+  t->n_dihedrals = bonded + 3 * t->mol_types;
+  t->n_vsites = bonded + 4 * t->mol_types;
+  t->n_constraints = bonded + 5 * t->mol_types;
+  t->vsite_funct = (int**)calloc(t->mol_types, sizeof(int*));
+  t->bond_param = (BondParam**)calloc(t->mol_types, sizeof(BondParam*));
+  t->pair_param = (PairParam**)calloc(t->mol_types, sizeof(PairParam*));
+  t->angle_param = (AngleParam**)calloc(t->mol_types, sizeof(AngleParam*));
+  t->dihedral_param = (DihedralParam**)calloc(t->mol_types, sizeof(DihedralParam*));
+  t->vsite_4 = (VSite_4**)calloc(t->mol_types, sizeof(VSite_4*));
+  t->constraint = (Constraint**)calloc(t->mol_types, sizeof(Constraint*));
+  t->exclude = (int***)calloc(t->mol_types, sizeof(int**));
+  if (m->min >= 0 && m->max >= m->min) m->inv_sqrt_dens = (double*)calloc(m->num, sizeof(double));
+  read_param_2(par, t, m);
+  R.nAtoms = t->n_atoms;
+  R.nMols = t->n_mols;
+  int nVS = 0, nCstr = 0;
+  for (int mol = 0; mol < t->mol_types; ++mol) {
+    nVS += t->n_vsites[mol] * t->mol_num[mol];
+    nCstr += t->n_constraints[mol] * t->mol_num[mol];
+  }
+  m->inv_polar = 1.0 / m->cpff_polar;
+  m->inv_capac = 1.0 / m->cpff_capac;
+  double R_q = sqrt(2.0 / M_PI) * m->cpff_capac;
+  double R_p = pow(sqrt(2.0 / M_PI) * m->cpff_polar / 3.0, 1.0 / 3.0);
+  m->inv_R_qq = 1.0 / sqrt(R_q * R_q + R_q * R_q);
+  m->inv_R_pq = 1.0 / sqrt(R_p * R_p + R_q * R_q);
+  m->inv_R_pp = 1.0 / sqrt(R_p * R_p + R_p * R_p);
+
+  Task* k = R.task = (Task*)calloc(1, sizeof(Task));
+  int* se = (int*)calloc(6, sizeof(int));
+  k->start_mol = se; k->end_mol = se + 1; k->start_atom = se + 2; k->end_atom = se + 3;
+  k->start_metal = se + 4; k->end_metal = se + 5;
+  find_start_end(k->start_mol, k->end_mol, R.nMols, 1);
+  find_start_end(k->start_atom, k->end_atom, R.nAtoms, 1);
+  find_start_end(k->start_metal, k->end_metal, m->num, 1);
+  long* lse = (long*)calloc(2, sizeof(long));
+  k->start_pair = lse; k->end_pair = lse + 1;
+  int nA = R.nAtoms;
+  long n_pairs = nA * (nA - 1) / 2;   // int arithmetic, as the reference (mpi-main.c:406)
+  find_start_end_long(k->start_pair, k->end_pair, n_pairs, 1);
+
+  R.ai = (Atom_Info*)calloc(R.nAtoms, sizeof(Atom_Info));
+  R.mi = (Mol_Info*)calloc(R.nMols, sizeof(Mol_Info));
+  assign_indices(t, m, R.ai, R.mi);
+
+  System* s = R.sys = (System*)calloc(1, sizeof(System));
+  double* pot = (double*)calloc(30, sizeof(double));
+  s->potential = pot; s->partial_pot = pot + 15;
+  s->virial = (double**)calloc(3, sizeof(double*));
+  s->partial_vir = (double**)calloc(3, sizeof(double*));
+  double* vir = (double*)calloc(18, sizeof(double));
+  for (int i = 0; i < 3; ++i) { s->virial[i] = vir + 3 * i; s->partial_vir[i] = vir + 9 + 3 * i; }
+  s->box = (double*)calloc(6, sizeof(double));
+  double* rvf = (double*)calloc((size_t)nA * 21, sizeof(double));
+  s->rx = rvf; s->ry = rvf + nA; s->rz = rvf + 2 * nA;
+  s->vx = rvf + 3 * nA; s->vy = rvf + 4 * nA; s->vz = rvf + 5 * nA;
+  s->fx = rvf + 6 * nA; s->fy = rvf + 7 * nA; s->fz = rvf + 8 * nA;
+  s->partial_fx = rvf + 9 * nA; s->partial_fy = rvf + 10 * nA; s->partial_fz = rvf + 11 * nA;
+  s->old_rx = rvf + 12 * nA; s->old_ry = rvf + 13 * nA; s->old_rz = rvf + 14 * nA;
+  s->old_fx = rvf + 15 * nA; s->old_fy = rvf + 16 * nA; s->old_fz = rvf + 17 * nA;
+  s->sx = rvf + 18 * nA; s->sy = rvf + 19 * nA; s->sz = rvf + 20 * nA;
+  s->num_nhc = 10;
+  s->vQ = (double*)calloc(10, sizeof(double));
+  s->aQ = (double*)calloc(10, sizeof(double));
+  int groN = 0;
+  read_gro(gro, s, &groN, R.ai);
+  if (groN != nA) { printf("atom count mismatch\n"); exit(1); }
+  s->box[3] = s->box[0] * 0.5; s->box[4] = s->box[1] * 0.5; s->box[5] = s->box[2] * 0.5;
+  s->volume = s->box[0] * s->box[1] * s->box[2];
+  s->inv_volume = 1.0 / s->volume;
+  s->ndf = 3 * (nA - nVS) - 3 - nCstr;
+  rs->kT = K_BOLTZ * rs->ref_temp;
+  s->qMass = (double)s->ndf * rs->kT * rs->tau_temp * rs->tau_temp;
+  rs->dt_2 = 0.5 * rs->dt;
+
+  R.n_mat = m->num * 4 + m->n_NPs;
+  int nm = R.n_mat > 0 ? R.n_mat : 1;
+  double* relay = (double*)calloc(nm * 3, sizeof(double));
+  m->vec_pq = relay; m->vec_ext = relay + nm; m->diag_relay = relay + 2 * nm;
+  for (int i = 0; i < nm; ++i) m->diag_relay[i] = 1.0;
+  double* b = (double*)calloc(nm * 11, sizeof(double));
+  R.bcg = (Bicgstab*)calloc(1, sizeof(Bicgstab));
+  double** vecs[11] = {&R.bcg->Ax, &R.bcg->r0, &R.bcg->r, &R.bcg->p, &R.bcg->v, &R.bcg->s,
+                       &R.bcg->t, &R.bcg->y, &R.bcg->z, &R.bcg->Kt, &R.bcg->K};
+  for (int i = 0; i < 11; ++i) *vecs[i] = b + i * nm;
+}
+
+static void dump_fpv(Ref& R, const char* tag) {
+  char nm[128];
+  System* s = R.sys;
+  snprintf(nm, sizeof nm, "%s_f", tag);
+  npy_d(nm, s->fx, 3 * (size_t)R.nAtoms);    // fx | fy | fz blocks are contiguous
+  snprintf(nm, sizeof nm, "%s_pot", tag);
+  npy_d(nm, s->potential, 15);
+  snprintf(nm, sizeof nm, "%s_vir", tag);
+  npy_d(nm, s->virial[0], 9);
+}
+
+static void phases(Ref& R) {
+  System* s = R.sys;
+  Metal* m = R.metal;
+  int nA = R.nAtoms;
+  npy_d("pos", s->rx, 3 * (size_t)nA);
+  npy_d("vel", s->vx, 3 * (size_t)nA);
+  double box[3] = {s->box[0], s->box[1], s->box[2]};
+  npy_d("box", box, 3);
+
+  zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);
+  mpi_nonb_pair(R.top, R.task, R.ai, R.rs, s, 0);
+  dump_fpv(R, "nonb");
+
+  if (m->min >= 0 && m->num > 0) {
+    zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);
+    mpi_qsc_dens(R.task, R.rs->rCut2, m, s, 0, 1);
+    npy_d("sc_inv_sqrt_dens", m->inv_sqrt_dens, m->num);
+    mpi_qsc_force(R.task, R.rs->rCut2, m, s, 0);
+    dump_fpv(R, "sc");
+
+    mpi_cpff_vec_ext(R.task, m, R.rs, R.ai, R.top, s, 0);
+    npy_d("vec_ext", m->vec_ext, R.n_mat);
+    long nnz = 0;
+    mpi_cpff_mat_relay_COO(R.task, m, s, R.rs->rCut2, 0, 1, &nnz);
+    npy_d("diag_relay", m->diag_relay, R.n_mat);
+    npy_d("coo_val", m->vec_val.data(), nnz);
+    npy("coo_row", m->vec_row_ind.data(), "<i8", nnz, 8);
+    npy("coo_col", m->vec_col_ind.data(), "<i8", nnz, 8);
+    // solve from a zero start, as at the first force call of a run
+    for (int i = 0; i < R.n_mat; ++i) m->vec_pq[i] = 0.0;
+    mpi_precon_bicg_stab_COO(R.task, m, R.n_mat, 0, 1, nnz, R.bcg);
+    npy_d("vec_pq", m->vec_pq, R.n_mat);
+
+    zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);
+    mpi_cpff_force(R.task, m, R.rs, s, m->vec_pq, nA, R.ai, 0);
+    dump_fpv(R, "cpff");
+    for (int i = 0; i < R.n_mat; ++i) m->vec_pq[i] = 0.0;
+  }
+
+  mpi_force(R.task, R.top, R.ai, R.mi, R.rs, m, s, R.bcg, 0, 1, R.time_used);
+  sum_potential(s->potential);
+  dump_fpv(R, "force");
+  if (R.n_mat > 0) npy_d("force_vec_pq", m->vec_pq, R.n_mat);
+}
+
+// MD loop body, restating mpi-main.c:874-1013 (one rank, no file output)
+static void md(Ref& R, int nsteps) {
+  System* s = R.sys;
+  RunSet* rs = R.rs;
+  Metal* m = R.metal;
+  int nA = R.nAtoms;
+  remove_comm(nA, R.ai, s);
+  kinetic_energy(s, nA, R.ai);
+  s->first_temp = s->inst_temp;
+  s->ext_temp = rs->ref_temp;
+  std::vector<double> temps, ekin, nhc;
+  for (int step = 1; step <= nsteps; ++step) {
+    if (step < rs->nHeating)
+      rs->ref_temp = s->first_temp + (s->ext_temp - s->first_temp) * step / rs->nHeating;
+    else
+      rs->ref_temp = s->ext_temp;
+    rs->kT = K_BOLTZ * rs->ref_temp;
+    if (!strcmp(rs->ensemble, "nvt")) nose_hoover_chain(rs, s, nA);
+    for (int i = 0; i < nA; ++i) {
+      if (m->fix_pos == 1 && R.ai[i].is_metal == 1) continue;
+      double im = R.ai[i].inv_mass;
+      s->vx[i] += rs->dt_2 * s->fx[i] * im;
+      s->vy[i] += rs->dt_2 * s->fy[i] * im;
+      s->vz[i] += rs->dt_2 * s->fz[i] * im;
+    }
+    for (int i = 0; i < nA; ++i) {
+      if (m->fix_pos == 1 && R.ai[i].is_metal == 1) continue;
+      s->old_rx[i] = s->rx[i]; s->old_ry[i] = s->ry[i]; s->old_rz[i] = s->rz[i];
+      s->rx[i] += rs->dt * s->vx[i];
+      s->ry[i] += rs->dt * s->vy[i];
+      s->rz[i] += rs->dt * s->vz[i];
+    }
+    rattle_1st(rs->dt, R.mi, R.ai, R.top, s);
+    mpi_force(R.task, R.top, R.ai, R.mi, rs, m, s, R.bcg, 0, 1, R.time_used);
+    sum_potential(s->potential);
+    for (int i = 0; i < nA; ++i) {
+      if (m->fix_pos == 1 && R.ai[i].is_metal == 1) continue;
+      double im = R.ai[i].inv_mass;
+      s->vx[i] += rs->dt_2 * s->fx[i] * im;
+      s->vy[i] += rs->dt_2 * s->fy[i] * im;
+      s->vz[i] += rs->dt_2 * s->fz[i] * im;
+    }
+    rattle_2nd(rs->dt, R.mi, R.ai, R.top, s);
+    kinetic_energy(s, nA, R.ai);
+    if (!strcmp(rs->ensemble, "nvt")) nose_hoover_chain(rs, s, nA);
+    remove_comm(nA, R.ai, s);
+    kinetic_energy(s, nA, R.ai);
+    if (step % rs->nSave == 0) apply_pbc(R.nMols, R.mi, s->rx, s->ry, s->rz, s->box);
+    char tag[64];
+    snprintf(tag, sizeof tag, "md%03d", step);
+    dump_fpv(R, tag);
+    char nm[96];
+    snprintf(nm, sizeof nm, "%s_pos", tag);
+    npy_d(nm, s->rx, 3 * (size_t)nA);
+    snprintf(nm, sizeof nm, "%s_vel", tag);
+    npy_d(nm, s->vx, 3 * (size_t)nA);
+    snprintf(nm, sizeof nm, "%s_vir", tag);   // virial after RATTLE_2 additions
+    npy_d(nm, s->virial[0], 9);
+    if (R.n_mat > 0) {
+      snprintf(nm, sizeof nm, "%s_vec_pq", tag);
+      npy_d(nm, m->vec_pq, R.n_mat);
+    }
+    temps.push_back(s->inst_temp);
+    ekin.push_back(s->eKsum);
+    for (int k = 0; k < 10; ++k) nhc.push_back(s->vQ[k]);
+  }
+  npy_d("md_temp", temps.data(), temps.size());
+  npy_d("md_eksum", ekin.data(), ekin.size());
+  npy_d("md_vq", nhc.data(), nhc.size());
+}
+
+int main(int argc, char** argv) {
+  if (argc < 5) {
+    fprintf(stderr, "usage: %s init.gro param.txt mdset.txt OUTDIR [nsteps]\n", argv[0]);
+    return 2;
+  }
+  MPI_Init(&argc, &argv);
+  g_out = argv[4];
+  Ref R;
+  setup(R, argv[1], argv[2], argv[3]);
+  phases(R);
+  if (argc > 5) md(R, atoi(argv[5]));
+  MPI_Finalize();
+  return 0;
+}
