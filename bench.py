@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark: ns/day and CPIM solve ms/step of the MI355X force + CPIM path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+                    [--precision fp64|fp32|both] [--no-cpu]
+
+Workload (default): Au561 Mackay icosahedron (6 shells) in a 6 nm box with
+SPC/E water at 1.0 g/cm^3 (the seeded config-1 deck of capacmd_amd.gen),
+Sutton-Chen gold, DSF water electrostatics (alpha 2 nm^-1, rc 1.2 nm), CPIM
+on the gold (2,245 unknowns), Nose-Hoover NVT, dt = 1 fs.  A "step" is one
+full MD step on the device: NHC, velocity Verlet + RATTLE, neighbour build,
+pair/Sutton-Chen/CPIM forces with the BiCGSTAB solve, the molecule wrap at
+the save cadence.  Inputs are resident in HBM before the timed region; the
+library returns only after each step's work has finished (it synchronises on
+the solver's convergence flag), so wall time around the K steps is device
+time.  ns/day = 0.0864 / (seconds per step) at dt = 1 fs.
+
+Prints one JSON line (rank 0; there is one GPU).  The roofline object is for
+the relay SpMV, the kernel inside the solve (timed live with HIP events on
+the library's stream); cpu_baseline times the reference program built from
+its own sources (oracle/_ref, one MPI rank on one core) on a one-step sample
+of the same deck, or the oracle restatement where that binary cannot run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from capacmd_amd import gen, lib  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nm: int, n_mat: int, w: int) -> float:
+    """Algorithmic bytes of one relay SpMV: the unique entries of the
+    symmetric relay, w (8 N_m^2 - 3 N_m), plus 16 n_mat for the vectors."""
+    return w * (8.0 * nm * nm - 3.0 * nm) + 16.0 * n_mat
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), as committed under
+    profiles/; None when no such summary exists."""
+    path = os.path.join(PROFILES, "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def run_gpu(deck, precision: int, steps: int, warmup: int):
+    ctx = lib.Context(deck, precision=precision)
+    ctx.force()
+    ctx.md_begin()
+    n_save = max(deck.n_save, 1)
+    step = 1
+    if warmup:
+        done = ctx.md_run(step, warmup, n_save)
+        step += done
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    done = ctx.md_run(step, steps, n_save)
+    t1 = time.perf_counter()
+    assert done == steps, done
+    t, calls, iters = ctx.timing()
+    obs = ctx.observables()
+    ctx.enable_timing(False)
+    sec = (t1 - t0) / steps
+    out = {
+        "ms_per_step": 1e3 * sec,
+        "ns_per_day": 0.0864 / sec,
+        "solve_ms_per_step": 1e3 * t[7] / steps,
+        "matrix_ms_per_step": 1e3 * t[5] / steps,
+        "pair_ms_per_step": 1e3 * t[4] / steps,
+        "neighbour_ms_per_step": 1e3 * t[10] / steps,
+        "cpim_force_ms_per_step": 1e3 * t[8] / steps,
+        "sc_ms_per_step": 1e3 * t[2] / steps,
+        "rhs_ms_per_step": 1e3 * t[6] / steps,
+        "integrator_ms_per_step": 1e3 * t[12] / steps,
+        "solve_iters_per_step": iters / steps,
+        "final_temp": obs["temp"],
+        "final_energy": float(obs["pot"][0]),
+    }
+    kern = {}
+    for name, kid in (("spmv", lib.KERNEL_SPMV), ("assemble", lib.KERNEL_ASSEMBLE),
+                      ("pair", lib.KERNEL_PAIR), ("neighbour", lib.KERNEL_NEIGHBOUR),
+                      ("cpim_mm", lib.KERNEL_CPIM_MM)):
+        kern[name] = ctx.time_kernel(kid, 20)
+    out["kernel_ms"] = kern
+    ctx.close()
+    return out
+
+
+def cpu_reference(deck_dir: str, steps: int = 1):
+    """The reference program on the host: one MPI rank, `steps` MD steps."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "capacmd_ref")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in ("init.gro", "param.txt", "mdset.txt"):
+            shutil.copy(os.path.join(deck_dir, f), tmp)
+        mds = os.path.join(tmp, "mdset.txt")
+        txt = open(mds).read()
+        txt = re.sub(r"number_of_steps\s+\d+", f"number_of_steps   {steps}", txt)
+        txt = re.sub(r"save_step\s+\d+", "save_step         1000000", txt)
+        open(mds, "w").write(txt)
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        try:
+            res = subprocess.run([exe, "-gro", "init.gro", "-par", "param.txt", "-mds", "mdset.txt"],
+                                 cwd=tmp, env=env, capture_output=True, text=True, timeout=300)
+        except Exception as e:  # noqa: BLE001
+            log("reference program did not run:", e)
+            return None
+        if res.returncode != 0:
+            log("reference program failed:", res.stdout[-500:], res.stderr[-500:])
+            return None
+        m = re.search(r"Total time used in force:\s+([0-9.]+)", res.stdout)
+        s = re.search(r"Time used in CPIM-solve:\s+([0-9.]+)", res.stdout)
+        if not m:
+            return None
+        calls = steps + 1                       # the initial force call + one per step
+        sec = float(m.group(1)) / calls
+        return {"value": 0.0864 / sec, "unit": "ns/day", "cores": 1, "kind": "reference",
+                "ms_per_step": 1e3 * sec,
+                "solve_ms_per_step": 1e3 * float(s.group(1)) / calls if s else None,
+                "sample": f"reference program (oracle/_ref/capacmd_ref, 1 MPI rank), same deck, "
+                          f"{steps} MD step(s) = {calls} force calls incl. the first solve from zero"}
+
+
+def cpu_port(deck, steps: int = 1):
+    """The oracle restatement on one core, same sample."""
+    from oracle.oracle import Oracle
+    o = Oracle(deck)
+    t0 = time.perf_counter()
+    o.force()
+    o.md_begin()
+    for s in range(1, steps + 1):
+        o.md_step(s, 10 ** 9)
+    sec = (time.perf_counter() - t0) / (steps + 1)
+    t = o.timers()
+    return {"value": 0.0864 / sec, "unit": "ns/day", "cores": 1, "kind": "port",
+            "ms_per_step": 1e3 * sec, "solve_ms_per_step": 1e3 * t[7] / (steps + 1),
+            "sample": f"oracle restatement, 1 core, same deck, {steps} MD step(s) = "
+                      f"{steps + 1} force calls"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the whole box lives on one MI355X)", file=sys.stderr)
+        return 2
+    deck_spec = gen.config(a.config)
+    tmp = tempfile.mkdtemp(prefix="capac_bench_")
+    gen.write_deck(deck_spec, tmp)
+    deck = lib.Deck.from_dir(tmp)
+    log(f"deck {deck_spec.name}: N={deck.n_atoms} Nm={deck.n_metal} n_mat={deck.n_mat}")
+
+    paths = {}
+    precs = {"fp64": lib.PREC_FP64, "fp32": lib.PREC_FP32_REFINED}
+    wanted = ["fp64", "fp32"] if a.precision == "both" else [a.precision]
+    for name in wanted:
+        try:
+            paths[name] = run_gpu(deck, precs[name], a.steps, a.warmup)
+            log(name, json.dumps(paths[name]))
+        except lib.CapacError as e:
+            log(f"{name} path unavailable: {e}")
+    if not paths:
+        print("no GPU path ran", file=sys.stderr)
+        return 1
+    best = "fp32" if "fp32" in paths else "fp64"
+    main_path = paths[best]
+
+    nm, n_mat = deck.n_metal, deck.n_mat
+    w = 8 if best == "fp64" else 4
+    spmv_ms = main_path["kernel_ms"]["spmv"]
+    achieved = spmv_bytes(nm, n_mat, 8) / (spmv_ms * 1e-3) / 1e9 if best == "fp64" else \
+        spmv_bytes(nm, n_mat, 4) / (spmv_ms * 1e-3) / 1e9
+    roofline = {"kernel": "relay SpMV (k_spmv64)" if best == "fp64" else "relay SpMV (FP32)",
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles("spmv_" + best),
+                "algorithmic_bytes_per_launch": spmv_bytes(nm, n_mat, w),
+                "ms_per_launch": spmv_ms}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_reference(tmp, 1)
+        if cpu is None:
+            cpu = cpu_port(deck, 1)
+    shutil.rmtree(tmp, ignore_errors=True)
+
+    line = {
+        "metric": "ns/day + CPIM solve ms/step (Au NP in water); FP32-refined vs FP64, 1 GPU",
+        "value": round(main_path["ns_per_day"], 4),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(main_path["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32-matrix/f64-refined",
+        "data": "synthetic (seeded generator: Mackay icosahedron + SPC/E lattice water)",
+        "config": {"workload": f"{deck_spec.name}: Au{nm} + {deck_spec.n_water} SPC/E waters, "
+                               f"{deck_spec.box[0]:g} nm box, {n_mat} CPIM unknowns, MD step dt 1 fs",
+                   "atoms": deck.n_atoms, "path": best},
+        "solve_ms_per_step": round(main_path["solve_ms_per_step"], 4),
+        "paths": paths,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,858 @@
+// C ABI of libcapac_gpu.so (include/capac_gpu.h): context set-up, the force
+// call that replaces the reference's mpi_force (force.c:2097-2320), the
+// sub-operator entry points and the device MD loop (mpi-main.c:874-1013).
+#include "capac_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static thread_local char g_err[1024] = "";
+
+void capac_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char* capac_last_error(void) { return g_err; }
+extern "C" const char* capac_version(void) { return "capac_gpu 0.1 (gfx950, FP64 + FP32-refined CPIM)"; }
+
+int pair_blocks(int N);
+
+namespace {
+
+__global__ void k_reduce(const double* __restrict__ partial, int nblk, int nval, double* __restrict__ out) {
+    __shared__ double s_w[16][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] += partial[b * 16 + v];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        double s = wave_sum(acc[v]);
+        if (lane == 0) s_w[wave][v] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < nval) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+__global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+__global__ void k_fill_i(int* __restrict__ p, size_t n, int v) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+__global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, size_t n) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = a[k];
+}
+
+}  // namespace
+
+int reduce_partials(capac_ctx* c, int nblk, int nval, int off) {
+    k_reduce<<<1, 1024, 0, c->stream>>>(c->partial, nblk, nval, c->red + off);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+static int fill_d(capac_ctx* c, double* p, size_t n, double v) {
+    if (n == 0) return 0;
+    k_fill_d<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n, v);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+static int fill_i(capac_ctx* c, int* p, size_t n, int v) {
+    if (n == 0) return 0;
+    k_fill_i<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n, v);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int fill_zero_i(capac_ctx* c, int* p, size_t n) { return fill_i(c, p, n, 0); }
+int fill_zero_d(capac_ctx* c, double* p, size_t n) { return fill_d(c, p, n, 0.0); }
+
+static int copy_d(capac_ctx* c, const double* a, double* b, size_t n) {
+    if (n == 0) return 0;
+    k_copy_d<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(a, b, n);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int zero_outputs(capac_ctx* c) {
+    int rc = fill_d(c, c->red, 256, 0.0);
+    if (!rc) rc = fill_d(c, c->f, 3 * (size_t)c->bc.N, 0.0);
+    return rc;
+}
+
+// ------------------------------------------------------------------ set-up
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t n) {
+    size_t bytes = (n ? n : 1) * sizeof(T);
+    CHECK_HIP(hipMalloc(dst, bytes));
+    if (src && n) CHECK_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    else CHECK_HIP(hipMemset(*dst, 0, bytes));
+    return 0;
+}
+
+static int alloc_nbr(capac_ctx* c, int stride) {
+    if (c->nbr) (void)hipFree(c->nbr);
+    c->nbr = NULL;
+    c->nb_stride = stride;
+    CHECK_HIP(hipMalloc(&c->nbr, (size_t)c->bc.N * stride * sizeof(int)));
+    return 0;
+}
+
+static int alloc_cells(capac_ctx* c) {
+    int* arrs[4] = {c->cell_count, c->cell_start, c->cell_fill, NULL};
+    for (int k = 0; k < 3; ++k)
+        if (arrs[k]) (void)hipFree(arrs[k]);
+    CHECK_HIP(hipMalloc(&c->cell_count, sizeof(int) * (c->n_cell + 1)));
+    CHECK_HIP(hipMalloc(&c->cell_start, sizeof(int) * (c->n_cell + 1)));
+    CHECK_HIP(hipMalloc(&c->cell_fill, sizeof(int) * (c->n_cell + 1)));
+    return 0;
+}
+
+static void derive_constants(capac_ctx* c, const capac_params* p) {
+    BoxConst& b = c->bc;
+    for (int d = 0; d < 3; ++d) { b.box[d] = p->box[d]; b.half[d] = p->box[d] * 0.5; }
+    // the reference's derived constants, same expressions (mpi-main.c:170-178,
+    // 202-203, 315-323)
+    b.rc = p->rcut;
+    b.rc2 = p->rcut * p->rcut;
+    b.alpha = p->dsf_alpha;
+    b.a2_sqrtPI = b.alpha * 2.0 * K_INV_SQRT_PI;
+    b.wolfConst = erfc(b.alpha * b.rc) / b.rc2 + b.a2_sqrtPI * exp(-b.alpha * b.alpha * b.rc2) / b.rc;
+    b.erfc_arCut = erfc(b.alpha * b.rc) / b.rc;
+    b.inv_rc12 = 1.0 / pow(b.rc, 12.0);
+    b.inv_rc6 = 1.0 / pow(b.rc, 6.0);
+    b.use_coulomb = p->use_coulomb;
+    b.use_vdw = p->use_vdw;
+    b.qsc_n = p->qsc_n; b.qsc_m = p->qsc_m;
+    b.qsc_eps = p->qsc_eps; b.qsc_c = p->qsc_c; b.qsc_a = p->qsc_a;
+    b.inv_polar = 1.0 / p->cpff_polar;
+    b.inv_capac = 1.0 / p->cpff_capac;
+    const double pi = 3.14159265358979323846;
+    double R_q = sqrt(2.0 / pi) * p->cpff_capac;
+    double R_p = pow(sqrt(2.0 / pi) * p->cpff_polar / 3.0, 1.0 / 3.0);
+    b.inv_R_qq = 1.0 / sqrt(R_q * R_q + R_q * R_q);
+    b.inv_R_pq = 1.0 / sqrt(R_p * R_p + R_q * R_q);
+    b.inv_R_pp = 1.0 / sqrt(R_p * R_p + R_p * R_p);
+    for (int d = 0; d < 3; ++d) b.efield[d] = p->efield[d];
+}
+
+static int validate(const capac_params* p) {
+    if (!p || p->n_atoms <= 0) { capac_set_error("n_atoms must be positive"); return CAPAC_ERR_ARG; }
+    if (p->n_metal < 0 || p->n_metal > p->n_atoms) {
+        capac_set_error("n_metal %d outside [0, n_atoms]", p->n_metal);
+        return CAPAC_ERR_ARG;
+    }
+    if (p->n_types < 1 || p->n_types > 8) {
+        capac_set_error("%d vdW types; this build supports 1..8", p->n_types);
+        return CAPAC_ERR_UNSUPPORTED;
+    }
+    for (int k = 0; k < p->n_types * p->n_types; ++k)
+        if (p->nb_funct[k] != 1 && p->nb_funct[k] != 4) {
+            capac_set_error("vdW funct %d (Buckingham/Morse) is not supported by this build",
+                            p->nb_funct[k]);
+            return CAPAC_ERR_UNSUPPORTED;
+        }
+    for (int i = 0; i < p->n_atoms; ++i) {
+        if (p->atomtype[i] < 0 || p->atomtype[i] >= p->n_types) {
+            capac_set_error("atom %d has type %d outside the pair table", i, p->atomtype[i]);
+            return CAPAC_ERR_ARG;
+        }
+        if (p->local_idx[i] < 0 || p->local_idx[i] >= 32) {
+            capac_set_error("atom %d: molecules of more than 32 atoms are not supported", i);
+            return CAPAC_ERR_UNSUPPORTED;
+        }
+    }
+    for (int i = 0; i < p->n_metal; ++i) {
+        int m = p->mol_id[i];
+        if (p->excl_mask[i] != 0 && p->mol_last[m] > p->mol_first[m]) {
+            capac_set_error("metal atom %d has intramolecular exclusions; metal atoms must not "
+                            "exclude each other", i);
+            return CAPAC_ERR_UNSUPPORTED;
+        }
+    }
+    for (int k = 0; k < p->n_nps; ++k) {
+        if (p->np_first[k] < 0 || p->np_last[k] >= p->n_metal || p->np_first[k] > p->np_last[k]) {
+            capac_set_error("nanoparticle %d range %d..%d outside the metal atoms", k, p->np_first[k],
+                            p->np_last[k]);
+            return CAPAC_ERR_ARG;
+        }
+        for (int l = 0; l < k; ++l)
+            if (p->np_first[k] <= p->np_last[l] && p->np_first[l] <= p->np_last[k]) {
+                capac_set_error("nanoparticles %d and %d overlap", l, k);
+                return CAPAC_ERR_ARG;
+            }
+    }
+    for (int d = 0; d < 3; ++d)
+        if (!(p->box[d] > 0.0)) { capac_set_error("box must be positive"); return CAPAC_ERR_ARG; }
+    if (!(p->rcut > 0.0)) { capac_set_error("cut-off must be positive"); return CAPAC_ERR_ARG; }
+    if (p->n_metal > 0 && p->use_cpff && (!(p->cpff_polar > 0.0) || !(p->cpff_capac > 0.0))) {
+        capac_set_error("CPIM polarizability and capacitance must be positive");
+        return CAPAC_ERR_ARG;
+    }
+    return 0;
+}
+
+extern "C" int capac_create(const capac_params* p, int precision, int device, capac_ctx** out) {
+    if (!out) { capac_set_error("out is NULL"); return CAPAC_ERR_ARG; }
+    *out = NULL;
+    int rc = validate(p);
+    if (rc) return rc;
+    if (precision == CAPAC_PREC_FP32_REFINED) {
+        capac_set_error("the FP32-refined relay path is not built yet");
+        return CAPAC_ERR_UNSUPPORTED;
+    }
+    if (precision != CAPAC_PREC_FP64) {
+        capac_set_error("unknown precision %d", precision);
+        return CAPAC_ERR_ARG;
+    }
+    CHECK_HIP(hipSetDevice(device));
+    capac_ctx* c = (capac_ctx*)calloc(1, sizeof(capac_ctx));
+    if (!c) { capac_set_error("out of host memory"); return CAPAC_ERR_ARG; }
+    c->device = device;
+    c->precision = precision;
+    const int N = p->n_atoms, Nm = p->n_metal;
+    BoxConst& b = c->bc;
+    b.N = N; b.Nm = Nm; b.n_np = Nm > 0 ? p->n_nps : 0; b.n_types = p->n_types; b.n_mols = p->n_mols;
+    derive_constants(c, p);
+    c->n_mat = (Nm > 0) ? 4 * Nm + b.n_np : 0;
+    c->n_cstr = p->n_cstr;
+    c->ndf = p->ndf;
+    c->fix_metal = p->fix_metal;
+    c->nvt = p->nvt;
+    c->n_heating = p->n_heating;
+    c->dt = p->dt;
+    c->tau_t = p->tau_t;
+    c->ref_temp0 = p->ref_temp;
+    c->ext_temp = p->ref_temp;
+    c->cur_ref_temp = p->ref_temp;
+    // n_cstr and use_cpff bookkeeping
+    int use_cpff = p->use_cpff && Nm > 0;
+    if (!use_cpff) c->n_mat = 0;
+
+    rc = 0;
+#define TRY(x) do { if ((rc = (x)) != 0) { capac_destroy(c); return rc; } } while (0)
+    TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+    size_t n3 = 3 * (size_t)N;
+    TRY(upload(&c->r, (const double*)NULL, n3));
+    TRY(upload(&c->v, (const double*)NULL, n3));
+    TRY(upload(&c->f, (const double*)NULL, n3));
+    TRY(upload(&c->old_r, (const double*)NULL, n3));
+    TRY(upload(&c->charge, p->charge, N));
+    TRY(upload(&c->mass, p->mass, N));
+    {
+        double* im = (double*)malloc(sizeof(double) * N);
+        for (int i = 0; i < N; ++i) im[i] = p->mass[i] > 0.0 ? 1.0 / p->mass[i] : 0.0;
+        rc = upload(&c->inv_mass, (const double*)im, N);
+        free(im);
+        if (rc) { capac_destroy(c); return rc; }
+    }
+    TRY(upload(&c->type, p->atomtype, N));
+    TRY(upload(&c->mol_id, p->mol_id, N));
+    TRY(upload(&c->local_idx, p->local_idx, N));
+    TRY(upload(&c->excl, p->excl_mask, N));
+    {
+        int nt = p->n_types;
+        PairParam* t = (PairParam*)calloc(nt * nt, sizeof(PairParam));
+        for (int k = 0; k < nt * nt; ++k) {
+            t[k].funct = p->nb_funct[k];
+            t[k].c6 = p->nb_c6[k];
+            t[k].c12 = p->nb_c12[k];
+            t[k].width = p->nb_a[k];
+        }
+        rc = upload(&c->pair, (const PairParam*)t, nt * nt);
+        free(t);
+        if (rc) { capac_destroy(c); return rc; }
+    }
+    TRY(upload(&c->mol_first, p->mol_first, p->n_mols));
+    TRY(upload(&c->mol_last, p->mol_last, p->n_mols));
+    TRY(upload(&c->mol_cstr, p->mol_cstr, p->n_mols + 1));
+    TRY(upload(&c->cstr_i, p->cstr_i, p->n_cstr));
+    TRY(upload(&c->cstr_j, p->cstr_j, p->n_cstr));
+    TRY(upload(&c->cstr_r, p->cstr_r, p->n_cstr));
+    TRY(upload(&c->np_first, p->np_first, b.n_np));
+    TRY(upload(&c->np_last, p->np_last, b.n_np));
+    TRY(upload(&c->np_charge, p->np_charge, b.n_np));
+    {
+        int* npo = (int*)malloc(sizeof(int) * (Nm > 0 ? Nm : 1));
+        for (int i = 0; i < Nm; ++i) npo[i] = -1;
+        for (int k = 0; k < b.n_np; ++k)
+            for (int i = p->np_first[k]; i <= p->np_last[k]; ++i) npo[i] = k;
+        rc = upload(&c->np_of_metal, (const int*)npo, Nm);
+        free(npo);
+        if (rc) { capac_destroy(c); return rc; }
+    }
+    // cell list and neighbour rows
+    nb_grid_setup(c);
+    TRY(alloc_cells(c));
+    TRY(upload(&c->cell_of, (const int*)NULL, N));
+    TRY(upload(&c->cell_atoms, (const int*)NULL, N));
+    TRY(upload(&c->cell_tmp, (const int*)NULL, N));
+    TRY(upload(&c->n_front, (const int*)NULL, N));
+    TRY(upload(&c->n_back, (const int*)NULL, N));
+    TRY(upload(&c->nb_overflow, (const int*)NULL, 4));
+    {
+        // room for the densest liquid or metal around a sphere of radius rc
+        double vol = 4.0 / 3.0 * 3.14159265358979 * b.rc * b.rc * b.rc;
+        int est = (int)(vol * 140.0) + 64;
+        int stride = ((est + 63) / 64) * 64;
+        if (stride > N) stride = ((N + 63) / 64) * 64;
+        if (stride < 64) stride = 64;
+        TRY(alloc_nbr(c, stride));
+    }
+    // reductions
+    {
+        int need = pair_blocks(N);
+        int m = (Nm + 3) / 4 + 1;
+        if (m > need) need = m;
+        m = (p->n_mols + 255) / 256 + 1;
+        if (m > need) need = m;
+        m = (N + 255) / 256 + 1;
+        if (m > need) need = m;
+        c->n_partial = need;
+        TRY(upload(&c->partial, (const double*)NULL, (size_t)need * 16));
+        TRY(upload(&c->red, (const double*)NULL, 256));
+        TRY(hipHostMalloc(&c->red_host, 320 * sizeof(double), hipHostMallocDefault) == hipSuccess
+                ? 0 : CAPAC_ERR_HIP);
+    }
+    TRY(upload(&c->isd, (const double*)NULL, Nm));
+    TRY(upload(&c->md, (const double*)NULL, 32));
+    TRY(upload(&c->md_status, (const int*)NULL, 4));
+    {
+        double qm = (double)p->ndf * (K_BOLTZ * p->ref_temp) * p->tau_t * p->tau_t;  // mpi-main.c:552
+        double h[32] = {0};
+        h[25] = qm;
+        h[22] = 1.0;
+        TRY(hipMemcpy(c->md, h, sizeof h, hipMemcpyHostToDevice) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+    }
+    if (c->n_mat > 0) {
+        TRY(upload(&c->b, (const double*)NULL, c->n_mat));
+        TRY(upload(&c->x, (const double*)NULL, c->n_mat));
+        TRY(upload(&c->x_backup, (const double*)NULL, c->n_mat));
+        TRY(upload(&c->diag, (const double*)NULL, c->n_mat));
+        c->relay_ld = ((Nm + 63) / 64) * 64;
+        c->relay_bytes = (size_t)Nm * 16 * c->relay_ld * sizeof(double);
+        CHECK_HIP(hipMalloc(&c->relay, c->relay_bytes));
+        TRY(solver_create(c));
+    }
+    for (int k = 0; k < 16; ++k) TRY(hipEventCreate(&c->ev[k]) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+#undef TRY
+    *out = c;
+    return 0;
+}
+
+extern "C" void capac_destroy(capac_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->r, c->v, c->f, c->old_r, c->charge, c->mass, c->inv_mass, c->type, c->mol_id,
+                    c->local_idx, c->excl, c->pair, c->mol_first, c->mol_last, c->mol_cstr,
+                    c->cstr_i, c->cstr_j, c->cstr_r, c->np_first, c->np_last, c->np_of_metal,
+                    c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
+                    c->cell_tmp, c->cell_fill, c->n_front, c->n_back, c->nbr, c->nb_overflow,
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->md, c->md_status};
+    for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
+        if (ptrs[k]) (void)hipFree(ptrs[k]);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    solver_destroy(c);
+    for (int k = 0; k < 16; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    free(c);
+}
+
+// ------------------------------------------------------------------ state
+
+extern "C" int capac_set_state(capac_ctx* c, const double* r, const double* v, const double* x) {
+    if (!c) return CAPAC_ERR_ARG;
+    size_t n3 = 3 * (size_t)c->bc.N;
+    if (r) CHECK_HIP(hipMemcpyAsync(c->r, r, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (v) CHECK_HIP(hipMemcpyAsync(c->v, v, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (x && c->n_mat)
+        CHECK_HIP(hipMemcpyAsync(c->x, x, c->n_mat * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_get_state(capac_ctx* c, double* r, double* v, double* x) {
+    if (!c) return CAPAC_ERR_ARG;
+    size_t n3 = 3 * (size_t)c->bc.N;
+    if (r) CHECK_HIP(hipMemcpyAsync(r, c->r, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (v) CHECK_HIP(hipMemcpyAsync(v, c->v, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (x && c->n_mat)
+        CHECK_HIP(hipMemcpyAsync(x, c->x, c->n_mat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_get_forces(capac_ctx* c, double* f) {
+    if (!c || !f) return CAPAC_ERR_ARG;
+    CHECK_HIP(hipMemcpyAsync(f, c->f, 3 * (size_t)c->bc.N * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_set_box(capac_ctx* c, const double box[3]) {
+    if (!c || !box) return CAPAC_ERR_ARG;
+    for (int d = 0; d < 3; ++d) {
+        if (!(box[d] > 0.0)) { capac_set_error("box must be positive"); return CAPAC_ERR_ARG; }
+        c->bc.box[d] = box[d];
+        c->bc.half[d] = 0.5 * box[d];
+    }
+    int old = c->n_cell;
+    nb_grid_setup(c);
+    if (c->n_cell > old) return alloc_cells(c);
+    return 0;
+}
+
+// ------------------------------------------------------------------ force
+
+static int fetch_red(capac_ctx* c) {
+    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, 256 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->red_host + 256, c->nb_overflow, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int nb_overflowed(capac_ctx* c) { return ((int*)(c->red_host + 256))[0] != 0; }
+
+static int grow_nbr(capac_ctx* c) {
+    int stride = c->nb_stride * 2;
+    if (stride > ((c->bc.N + 63) / 64) * 64 + 64) stride = ((c->bc.N + 63) / 64) * 64 + 64;
+    return alloc_nbr(c, stride);
+}
+
+static void mark(capac_ctx* c, int k) {
+    if (c->timing_on) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+// one mpi_force: forces into c->f, pot/vir into c->pot/c->vir
+static int force_compute(capac_ctx* c) {
+    const int Nm = c->bc.Nm;
+    int rc;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        mark(c, 0);
+        if ((rc = fill_d(c, c->red, 256, 0.0))) return rc;
+        if (c->n_mat && (rc = copy_d(c, c->x, c->x_backup, c->n_mat))) return rc;
+        if ((rc = nb_build(c))) return rc;
+        mark(c, 1);
+        if ((rc = pair_nonbonded(c, NULL))) return rc;
+        mark(c, 2);
+        if (Nm > 0 && (rc = sc_forces(c))) return rc;
+        mark(c, 3);
+        if (c->n_mat) {
+            if ((rc = cpim_rhs(c))) return rc;
+            mark(c, 4);
+            if ((rc = cpim_assemble(c))) return rc;
+            mark(c, 5);
+            rc = solver_run(c, c->b, c->x);
+            if (rc && rc != CAPAC_ERR_SOLVER) return rc;
+            int solve_rc = rc;
+            mark(c, 6);
+            if ((rc = cpim_forces(c))) return rc;
+            rc = solve_rc;
+        } else {
+            mark(c, 4); mark(c, 5); mark(c, 6);
+        }
+        int rc2 = ext_field(c);
+        if (rc2) return rc2;
+        mark(c, 7);
+        rc2 = fetch_red(c);
+        if (rc2) return rc2;
+        if (!nb_overflowed(c)) break;
+        // a neighbour row overflowed: grow, restore the warm start, redo
+        if ((rc2 = grow_nbr(c))) return rc2;
+        if (c->n_mat && (rc2 = copy_d(c, c->x_backup, c->x, c->n_mat))) return rc2;
+        if (attempt == 7) { capac_set_error("neighbour rows keep overflowing"); return CAPAC_ERR_HIP; }
+    }
+    const double* h = c->red_host;
+    memset(c->pot, 0, sizeof c->pot);
+    memset(c->vir, 0, sizeof c->vir);
+    c->pot[7] = h[0];
+    c->pot[8] = h[1];
+    c->pot[1] = h[16];
+    c->pot[10] = h[32];
+    c->pot[11] = h[33];
+    c->pot[9] = h[48] + h[80];
+    c->pot[12] = h[49];
+    c->pot[13] = h[50];
+    c->pot[14] = h[51];
+    for (int k = 0; k < 9; ++k) c->vir[k] = h[2 + k] + h[17 + k] + h[34 + k] + h[52 + k];
+    c->timing.n_force_calls++;
+    if (c->timing_on) {
+        float ms;
+        const int cat[7] = {10, 4, 2, 6, 5, 7, 8};   // build, pair, SC, RHS, matrix, solve, CPIM force
+        for (int k = 0; k < 7; ++k)
+            if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess)
+                c->timing.t[cat[k]] += 1e-3 * ms;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->timing.t[0] += 1e-3 * ms;
+    }
+    return rc;
+}
+
+extern "C" int capac_force(capac_ctx* c, double pot[CAPAC_NPOT], double vir[9], double* f) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = force_compute(c);
+    if (rc && rc != CAPAC_ERR_SOLVER) return rc;
+    if (pot) memcpy(pot, c->pot, sizeof c->pot);
+    if (vir) memcpy(vir, c->vir, sizeof c->vir);
+    if (f) {
+        int rc2 = capac_get_forces(c, f);
+        if (rc2) return rc2;
+    }
+    return rc;
+}
+
+// ------------------------------------------------------------------ sub-operators
+
+static int build_until_fits(capac_ctx* c) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int rc = nb_build(c);
+        if (rc) return rc;
+        if ((rc = fetch_red(c))) return rc;
+        if (!nb_overflowed(c)) return 0;
+        if ((rc = grow_nbr(c))) return rc;
+    }
+    capac_set_error("neighbour rows keep overflowing");
+    return CAPAC_ERR_HIP;
+}
+
+extern "C" int capac_neighbours(capac_ctx* c, int* count, int* list, int max_per_atom) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = build_until_fits(c);
+    if (rc) return rc;
+    const int N = c->bc.N, S = c->nb_stride;
+    int* nf = (int*)malloc(sizeof(int) * N);
+    int* nb = (int*)malloc(sizeof(int) * N);
+    int* rows = (int*)malloc(sizeof(int) * (size_t)N * S);
+    if (!nf || !nb || !rows) { free(nf); free(nb); free(rows); capac_set_error("out of host memory"); return CAPAC_ERR_ARG; }
+    hipError_t e1 = hipMemcpy(nf, c->n_front, sizeof(int) * N, hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(nb, c->n_back, sizeof(int) * N, hipMemcpyDeviceToHost);
+    hipError_t e3 = hipMemcpy(rows, c->nbr, sizeof(int) * (size_t)N * S, hipMemcpyDeviceToHost);
+    int mx = 0;
+    if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) {
+        for (int i = 0; i < N; ++i) {
+            int n = nf[i] + nb[i];
+            if (n > mx) mx = n;
+            if (count) count[i] = n;
+            if (list && n <= max_per_atom) {
+                int* out = list + (size_t)i * max_per_atom;
+                memcpy(out, rows + (size_t)i * S, sizeof(int) * nf[i]);
+                memcpy(out + nf[i], rows + (size_t)i * S + S - nb[i], sizeof(int) * nb[i]);
+            }
+        }
+    } else {
+        capac_set_error("neighbour download failed");
+        mx = CAPAC_ERR_HIP;
+    }
+    free(nf); free(nb); free(rows);
+    if (mx > max_per_atom && list) {
+        capac_set_error("a neighbour row has %d entries, more than max_per_atom %d", mx, max_per_atom);
+        return CAPAC_ERR_ARG;
+    }
+    return mx;
+}
+
+static void pot_out(capac_ctx* c, double* pot, double* vir, const int* pk, const int* hk, int npk,
+                    int vir_at) {
+    if (pot) {
+        memset(pot, 0, sizeof(double) * CAPAC_NPOT);
+        for (int k = 0; k < npk; ++k) pot[pk[k]] = c->red_host[hk[k]];
+    }
+    if (vir)
+        for (int k = 0; k < 9; ++k) vir[k] = c->red_host[vir_at + k];
+}
+
+extern "C" int capac_nonbonded(capac_ctx* c, double* f, double pot[CAPAC_NPOT], double vir[9]) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = build_until_fits(c);
+    if (!rc) rc = fill_d(c, c->red, 256, 0.0);
+    if (!rc) rc = pair_nonbonded(c, NULL);
+    if (!rc) rc = fetch_red(c);
+    if (rc) return rc;
+    const int pk[2] = {7, 8}, hk[2] = {0, 1};
+    pot_out(c, pot, vir, pk, hk, 2, 2);
+    return f ? capac_get_forces(c, f) : 0;
+}
+
+extern "C" int capac_sutton_chen(capac_ctx* c, double* f, double pot[CAPAC_NPOT], double vir[9],
+                                 double* isd) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = build_until_fits(c);
+    if (!rc) rc = zero_outputs(c);
+    if (!rc) rc = sc_forces(c);
+    if (!rc) rc = fetch_red(c);
+    if (rc) return rc;
+    const int pk[1] = {1}, hk[1] = {16};
+    pot_out(c, pot, vir, pk, hk, 1, 17);
+    if (isd && c->bc.Nm)
+        CHECK_HIP(hipMemcpy(isd, c->isd, sizeof(double) * c->bc.Nm, hipMemcpyDeviceToHost));
+    return f ? capac_get_forces(c, f) : 0;
+}
+
+extern "C" int capac_cpim_rhs(capac_ctx* c, double* b) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!c->n_mat) { capac_set_error("no CPIM unknowns (no metal or use_cpff 0)"); return CAPAC_ERR_ARG; }
+    int rc = build_until_fits(c);
+    if (!rc) rc = cpim_rhs(c);
+    if (rc) return rc;
+    if (b) CHECK_HIP(hipMemcpyAsync(b, c->b, sizeof(double) * c->n_mat, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_cpim_assemble(capac_ctx* c, double* dense, double* diag) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!c->n_mat) { capac_set_error("no CPIM unknowns (no metal or use_cpff 0)"); return CAPAC_ERR_ARG; }
+    int rc = cpim_assemble(c);
+    if (rc) return rc;
+    if (dense) {
+        size_t n4 = 4 * (size_t)c->bc.Nm;
+        double* d = NULL;
+        CHECK_HIP(hipMalloc(&d, n4 * n4 * sizeof(double)));
+        rc = cpim_dense_expand(c, d);
+        if (!rc && hipMemcpyAsync(dense, d, n4 * n4 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = CAPAC_ERR_HIP;
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(d);
+        if (rc) return rc;
+    }
+    if (diag) CHECK_HIP(hipMemcpyAsync(diag, c->diag, sizeof(double) * c->n_mat, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_cpim_solve(capac_ctx* c, const double* b, double* x, int* iters, double* rTr,
+                                double* bTb, int* kind) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!c->n_mat) { capac_set_error("no CPIM unknowns (no metal or use_cpff 0)"); return CAPAC_ERR_ARG; }
+    size_t bytes = sizeof(double) * c->n_mat;
+    if (b) CHECK_HIP(hipMemcpyAsync(c->b, b, bytes, hipMemcpyHostToDevice, c->stream));
+    if (x) CHECK_HIP(hipMemcpyAsync(c->x, x, bytes, hipMemcpyHostToDevice, c->stream));
+    int rc = cpim_assemble(c);
+    if (rc) return rc;
+    rc = solver_run(c, c->b, c->x);
+    if (rc && rc != CAPAC_ERR_SOLVER) return rc;
+    if (x) CHECK_HIP(hipMemcpyAsync(x, c->x, bytes, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = c->solve_iters;
+    if (rTr) *rTr = c->solve_rTr;
+    if (bTb) *bTb = c->solve_bTb;
+    if (kind) *kind = c->solve_kind;
+    return rc;
+}
+
+extern "C" int capac_cpim_force(capac_ctx* c, const double* x, double* f, double pot[CAPAC_NPOT],
+                                double vir[9]) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!c->n_mat) { capac_set_error("no CPIM unknowns (no metal or use_cpff 0)"); return CAPAC_ERR_ARG; }
+    int rc = build_until_fits(c);
+    if (!rc) rc = zero_outputs(c);
+    if (rc) return rc;
+    const double* xd = c->x;
+    if (x) {
+        CHECK_HIP(hipMemcpyAsync(c->x_backup, x, sizeof(double) * c->n_mat,
+                                 hipMemcpyHostToDevice, c->stream));
+        xd = c->x_backup;
+    }
+    rc = cpim_forces_x(c, xd);
+    if (!rc) rc = fetch_red(c);
+    if (rc) return rc;
+    if (pot) {
+        memset(pot, 0, sizeof(double) * CAPAC_NPOT);
+        pot[9] = c->red_host[48];
+        pot[10] = c->red_host[32];
+        pot[11] = c->red_host[33];
+        pot[12] = c->red_host[49];
+        pot[13] = c->red_host[50];
+        pot[14] = c->red_host[51];
+    }
+    if (vir)
+        for (int k = 0; k < 9; ++k) vir[k] = c->red_host[34 + k] + c->red_host[52 + k];
+    return f ? capac_get_forces(c, f) : 0;
+}
+
+extern "C" int capac_solve_stats(capac_ctx* c, int* iters, double* rTr, double* bTb, int* kind) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (iters) *iters = c->solve_iters;
+    if (rTr) *rTr = c->solve_rTr;
+    if (bTb) *bTb = c->solve_bTb;
+    if (kind) *kind = c->solve_kind;
+    return 0;
+}
+
+// ------------------------------------------------------------------ MD
+
+static int fetch_md(capac_ctx* c) {
+    CHECK_HIP(hipMemcpyAsync(c->red_host + 260, c->md, 32 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->red_host + 292, c->md_status, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->red_host + 300, c->red + 64, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_md_begin(capac_ctx* c) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = md_remove_com(c);
+    if (!rc) rc = md_kinetic(c);
+    if (!rc) rc = md_fmax(c);
+    if (!rc) rc = fill_i(c, c->md_status, 4, 0);
+    if (!rc) rc = fill_d(c, c->red + 64, 16, 0.0);
+    if (!rc) rc = fetch_md(c);
+    if (rc) return rc;
+    c->first_temp = c->red_host[260 + 1];
+    c->ext_temp = c->ref_temp0;
+    c->cur_ref_temp = c->ref_temp0;
+    return 0;
+}
+
+extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_save, int stop_at_save) {
+    if (!c) return CAPAC_ERR_ARG;
+    int done = 0;
+    for (int step = first_step; step < first_step + n_steps; ++step) {
+        double ref_temp = step < c->n_heating
+                              ? c->first_temp + (c->ext_temp - c->first_temp) * step / c->n_heating
+                              : c->ext_temp;
+        c->cur_ref_temp = ref_temp;
+        double kT = K_BOLTZ * ref_temp;
+        int rc = 0;
+        mark(c, 8);
+        if (c->nvt) rc = md_nhc(c, kT);
+        if (!rc) rc = md_kick(c);
+        if (!rc) rc = md_drift(c);
+        if (!rc) rc = md_rattle1(c);
+        if (rc) return rc;
+        rc = force_compute(c);
+        if (rc) return rc;
+        mark(c, 9);
+        if (!rc) rc = md_kick(c);
+        if (!rc) rc = md_rattle2(c);
+        if (!rc) rc = md_kinetic(c);
+        if (!rc && c->nvt) rc = md_nhc(c, kT);
+        if (!rc) rc = md_remove_com(c);
+        if (!rc) rc = md_kinetic(c);
+        int save = n_save > 0 && step % n_save == 0;
+        if (!rc && save) rc = md_wrap(c);
+        if (!rc && save) rc = md_fmax(c);
+        mark(c, 10);
+        if (!rc) rc = fetch_md(c);
+        if (rc) return rc;
+        if (c->timing_on) {
+            float a, b2;
+            if (hipEventElapsedTime(&a, c->ev[8], c->ev[0]) == hipSuccess &&
+                hipEventElapsedTime(&b2, c->ev[9], c->ev[10]) == hipSuccess)
+                c->timing.t[12] += 1e-3 * (a + b2);
+        }
+        const int* st = (const int*)(c->red_host + 292);
+        if (st[0]) { capac_set_error("RATTLE_1 not converged!"); return CAPAC_ERR_RATTLE1; }
+        if (st[1]) { capac_set_error("RATTLE_2 not converged!"); return CAPAC_ERR_RATTLE2; }
+        ++done;
+        if (save && stop_at_save) break;
+    }
+    return done;
+}
+
+extern "C" int capac_md_observables(capac_ctx* c, double* eksum, double* temp, double* ref_temp,
+                                    double* vq, double* fmax, double* frms, double pot[CAPAC_NPOT],
+                                    double vir[9]) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = fetch_md(c);
+    if (rc) return rc;
+    const double* m = c->red_host + 260;
+    if (eksum) *eksum = m[0];
+    if (temp) *temp = m[1];
+    if (ref_temp) *ref_temp = c->cur_ref_temp;
+    if (vq) memcpy(vq, m + 2, sizeof(double) * CAPAC_NHC);
+    if (fmax) *fmax = m[23];
+    if (frms) *frms = m[24];
+    if (pot) {
+        memcpy(pot, c->pot, sizeof c->pot);
+        pot[0] = 0.0;
+        for (int k = 1; k < CAPAC_NPOT; ++k) pot[0] += pot[k];    // sum_potential (force.c:322-331)
+    }
+    if (vir)
+        for (int k = 0; k < 9; ++k) vir[k] = c->vir[k] + c->red_host[300 + k];
+    return 0;
+}
+
+extern "C" int capac_set_thermostat(capac_ctx* c, const double* vq, double vp) {
+    (void)vp;
+    if (!c || !vq) return CAPAC_ERR_ARG;
+    CHECK_HIP(hipMemcpyAsync(c->md + 2, vq, sizeof(double) * CAPAC_NHC, hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms) {
+    if (!c || reps <= 0 || !ms) return CAPAC_ERR_ARG;
+    if ((kernel == CAPAC_KERNEL_SPMV || kernel == CAPAC_KERNEL_ASSEMBLE || kernel == CAPAC_KERNEL_CPIM_MM) &&
+        !c->n_mat) {
+        capac_set_error("no CPIM unknowns");
+        return CAPAC_ERR_ARG;
+    }
+    // the SpMV writes into the solver's scratch vector, the others into their
+    // usual outputs; positions are not changed
+    double* scratch = c->x_backup;
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    CHECK_HIP(hipEventRecord(c->ev[14], c->stream));
+    for (int k = 0; k < reps; ++k) {
+        int rc = 0;
+        switch (kernel) {
+            case CAPAC_KERNEL_SPMV: rc = solver_spmv_only(c, c->x, scratch); break;
+            case CAPAC_KERNEL_ASSEMBLE: rc = cpim_assemble(c); break;
+            case CAPAC_KERNEL_PAIR: rc = pair_nonbonded(c, NULL); break;
+            case CAPAC_KERNEL_NEIGHBOUR: rc = nb_build(c); break;
+            case CAPAC_KERNEL_CPIM_MM: rc = cpim_mm_only(c); break;
+            default: capac_set_error("unknown kernel %d", kernel); return CAPAC_ERR_ARG;
+        }
+        if (rc) return rc;
+    }
+    CHECK_HIP(hipEventRecord(c->ev[15], c->stream));
+    CHECK_HIP(hipEventSynchronize(c->ev[15]));
+    float t = 0.f;
+    CHECK_HIP(hipEventElapsedTime(&t, c->ev[14], c->ev[15]));
+    *ms = (double)t / reps;
+    return 0;
+}
+
+extern "C" int capac_get_timing(capac_ctx* c, capac_timing* t) {
+    if (!c || !t) return CAPAC_ERR_ARG;
+    *t = c->timing;
+    return 0;
+}
+
+extern "C" int capac_reset_timing(capac_ctx* c) {
+    if (!c) return CAPAC_ERR_ARG;
+    memset(&c->timing, 0, sizeof c->timing);
+    return 0;
+}
+
+extern "C" int capac_enable_timing(capac_ctx* c, int on) {
+    if (!c) return CAPAC_ERR_ARG;
+    c->timing_on = on;
+    return 0;
+}

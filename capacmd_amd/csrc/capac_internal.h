@@ -1,0 +1,190 @@
+// Internal declarations of libcapac_gpu.so (HIP, gfx950).
+//
+// Device layout of one box (all FP64 unless noted):
+//   r, v, f, old_r        3N SoA blocks x|y|z, the reference's rx/ry/rz layout
+//   charge, inv_mass, mass, type(int), mol_id(int), local_idx(int), excl(uint)
+//   cell list             cell_start[n_cell+1], cell_atoms[N] sorted by (cell, index)
+//   neighbour rows        nbr[i * nb_stride + k]: non-metal neighbours from the
+//                         front (k < n_front[i]), metal neighbours from the back
+//                         (k >= nb_stride - n_back[i]); int32
+//   relay blocks          FP64: Nm x Nm 4x4 blocks [A(3x3) M(3x1); M^T(1x3) C],
+//                         element k of block (i,j) at relay[(i*16 + k)*relay_ld + j]
+//   CPIM vector           x = [p (3Nm) | q (Nm) | lambda (n_np)], atomic units
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/capac_gpu.h"
+
+#define CAPAC_WAVE 64
+
+// reference literals (typedef.h:44,67,73,77; cpff.c:72,75,758,759)
+#define K_INV_SQRT_PI 0.564189583547756
+#define K_FQQ 138.935457696552
+#define K_NM2BOHR 18.8972612456506
+#define K_BOLTZ 0.00831446214546895
+#define K_CONV_E 2.01552955019358e-05
+#define K_CONV_V 3.80879884583368e-04
+#define K_FQQ_AU2 49614.7526045429
+#define K_FQQ_AU1 2625.49964037578
+
+#define CHECK_HIP(x)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            capac_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return CAPAC_ERR_HIP;                                                       \
+        }                                                                               \
+    } while (0)
+
+void capac_set_error(const char* fmt, ...);
+
+// scalar constants shared by every kernel of a box
+struct BoxConst {
+    int N, Nm, n_np, n_types, n_mols;
+    double box[3], half[3];
+    double rc, rc2, alpha, a2_sqrtPI, wolfConst, erfc_arCut, inv_rc12, inv_rc6;
+    int use_coulomb, use_vdw;
+    int qsc_n, qsc_m;
+    double qsc_eps, qsc_c, qsc_a;
+    double inv_polar, inv_capac, inv_R_pp, inv_R_pq, inv_R_qq;
+    double efield[3];
+};
+
+struct PairParam {          // one entry of the n_types x n_types table
+    int funct;
+    double c6, c12, width;
+};
+
+// BiCGSTAB scalars, resident on the device (bicg_stab.c:289-292)
+struct SolveScalars {
+    double rho, rho_old, alpha, omega, bTb, criteria, rTr;
+    int iter, done, kind;
+};
+
+struct Solver;              // solver.hip
+
+struct capac_ctx {
+    int device, precision;
+    BoxConst bc;
+    int n_mat, n_cstr, ndf, fix_metal, nvt, n_heating;
+    double dt, tau_t, ref_temp0;
+    hipStream_t stream;
+
+    // per atom
+    double *r, *v, *f, *old_r;
+    double *charge, *mass, *inv_mass;
+    int *type, *mol_id, *local_idx;
+    unsigned* excl;
+    PairParam* pair;
+    // molecules / constraints / nanoparticles
+    int *mol_first, *mol_last, *mol_cstr, *cstr_i, *cstr_j;
+    double* cstr_r;
+    int *np_first, *np_last, *np_of_metal;
+    double* np_charge;
+
+    // cell list
+    int ncell[3], n_cell;
+    int *cell_of, *cell_count, *cell_start, *cell_atoms, *cell_tmp, *cell_fill;
+    int max_cell;               // largest cell occupancy (for LDS staging)
+    // neighbour rows
+    int nb_stride;
+    int *n_front, *n_back, *nbr, *nb_overflow;
+    int nb_valid;
+
+    // reductions: per-block partials and a small result area
+    double* partial;            // n_partial_blocks * 16
+    int n_partial;
+    double* red;                // reduction results (64 doubles)
+    double* red_host;           // pinned mirror
+
+    // Sutton-Chen
+    double* isd;
+    // CPIM
+    double *b, *x, *x_backup, *diag, *relay;
+    int relay_ld;
+    size_t relay_bytes;
+    Solver* solver;
+    int solve_iters, solve_kind;
+    double solve_rTr, solve_bTb;
+
+    // MD scalars (host copy of the small device state)
+    double* md;                 // device: [eKsum, temp, vQ[10], aQ[10], scale, fmax2, frms]
+    double first_temp, ext_temp, cur_ref_temp;
+    int md_status_host;
+    int* md_status;             // device RATTLE failure flags
+
+    // results of the last force call
+    double pot[CAPAC_NPOT], vir[9];
+    double pot_dev_ofs;         // unused
+    // timing
+    int timing_on;
+    hipEvent_t ev[16];
+    capac_timing timing;
+};
+
+// ------------------------------------------------------------------ device helpers
+
+// pbc_12 (force.c:111-121)
+__device__ __forceinline__ void pbc12(const BoxConst& b, double& x, double& y, double& z) {
+    if (x < -b.half[0]) x += b.box[0]; else if (x > b.half[0]) x -= b.box[0];
+    if (y < -b.half[1]) y += b.box[1]; else if (y > b.half[1]) y -= b.box[1];
+    if (z < -b.half[2]) z += b.box[2]; else if (z > b.half[2]) z -= b.box[2];
+}
+
+// dist_2 (force.c:41-44) evaluated exactly as the reference: (x*x + y*y) + z*z,
+// each product and sum rounded once, no fused multiply-add
+__device__ __forceinline__ double dist2_exact(double x, double y, double z) {
+#pragma clang fp contract(off)
+    double xx = x * x;
+    double yy = y * y;
+    double zz = z * z;
+    return (xx + yy) + zz;
+}
+
+// wavefront sum of a double, fixed butterfly order (deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, CAPAC_WAVE);
+    return v;
+}
+
+// ------------------------------------------------------------------ launchers
+
+// neighbour.hip
+int nb_grid_setup(capac_ctx* c);
+int nb_build(capac_ctx* c);
+// pair.hip
+int pair_nonbonded(capac_ctx* c, double* f_out_accumulate);
+// sc.hip
+int sc_forces(capac_ctx* c);
+// cpim.hip
+int cpim_rhs(capac_ctx* c);
+int cpim_assemble(capac_ctx* c);
+int cpim_forces(capac_ctx* c);
+int cpim_forces_x(capac_ctx* c, const double* x);
+int cpim_dense_expand(capac_ctx* c, double* dense_dev);
+// solver.hip
+int solver_create(capac_ctx* c);
+void solver_destroy(capac_ctx* c);
+int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
+int solver_spmv_only(capac_ctx* c, const double* x, double* y);
+int cpim_mm_only(capac_ctx* c);
+// md.hip
+int md_kick(capac_ctx* c);
+int md_drift(capac_ctx* c);
+int md_rattle1(capac_ctx* c);
+int md_rattle2(capac_ctx* c);
+int md_kinetic(capac_ctx* c);
+int md_nhc(capac_ctx* c, double kT);
+int md_remove_com(capac_ctx* c);
+int md_wrap(capac_ctx* c);
+int md_fmax(capac_ctx* c);
+int ext_field(capac_ctx* c);
+// reductions (api.hip): sum per-block partials [nblk x 16] into red[0..nval)
+int reduce_partials(capac_ctx* c, int nblk, int nval, int red_offset);
+int zero_outputs(capac_ctx* c);
+// graph-safe clears (fill kernels, not hipMemsetAsync)
+int fill_zero_i(capac_ctx* c, int* p, size_t n);
+int fill_zero_d(capac_ctx* c, double* p, size_t n);

@@ -1,0 +1,439 @@
+// Capacitance-polarizability interaction model (CPIM) on the device:
+//   cpim_rhs       mpi_cpff_vec_ext        (cpff.c:43-198)
+//   cpim_assemble  mpi_cpff_mat_relay_COO  (cpff.c:205-519)
+//   cpim_forces    mpi_cpff_force          (cpff.c:526-950)
+//
+// Relay layout (FP64 path): every ordered metal pair (i, j) owns one 4x4 block
+//     [ A_ab (3x3)   M_a = c3 r_a ]        rows: dipole p_i (a), charge q_i
+//     [ -c3 r_b      C            ]        cols: dipole p_j (b), charge q_j
+// with r = (r_j - r_i) in bohr, the Gaussian-smeared T-tensors of the
+// reference, and the diagonal blocks diag(1/alpha) and -1/c.  The relay has
+// no cut-off (every metal pair), so each row-block i holds Nm blocks, stored
+// component-major: element k (row-major index in the block) of block (i, j)
+// is relay[(i*16 + k)*ld + j], ld = Nm rounded up to 64.  Lanes that take
+// consecutive j then read and write 512 contiguous bytes per instruction.
+// The Lagrange rows and columns of the charge-neutrality constraint are
+// applied analytically by the solver.
+#include "capac_internal.h"
+
+namespace {
+
+// ------------------------------------------------------------------ RHS
+// One wavefront per metal atom over the non-metal front of its row.
+__global__ void __launch_bounds__(256)
+k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+      const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+      const double* __restrict__ np_charge, double* __restrict__ b) {
+    const int N = bc.N, Nm = bc.Nm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int i = blockIdx.x * 4 + wave;
+    if (i >= Nm) return;
+    double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+    int nf = n_front[i];
+    const int* row = nbr + (size_t)i * stride;
+    double ex = 0.0, ey = 0.0, ez = 0.0, pm = 0.0;
+    for (int k = lane; k < nf; k += 64) {
+        int a = row[k];
+        double q = charge[a];
+        if (q == 0.0) continue;
+        // rvec = r_metal - r_atom (cpff.c:105-108)
+        double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
+        pbc12(bc, dx, dy, dz);
+        double r2 = dist2_exact(dx, dy, dz);
+        double rij = sqrt(r2);
+        double e_field, v_field;
+        if (bc.use_coulomb == 1) {
+            double erfc_a = erfc(bc.alpha * rij);
+            double e2 = exp(-bc.alpha * bc.alpha * r2);
+            e_field = K_FQQ * q * (erfc_a / r2 + bc.a2_sqrtPI * e2 / rij - bc.wolfConst) / rij;
+            v_field = K_FQQ * q * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+        } else {
+            e_field = K_FQQ * q / (r2 * rij);
+            v_field = K_FQQ * q / rij;
+        }
+        double s = e_field * K_CONV_E;
+        ex += s * dx; ey += s * dy; ez += s * dz;
+        pm += v_field * K_CONV_V;
+    }
+    ex = wave_sum(ex); ey = wave_sum(ey); ez = wave_sum(ez); pm = wave_sum(pm);
+    if (lane == 0) {
+        b[3 * i + 0] = ex + bc.efield[0] * K_CONV_E;
+        b[3 * i + 1] = ey + bc.efield[1] * K_CONV_E;
+        b[3 * i + 2] = ez + bc.efield[2] * K_CONV_E;
+        double pot_ext = bc.efield[0] * xi + bc.efield[1] * yi + bc.efield[2] * zi;
+        b[3 * Nm + i] = pm - pot_ext * K_CONV_V;
+        if (i < bc.n_np) b[4 * Nm + i] = np_charge[i];
+    }
+}
+
+// ------------------------------------------------------------------ relay
+// Gaussian-smeared T-tensor coefficients of one metal pair (cpff.c:266-287):
+//   c1 = [erf(r/R_pp) - 2/sqrt(pi) (r/R_pp) e^{-r^2/R_pp^2}] / r^5
+//   c2 = 4/sqrt(pi) R_pp^-3 e^{-r^2/R_pp^2} / r^2
+//   c3 = [erf(r/R_pq) - 2/sqrt(pi) (r/R_pq) e^{-r^2/R_pq^2}] / r^3
+//   cq = -erf(r/R_qq) / r
+struct TCoef {
+    double c1, c2, c3, cq;
+};
+
+__device__ __forceinline__ TCoef t_coef(const BoxConst& bc, double r2) {
+    TCoef t;
+    double rij = sqrt(r2);
+    double inv_r = 1.0 / rij;
+    double inv_r2 = inv_r * inv_r;
+    double inv_r3 = inv_r2 * inv_r;
+    double e_pp = exp(-(r2 * bc.inv_R_pp * bc.inv_R_pp));
+    double inv_Rpp3 = bc.inv_R_pp * bc.inv_R_pp * bc.inv_R_pp;
+    t.c1 = inv_r3 * inv_r2 * (erf(rij * bc.inv_R_pp) - 2.0 * K_INV_SQRT_PI * bc.inv_R_pp * rij * e_pp);
+    t.c2 = 4.0 * K_INV_SQRT_PI * inv_Rpp3 * inv_r2 * e_pp;
+    double e_pq = exp(-(r2 * bc.inv_R_pq * bc.inv_R_pq));
+    t.c3 = inv_r3 * (erf(rij * bc.inv_R_pq) - 2.0 * K_INV_SQRT_PI * bc.inv_R_pq * rij * e_pq);
+    t.cq = -erf(rij * bc.inv_R_qq) * inv_r;
+    return t;
+}
+
+// one thread per ordered pair (i, j): grid (ceil(Nm/256), Nm)
+__global__ void __launch_bounds__(256)
+k_assemble64(const double* __restrict__ r, BoxConst bc, int ld, double* __restrict__ relay) {
+    const int N = bc.N, Nm = bc.Nm;
+    int i = blockIdx.y;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nm) return;
+    double blk[16];
+    if (i == j) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) blk[k] = 0.0;
+        blk[0] = blk[5] = blk[10] = bc.inv_polar;
+        blk[15] = -bc.inv_capac;
+    } else {
+        double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
+        pbc12(bc, dx, dy, dz);
+        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+        double r2 = dist2_exact(dx, dy, dz);
+        TCoef t = t_coef(bc, r2);
+        double rv[3] = {dx, dy, dz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+                blk[a * 4 + bb] = -(t.c1 * (3.0 * rv[a] * rv[bb] - (a == bb ? r2 : 0.0)) -
+                                    t.c2 * rv[a] * rv[bb]);
+            blk[a * 4 + 3] = t.c3 * rv[a];
+            blk[12 + a] = -t.c3 * rv[a];
+        }
+        blk[15] = t.cq;
+    }
+    double* dst = relay + (size_t)i * 16 * ld + j;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dst[(size_t)k * ld] = blk[k];
+}
+
+__global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_mat) return;
+    int Nm = bc.Nm;
+    diag[k] = k < 3 * Nm ? bc.inv_polar : (k < 4 * Nm ? -bc.inv_capac : 1.0);
+}
+
+__global__ void k_dense(const double* __restrict__ relay, int Nm, int ld, double* __restrict__ dense) {
+    long n4 = 4L * Nm;
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n4 * n4) return;
+    long row = idx / n4, col = idx % n4;
+    int i = row < 3L * Nm ? (int)(row / 3) : (int)(row - 3L * Nm);
+    int a = row < 3L * Nm ? (int)(row % 3) : 3;
+    int j = col < 3L * Nm ? (int)(col / 3) : (int)(col - 3L * Nm);
+    int b = col < 3L * Nm ? (int)(col % 3) : 3;
+    dense[idx] = relay[((size_t)i * 16 + a * 4 + b) * ld + j];
+}
+
+// ------------------------------------------------------------------ forces
+// Metal-water terms (cpff.c:581-721), evaluated for r = r_metal - r_atom in nm:
+// DSF charge(q_i)-charge(q_a) and induced dipole(p_i)-charge(q_a).
+__device__ __forceinline__ void mw_pair(const BoxConst& bc, double dx, double dy, double dz, double r2,
+                                        double qa, double qm, const double* pm, double f[3],
+                                        double* e10, double* e11) {
+    double rij = sqrt(r2);
+    double rv[3] = {dx, dy, dz};
+    double const_a0, erfc_a = 0.0, e2 = 0.0;
+    if (bc.use_coulomb == 1) {
+        erfc_a = erfc(bc.alpha * rij);
+        e2 = exp(-bc.alpha * bc.alpha * r2);
+        const_a0 = (erfc_a / r2 + bc.a2_sqrtPI * e2 / rij - bc.wolfConst) / rij;
+        *e10 += K_FQQ * qa * qm * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+    } else {
+        const_a0 = 1.0 / (r2 * rij);
+        *e10 += K_FQQ * qa * qm / rij;
+    }
+    double sq = const_a0 * K_FQQ * qa * qm;
+    f[0] = sq * dx; f[1] = sq * dy; f[2] = sq * dz;
+    // dipole in e*nm
+    double p[3] = {pm[0] / K_NM2BOHR, pm[1] / K_NM2BOHR, pm[2] / K_NM2BOHR};
+    double pr = p[0] * dx + p[1] * dy + p[2] * dz;
+    double A1, B1 = 0.0, C1 = 0.0;
+    double inv_r = 1.0 / rij;
+    double inv_r3 = inv_r * inv_r * inv_r;
+    if (bc.use_coulomb == 1) {
+        A1 = inv_r3 * inv_r * inv_r * (erfc_a + bc.a2_sqrtPI * rij * e2);
+        double alpha3 = bc.alpha * bc.alpha * bc.alpha;
+        B1 = 4.0 * K_INV_SQRT_PI * alpha3 / r2 * e2;
+        C1 = bc.wolfConst * inv_r3;
+    } else {
+        A1 = inv_r3 * inv_r * inv_r;
+    }
+    // f_b += -sum_a [A1 (3 r_a r_b - d_ab r^2) + B1 r_a r_b - C1 (r_a r_b - d_ab r^2)] FQQ q_a p_a
+    double s = K_FQQ * qa;
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+        double t = A1 * (3.0 * pr * rv[bb] - p[bb] * r2) + B1 * pr * rv[bb] - C1 * (pr * rv[bb] - p[bb] * r2);
+        f[bb] -= t * s;
+    }
+    *e11 += -K_FQQ * qa * pr * const_a0;
+}
+
+// metal side: one wavefront per metal atom over its charged non-metal neighbours
+__global__ void __launch_bounds__(256)
+k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+                const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+                const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
+    __shared__ double s_red[4][11];
+    const int N = bc.N, Nm = bc.Nm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int i = blockIdx.x * 4 + wave;
+    double e10 = 0.0, e11 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < Nm) {
+        double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+        double qm = x[3 * Nm + i];
+        double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        int nf = n_front[i];
+        const int* row = nbr + (size_t)i * stride;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int k = lane; k < nf; k += 64) {
+            int a = row[k];
+            double qa = charge[a];
+            if (qa == 0.0) continue;
+            double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
+            pbc12(bc, dx, dy, dz);
+            double r2 = dist2_exact(dx, dy, dz);
+            double fp[3];
+            mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11);
+            fx += fp[0]; fy += fp[1]; fz += fp[2];
+            double rv[3] = {dx, dy, dz};
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2) v[a2 * 3 + b2] -= rv[a2] * fp[b2];
+        }
+        fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+        if (lane == 0) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
+    }
+    double vals[11] = {e10, e11, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+#pragma unroll
+    for (int c = 0; c < 11; ++c) {
+        double s = wave_sum(vals[c]);
+        if (lane == 0) s_red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = 0.0;
+        if (threadIdx.x < 11)
+            for (int w = 0; w < 4; ++w) s += s_red[w][threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = s;
+    }
+}
+
+// atom side: one thread per non-metal atom over the metal tail of its row;
+// the reaction force of the same pair term (the reference's f_atom -= f)
+__global__ void __launch_bounds__(256)
+k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+               const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
+               const double* __restrict__ x, double* __restrict__ f) {
+    const int N = bc.N, Nm = bc.Nm;
+    int a = Nm + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    double qa = charge[a];
+    int nb = n_back[a];
+    if (qa == 0.0 || nb == 0) return;
+    const int* tail = nbr + (size_t)a * stride + stride - nb;
+    double xa = r[a], ya = r[N + a], za = r[2 * N + a];
+    double fx = 0.0, fy = 0.0, fz = 0.0, e10 = 0.0, e11 = 0.0;
+    for (int k = 0; k < nb; ++k) {
+        int i = tail[k];
+        double dx = r[i] - xa, dy = r[N + i] - ya, dz = r[2 * N + i] - za;
+        pbc12(bc, dx, dy, dz);
+        double r2 = dist2_exact(dx, dy, dz);
+        double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        double fp[3];
+        mw_pair(bc, dx, dy, dz, r2, qa, x[3 * Nm + i], pm, fp, &e10, &e11);
+        fx += fp[0]; fy += fp[1]; fz += fp[2];
+    }
+    f[a] -= fx; f[N + a] -= fy; f[2 * N + a] -= fz;
+}
+
+// Metal-metal terms, all pairs, force on i only (cpff.c:762-927), plus the
+// self terms (cpff.c:727-742) and the external field on the induced charge
+// and dipole (cpff.c:930-948).  One wavefront per metal atom i, lanes over j.
+__global__ void __launch_bounds__(256)
+k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
+          double* __restrict__ f, double* __restrict__ partial) {
+    __shared__ double s_red[4][13];
+    const int N = bc.N, Nm = bc.Nm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int i = blockIdx.x * 4 + wave;
+    double e9 = 0.0, e12 = 0.0, e13 = 0.0, e14 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < Nm) {
+        double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+        double qi = x[3 * Nm + i];
+        double pi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        const double iRqq = bc.inv_R_qq, iRpq = bc.inv_R_pq, iRpp = bc.inv_R_pp;
+        const double iRpq3 = iRpq * iRpq * iRpq, iRpp3 = iRpp * iRpp * iRpp;
+        const double iRpp5 = iRpp3 * iRpp * iRpp;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int j = lane; j < Nm; j += 64) {
+            if (j == i) continue;
+            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+            pbc12(bc, dx, dy, dz);
+            dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+            double r2 = dist2_exact(dx, dy, dz);
+            double rij = sqrt(r2);
+            double inv_r = 1.0 / rij, inv_r2 = inv_r * inv_r;
+            double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
+            double qj = x[3 * Nm + j];
+            double pj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+            double rv[3] = {dx, dy, dz};
+            double g[3] = {0.0, 0.0, 0.0};
+            // charge-charge, T0
+            double e_qq = exp(-(r2 * iRqq * iRqq));
+            double erf_qq = erf(rij * iRqq);
+            double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * iRqq * rij * e_qq);
+            double sqq = A0 * K_FQQ_AU2 * qi * qj;
+            g[0] += sqq * dx; g[1] += sqq * dy; g[2] += sqq * dz;
+            e12 += 0.5 * K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
+            // charge-dipole, T1 both directions
+            double e_pq = exp(-(r2 * iRpq * iRpq));
+            double br_pq = erf(rij * iRpq) - 2.0 * K_INV_SQRT_PI * iRpq * rij * e_pq;
+            double Apq0 = inv_r3 * br_pq;
+            double A1 = inv_r5 * br_pq;
+            double B1 = 4.0 * K_INV_SQRT_PI * iRpq3 * inv_r2 * e_pq;
+            // w = qj p_i - qi p_j:  f_b += sum_a (A1 (3 r_a r_b - d r^2) - B1 r_a r_b) au2 w_a
+            double w[3] = {qj * pi[0] - qi * pj[0], qj * pi[1] - qi * pj[1], qj * pi[2] - qi * pj[2]};
+            double wr = w[0] * dx + w[1] * dy + w[2] * dz;
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                g[b] += (A1 * (3.0 * wr * rv[b] - w[b] * r2) - B1 * wr * rv[b]) * K_FQQ_AU2;
+            double pjr = pj[0] * dx + pj[1] * dy + pj[2] * dz;
+            double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
+            e13 += 0.5 * K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
+            // dipole-dipole, T2
+            double e_pp = exp(-(r2 * iRpp * iRpp));
+            double erf_pp = erf(rij * iRpp);
+            double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * iRpp * rij * e_pp);
+            double Bpp1 = 4.0 * K_INV_SQRT_PI * iRpp3 * inv_r2 * e_pp;
+            double A2 = inv_r7 * (3.0 * erf_pp - 6.0 * K_INV_SQRT_PI * iRpp * rij * e_pp -
+                                  4.0 * r2 * rij * K_INV_SQRT_PI * iRpp3 * e_pp);
+            double B2 = 8.0 * K_INV_SQRT_PI * iRpp5 * inv_r2 * e_pp;
+            double pipj = pi[0] * pj[0] + pi[1] * pj[1] + pi[2] * pj[2];
+            // sum_ab p_ia p_jb [5 r_a r_b r_c - r^2 (r_a d_bc + r_b d_ca + r_c d_ab)]
+            //   = 5 (pi.r)(pj.r) r_c - r^2 ((pi.r) pj_c + (pj.r) pi_c + (pi.pj) r_c)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double t5 = 5.0 * pir * pjr * rv[c] - r2 * (pir * pj[c] + pjr * pi[c] + pipj * rv[c]);
+                g[c] -= (A2 * t5 - B2 * pir * pjr * rv[c]) * K_FQQ_AU2;
+            }
+            e14 -= 0.5 * K_FQQ_AU1 * ((3.0 * pir * pjr - pipj * r2) * App1 - pir * pjr * Bpp1);
+            fx -= g[0]; fy -= g[1]; fz -= g[2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) v[a * 3 + b] -= 0.5 * rv[a] * g[b];
+        }
+        fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+        if (lane == 0) {
+            // self terms (inv_capac, inv_polar in atomic units, FQQ in MD units)
+            e12 += 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
+            e14 += 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
+            // external field on the induced charge (force) and charge + dipole (energy)
+            fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
+            e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+            e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
+            f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz;
+        }
+    }
+    double vals[13] = {e9, e12, e13, e14, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+#pragma unroll
+    for (int c = 0; c < 13; ++c) {
+        double s = wave_sum(vals[c]);
+        if (lane == 0) s_red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = 0.0;
+        if (threadIdx.x < 13)
+            for (int w = 0; w < 4; ++w) s += s_red[w][threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+// b (n_mat) from the current positions
+int cpim_rhs(capac_ctx* c) {
+    int Nm = c->bc.Nm;
+    if (Nm == 0) return 0;
+    if (c->bc.n_np > Nm) {
+        capac_set_error("more nanoparticles (%d) than metal atoms (%d)", c->bc.n_np, Nm);
+        return CAPAC_ERR_ARG;
+    }
+    k_rhs<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
+                                               c->nb_stride, c->np_charge, c->b);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int cpim_assemble(capac_ctx* c) {
+    int Nm = c->bc.Nm;
+    if (Nm == 0) return 0;
+    dim3 grid((Nm + 255) / 256, Nm);
+    k_assemble64<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay);
+    k_diag<<<(c->n_mat + 255) / 256, 256, 0, c->stream>>>(c->bc, c->n_mat, c->diag);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int cpim_dense_expand(capac_ctx* c, double* dense) {
+    long n4 = 4L * c->bc.Nm;
+    k_dense<<<(unsigned)((n4 * n4 + 255) / 256), 256, 0, c->stream>>>(c->relay, c->bc.Nm, c->relay_ld,
+                                                                       dense);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+// adds the CPIM forces for x (device) to c->f;
+// red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
+int cpim_forces_x(capac_ctx* c, const double* x) {
+    int Nm = c->bc.Nm, N = c->bc.N;
+    if (Nm == 0) return 0;
+    int nblk = (Nm + 3) / 4;
+    k_cpim_mw_metal<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
+                                                 c->nb_stride, x, c->f, c->partial);
+    CHECK_HIP(hipGetLastError());
+    int rc = reduce_partials(c, nblk, 11, 32);
+    if (rc) return rc;
+    if (N > Nm)
+        k_cpim_mw_atom<<<(N - Nm + 255) / 256, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
+                                                                    c->nbr, c->nb_stride, x, c->f);
+    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return reduce_partials(c, nblk, 13, 48);
+}
+
+int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
+
+// metal-metal CPIM forces alone (for kernel timing)
+int cpim_mm_only(capac_ctx* c) {
+    int Nm = c->bc.Nm;
+    if (Nm == 0) return 0;
+    k_cpim_mm<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}

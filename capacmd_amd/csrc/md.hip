@@ -1,0 +1,384 @@
+// MD step body on the device (reference mpi-main.c:874-1013): velocity
+// Verlet half kicks and drift, RATTLE (force.c:1907-2090, one thread per
+// molecule, Gauss-Seidel over its constraints in file order), kinetic energy,
+// the Nose-Hoover chain (thermostat.c:117-191, one thread: M=10, Nc=5,
+// Suzuki-Yoshida order 5), centre-of-mass removal (thermostat.c:69-108) and
+// the molecule wrap (force.c:127-165).  All FP64; reductions are fixed-order
+// single-workgroup sums so repeated runs are bit-identical.
+//
+// md[] layout: [0] eKsum [1] T [2..12) vQ [12..22) aQ [22] scale [23] F_max
+//              [24] F_rms [25] qMass
+#include "capac_internal.h"
+
+namespace {
+
+constexpr int kRedThreads = 1024;
+
+__device__ double block_sum1024(double v, double* s_w) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) s_w[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += s_w[w];
+    return t;
+}
+
+__global__ void k_kick(int N, int Nm, int fix_metal, double dt_2, const double* __restrict__ f,
+                       const double* __restrict__ inv_mass, double* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (fix_metal == 1 && i < Nm) return;
+    double im = inv_mass[i];
+    v[i] += dt_2 * f[i] * im;
+    v[N + i] += dt_2 * f[N + i] * im;
+    v[2 * N + i] += dt_2 * f[2 * N + i] * im;
+}
+
+__global__ void k_drift(int N, int Nm, int fix_metal, double dt, double* __restrict__ r,
+                        const double* __restrict__ v, double* __restrict__ old_r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (fix_metal == 1 && i < Nm) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        old_r[c * N + i] = r[c * N + i];
+        r[c * N + i] += dt * v[c * N + i];
+    }
+}
+
+// pbc (force.c:104-109)
+__device__ __forceinline__ double pbc1(double dx, double L) {
+    if (dx < -0.5 * L) dx += L; else if (dx > 0.5 * L) dx -= L;
+    return dx;
+}
+
+__global__ void k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+                          const int* __restrict__ ci, const int* __restrict__ cj,
+                          const double* __restrict__ cr, const double* __restrict__ inv_mass,
+                          double* __restrict__ r, double* __restrict__ v,
+                          const double* __restrict__ old_r, int* __restrict__ status) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_mols) return;
+    int c0 = mol_cstr[m], c1 = mol_cstr[m + 1];
+    if (c0 == c1) return;
+    const double tol2 = 2.0e-05;
+    double *rx = r, *ry = r + N, *rz = r + 2 * N;
+    double *vx = v, *vy = v + N, *vz = v + 2 * N;
+    int iter = 0, done = 0;
+    while (!done) {
+        done = 1;
+        for (int c = c0; c < c1; ++c) {
+            int a = ci[c], b = cj[c];
+            double rab = cr[c];
+            double px = pbc1(rx[a] - rx[b], bc.box[0]);
+            double py = pbc1(ry[a] - ry[b], bc.box[1]);
+            double pz = pbc1(rz[a] - rz[b], bc.box[2]);
+            double pab2 = px * px + py * py + pz * pz;
+            double rab2 = rab * rab;
+            double diff = rab2 - pab2;
+            if (fabs(diff) > rab2 * tol2) {
+                double qx = pbc1(old_r[a] - old_r[b], bc.box[0]);
+                double qy = pbc1(old_r[N + a] - old_r[N + b], bc.box[1]);
+                double qz = pbc1(old_r[2 * N + a] - old_r[2 * N + b], bc.box[2]);
+                double rp = qx * px + qy * py + qz * pz;
+                double ma = inv_mass[a], mb = inv_mass[b];
+                double g = diff / (2.0 * (ma + mb) * rp);
+                double dx = qx * g, dy = qy * g, dz = qz * g;
+                rx[a] += ma * dx; ry[a] += ma * dy; rz[a] += ma * dz;
+                rx[b] -= mb * dx; ry[b] -= mb * dy; rz[b] -= mb * dz;
+                dx /= dt; dy /= dt; dz /= dt;
+                vx[a] += ma * dx; vy[a] += ma * dy; vz[a] += ma * dz;
+                vx[b] -= mb * dx; vy[b] -= mb * dy; vz[b] -= mb * dz;
+                done = 0;
+            }
+        }
+        if (++iter > 1000) break;
+    }
+    if (!done) status[0] = 1;
+}
+
+__global__ void __launch_bounds__(256)
+k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+          const int* __restrict__ ci, const int* __restrict__ cj, const double* __restrict__ cr,
+          const double* __restrict__ inv_mass, const double* __restrict__ r, double* __restrict__ v,
+          int* __restrict__ status, double* __restrict__ partial) {
+    __shared__ double s_red[4][9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (m < n_mols) {
+        int c0 = mol_cstr[m], c1 = mol_cstr[m + 1];
+        const double tol = 1.0e-05, half_dt = 0.5 * dt;
+        const double *rx = r, *ry = r + N, *rz = r + 2 * N;
+        double *vx = v, *vy = v + N, *vz = v + 2 * N;
+        int iter = 0, done = c0 == c1;
+        while (!done) {
+            done = 1;
+            for (int c = c0; c < c1; ++c) {
+                int a = ci[c], b = cj[c];
+                double rab = cr[c];
+                double ux = vx[a] - vx[b], uy = vy[a] - vy[b], uz = vz[a] - vz[b];
+                double px = pbc1(rx[a] - rx[b], bc.box[0]);
+                double py = pbc1(ry[a] - ry[b], bc.box[1]);
+                double pz = pbc1(rz[a] - rz[b], bc.box[2]);
+                double rv = px * ux + py * uy + pz * uz;
+                double ma = inv_mass[a], mb = inv_mass[b];
+                double g = -rv / ((ma + mb) * (rab * rab));
+                if (fabs(g) > tol) {
+                    double dx = px * g, dy = py * g, dz = pz * g;
+                    vx[a] += ma * dx; vy[a] += ma * dy; vz[a] += ma * dz;
+                    vx[b] -= mb * dx; vy[b] -= mb * dy; vz[b] -= mb * dz;
+                    dx /= half_dt; dy /= half_dt; dz /= half_dt;
+                    w[0] -= px * dx; w[1] -= px * dy; w[2] -= px * dz;
+                    w[3] -= py * dx; w[4] -= py * dy; w[5] -= py * dz;
+                    w[6] -= pz * dx; w[7] -= pz * dy; w[8] -= pz * dz;
+                    done = 0;
+                }
+            }
+            if (++iter > 1000) break;
+        }
+        if (!done) status[1] = 1;
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        double s = wave_sum(w[c]);
+        if (lane == 0) s_red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = 0.0;
+        if (threadIdx.x < 9)
+            for (int q = 0; q < 4; ++q) s += s_red[q][threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = s;
+    }
+}
+
+// kinetic_energy (thermostat.c:34-63)
+__global__ void __launch_bounds__(kRedThreads)
+k_kinetic(int N, int ndf, const double* __restrict__ mass, const double* __restrict__ v,
+          double* __restrict__ md) {
+    __shared__ double s_w[16];
+    double xx = 0.0, yy = 0.0, zz = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double m = mass[i];
+        if (m > 0.0) {
+            xx += m * v[i] * v[i];
+            yy += m * v[N + i] * v[N + i];
+            zz += m * v[2 * N + i] * v[2 * N + i];
+        }
+    }
+    xx = block_sum1024(xx, s_w);
+    yy = block_sum1024(yy, s_w);
+    zz = block_sum1024(zz, s_w);
+    if (threadIdx.x == 0) {
+        double ek = xx + yy + zz;
+        md[0] = ek;
+        md[1] = ek / (K_BOLTZ * ndf);
+    }
+}
+
+// remove_comm (thermostat.c:69-108)
+__global__ void __launch_bounds__(kRedThreads)
+k_com(int N, const double* __restrict__ mass, double* __restrict__ v) {
+    __shared__ double s_w[16];
+    double px = 0.0, py = 0.0, pz = 0.0, ms = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double m = mass[i];
+        if (m > 0.0) { px += v[i] * m; py += v[N + i] * m; pz += v[2 * N + i] * m; ms += m; }
+    }
+    px = block_sum1024(px, s_w);
+    py = block_sum1024(py, s_w);
+    pz = block_sum1024(pz, s_w);
+    ms = block_sum1024(ms, s_w);
+    px /= ms; py /= ms; pz /= ms;
+    for (int i = threadIdx.x; i < N; i += blockDim.x)
+        if (mass[i] > 0.0) { v[i] -= px; v[N + i] -= py; v[2 * N + i] -= pz; }
+}
+
+// nose_hoover_chain (thermostat.c:117-191); the velocity scale goes to md[22]
+__global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int Nc = 5, Nys = 5, M = CAPAC_NHC;
+    const double w1 = 0.41449077179437571194, w3 = -0.65796308717750284778;
+    const double w[5] = {w1, w1, w3, w1, w1};
+    double* vQ = md + 2;
+    double* aQ = md + 12;
+    double qMass = md[25], eKsum = md[0];
+    double NkT = kT * ndf;
+    double aa, scale = 1.0;
+    aQ[0] = (eKsum - NkT) / qMass;
+    for (int im = 0; im < M - 1; ++im) aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) / qMass;
+    for (int ic = 0; ic < Nc; ++ic)
+        for (int iys = 0; iys < Nys; ++iys) {
+            double wdt = w[iys] * dt;
+            vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
+            for (int im = 0; im < M - 1; ++im) {
+                aa = exp(-wdt / (8 * Nc) * vQ[M - (im + 1)]);
+                vQ[M - 1 - (im + 1)] = vQ[M - 1 - (im + 1)] * aa * aa + wdt / (4 * Nc) * aQ[M - 1 - (im + 1)] * aa;
+            }
+            aa = exp(-wdt / (2 * Nc) * vQ[0]);
+            scale *= aa;
+            aQ[0] = (eKsum * scale * scale - NkT) / qMass;
+            for (int im = 0; im < M - 1; ++im) {
+                aa = exp(-wdt / (8 * Nc) * vQ[im + 1]);
+                vQ[im] = vQ[im] * aa * aa + wdt / (4 * Nc) * aQ[im] * aa;
+                aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) / qMass;
+            }
+            vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
+        }
+    md[22] = scale;
+    md[0] = eKsum * (scale * scale);
+    md[1] *= scale * scale;
+}
+
+__global__ void k_scale_v(int n3, const double* __restrict__ md, double* __restrict__ v) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3) v[k] *= md[22];
+}
+
+// apply_pbc (force.c:127-165): wrap the middle atom of each molecule into the
+// box, the others to within half a box of it
+__global__ void k_wrap(int n_mols, int N, BoxConst bc, const int* __restrict__ first,
+                       const int* __restrict__ last, double* __restrict__ r) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_mols) return;
+    int a0 = first[m], a1 = last[m], mid = (a0 + a1) / 2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double* x = r + c * N;
+        double L = bc.box[c], h = 0.5 * bc.box[c];
+        if (x[mid] > L) x[mid] -= L; else if (x[mid] < 0.0) x[mid] += L;
+        for (int i = a0; i <= a1; ++i) {
+            if (i == mid) continue;
+            if (x[i] - x[mid] > h) x[i] -= L; else if (x[i] - x[mid] < -h) x[i] += L;
+        }
+    }
+}
+
+// get_fmax_rms (force.c:361-378)
+__global__ void __launch_bounds__(kRedThreads)
+k_fmax(int N, const double* __restrict__ f, double* __restrict__ md) {
+    __shared__ double s_w[16];
+    __shared__ double s_m[16];
+    double mx = 0.0, sum = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double f2 = f[i] * f[i] + f[N + i] * f[N + i] + f[2 * N + i] * f[2 * N + i];
+        sum += f2;
+        mx = fmax(mx, f2);
+    }
+    sum = block_sum1024(sum, s_w);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, s_m[w]);
+        md[23] = sqrt(m);
+        md[24] = sqrt(sum / N);
+    }
+}
+
+// external field on all charges (force.c:2251-2260); pot[9] partials
+__global__ void __launch_bounds__(256)
+k_ext(int N, BoxConst bc, const double* __restrict__ charge, const double* __restrict__ r,
+      double* __restrict__ f, double* __restrict__ partial) {
+    __shared__ double s_red[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < N) {
+        double q = charge[i];
+        f[i] += bc.efield[0] * q;
+        f[N + i] += bc.efield[1] * q;
+        f[2 * N + i] += bc.efield[2] * q;
+        e = -(bc.efield[0] * q * r[i] + bc.efield[1] * q * r[N + i] + bc.efield[2] * q * r[2 * N + i]);
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x < 16) partial[blockIdx.x * 16 + threadIdx.x] =
+        threadIdx.x == 0 ? s_red[0] + s_red[1] + s_red[2] + s_red[3] : 0.0;
+}
+
+}  // namespace
+
+static inline int grid(int n, int b) { return (n + b - 1) / b; }
+
+int md_kick(capac_ctx* c) {
+    k_kick<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->bc.Nm, c->fix_metal, 0.5 * c->dt,
+                                                      c->f, c->inv_mass, c->v);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_drift(capac_ctx* c) {
+    k_drift<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->bc.Nm, c->fix_metal, c->dt, c->r,
+                                                       c->v, c->old_r);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_rattle1(capac_ctx* c) {
+    if (c->n_cstr == 0) return 0;
+    k_rattle1<<<grid(c->bc.n_mols, 256), 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt,
+                                                              c->mol_cstr, c->cstr_i, c->cstr_j,
+                                                              c->cstr_r, c->inv_mass, c->r, c->v,
+                                                              c->old_r, c->md_status);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+// red[64..73) = virial of the velocity constraints
+int md_rattle2(capac_ctx* c) {
+    if (c->n_cstr == 0) {
+        return fill_zero_d(c, c->red + 64, 16);
+    }
+    int nblk = grid(c->bc.n_mols, 256);
+    k_rattle2<<<nblk, 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt, c->mol_cstr,
+                                           c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
+                                           c->md_status, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return reduce_partials(c, nblk, 9, 64);
+}
+
+int md_kinetic(capac_ctx* c) {
+    k_kinetic<<<1, kRedThreads, 0, c->stream>>>(c->bc.N, c->ndf, c->mass, c->v, c->md);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_nhc(capac_ctx* c, double kT) {
+    k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf);
+    k_scale_v<<<grid(3 * c->bc.N, 256), 256, 0, c->stream>>>(3 * c->bc.N, c->md, c->v);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_remove_com(capac_ctx* c) {
+    k_com<<<1, kRedThreads, 0, c->stream>>>(c->bc.N, c->mass, c->v);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_wrap(capac_ctx* c) {
+    k_wrap<<<grid(c->bc.n_mols, 256), 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->mol_first,
+                                                           c->mol_last, c->r);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int md_fmax(capac_ctx* c) {
+    k_fmax<<<1, kRedThreads, 0, c->stream>>>(c->bc.N, c->f, c->md);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+// red[80] = pot9 from the charges
+int ext_field(capac_ctx* c) {
+    int nblk = grid(c->bc.N, 256);
+    k_ext<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->bc, c->charge, c->r, c->f, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return reduce_partials(c, nblk, 1, 80);
+}

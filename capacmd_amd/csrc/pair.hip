@@ -1,0 +1,146 @@
+// Non-bonded pair forces: shifted-force LJ + damped-shifted-force Coulomb
+// (funct 1) and erf-damped vdW (funct 4), the reference's mpi_nonb_pair +
+// compute_nonbonded + compute_erf_vdw (force.c:1402-1540, 710-816, 591-632).
+//
+// One wavefront per atom i over its full neighbour row (front and back
+// segments); lanes take consecutive neighbours, so the index stream is
+// coalesced and the j-position gathers hit L2.  The force on i is a fixed-order
+// wavefront reduction written once (no atomics); each pair is evaluated from
+// both ends, so pair energies and the virial carry a factor 1/2.
+// FP64-VALU bound (erfc + exp per charged pair).
+#include "capac_internal.h"
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+struct PairOut {
+    double fx, fy, fz, e7, e8, v[9];
+};
+
+__device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
+                                          double dx, double dy, double dz, double r2, PairOut& o) {
+    double flj = 0.0, fqq = 0.0;
+    double rij = sqrt(r2);
+    if (pp.funct == 1) {
+        if (pp.c6 != 0.0 || pp.c12 != 0.0) {
+            double r6 = r2 * r2 * r2;
+            double r12 = r6 * r6;
+            double cr12 = pp.c12 / r12, cr6 = pp.c6 / r6;
+            if (bc.use_vdw == 1) {
+                // shifted force LJ (force.c:748-757)
+                double f_r = 12.0 * cr12 - 6.0 * cr6;
+                double f_c = 12.0 * pp.c12 * bc.inv_rc12 - 6.0 * pp.c6 * bc.inv_rc6;
+                flj = (f_r - f_c) / r2;
+                o.e8 += (cr12 - cr6) + (rij - bc.rc) * f_c / bc.rc -
+                        (pp.c12 * bc.inv_rc12 - pp.c6 * bc.inv_rc6);
+            } else {
+                flj = (2.0 * cr12 - cr6) * 6.0 / r2;
+                o.e8 += cr12 - cr6;
+            }
+        }
+        if (qi != 0.0 && qj != 0.0) {
+            double qq = K_FQQ * qi * qj;
+            if (bc.use_coulomb == 1) {
+                // damped shifted force (force.c:787-795)
+                double erfc_a = erfc(bc.alpha * rij);
+                double ex = exp(-bc.alpha * bc.alpha * r2);
+                fqq = qq * ((erfc_a / r2 + bc.a2_sqrtPI * ex / rij) - bc.wolfConst) / rij;
+                o.e7 += qq * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+            } else {
+                fqq = qq / (r2 * rij);
+                o.e7 += qq / rij;
+            }
+        }
+    } else if (pp.funct == 4) {
+        if (pp.c6 != 0.0 || pp.c12 != 0.0) {
+            // erf-damped vdW (force.c:602-628)
+            double r6 = r2 * r2 * r2;
+            double r12 = r6 * r6;
+            double cr12 = pp.c12 / r12, cr6 = pp.c6 / r6;
+            double rR = rij / pp.width;
+            double erf_rR = erf(rR);
+            o.e8 += (cr12 - cr6) * erf_rR;
+            flj = ((12.0 * cr12 - 6.0 * cr6) / rij * erf_rR -
+                   (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * exp(-rR * rR) / pp.width) / rij;
+        }
+    }
+    double fs = flj + fqq;               // f_j = fs * r_ij, f_i = -f_j
+    double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
+    o.fx -= fjx; o.fy -= fjy; o.fz -= fjz;
+    o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
+    o.v[3] -= dy * fjx; o.v[4] -= dy * fjy; o.v[5] -= dy * fjz;
+    o.v[6] -= dz * fjx; o.v[7] -= dz * fjy; o.v[8] -= dz * fjz;
+}
+
+__global__ void __launch_bounds__(256)
+k_pair(const double* __restrict__ r, const double* __restrict__ charge, const int* __restrict__ type,
+       const PairParam* __restrict__ table, BoxConst bc, const int* __restrict__ n_front,
+       const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
+       double* __restrict__ f, int accumulate, double* __restrict__ partial) {
+    __shared__ PairParam s_tab[64];
+    __shared__ double s_red[kWavesPerBlock][11];
+    const int N = bc.N, nt = bc.n_types;
+    for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) s_tab[k] = table[k];
+    __syncthreads();
+    const int lane = threadIdx.x & (CAPAC_WAVE - 1);
+    const int wave = threadIdx.x / CAPAC_WAVE;
+    PairOut acc = {};
+    for (int i = blockIdx.x * kWavesPerBlock + wave; i < N; i += gridDim.x * kWavesPerBlock) {
+        double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+        double qi = charge[i];
+        int ti = type[i] * nt;
+        int nf = n_front[i], nb = n_back[i];
+        const int* row = nbr + (size_t)i * stride;
+        PairOut o = {};
+        int total = nf + nb;
+        for (int k = lane; k < total; k += CAPAC_WAVE) {
+            int j = k < nf ? row[k] : row[stride - nb + (k - nf)];
+            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+            pbc12(bc, dx, dy, dz);
+            double r2 = dist2_exact(dx, dy, dz);
+            pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o);
+        }
+        double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
+        if (lane == 0) {
+            if (accumulate) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
+            else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
+        }
+        acc.e7 += o.e7; acc.e8 += o.e8;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc.v[c] += o.v[c];
+    }
+    double vals[11] = {acc.e7, acc.e8, acc.v[0], acc.v[1], acc.v[2], acc.v[3], acc.v[4],
+                       acc.v[5], acc.v[6], acc.v[7], acc.v[8]};
+#pragma unroll
+    for (int c = 0; c < 11; ++c) {
+        double s = wave_sum(vals[c]);
+        if (lane == 0) s_red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = 0.0;
+        if (threadIdx.x < 11)
+            for (int w = 0; w < kWavesPerBlock; ++w) s += s_red[w][threadIdx.x];
+        // each ordered pair was visited from both atoms
+        partial[blockIdx.x * 16 + threadIdx.x] = 0.5 * s;
+    }
+}
+
+}  // namespace
+
+int pair_blocks(int N) {
+    int nb = (N + kWavesPerBlock - 1) / kWavesPerBlock;
+    return nb < 2048 ? nb : 2048;
+}
+
+// writes (or adds to) f; red[0..11) = pot7, pot8, vir[9]
+int pair_nonbonded(capac_ctx* c, double* f_accumulate) {
+    int nblk = pair_blocks(c->bc.N);
+    double* f = f_accumulate ? f_accumulate : c->f;
+    k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
+                                        c->n_back, c->nbr, c->nb_stride, f,
+                                        f_accumulate ? 1 : 0, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return reduce_partials(c, nblk, 11, 0);
+}

@@ -1,0 +1,171 @@
+/* capacmd -- CLI of the MI355X build, the reference's program surface
+ * (reference mpi-main.c:41-1219, one GPU in place of the MPI ranks):
+ *
+ *   capacmd -gro init.gro -par param.txt -mds mdset.txt [-prec fp64|fp32] [-dev N]
+ *
+ * Reads the three inputs with the reference's grammar, runs the force call
+ * and the MD loop on the device, and writes traj.gro (every 10 save steps),
+ * vec_pq.txt (with traj.gro), traj.dat (every save step) and the stdout step
+ * lines, final energy table and phase timings in the reference's formats.
+ * Failures print the reference's message and exit(1), as the reference does.
+ * run_type em/cg (energy minimisation) is not part of this build.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <time.h>
+
+#include "capac_gpu.h"
+#include "capac_host.h"
+
+static double wall(void) {
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec + tv.tv_usec * 1e-6;
+}
+
+static void die(const char* msg) {
+    printf("%s\n", msg);
+    exit(1);
+}
+
+int main(int argc, char** argv) {
+    const char* gro = "init.gro";
+    const char* par = "param.txt";
+    const char* mds = "mdset.txt";
+    int precision = CAPAC_PREC_FP64, device = 0;
+    for (int i = 1; i < argc; ++i) {
+        if (!strcmp(argv[i], "-gro") && i + 1 < argc) gro = argv[++i];
+        else if (!strcmp(argv[i], "-par") && i + 1 < argc) par = argv[++i];
+        else if (!strcmp(argv[i], "-mds") && i + 1 < argc) mds = argv[++i];
+        else if (!strcmp(argv[i], "-prec") && i + 1 < argc) {
+            ++i;
+            precision = !strcmp(argv[i], "fp32") ? CAPAC_PREC_FP32_REFINED : CAPAC_PREC_FP64;
+        } else if (!strcmp(argv[i], "-dev") && i + 1 < argc) device = atoi(argv[++i]);
+    }
+    time_t start_t = time(NULL);
+    double t_start = wall();
+
+    char err[1024];
+    capac_deck* d = NULL;
+    if (capac_deck_load(gro, par, mds, &d, err, sizeof err) != 0) {
+        printf("%s", err);
+        return 1;
+    }
+    const capac_params* p = &d->p;
+    const int N = p->n_atoms;
+    const int n_mat = capac_deck_n_mat(d);
+
+    printf("\n");
+    printf("              +-----------------------------------------------------+\n");
+    printf("              |      CapacMD force + CPIM path, MI355X (gfx950)     |\n");
+    printf("              +-----------------------------------------------------+\n\n");
+    printf("    Calculation started at %s", ctime(&start_t));
+    printf("    One GPU (device %d), %s relay matrix\n\n", device,
+           precision == CAPAC_PREC_FP64 ? "FP64" : "FP32-refined");
+    printf("              .------------------ run parameters -------------------.\n\n");
+    printf("    run_type = %s, ensemble = %s\n", d->run_type, d->ensemble);
+    printf("    vdw_type = %s, coulomb_type = %s\n", d->vdw_type, d->coulomb_type);
+    printf("    rCut = %.3f nm, ", p->rcut);
+    if (p->use_coulomb == 1) printf("alpha = %.3f nm^-1", p->dsf_alpha);
+    printf("\n    ref_T = %.1f K\n\n\n", p->ref_temp);
+    printf("              .------------------- molecule info -------------------.\n\n");
+    printf("    There are %d types of molecules.\n\n", d->mol_types);
+    printf("    %d atoms, %d metal atoms, %d nanoparticles, %d CPIM unknowns\n\n", N, p->n_metal,
+           p->n_nps, n_mat);
+
+    if (strcmp(d->run_type, "md") != 0) {
+        printf("Error: run_type %s is not supported by this build (md only)\n", d->run_type);
+        return 1;
+    }
+
+    capac_ctx* ctx = NULL;
+    if (capac_create(p, precision, device, &ctx) != 0) {
+        printf("Error: %s\n", capac_last_error());
+        return 1;
+    }
+    capac_enable_timing(ctx, 1);
+    double* x = (double*)calloc(n_mat > 0 ? n_mat : 1, sizeof(double));
+    double* r = (double*)malloc(sizeof(double) * 3 * (size_t)N);
+    double* v = (double*)malloc(sizeof(double) * 3 * (size_t)N);
+    if (capac_set_state(ctx, d->r, d->v, x) != 0) die(capac_last_error());
+    double vq[CAPAC_NHC] = {0};
+    vq[0] = d->vq0;
+    capac_set_thermostat(ctx, vq, d->vp);
+
+    double pot[CAPAC_NPOT], vir[9];
+    int rc = capac_force(ctx, pot, vir, NULL);
+    if (rc == CAPAC_ERR_SOLVER) die(capac_last_error());
+    if (rc != 0) { printf("Error: %s\n", capac_last_error()); return 1; }
+    if (capac_md_begin(ctx) != 0) die(capac_last_error());
+
+    FILE* fgro = fopen("traj.gro", "w");
+    if (!fgro) die("Cannot write to traj.gro!");
+    FILE* fpq = fopen("vec_pq.txt", "w");
+    if (!fpq) die("Cannot write to vec_pq.txt!");
+    FILE* fdat = fopen("traj.dat", "w");
+    if (!fdat) die("Cannot write to traj.dat!");
+
+    double ek, temp, reft, fmax, frms;
+    capac_md_observables(ctx, &ek, &temp, &reft, vq, &fmax, &frms, pot, vir);
+    capac_get_state(ctx, r, v, x);
+    capac_write_gro(fgro, d, r, v, vq[0], d->vp, 0);
+    capac_write_binary(fdat, d, r, 0);
+    if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, 0);
+    printf("              .---------------- start MD calculation ---------------.\n\n");
+    printf("  %10.3f  Fmax=%.2e  E=%.6e  T=%.3e\n", 0.0, fmax, pot[0], temp);
+    fflush(stdout);
+
+    int nsave = d->n_save > 0 ? d->n_save : 1;
+    int step = 1;
+    while (step <= d->n_steps) {
+        int done = capac_md_run(ctx, step, d->n_steps - step + 1, nsave, 1);
+        if (done < 0) {
+            if (done == CAPAC_ERR_SOLVER || done == CAPAC_ERR_RATTLE1 || done == CAPAC_ERR_RATTLE2)
+                die(capac_last_error());
+            printf("Error: %s\n", capac_last_error());
+            return 1;
+        }
+        step += done;
+        int last = step - 1;
+        if (last % nsave == 0) {
+            capac_md_observables(ctx, &ek, &temp, &reft, vq, &fmax, &frms, pot, vir);
+            capac_get_state(ctx, r, v, x);
+            printf("  %10.3f  Fmax=%.2e  E=%.6e  T=%.3e (%.1f)\n", p->dt * last, fmax, pot[0], temp, reft);
+            fflush(stdout);
+            capac_write_binary(fdat, d, r, last);
+            if (last % (nsave * 10) == 0) {
+                capac_write_gro(fgro, d, r, v, vq[0], d->vp, last);
+                if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, last);
+            }
+        }
+        if (done == 0) break;
+    }
+    fclose(fdat);
+    fclose(fpq);
+    fclose(fgro);
+
+    capac_md_observables(ctx, &ek, &temp, &reft, vq, &fmax, &frms, pot, vir);
+    printf("\n\n              .------------- final potential energies --------------.\n\n");
+    capac_print_potential(stdout, pot);
+    time_t end_t = time(NULL);
+    printf("    Calculation ended normally at %s", ctime(&end_t));
+    printf("    %.3f seconds were used\n\n\n", wall() - t_start);
+    capac_timing tm;
+    capac_get_timing(ctx, &tm);
+    printf("              .------------------- time usage ----------------------.\n\n");
+    printf("    Total time used in force: %.3f seconds\n", tm.t[0]);
+    printf("    Time used in QSC:         %.3f seconds\n", tm.t[2]);
+    printf("    Time used in Nonbonded:   %.3f seconds\n", tm.t[4]);
+    printf("    Time used in CPIM-matrix: %.3f seconds\n", tm.t[5]);
+    printf("    Time used in CPIM-vector: %.3f seconds\n", tm.t[6]);
+    printf("    Time used in CPIM-solve:  %.3f seconds (%lld iterations)\n", tm.t[7], tm.solve_iters);
+    printf("    Time used in CPIM-force:  %.3f seconds\n", tm.t[8]);
+    printf("    Time used in neighbours:  %.3f seconds\n", tm.t[10]);
+    printf("    Time used in integrator:  %.3f seconds\n\n", tm.t[12]);
+    capac_destroy(ctx);
+    capac_deck_free(d);
+    free(x); free(r); free(v);
+    return 0;
+}

@@ -164,7 +164,10 @@ GPU_EXPORTS = [
     "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_force",
     "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
+    "capac_time_kernel",
 ]
+
+KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM = range(5)
 
 
 def gpu_lib() -> C.CDLL:
@@ -198,6 +201,7 @@ def gpu_lib() -> C.CDLL:
         lib.capac_get_timing.argtypes = [C.c_void_p, C.POINTER(CapacTiming)]
         lib.capac_reset_timing.argtypes = [C.c_void_p]
         lib.capac_enable_timing.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         _gpu = lib
     return _gpu
 
@@ -342,6 +346,11 @@ class Context:
 
     def reset_timing(self):
         _check(self.lib.capac_reset_timing(self.h), "capac_reset_timing")
+
+    def time_kernel(self, kernel: int, reps: int = 20) -> float:
+        ms = C.c_double()
+        _check(self.lib.capac_time_kernel(self.h, kernel, reps, C.byref(ms)), "capac_time_kernel")
+        return ms.value
 
     def enable_timing(self, on: bool = True):
         _check(self.lib.capac_enable_timing(self.h, int(on)), "capac_enable_timing")

@@ -211,6 +211,22 @@ int capac_md_observables(capac_ctx* ctx, double* eksum, double* temp, double* re
                          double vir[9]);
 int capac_set_thermostat(capac_ctx* ctx, const double* vq, double vp);
 
+/* Average device time of one launch of a hot kernel, measured with HIP
+ * events on the context's stream over `reps` back-to-back launches on the
+ * resident data (no counterpart in the reference; for benchmarks):
+ *   CAPAC_KERNEL_SPMV      relay SpMV (one BiCGSTAB half-iteration)
+ *   CAPAC_KERNEL_ASSEMBLE  relay assembly
+ *   CAPAC_KERNEL_PAIR      pair forces over the resident neighbour rows
+ *   CAPAC_KERNEL_NEIGHBOUR cell list + neighbour rows
+ *   CAPAC_KERNEL_CPIM_MM   metal-metal CPIM forces
+ * Call after one capac_force so that every input is resident. */
+#define CAPAC_KERNEL_SPMV      0
+#define CAPAC_KERNEL_ASSEMBLE  1
+#define CAPAC_KERNEL_PAIR      2
+#define CAPAC_KERNEL_NEIGHBOUR 3
+#define CAPAC_KERNEL_CPIM_MM   4
+int capac_time_kernel(capac_ctx* ctx, int kernel, int reps, double* ms_per_launch);
+
 int capac_get_timing(capac_ctx* ctx, capac_timing* t);
 int capac_reset_timing(capac_ctx* ctx);
 int capac_enable_timing(capac_ctx* ctx, int on);
