@@ -85,6 +85,7 @@ def run_gpu(deck, precision: int, steps: int, warmup: int):
     t1 = time.perf_counter()
     assert done == steps, done
     t, calls, iters = ctx.timing()
+    builds = ctx.nb_builds()
     obs = ctx.observables()
     ctx.enable_timing(False)
     sec = (t1 - t0) / steps
@@ -100,6 +101,8 @@ def run_gpu(deck, precision: int, steps: int, warmup: int):
         "rhs_ms_per_step": 1e3 * t[6] / steps,
         "integrator_ms_per_step": 1e3 * t[12] / steps,
         "solve_iters_per_step": iters / steps,
+        "nb_builds_per_step": builds / steps,
+        "nb_skin_nm": ctx.skin,
         "final_temp": obs["temp"],
         "final_energy": float(obs["pot"][0]),
     }

@@ -302,18 +302,27 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         free(npo);
         if (rc) { capac_destroy(c); return rc; }
     }
-    // cell list and neighbour rows
-    nb_grid_setup(c);
+    // cell list and neighbour rows (list radius rc + skin)
+    c->skin_req = CAPAC_DEFAULT_SKIN;
+    c->nb_dirty = 1;
+    TRY(nb_set_radius(c, c->skin_req));
     TRY(alloc_cells(c));
+    TRY(upload(&c->r_build, (const double*)NULL, 3 * (size_t)N));
+    TRY(upload(&c->nb_ctl, (const unsigned long long*)NULL, 4));
     TRY(upload(&c->cell_of, (const int*)NULL, N));
+    TRY(upload(&c->cell_rank, (const int*)NULL, N));
+    TRY(upload(&c->sorted_r, (const double*)NULL, 3 * (size_t)N));
+    TRY(upload(&c->sorted_mol, (const int*)NULL, N));
+    TRY(upload(&c->sorted_loc, (const int*)NULL, N));
     TRY(upload(&c->cell_atoms, (const int*)NULL, N));
     TRY(upload(&c->cell_tmp, (const int*)NULL, N));
     TRY(upload(&c->n_front, (const int*)NULL, N));
     TRY(upload(&c->n_back, (const int*)NULL, N));
     TRY(upload(&c->nb_overflow, (const int*)NULL, 4));
     {
-        // room for the densest liquid or metal around a sphere of radius rc
-        double vol = 4.0 / 3.0 * 3.14159265358979 * b.rc * b.rc * b.rc;
+        // room for the densest liquid or metal around a sphere of radius rc + skin
+        double rl = b.rc + c->skin;
+        double vol = 4.0 / 3.0 * 3.14159265358979 * rl * rl * rl;
         int est = (int)(vol * 140.0) + 64;
         int stride = ((est + 63) / 64) * 64;
         if (stride > N) stride = ((N + 63) / 64) * 64;
@@ -369,7 +378,8 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->local_idx, c->excl, c->pair, c->mol_first, c->mol_last, c->mol_cstr,
                     c->cstr_i, c->cstr_j, c->cstr_r, c->np_first, c->np_last, c->np_of_metal,
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
-                    c->cell_tmp, c->cell_fill, c->n_front, c->n_back, c->nbr, c->nb_overflow,
+                    c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
+                    c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->nb_overflow, c->r_build, c->nb_ctl,
                     c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->md, c->md_status};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
@@ -391,6 +401,7 @@ extern "C" int capac_set_state(capac_ctx* c, const double* r, const double* v, c
     if (x && c->n_mat)
         CHECK_HIP(hipMemcpyAsync(c->x, x, c->n_mat * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (r) c->nb_dirty = 1;
     return 0;
 }
 
@@ -413,6 +424,8 @@ extern "C" int capac_get_forces(capac_ctx* c, double* f) {
     return 0;
 }
 
+static int nb_regrid(capac_ctx* c);
+
 extern "C" int capac_set_box(capac_ctx* c, const double box[3]) {
     if (!c || !box) return CAPAC_ERR_ARG;
     for (int d = 0; d < 3; ++d) {
@@ -420,11 +433,30 @@ extern "C" int capac_set_box(capac_ctx* c, const double box[3]) {
         c->bc.box[d] = box[d];
         c->bc.half[d] = 0.5 * box[d];
     }
+    return nb_regrid(c);
+}
+
+// list radius and cell grid for the current box and requested skin
+static int nb_regrid(capac_ctx* c) {
     int old = c->n_cell;
-    nb_grid_setup(c);
+    int rc = nb_set_radius(c, c->skin_req);
+    if (rc) return rc;
+    c->nb_dirty = 1;
     if (c->n_cell > old) return alloc_cells(c);
     return 0;
 }
+
+extern "C" int capac_set_skin(capac_ctx* c, double skin) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!(skin >= 0.0) || !(skin < 0.5 * c->bc.rc + 1.0)) {
+        capac_set_error("skin must be >= 0 and finite");
+        return CAPAC_ERR_ARG;
+    }
+    c->skin_req = skin;
+    return nb_regrid(c);
+}
+
+extern "C" double capac_skin(const capac_ctx* c) { return c ? c->skin : -1.0; }
 
 // ------------------------------------------------------------------ force
 
@@ -456,7 +488,8 @@ static int force_compute(capac_ctx* c) {
         mark(c, 0);
         if ((rc = fill_d(c, c->red, 256, 0.0))) return rc;
         if (c->n_mat && (rc = copy_d(c, c->x, c->x_backup, c->n_mat))) return rc;
-        if ((rc = nb_build(c))) return rc;
+        if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
+        c->nb_dirty = 0;
         mark(c, 1);
         if ((rc = pair_nonbonded(c, NULL))) return rc;
         mark(c, 2);
@@ -483,6 +516,7 @@ static int force_compute(capac_ctx* c) {
         if (rc2) return rc2;
         if (!nb_overflowed(c)) break;
         // a neighbour row overflowed: grow, restore the warm start, redo
+        c->nb_dirty = 1;
         if ((rc2 = grow_nbr(c))) return rc2;
         if (c->n_mat && (rc2 = copy_d(c, c->x_backup, c->x, c->n_mat))) return rc2;
         if (attempt == 7) { capac_set_error("neighbour rows keep overflowing"); return CAPAC_ERR_HIP; }
@@ -532,7 +566,9 @@ static int build_until_fits(capac_ctx* c) {
         int rc = nb_build(c);
         if (rc) return rc;
         if ((rc = fetch_red(c))) return rc;
+        c->nb_dirty = 0;
         if (!nb_overflowed(c)) return 0;
+        c->nb_dirty = 1;
         if ((rc = grow_nbr(c))) return rc;
     }
     capac_set_error("neighbour rows keep overflowing");
@@ -839,16 +875,28 @@ extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms)
     return 0;
 }
 
+static int nb_build_count(capac_ctx* c, long long* n) {
+    unsigned long long v = 0;
+    CHECK_HIP(hipMemcpyAsync(&v, c->nb_ctl + 2, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    *n = (long long)v;
+    return 0;
+}
+
 extern "C" int capac_get_timing(capac_ctx* c, capac_timing* t) {
     if (!c || !t) return CAPAC_ERR_ARG;
+    long long n = 0;
+    int rc = nb_build_count(c, &n);
+    if (rc) return rc;
     *t = c->timing;
+    t->nb_builds = n - c->nb_builds_base;
     return 0;
 }
 
 extern "C" int capac_reset_timing(capac_ctx* c) {
     if (!c) return CAPAC_ERR_ARG;
     memset(&c->timing, 0, sizeof c->timing);
-    return 0;
+    return nb_build_count(c, &c->nb_builds_base);
 }
 
 extern "C" int capac_enable_timing(capac_ctx* c, int on) {

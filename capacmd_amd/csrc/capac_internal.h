@@ -45,6 +45,7 @@ struct BoxConst {
     int N, Nm, n_np, n_types, n_mols;
     double box[3], half[3];
     double rc, rc2, alpha, a2_sqrtPI, wolfConst, erfc_arCut, inv_rc12, inv_rc6;
+    double rl2;                 // neighbour-row radius squared: (rc + skin)^2
     int use_coulomb, use_vdw;
     int qsc_n, qsc_m;
     double qsc_eps, qsc_c, qsc_a;
@@ -86,12 +87,20 @@ struct capac_ctx {
 
     // cell list
     int ncell[3], n_cell;
-    int *cell_of, *cell_count, *cell_start, *cell_atoms, *cell_tmp, *cell_fill;
-    int max_cell;               // largest cell occupancy (for LDS staging)
+    int *cell_of, *cell_count, *cell_start, *cell_atoms, *cell_tmp, *cell_fill, *cell_rank;
+    double* sorted_r;           // 3N, positions in cell order
+    int *sorted_mol, *sorted_loc;
+    void* stencil;              // device copy of the cell stencil
     // neighbour rows
     int nb_stride;
     int *n_front, *n_back, *nbr, *nb_overflow;
-    int nb_valid;
+    // Verlet skin: rows hold pairs within rc + skin; the device rebuilds them
+    // when an atom has moved more than skin/2 since the last build
+    double skin, skin_req;      // in use / requested (nm)
+    int nb_dirty;               // positions or box replaced: build unconditionally
+    long long nb_builds_base;   // build counter at the last timing reset
+    double* r_build;            // 3N positions at the last build
+    unsigned long long* nb_ctl; // [0] max displacement^2 bits, [1] rebuild flag, [2] builds
 
     // reductions: per-block partials and a small result area
     double* partial;            // n_partial_blocks * 16
@@ -154,7 +163,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // neighbour.hip
 int nb_grid_setup(capac_ctx* c);
-int nb_build(capac_ctx* c);
+int nb_build(capac_ctx* c);                 // unconditional build at rc + skin
+int nb_build_if_needed(capac_ctx* c);       // device-side decision from displacements
+int nb_set_radius(capac_ctx* c, double skin);
 // pair.hip
 int pair_nonbonded(capac_ctx* c, double* f_out_accumulate);
 // sc.hip

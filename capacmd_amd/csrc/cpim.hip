@@ -40,6 +40,7 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
         double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
         pbc12(bc, dx, dy, dz);
         double r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < bc.rc2)) continue;             // rows reach rc + skin
         double rij = sqrt(r2);
         double e_field, v_field;
         if (bc.use_coulomb == 1) {
@@ -215,6 +216,7 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
             double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < bc.rc2)) continue;         // rows reach rc + skin
             double fp[3];
             mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11);
             fx += fp[0]; fy += fp[1]; fz += fp[2];
@@ -242,14 +244,15 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
     }
 }
 
-// atom side: one thread per non-metal atom over the metal tail of its row;
+// atom side: one wavefront per non-metal atom over the metal tail of its row;
 // the reaction force of the same pair term (the reference's f_atom -= f)
 __global__ void __launch_bounds__(256)
 k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
                const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
                const double* __restrict__ x, double* __restrict__ f) {
     const int N = bc.N, Nm = bc.Nm;
-    int a = Nm + blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int a = Nm + blockIdx.x * 4 + (threadIdx.x >> 6);
     if (a >= N) return;
     double qa = charge[a];
     int nb = n_back[a];
@@ -257,17 +260,19 @@ k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, 
     const int* tail = nbr + (size_t)a * stride + stride - nb;
     double xa = r[a], ya = r[N + a], za = r[2 * N + a];
     double fx = 0.0, fy = 0.0, fz = 0.0, e10 = 0.0, e11 = 0.0;
-    for (int k = 0; k < nb; ++k) {
+    for (int k = lane; k < nb; k += 64) {
         int i = tail[k];
         double dx = r[i] - xa, dy = r[N + i] - ya, dz = r[2 * N + i] - za;
         pbc12(bc, dx, dy, dz);
         double r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < bc.rc2)) continue;             // rows reach rc + skin
         double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
         double fp[3];
         mw_pair(bc, dx, dy, dz, r2, qa, x[3 * Nm + i], pm, fp, &e10, &e11);
         fx += fp[0]; fy += fp[1]; fz += fp[2];
     }
-    f[a] -= fx; f[N + a] -= fy; f[2 * N + a] -= fz;
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) { f[a] -= fx; f[N + a] -= fy; f[2 * N + a] -= fz; }
 }
 
 // Metal-metal terms, all pairs, force on i only (cpff.c:762-927), plus the
@@ -420,7 +425,7 @@ int cpim_forces_x(capac_ctx* c, const double* x) {
     int rc = reduce_partials(c, nblk, 11, 32);
     if (rc) return rc;
     if (N > Nm)
-        k_cpim_mw_atom<<<(N - Nm + 255) / 256, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
+        k_cpim_mw_atom<<<(N - Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
                                                                     c->nbr, c->nb_stride, x, c->f);
     k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, c->partial);
     CHECK_HIP(hipGetLastError());

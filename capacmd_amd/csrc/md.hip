@@ -10,6 +10,11 @@
 //              [24] F_rms [25] qMass
 #include "capac_internal.h"
 
+// Integrator arithmetic is evaluated as written, without fused multiply-add,
+// as the reference's x86-64 build does, so that RATTLE's tolerance decisions
+// see the same numbers.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int kRedThreads = 1024;
@@ -54,6 +59,8 @@ __device__ __forceinline__ double pbc1(double dx, double L) {
     return dx;
 }
 
+// One thread per molecule; Gauss-Seidel sweeps over its constraints in file
+// order with the reference's arithmetic (no contraction, see the file pragma).
 __global__ void k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
                           const int* __restrict__ ci, const int* __restrict__ cj,
                           const double* __restrict__ cr, const double* __restrict__ inv_mass,
@@ -155,65 +162,78 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
     }
 }
 
-// kinetic_energy (thermostat.c:34-63)
-__global__ void __launch_bounds__(kRedThreads)
-k_kinetic(int N, int ndf, const double* __restrict__ mass, const double* __restrict__ v,
-          double* __restrict__ md) {
-    __shared__ double s_w[16];
-    double xx = 0.0, yy = 0.0, zz = 0.0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+// Momentum / kinetic-energy partial sums per block of 256 atoms:
+// [sum m vx, m vy, m vz, m, m vx^2, m vy^2, m vz^2] over atoms with m > 0
+// (kinetic_energy thermostat.c:34-63, remove_comm thermostat.c:69-108)
+__global__ void __launch_bounds__(256)
+k_mom_partial(int N, const double* __restrict__ mass, const double* __restrict__ v,
+              double* __restrict__ partial) {
+    __shared__ double s_red[4][7];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < N) {
         double m = mass[i];
         if (m > 0.0) {
-            xx += m * v[i] * v[i];
-            yy += m * v[N + i] * v[N + i];
-            zz += m * v[2 * N + i] * v[2 * N + i];
+            double vx = v[i], vy = v[N + i], vz = v[2 * N + i];
+            a[0] = vx * m; a[1] = vy * m; a[2] = vz * m; a[3] = m;
+            a[4] = m * vx * vx; a[5] = m * vy * vy; a[6] = m * vz * vz;
         }
     }
-    xx = block_sum1024(xx, s_w);
-    yy = block_sum1024(yy, s_w);
-    zz = block_sum1024(zz, s_w);
-    if (threadIdx.x == 0) {
-        double ek = xx + yy + zz;
-        md[0] = ek;
-        md[1] = ek / (K_BOLTZ * ndf);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        double sm = wave_sum(a[c]);
+        if (lane == 0) s_red[wave][c] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double sm = 0.0;
+        if (threadIdx.x < 7)
+            for (int q = 0; q < 4; ++q) sm += s_red[q][threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = sm;
     }
 }
 
-// remove_comm (thermostat.c:69-108)
-__global__ void __launch_bounds__(kRedThreads)
-k_com(int N, const double* __restrict__ mass, double* __restrict__ v) {
-    __shared__ double s_w[16];
-    double px = 0.0, py = 0.0, pz = 0.0, ms = 0.0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        double m = mass[i];
-        if (m > 0.0) { px += v[i] * m; py += v[N + i] * m; pz += v[2 * N + i] * m; ms += m; }
-    }
-    px = block_sum1024(px, s_w);
-    py = block_sum1024(py, s_w);
-    pz = block_sum1024(pz, s_w);
-    ms = block_sum1024(ms, s_w);
-    px /= ms; py /= ms; pz /= ms;
-    for (int i = threadIdx.x; i < N; i += blockDim.x)
-        if (mass[i] > 0.0) { v[i] -= px; v[N + i] -= py; v[2 * N + i] -= pz; }
+// eKsum and the instantaneous temperature from the reduced sums
+__global__ void k_ke_finish(const double* __restrict__ sums, int ndf, double* __restrict__ md) {
+    if (threadIdx.x != 0) return;
+    double ek = sums[4] + sums[5] + sums[6];
+    md[0] = ek;
+    md[1] = ek / (K_BOLTZ * ndf);
 }
 
-// nose_hoover_chain (thermostat.c:117-191); the velocity scale goes to md[22]
+// subtract the centre-of-mass velocity from every atom with m > 0
+__global__ void k_com_apply(int N, const double* __restrict__ mass, const double* __restrict__ sums,
+                            double* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !(mass[i] > 0.0)) return;
+    double ms = sums[3];
+    v[i] -= sums[0] / ms;
+    v[N + i] -= sums[1] / ms;
+    v[2 * N + i] -= sums[2] / ms;
+}
+
+// nose_hoover_chain (thermostat.c:117-191); the velocity scale goes to md[22].
+// The chain lives in registers (all indices are compile-time after unrolling).
 __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int Nc = 5, Nys = 5, M = CAPAC_NHC;
+    constexpr int Nc = 5, Nys = 5, M = CAPAC_NHC;
     const double w1 = 0.41449077179437571194, w3 = -0.65796308717750284778;
-    const double w[5] = {w1, w1, w3, w1, w1};
-    double* vQ = md + 2;
-    double* aQ = md + 12;
-    double qMass = md[25], eKsum = md[0];
-    double NkT = kT * ndf;
+    double vQ[M], aQ[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) { vQ[k] = md[2 + k]; aQ[k] = md[12 + k]; }
+    const double qMass = md[25], eKsum = md[0];
+    const double NkT = kT * ndf;
     double aa, scale = 1.0;
     aQ[0] = (eKsum - NkT) / qMass;
+#pragma unroll
     for (int im = 0; im < M - 1; ++im) aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) / qMass;
     for (int ic = 0; ic < Nc; ++ic)
+#pragma unroll
         for (int iys = 0; iys < Nys; ++iys) {
-            double wdt = w[iys] * dt;
+            const double wdt = (iys == 2 ? w3 : w1) * dt;
             vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
+#pragma unroll
             for (int im = 0; im < M - 1; ++im) {
                 aa = exp(-wdt / (8 * Nc) * vQ[M - (im + 1)]);
                 vQ[M - 1 - (im + 1)] = vQ[M - 1 - (im + 1)] * aa * aa + wdt / (4 * Nc) * aQ[M - 1 - (im + 1)] * aa;
@@ -221,6 +241,7 @@ __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf) {
             aa = exp(-wdt / (2 * Nc) * vQ[0]);
             scale *= aa;
             aQ[0] = (eKsum * scale * scale - NkT) / qMass;
+#pragma unroll
             for (int im = 0; im < M - 1; ++im) {
                 aa = exp(-wdt / (8 * Nc) * vQ[im + 1]);
                 vQ[im] = vQ[im] * aa * aa + wdt / (4 * Nc) * aQ[im] * aa;
@@ -228,6 +249,8 @@ __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf) {
             }
             vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
         }
+#pragma unroll
+    for (int k = 0; k < M; ++k) { md[2 + k] = vQ[k]; md[12 + k] = aQ[k]; }
     md[22] = scale;
     md[0] = eKsum * (scale * scale);
     md[1] *= scale * scale;
@@ -343,8 +366,18 @@ int md_rattle2(capac_ctx* c) {
     return reduce_partials(c, nblk, 9, 64);
 }
 
+// red[96..103) = momentum / kinetic sums
+static int mom_sums(capac_ctx* c) {
+    int nblk = grid(c->bc.N, 256);
+    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, c->partial);
+    CHECK_HIP(hipGetLastError());
+    return reduce_partials(c, nblk, 7, 96);
+}
+
 int md_kinetic(capac_ctx* c) {
-    k_kinetic<<<1, kRedThreads, 0, c->stream>>>(c->bc.N, c->ndf, c->mass, c->v, c->md);
+    int rc = mom_sums(c);
+    if (rc) return rc;
+    k_ke_finish<<<1, 64, 0, c->stream>>>(c->red + 96, c->ndf, c->md);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -357,7 +390,9 @@ int md_nhc(capac_ctx* c, double kT) {
 }
 
 int md_remove_com(capac_ctx* c) {
-    k_com<<<1, kRedThreads, 0, c->stream>>>(c->bc.N, c->mass, c->v);
+    int rc = mom_sums(c);
+    if (rc) return rc;
+    k_com_apply<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->mass, c->red + 96, c->v);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

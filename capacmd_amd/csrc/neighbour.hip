@@ -2,52 +2,101 @@
 //
 // The reference has no neighbour list: every pair loop is a brute-force scan
 // followed by `rij2 < rCut2` on pbc_12(r_j - r_i) (force.c:1440-1473,
-// 401-414; cpff.c:94-120, 581-596).  Here candidates come from a cell grid of
-// width >= rc and are then put through that same FP64 test, evaluated with
-// the same operations in the same order (no contraction), so the accepted set
-// equals the brute-force set exactly: the single-shift distance is never
-// shorter than the minimum-image distance, so every accepted pair lies in
-// neighbouring cells.
+// 401-414; cpff.c:94-120, 581-596).  Here candidates come from a cell grid
+// and are then put through that same FP64 test, evaluated with the same
+// operations in the same order (no contraction), so the accepted set equals
+// the brute-force set exactly: the single-shift distance is never shorter
+// than the minimum-image distance, so every accepted pair lies within the
+// stencil.
 //
-// Rows: for atom i, non-metal neighbours fill row i from the front, metal
-// neighbours from the back, so metal-only consumers (Sutton-Chen, the water
-// side of the CPIM force) read a short tail.  The row order is deterministic:
-// stencil cells in fixed order, atoms in index order within a cell.
+// Verlet skin: rows hold the pairs within rl = rc + skin (the same exact test
+// with rl^2), and every force kernel applies the reference's r^2 < rc^2
+// itself, so the pairs used are exactly the brute-force set as long as no
+// atom has moved more than skin/2 since the build.  With box >= 2 rl the
+// single-shift image is then the same at build time and now for every pair
+// that can come within rc.  The rebuild decision is made on the device (the
+// largest raw displacement against the positions of the last build); the
+// build kernels return at once when no rebuild is due, so no host round trip
+// is added.  Smaller boxes use no skin (a build at every force call).
+//
+// Grid: cells of width >= rl (1 + 1e-9) / 2 and a +-2 stencil, so rounding in
+// the binning cannot separate an in-range pair by more than two cells; along
+// an axis with fewer than 5 cells every cell is visited once.
+//
+// Build: one workgroup per home cell; the stencil's atoms are staged through
+// LDS once and shared by every atom of the cell; each wavefront tests 64
+// candidates per pass and compacts the accepted ones with a ballot, so each
+// row is written in contiguous runs.
+// Rows: non-metal neighbours fill row i from the front, metal neighbours from
+// the back, so metal-only consumers (Sutton-Chen, the water side of the CPIM
+// force) read a short tail.  The order is deterministic: stencil cells in a
+// fixed order, atoms by index within a cell.
 #include "capac_internal.h"
 
 namespace {
 
-__global__ void k_cell_assign(const double* __restrict__ r, BoxConst bc, int n0, int n1, int n2,
-                              int* __restrict__ cell_of, int* __restrict__ cell_count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= bc.N) return;
+constexpr int kHalf = 2;                 // stencil half-width in cells
+constexpr int kHistLds = 4096;           // cells handled by the LDS histogram
+
+__device__ __forceinline__ int cell_of_pos(const BoxConst& bc, double x, double y, double z,
+                                           int n0, int n1, int n2) {
     int n[3] = {n0, n1, n2};
+    double p[3] = {x, y, z};
     int c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double x = r[d * bc.N + i];
         double L = bc.box[d];
-        double xw = x - L * floor(x / L);
+        double xw = p[d] - L * floor(p[d] / L);
         double t = xw * n[d] / L;
         // clamp before the conversion (also maps a non-finite coordinate to 0)
         if (!(t >= 0.0)) t = 0.0;
         if (t > (double)(n[d] - 1)) t = (double)(n[d] - 1);
         c[d] = (int)t;
     }
-    int cell = (c[0] * n1 + c[1]) * n2 + c[2];
-    cell_of[i] = cell;
-    atomicAdd(&cell_count[cell], 1);
+    return (c[0] * n1 + c[1]) * n2 + c[2];
 }
 
-// exclusive scan of cell counts; single workgroup, sequential per thread chunk
+// cell of every atom; counts through a per-block LDS histogram so the global
+// counters see one atomic per (block, occupied cell)
+__global__ void __launch_bounds__(256)
+k_cell_assign(const double* __restrict__ r, BoxConst bc, int n0, int n1, int n2, int n_cell,
+              int* __restrict__ cell_of, int* __restrict__ rank_in_block,
+              int* __restrict__ cell_count, const unsigned long long* __restrict__ need) {
+    __shared__ int s_hist[kHistLds];
+    if (!need[1]) return;
+    const int N = bc.N;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool lds = n_cell <= kHistLds;
+    if (lds) {
+        for (int k = threadIdx.x; k < n_cell; k += blockDim.x) s_hist[k] = 0;
+        __syncthreads();
+    }
+    int cell = -1;
+    if (i < N) {
+        cell = cell_of_pos(bc, r[i], r[N + i], r[2 * N + i], n0, n1, n2);
+        cell_of[i] = cell;
+        if (lds) rank_in_block[i] = atomicAdd(&s_hist[cell], 1);
+        else rank_in_block[i] = -1;
+    }
+    if (lds) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < n_cell; k += blockDim.x)
+            if (s_hist[k]) atomicAdd(&cell_count[k], s_hist[k]);
+    } else if (i < N) {
+        atomicAdd(&cell_count[cell], 1);
+    }
+}
+
+// exclusive scan of cell counts; single workgroup
 __global__ void k_cell_scan(const int* __restrict__ count, int n_cell, int* __restrict__ start,
-                            int* __restrict__ fill, int* __restrict__ max_cell) {
+                            int* __restrict__ fill, const unsigned long long* __restrict__ need) {
     __shared__ int s_part[1024];
+    if (!need[1]) return;
     int t = threadIdx.x, nt = blockDim.x;
     int per = (n_cell + nt - 1) / nt;
     int lo = t * per, hi = min(lo + per, n_cell);
-    int sum = 0, mx = 0;
-    for (int k = lo; k < hi; ++k) { sum += count[k]; mx = max(mx, count[k]); }
+    int sum = 0;
+    for (int k = lo; k < hi; ++k) sum += count[k];
     s_part[t] = sum;
     __syncthreads();
     if (t == 0) {
@@ -58,23 +107,45 @@ __global__ void k_cell_scan(const int* __restrict__ count, int n_cell, int* __re
     __syncthreads();
     int run = s_part[t];
     for (int k = lo; k < hi; ++k) { start[k] = run; fill[k] = run; run += count[k]; }
-    atomicMax(max_cell, mx);
 }
 
-__global__ void k_cell_fill(const int* __restrict__ cell_of, int N, int* __restrict__ fill,
-                            int* __restrict__ cell_atoms) {
+// slots: one global atomic per (block, occupied cell) reserves a range, the
+// block's atoms take consecutive slots in it by their LDS rank
+__global__ void __launch_bounds__(256)
+k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ rank_in_block, int N,
+            int n_cell, int* __restrict__ fill, int* __restrict__ cell_tmp,
+            const unsigned long long* __restrict__ need) {
+    __shared__ int s_hist[kHistLds];
+    __shared__ int s_base[kHistLds];
+    if (!need[1]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int slot = atomicAdd(&fill[cell_of[i]], 1);
-    cell_atoms[slot] = i;
+    if (n_cell <= kHistLds) {
+        for (int k = threadIdx.x; k < n_cell; k += blockDim.x) s_hist[k] = 0;
+        __syncthreads();
+        int cell = i < N ? cell_of[i] : -1;
+        if (cell >= 0) atomicAdd(&s_hist[cell], 1);
+        __syncthreads();
+        for (int k = threadIdx.x; k < n_cell; k += blockDim.x)
+            if (s_hist[k]) s_base[k] = atomicAdd(&fill[k], s_hist[k]);
+        __syncthreads();
+        if (cell >= 0) cell_tmp[s_base[cell] + rank_in_block[i]] = i;
+    } else if (i < N) {
+        int slot = atomicAdd(&fill[cell_of[i]], 1);
+        cell_tmp[slot] = i;
+    }
 }
 
-// order each cell's atoms by index: rank = number of smaller ids in the cell.
-// Cells that fit SORT_LDS ids are ranked from LDS, larger ones from global.
+// order each cell's atoms by index (rank = number of smaller ids in the cell)
+// and gather the cell-sorted SoA copies used by the candidate scan
 #define SORT_LDS 4096
 __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict__ unsorted,
-                            int* __restrict__ cell_atoms) {
+                            const double* __restrict__ r, const int* __restrict__ mol_id,
+                            const int* __restrict__ local_idx, int N, int* __restrict__ cell_atoms,
+                            double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
+                            int* __restrict__ smol, int* __restrict__ sloc,
+                            const unsigned long long* __restrict__ need) {
     __shared__ int s_ids[SORT_LDS];
+    if (!need[1]) return;
     int c = blockIdx.x;
     int lo = start[c], n = start[c + 1] - lo;
     const int* src = unsorted + lo;
@@ -86,88 +157,200 @@ __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         int v = src[k], rank = 0;
         for (int m = 0; m < n; ++m) rank += src[m] < v;
-        cell_atoms[lo + rank] = v;
+        int slot = lo + rank;
+        cell_atoms[slot] = v;
+        sx[slot] = r[v];
+        sy[slot] = r[N + v];
+        sz[slot] = r[2 * N + v];
+        smol[slot] = mol_id[v];
+        sloc[slot] = local_idx[v];
     }
 }
 
 struct Stencil {
     int n;
-    int off[27][3];
+    short off[125][3];
 };
 
-// One workgroup per cell; each thread owns one atom i of the cell and scans
-// the stencil cells, staged through LDS in chunks of blockDim atoms.
+// One workgroup per home cell.  The stencil cells' atoms are staged through
+// LDS as one flat, densely packed candidate array (up to kCap at a time, in
+// stencil order), so every atom of the home cell reuses each staged
+// candidate; each wavefront owns a set of home atoms and tests 64 staged
+// candidates per pass, compacting the accepted ones with a ballot.
+constexpr int kCap = 1024;        // staged candidates per pass
+constexpr int kMaxHome = 512;     // home atoms per pass
+
 __global__ void __launch_bounds__(256)
-k_nb_build(const double* __restrict__ r, const int* __restrict__ mol_id,
-           const int* __restrict__ local_idx, const unsigned* __restrict__ excl,
-           const int* __restrict__ cell_start, const int* __restrict__ cell_atoms,
-           BoxConst bc, int n0, int n1, int n2, Stencil st, int stride,
+k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
+           const double* __restrict__ sz, const int* __restrict__ smol,
+           const int* __restrict__ sloc, const int* __restrict__ cell_atoms,
+           const int* __restrict__ cell_start, const unsigned* __restrict__ excl, BoxConst bc,
+           int n0, int n1, int n2, const Stencil* __restrict__ st, int stride,
            int* __restrict__ n_front, int* __restrict__ n_back, int* __restrict__ nbr,
-           int* __restrict__ overflow) {
-    __shared__ double s_x[256], s_y[256], s_z[256];
-    __shared__ int s_id[256], s_mol[256], s_loc[256];
-    const int N = bc.N, Nm = bc.Nm;
-    int cell = blockIdx.x;
-    int cz = cell % n2, cy = (cell / n2) % n1, cx = cell / (n1 * n2);
-    int lo = cell_start[cell], cnt = cell_start[cell + 1] - lo;
-    for (int base = 0; base < cnt; base += blockDim.x) {
-        int li = base + threadIdx.x;
-        bool active = li < cnt;
-        int i = active ? cell_atoms[lo + li] : 0;
-        double xi = active ? r[i] : 0.0, yi = active ? r[N + i] : 0.0, zi = active ? r[2 * N + i] : 0.0;
-        int moli = active ? mol_id[i] : -1;
-        unsigned exi = active ? excl[i] : 0u;
-        int nf = 0, nb = 0;
-        int* row = nbr + (size_t)i * stride;
-        for (int s = 0; s < st.n; ++s) {
-            int ox = cx + st.off[s][0], oy = cy + st.off[s][1], oz = cz + st.off[s][2];
-            ox = (ox + n0) % n0; oy = (oy + n1) % n1; oz = (oz + n2) % n2;
-            int oc = (ox * n1 + oy) * n2 + oz;
-            int olo = cell_start[oc], ocnt = cell_start[oc + 1] - olo;
-            for (int cb = 0; cb < ocnt; cb += 256) {
-                __syncthreads();
-                int k = cb + threadIdx.x;
-                if (k < ocnt) {
-                    int j = cell_atoms[olo + k];
-                    s_id[threadIdx.x] = j;
-                    s_x[threadIdx.x] = r[j];
-                    s_y[threadIdx.x] = r[N + j];
-                    s_z[threadIdx.x] = r[2 * N + j];
-                    s_mol[threadIdx.x] = mol_id[j];
-                    s_loc[threadIdx.x] = local_idx[j];
+           int* __restrict__ overflow, const unsigned long long* __restrict__ need) {
+    __shared__ double c_x[kCap], c_y[kCap], c_z[kCap];
+    if (!need[1]) return;
+    __shared__ int c_id[kCap], c_mol[kCap], c_loc[kCap];
+    __shared__ int h_nf[kMaxHome], h_nb[kMaxHome];
+    __shared__ int seg_lo[125], seg_off[126];
+    const int Nm = bc.Nm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int cell = blockIdx.x;
+    const int cz = cell % n2, cy = (cell / n2) % n1, cx = cell / (n1 * n2);
+    const int hlo = cell_start[cell], hhi = cell_start[cell + 1];
+    if (hlo == hhi) return;
+    const int ns = st->n;
+    // stencil cells as segments of one flat candidate index space, in order
+    if (threadIdx.x < ns) {
+        int t = threadIdx.x;
+        int ox = cx + st->off[t][0], oy = cy + st->off[t][1], oz = cz + st->off[t][2];
+        ox = (ox % n0 + n0) % n0; oy = (oy % n1 + n1) % n1; oz = (oz % n2 + n2) % n2;
+        const int oc = (ox * n1 + oy) * n2 + oz;
+        seg_lo[t] = cell_start[oc];
+        seg_off[t + 1] = cell_start[oc + 1] - cell_start[oc];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        seg_off[0] = 0;
+        for (int t = 0; t < ns; ++t) seg_off[t + 1] += seg_off[t];
+    }
+    __syncthreads();
+    const int total_all = seg_off[ns];
+    for (int hbase = hlo; hbase < hhi; hbase += kMaxHome) {
+        const int nh = min(kMaxHome, hhi - hbase);
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) { h_nf[h] = 0; h_nb[h] = 0; }
+        for (int w0 = 0; w0 < total_all; w0 += kCap) {
+            const int total = min(kCap, total_all - w0);
+            __syncthreads();
+            for (int f = threadIdx.x; f < total; f += blockDim.x) {
+                const int g = w0 + f;
+                int lo = 0, hi = ns - 1;             // segment holding flat index g
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (seg_off[mid] <= g) lo = mid; else hi = mid - 1;
                 }
-                __syncthreads();
-                if (!active) continue;
-                int m = min(256, ocnt - cb);
-                for (int q = 0; q < m; ++q) {
-                    int j = s_id[q];
-                    if (j == i) continue;
-                    if (s_mol[q] == moli && ((exi >> s_loc[q]) & 1u)) continue;
-                    double dx = s_x[q] - xi, dy = s_y[q] - yi, dz = s_z[q] - zi;
-                    pbc12(bc, dx, dy, dz);
-                    double r2 = dist2_exact(dx, dy, dz);
-                    if (!(r2 < bc.rc2)) continue;
-                    if (nf + nb >= stride) { overflow[0] = 1; continue; }
-                    if (j < Nm) { ++nb; row[stride - nb] = j; }
-                    else { row[nf] = j; ++nf; }
+                const int k = seg_lo[lo] + (g - seg_off[lo]);
+                c_x[f] = sx[k]; c_y[f] = sy[k]; c_z[f] = sz[k];
+                c_id[f] = cell_atoms[k]; c_mol[f] = smol[k]; c_loc[f] = sloc[k];
+            }
+            __syncthreads();
+            for (int h = wave; h < nh; h += 4) {
+                const int hs = hbase + h;
+                const int i = cell_atoms[hs];
+                const double xi = sx[hs], yi = sy[hs], zi = sz[hs];
+                const int moli = smol[hs];
+                const unsigned exi = excl[i];
+                int* row = nbr + (size_t)i * stride;
+                int nf = h_nf[h], nb = h_nb[h];
+                if (nf < 0) continue;                 // this row already overflowed
+                for (int base = 0; base < total; base += 64) {
+                    const int q = base + lane;
+                    bool acc = false;
+                    int j = -1;
+                    if (q < total) {
+                        j = c_id[q];
+                        bool excluded = (c_mol[q] == moli) && ((exi >> c_loc[q]) & 1u);
+                        if (j != i && !excluded) {
+                            double dx = c_x[q] - xi, dy = c_y[q] - yi, dz = c_z[q] - zi;
+                            pbc12(bc, dx, dy, dz);
+                            double r2 = dist2_exact(dx, dy, dz);
+                            acc = r2 < bc.rl2;
+                        }
+                    }
+                    const unsigned long long mf = __ballot(acc && j >= Nm);
+                    const unsigned long long mb = __ballot(acc && j < Nm);
+                    const int cf = __popcll(mf), cb = __popcll(mb);
+                    if (nf + nb + cf + cb > stride) {
+                        if (lane == 0) overflow[0] = 1;
+                        nf = -1;
+                        break;
+                    }
+                    if (acc) {
+                        if (j >= Nm) row[nf + __popcll(mf & below)] = j;
+                        else row[stride - 1 - (nb + __popcll(mb & below))] = j;
+                    }
+                    nf += cf;
+                    nb += cb;
                 }
+                if (lane == 0) { h_nf[h] = nf; h_nb[h] = nb; }
             }
         }
-        if (active) { n_front[i] = nf; n_back[i] = nb; }
+        __syncthreads();
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const int i = cell_atoms[hbase + h];
+            n_front[i] = h_nf[h] < 0 ? 0 : h_nf[h];
+            n_back[i] = h_nf[h] < 0 ? 0 : h_nb[h];
+        }
+        __syncthreads();
     }
+}
+
+// largest squared raw displacement since the last build (atomicMax on the
+// bit pattern: non-negative doubles order like their bits); decides the rebuild
+__global__ void __launch_bounds__(256)
+k_disp(const double* __restrict__ r, const double* __restrict__ r_build, int N,
+       unsigned long long* __restrict__ ctl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < N) {
+        double dx = r[i] - r_build[i], dy = r[N + i] - r_build[N + i], dz = r[2 * N + i] - r_build[2 * N + i];
+        d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 == d2)) d2 = 1e300;           // non-finite positions force a rebuild
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&ctl[0], (unsigned long long)__double_as_longlong(d2));
+}
+
+// rebuild flag from the displacement; when set, also clears the counters the
+// build accumulates into
+__global__ void k_decide(unsigned long long* __restrict__ ctl, double limit2, int* __restrict__ cell_count,
+                         int n_cell, int* __restrict__ overflow) {
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        double d2 = __longlong_as_double((long long)ctl[0]);
+        int go = ctl[1] != 0ull || d2 > limit2;
+        if (go) ctl[1] = 1ull;
+        ctl[0] = 0ull;
+        s_go = go;
+    }
+    __syncthreads();
+    if (!s_go) return;
+    for (int k = threadIdx.x; k <= n_cell; k += blockDim.x) cell_count[k] = 0;
+    if (threadIdx.x < 2) overflow[threadIdx.x] = 0;
+}
+
+// after a build: remember the positions and clear the flag
+__global__ void __launch_bounds__(256)
+k_mark_built(const double* __restrict__ r, double* __restrict__ r_build, int n3,
+             unsigned long long* __restrict__ ctl) {
+    if (!ctl[1]) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3) r_build[k] = r[k];
+}
+
+__global__ void k_clear_flag(unsigned long long* __restrict__ ctl) {
+    if (threadIdx.x == 0 && ctl[1]) { ctl[1] = 0ull; ctl[2] += 1ull; }
+}
+
+__global__ void k_set_flag(unsigned long long* __restrict__ ctl) {
+    if (threadIdx.x == 0) { ctl[0] = 0ull; ctl[1] = 1ull; }
 }
 
 }  // namespace
 
-// Cell grid: width >= rc (1 + 1e-9) so that rounding in the binning can never
-// separate an in-range pair by more than one cell; along an axis with fewer
-// than 3 cells every cell is a neighbour and duplicates are dropped.
 static void make_stencil(const int n[3], Stencil* st) {
-    int offs[3][3], cnt[3];
+    short offs[3][5];
+    int cnt[3];
     for (int d = 0; d < 3; ++d) {
-        if (n[d] >= 3) { offs[d][0] = -1; offs[d][1] = 0; offs[d][2] = 1; cnt[d] = 3; }
-        else if (n[d] == 2) { offs[d][0] = 0; offs[d][1] = 1; cnt[d] = 2; }
-        else { offs[d][0] = 0; cnt[d] = 1; }
+        if (n[d] >= 2 * kHalf + 1) {
+            cnt[d] = 2 * kHalf + 1;
+            for (int k = 0; k < cnt[d]; ++k) offs[d][k] = (short)(k - kHalf);
+        } else {
+            cnt[d] = n[d];                       // every cell once
+            for (int k = 0; k < cnt[d]; ++k) offs[d][k] = (short)k;
+        }
     }
     st->n = 0;
     for (int a = 0; a < cnt[0]; ++a)
@@ -180,34 +363,77 @@ static void make_stencil(const int n[3], Stencil* st) {
             }
 }
 
+// list radius: rc + skin where every box edge is at least 2 (rc + skin), else rc
+int nb_set_radius(capac_ctx* c, double skin) {
+    double rl = c->bc.rc + skin;
+    for (int d = 0; d < 3; ++d)
+        if (c->bc.box[d] < 2.0 * rl) { skin = 0.0; rl = c->bc.rc; }
+    c->skin = skin;
+    c->bc.rl2 = skin > 0.0 ? rl * rl : c->bc.rc2;
+    return nb_grid_setup(c);
+}
+
 int nb_grid_setup(capac_ctx* c) {
+    const double rl = c->bc.rc + c->skin;
     for (int d = 0; d < 3; ++d) {
-        int n = (int)floor(c->bc.box[d] / (c->bc.rc * (1.0 + 1e-9)));
+        int n = (int)floor(kHalf * c->bc.box[d] / (rl * (1.0 + 1e-9)));
         c->ncell[d] = n < 1 ? 1 : n;
     }
     c->n_cell = c->ncell[0] * c->ncell[1] * c->ncell[2];
+    Stencil st;
+    make_stencil(c->ncell, &st);
+    if (!c->stencil) {
+        if (hipMalloc(&c->stencil, sizeof(Stencil)) != hipSuccess) {
+            capac_set_error("stencil allocation failed");
+            return CAPAC_ERR_HIP;
+        }
+    }
+    if (hipMemcpy(c->stencil, &st, sizeof(Stencil), hipMemcpyHostToDevice) != hipSuccess) {
+        capac_set_error("stencil upload failed");
+        return CAPAC_ERR_HIP;
+    }
     return 0;
 }
 
-int nb_build(capac_ctx* c) {
+static int nb_launch(capac_ctx* c) {
     const int N = c->bc.N;
     hipStream_t s = c->stream;
-    int rc = fill_zero_i(c, c->cell_count, c->n_cell + 1);
-    if (!rc) rc = fill_zero_i(c, c->nb_overflow, 2);
-    if (rc) return rc;
+    const unsigned long long* need = c->nb_ctl;
     int tb = 256, gb = (N + tb - 1) / tb;
-    k_cell_assign<<<gb, tb, 0, s>>>(c->r, c->bc, c->ncell[0], c->ncell[1], c->ncell[2], c->cell_of,
-                                    c->cell_count);
-    k_cell_scan<<<1, 1024, 0, s>>>(c->cell_count, c->n_cell, c->cell_start, c->cell_fill,
-                                   c->nb_overflow + 1);
-    k_cell_fill<<<gb, tb, 0, s>>>(c->cell_of, N, c->cell_fill, c->cell_tmp);
-    k_cell_sort<<<c->n_cell, 256, 0, s>>>(c->cell_start, c->cell_tmp, c->cell_atoms);
-    Stencil st;
-    make_stencil(c->ncell, &st);
-    k_nb_build<<<c->n_cell, 256, 0, s>>>(c->r, c->mol_id, c->local_idx, c->excl, c->cell_start,
-                                         c->cell_atoms, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
-                                         st, c->nb_stride, c->n_front, c->n_back, c->nbr,
-                                         c->nb_overflow);
+    k_cell_assign<<<gb, tb, 0, s>>>(c->r, c->bc, c->ncell[0], c->ncell[1], c->ncell[2], c->n_cell,
+                                    c->cell_of, c->cell_rank, c->cell_count, need);
+    k_cell_scan<<<1, 1024, 0, s>>>(c->cell_count, c->n_cell, c->cell_start, c->cell_fill, need);
+    k_cell_fill<<<gb, tb, 0, s>>>(c->cell_of, c->cell_rank, N, c->n_cell, c->cell_fill, c->cell_tmp, need);
+    k_cell_sort<<<c->n_cell, 256, 0, s>>>(c->cell_start, c->cell_tmp, c->r, c->mol_id, c->local_idx, N,
+                                          c->cell_atoms, c->sorted_r, c->sorted_r + N,
+                                          c->sorted_r + 2 * N, c->sorted_mol, c->sorted_loc, need);
+    k_nb_build<<<c->n_cell, 256, 0, s>>>(c->sorted_r, c->sorted_r + N, c->sorted_r + 2 * N,
+                                         c->sorted_mol, c->sorted_loc, c->cell_atoms, c->cell_start,
+                                         c->excl, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
+                                         (const Stencil*)c->stencil, c->nb_stride, c->n_front,
+                                         c->n_back, c->nbr, c->nb_overflow, need);
+    k_mark_built<<<(3 * N + 255) / 256, 256, 0, s>>>(c->r, c->r_build, 3 * N, c->nb_ctl);
+    k_clear_flag<<<1, 64, 0, s>>>(c->nb_ctl);
     CHECK_HIP(hipGetLastError());
     return 0;
+}
+
+// unconditional build
+int nb_build(capac_ctx* c) {
+    k_set_flag<<<1, 64, 0, c->stream>>>(c->nb_ctl);
+    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, 0.0, c->cell_count, c->n_cell, c->nb_overflow);
+    return nb_launch(c);
+}
+
+// build only when an atom moved more than skin/2 since the last build (always
+// without a skin)
+int nb_build_if_needed(capac_ctx* c) {
+    if (c->skin <= 0.0) return nb_build(c);
+    const int N = c->bc.N;
+    k_disp<<<(N + 255) / 256, 256, 0, c->stream>>>(c->r, c->r_build, N, c->nb_ctl);
+    // half the skin, less a relative 1e-6 so rounding in the distance tests
+    // cannot matter at the boundary
+    double half = 0.5 * c->skin * (1.0 - 1e-6);
+    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, half * half, c->cell_count, c->n_cell, c->nb_overflow);
+    return nb_launch(c);
 }

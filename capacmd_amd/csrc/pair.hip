@@ -99,6 +99,7 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
             double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < bc.rc2)) continue;     // rows reach rc + skin
             pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o);
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);

@@ -38,6 +38,7 @@ k_sc_dens(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_b
         double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
         pbc12(bc, dx, dy, dz);
         double r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < bc.rc2)) continue;         // rows reach rc + skin
         dens += ipow(bc.qsc_a / sqrt(r2), bc.qsc_m);
     }
     dens = wave_sum(dens);
@@ -65,6 +66,7 @@ k_sc_force(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_
             double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < bc.rc2)) continue;     // rows reach rc + skin
             double rij = sqrt(r2);
             double a_r = bc.qsc_a / rij;
             double a_n = ipow(a_r, bc.qsc_n);

@@ -11,11 +11,14 @@
 // preconditioner entries are 1.
 //
 // Per iteration: two fused SpMV+dot launches (one workgroup per metal
-// row-block, streaming its 16 x Nm block components once) and three
+// row-block, streaming its 16 x Nm block components once) and two
 // single-workgroup vector kernels that finish the dots in a fixed order and
-// update the vectors.  All scalars live on the device; a `done` flag turns
-// the remaining launches of a batch into no-ops, so the host synchronises
-// once per batch of iterations, not once per iteration.
+// update the vectors (the second also opens the next iteration).  All scalars
+// live on the device; a `done` flag turns the remaining launches of a batch
+// into no-ops, so the host synchronises once per batch, not per iteration.
+// Cross-workgroup hand-off stays at kernel boundaries: on this GPU the eight
+// XCDs have separate L2 caches, and an in-kernel device-scope fence costs
+// more than a launch.
 #include "capac_internal.h"
 
 #include <stdlib.h>
@@ -43,12 +46,6 @@ __device__ double block_sum(double v, double* s_w) {
     return t;
 }
 
-// reference dot over the whole vector (bicg_stab.c:121-178), fixed order
-__device__ double vdot(const double* a, const double* b, int n, double* s_w) {
-    double acc = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) acc += a[k] * b[k];
-    return block_sum(acc, s_w);
-}
 
 // y = A x for the 4 Nm relay rows, plus partial dots; blocks >= Nm compute
 // the Lagrange rows.  mode 0: no dots; 1: part[i] = (r0, y) rows of i;
@@ -117,60 +114,48 @@ k_spmv64(const double* __restrict__ relay, int ld, int Nm, int n_np, const int* 
     }
 }
 
+// start of BiCGSTAB (bicg_stab.c:294-404): bTb, criteria, r0 = r = b - Ax,
+// v = 0, K = 1/diag, rho = (r0, r) with the two early exits, and the first
+// direction p = r + rho (0 - 0) = r, y = K p
 __global__ void __launch_bounds__(kVecThreads)
 k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __restrict__ Ax,
             const double* __restrict__ diag, double* __restrict__ r0, double* __restrict__ r,
             double* __restrict__ p, double* __restrict__ v, double* __restrict__ K,
-            SolveScalars* sc) {
+            double* __restrict__ y, SolveScalars* sc) {
     __shared__ double s_w[16];
-    double bTb = vdot(b, b, n_mat, s_w);
+    double bb = 0.0, rr = 0.0;
     for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
-        double rk = b[k] - Ax[k];
+        double bk = b[k];
+        double rk = bk - Ax[k];
         r0[k] = rk;
         r[k] = rk;
-        p[k] = 0.0;
         v[k] = 0.0;
         K[k] = k < n4 ? 1.0 / diag[k] : 1.0;
+        bb += bk * bk;
+        rr += rk * rk;
     }
+    double bTb = block_sum(bb, s_w);
+    double rho = block_sum(rr, s_w);
+    const bool zero_rhs = bTb < 1.0e-10;
+    const bool zero_rho = !zero_rhs && rho < 1.0e-10;
+    if (!zero_rhs && !zero_rho)
+        for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
+            double pk = r[k];
+            p[k] = pk;
+            y[k] = pk * K[k];
+        }
     if (threadIdx.x == 0) {
         sc->bTb = bTb;
         sc->criteria = bTb * 1.0e-08;
         sc->rho_old = 1.0;
         sc->alpha = 1.0;
         sc->omega = 1.0;
-        sc->rho = 0.0;
+        sc->rho = rho;
         sc->rTr = 0.0;
         sc->iter = 0;
-        sc->done = bTb < 1.0e-10 ? 1 : 0;
-        sc->kind = bTb < 1.0e-10 ? CAPAC_SOLVE_ZERO_RHS : CAPAC_SOLVE_CONVERGED;
+        sc->done = (zero_rhs || zero_rho) ? 1 : 0;
+        sc->kind = zero_rhs ? CAPAC_SOLVE_ZERO_RHS : (zero_rho ? CAPAC_SOLVE_ZERO_RHO : CAPAC_SOLVE_CONVERGED);
     }
-}
-
-// rho = (r0, r); beta; p = r + beta (p - omega v); y = K p
-__global__ void __launch_bounds__(kVecThreads)
-k_bicg_p(int n_mat, const double* __restrict__ r0, const double* __restrict__ r,
-         double* __restrict__ p, const double* __restrict__ v, const double* __restrict__ K,
-         double* __restrict__ y, SolveScalars* sc) {
-    __shared__ double s_w[16];
-    __shared__ int s_skip;
-    if (threadIdx.x == 0) s_skip = sc->done;
-    __syncthreads();
-    if (s_skip) return;
-    double rho = vdot(r0, r, n_mat, s_w);
-    int iter = sc->iter;
-    if (rho < 1.0e-10 && iter == 0) {
-        if (threadIdx.x == 0) { sc->done = 1; sc->kind = CAPAC_SOLVE_ZERO_RHO; sc->rho = rho; }
-        return;
-    }
-    double beta = (rho / sc->rho_old) * (sc->alpha / sc->omega);
-    double omega = sc->omega;
-    for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
-        double pk = r[k] + beta * (p[k] - omega * v[k]);
-        p[k] = pk;
-        y[k] = pk * K[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) sc->rho = rho;
 }
 
 // alpha = rho / (r0, v); s = r - alpha v; z = K s
@@ -196,11 +181,16 @@ k_bicg_s(int n_mat, int n_part, const double* __restrict__ part, const double* _
     if (threadIdx.x == 0) sc->alpha = alpha;
 }
 
-// omega = (Kt, z) / (Kt, Kt); x += alpha y + omega z; r = s - omega t; rTr
+// omega = (Kt, z) / (Kt, Kt); x += alpha y + omega z; r = s - omega t; rTr and
+// the stop tests (bicg_stab.c:483-571); when not done, the next iteration's
+// rho = (r0, r), beta = (rho/rho_old)(alpha/omega), p = r + beta (p - omega v)
+// and y = K p in the same pass (bicg_stab.c:365-404)
 __global__ void __launch_bounds__(kVecThreads)
-k_bicg_x(int n_mat, int n_part, const double* __restrict__ part, double* __restrict__ x,
-         const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ s,
-         const double* __restrict__ t, double* __restrict__ r, SolveScalars* sc) {
+k_bicg_xp(int n_mat, int n_part, const double* __restrict__ part, double* __restrict__ x,
+          double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ s,
+          const double* __restrict__ t, double* __restrict__ r, const double* __restrict__ r0,
+          double* __restrict__ p, const double* __restrict__ v, const double* __restrict__ K,
+          SolveScalars* sc) {
     __shared__ double s_w[16];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
@@ -212,27 +202,42 @@ k_bicg_x(int n_mat, int n_part, const double* __restrict__ part, double* __restr
     double ktkt = block_sum(a1, s_w);
     double omega = ktks / ktkt;
     double alpha = sc->alpha;
-    double acc = 0.0;
+    double rho = sc->rho;
+    double acc = 0.0, acc_r0 = 0.0;
     for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
         x[k] += alpha * y[k] + omega * z[k];
         double rk = s[k] - omega * t[k];
         r[k] = rk;
         acc += rk * rk;
+        acc_r0 += r0[k] * rk;
     }
     double rTr = block_sum(acc, s_w);
+    double rho_next = block_sum(acc_r0, s_w);
+    const int iter = sc->iter + 1;
+    int done = 0, kind = CAPAC_SOLVE_CONVERGED;
+    if (rTr < sc->criteria) {
+        done = 1;
+    } else if (iter > 1000) {
+        done = 1;
+        kind = rTr < sc->criteria * 100.0 ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
+    }
+    if (!done) {
+        double beta = (rho_next / rho) * (alpha / omega);
+        for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
+            double pk = r[k] + beta * (p[k] - omega * v[k]);
+            p[k] = pk;
+            y[k] = pk * K[k];
+        }
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         sc->omega = omega;
-        sc->rho_old = sc->rho;
-        int iter = sc->iter + 1;
+        sc->rho_old = rho;
+        sc->rho = rho_next;
         sc->iter = iter;
         sc->rTr = rTr;
-        if (rTr < sc->criteria) {
-            sc->done = 1;
-            sc->kind = CAPAC_SOLVE_CONVERGED;
-        } else if (iter > 1000) {
-            sc->done = 1;
-            sc->kind = rTr < sc->criteria * 100.0 ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
-        }
+        sc->done = done;
+        sc->kind = kind;
     }
 }
 
@@ -286,17 +291,19 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
     hipStream_t st = c->stream;
     spmv(c, x, S->Ax, 0, NULL);
     k_bicg_init<<<1, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
-                                           S->sc);
-    int batch = 4, launched = 0;
+                                           S->y, S->sc);
+    // first batch: the previous solve's iteration count (warm starts change
+    // slowly), then small increments, so few launches are wasted as no-ops
+    const int first = c->solve_iters > 2 ? c->solve_iters : 4;
+    int batch = first, launched = 0;
     for (;;) {
         for (int it = 0; it < batch; ++it) {
-            k_bicg_p<<<1, kVecThreads, 0, st>>>(n, S->r0, S->r, S->p, S->v, S->K, S->y, S->sc);
             spmv(c, S->y, S->v, 1, S->sc);
             k_bicg_s<<<1, kVecThreads, 0, st>>>(n, n_part, S->part, S->r, S->v, S->K, S->s, S->z,
                                                 S->sc);
             spmv(c, S->z, S->t, 2, S->sc);
-            k_bicg_x<<<1, kVecThreads, 0, st>>>(n, n_part, S->part, x, S->y, S->z, S->s, S->t, S->r,
-                                                S->sc);
+            k_bicg_xp<<<1, kVecThreads, 0, st>>>(n, n_part, S->part, x, S->y, S->z, S->s, S->t, S->r,
+                                                 S->r0, S->p, S->v, S->K, S->sc);
         }
         launched += batch;
         CHECK_HIP(hipGetLastError());
@@ -307,7 +314,7 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
             capac_set_error("BiCGSTAB did not report completion after %d iterations", launched);
             return CAPAC_ERR_SOLVER;
         }
-        if (batch < 32) batch *= 2;
+        batch = (launched == first) ? 2 : (batch < 16 ? 2 * batch : 16);
     }
     c->solve_iters = S->sc_host->iter;
     c->solve_kind = S->sc_host->kind;

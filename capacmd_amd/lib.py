@@ -54,7 +54,7 @@ class CapacParams(C.Structure):
 
 class CapacTiming(C.Structure):
     _fields_ = [("t", C.c_double * NPOT), ("n_force_calls", C.c_longlong),
-                ("solve_iters", C.c_longlong)]
+                ("solve_iters", C.c_longlong), ("nb_builds", C.c_longlong)]
 
 
 def dptr(a: Optional[np.ndarray]):
@@ -164,10 +164,11 @@ GPU_EXPORTS = [
     "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_force",
     "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
-    "capac_time_kernel",
+    "capac_time_kernel", "capac_set_skin", "capac_skin",
 ]
 
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM = range(5)
+DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
 
 
 def gpu_lib() -> C.CDLL:
@@ -202,6 +203,9 @@ def gpu_lib() -> C.CDLL:
         lib.capac_reset_timing.argtypes = [C.c_void_p]
         lib.capac_enable_timing.argtypes = [C.c_void_p, C.c_int]
         lib.capac_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        lib.capac_set_skin.argtypes = [C.c_void_p, C.c_double]
+        lib.capac_skin.argtypes = [C.c_void_p]
+        lib.capac_skin.restype = C.c_double
         _gpu = lib
     return _gpu
 
@@ -258,6 +262,14 @@ class Context:
         f = np.empty(3 * self.N)
         _check(self.lib.capac_get_forces(self.h, dptr(f)), "capac_get_forces")
         return f
+
+    def set_skin(self, skin: float):
+        """Verlet skin of the neighbour rows in nm (0: rebuild every call)."""
+        _check(self.lib.capac_set_skin(self.h, float(skin)), "capac_set_skin")
+
+    @property
+    def skin(self) -> float:
+        return float(self.lib.capac_skin(self.h))
 
     # operators
     def force(self):
@@ -343,6 +355,12 @@ class Context:
         t = CapacTiming()
         _check(self.lib.capac_get_timing(self.h, C.byref(t)), "capac_get_timing")
         return np.array(t.t[:]), t.n_force_calls, t.solve_iters
+
+    def nb_builds(self) -> int:
+        """Neighbour-row builds since the last reset_timing."""
+        t = CapacTiming()
+        _check(self.lib.capac_get_timing(self.h, C.byref(t)), "capac_get_timing")
+        return int(t.nb_builds)
 
     def reset_timing(self):
         _check(self.lib.capac_reset_timing(self.h), "capac_reset_timing")

@@ -51,6 +51,9 @@ extern "C" {
 #define CAPAC_PREC_FP32_REFINED  1  /* FP32 matrix + FP32 inner solve in FP64 refinement */
 
 /* solver exit kinds (bicg_stab.c:300,368,539-571) */
+/* Verlet skin of the neighbour rows (nm), see capac_set_skin */
+#define CAPAC_DEFAULT_SKIN 0.1
+
 #define CAPAC_SOLVE_CONVERGED   0   /* rTr < 1e-8 bTb */
 #define CAPAC_SOLVE_LOOSENED    1   /* iter > 1000 and rTr < 1e-6 bTb */
 #define CAPAC_SOLVE_FAILED      2   /* iter > 1000 and rTr >= 1e-6 bTb */
@@ -126,6 +129,7 @@ typedef struct capac_timing {
     double t[CAPAC_NPOT];
     long long n_force_calls;
     long long solve_iters;      /* BiCGSTAB iterations, summed               */
+    long long nb_builds;        /* neighbour-row builds (see capac_set_skin) */
 } capac_timing;
 
 typedef struct capac_ctx capac_ctx;
@@ -146,6 +150,16 @@ int capac_get_state(capac_ctx* ctx, double* r, double* v, double* vec_pq);
 int capac_get_forces(capac_ctx* ctx, double* f);
 int capac_set_box(capac_ctx* ctx, const double box[3]);
 
+/* Verlet skin of the neighbour rows, nm (no counterpart in the reference,
+ * which scans all pairs at every call).  Rows hold the pairs within
+ * rc + skin and are rebuilt, on the device, once an atom has moved more than
+ * skin/2 since the last build; every force term still applies the exact
+ * r^2 < rc^2 test, so results do not depend on the skin.  0 rebuilds at every
+ * force call.  A box edge shorter than 2 (rc + skin) turns the skin off.
+ * capac_skin returns the skin in use. */
+int capac_set_skin(capac_ctx* ctx, double skin);
+double capac_skin(const capac_ctx* ctx);
+
 /* One force evaluation = mpi_force (force.c:2097-2320): zeroes, pair forces,
  * Sutton-Chen, CPIM right-hand side, relay assembly, BiCGSTAB (warm start
  * from the resident vec_pq), CPIM forces and the external field.  pot[0] is
@@ -156,7 +170,8 @@ int capac_force(capac_ctx* ctx, double pot[CAPAC_NPOT], double vir[9], double* f
 /* ---- sub-operators, each one kernel family, for parity tests ---- */
 
 /* Neighbour list (cell list + the reference's exact cut-off test):
- * per atom i, every j != i not excluded with |pbc_12(r_j - r_i)|^2 < rc^2.
+ * per atom i, every j != i not excluded with |pbc_12(r_j - r_i)|^2 < rl^2,
+ * rl = rc + capac_skin(ctx) (rc itself when the skin is 0), built now.
  * count[N]; list[N * max_per_atom] (j indices, unordered); returns the
  * largest row length, or < 0. */
 int capac_neighbours(capac_ctx* ctx, int* count, int* list, int max_per_atom);

@@ -54,8 +54,10 @@ def ctxs(decks):
         c.close()
 
 
-def brute_pairs(deck):
-    """All ordered (i, j) with the reference's exclusion and cut-off test."""
+def brute_pairs(deck, skin=0.0):
+    """All ordered (i, j) with the reference's exclusion and cut-off test,
+    at radius rc + skin (computed as the library does: rl = rc + skin,
+    rl^2 = rl * rl)."""
     p = deck.p
     N = deck.n_atoms
     r = deck.r0.reshape(3, N)
@@ -64,7 +66,8 @@ def brute_pairs(deck):
     mol = deck.array("mol_id", N, np.int64)
     loc = deck.array("local_idx", N, np.int64)
     exc = deck.array("excl_mask", N, np.int64)
-    rc2 = p.rcut * p.rcut
+    rl = p.rcut + skin
+    rc2 = rl * rl
     pairs = set()
     for i in range(N):
         d = r[:, :] - r[:, i:i + 1]          # r_j - r_i
@@ -85,16 +88,26 @@ def brute_pairs(deck):
     return pairs
 
 
+@pytest.mark.parametrize("skin", [None, 0.0])
 @pytest.mark.parametrize("name", DECKS)
-def test_neighbour_sets_exact(name, decks, ctxs):
+def test_neighbour_sets_exact(name, skin, decks, ctxs):
+    """Rows = the brute-force set at rc + skin: the default skin (turned off
+    by the library in a box shorter than 2 (rc + skin)) and none."""
     deck, ctx = decks[name], ctxs[name]
-    cnt, lst, m = ctx.neighbours(2048)
+    default = lib.DEFAULT_SKIN
+    ctx.set_skin(default if skin is None else skin)
+    used = ctx.skin
+    print(name, "skin in use", used, flush=True)
+    try:
+        cnt, lst, m = ctx.neighbours(2048)
+    finally:
+        ctx.set_skin(default)
     got = set()
     for i in range(deck.n_atoms):
         for j in lst[i, : cnt[i]]:
             got.add((i, int(j)))
     assert sum(cnt) == len(got), "duplicate neighbours"
-    want = brute_pairs(deck)
+    want = brute_pairs(deck, used)
     assert got == want
     # metal neighbours sit at the row tail
     Nm = deck.n_metal
@@ -279,3 +292,53 @@ def test_md_three_steps(name, decks, golden_dir):
         for k in range(1, 15):
             assert abs(obg["pot"][k] - obo["pot"][k]) <= 1e3 * tol_v * max(1.0, abs(obo["pot"][k])), k
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["water_only", "au13_w", "2au13_field"])
+def test_force_on_reused_rows(name, decks):
+    """Rows built with a skin are reused over MD steps and rebuilt on the
+    device once an atom has moved more than skin/2.  After 25 steps with a
+    0.02 nm skin (several rebuilds, most steps on reused rows) the force call
+    still matches the oracle's brute-force scan at the GPU's positions, as in
+    test_force_call (warm start: the GPU's previous solution)."""
+    deck = decks[name]
+    ctx = lib.Context(deck)
+    try:
+        ctx.set_skin(0.02)
+        assert ctx.skin == 0.02
+        ctx.force()
+        ctx.md_begin()
+        ctx.reset_timing()
+        assert ctx.md_run(1, 25, 10 ** 6) == 25
+        builds = ctx.nb_builds()
+        print(name, "row builds in 25 steps:", builds, flush=True)
+        assert 1 <= builds < 25
+        r, _, x_prev = ctx.get_state()
+        fg, pg, vg = ctx.force()
+        _, _, xg = ctx.get_state()
+        o = Oracle(deck)
+        f, pot, vir = o.nonb(r)
+        if deck.n_metal:
+            f2, p2, v2, _ = o.qsc(r)
+            f, pot, vir = f + f2, pot + p2, vir + v2
+            b = o.vec_ext(r)
+            o.relay(r)
+            xo, ito, _, _, kindo = o.solve(b, x_prev)
+            it, _, _, kind = ctx.solve_stats()
+            check_solution(deck, o, b, xg, kind, it, xo, kindo, ito)
+            f3, p3, v3 = o.cpff_force(r, xg)
+            f, pot, vir = f + f3, pot + p3, vir + v3
+        N = deck.n_atoms
+        q = deck.array("charge", N)
+        E = np.array(deck.p.efield[:])
+        R = r.reshape(3, N)
+        for c in range(3):
+            f[c * N:(c + 1) * N] += E[c] * q
+        pot[9] -= np.sum(E[0] * q * R[0] + E[1] * q * R[1] + E[2] * q * R[2])
+        print(name, "force rel", rel(fg, f))
+        assert rel(fg, f) < 1e-10
+        for k in range(1, 15):
+            assert abs(pg[k] - pot[k]) <= 1e-10 * max(1.0, abs(pot[k])), k
+        assert rel(vg, vir) < 1e-10
+    finally:
+        ctx.close()
