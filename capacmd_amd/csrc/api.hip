@@ -218,11 +218,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     *out = NULL;
     int rc = validate(p);
     if (rc) return rc;
-    if (precision == CAPAC_PREC_FP32_REFINED) {
-        capac_set_error("the FP32-refined relay path is not built yet");
-        return CAPAC_ERR_UNSUPPORTED;
-    }
-    if (precision != CAPAC_PREC_FP64) {
+    if (precision != CAPAC_PREC_FP64 && precision != CAPAC_PREC_FP32_REFINED) {
         capac_set_error("unknown precision %d", precision);
         return CAPAC_ERR_ARG;
     }
@@ -360,8 +356,13 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         TRY(upload(&c->x_backup, (const double*)NULL, c->n_mat));
         TRY(upload(&c->diag, (const double*)NULL, c->n_mat));
         c->relay_ld = ((Nm + 63) / 64) * 64;
-        c->relay_bytes = (size_t)Nm * 16 * c->relay_ld * sizeof(double);
-        CHECK_HIP(hipMalloc(&c->relay, c->relay_bytes));
+        if (precision == CAPAC_PREC_FP32_REFINED) {
+            c->relay_bytes = (size_t)Nm * 7 * c->relay_ld * sizeof(float);
+            CHECK_HIP(hipMalloc(&c->relay32, c->relay_bytes));
+        } else {
+            c->relay_bytes = (size_t)Nm * 16 * c->relay_ld * sizeof(double);
+            CHECK_HIP(hipMalloc(&c->relay, c->relay_bytes));
+        }
         TRY(solver_create(c));
     }
     for (int k = 0; k < 16; ++k) TRY(hipEventCreate(&c->ev[k]) == hipSuccess ? 0 : CAPAC_ERR_HIP);
@@ -380,7 +381,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->nb_overflow, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->md, c->md_status};
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32, c->md, c->md_status};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     if (c->red_host) (void)hipHostFree(c->red_host);
@@ -845,7 +846,8 @@ extern "C" int capac_set_thermostat(capac_ctx* c, const double* vq, double vp) {
 
 extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms) {
     if (!c || reps <= 0 || !ms) return CAPAC_ERR_ARG;
-    if ((kernel == CAPAC_KERNEL_SPMV || kernel == CAPAC_KERNEL_ASSEMBLE || kernel == CAPAC_KERNEL_CPIM_MM) &&
+    if ((kernel == CAPAC_KERNEL_SPMV || kernel == CAPAC_KERNEL_ASSEMBLE || kernel == CAPAC_KERNEL_CPIM_MM ||
+         kernel == CAPAC_KERNEL_APPLY64) &&
         !c->n_mat) {
         capac_set_error("no CPIM unknowns");
         return CAPAC_ERR_ARG;
@@ -863,6 +865,7 @@ extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms)
             case CAPAC_KERNEL_PAIR: rc = pair_nonbonded(c, NULL); break;
             case CAPAC_KERNEL_NEIGHBOUR: rc = nb_build(c); break;
             case CAPAC_KERNEL_CPIM_MM: rc = cpim_mm_only(c); break;
+            case CAPAC_KERNEL_APPLY64: rc = cpim_apply64(c, c->x, NULL, scratch, NULL); break;
             default: capac_set_error("unknown kernel %d", kernel); return CAPAC_ERR_ARG;
         }
         if (rc) return rc;

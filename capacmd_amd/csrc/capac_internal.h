@@ -9,6 +9,10 @@
 //                         (k >= nb_stride - n_back[i]); int32
 //   relay blocks          FP64: Nm x Nm 4x4 blocks [A(3x3) M(3x1); M^T(1x3) C],
 //                         element k of block (i,j) at relay[(i*16 + k)*relay_ld + j]
+//   relay (FP32-refined)  7 FP32 coefficient planes per metal row-group i,
+//                         plane k of pair (i,j) at relay32[(i*7 + k)*relay_ld + j]:
+//                         r_x, r_y, r_z (bohr), g_a = c2 - 3 c1, g_b = c1 r^2, c3, cq;
+//                         the diagonal blocks are applied analytically
 //   CPIM vector           x = [p (3Nm) | q (Nm) | lambda (n_np)], atomic units
 #pragma once
 
@@ -60,7 +64,8 @@ struct PairParam {          // one entry of the n_types x n_types table
 
 // BiCGSTAB scalars, resident on the device (bicg_stab.c:289-292)
 struct SolveScalars {
-    double rho, rho_old, alpha, omega, bTb, criteria, rTr;
+    double rho_s[2];        // rho of iteration n in slot n & 1
+    double alpha, omega, bTb, criteria, rTr;
     int iter, done, kind;
 };
 
@@ -112,10 +117,11 @@ struct capac_ctx {
     double* isd;
     // CPIM
     double *b, *x, *x_backup, *diag, *relay;
+    float* relay32;
     int relay_ld;
     size_t relay_bytes;
     Solver* solver;
-    int solve_iters, solve_kind;
+    int solve_iters, solve_kind, solve_refine_rounds;
     double solve_rTr, solve_bTb;
 
     // MD scalars (host copy of the small device state)
@@ -176,6 +182,9 @@ int cpim_assemble(capac_ctx* c);
 int cpim_forces(capac_ctx* c);
 int cpim_forces_x(capac_ctx* c, const double* x);
 int cpim_dense_expand(capac_ctx* c, double* dense_dev);
+// out = b - A x (b non-NULL) or A x, with the FP64 relay recomputed on the fly
+// (no stored matrix); part[g] = (out, out) over the rows of row-group g
+int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part);
 // solver.hip
 int solver_create(capac_ctx* c);
 void solver_destroy(capac_ctx* c);

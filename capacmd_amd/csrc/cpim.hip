@@ -93,10 +93,36 @@ __device__ __forceinline__ TCoef t_coef(const BoxConst& bc, double r2) {
     return t;
 }
 
+// relay block (i, j), i != j, with r = (r_j - r_i) NM2BOHR (cpff.c:259-263):
+//   A_ab = -(c1 (3 r_a r_b - delta_ab r^2) - c2 r_a r_b)      (cpff.c:326-328)
+//   dipole row / charge column  c3 r_a                        (cpff.c:353)
+//   charge row / dipole column  -c3 r_b                       (cpff.c:386)
+//   charge / charge             cq                            (cpff.c:428)
+__device__ __forceinline__ void relay_block64(const double* __restrict__ r, const BoxConst& bc, int i,
+                                              int j, double blk[16]) {
+    const int N = bc.N;
+    double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
+    pbc12(bc, dx, dy, dz);
+    dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+    double r2 = dist2_exact(dx, dy, dz);
+    TCoef t = t_coef(bc, r2);
+    double rv[3] = {dx, dy, dz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+            blk[a * 4 + bb] = -(t.c1 * (3.0 * rv[a] * rv[bb] - (a == bb ? r2 : 0.0)) -
+                                t.c2 * rv[a] * rv[bb]);
+        blk[a * 4 + 3] = t.c3 * rv[a];
+        blk[12 + a] = -t.c3 * rv[a];
+    }
+    blk[15] = t.cq;
+}
+
 // one thread per ordered pair (i, j): grid (ceil(Nm/256), Nm)
 __global__ void __launch_bounds__(256)
 k_assemble64(const double* __restrict__ r, BoxConst bc, int ld, double* __restrict__ relay) {
-    const int N = bc.N, Nm = bc.Nm;
+    const int Nm = bc.Nm;
     int i = blockIdx.y;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Nm) return;
@@ -104,29 +130,126 @@ k_assemble64(const double* __restrict__ r, BoxConst bc, int ld, double* __restri
     if (i == j) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) blk[k] = 0.0;
-        blk[0] = blk[5] = blk[10] = bc.inv_polar;
-        blk[15] = -bc.inv_capac;
+        blk[0] = blk[5] = blk[10] = bc.inv_polar;     // cpff.c:305
+        blk[15] = -bc.inv_capac;                      // cpff.c:408
     } else {
-        double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
-        pbc12(bc, dx, dy, dz);
-        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
-        double r2 = dist2_exact(dx, dy, dz);
-        TCoef t = t_coef(bc, r2);
-        double rv[3] = {dx, dy, dz};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-#pragma unroll
-            for (int bb = 0; bb < 3; ++bb)
-                blk[a * 4 + bb] = -(t.c1 * (3.0 * rv[a] * rv[bb] - (a == bb ? r2 : 0.0)) -
-                                    t.c2 * rv[a] * rv[bb]);
-            blk[a * 4 + 3] = t.c3 * rv[a];
-            blk[12 + a] = -t.c3 * rv[a];
-        }
-        blk[15] = t.cq;
+        relay_block64(r, bc, i, j, blk);
     }
     double* dst = relay + (size_t)i * 16 * ld + j;
 #pragma unroll
     for (int k = 0; k < 16; ++k) dst[(size_t)k * ld] = blk[k];
+}
+
+// FP32 T-tensor coefficients: the same expressions as t_coef, evaluated in
+// single precision
+struct TCoef32 {
+    float c1, c2, c3, cq;
+};
+
+__device__ __forceinline__ TCoef32 t_coef32(float r2, float iRpp, float iRpq, float iRqq) {
+    const float isp = (float)K_INV_SQRT_PI;
+    TCoef32 t;
+    float rij = sqrtf(r2);
+    float inv_r = 1.0f / rij;
+    float inv_r2 = inv_r * inv_r;
+    float inv_r3 = inv_r2 * inv_r;
+    float e_pp = expf(-(r2 * iRpp * iRpp));
+    t.c1 = inv_r3 * inv_r2 * (erff(rij * iRpp) - 2.0f * isp * iRpp * rij * e_pp);
+    t.c2 = 4.0f * isp * (iRpp * iRpp * iRpp) * inv_r2 * e_pp;
+    float e_pq = expf(-(r2 * iRpq * iRpq));
+    t.c3 = inv_r3 * (erff(rij * iRpq) - 2.0f * isp * iRpq * rij * e_pq);
+    t.cq = -erff(rij * iRqq) * inv_r;
+    return t;
+}
+
+// FP32-refined relay: 7 coefficient planes per row-group (the layout and
+// block algebra are described with RelayC32 in solver.hip); the separation is
+// formed in FP64 with the reference's pbc_12 and rounded once to FP32.
+// One thread per ordered pair: grid (ceil(Nm/256), Nm).
+__global__ void __launch_bounds__(256)
+k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restrict__ relay) {
+    const int N = bc.N, Nm = bc.Nm;
+    int i = blockIdx.y;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nm) return;
+    float cf[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (i != j) {
+        double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
+        pbc12(bc, dx, dy, dz);
+        float rx = (float)(dx * K_NM2BOHR), ry = (float)(dy * K_NM2BOHR), rz = (float)(dz * K_NM2BOHR);
+        float r2 = (rx * rx + ry * ry) + rz * rz;
+        TCoef32 t = t_coef32(r2, (float)bc.inv_R_pp, (float)bc.inv_R_pq, (float)bc.inv_R_qq);
+        cf[0] = rx; cf[1] = ry; cf[2] = rz;
+        cf[3] = t.c2 - 3.0f * t.c1;
+        cf[4] = t.c1 * r2;
+        cf[5] = t.c3;
+        cf[6] = t.cq;
+    }
+    float* dst = relay + (size_t)i * 7 * ld + j;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) dst[(size_t)k * ld] = cf[k];
+}
+
+// y = A x (or b - A x) with the FP64 relay recomputed pair by pair, the
+// entries bitwise those of k_assemble64; one workgroup per row-group (metal
+// i, then the Lagrange rows); part[g] = (out, out) over the group's rows
+__global__ void __launch_bounds__(256)
+k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_of,
+          const int* __restrict__ np_first, const int* __restrict__ np_last,
+          const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ out,
+          double* __restrict__ part) {
+    __shared__ double s_w[4][4];
+    const int Nm = bc.Nm;
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g >= Nm) {
+        const int k = g - Nm;
+        for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += blockDim.x) o[3] += x[3 * Nm + mi];
+    } else {
+        const int i = g;
+        for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
+            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2], x3 = x[3 * Nm + j];
+            if (j == i) {
+                o[0] += bc.inv_polar * x0;
+                o[1] += bc.inv_polar * x1;
+                o[2] += bc.inv_polar * x2;
+                o[3] += -bc.inv_capac * x3;
+                continue;
+            }
+            double blk[16];
+            relay_block64(r, bc, i, j, blk);
+            o[0] += blk[0] * x0 + blk[1] * x1 + blk[2] * x2 + blk[3] * x3;
+            o[1] += blk[4] * x0 + blk[5] * x1 + blk[6] * x2 + blk[7] * x3;
+            o[2] += blk[8] * x0 + blk[9] * x1 + blk[10] * x2 + blk[11] * x3;
+            o[3] += blk[12] * x0 + blk[13] * x1 + blk[14] * x2 + blk[15] * x3;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = wave_sum(o[c]);
+    if (lane == 0)
+        for (int c = 0; c < 4; ++c) s_w[wave][c] = o[c];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 4; ++c) o[c] = s_w[0][c] + s_w[1][c] + s_w[2][c] + s_w[3][c];
+    double acc = 0.0;
+    if (g >= Nm) {
+        const int row = 4 * Nm + (g - Nm);
+        double val = b ? b[row] - o[3] : o[3];
+        out[row] = val;
+        acc = val * val;
+    } else {
+        const int i = g;
+        const int np = np_of[i];
+        if (np >= 0) o[3] += x[4 * Nm + np];
+        const int rows[4] = {3 * i, 3 * i + 1, 3 * i + 2, 3 * Nm + i};
+        for (int c = 0; c < 4; ++c) {
+            double val = b ? b[rows[c]] - o[c] : o[c];
+            out[rows[c]] = val;
+            acc += val * val;
+        }
+    }
+    if (part) part[g] = acc;
 }
 
 __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
@@ -134,6 +257,32 @@ __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
     if (k >= n_mat) return;
     int Nm = bc.Nm;
     diag[k] = k < 3 * Nm ? bc.inv_polar : (k < 4 * Nm ? -bc.inv_capac : 1.0);
+}
+
+// dense 4Nm x 4Nm metal block from the FP32 coefficients (tests)
+__global__ void k_dense32(const float* __restrict__ relay, BoxConst bc, int ld, double* __restrict__ dense) {
+    const int Nm = bc.Nm;
+    long n4 = 4L * Nm;
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n4 * n4) return;
+    long row = idx / n4, col = idx % n4;
+    int i = row < 3L * Nm ? (int)(row / 3) : (int)(row - 3L * Nm);
+    int a = row < 3L * Nm ? (int)(row % 3) : 3;
+    int j = col < 3L * Nm ? (int)(col / 3) : (int)(col - 3L * Nm);
+    int b = col < 3L * Nm ? (int)(col % 3) : 3;
+    double v;
+    if (i == j) {
+        v = (a != b) ? 0.0 : (a < 3 ? bc.inv_polar : -bc.inv_capac);
+    } else {
+        const float* cp = relay + (size_t)i * 7 * ld + j;
+        double rv[3] = {cp[0], cp[ld], cp[2 * ld]};
+        double ga = cp[3 * ld], gb = cp[4 * ld], c3 = cp[5 * ld], cq = cp[6 * ld];
+        if (a < 3 && b < 3) v = ga * rv[a] * rv[b] + (a == b ? gb : 0.0);
+        else if (a < 3) v = c3 * rv[a];
+        else if (b < 3) v = -c3 * rv[b];
+        else v = cq;
+    }
+    dense[idx] = v;
 }
 
 __global__ void k_dense(const double* __restrict__ relay, int Nm, int ld, double* __restrict__ dense) {
@@ -399,7 +548,10 @@ int cpim_assemble(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
     dim3 grid((Nm + 255) / 256, Nm);
-    k_assemble64<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay);
+    if (c->precision == CAPAC_PREC_FP32_REFINED)
+        k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
+    else
+        k_assemble64<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay);
     k_diag<<<(c->n_mat + 255) / 256, 256, 0, c->stream>>>(c->bc, c->n_mat, c->diag);
     CHECK_HIP(hipGetLastError());
     return 0;
@@ -407,8 +559,20 @@ int cpim_assemble(capac_ctx* c) {
 
 int cpim_dense_expand(capac_ctx* c, double* dense) {
     long n4 = 4L * c->bc.Nm;
-    k_dense<<<(unsigned)((n4 * n4 + 255) / 256), 256, 0, c->stream>>>(c->relay, c->bc.Nm, c->relay_ld,
-                                                                       dense);
+    unsigned blocks = (unsigned)((n4 * n4 + 255) / 256);
+    if (c->precision == CAPAC_PREC_FP32_REFINED)
+        k_dense32<<<blocks, 256, 0, c->stream>>>(c->relay32, c->bc, c->relay_ld, dense);
+    else
+        k_dense<<<blocks, 256, 0, c->stream>>>(c->relay, c->bc.Nm, c->relay_ld, dense);
+    CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part) {
+    const int groups = c->bc.Nm + c->bc.n_np;
+    if (c->bc.Nm == 0) return 0;
+    k_apply64<<<groups, 256, 0, c->stream>>>(c->r, c->bc, c->np_of_metal, c->np_first, c->np_last, x, b,
+                                             out, part);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

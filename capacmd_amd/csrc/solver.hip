@@ -10,31 +10,103 @@
 // i gets + lambda_k(i); the row of nanoparticle k is sum_{i in k} q_i; their
 // preconditioner entries are 1.
 //
-// Per iteration: two fused SpMV+dot launches (one workgroup per metal
-// row-block, streaming its 16 x Nm block components once) and two
-// single-workgroup vector kernels that finish the dots in a fixed order and
-// update the vectors (the second also opens the next iteration).  All scalars
-// live on the device; a `done` flag turns the remaining launches of a batch
-// into no-ops, so the host synchronises once per batch, not per iteration.
-// Cross-workgroup hand-off stays at kernel boundaries: on this GPU the eight
-// XCDs have separate L2 caches, and an in-kernel device-scope fence costs
-// more than a launch.
+// Iteration n is four launches:
+//   K1  v = A y                SpMV, per row-group partial (r0, v)
+//   K2  alpha = rho/(r0, v)    multi-block vector kernel: s = r - alpha v,
+//       z = K s, partial (r0, s); it also finishes the stop test of
+//       iteration n-1 from the partials (r, r) of K4
+//   K3  t = A z                SpMV, partials (Kt, z), (Kt, Kt), (r0, t)
+//   K4  omega = (Kt,z)/(Kt,Kt) multi-block: x += alpha y + omega z,
+//       r = s - omega t, partial (r, r); rho' = (r0, r) taken as
+//       (r0, s) - omega (r0, t) so that beta and p = r + beta (p - omega v),
+//       y = K p follow in the same pass
+// Every block of a vector kernel reduces the partials of the previous launch
+// itself, in the same fixed order, so all blocks hold bitwise-equal scalars
+// without a grid barrier (on this GPU the eight XCDs have separate L2
+// caches, and an in-kernel device-scope fence costs more than a launch).
+// The stop test of iteration n runs one launch later, in K2 of n+1, so one
+// speculative SpMV is spent at the end of a solve.  All scalars live on the
+// device; a `done` flag turns the remaining launches of a batch into no-ops
+// and the host synchronises once per batch.
+//
+// Two relay formats share these kernels (template parameter):
+//   RelayF64  the reference's 4x4 blocks, FP64, diagonal blocks stored
+//   RelayC32  7 FP32 coefficients per metal pair; products accumulate in
+//             FP64 with FP64 vectors; diagonal blocks applied analytically
+// The FP32-refined solve runs BiCGSTAB on RelayC32, then checks the true
+// residual b - A x with the FP64 relay recomputed on the fly (no FP64 matrix
+// is stored) and, while the reference's criterion fails on it, solves for
+// the correction on RelayC32 and adds it (iterative refinement).
 #include "capac_internal.h"
 
 #include <stdlib.h>
 
 struct Solver {
-    int n_mat, Nm, n_np;
+    int n_mat, Nm, n_np, n_groups, n_vb;
     double *r0, *r, *p, *v, *s, *t, *y, *z, *K, *Ax;
-    double* part;           // (Nm + n_np) * 2 partial dots
+    double* part1;          // n_groups: (r0, v)
+    double* part3;          // 3 n_groups: (Kt, z), (Kt, Kt), (r0, t)
+    double* part2;          // n_vb: (r0, s)
+    double* part4;          // n_vb: (r, r)
+    double *d, *rr, *part_res;   // refinement: correction, FP64 residual, its partials
+    double* res;            // device: [0] (rr, rr)
     SolveScalars* sc;       // device
     SolveScalars* sc_host;  // pinned
+    double* res_host;       // pinned
 };
 
 namespace {
 
-constexpr int kVecThreads = 1024;
+constexpr int kVecThreads = 256;       // vector kernels: one row per thread
+constexpr int kInitThreads = 1024;
 
+// 4 x 4 FP64 blocks, component-major (element k of block (i,j) at
+// a[(i*16 + k)*ld + j]); diagonal blocks are part of the storage
+struct RelayF64 {
+    const double* a;
+    int ld;
+    static constexpr int kThreadsPerGroup = 256;
+    __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
+                                        double o[4]) const {
+        const double* bp = a + (size_t)i * 16 * ld + j;
+        o[0] += bp[0 * ld] * x0 + bp[1 * ld] * x1 + bp[2 * ld] * x2 + bp[3 * ld] * x3;
+        o[1] += bp[4 * ld] * x0 + bp[5 * ld] * x1 + bp[6 * ld] * x2 + bp[7 * ld] * x3;
+        o[2] += bp[8 * ld] * x0 + bp[9 * ld] * x1 + bp[10 * ld] * x2 + bp[11 * ld] * x3;
+        o[3] += bp[12 * ld] * x0 + bp[13 * ld] * x1 + bp[14 * ld] * x2 + bp[15 * ld] * x3;
+    }
+    __device__ __forceinline__ void diag(double, double, double, double, double[4]) const {}
+};
+
+// 7 FP32 coefficients per ordered pair (i != j), r = r_j - r_i in bohr:
+//   A_ab = g_a r_a r_b + g_b delta_ab,  M_a = c3 r_a,  M^T_b = -c3 r_b,  C = cq
+// so one block times (p_j, q_j) is
+//   dipole rows  (g_a (r.p) + c3 q) r + g_b p,   charge row  cq q - c3 (r.p)
+struct RelayC32 {
+    const float* a;
+    int ld;
+    double inv_polar, inv_capac;
+    static constexpr int kThreadsPerGroup = 64;
+    __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
+                                        double o[4]) const {
+        const float* cp = a + (size_t)i * 7 * ld + j;
+        double rx = cp[0], ry = cp[ld], rz = cp[2 * ld];
+        double ga = cp[3 * ld], gb = cp[4 * ld], c3 = cp[5 * ld], cq = cp[6 * ld];
+        double rp = rx * x0 + ry * x1 + rz * x2;
+        double gr = ga * rp + c3 * x3;
+        o[0] += gr * rx + gb * x0;
+        o[1] += gr * ry + gb * x1;
+        o[2] += gr * rz + gb * x2;
+        o[3] += cq * x3 - c3 * rp;
+    }
+    __device__ __forceinline__ void diag(double x0, double x1, double x2, double x3, double o[4]) const {
+        o[0] += inv_polar * x0;
+        o[1] += inv_polar * x1;
+        o[2] += inv_polar * x2;
+        o[3] -= inv_capac * x3;
+    }
+};
+
+// fixed-order block sum (blockDim a multiple of 64, <= 1024)
 __device__ double block_sum(double v, double* s_w) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     v = wave_sum(v);
@@ -46,78 +118,103 @@ __device__ double block_sum(double v, double* s_w) {
     return t;
 }
 
+// fixed-order sum of `n` strided values src[k*stride + off] over one block
+__device__ double block_reduce(const double* __restrict__ src, int n, int stride, int off, double* s_w) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc += src[(size_t)k * stride + off];
+    return block_sum(acc, s_w);
+}
 
-// y = A x for the 4 Nm relay rows, plus partial dots; blocks >= Nm compute
-// the Lagrange rows.  mode 0: no dots; 1: part[i] = (r0, y) rows of i;
-// 2: part[2i] = (K y, x), part[2i+1] = (K y, K y)
+// y = A x over row-groups: group g < Nm is metal i's four rows (3 dipole, 1
+// charge), group Nm + k the Lagrange row of nanoparticle k.
+//   mode 0: y only
+//   mode 1: part[g] = (r0, y)_g
+//   mode 2: part[3g..3g+2] = (K y, x)_g, (K y, K y)_g, (r0, y)_g
+template <class M>
 __global__ void __launch_bounds__(256)
-k_spmv64(const double* __restrict__ relay, int ld, int Nm, int n_np, const int* __restrict__ np_of,
-         const int* __restrict__ np_first, const int* __restrict__ np_last,
-         const double* __restrict__ x, double* __restrict__ y, int mode,
-         const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
-         const SolveScalars* __restrict__ sc) {
+k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restrict__ np_first,
+       const int* __restrict__ np_last, const double* __restrict__ x, double* __restrict__ y, int mode,
+       const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
+       const SolveScalars* __restrict__ sc) {
+    constexpr int TPG = M::kThreadsPerGroup;
+    constexpr int GPB = 256 / TPG;                 // groups per block
+    constexpr int WPG = TPG / 64;                  // waves per group
     __shared__ double s_w[4][4];
     if (sc && sc->done) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int i = blockIdx.x;
-    if (i >= Nm) {
-        // Lagrange row of nanoparticle k: sum of its metal charges
-        int k = i - Nm;
-        double acc = 0.0;
-        for (int m = np_first[k] + threadIdx.x; m <= np_last[k]; m += blockDim.x) acc += x[3 * Nm + m];
-        acc = wave_sum(acc);
-        if (lane == 0) s_w[wave][0] = acc;
+    const int t = threadIdx.x % TPG;
+    const int g = blockIdx.x * GPB + threadIdx.x / TPG;
+    const int n_groups = Nm + n_np;
+    // a group spans whole waves, and the barrier below is used only when a
+    // group is the whole block, so returning here never splits a barrier
+    if (g >= n_groups) return;
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g >= Nm) {
+        const int k = g - Nm;
+        for (int mi = np_first[k] + t; mi <= np_last[k]; mi += TPG) o[3] += x[3 * Nm + mi];
+    } else {
+        const int i = g;
+        for (int j = t; j < Nm; j += TPG)
+            m.acc(i, j, x[3 * j], x[3 * j + 1], x[3 * j + 2], x[3 * Nm + j], o);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = wave_sum(o[c]);
+    if (WPG > 1) {
+        if (lane == 0)
+            for (int c = 0; c < 4; ++c) s_w[wave][c] = o[c];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = s_w[0][0] + s_w[1][0] + s_w[2][0] + s_w[3][0];
-            int row = 4 * Nm + k;
-            y[row] = t;
-            if (mode == 1) part[i] = r0[row] * t;
-            if (mode == 2) { part[2 * i] = t * x[row]; part[2 * i + 1] = t * t; }
+        if (t == 0)
+            for (int c = 0; c < 4; ++c) {
+                double acc = 0.0;
+                for (int w = 0; w < WPG; ++w) acc += s_w[w][c];
+                o[c] = acc;
+            }
+    }
+    if (t != 0) return;
+    if (g >= Nm) {
+        const int row = 4 * Nm + (g - Nm);
+        const double val = o[3];
+        y[row] = val;
+        if (mode == 1) part[g] = r0[row] * val;
+        if (mode == 2) {
+            double kt = K[row] * val;
+            part[3 * g] = kt * x[row];
+            part[3 * g + 1] = kt * kt;
+            part[3 * g + 2] = r0[row] * val;
         }
         return;
     }
-    const double* base = relay + (size_t)i * 16 * ld;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
-        double xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2], xj3 = x[3 * Nm + j];
-        const double* bp = base + j;
-        a0 += bp[0 * ld] * xj0 + bp[1 * ld] * xj1 + bp[2 * ld] * xj2 + bp[3 * ld] * xj3;
-        a1 += bp[4 * ld] * xj0 + bp[5 * ld] * xj1 + bp[6 * ld] * xj2 + bp[7 * ld] * xj3;
-        a2 += bp[8 * ld] * xj0 + bp[9 * ld] * xj1 + bp[10 * ld] * xj2 + bp[11 * ld] * xj3;
-        a3 += bp[12 * ld] * xj0 + bp[13 * ld] * xj1 + bp[14 * ld] * xj2 + bp[15 * ld] * xj3;
-    }
-    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_sum(a3);
-    if (lane == 0) { s_w[wave][0] = a0; s_w[wave][1] = a1; s_w[wave][2] = a2; s_w[wave][3] = a3; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double o[4];
-        for (int c = 0; c < 4; ++c) o[c] = s_w[0][c] + s_w[1][c] + s_w[2][c] + s_w[3][c];
-        int np = np_of[i];
-        if (np >= 0) o[3] += x[4 * Nm + np];
-        int rows[4] = {3 * i, 3 * i + 1, 3 * i + 2, 3 * Nm + i};
-        for (int c = 0; c < 4; ++c) y[rows[c]] = o[c];
-        if (mode == 1) {
-            double d = 0.0;
-            for (int c = 0; c < 4; ++c) d += r0[rows[c]] * o[c];
-            part[i] = d;
-        } else if (mode == 2) {
-            double d0 = 0.0, d1 = 0.0;
-            for (int c = 0; c < 4; ++c) {
-                double kt = K[rows[c]] * o[c];
-                d0 += kt * x[rows[c]];
-                d1 += kt * kt;
-            }
-            part[2 * i] = d0;
-            part[2 * i + 1] = d1;
+    const int i = g;
+    m.diag(x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * Nm + i], o);
+    const int np = np_of[i];
+    if (np >= 0) o[3] += x[4 * Nm + np];
+    const int rows[4] = {3 * i, 3 * i + 1, 3 * i + 2, 3 * Nm + i};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) y[rows[c]] = o[c];
+    if (mode == 1) {
+        double d = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d += r0[rows[c]] * o[c];
+        part[g] = d;
+    } else if (mode == 2) {
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double kt = K[rows[c]] * o[c];
+            d0 += kt * x[rows[c]];
+            d1 += kt * kt;
+            d2 += r0[rows[c]] * o[c];
         }
+        part[3 * g] = d0;
+        part[3 * g + 1] = d1;
+        part[3 * g + 2] = d2;
     }
 }
 
 // start of BiCGSTAB (bicg_stab.c:294-404): bTb, criteria, r0 = r = b - Ax,
 // v = 0, K = 1/diag, rho = (r0, r) with the two early exits, and the first
 // direction p = r + rho (0 - 0) = r, y = K p
-__global__ void __launch_bounds__(kVecThreads)
+__global__ void __launch_bounds__(kInitThreads)
 k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __restrict__ Ax,
             const double* __restrict__ diag, double* __restrict__ r0, double* __restrict__ r,
             double* __restrict__ p, double* __restrict__ v, double* __restrict__ K,
@@ -147,10 +244,10 @@ k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __res
     if (threadIdx.x == 0) {
         sc->bTb = bTb;
         sc->criteria = bTb * 1.0e-08;
-        sc->rho_old = 1.0;
+        sc->rho_s[0] = 1.0;
+        sc->rho_s[1] = rho;              // rho of iteration 1
         sc->alpha = 1.0;
         sc->omega = 1.0;
-        sc->rho = rho;
         sc->rTr = 0.0;
         sc->iter = 0;
         sc->done = (zero_rhs || zero_rho) ? 1 : 0;
@@ -158,87 +255,115 @@ k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __res
     }
 }
 
-// alpha = rho / (r0, v); s = r - alpha v; z = K s
+// K2 of iteration n.  For n >= 2 it first completes iteration n-1: rTr from
+// the (r, r) partials of K4, the stop test and the 1000-iteration rule
+// (bicg_stab.c:539-571); then alpha = rho_n / (r0, v), s = r - alpha v, z = K s
+// (bicg_stab.c:416-460)
 __global__ void __launch_bounds__(kVecThreads)
-k_bicg_s(int n_mat, int n_part, const double* __restrict__ part, const double* __restrict__ r,
-         const double* __restrict__ v, const double* __restrict__ K, double* __restrict__ s,
-         double* __restrict__ z, SolveScalars* sc) {
+k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_vb,
+        const double* __restrict__ part4, const double* __restrict__ r, const double* __restrict__ v,
+        const double* __restrict__ K, const double* __restrict__ r0, double* __restrict__ s,
+        double* __restrict__ z, double* __restrict__ part2, SolveScalars* sc) {
     __shared__ double s_w[16];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
     __syncthreads();
     if (s_skip) return;
+    if (n >= 2) {
+        const double rTr = block_reduce(part4, n_vb, 1, 0, s_w);
+        const int iter = n - 1;
+        const double crit = sc->criteria;
+        int done = 0, kind = CAPAC_SOLVE_CONVERGED;
+        if (rTr < crit) {
+            done = 1;
+        } else if (iter > 1000) {
+            done = 1;
+            kind = rTr < crit * 100.0 ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->iter = iter;
+            sc->rTr = rTr;
+            if (done) { sc->kind = kind; sc->done = 1; }
+        }
+        if (done) return;
+    }
+    const double r0v = block_reduce(part1, n_groups, 1, 0, s_w);
+    const double alpha = sc->rho_s[n & 1] / r0v;
     double acc = 0.0;
-    for (int k = threadIdx.x; k < n_part; k += blockDim.x) acc += part[k];
-    double r0v = block_sum(acc, s_w);
-    double alpha = sc->rho / r0v;
-    for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_mat) {
         double sk = r[k] - alpha * v[k];
         s[k] = sk;
         z[k] = sk * K[k];
+        acc = r0[k] * sk;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) sc->alpha = alpha;
+    double r0s = block_sum(acc, s_w);
+    if (threadIdx.x == 0) {
+        part2[blockIdx.x] = r0s;
+        if (blockIdx.x == 0) sc->alpha = alpha;
+    }
 }
 
-// omega = (Kt, z) / (Kt, Kt); x += alpha y + omega z; r = s - omega t; rTr and
-// the stop tests (bicg_stab.c:483-571); when not done, the next iteration's
-// rho = (r0, r), beta = (rho/rho_old)(alpha/omega), p = r + beta (p - omega v)
-// and y = K p in the same pass (bicg_stab.c:365-404)
+// K4 of iteration n: omega, x += alpha y + omega z, r = s - omega t
+// (bicg_stab.c:483-535), then rho_{n+1}, beta and the next direction
+// p = r + beta (p - omega v), y = K p (bicg_stab.c:365-404)
 __global__ void __launch_bounds__(kVecThreads)
-k_bicg_xp(int n_mat, int n_part, const double* __restrict__ part, double* __restrict__ x,
-          double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ s,
-          const double* __restrict__ t, double* __restrict__ r, const double* __restrict__ r0,
-          double* __restrict__ p, const double* __restrict__ v, const double* __restrict__ K,
-          SolveScalars* sc) {
+k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n_vb,
+         const double* __restrict__ part2, double* __restrict__ x, double* __restrict__ y,
+         const double* __restrict__ z, const double* __restrict__ s, const double* __restrict__ t,
+         double* __restrict__ r, double* __restrict__ p, const double* __restrict__ v,
+         const double* __restrict__ K, double* __restrict__ part4, SolveScalars* sc) {
     __shared__ double s_w[16];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
     __syncthreads();
     if (s_skip) return;
-    double a0 = 0.0, a1 = 0.0;
-    for (int k = threadIdx.x; k < n_part; k += blockDim.x) { a0 += part[2 * k]; a1 += part[2 * k + 1]; }
-    double ktks = block_sum(a0, s_w);
-    double ktkt = block_sum(a1, s_w);
-    double omega = ktks / ktkt;
-    double alpha = sc->alpha;
-    double rho = sc->rho;
-    double acc = 0.0, acc_r0 = 0.0;
-    for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
+    const double ktz = block_reduce(part3, n_groups, 3, 0, s_w);
+    const double ktkt = block_reduce(part3, n_groups, 3, 1, s_w);
+    const double r0t = block_reduce(part3, n_groups, 3, 2, s_w);
+    const double r0s = block_reduce(part2, n_vb, 1, 0, s_w);
+    const double omega = ktz / ktkt;
+    const double alpha = sc->alpha;
+    const double rho = sc->rho_s[n & 1];
+    const double rho_next = r0s - omega * r0t;
+    const double beta = (rho_next / rho) * (alpha / omega);
+    double acc = 0.0;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_mat) {
         x[k] += alpha * y[k] + omega * z[k];
         double rk = s[k] - omega * t[k];
         r[k] = rk;
-        acc += rk * rk;
-        acc_r0 += r0[k] * rk;
+        acc = rk * rk;
+        double pk = rk + beta * (p[k] - omega * v[k]);
+        p[k] = pk;
+        y[k] = pk * K[k];
     }
-    double rTr = block_sum(acc, s_w);
-    double rho_next = block_sum(acc_r0, s_w);
-    const int iter = sc->iter + 1;
-    int done = 0, kind = CAPAC_SOLVE_CONVERGED;
-    if (rTr < sc->criteria) {
-        done = 1;
-    } else if (iter > 1000) {
-        done = 1;
-        kind = rTr < sc->criteria * 100.0 ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
-    }
-    if (!done) {
-        double beta = (rho_next / rho) * (alpha / omega);
-        for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
-            double pk = r[k] + beta * (p[k] - omega * v[k]);
-            p[k] = pk;
-            y[k] = pk * K[k];
+    double rr = block_sum(acc, s_w);
+    if (threadIdx.x == 0) {
+        part4[blockIdx.x] = rr;
+        if (blockIdx.x == 0) {
+            sc->rho_s[(n + 1) & 1] = rho_next;
+            sc->omega = omega;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        sc->omega = omega;
-        sc->rho_old = rho;
-        sc->rho = rho_next;
-        sc->iter = iter;
-        sc->rTr = rTr;
-        sc->done = done;
-        sc->kind = kind;
-    }
+}
+
+// (rr, rr) of the FP64 residual from its per-group partials
+__global__ void __launch_bounds__(256)
+k_res_finish(const double* __restrict__ part, int n, double* __restrict__ res) {
+    __shared__ double s_w[16];
+    double v = block_reduce(part, n, 1, 0, s_w);
+    if (threadIdx.x == 0) res[0] = v;
+}
+
+__global__ void __launch_bounds__(256) k_axpy1(int n, const double* __restrict__ d, double* __restrict__ x) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] += d[k];
+}
+
+__global__ void __launch_bounds__(256) k_zero(int n, double* __restrict__ x) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = 0.0;
 }
 
 }  // namespace
@@ -250,60 +375,94 @@ int solver_create(capac_ctx* c) {
     S->n_mat = c->n_mat;
     S->Nm = c->bc.Nm;
     S->n_np = c->bc.n_np;
+    S->n_groups = S->Nm + S->n_np;
+    S->n_vb = (S->n_mat + kVecThreads - 1) / kVecThreads;
     size_t n = (size_t)(c->n_mat > 0 ? c->n_mat : 1);
-    double** vecs[10] = {&S->r0, &S->r, &S->p, &S->v, &S->s, &S->t, &S->y, &S->z, &S->K, &S->Ax};
-    for (int k = 0; k < 10; ++k) {
+    double** vecs[12] = {&S->r0, &S->r, &S->p, &S->v, &S->s, &S->t, &S->y, &S->z, &S->K, &S->Ax,
+                         &S->d, &S->rr};
+    for (int k = 0; k < 12; ++k) {
         CHECK_HIP(hipMalloc(vecs[k], n * sizeof(double)));
         CHECK_HIP(hipMemset(*vecs[k], 0, n * sizeof(double)));
     }
-    CHECK_HIP(hipMalloc(&S->part, 2 * (size_t)(S->Nm + S->n_np + 1) * sizeof(double)));
+    const size_t ng = (size_t)S->n_groups + 1, nvb = (size_t)S->n_vb + 1;
+    CHECK_HIP(hipMalloc(&S->part1, ng * sizeof(double)));
+    CHECK_HIP(hipMalloc(&S->part3, 3 * ng * sizeof(double)));
+    CHECK_HIP(hipMalloc(&S->part_res, ng * sizeof(double)));
+    CHECK_HIP(hipMalloc(&S->part2, nvb * sizeof(double)));
+    CHECK_HIP(hipMalloc(&S->part4, nvb * sizeof(double)));
+    CHECK_HIP(hipMalloc(&S->res, 4 * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->sc, sizeof(SolveScalars)));
     CHECK_HIP(hipMemset(S->sc, 0, sizeof(SolveScalars)));
     CHECK_HIP(hipHostMalloc(&S->sc_host, sizeof(SolveScalars), hipHostMallocDefault));
+    CHECK_HIP(hipHostMalloc(&S->res_host, 4 * sizeof(double), hipHostMallocDefault));
     return 0;
 }
 
 void solver_destroy(capac_ctx* c) {
     Solver* S = c->solver;
     if (!S) return;
-    double* vecs[10] = {S->r0, S->r, S->p, S->v, S->s, S->t, S->y, S->z, S->K, S->Ax};
-    for (int k = 0; k < 10; ++k) (void)hipFree(vecs[k]);
-    (void)hipFree(S->part);
-    (void)hipFree(S->sc);
-    (void)hipHostFree(S->sc_host);
+    double* vecs[] = {S->r0, S->r, S->p, S->v, S->s, S->t, S->y, S->z, S->K, S->Ax, S->d, S->rr,
+                      S->part1, S->part3, S->part_res, S->part2, S->part4, S->res};
+    for (size_t k = 0; k < sizeof vecs / sizeof vecs[0]; ++k)
+        if (vecs[k]) (void)hipFree(vecs[k]);
+    if (S->sc) (void)hipFree(S->sc);
+    if (S->sc_host) (void)hipHostFree(S->sc_host);
+    if (S->res_host) (void)hipHostFree(S->res_host);
     free(S);
     c->solver = NULL;
 }
 
-static void spmv(capac_ctx* c, const double* x, double* y, int mode, const SolveScalars* sc) {
+template <class M>
+static void spmv_t(capac_ctx* c, const M& m, const double* x, double* y, int mode, double* part,
+                   const SolveScalars* sc) {
     Solver* S = c->solver;
-    k_spmv64<<<S->Nm + S->n_np, 256, 0, c->stream>>>(c->relay, c->relay_ld, S->Nm, S->n_np,
-                                                     c->np_of_metal, c->np_first, c->np_last, x, y,
-                                                     mode, S->r0, S->K, S->part, sc);
+    constexpr int GPB = 256 / M::kThreadsPerGroup;
+    const int blocks = (S->n_groups + GPB - 1) / GPB;
+    k_spmv<M><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
+                                             c->np_last, x, y, mode, S->r0, S->K, part, sc);
 }
 
-// Solves relay * x = b with the resident matrix; x is the warm start and the
-// result.  Fills c->solve_*; returns CAPAC_ERR_SOLVER when the reference
-// would exit (bicg_stab.c:563-569).
-int solver_run(capac_ctx* c, const double* b, double* x) {
+static void spmv(capac_ctx* c, const double* x, double* y, int mode, double* part,
+                 const SolveScalars* sc) {
+    if (c->precision == CAPAC_PREC_FP32_REFINED) {
+        RelayC32 m{c->relay32, c->relay_ld, c->bc.inv_polar, c->bc.inv_capac};
+        spmv_t(c, m, x, y, mode, part, sc);
+    } else {
+        RelayF64 m{c->relay, c->relay_ld};
+        spmv_t(c, m, x, y, mode, part, sc);
+    }
+}
+
+struct SolveOut {
+    int iters, kind;
+    double rTr, bTb;
+};
+
+// BiCGSTAB on the resident relay (format by precision) for A x = b, x the
+// warm start and the result
+static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, SolveOut* out) {
     Solver* S = c->solver;
-    const int n = S->n_mat, n4 = 4 * S->Nm, n_part = S->Nm + S->n_np;
+    const int n = S->n_mat, n4 = 4 * S->Nm;
+    const int vb = S->n_vb;
     hipStream_t st = c->stream;
-    spmv(c, x, S->Ax, 0, NULL);
-    k_bicg_init<<<1, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
-                                           S->y, S->sc);
-    // first batch: the previous solve's iteration count (warm starts change
-    // slowly), then small increments, so few launches are wasted as no-ops
-    const int first = c->solve_iters > 2 ? c->solve_iters : 4;
-    int batch = first, launched = 0;
+    spmv(c, x, S->Ax, 0, NULL, NULL);
+    k_bicg_init<<<1, kInitThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
+                                            S->y, S->sc);
+    // K1, K2 of iteration 1, then units (K3, K4 of n; K1, K2 of n+1): each
+    // unit ends with a stop test, so a batch of units can end the solve
+    spmv(c, S->y, S->v, 1, S->part1, S->sc);
+    k_vec_s<<<vb, kVecThreads, 0, st>>>(n, 1, S->n_groups, S->part1, vb, S->part4, S->r, S->v, S->K,
+                                        S->r0, S->s, S->z, S->part2, S->sc);
+    int it = 1, batch = first_batch > 1 ? first_batch : 2, launched = 0;
     for (;;) {
-        for (int it = 0; it < batch; ++it) {
-            spmv(c, S->y, S->v, 1, S->sc);
-            k_bicg_s<<<1, kVecThreads, 0, st>>>(n, n_part, S->part, S->r, S->v, S->K, S->s, S->z,
-                                                S->sc);
-            spmv(c, S->z, S->t, 2, S->sc);
-            k_bicg_xp<<<1, kVecThreads, 0, st>>>(n, n_part, S->part, x, S->y, S->z, S->s, S->t, S->r,
-                                                 S->r0, S->p, S->v, S->K, S->sc);
+        for (int u = 0; u < batch; ++u, ++it) {
+            spmv(c, S->z, S->t, 2, S->part3, S->sc);
+            k_vec_xp<<<vb, kVecThreads, 0, st>>>(n, it, S->n_groups, S->part3, vb, S->part2, x, S->y,
+                                                 S->z, S->s, S->t, S->r, S->p, S->v, S->K, S->part4,
+                                                 S->sc);
+            spmv(c, S->y, S->v, 1, S->part1, S->sc);
+            k_vec_s<<<vb, kVecThreads, 0, st>>>(n, it + 1, S->n_groups, S->part1, vb, S->part4, S->r,
+                                                S->v, S->K, S->r0, S->s, S->z, S->part2, S->sc);
         }
         launched += batch;
         CHECK_HIP(hipGetLastError());
@@ -314,13 +473,71 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
             capac_set_error("BiCGSTAB did not report completion after %d iterations", launched);
             return CAPAC_ERR_SOLVER;
         }
-        batch = (launched == first) ? 2 : (batch < 16 ? 2 * batch : 16);
+        batch = batch < 16 ? 2 * batch : 16;
+        if (batch < 2) batch = 2;
     }
-    c->solve_iters = S->sc_host->iter;
-    c->solve_kind = S->sc_host->kind;
-    c->solve_rTr = S->sc_host->rTr;
-    c->solve_bTb = S->sc_host->bTb;
-    c->timing.solve_iters += S->sc_host->iter;
+    out->iters = S->sc_host->iter;
+    out->kind = S->sc_host->kind;
+    out->rTr = S->sc_host->rTr;
+    out->bTb = S->sc_host->bTb;
+    return 0;
+}
+
+// FP64 residual rr = b - A64 x with the relay recomputed on the fly; returns (rr, rr)
+static int residual64(capac_ctx* c, const double* b, const double* x, double* rTr) {
+    Solver* S = c->solver;
+    int rc = cpim_apply64(c, x, b, S->rr, S->part_res);
+    if (rc) return rc;
+    k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, S->n_groups, S->res);
+    CHECK_HIP(hipGetLastError());
+    CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    *rTr = S->res_host[0];
+    return 0;
+}
+
+static void record(capac_ctx* c, const SolveOut& o) {
+    c->solve_iters = o.iters;
+    c->solve_kind = o.kind;
+    c->solve_rTr = o.rTr;
+    c->solve_bTb = o.bTb;
+    c->timing.solve_iters += o.iters;
+}
+
+// Solves relay * x = b; x is the warm start and the result.  Fills c->solve_*;
+// returns CAPAC_ERR_SOLVER when the reference would exit (bicg_stab.c:563-569).
+int solver_run(capac_ctx* c, const double* b, double* x) {
+    Solver* S = c->solver;
+    // first batch: the previous solve's iteration count (warm starts change
+    // slowly), so few launches are wasted as no-ops
+    const int first = c->solve_iters > 1 ? c->solve_iters : 2;
+    SolveOut o;
+    int rc = bicgstab(c, b, x, first, &o);
+    if (rc) return rc;
+    c->solve_refine_rounds = 0;
+    if (c->precision == CAPAC_PREC_FP32_REFINED &&
+        (o.kind == CAPAC_SOLVE_CONVERGED || o.kind == CAPAC_SOLVE_LOOSENED)) {
+        // FP64 refinement: the reference's criterion on the true residual
+        const double crit = (o.kind == CAPAC_SOLVE_LOOSENED ? 1.0e-6 : 1.0e-8) * o.bTb;
+        double rTr = 0.0;
+        if ((rc = residual64(c, b, x, &rTr))) return rc;
+        int total = o.iters, rounds = 0;
+        while (!(rTr < crit) && rounds < 4) {
+            const int n = S->n_mat;
+            k_zero<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d);
+            SolveOut od;
+            if ((rc = bicgstab(c, S->rr, S->d, 4, &od))) return rc;
+            total += od.iters;
+            k_axpy1<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d, x);
+            if ((rc = residual64(c, b, x, &rTr))) return rc;
+            ++rounds;
+        }
+        o.iters = total;
+        o.rTr = rTr;
+        if (!(rTr < crit)) o.kind = CAPAC_SOLVE_FAILED;
+        c->solve_refine_rounds = rounds;
+    }
+    record(c, o);
     if (c->solve_kind == CAPAC_SOLVE_FAILED) {
         capac_set_error("Error: bicg_stab not converged. rTr= %e, bTb= %e", c->solve_rTr, c->solve_bTb);
         return CAPAC_ERR_SOLVER;
@@ -330,7 +547,7 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
 
 // one plain SpMV over the resident matrix (for kernel timing)
 int solver_spmv_only(capac_ctx* c, const double* x, double* y) {
-    spmv(c, x, y, 0, NULL);
+    spmv(c, x, y, 0, NULL, NULL);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

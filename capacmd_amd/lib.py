@@ -167,7 +167,7 @@ GPU_EXPORTS = [
     "capac_time_kernel", "capac_set_skin", "capac_skin",
 ]
 
-KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM = range(5)
+KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64 = range(6)
 DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
 
 

@@ -48,12 +48,17 @@ extern "C" {
 
 /* precision of the CPIM relay matrix and inner solve */
 #define CAPAC_PREC_FP64          0  /* FP64 matrix, FP64 BiCGSTAB (the reference's arithmetic) */
-#define CAPAC_PREC_FP32_REFINED  1  /* FP32 matrix + FP32 inner solve in FP64 refinement */
+/* FP32 relay (T-tensors evaluated in FP32, 7 coefficients per metal pair),
+ * BiCGSTAB on it with FP64 vectors and dots, and FP64 iterative refinement:
+ * the reference's stop test is applied to the true residual b - A x with the
+ * FP64 relay recomputed on the fly, and corrections are solved for until it
+ * holds.  No counterpart in the reference (its arithmetic is FP64 only). */
+#define CAPAC_PREC_FP32_REFINED  1
 
-/* solver exit kinds (bicg_stab.c:300,368,539-571) */
 /* Verlet skin of the neighbour rows (nm), see capac_set_skin */
 #define CAPAC_DEFAULT_SKIN 0.1
 
+/* solver exit kinds (bicg_stab.c:300,368,539-571) */
 #define CAPAC_SOLVE_CONVERGED   0   /* rTr < 1e-8 bTb */
 #define CAPAC_SOLVE_LOOSENED    1   /* iter > 1000 and rTr < 1e-6 bTb */
 #define CAPAC_SOLVE_FAILED      2   /* iter > 1000 and rTr >= 1e-6 bTb */
@@ -240,6 +245,7 @@ int capac_set_thermostat(capac_ctx* ctx, const double* vq, double vp);
 #define CAPAC_KERNEL_PAIR      2
 #define CAPAC_KERNEL_NEIGHBOUR 3
 #define CAPAC_KERNEL_CPIM_MM   4
+#define CAPAC_KERNEL_APPLY64   5   /* FP64 relay product recomputed on the fly (refinement residual) */
 int capac_time_kernel(capac_ctx* ctx, int kernel, int reps, double* ms_per_launch);
 
 int capac_get_timing(capac_ctx* ctx, capac_timing* t);
