@@ -25,9 +25,19 @@ int pair_blocks(int N);
 
 namespace {
 
-__global__ void k_reduce(const double* __restrict__ partial, int nblk, int nval, double* __restrict__ out) {
+struct ReduceJobs {
+    int part_off[SEG_COUNT], nblk[SEG_COUNT], nval[SEG_COUNT], red_off[SEG_COUNT];
+};
+
+// one workgroup per queued reduction: fixed-order sums of a segment's
+// per-block partials [nblk x 16] into red
+__global__ void k_reduce(const double* __restrict__ partial_all, ReduceJobs jobs, double* __restrict__ red) {
     __shared__ double s_w[16][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int job = blockIdx.x;
+    const double* partial = partial_all + (size_t)jobs.part_off[job] * 16;
+    const int nblk = jobs.nblk[job], nval = jobs.nval[job];
+    double* out = red + jobs.red_off[job];
     double acc[16];
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0;
@@ -64,9 +74,40 @@ __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, s
 
 }  // namespace
 
-int reduce_partials(capac_ctx* c, int nblk, int nval, int off) {
-    k_reduce<<<1, 1024, 0, c->stream>>>(c->partial, nblk, nval, c->red + off);
+double* partial_seg(capac_ctx* c, int seg) { return c->partial + (size_t)c->seg_off[seg] * 16; }
+
+int flush_reductions(capac_ctx* c) {
+    if (c->n_pending == 0) return 0;
+    ReduceJobs jobs;
+    for (int k = 0; k < c->n_pending; ++k) {
+        jobs.part_off[k] = c->seg_off[c->pend_seg[k]];
+        jobs.nblk[k] = c->pend_nblk[k];
+        jobs.nval[k] = c->pend_nval[k];
+        jobs.red_off[k] = c->pend_red[k];
+    }
+    k_reduce<<<c->n_pending, 1024, 0, c->stream>>>(c->partial, jobs, c->red);
+    c->n_pending = 0;
     CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int off) {
+    if (seg < 0 || seg >= SEG_COUNT || nblk > c->seg_cap[seg] || nval > 16) {
+        capac_set_error("internal: reduction of %d blocks exceeds segment %d", nblk, seg);
+        return CAPAC_ERR_HIP;
+    }
+    // a segment still queued would be overwritten by its next producer
+    for (int k = 0; k < c->n_pending; ++k)
+        if (c->pend_seg[k] == seg) {
+            int rc = flush_reductions(c);
+            if (rc) return rc;
+            break;
+        }
+    const int k = c->n_pending++;
+    c->pend_seg[k] = seg;
+    c->pend_nblk[k] = nblk;
+    c->pend_nval[k] = nval;
+    c->pend_red[k] = off;
     return 0;
 }
 
@@ -75,6 +116,13 @@ static int fill_d(capac_ctx* c, double* p, size_t n, double v) {
     k_fill_d<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n, v);
     CHECK_HIP(hipGetLastError());
     return 0;
+}
+
+// zeroes red[off..off+n) after launching any queued reduction, so a queued
+// result cannot land after the clear
+static int clear_red(capac_ctx* c, int off, int n) {
+    int rc = flush_reductions(c);
+    return rc ? rc : fill_d(c, c->red + off, (size_t)n, 0.0);
 }
 
 static int fill_i(capac_ctx* c, int* p, size_t n, int v) {
@@ -95,7 +143,7 @@ static int copy_d(capac_ctx* c, const double* a, double* b, size_t n) {
 }
 
 int zero_outputs(capac_ctx* c) {
-    int rc = fill_d(c, c->red, 256, 0.0);
+    int rc = clear_red(c, 0, 256);
     if (!rc) rc = fill_d(c, c->f, 3 * (size_t)c->bc.N, 0.0);
     return rc;
 }
@@ -325,17 +373,21 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         if (stride < 64) stride = 64;
         TRY(alloc_nbr(c, stride));
     }
-    // reductions
+    // reductions: one partial segment per producer
     {
-        int need = pair_blocks(N);
-        int m = (Nm + 3) / 4 + 1;
-        if (m > need) need = m;
-        m = (p->n_mols + 255) / 256 + 1;
-        if (m > need) need = m;
-        m = (N + 255) / 256 + 1;
-        if (m > need) need = m;
-        c->n_partial = need;
-        TRY(upload(&c->partial, (const double*)NULL, (size_t)need * 16));
+        const int metal_blocks = (Nm + 3) / 4 + 1;
+        c->seg_cap[SEG_PAIR] = pair_blocks(N);
+        c->seg_cap[SEG_SC] = metal_blocks;
+        c->seg_cap[SEG_MW] = metal_blocks;
+        c->seg_cap[SEG_MM] = metal_blocks;
+        c->seg_cap[SEG_EXT] = (N + 255) / 256 + 1;
+        c->seg_cap[SEG_RATTLE] = (p->n_mols + 255) / 256 + 1;
+        c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;
+        int total = 0;
+        for (int s = 0; s < SEG_COUNT; ++s) { c->seg_off[s] = total; total += c->seg_cap[s]; }
+        c->n_partial = total;
+        c->n_pending = 0;
+        TRY(upload(&c->partial, (const double*)NULL, (size_t)total * 16));
         TRY(upload(&c->red, (const double*)NULL, 256));
         TRY(hipHostMalloc(&c->red_host, 320 * sizeof(double), hipHostMallocDefault) == hipSuccess
                 ? 0 : CAPAC_ERR_HIP);
@@ -462,6 +514,8 @@ extern "C" double capac_skin(const capac_ctx* c) { return c ? c->skin : -1.0; }
 // ------------------------------------------------------------------ force
 
 static int fetch_red(capac_ctx* c) {
+    int rc = flush_reductions(c);
+    if (rc) return rc;
     CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, 256 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->red_host + 256, c->nb_overflow, 4 * sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
@@ -487,7 +541,7 @@ static int force_compute(capac_ctx* c) {
     int rc;
     for (int attempt = 0; attempt < 8; ++attempt) {
         mark(c, 0);
-        if ((rc = fill_d(c, c->red, 256, 0.0))) return rc;
+        if ((rc = clear_red(c, 0, 256))) return rc;
         if (c->n_mat && (rc = copy_d(c, c->x, c->x_backup, c->n_mat))) return rc;
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
         c->nb_dirty = 0;
@@ -625,7 +679,7 @@ static void pot_out(capac_ctx* c, double* pot, double* vir, const int* pk, const
 extern "C" int capac_nonbonded(capac_ctx* c, double* f, double pot[CAPAC_NPOT], double vir[9]) {
     if (!c) return CAPAC_ERR_ARG;
     int rc = build_until_fits(c);
-    if (!rc) rc = fill_d(c, c->red, 256, 0.0);
+    if (!rc) rc = clear_red(c, 0, 256);
     if (!rc) rc = pair_nonbonded(c, NULL);
     if (!rc) rc = fetch_red(c);
     if (rc) return rc;
@@ -743,6 +797,8 @@ extern "C" int capac_solve_stats(capac_ctx* c, int* iters, double* rTr, double* 
 // ------------------------------------------------------------------ MD
 
 static int fetch_md(capac_ctx* c) {
+    int rc = flush_reductions(c);
+    if (rc) return rc;
     CHECK_HIP(hipMemcpyAsync(c->red_host + 260, c->md, 32 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->red_host + 292, c->md_status, 4 * sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
@@ -758,7 +814,7 @@ extern "C" int capac_md_begin(capac_ctx* c) {
     if (!rc) rc = md_kinetic(c);
     if (!rc) rc = md_fmax(c);
     if (!rc) rc = fill_i(c, c->md_status, 4, 0);
-    if (!rc) rc = fill_d(c, c->red + 64, 16, 0.0);
+    if (!rc) rc = clear_red(c, 64, 16);
     if (!rc) rc = fetch_md(c);
     if (rc) return rc;
     c->first_temp = c->red_host[260 + 1];
@@ -865,7 +921,7 @@ extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms)
             case CAPAC_KERNEL_PAIR: rc = pair_nonbonded(c, NULL); break;
             case CAPAC_KERNEL_NEIGHBOUR: rc = nb_build(c); break;
             case CAPAC_KERNEL_CPIM_MM: rc = cpim_mm_only(c); break;
-            case CAPAC_KERNEL_APPLY64: rc = cpim_apply64(c, c->x, NULL, scratch, NULL); break;
+            case CAPAC_KERNEL_APPLY64: rc = cpim_apply64(c, c->x, NULL, scratch, NULL, NULL); break;
             default: capac_set_error("unknown kernel %d", kernel); return CAPAC_ERR_ARG;
         }
         if (rc) return rc;

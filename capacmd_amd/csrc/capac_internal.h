@@ -108,8 +108,11 @@ struct capac_ctx {
     unsigned long long* nb_ctl; // [0] max displacement^2 bits, [1] rebuild flag, [2] builds
 
     // reductions: per-block partials and a small result area
-    double* partial;            // n_partial_blocks * 16
+    double* partial;            // per-block partials, one segment per producer (PartialSeg)
     int n_partial;
+    int seg_off[8], seg_cap[8];  // segment offsets and capacities, in blocks of 16 doubles
+    int n_pending;               // queued reductions, launched together by flush_reductions
+    int pend_seg[8], pend_nblk[8], pend_nval[8], pend_red[8];
     double* red;                // reduction results (64 doubles)
     double* red_host;           // pinned mirror
 
@@ -183,8 +186,11 @@ int cpim_forces(capac_ctx* c);
 int cpim_forces_x(capac_ctx* c, const double* x);
 int cpim_dense_expand(capac_ctx* c, double* dense_dev);
 // out = b - A x (b non-NULL) or A x, with the FP64 relay recomputed on the fly
-// (no stored matrix); part[g] = (out, out) over the rows of row-group g
-int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part);
+// (no stored matrix); part[g] = (out, out) over the rows of row-group g.
+// With a non-NULL guard the launch does nothing unless that solve has
+// finished with kind converged or loosened.
+int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part,
+                 const SolveScalars* guard);
 // solver.hip
 int solver_create(capac_ctx* c);
 void solver_destroy(capac_ctx* c);
@@ -203,7 +209,14 @@ int md_wrap(capac_ctx* c);
 int md_fmax(capac_ctx* c);
 int ext_field(capac_ctx* c);
 // reductions (api.hip): sum per-block partials [nblk x 16] into red[0..nval)
-int reduce_partials(capac_ctx* c, int nblk, int nval, int red_offset);
+// Per-block partial sums [nblk x 16] are written into the producer's own
+// segment of c->partial; reduce_partials queues their fixed-order reduction
+// into red[red_offset..+nval), and flush_reductions launches every queued one
+// in a single kernel (before anything reads red).
+enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_EXT, SEG_RATTLE, SEG_MOM, SEG_COUNT };
+double* partial_seg(capac_ctx* c, int seg);
+int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int red_offset);
+int flush_reductions(capac_ctx* c);
 int zero_outputs(capac_ctx* c);
 // graph-safe clears (fill kernels, not hipMemsetAsync)
 int fill_zero_i(capac_ctx* c, int* p, size_t n);

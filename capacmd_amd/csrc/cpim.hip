@@ -197,8 +197,10 @@ __global__ void __launch_bounds__(256)
 k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_of,
           const int* __restrict__ np_first, const int* __restrict__ np_last,
           const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ out,
-          double* __restrict__ part) {
+          double* __restrict__ part, const SolveScalars* __restrict__ guard) {
     __shared__ double s_w[4][4];
+    if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
+        return;
     const int Nm = bc.Nm;
     const int g = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -568,11 +570,12 @@ int cpim_dense_expand(capac_ctx* c, double* dense) {
     return 0;
 }
 
-int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part) {
+int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part,
+                 const SolveScalars* guard) {
     const int groups = c->bc.Nm + c->bc.n_np;
     if (c->bc.Nm == 0) return 0;
     k_apply64<<<groups, 256, 0, c->stream>>>(c->r, c->bc, c->np_of_metal, c->np_first, c->np_last, x, b,
-                                             out, part);
+                                             out, part, guard);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -584,16 +587,16 @@ int cpim_forces_x(capac_ctx* c, const double* x) {
     if (Nm == 0) return 0;
     int nblk = (Nm + 3) / 4;
     k_cpim_mw_metal<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
-                                                 c->nb_stride, x, c->f, c->partial);
+                                                 c->nb_stride, x, c->f, partial_seg(c, SEG_MW));
     CHECK_HIP(hipGetLastError());
-    int rc = reduce_partials(c, nblk, 11, 32);
+    int rc = reduce_partials(c, SEG_MW, nblk, 11, 32);
     if (rc) return rc;
     if (N > Nm)
         k_cpim_mw_atom<<<(N - Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
                                                                     c->nbr, c->nb_stride, x, c->f);
-    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, c->partial);
+    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, partial_seg(c, SEG_MM));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 13, 48);
+    return reduce_partials(c, SEG_MM, nblk, 13, 48);
 }
 
 int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
@@ -602,7 +605,7 @@ int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
 int cpim_mm_only(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
-    k_cpim_mm<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, c->partial);
+    k_cpim_mm<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM));
     CHECK_HIP(hipGetLastError());
     return 0;
 }

@@ -59,9 +59,80 @@ __device__ __forceinline__ double pbc1(double dx, double L) {
     return dx;
 }
 
+// Three-site molecules whose constraints are (0,1), (0,2), (1,2) in file
+// order (rigid water) run a copy of the Gauss-Seidel loop with compile-time
+// atom indices, so the whole molecule stays in registers; operations and
+// their order are those of the general loop, so results are identical.
+__device__ __forceinline__ bool is_rigid3(int f0, int nat, int c0, int nc, const int* __restrict__ ci,
+                                          const int* __restrict__ cj) {
+    return nat == 3 && nc == 3 && ci[c0] == f0 && cj[c0] == f0 + 1 && ci[c0 + 1] == f0 &&
+           cj[c0 + 1] == f0 + 2 && ci[c0 + 2] == f0 + 1 && cj[c0 + 2] == f0 + 2;
+}
+
+struct Site {
+    double x, y, z, vx, vy, vz, ox, oy, oz, im;
+};
+
+// one RATTLE_1 correction of constraint (a, b) (force.c:1949-1988)
+__device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, double rab, const BoxConst& bc, double dt) {
+    const double tol2 = 2.0e-05;
+    double px = pbc1(a.x - b.x, bc.box[0]);
+    double py = pbc1(a.y - b.y, bc.box[1]);
+    double pz = pbc1(a.z - b.z, bc.box[2]);
+    double pab2 = px * px + py * py + pz * pz;
+    double rab2 = rab * rab;
+    double diff = rab2 - pab2;
+    if (!(fabs(diff) > rab2 * tol2)) return false;
+    double qx = pbc1(a.ox - b.ox, bc.box[0]);
+    double qy = pbc1(a.oy - b.oy, bc.box[1]);
+    double qz = pbc1(a.oz - b.oz, bc.box[2]);
+    double rp = qx * px + qy * py + qz * pz;
+    double ma = a.im, mb = b.im;
+    double g = diff / (2.0 * (ma + mb) * rp);
+    double dx = qx * g, dy = qy * g, dz = qz * g;
+    a.x += ma * dx; a.y += ma * dy; a.z += ma * dz;
+    b.x -= mb * dx; b.y -= mb * dy; b.z -= mb * dz;
+    dx /= dt; dy /= dt; dz /= dt;
+    a.vx += ma * dx; a.vy += ma * dy; a.vz += ma * dz;
+    b.vx -= mb * dx; b.vy -= mb * dy; b.vz -= mb * dz;
+    return true;
+}
+
+// one RATTLE_2 correction of constraint (a, b) (force.c:2046-2081)
+__device__ __forceinline__ bool rattle2_pair(Site& a, Site& b, double rab, const BoxConst& bc,
+                                             double half_dt, double w[9]) {
+    const double tol = 1.0e-05;
+    double ux = a.vx - b.vx, uy = a.vy - b.vy, uz = a.vz - b.vz;
+    double px = pbc1(a.x - b.x, bc.box[0]);
+    double py = pbc1(a.y - b.y, bc.box[1]);
+    double pz = pbc1(a.z - b.z, bc.box[2]);
+    double rv = px * ux + py * uy + pz * uz;
+    double ma = a.im, mb = b.im;
+    double g = -rv / ((ma + mb) * (rab * rab));
+    if (!(fabs(g) > tol)) return false;
+    double dx = px * g, dy = py * g, dz = pz * g;
+    a.vx += ma * dx; a.vy += ma * dy; a.vz += ma * dz;
+    b.vx -= mb * dx; b.vy -= mb * dy; b.vz -= mb * dz;
+    dx /= half_dt; dy /= half_dt; dz /= half_dt;
+    w[0] -= px * dx; w[1] -= px * dy; w[2] -= px * dz;
+    w[3] -= py * dx; w[4] -= py * dy; w[5] -= py * dz;
+    w[6] -= pz * dx; w[7] -= pz * dy; w[8] -= pz * dz;
+    return true;
+}
+
+__device__ __forceinline__ void load_site(Site& s, int i, int N, const double* __restrict__ r,
+                                          const double* __restrict__ v, const double* __restrict__ old_r,
+                                          const double* __restrict__ inv_mass) {
+    s.x = r[i]; s.y = r[N + i]; s.z = r[2 * N + i];
+    s.vx = v[i]; s.vy = v[N + i]; s.vz = v[2 * N + i];
+    if (old_r) { s.ox = old_r[i]; s.oy = old_r[N + i]; s.oz = old_r[2 * N + i]; }
+    s.im = inv_mass[i];
+}
+
 // One thread per molecule; Gauss-Seidel sweeps over its constraints in file
 // order with the reference's arithmetic (no contraction, see the file pragma).
-__global__ void k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+__global__ void __launch_bounds__(256) k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+                          const int* __restrict__ mol_first, const int* __restrict__ mol_last,
                           const int* __restrict__ ci, const int* __restrict__ cj,
                           const double* __restrict__ cr, const double* __restrict__ inv_mass,
                           double* __restrict__ r, double* __restrict__ v,
@@ -70,6 +141,31 @@ __global__ void k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* 
     if (m >= n_mols) return;
     int c0 = mol_cstr[m], c1 = mol_cstr[m + 1];
     if (c0 == c1) return;
+    const int f0 = mol_first[m], nat = mol_last[m] - f0 + 1, nc = c1 - c0;
+    if (is_rigid3(f0, nat, c0, nc, ci, cj)) {
+        Site s0, s1, s2;
+        load_site(s0, f0, N, r, v, old_r, inv_mass);
+        load_site(s1, f0 + 1, N, r, v, old_r, inv_mass);
+        load_site(s2, f0 + 2, N, r, v, old_r, inv_mass);
+        const double r01 = cr[c0], r02 = cr[c0 + 1], r12 = cr[c0 + 2];
+        int iter = 0, done = 0;
+        while (!done) {
+            done = 1;
+            if (rattle1_pair(s0, s1, r01, bc, dt)) done = 0;
+            if (rattle1_pair(s0, s2, r02, bc, dt)) done = 0;
+            if (rattle1_pair(s1, s2, r12, bc, dt)) done = 0;
+            if (++iter > 1000) break;
+        }
+        const Site* st[3] = {&s0, &s1, &s2};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = f0 + q;
+            r[i] = st[q]->x; r[N + i] = st[q]->y; r[2 * N + i] = st[q]->z;
+            v[i] = st[q]->vx; v[N + i] = st[q]->vy; v[2 * N + i] = st[q]->vz;
+        }
+        if (!done) status[0] = 1;
+        return;
+    }
     const double tol2 = 2.0e-05;
     double *rx = r, *ry = r + N, *rz = r + 2 * N;
     double *vx = v, *vy = v + N, *vz = v + 2 * N;
@@ -108,6 +204,7 @@ __global__ void k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* 
 
 __global__ void __launch_bounds__(256)
 k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+          const int* __restrict__ mol_first, const int* __restrict__ mol_last,
           const int* __restrict__ ci, const int* __restrict__ cj, const double* __restrict__ cr,
           const double* __restrict__ inv_mass, const double* __restrict__ r, double* __restrict__ v,
           int* __restrict__ status, double* __restrict__ partial) {
@@ -115,8 +212,33 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (m < n_mols) {
-        int c0 = mol_cstr[m], c1 = mol_cstr[m + 1];
+    const int c0m = m < n_mols ? mol_cstr[m] : 0, c1m = m < n_mols ? mol_cstr[m + 1] : 0;
+    const int f0 = m < n_mols ? mol_first[m] : 0;
+    const int nat = m < n_mols ? mol_last[m] - f0 + 1 : 0, ncm = c1m - c0m;
+    if (m < n_mols && is_rigid3(f0, nat, c0m, ncm, ci, cj)) {
+        const double half_dt = 0.5 * dt;
+        Site s0, s1, s2;
+        load_site(s0, f0, N, r, v, NULL, inv_mass);
+        load_site(s1, f0 + 1, N, r, v, NULL, inv_mass);
+        load_site(s2, f0 + 2, N, r, v, NULL, inv_mass);
+        const double r01 = cr[c0m], r02 = cr[c0m + 1], r12 = cr[c0m + 2];
+        int iter = 0, done = 0;
+        while (!done) {
+            done = 1;
+            if (rattle2_pair(s0, s1, r01, bc, half_dt, w)) done = 0;
+            if (rattle2_pair(s0, s2, r02, bc, half_dt, w)) done = 0;
+            if (rattle2_pair(s1, s2, r12, bc, half_dt, w)) done = 0;
+            if (++iter > 1000) break;
+        }
+        const Site* st[3] = {&s0, &s1, &s2};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int i = f0 + q;
+            v[i] = st[q]->vx; v[N + i] = st[q]->vy; v[2 * N + i] = st[q]->vz;
+        }
+        if (!done) status[1] = 1;
+    } else if (m < n_mols) {
+        int c0 = c0m, c1 = c1m;
         const double tol = 1.0e-05, half_dt = 0.5 * dt;
         const double *rx = r, *ry = r + N, *rz = r + 2 * N;
         double *vx = v, *vy = v + N, *vz = v + 2 * N;
@@ -346,7 +468,8 @@ int md_drift(capac_ctx* c) {
 int md_rattle1(capac_ctx* c) {
     if (c->n_cstr == 0) return 0;
     k_rattle1<<<grid(c->bc.n_mols, 256), 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt,
-                                                              c->mol_cstr, c->cstr_i, c->cstr_j,
+                                                              c->mol_cstr, c->mol_first, c->mol_last,
+                                                              c->cstr_i, c->cstr_j,
                                                               c->cstr_r, c->inv_mass, c->r, c->v,
                                                               c->old_r, c->md_status);
     CHECK_HIP(hipGetLastError());
@@ -360,18 +483,19 @@ int md_rattle2(capac_ctx* c) {
     }
     int nblk = grid(c->bc.n_mols, 256);
     k_rattle2<<<nblk, 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt, c->mol_cstr,
-                                           c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
-                                           c->md_status, c->partial);
+                                           c->mol_first, c->mol_last, c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
+                                           c->md_status, partial_seg(c, SEG_RATTLE));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 9, 64);
+    return reduce_partials(c, SEG_RATTLE, nblk, 9, 64);
 }
 
 // red[96..103) = momentum / kinetic sums
 static int mom_sums(capac_ctx* c) {
     int nblk = grid(c->bc.N, 256);
-    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, c->partial);
+    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 7, 96);
+    int rc = reduce_partials(c, SEG_MOM, nblk, 7, 96);
+    return rc ? rc : flush_reductions(c);      // read on the device right after
 }
 
 int md_kinetic(capac_ctx* c) {
@@ -413,7 +537,7 @@ int md_fmax(capac_ctx* c) {
 // red[80] = pot9 from the charges
 int ext_field(capac_ctx* c) {
     int nblk = grid(c->bc.N, 256);
-    k_ext<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->bc, c->charge, c->r, c->f, c->partial);
+    k_ext<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->bc, c->charge, c->r, c->f, partial_seg(c, SEG_EXT));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 1, 80);
+    return reduce_partials(c, SEG_EXT, nblk, 1, 80);
 }

@@ -141,7 +141,7 @@ int pair_nonbonded(capac_ctx* c, double* f_accumulate) {
     double* f = f_accumulate ? f_accumulate : c->f;
     k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
                                         c->n_back, c->nbr, c->nb_stride, f,
-                                        f_accumulate ? 1 : 0, c->partial);
+                                        f_accumulate ? 1 : 0, partial_seg(c, SEG_PAIR));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 11, 0);
+    return reduce_partials(c, SEG_PAIR, nblk, 11, 0);
 }

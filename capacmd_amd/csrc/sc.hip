@@ -108,7 +108,7 @@ int sc_forces(capac_ctx* c) {
     int nblk = (c->bc.Nm + 3) / 4;
     k_sc_dens<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, c->n_back, c->nbr, c->nb_stride, c->isd);
     k_sc_force<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, c->n_back, c->nbr, c->nb_stride, c->isd,
-                                            c->f, c->partial);
+                                            c->f, partial_seg(c, SEG_SC));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, nblk, 10, 16);
+    return reduce_partials(c, SEG_SC, nblk, 10, 16);
 }

@@ -65,7 +65,6 @@ constexpr int kInitThreads = 1024;
 struct RelayF64 {
     const double* a;
     int ld;
-    static constexpr int kThreadsPerGroup = 256;
     __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
                                         double o[4]) const {
         const double* bp = a + (size_t)i * 16 * ld + j;
@@ -85,7 +84,6 @@ struct RelayC32 {
     const float* a;
     int ld;
     double inv_polar, inv_capac;
-    static constexpr int kThreadsPerGroup = 64;
     __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
                                         double o[4]) const {
         const float* cp = a + (size_t)i * 7 * ld + j;
@@ -125,18 +123,36 @@ __device__ double block_reduce(const double* __restrict__ src, int n, int stride
     return block_sum(acc, s_w);
 }
 
+// NV fixed-order block sums at once (one LDS round); s_w holds 16 x NV
+template <int NV>
+__device__ void block_sum_n(double v[NV], double* s_w) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = wave_sum(v[c]);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) s_w[wave * NV + c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += s_w[w * NV + c];
+        v[c] = t;
+    }
+}
+
 // y = A x over row-groups: group g < Nm is metal i's four rows (3 dipole, 1
 // charge), group Nm + k the Lagrange row of nanoparticle k.
 //   mode 0: y only
 //   mode 1: part[g] = (r0, y)_g
 //   mode 2: part[3g..3g+2] = (K y, x)_g, (K y, K y)_g, (r0, y)_g
-template <class M>
+// TPG threads per row-group (64: one wavefront, 256: the whole workgroup)
+template <class M, int TPG>
 __global__ void __launch_bounds__(256)
 k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restrict__ np_first,
        const int* __restrict__ np_last, const double* __restrict__ x, double* __restrict__ y, int mode,
        const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
        const SolveScalars* __restrict__ sc) {
-    constexpr int TPG = M::kThreadsPerGroup;
     constexpr int GPB = 256 / TPG;                 // groups per block
     constexpr int WPG = TPG / 64;                  // waves per group
     __shared__ double s_w[4][4];
@@ -264,13 +280,18 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
         const double* __restrict__ part4, const double* __restrict__ r, const double* __restrict__ v,
         const double* __restrict__ K, const double* __restrict__ r0, double* __restrict__ s,
         double* __restrict__ z, double* __restrict__ part2, SolveScalars* sc) {
-    __shared__ double s_w[16];
+    __shared__ double s_w[16 * 2];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
     __syncthreads();
     if (s_skip) return;
+    double red[2] = {0.0, 0.0};          // (r0, v), (r, r) of iteration n-1
+    for (int k = threadIdx.x; k < n_groups; k += blockDim.x) red[0] += part1[k];
+    if (n >= 2)
+        for (int k = threadIdx.x; k < n_vb; k += blockDim.x) red[1] += part4[k];
+    block_sum_n<2>(red, s_w);
     if (n >= 2) {
-        const double rTr = block_reduce(part4, n_vb, 1, 0, s_w);
+        const double rTr = red[1];
         const int iter = n - 1;
         const double crit = sc->criteria;
         int done = 0, kind = CAPAC_SOLVE_CONVERGED;
@@ -287,7 +308,7 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
         }
         if (done) return;
     }
-    const double r0v = block_reduce(part1, n_groups, 1, 0, s_w);
+    const double r0v = red[0];
     const double alpha = sc->rho_s[n & 1] / r0v;
     double acc = 0.0;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,6 +318,7 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
         z[k] = sk * K[k];
         acc = r0[k] * sk;
     }
+    __syncthreads();                      // s_w is reused
     double r0s = block_sum(acc, s_w);
     if (threadIdx.x == 0) {
         part2[blockIdx.x] = r0s;
@@ -313,15 +335,20 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
          const double* __restrict__ z, const double* __restrict__ s, const double* __restrict__ t,
          double* __restrict__ r, double* __restrict__ p, const double* __restrict__ v,
          const double* __restrict__ K, double* __restrict__ part4, SolveScalars* sc) {
-    __shared__ double s_w[16];
+    __shared__ double s_w[16 * 4];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
     __syncthreads();
     if (s_skip) return;
-    const double ktz = block_reduce(part3, n_groups, 3, 0, s_w);
-    const double ktkt = block_reduce(part3, n_groups, 3, 1, s_w);
-    const double r0t = block_reduce(part3, n_groups, 3, 2, s_w);
-    const double r0s = block_reduce(part2, n_vb, 1, 0, s_w);
+    double red[4] = {0.0, 0.0, 0.0, 0.0};   // (Kt, z), (Kt, Kt), (r0, t), (r0, s)
+    for (int k = threadIdx.x; k < n_groups; k += blockDim.x) {
+        red[0] += part3[3 * k];
+        red[1] += part3[3 * k + 1];
+        red[2] += part3[3 * k + 2];
+    }
+    for (int k = threadIdx.x; k < n_vb; k += blockDim.x) red[3] += part2[k];
+    block_sum_n<4>(red, s_w);
+    const double ktz = red[0], ktkt = red[1], r0t = red[2], r0s = red[3];
     const double omega = ktz / ktkt;
     const double alpha = sc->alpha;
     const double rho = sc->rho_s[n & 1];
@@ -338,6 +365,7 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
         p[k] = pk;
         y[k] = pk * K[k];
     }
+    __syncthreads();                      // s_w is reused
     double rr = block_sum(acc, s_w);
     if (threadIdx.x == 0) {
         part4[blockIdx.x] = rr;
@@ -348,10 +376,14 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
     }
 }
 
-// (rr, rr) of the FP64 residual from its per-group partials
+// (rr, rr) of the FP64 residual from its per-group partials (with a guard:
+// only after a solve that finished converged or loosened)
 __global__ void __launch_bounds__(256)
-k_res_finish(const double* __restrict__ part, int n, double* __restrict__ res) {
+k_res_finish(const double* __restrict__ part, int n, double* __restrict__ res,
+             const SolveScalars* __restrict__ guard) {
     __shared__ double s_w[16];
+    if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
+        return;
     double v = block_reduce(part, n, 1, 0, s_w);
     if (threadIdx.x == 0) res[0] = v;
 }
@@ -412,24 +444,29 @@ void solver_destroy(capac_ctx* c) {
     c->solver = NULL;
 }
 
-template <class M>
+template <class M, int TPG>
 static void spmv_t(capac_ctx* c, const M& m, const double* x, double* y, int mode, double* part,
                    const SolveScalars* sc) {
     Solver* S = c->solver;
-    constexpr int GPB = 256 / M::kThreadsPerGroup;
+    constexpr int GPB = 256 / TPG;
     const int blocks = (S->n_groups + GPB - 1) / GPB;
-    k_spmv<M><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
-                                             c->np_last, x, y, mode, S->r0, S->K, part, sc);
+    k_spmv<M, TPG><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
+                                                  c->np_last, x, y, mode, S->r0, S->K, part, sc);
 }
 
+// Row-group width: the FP64 rows (128 N_m bytes each) always get a whole
+// workgroup; the compact FP32 rows (28 N_m bytes) get a workgroup while
+// N_m is small enough that one wavefront per group would leave most SIMDs
+// idle, a wavefront beyond that.
 static void spmv(capac_ctx* c, const double* x, double* y, int mode, double* part,
                  const SolveScalars* sc) {
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
         RelayC32 m{c->relay32, c->relay_ld, c->bc.inv_polar, c->bc.inv_capac};
-        spmv_t(c, m, x, y, mode, part, sc);
+        if (c->bc.Nm < 1024) spmv_t<RelayC32, 256>(c, m, x, y, mode, part, sc);
+        else spmv_t<RelayC32, 64>(c, m, x, y, mode, part, sc);
     } else {
         RelayF64 m{c->relay, c->relay_ld};
-        spmv_t(c, m, x, y, mode, part, sc);
+        spmv_t<RelayF64, 256>(c, m, x, y, mode, part, sc);
     }
 }
 
@@ -439,8 +476,12 @@ struct SolveOut {
 };
 
 // BiCGSTAB on the resident relay (format by precision) for A x = b, x the
-// warm start and the result
-static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, SolveOut* out) {
+// warm start and the result.  With check64, every batch ends with the FP64
+// residual b - A x (guarded: it runs only once the solve has finished), so
+// the refinement test needs no extra synchronisation; (rr, rr) lands in
+// res_host[0].
+static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, bool check64,
+                    SolveOut* out) {
     Solver* S = c->solver;
     const int n = S->n_mat, n4 = 4 * S->Nm;
     const int vb = S->n_vb;
@@ -465,6 +506,12 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, S
                                                 S->v, S->K, S->r0, S->s, S->z, S->part2, S->sc);
         }
         launched += batch;
+        if (check64) {
+            int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
+            if (rc) return rc;
+            k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
+            CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
+        }
         CHECK_HIP(hipGetLastError());
         CHECK_HIP(hipMemcpyAsync(S->sc_host, S->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st));
         CHECK_HIP(hipStreamSynchronize(st));
@@ -486,9 +533,9 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, S
 // FP64 residual rr = b - A64 x with the relay recomputed on the fly; returns (rr, rr)
 static int residual64(capac_ctx* c, const double* b, const double* x, double* rTr) {
     Solver* S = c->solver;
-    int rc = cpim_apply64(c, x, b, S->rr, S->part_res);
+    int rc = cpim_apply64(c, x, b, S->rr, S->part_res, NULL);
     if (rc) return rc;
-    k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, S->n_groups, S->res);
+    k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, S->n_groups, S->res, NULL);
     CHECK_HIP(hipGetLastError());
     CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -510,23 +557,23 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
     Solver* S = c->solver;
     // first batch: the previous solve's iteration count (warm starts change
     // slowly), so few launches are wasted as no-ops
-    const int first = c->solve_iters > 1 ? c->solve_iters : 2;
+    const int first = c->solve_iters > 1 ? c->solve_iters + 1 : 2;
+    const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
     SolveOut o;
-    int rc = bicgstab(c, b, x, first, &o);
+    int rc = bicgstab(c, b, x, first, refine, &o);
     if (rc) return rc;
     c->solve_refine_rounds = 0;
-    if (c->precision == CAPAC_PREC_FP32_REFINED &&
-        (o.kind == CAPAC_SOLVE_CONVERGED || o.kind == CAPAC_SOLVE_LOOSENED)) {
-        // FP64 refinement: the reference's criterion on the true residual
+    if (refine && (o.kind == CAPAC_SOLVE_CONVERGED || o.kind == CAPAC_SOLVE_LOOSENED)) {
+        // FP64 refinement: the reference's criterion on the true residual,
+        // already computed at the end of the last batch
         const double crit = (o.kind == CAPAC_SOLVE_LOOSENED ? 1.0e-6 : 1.0e-8) * o.bTb;
-        double rTr = 0.0;
-        if ((rc = residual64(c, b, x, &rTr))) return rc;
+        double rTr = S->res_host[0];
         int total = o.iters, rounds = 0;
         while (!(rTr < crit) && rounds < 4) {
             const int n = S->n_mat;
             k_zero<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d);
             SolveOut od;
-            if ((rc = bicgstab(c, S->rr, S->d, 4, &od))) return rc;
+            if ((rc = bicgstab(c, S->rr, S->d, 4, false, &od))) return rc;
             total += od.iters;
             k_axpy1<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d, x);
             if ((rc = residual64(c, b, x, &rTr))) return rc;
