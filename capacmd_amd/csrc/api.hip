@@ -348,6 +348,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     }
     // cell list and neighbour rows (list radius rc + skin)
     c->skin_req = CAPAC_DEFAULT_SKIN;
+    c->solve_crit = CAPAC_DEFAULT_SOLVE_CRIT;
     c->nb_dirty = 1;
     TRY(nb_set_radius(c, c->skin_req));
     TRY(alloc_cells(c));
@@ -510,6 +511,16 @@ extern "C" int capac_set_skin(capac_ctx* c, double skin) {
 }
 
 extern "C" double capac_skin(const capac_ctx* c) { return c ? c->skin : -1.0; }
+
+extern "C" int capac_set_solve_tolerance(capac_ctx* c, double rtr_over_btb) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!(rtr_over_btb > 0.0) || !(rtr_over_btb <= 1.0)) {
+        capac_set_error("solve tolerance must be in (0, 1]");
+        return CAPAC_ERR_ARG;
+    }
+    c->solve_crit = rtr_over_btb;
+    return 0;
+}
 
 // ------------------------------------------------------------------ force
 

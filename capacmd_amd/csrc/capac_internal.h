@@ -125,6 +125,7 @@ struct capac_ctx {
     size_t relay_bytes;
     Solver* solver;
     int solve_iters, solve_kind, solve_refine_rounds;
+    double solve_crit;          // stop when rTr < solve_crit bTb (the reference: 1e-8)
     double solve_rTr, solve_bTb;
 
     // MD scalars (host copy of the small device state)

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kInitThreads)
 k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __restrict__ Ax,
             const double* __restrict__ diag, double* __restrict__ r0, double* __restrict__ r,
             double* __restrict__ p, double* __restrict__ v, double* __restrict__ K,
-            double* __restrict__ y, SolveScalars* sc) {
+            double* __restrict__ y, SolveScalars* sc, double crit_factor) {
     __shared__ double s_w[16];
     double bb = 0.0, rr = 0.0;
     for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
@@ -259,7 +259,7 @@ k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __res
         }
     if (threadIdx.x == 0) {
         sc->bTb = bTb;
-        sc->criteria = bTb * 1.0e-08;
+        sc->criteria = bTb * crit_factor;          // 1e-8 in the reference
         sc->rho_s[0] = 1.0;
         sc->rho_s[1] = rho;              // rho of iteration 1
         sc->alpha = 1.0;
@@ -488,7 +488,7 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
     hipStream_t st = c->stream;
     spmv(c, x, S->Ax, 0, NULL, NULL);
     k_bicg_init<<<1, kInitThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
-                                            S->y, S->sc);
+                                            S->y, S->sc, c->solve_crit);
     // K1, K2 of iteration 1, then units (K3, K4 of n; K1, K2 of n+1): each
     // unit ends with a stop test, so a batch of units can end the solve
     spmv(c, S->y, S->v, 1, S->part1, S->sc);
@@ -566,7 +566,7 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
     if (refine && (o.kind == CAPAC_SOLVE_CONVERGED || o.kind == CAPAC_SOLVE_LOOSENED)) {
         // FP64 refinement: the reference's criterion on the true residual,
         // already computed at the end of the last batch
-        const double crit = (o.kind == CAPAC_SOLVE_LOOSENED ? 1.0e-6 : 1.0e-8) * o.bTb;
+        const double crit = (o.kind == CAPAC_SOLVE_LOOSENED ? 100.0 : 1.0) * c->solve_crit * o.bTb;
         double rTr = S->res_host[0];
         int total = o.iters, rounds = 0;
         while (!(rTr < crit) && rounds < 4) {

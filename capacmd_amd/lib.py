@@ -164,7 +164,7 @@ GPU_EXPORTS = [
     "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_force",
     "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
-    "capac_time_kernel", "capac_set_skin", "capac_skin",
+    "capac_time_kernel", "capac_set_skin", "capac_skin", "capac_set_solve_tolerance",
 ]
 
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64 = range(6)
@@ -206,6 +206,7 @@ def gpu_lib() -> C.CDLL:
         lib.capac_set_skin.argtypes = [C.c_void_p, C.c_double]
         lib.capac_skin.argtypes = [C.c_void_p]
         lib.capac_skin.restype = C.c_double
+        lib.capac_set_solve_tolerance.argtypes = [C.c_void_p, C.c_double]
         _gpu = lib
     return _gpu
 
@@ -266,6 +267,10 @@ class Context:
     def set_skin(self, skin: float):
         """Verlet skin of the neighbour rows in nm (0: rebuild every call)."""
         _check(self.lib.capac_set_skin(self.h, float(skin)), "capac_set_skin")
+
+    def set_solve_tolerance(self, crit: float):
+        """BiCGSTAB stop test rTr < crit bTb (the reference's crit is 1e-8)."""
+        _check(self.lib.capac_set_solve_tolerance(self.h, float(crit)), "capac_set_solve_tolerance")
 
     @property
     def skin(self) -> float:

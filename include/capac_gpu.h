@@ -58,6 +58,9 @@ extern "C" {
 /* Verlet skin of the neighbour rows (nm), see capac_set_skin */
 #define CAPAC_DEFAULT_SKIN 0.1
 
+/* BiCGSTAB stop test rTr < crit bTb; the reference's crit (bicg_stab.c:303-306) */
+#define CAPAC_DEFAULT_SOLVE_CRIT 1.0e-8
+
 /* solver exit kinds (bicg_stab.c:300,368,539-571) */
 #define CAPAC_SOLVE_CONVERGED   0   /* rTr < 1e-8 bTb */
 #define CAPAC_SOLVE_LOOSENED    1   /* iter > 1000 and rTr < 1e-6 bTb */
@@ -164,6 +167,12 @@ int capac_set_box(capac_ctx* ctx, const double box[3]);
  * capac_skin returns the skin in use. */
 int capac_set_skin(capac_ctx* ctx, double skin);
 double capac_skin(const capac_ctx* ctx);
+
+/* Stop test of the CPIM solve, rTr < crit bTb (and the 1000-iteration
+ * loosening at 100 crit); default CAPAC_DEFAULT_SOLVE_CRIT, the reference's
+ * fixed value.  No counterpart in the reference: it exists so that accuracy
+ * studies can solve both precisions far below the reference's tolerance. */
+int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 
 /* One force evaluation = mpi_force (force.c:2097-2320): zeroes, pair forces,
  * Sutton-Chen, CPIM right-hand side, relay assembly, BiCGSTAB (warm start
