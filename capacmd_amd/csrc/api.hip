@@ -349,6 +349,16 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     // cell list and neighbour rows (list radius rc + skin)
     c->skin_req = CAPAC_DEFAULT_SKIN;
     c->solve_crit = CAPAC_DEFAULT_SOLVE_CRIT;
+    {
+        // fitting parameters start at the deck's values, widths derived as the reference does
+        const double pi = 3.14159265358979323846;
+        c->theta[0] = p->cpff_polar;
+        c->theta[1] = p->cpff_capac;
+        c->theta[2] = sqrt(2.0 / pi) * p->cpff_capac;
+        c->theta[3] = pow(sqrt(2.0 / pi) * p->cpff_polar / 3.0, 1.0 / 3.0);
+        c->theta[4] = p->qsc_eps;
+        c->theta[5] = p->qsc_c;
+    }
     c->nb_dirty = 1;
     TRY(nb_set_radius(c, c->skin_req));
     TRY(alloc_cells(c));
@@ -439,6 +449,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     if (c->red_host) (void)hipHostFree(c->red_host);
     solver_destroy(c);
+    fit_destroy(c);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -511,6 +522,30 @@ extern "C" int capac_set_skin(capac_ctx* c, double skin) {
 }
 
 extern "C" double capac_skin(const capac_ctx* c) { return c ? c->skin : -1.0; }
+
+extern "C" int capac_set_theta(capac_ctx* c, const double theta[CAPAC_NTHETA]) {
+    if (!c || !theta) return CAPAC_ERR_ARG;
+    for (int k = 0; k < CAPAC_NTHETA; ++k)
+        if (!(theta[k] > 0.0) || !(theta[k] < 1e300)) {
+            capac_set_error("theta[%d] = %g must be positive and finite", k, theta[k]);
+            return CAPAC_ERR_ARG;
+        }
+    memcpy(c->theta, theta, sizeof(double) * CAPAC_NTHETA);
+    fit_apply_theta(c);
+    return 0;
+}
+
+extern "C" int capac_get_theta(capac_ctx* c, double theta[CAPAC_NTHETA]) {
+    if (!c || !theta) return CAPAC_ERR_ARG;
+    memcpy(theta, c->theta, sizeof(double) * CAPAC_NTHETA);
+    return 0;
+}
+
+extern "C" int capac_fit_frame(capac_ctx* c, const double* f_ref, double e_ref, double w_e, double* loss,
+                               double grad[CAPAC_NTHETA]) {
+    if (!c || !f_ref) return CAPAC_ERR_ARG;
+    return fit_frame_loss_grad(c, f_ref, e_ref, w_e, loss, grad);
+}
 
 extern "C" int capac_set_solve_tolerance(capac_ctx* c, double rtr_over_btb) {
     if (!c) return CAPAC_ERR_ARG;

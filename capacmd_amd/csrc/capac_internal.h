@@ -126,6 +126,11 @@ struct capac_ctx {
     Solver* solver;
     int solve_iters, solve_kind, solve_refine_rounds;
     double solve_crit;          // stop when rTr < solve_crit bTb (the reference: 1e-8)
+
+    // fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC) and work space
+    double theta[CAPAC_NTHETA];
+    void* fit;
+    int fit_adjoint_iters;
     double solve_rTr, solve_bTb;
 
     // MD scalars (host copy of the small device state)
@@ -198,6 +203,11 @@ void solver_destroy(capac_ctx* c);
 int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
 int cpim_mm_only(capac_ctx* c);
+// fit.hip
+void fit_destroy(capac_ctx* c);
+void fit_apply_theta(capac_ctx* c);
+int fit_frame_loss_grad(capac_ctx* c, const double* f_ref, double e_ref, double w_e, double* loss,
+                        double* grad);
 // md.hip
 int md_kick(capac_ctx* c);
 int md_drift(capac_ctx* c);

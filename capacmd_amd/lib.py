@@ -165,7 +165,9 @@ GPU_EXPORTS = [
     "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
     "capac_time_kernel", "capac_set_skin", "capac_skin", "capac_set_solve_tolerance",
+    "capac_set_theta", "capac_get_theta", "capac_fit_frame",
 ]
+NTHETA = 6
 
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64 = range(6)
 DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
@@ -207,6 +209,9 @@ def gpu_lib() -> C.CDLL:
         lib.capac_skin.argtypes = [C.c_void_p]
         lib.capac_skin.restype = C.c_double
         lib.capac_set_solve_tolerance.argtypes = [C.c_void_p, C.c_double]
+        lib.capac_set_theta.argtypes = [C.c_void_p, _dp]
+        lib.capac_get_theta.argtypes = [C.c_void_p, _dp]
+        lib.capac_fit_frame.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, _dp]
         _gpu = lib
     return _gpu
 
@@ -271,6 +276,27 @@ class Context:
     def set_solve_tolerance(self, crit: float):
         """BiCGSTAB stop test rTr < crit bTb (the reference's crit is 1e-8)."""
         _check(self.lib.capac_set_solve_tolerance(self.h, float(crit)), "capac_set_solve_tolerance")
+
+    def set_theta(self, theta):
+        """Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC); see capac_set_theta."""
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        assert t.shape == (NTHETA,)
+        _check(self.lib.capac_set_theta(self.h, dptr(t)), "capac_set_theta")
+
+    def get_theta(self) -> np.ndarray:
+        t = np.empty(NTHETA)
+        _check(self.lib.capac_get_theta(self.h, dptr(t)), "capac_get_theta")
+        return t
+
+    def fit_frame(self, f_ref, e_ref: float, w_e: float, grad: bool = True):
+        """Loss of one frame at the resident state and, with grad, dL/dtheta."""
+        f = np.ascontiguousarray(f_ref, dtype=np.float64)
+        assert f.shape == (3 * self.n_metal,)
+        loss = C.c_double()
+        g = np.zeros(NTHETA) if grad else None
+        _check(self.lib.capac_fit_frame(self.h, dptr(f), float(e_ref), float(w_e), C.byref(loss),
+                                        dptr(g)), "capac_fit_frame")
+        return loss.value, g
 
     @property
     def skin(self) -> float:

@@ -174,6 +174,29 @@ double capac_skin(const capac_ctx* ctx);
  * studies can solve both precisions far below the reference's tolerance. */
 int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 
+/* ---- fitting (no counterpart in the reference) ----
+ * theta = (alpha [bohr^3], c [bohr], R_q [bohr], R_p [bohr], eps_SC [kJ/mol],
+ * c_SC): CPIM polarizability and capacitance, the two Gaussian widths as
+ * free parameters (the reference derives them from alpha and c,
+ * mpi-main.c:318-323; capac_create starts from those derived values) and
+ * the Sutton-Chen energy scale and embedding constant.  Setting theta
+ * changes every later force call. */
+#define CAPAC_NTHETA 6
+int capac_set_theta(capac_ctx* ctx, const double theta[CAPAC_NTHETA]);
+int capac_get_theta(capac_ctx* ctx, double theta[CAPAC_NTHETA]);
+
+/* One frame of the force-matching loss at the resident state (positions and
+ * the warm-start vec_pq set by the caller):
+ *   L = sum_{metal i} |F_i - f_ref_i|^2 + w_e (E - e_ref)^2,
+ * F from one capac_force call, E = sum of pot[1..14], f_ref 3 N_m SoA
+ * (x block, y block, z block).  grad (nullable) receives dL/dtheta from the
+ * adjoint method: A lambda = dL/dx solved with the BiCGSTAB kernels (A is
+ * symmetric), forward-mode derivative kernels that recompute the T-tensors
+ * for dA/dtheta, dF/dtheta and dE/dtheta, and dL/dx from the derivative of
+ * dE/dx along G = 2 (F - f_ref) (the CPIM forces are -dE/dr at fixed x). */
+int capac_fit_frame(capac_ctx* ctx, const double* f_ref, double e_ref, double w_e, double* loss,
+                    double grad[CAPAC_NTHETA]);
+
 /* One force evaluation = mpi_force (force.c:2097-2320): zeroes, pair forces,
  * Sutton-Chen, CPIM right-hand side, relay assembly, BiCGSTAB (warm start
  * from the resident vec_pq), CPIM forces and the external field.  pot[0] is

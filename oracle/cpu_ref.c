@@ -172,6 +172,19 @@ void oref_get_state(const oref* o, double* r, double* v, double* x, double* f) {
 
 void oref_set_int_pair_count(oref* o, int on) { o->int_pair_count = on; }
 
+/* Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC) with the widths
+ * free instead of derived from alpha and c (mpi-main.c:318-323); the
+ * checker for the GPU's gradients evaluates its loss through this. */
+void oref_set_theta(oref* o, const double* t) {
+    o->inv_polar = 1.0 / t[0];
+    o->inv_capac = 1.0 / t[1];
+    o->inv_R_qq = 1.0 / sqrt(t[2] * t[2] + t[2] * t[2]);
+    o->inv_R_pq = 1.0 / sqrt(t[3] * t[3] + t[2] * t[2]);
+    o->inv_R_pp = 1.0 / sqrt(t[3] * t[3] + t[3] * t[3]);
+    o->p.qsc_eps = t[4];
+    o->p.qsc_c = t[5];
+}
+
 /* last pair index of rank 0 of 1 with the reference's int product
  * nAtoms*(nAtoms-1)/2 (mpi-main.c:406), which wraps above 46,341 atoms */
 long oref_reference_pair_end(int n) {

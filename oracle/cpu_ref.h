@@ -28,6 +28,8 @@ void oref_get_state(const oref* o, double* r, double* v, double* vec_pq, double*
 /* Emulates the reference's 32-bit pair count (mpi-main.c:406, force.c:1421,
  * 1442) when on; off computes every pair. */
 void oref_set_int_pair_count(oref* o, int on);
+/* fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC); widths free */
+void oref_set_theta(oref* o, const double* theta6);
 long oref_reference_pair_end(int n_atoms);
 
 /* phases; f/pot/vir accumulate (callers zero them) */

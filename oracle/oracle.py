@@ -33,6 +33,7 @@ def lib(omp: bool = False) -> C.CDLL:
         L.oref_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.oref_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.oref_set_int_pair_count.argtypes = [C.c_void_p, C.c_int]
+        L.oref_set_theta.argtypes = [C.c_void_p, _dp]
         L.oref_reference_pair_end.argtypes = [C.c_int]
         L.oref_reference_pair_end.restype = C.c_long
         L.oref_nonb.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
@@ -97,6 +98,11 @@ class Oracle:
 
     def int_pair_count(self, on: bool):
         self.L.oref_set_int_pair_count(self.h, int(on))
+
+    def set_theta(self, theta):
+        """Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC), widths free."""
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        self.L.oref_set_theta(self.h, t.ctypes.data_as(_dp))
 
     def nonb(self, r):
         f = np.zeros(3 * self.N); pot = np.zeros(15); vir = np.zeros(9)
