@@ -93,3 +93,90 @@ class CpimFitLoss(torch.autograd.Function):
     def backward(ctx, grad_out):
         (g,) = ctx.saved_tensors
         return grad_out * g.to(grad_out.device), None
+
+
+def main(argv=None) -> int:
+    """Fitting workflow on the Au309 deck: frames from an FP64 run, the
+    parameters moved off the deck's values, then L-BFGS on the GPU loss and
+    its adjoint gradient.  Prints one JSON line."""
+    import argparse
+    import json
+    import sys
+    import tempfile
+    import time
+
+    from . import gen
+
+    ap = argparse.ArgumentParser(description=main.__doc__.splitlines()[0])
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--stride", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=6, help="L-BFGS iterations")
+    ap.add_argument("--solve-crit", type=float, default=1e-14)
+    ap.add_argument("--w-e", type=float, default=1e-2)
+    ap.add_argument("--fd-check", action="store_true",
+                    help="central differences of the loss (h = 1e-5 theta_k) at the start")
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
+
+    spec = gen.config(a.config)
+    tmp = tempfile.mkdtemp(prefix="capac_fit_")
+    gen.write_deck(spec, tmp)
+    deck = lib.Deck.from_dir(tmp)
+    log(f"deck {spec.name}: N={deck.n_atoms} Nm={deck.n_metal}")
+    t0 = time.perf_counter()
+    frames = make_frames(deck, a.frames, a.stride)
+    t_frames = time.perf_counter() - t0
+    log(f"{len(frames)} frames in {t_frames:.1f} s")
+    h = Harness(deck, frames, w_e=a.w_e, solve_crit=a.solve_crit)
+    perturb = np.array([0.03, -0.02, 0.04, -0.03, 0.02, -0.04])
+    theta0 = h.theta0.copy()
+    start = theta0 * (1.0 + perturb)
+    t0 = time.perf_counter()
+    loss0, g0 = h.loss_grad(start)
+    t_eval = time.perf_counter() - t0
+    out = {"workload": f"{spec.name}: {deck.n_metal} Au, {deck.n_atoms} atoms, {len(frames)} frames "
+                       f"every {a.stride} steps", "solve_crit": a.solve_crit,
+           "seconds_per_loss_and_gradient": round(t_eval, 4),
+           "ms_per_frame": round(1e3 * t_eval / len(frames), 3),
+           "loss_start": loss0, "theta_names": list(THETA_NAMES),
+           "theta_reference": theta0.tolist(), "theta_start": start.tolist()}
+    if a.fd_check:
+        fd = np.zeros(len(start))
+        for k in range(len(start)):
+            hk = 1e-5 * start[k]
+            tp, tm = start.copy(), start.copy()
+            tp[k] += hk
+            tm[k] -= hk
+            fd[k] = (h.loss_grad(tp, grad=False)[0] - h.loss_grad(tm, grad=False)[0]) / (2 * hk)
+        floor = 1e-3 * np.abs(fd).max()
+        out["fd_check"] = {"adjoint": g0.tolist(), "central_difference": fd.tolist(),
+                           "max_rel": float((np.abs(g0 - fd) / np.maximum(np.abs(fd), floor)).max())}
+    # L-BFGS in scaled coordinates theta = theta_ref_scale * u
+    scale = torch.tensor(start, dtype=torch.float64)
+    u = torch.ones(len(start), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([u], lr=1.0, max_iter=a.steps, line_search_fn="strong_wolfe")
+    history = []
+
+    def closure():
+        opt.zero_grad()
+        loss = CpimFitLoss.apply(u * scale, h)
+        loss.backward()
+        history.append(loss.item())
+        return loss
+
+    t0 = time.perf_counter()
+    opt.step(closure)
+    t_fit = time.perf_counter() - t0
+    theta = (u.detach() * scale).numpy()
+    out.update({"lbfgs_evaluations": len(history), "loss_history": history, "fit_seconds": round(t_fit, 3),
+                "theta_fit": theta.tolist(),
+                "theta_rel_error_start": (np.abs(start - theta0) / theta0).tolist(),
+                "theta_rel_error_fit": (np.abs(theta - theta0) / theta0).tolist()})
+    h.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
