@@ -121,8 +121,9 @@ def run_gpu(deck, precision: int, steps: int, warmup: int):
                ("cpim_mm", lib.KERNEL_CPIM_MM)]
     if precision == lib.PREC_FP32_REFINED:
         kernels.append(("apply64", lib.KERNEL_APPLY64))
+    kernels.append(("solve", lib.KERNEL_SOLVE))       # last: from the last MD step's warm start
     for name, kid in kernels:
-        kern[name] = ctx.time_kernel(kid, 20)
+        kern[name] = ctx.time_kernel(kid, 10 if kid == lib.KERNEL_SOLVE else 20)
     out["kernel_ms"] = kern
     ctx.close()
     return out
@@ -286,13 +287,15 @@ def main() -> int:
     speedup = None
     err32 = None
     if "fp32" in paths and "fp64" in paths:
-        speedup = round(paths["fp64"]["solve_ms_per_step"] / paths["fp32"]["solve_ms_per_step"], 3)
+        # the solve alone (the reference's timer category 7), from the last MD
+        # step's system and warm start; in MD the pair forces overlap it
+        speedup = round(paths["fp64"]["kernel_ms"]["solve"] / paths["fp32"]["kernel_ms"]["solve"], 3)
         # at the reference's stop test, and with both solves taken far below
         # it, so that what remains is the FP32 relay's own error
         err32 = {"note": "FP32-refined vs FP64 path from identical states and warm starts, max "
                          "over states; solve_crit is the stop test rTr < crit bTb",
                  "reference_tolerance": None, "tight_tolerance": None}
-        for key, crit in (("reference_tolerance", 1e-8), ("tight_tolerance", 1e-12)):
+        for key, crit in (("reference_tolerance", 1e-8), ("tight_tolerance", 1e-11)):
             try:
                 err32[key] = fp32_error(deck, crit=crit)
             except lib.CapacError as e:
@@ -323,6 +326,7 @@ def main() -> int:
                                f"{deck_spec.box[0]:g} nm box, {n_mat} CPIM unknowns, MD step dt 1 fs",
                    "atoms": deck.n_atoms, "path": best},
         "solve_ms_per_step": round(main_path["solve_ms_per_step"], 4),
+        "solve_ms_alone": round(main_path["kernel_ms"]["solve"], 4),
         "solve_speedup_fp32_refined_vs_fp64": speedup,
         "fp32_refined_error": err32,
         "paths": paths,

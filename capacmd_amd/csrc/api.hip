@@ -297,6 +297,9 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     rc = 0;
 #define TRY(x) do { if ((rc = (x)) != 0) { capac_destroy(c); return rc; } } while (0)
     TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+    TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+    TRY(hipEventCreateWithFlags(&c->ev_rows, hipEventDisableTiming) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+    TRY(hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming) == hipSuccess ? 0 : CAPAC_ERR_HIP);
     size_t n3 = 3 * (size_t)N;
     TRY(upload(&c->r, (const double*)NULL, n3));
     TRY(upload(&c->v, (const double*)NULL, n3));
@@ -452,6 +455,10 @@ extern "C" void capac_destroy(capac_ctx* c) {
     fit_destroy(c);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_rows) (void)hipEventDestroy(c->ev_rows);
+    if (c->ev_side) (void)hipEventDestroy(c->ev_side);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     free(c);
 }
@@ -581,9 +588,30 @@ static void mark(capac_ctx* c, int k) {
     if (c->timing_on) (void)hipEventRecord(c->ev[k], c->stream);
 }
 
+// pair and Sutton-Chen forces on the side stream, after the rows are built
+// (event ev_rows); they write f first, so everything that accumulates into f
+// afterwards waits for ev_side.  Their reductions are queued here and
+// launched on the main stream once it has waited.
+static int side_forces(capac_ctx* c) {
+    int rc = flush_reductions(c);
+    if (rc) return rc;
+    CHECK_HIP(hipEventRecord(c->ev_rows, c->stream));
+    CHECK_HIP(hipStreamWaitEvent(c->side, c->ev_rows, 0));
+    hipStream_t main_stream = c->stream;
+    c->stream = c->side;
+    mark(c, 8);
+    rc = pair_nonbonded(c, NULL);
+    mark(c, 9);
+    if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
+    mark(c, 10);
+    c->stream = main_stream;
+    if (rc) return rc;
+    CHECK_HIP(hipEventRecord(c->ev_side, c->side));
+    return 0;
+}
+
 // one mpi_force: forces into c->f, pot/vir into c->pot/c->vir
 static int force_compute(capac_ctx* c) {
-    const int Nm = c->bc.Nm;
     int rc;
     for (int attempt = 0; attempt < 8; ++attempt) {
         mark(c, 0);
@@ -592,24 +620,25 @@ static int force_compute(capac_ctx* c) {
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
         c->nb_dirty = 0;
         mark(c, 1);
-        if ((rc = pair_nonbonded(c, NULL))) return rc;
-        mark(c, 2);
-        if (Nm > 0 && (rc = sc_forces(c))) return rc;
-        mark(c, 3);
+        if ((rc = side_forces(c))) return rc;
+        int solve_rc = 0;
         if (c->n_mat) {
             if ((rc = cpim_rhs(c))) return rc;
             mark(c, 4);
             if ((rc = cpim_assemble(c))) return rc;
             mark(c, 5);
-            rc = solver_run(c, c->b, c->x);
-            if (rc && rc != CAPAC_ERR_SOLVER) return rc;
-            int solve_rc = rc;
-            mark(c, 6);
-            if ((rc = cpim_forces(c))) return rc;
-            rc = solve_rc;
+            solve_rc = solver_run(c, c->b, c->x);
+            if (solve_rc && solve_rc != CAPAC_ERR_SOLVER) {
+                (void)hipStreamSynchronize(c->side);
+                return solve_rc;
+            }
         } else {
-            mark(c, 4); mark(c, 5); mark(c, 6);
+            mark(c, 4); mark(c, 5);
         }
+        mark(c, 6);
+        CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_side, 0));
+        if (c->n_mat && (rc = cpim_forces(c))) return rc;
+        rc = solve_rc;
         int rc2 = ext_field(c);
         if (rc2) return rc2;
         mark(c, 7);
@@ -638,10 +667,12 @@ static int force_compute(capac_ctx* c) {
     c->timing.n_force_calls++;
     if (c->timing_on) {
         float ms;
-        const int cat[7] = {10, 4, 2, 6, 5, 7, 8};   // build, pair, SC, RHS, matrix, solve, CPIM force
+        // (from, to, category): build, RHS, matrix, solve, CPIM force on the
+        // main stream; pair and SC on the side stream, overlapping the solve
+        const int spans[7][3] = {{0, 1, 10}, {1, 4, 6}, {4, 5, 5}, {5, 6, 7}, {6, 7, 8}, {8, 9, 4}, {9, 10, 2}};
         for (int k = 0; k < 7; ++k)
-            if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess)
-                c->timing.t[cat[k]] += 1e-3 * ms;
+            if (hipEventElapsedTime(&ms, c->ev[spans[k][0]], c->ev[spans[k][1]]) == hipSuccess)
+                c->timing.t[spans[k][2]] += 1e-3 * ms;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->timing.t[0] += 1e-3 * ms;
     }
     return rc;
@@ -949,14 +980,14 @@ extern "C" int capac_set_thermostat(capac_ctx* c, const double* vq, double vp) {
 extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms) {
     if (!c || reps <= 0 || !ms) return CAPAC_ERR_ARG;
     if ((kernel == CAPAC_KERNEL_SPMV || kernel == CAPAC_KERNEL_ASSEMBLE || kernel == CAPAC_KERNEL_CPIM_MM ||
-         kernel == CAPAC_KERNEL_APPLY64) &&
+         kernel == CAPAC_KERNEL_APPLY64 || kernel == CAPAC_KERNEL_SOLVE) &&
         !c->n_mat) {
         capac_set_error("no CPIM unknowns");
         return CAPAC_ERR_ARG;
     }
-    // the SpMV writes into the solver's scratch vector, the others into their
-    // usual outputs; positions are not changed
-    double* scratch = c->x_backup;
+    // the SpMV and the FP64 apply write into a solver work vector, the others
+    // into their usual outputs; positions, b and the warm start are kept
+    double* scratch = c->n_mat ? solver_scratch(c) : NULL;
     CHECK_HIP(hipStreamSynchronize(c->stream));
     CHECK_HIP(hipEventRecord(c->ev[14], c->stream));
     for (int k = 0; k < reps; ++k) {
@@ -968,6 +999,15 @@ extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms)
             case CAPAC_KERNEL_NEIGHBOUR: rc = nb_build(c); break;
             case CAPAC_KERNEL_CPIM_MM: rc = cpim_mm_only(c); break;
             case CAPAC_KERNEL_APPLY64: rc = cpim_apply64(c, c->x, NULL, scratch, NULL, NULL); break;
+            case CAPAC_KERNEL_SOLVE:
+                // the last force call's system from its warm start (kept in
+                // x_backup); the result lands in x as it did then
+                rc = copy_d(c, c->x_backup, c->x, c->n_mat);
+                if (!rc) {
+                    rc = solver_run(c, c->b, c->x);
+                    if (rc == CAPAC_ERR_SOLVER) rc = 0;
+                }
+                break;
             default: capac_set_error("unknown kernel %d", kernel); return CAPAC_ERR_ARG;
         }
         if (rc) return rc;

@@ -145,6 +145,10 @@ struct capac_ctx {
     // timing
     int timing_on;
     hipEvent_t ev[16];
+    // second stream (default priority): pair and Sutton-Chen forces run on it
+    // while the main stream assembles and solves the CPIM system
+    hipStream_t side;
+    hipEvent_t ev_rows, ev_side;
     capac_timing timing;
 };
 
@@ -202,6 +206,7 @@ int solver_create(capac_ctx* c);
 void solver_destroy(capac_ctx* c);
 int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
+double* solver_scratch(capac_ctx* c);
 int cpim_mm_only(capac_ctx* c);
 // fit.hip
 void fit_destroy(capac_ctx* c);

@@ -592,6 +592,9 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
     return 0;
 }
 
+// a solver work vector that kernel timing may overwrite (rebuilt by every solve)
+double* solver_scratch(capac_ctx* c) { return c->solver->Ax; }
+
 // one plain SpMV over the resident matrix (for kernel timing)
 int solver_spmv_only(capac_ctx* c, const double* x, double* y) {
     spmv(c, x, y, 0, NULL, NULL);

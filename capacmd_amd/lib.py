@@ -169,7 +169,7 @@ GPU_EXPORTS = [
 ]
 NTHETA = 6
 
-KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64 = range(6)
+KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64, KERNEL_SOLVE = range(7)
 DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
 
 

@@ -278,6 +278,8 @@ int capac_set_thermostat(capac_ctx* ctx, const double* vq, double vp);
 #define CAPAC_KERNEL_NEIGHBOUR 3
 #define CAPAC_KERNEL_CPIM_MM   4
 #define CAPAC_KERNEL_APPLY64   5   /* FP64 relay product recomputed on the fly (refinement residual) */
+#define CAPAC_KERNEL_SOLVE     6   /* the whole CPIM solve of the last force call, from its warm start,
+                                      nothing else running: the reference's timer category 7 */
 int capac_time_kernel(capac_ctx* ctx, int kernel, int reps, double* ms_per_launch);
 
 int capac_get_timing(capac_ctx* ctx, capac_timing* t);
