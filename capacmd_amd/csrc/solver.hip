@@ -283,6 +283,11 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
     __shared__ double s_w[16 * 2];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
+    // this thread's row is loaded before the reductions (it does not depend
+    // on them), so the two memory round trips overlap
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double rk = 0.0, vk = 0.0, Kk = 0.0, r0k = 0.0;
+    if (k < n_mat) { rk = r[k]; vk = v[k]; Kk = K[k]; r0k = r0[k]; }
     __syncthreads();
     if (s_skip) return;
     double red[2] = {0.0, 0.0};          // (r0, v), (r, r) of iteration n-1
@@ -311,12 +316,11 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
     const double r0v = red[0];
     const double alpha = sc->rho_s[n & 1] / r0v;
     double acc = 0.0;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n_mat) {
-        double sk = r[k] - alpha * v[k];
+        double sk = rk - alpha * vk;
         s[k] = sk;
-        z[k] = sk * K[k];
-        acc = r0[k] * sk;
+        z[k] = sk * Kk;
+        acc = r0k * sk;
     }
     __syncthreads();                      // s_w is reused
     double r0s = block_sum(acc, s_w);
@@ -338,6 +342,12 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
     __shared__ double s_w[16 * 4];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
+    // this thread's row is loaded before the reductions (see k_vec_s)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double xk = 0.0, yk = 0.0, zk = 0.0, sk = 0.0, tk = 0.0, pk0 = 0.0, vk = 0.0, Kk = 0.0;
+    if (k < n_mat) {
+        xk = x[k]; yk = y[k]; zk = z[k]; sk = s[k]; tk = t[k]; pk0 = p[k]; vk = v[k]; Kk = K[k];
+    }
     __syncthreads();
     if (s_skip) return;
     double red[4] = {0.0, 0.0, 0.0, 0.0};   // (Kt, z), (Kt, Kt), (r0, t), (r0, s)
@@ -355,15 +365,14 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
     const double rho_next = r0s - omega * r0t;
     const double beta = (rho_next / rho) * (alpha / omega);
     double acc = 0.0;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n_mat) {
-        x[k] += alpha * y[k] + omega * z[k];
-        double rk = s[k] - omega * t[k];
+        x[k] = xk + (alpha * yk + omega * zk);
+        double rk = sk - omega * tk;
         r[k] = rk;
         acc = rk * rk;
-        double pk = rk + beta * (p[k] - omega * v[k]);
+        double pk = rk + beta * (pk0 - omega * vk);
         p[k] = pk;
-        y[k] = pk * K[k];
+        y[k] = pk * Kk;
     }
     __syncthreads();                      // s_w is reused
     double rr = block_sum(acc, s_w);
