@@ -326,6 +326,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
             t[k].c6 = p->nb_c6[k];
             t[k].c12 = p->nb_c12[k];
             t[k].width = p->nb_a[k];
+            t[k].inv_width = p->nb_a[k] != 0.0 ? 1.0 / p->nb_a[k] : 0.0;
         }
         rc = upload(&c->pair, (const PairParam*)t, nt * nt);
         free(t);

@@ -59,7 +59,7 @@ struct BoxConst {
 
 struct PairParam {          // one entry of the n_types x n_types table
     int funct;
-    double c6, c12, width;
+    double c6, c12, width, inv_width;
 };
 
 // BiCGSTAB scalars, resident on the device (bicg_stab.c:289-292)

@@ -18,24 +18,28 @@ struct PairOut {
     double fx, fy, fz, e7, e8, v[9];
 };
 
+// The reference's expressions with each division by r, r^2, r^6 or the erf
+// width replaced by a product with the reciprocal, computed once per pair
+// (one FP64 division instead of up to five; results move by rounding only).
 __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
                                           double dx, double dy, double dz, double r2, PairOut& o) {
     double flj = 0.0, fqq = 0.0;
-    double rij = sqrt(r2);
+    const double rij = sqrt(r2);
+    const double inv_r = 1.0 / rij;
+    const double inv_r2 = inv_r * inv_r;
     if (pp.funct == 1) {
         if (pp.c6 != 0.0 || pp.c12 != 0.0) {
-            double r6 = r2 * r2 * r2;
-            double r12 = r6 * r6;
-            double cr12 = pp.c12 / r12, cr6 = pp.c6 / r6;
+            const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+            const double cr12 = pp.c12 * (inv_r6 * inv_r6), cr6 = pp.c6 * inv_r6;
             if (bc.use_vdw == 1) {
                 // shifted force LJ (force.c:748-757)
                 double f_r = 12.0 * cr12 - 6.0 * cr6;
                 double f_c = 12.0 * pp.c12 * bc.inv_rc12 - 6.0 * pp.c6 * bc.inv_rc6;
-                flj = (f_r - f_c) / r2;
+                flj = (f_r - f_c) * inv_r2;
                 o.e8 += (cr12 - cr6) + (rij - bc.rc) * f_c / bc.rc -
                         (pp.c12 * bc.inv_rc12 - pp.c6 * bc.inv_rc6);
             } else {
-                flj = (2.0 * cr12 - cr6) * 6.0 / r2;
+                flj = (2.0 * cr12 - cr6) * 6.0 * inv_r2;
                 o.e8 += cr12 - cr6;
             }
         }
@@ -45,24 +49,23 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
                 // damped shifted force (force.c:787-795)
                 double erfc_a = erfc(bc.alpha * rij);
                 double ex = exp(-bc.alpha * bc.alpha * r2);
-                fqq = qq * ((erfc_a / r2 + bc.a2_sqrtPI * ex / rij) - bc.wolfConst) / rij;
-                o.e7 += qq * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+                fqq = qq * ((erfc_a * inv_r2 + bc.a2_sqrtPI * ex * inv_r) - bc.wolfConst) * inv_r;
+                o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
             } else {
-                fqq = qq / (r2 * rij);
-                o.e7 += qq / rij;
+                fqq = qq * inv_r2 * inv_r;
+                o.e7 += qq * inv_r;
             }
         }
     } else if (pp.funct == 4) {
         if (pp.c6 != 0.0 || pp.c12 != 0.0) {
             // erf-damped vdW (force.c:602-628)
-            double r6 = r2 * r2 * r2;
-            double r12 = r6 * r6;
-            double cr12 = pp.c12 / r12, cr6 = pp.c6 / r6;
-            double rR = rij / pp.width;
+            const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+            const double cr12 = pp.c12 * (inv_r6 * inv_r6), cr6 = pp.c6 * inv_r6;
+            double rR = rij * pp.inv_width;
             double erf_rR = erf(rR);
             o.e8 += (cr12 - cr6) * erf_rR;
-            flj = ((12.0 * cr12 - 6.0 * cr6) / rij * erf_rR -
-                   (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * exp(-rR * rR) / pp.width) / rij;
+            flj = ((12.0 * cr12 - 6.0 * cr6) * inv_r * erf_rR -
+                   (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * exp(-rR * rR) * pp.inv_width) * inv_r;
         }
     }
     double fs = flj + fqq;               // f_j = fs * r_ij, f_i = -f_j
