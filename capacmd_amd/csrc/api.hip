@@ -589,6 +589,15 @@ static void mark(capac_ctx* c, int k) {
     if (c->timing_on) (void)hipEventRecord(c->ev[k], c->stream);
 }
 
+// grid of the pair kernel while it runs beside the solve: half the compute
+// units for boxes of up to ~20k atoms (the kernel then still ends before the
+// solve), growing with N so that larger boxes, whose pair work outlasts a
+// capped grid, get more
+static int side_pair_blocks(const capac_ctx* c) {
+    int b = c->bc.N / 160;
+    return b < 128 ? 128 : b;
+}
+
 // pair and Sutton-Chen forces on the side stream, after the rows are built
 // (event ev_rows); they write f first, so everything that accumulates into f
 // afterwards waits for ev_side.  Their reductions are queued here and
@@ -601,7 +610,7 @@ static int side_forces(capac_ctx* c) {
     hipStream_t main_stream = c->stream;
     c->stream = c->side;
     mark(c, 8);
-    rc = pair_nonbonded(c, NULL);
+    rc = pair_nonbonded(c, NULL, side_pair_blocks(c));
     mark(c, 9);
     if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
     mark(c, 10);

@@ -186,7 +186,7 @@ int nb_build(capac_ctx* c);                 // unconditional build at rc + skin
 int nb_build_if_needed(capac_ctx* c);       // device-side decision from displacements
 int nb_set_radius(capac_ctx* c, double skin);
 // pair.hip
-int pair_nonbonded(capac_ctx* c, double* f_out_accumulate);
+int pair_nonbonded(capac_ctx* c, double* f_out_accumulate, int max_blocks = 0);
 // sc.hip
 int sc_forces(capac_ctx* c);
 // cpim.hip

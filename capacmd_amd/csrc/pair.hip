@@ -138,9 +138,12 @@ int pair_blocks(int N) {
     return nb < 2048 ? nb : 2048;
 }
 
-// writes (or adds to) f; red[0..11) = pot7, pot8, vir[9]
-int pair_nonbonded(capac_ctx* c, double* f_accumulate) {
+// writes (or adds to) f; red[0..11) = pot7, pot8, vir[9]; max_blocks > 0
+// caps the grid (the kernel strides over atoms), leaving compute units to
+// work that runs beside it
+int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks) {
     int nblk = pair_blocks(c->bc.N);
+    if (max_blocks > 0 && nblk > max_blocks) nblk = max_blocks;
     double* f = f_accumulate ? f_accumulate : c->f;
     k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
                                         c->n_back, c->nbr, c->nb_stride, f,
