@@ -454,6 +454,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
     if (c->red_host) (void)hipHostFree(c->red_host);
     solver_destroy(c);
     fit_destroy(c);
+    em_destroy(c);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -609,11 +610,11 @@ static int side_forces(capac_ctx* c) {
     CHECK_HIP(hipStreamWaitEvent(c->side, c->ev_rows, 0));
     hipStream_t main_stream = c->stream;
     c->stream = c->side;
-    mark(c, 8);
+    mark(c, 11);                     // events 11-13: the side stream's phases
     rc = pair_nonbonded(c, NULL, side_pair_blocks(c));
-    mark(c, 9);
+    mark(c, 12);
     if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
-    mark(c, 10);
+    mark(c, 13);
     c->stream = main_stream;
     if (rc) return rc;
     CHECK_HIP(hipEventRecord(c->ev_side, c->side));
@@ -679,7 +680,7 @@ static int force_compute(capac_ctx* c) {
         float ms;
         // (from, to, category): build, RHS, matrix, solve, CPIM force on the
         // main stream; pair and SC on the side stream, overlapping the solve
-        const int spans[7][3] = {{0, 1, 10}, {1, 4, 6}, {4, 5, 5}, {5, 6, 7}, {6, 7, 8}, {8, 9, 4}, {9, 10, 2}};
+        const int spans[7][3] = {{0, 1, 10}, {1, 4, 6}, {4, 5, 5}, {5, 6, 7}, {6, 7, 8}, {11, 12, 4}, {12, 13, 2}};
         for (int k = 0; k < 7; ++k)
             if (hipEventElapsedTime(&ms, c->ev[spans[k][0]], c->ev[spans[k][1]]) == hipSuccess)
                 c->timing.t[spans[k][2]] += 1e-3 * ms;
@@ -893,6 +894,45 @@ static int fetch_md(capac_ctx* c) {
                              c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// ------------------------------------------------------------------ minimisation
+
+extern "C" int capac_em_begin(capac_ctx* c) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = em_begin(c);
+    if (!rc) rc = fill_i(c, c->md_status, 4, 0);
+    if (!rc) rc = fetch_md(c);
+    return rc;
+}
+
+extern "C" int capac_em_step(capac_ctx* c, double length, double pot[CAPAC_NPOT], double* fmax, double* frms) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = em_move(c, length);
+    if (!rc) rc = md_rattle1(c);
+    if (!rc) rc = fill_d(c, c->v, 3 * (size_t)c->bc.N, 0.0);       // mpi-main.c:753-759
+    if (rc) return rc;
+    int frc = force_compute(c);
+    if (frc && frc != CAPAC_ERR_SOLVER) return frc;
+    rc = md_fmax(c);
+    if (!rc) rc = fetch_md(c);
+    if (rc) return rc;
+    const int* st = (const int*)(c->red_host + 292);
+    if (st[0]) { capac_set_error("RATTLE_1 not converged!"); return CAPAC_ERR_RATTLE1; }
+    if (pot) {
+        memcpy(pot, c->pot, sizeof c->pot);
+        double e = 0.0;
+        for (int k = 1; k < CAPAC_NPOT; ++k) e += pot[k];     // sum_potential (force.c:322-331)
+        pot[0] = e;
+    }
+    if (fmax) *fmax = c->red_host[260 + 23];
+    if (frms) *frms = c->red_host[260 + 24];
+    return frc;
+}
+
+extern "C" int capac_em_direction(capac_ctx* c, int cg) {
+    if (!c) return CAPAC_ERR_ARG;
+    return em_direction(c, cg);
 }
 
 extern "C" int capac_md_begin(capac_ctx* c) {

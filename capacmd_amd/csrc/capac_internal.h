@@ -130,6 +130,7 @@ struct capac_ctx {
     // fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC) and work space
     double theta[CAPAC_NTHETA];
     void* fit;
+    void* em;                   // minimiser work space (em.hip)
     int fit_adjoint_iters;
     double solve_rTr, solve_bTb;
 
@@ -208,6 +209,11 @@ int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
 double* solver_scratch(capac_ctx* c);
 int cpim_mm_only(capac_ctx* c);
+// em.hip
+void em_destroy(capac_ctx* c);
+int em_begin(capac_ctx* c);
+int em_move(capac_ctx* c, double length);
+int em_direction(capac_ctx* c, int cg);
 // fit.hip
 void fit_destroy(capac_ctx* c);
 void fit_apply_theta(capac_ctx* c);

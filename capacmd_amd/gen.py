@@ -160,6 +160,10 @@ class Deck:
     ensemble: str = "nvt"
     coulomb_alpha: float = 2.0
     rcut: float = 1.2
+    run_type: str = "md"            # md, em (steepest descent) or cg
+    em_steps: int = 100
+    em_length: float = 0.0001       # nm
+    em_tol: float = 1000.0          # kJ/mol/nm
 
     @property
     def n_metal(self) -> int:
@@ -295,6 +299,14 @@ def small(kind: str) -> Deck:
                          save_step=1, with_water=False)
     if kind == "water_only":
         return make_deck("water_only", [2.5] * 3, [], 10, 105, save_step=1)
+    if kind == "au13_em":
+        return make_deck("au13_em", [2.6] * 3, [(2, [1.3] * 3)], 10, 101, save_step=1,
+                         run_type="em", em_steps=30, em_length=0.002, em_tol=2000.0)
+    if kind == "2au13_cg":
+        return make_deck("2au13_cg", [4.4, 2.6, 2.6],
+                         [(2, [1.3, 1.3, 1.3]), (2, [3.1, 1.3, 1.3])], 10, 103,
+                         save_step=1, efield=(0.5, 0.0, 0.0), fix_metal=1,
+                         run_type="cg", em_steps=15, em_length=0.002, em_tol=2000.0)
     raise ValueError(kind)
 
 
@@ -377,12 +389,12 @@ def write_param(deck: Deck, path: str) -> None:
 
 def write_mdset(deck: Deck, path: str) -> None:
     out = [
-        "run_type          md",
+        "run_type          %s" % deck.run_type,
         "ensemble          %s" % deck.ensemble,
         "",
-        "em_steps          100",
-        "em_length_in_nm   0.0001",
-        "em_tolerance      1000.0",
+        "em_steps          %d" % deck.em_steps,
+        "em_length_in_nm   %g" % deck.em_length,
+        "em_tolerance      %r" % deck.em_tol,
         "",
         "number_of_steps   %d" % deck.n_steps,
         "time_step_in_ps   %g" % deck.dt,

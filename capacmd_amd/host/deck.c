@@ -457,6 +457,15 @@ void capac_deck_run(const capac_deck* d, int* n_steps, int* n_save, double* vq0,
     if (vq0) *vq0 = d->vq0;
     if (vp) *vp = d->vp;
 }
+int capac_deck_minimiser(const capac_deck* d, int* em_steps, double* em_length, double* em_tol) {
+    if (em_steps) *em_steps = d->em_steps;
+    if (em_length) *em_length = d->em_length;
+    if (em_tol) *em_tol = d->em_tol;
+    if (!strcmp(d->run_type, "md")) return 0;
+    if (!strcmp(d->run_type, "em")) return 1;
+    if (!strcmp(d->run_type, "cg")) return 2;
+    return -1;
+}
 int capac_params_size(void) { return (int)sizeof(capac_params); }
 
 /* ------------------------------------------------------------ writers */

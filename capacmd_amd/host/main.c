@@ -4,12 +4,14 @@
  *   capacmd -gro init.gro -par param.txt -mds mdset.txt [-prec fp64|fp32] [-dev N]
  *
  * Reads the three inputs with the reference's grammar, runs the force call
- * and the MD loop on the device, and writes traj.gro (every 10 save steps),
- * vec_pq.txt (with traj.gro), traj.dat (every save step) and the stdout step
- * lines, final energy table and phase timings in the reference's formats.
- * Failures print the reference's message and exit(1), as the reference does.
- * run_type em/cg (energy minimisation) is not part of this build.
+ * and then, by run_type, the MD loop (md: traj.gro and vec_pq.txt every 10
+ * save steps, traj.dat every save step) or the minimiser (em: steepest
+ * descent, cg: conjugate gradients, mpi-main.c:706-865: all three files every
+ * step) on the device, and writes the stdout step lines, final energy table
+ * and phase timings in the reference's formats.  Failures print the
+ * reference's message and exit(1), as the reference does.
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -75,8 +77,9 @@ int main(int argc, char** argv) {
     printf("    %d atoms, %d metal atoms, %d nanoparticles, %d CPIM unknowns\n\n", N, p->n_metal,
            p->n_nps, n_mat);
 
-    if (strcmp(d->run_type, "md") != 0) {
-        printf("Error: run_type %s is not supported by this build (md only)\n", d->run_type);
+    const int minimise = !strcmp(d->run_type, "em") || !strcmp(d->run_type, "cg");
+    if (strcmp(d->run_type, "md") != 0 && !minimise) {
+        printf("Error: unknown run_type %s (md, em or cg)\n", d->run_type);
         return 1;
     }
 
@@ -114,12 +117,45 @@ int main(int argc, char** argv) {
     capac_write_binary(fdat, d, r, 0);
     if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, 0);
     printf("              .---------------- start MD calculation ---------------.\n\n");
-    printf("  %10.3f  Fmax=%.2e  E=%.6e  T=%.3e\n", 0.0, fmax, pot[0], temp);
-    fflush(stdout);
+    if (minimise) {
+        printf("    Step %-5d Fmax=%10.3e  E=%15.8e\n", 0, fmax, pot[0]);
+        fflush(stdout);
+        if (capac_em_begin(ctx) != 0) die(capac_last_error());
+        const int cg = !strcmp(d->run_type, "cg");
+        double length = d->em_length, tol = d->em_tol;
+        for (int step = 1; step <= d->em_steps; ++step) {
+            const double old_pot = pot[0];
+            rc = capac_em_step(ctx, length, pot, &fmax, &frms);
+            if (rc == CAPAC_ERR_SOLVER || rc == CAPAC_ERR_RATTLE1) die(capac_last_error());
+            if (rc != 0) { printf("Error: %s\n", capac_last_error()); return 1; }
+            const double delta = pot[0] - old_pot;
+            length *= delta <= 0.0 ? 1.2 : 0.2;
+            printf("    Step %-5d Fmax=%10.3e  E=%15.8e\n", step, fmax, pot[0]);
+            fflush(stdout);
+            capac_get_state(ctx, r, v, x);
+            capac_write_gro(fgro, d, r, v, 0.0, 0.0, step);       /* vQ = vP = 0 in the minimiser */
+            capac_write_binary(fdat, d, r, step);
+            if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, step);
+            if (fmax < tol && frms < tol * 0.5 && fabs(delta) < tol * 0.1) {
+                printf("\n");
+                printf("    F_max   (%13.6e) smaller than %e\n", fmax, tol);
+                printf("    F_rms   (%13.6e) smaller than %e\n", frms, tol * 0.5);
+                printf("    delta_E (%13.6e) smaller than %e\n", delta, tol * 0.1);
+                printf("\n");
+                printf("    ===========  Optimization converged ============\n");
+                break;
+            }
+            if (capac_em_direction(ctx, cg) != 0) die(capac_last_error());
+        }
+    }
 
     int nsave = d->n_save > 0 ? d->n_save : 1;
     int step = 1;
-    while (step <= d->n_steps) {
+    if (!minimise) {
+        printf("  %10.3f  Fmax=%.2e  E=%.6e  T=%.3e\n", 0.0, fmax, pot[0], temp);
+        fflush(stdout);
+    }
+    while (!minimise && step <= d->n_steps) {
         int done = capac_md_run(ctx, step, d->n_steps - step + 1, nsave, 1);
         if (done < 0) {
             if (done == CAPAC_ERR_SOLVER || done == CAPAC_ERR_RATTLE1 || done == CAPAC_ERR_RATTLE2)

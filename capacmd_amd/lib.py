@@ -93,6 +93,7 @@ def host_lib() -> C.CDLL:
         lib.capac_deck_velocities.argtypes = [C.c_void_p]
         lib.capac_deck_velocities.restype = _dp
         lib.capac_deck_run.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
+        lib.capac_deck_minimiser.argtypes = [C.c_void_p, _ip, _dp, _dp]
         lib.capac_params_size.restype = C.c_int
         for fn in ("capac_write_gro_path",):
             getattr(lib, fn).argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, _dp, _dp,
@@ -129,6 +130,10 @@ class Deck:
         vq0, vp = C.c_double(), C.c_double()
         lib.capac_deck_run(h, C.byref(ns), C.byref(nsave), C.byref(vq0), C.byref(vp))
         self.n_steps, self.n_save, self.vq0, self.vp = ns.value, nsave.value, vq0.value, vp.value
+        es, el, et = C.c_int(), C.c_double(), C.c_double()
+        kind = lib.capac_deck_minimiser(h, C.byref(es), C.byref(el), C.byref(et))
+        self.run_type = {0: "md", 1: "em", 2: "cg"}.get(kind, "?")
+        self.em_steps, self.em_length, self.em_tol = es.value, el.value, et.value
 
     @classmethod
     def from_dir(cls, d: str) -> "Deck":
@@ -166,6 +171,7 @@ GPU_EXPORTS = [
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
     "capac_time_kernel", "capac_set_skin", "capac_skin", "capac_set_solve_tolerance",
     "capac_set_theta", "capac_get_theta", "capac_fit_frame",
+    "capac_em_begin", "capac_em_step", "capac_em_direction",
 ]
 NTHETA = 6
 
@@ -212,6 +218,9 @@ def gpu_lib() -> C.CDLL:
         lib.capac_set_theta.argtypes = [C.c_void_p, _dp]
         lib.capac_get_theta.argtypes = [C.c_void_p, _dp]
         lib.capac_fit_frame.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, _dp]
+        lib.capac_em_begin.argtypes = [C.c_void_p]
+        lib.capac_em_step.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+        lib.capac_em_direction.argtypes = [C.c_void_p, C.c_int]
         _gpu = lib
     return _gpu
 
@@ -287,6 +296,45 @@ class Context:
         t = np.empty(NTHETA)
         _check(self.lib.capac_get_theta(self.h, dptr(t)), "capac_get_theta")
         return t
+
+    # energy minimisation
+    def em_begin(self):
+        _check(self.lib.capac_em_begin(self.h), "capac_em_begin")
+
+    def em_step(self, length: float):
+        pot = np.zeros(NPOT); fmax = C.c_double(); frms = C.c_double()
+        _check(self.lib.capac_em_step(self.h, float(length), dptr(pot), C.byref(fmax), C.byref(frms)),
+               "capac_em_step")
+        return pot, fmax.value, frms.value
+
+    def em_direction(self, cg: bool):
+        _check(self.lib.capac_em_direction(self.h, int(bool(cg))), "capac_em_direction")
+
+    def minimise(self, cg: bool, steps: int, length: float, tol: float, on_step=None):
+        """The reference's minimiser loop (mpi-main.c:706-865) after the first
+        force call: capac_md_begin, then up to `steps` moves along s / F_max
+        scaled by `length`, which grows by 1.2 after a step that lowers the
+        energy and shrinks by 0.2 otherwise; stops when F_max < tol,
+        F_rms < tol / 2 and |dE| < tol / 10.  cg False is steepest descent.
+        on_step(step, fmax, frms, pot) runs after every step.  Returns the
+        step lines [(step, F_max, E)] from step 0 and whether it converged."""
+        self.md_begin()
+        obs = self.observables()
+        self.em_begin()
+        e = obs["pot"][0]
+        lines = [(0, obs["fmax"], e)]
+        for step in range(1, steps + 1):
+            pot, fmax, frms = self.em_step(length)
+            delta = pot[0] - e
+            e = pot[0]
+            length *= 1.2 if delta <= 0.0 else 0.2
+            lines.append((step, fmax, e))
+            if on_step is not None:
+                on_step(step, fmax, frms, pot)
+            if fmax < tol and frms < tol * 0.5 and abs(delta) < tol * 0.1:
+                return lines, True
+            self.em_direction(cg)
+        return lines, False
 
     def fit_frame(self, f_ref, e_ref: float, w_e: float, grad: bool = True):
         """Loss of one frame at the resident state and, with grad, dL/dtheta."""

@@ -174,6 +174,19 @@ double capac_skin(const capac_ctx* ctx);
  * studies can solve both precisions far below the reference's tolerance. */
 int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 
+/* ---- energy minimisation, run_type em / cg (mpi-main.c:706-865) ----
+ * capac_em_begin after the first capac_force: direction s = F, F_old = F.
+ * capac_em_step: r_old = r; r += s / F_max * length (metal stays with
+ * fix_metal); RATTLE_1 on the positions; v = 0; one force call; returns the
+ * potential energies with pot[0] = sum of pot[1..14], F_max and F_rms.
+ * capac_em_direction (only when not converged): s = F + gamma s with
+ * gamma = 0 (cg 0, steepest descent) or (F.F - F_old.F)/(F_old.F_old)
+ * (cg 1), then F_old = F.  The step-length rule and the convergence test are
+ * the caller's (the CLI follows mpi-main.c:781-819). */
+int capac_em_begin(capac_ctx* ctx);
+int capac_em_step(capac_ctx* ctx, double length, double pot[CAPAC_NPOT], double* fmax, double* frms);
+int capac_em_direction(capac_ctx* ctx, int cg);
+
 /* ---- fitting (no counterpart in the reference) ----
  * theta = (alpha [bohr^3], c [bohr], R_q [bohr], R_p [bohr], eps_SC [kJ/mol],
  * c_SC): CPIM polarizability and capacitance, the two Gaussian widths as

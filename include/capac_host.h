@@ -75,6 +75,9 @@ const capac_params* capac_deck_params(const capac_deck* d);
 const double* capac_deck_positions(const capac_deck* d);
 const double* capac_deck_velocities(const capac_deck* d);
 void capac_deck_run(const capac_deck* d, int* n_steps, int* n_save, double* vq0, double* vp);
+/* minimiser settings (mdset.txt em_steps, em_length_in_nm, em_tolerance);
+ * returns 0 for run_type md, 1 for em, 2 for cg, -1 otherwise */
+int  capac_deck_minimiser(const capac_deck* d, int* em_steps, double* em_length, double* em_tol);
 int  capac_params_size(void);                  /* sizeof(capac_params), an ABI check */
 
 /* Writers, byte-compatible with the reference's output files. */

@@ -68,6 +68,8 @@ struct oref {
     double solve_rTr, solve_bTb;
     int int_pair_count;
     double timers[15];
+    double solve_crit;          /* stop test rTr < solve_crit * bTb */
+    double *em_s, *em_old_f;    /* minimiser direction and previous forces */
 };
 
 static double now(void) {
@@ -101,6 +103,7 @@ static inline double pbc1(double dx, double L) {
 oref* oref_create(const capac_params* p) {
     oref* o = zalloc(1, sizeof(oref));
     o->p = *p;
+    o->solve_crit = 1.0e-08;
     int N = o->N = p->n_atoms;
     o->Nm = p->n_metal;
     o->n_np = p->n_nps;
@@ -150,6 +153,7 @@ void oref_destroy(oref* o) {
     double* vecs[11] = {o->Ax, o->r0, o->r, o->pp, o->v, o->s, o->t, o->y, o->z, o->Kt, o->K};
     for (int i = 0; i < 11; ++i) free(vecs[i]);
     free(o->rr); free(o->vv); free(o->ff); free(o->old_r);
+    free(o->em_s); free(o->em_old_f);
     free(o);
 }
 
@@ -597,7 +601,7 @@ int oref_solve(oref* o, const double* b, double* x, int* iters, double* rTr_out,
     int kind = CAPAC_SOLVE_CONVERGED, iter = 0;
     double rTr = 0.0;
     if (bTb < 1.0e-10) { kind = CAPAC_SOLVE_ZERO_RHS; goto out; }
-    double criteria = bTb * 1.0e-08;
+    double criteria = bTb * o->solve_crit;
     spmv_coo(o, x, o->Ax);
     for (int i = 0; i < n; ++i) {
         o->r0[i] = b[i] - o->Ax[i];
@@ -1128,3 +1132,67 @@ void oref_set_thermostat(oref* o, const double* vq) {
 }
 
 void oref_timers(const oref* o, double* t15) { memcpy(t15, o->timers, sizeof o->timers); }
+
+/* Stop test of the solve: rTr < crit * bTb; the reference's is 1e-8
+ * (bicg_stab.c:300).  Tighter values serve comparisons that should not see
+ * the solver tolerance. */
+void oref_set_solve_crit(oref* o, double crit) { o->solve_crit = crit; }
+
+/* ---------------------------------------------------- energy minimisation */
+
+/* run_type em / cg (mpi-main.c:706-865), after oref_force and oref_md_begin:
+ * s = F, F_old = F (mpi-main.c:716-726) */
+void oref_em_begin(oref* o) {
+    size_t n3 = 3 * (size_t)o->N;
+    if (!o->em_s) {
+        o->em_s = zalloc(n3, sizeof(double));
+        o->em_old_f = zalloc(n3, sizeof(double));
+    }
+    memcpy(o->em_s, o->ff, n3 * sizeof(double));
+    memcpy(o->em_old_f, o->ff, n3 * sizeof(double));
+}
+
+/* One minimiser step (mpi-main.c:730-776): move along s by length / F_max,
+ * RATTLE_1, zero velocities, force call, energies summed, F_max and F_rms.
+ * Returns 0, CAPAC_ERR_RATTLE1 or CAPAC_ERR_SOLVER. */
+int oref_em_step(oref* o, double length, double* pot, double* fmax, double* frms) {
+    const capac_params* p = &o->p;
+    int N = o->N, Nm = o->Nm;
+    for (int i = 0; i < N; ++i) {
+        for (int c = 0; c < 3; ++c) o->old_r[c * N + i] = o->rr[c * N + i];
+        if (p->fix_metal == 1 && i < Nm) continue;
+        for (int c = 0; c < 3; ++c) o->rr[c * N + i] += o->em_s[c * N + i] / o->fmax * length;
+    }
+    int rc = rattle1(o);
+    if (rc) return rc;
+    memset(o->vv, 0, 3 * (size_t)N * sizeof(double));
+    rc = oref_force(o);
+    sum_pot(o->pot);
+    fmax_rms(o);
+    if (pot) memcpy(pot, o->pot, sizeof o->pot);
+    if (fmax) *fmax = o->fmax;
+    if (frms) *frms = o->frms;
+    return rc;
+}
+
+/* New direction when not converged (mpi-main.c:821-849): s = F + gamma s,
+ * gamma = 0 (em) or (F.F - F_old.F) / (F_old.F_old) (cg), F_old = F. */
+void oref_em_direction(oref* o, int cg) {
+    int N = o->N;
+    const double *fx = o->ff, *fy = o->ff + N, *fz = o->ff + 2 * N;
+    double *ox = o->em_old_f, *oy = o->em_old_f + N, *oz = o->em_old_f + 2 * N;
+    double gamma = 0.0;
+    if (cg) {
+        double g22 = 0.0, g12 = 0.0, g11 = 0.0;
+        for (int i = 0; i < N; ++i) {
+            g22 += fx[i] * fx[i] + fy[i] * fy[i] + fz[i] * fz[i];
+            g12 += ox[i] * fx[i] + oy[i] * fy[i] + oz[i] * fz[i];
+            g11 += ox[i] * ox[i] + oy[i] * oy[i] + oz[i] * oz[i];
+        }
+        gamma = (g22 - g12) / g11;
+    }
+    for (size_t k = 0; k < 3 * (size_t)N; ++k) {
+        o->em_s[k] = o->ff[k] + gamma * o->em_s[k];
+        o->em_old_f[k] = o->ff[k];
+    }
+}

@@ -57,6 +57,14 @@ void oref_set_thermostat(oref* o, const double* vq);
 /* phase wall times (s), the reference's timer categories */
 void oref_timers(const oref* o, double* t15);
 
+/* solver stop test rTr < crit * bTb (reference: 1e-8) */
+void oref_set_solve_crit(oref* o, double crit);
+
+/* energy minimisation, run_type em / cg (mpi-main.c:706-865) */
+void oref_em_begin(oref* o);
+int  oref_em_step(oref* o, double length, double* pot, double* fmax, double* frms);
+void oref_em_direction(oref* o, int cg);
+
 #ifdef __cplusplus
 }
 #endif

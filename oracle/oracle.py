@@ -53,6 +53,10 @@ def lib(omp: bool = False) -> C.CDLL:
         L.oref_md_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oref_set_thermostat.argtypes = [C.c_void_p, _dp]
         L.oref_timers.argtypes = [C.c_void_p, _dp]
+        L.oref_set_solve_crit.argtypes = [C.c_void_p, C.c_double]
+        L.oref_em_begin.argtypes = [C.c_void_p]
+        L.oref_em_step.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+        L.oref_em_direction.argtypes = [C.c_void_p, C.c_int]
         _libs[name] = L
     return _libs[name]
 
@@ -173,6 +177,20 @@ class Oracle:
 
     def set_thermostat(self, vq):
         self.L.oref_set_thermostat(self.h, _d(np.ascontiguousarray(vq, dtype=np.float64)))
+
+    def set_solve_crit(self, crit: float):
+        self.L.oref_set_solve_crit(self.h, float(crit))
+
+    def em_begin(self):
+        self.L.oref_em_begin(self.h)
+
+    def em_step(self, length: float):
+        pot = np.zeros(15); fmax = C.c_double(); frms = C.c_double()
+        rc = self.L.oref_em_step(self.h, float(length), _d(pot), C.byref(fmax), C.byref(frms))
+        return rc, pot, fmax.value, frms.value
+
+    def em_direction(self, cg: bool):
+        self.L.oref_em_direction(self.h, int(bool(cg)))
 
     def timers(self):
         t = np.zeros(15)

@@ -2,7 +2,9 @@
 outputs for the same deck (tests/golden/*/ref_*, written by the reference
 built from its sources): the stdout step lines and final energy table
 (mpi-main.c:993-995, force.c:337-354), traj.gro (file.c:212-229),
-vec_pq.txt (file.c:236-260) and the last traj.dat frame (file.c:269-278).
+vec_pq.txt (file.c:236-260) and the last traj.dat frame (file.c:269-278),
+for MD (run_type md) and for the minimiser (run_type em and cg,
+mpi-main.c:706-865: step lines, convergence report, a frame every step).
 
 traj.gro's first frame is compared byte for byte.  The rest passes through
 the CPIM solve, whose results agree within the solver tolerance (see
@@ -23,6 +25,8 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "capacmd_amd", "capacmd")
 STEP = re.compile(r"^\s+([0-9.]+)\s+Fmax=(\S+)\s+E=(\S+)\s+T=(\S+)")
+EM_STEP = re.compile(r"^    Step (\d+)\s+Fmax=\s*(\S+)\s+E=\s*(\S+)$")
+CONVERGED = re.compile(r"^    (F_max  |F_rms  |delta_E) \(\s*(\S+)\) smaller than (\S+)$")
 
 
 def run_cli(golden_dir, name, tmp_path, prec):
@@ -49,6 +53,39 @@ def energy_table(text):
     return out
 
 
+def check_energy_table(out, ref):
+    """Same rows and labels; values to 1e-5 (1e-6 absolute floor), except the
+    CPIM rows, functions of the solved charges and dipoles.  The reference's
+    stop test bounds the relative residual by 1e-4, and on the field deck the
+    reference's own solution is 3.8e-4 from the exact one
+    (test_gpu_parity.py::test_bicgstab); the error that leaves in each CPIM
+    term scales with the CPIM energy as a whole, not with the (sometimes
+    small) term, so each CPIM row is held to 1e-4 of the summed magnitude of
+    the CPIM rows."""
+    ta, tb = energy_table(out), energy_table(ref)
+    assert [l for _, l in ta] == [l for _, l in tb]
+    cpim_scale = sum(abs(v) for v, l in tb if l.startswith("CPIM"))
+    for (va, la), (vb, _) in zip(ta, tb):
+        tol = 1e-4 * cpim_scale if la.startswith("CPIM") else max(1e-5 * abs(vb), 1e-6)
+        assert abs(va - vb) <= tol, (la, va, vb)
+
+
+def check_vec_pq(path, ref_path, tol):
+    va = open(path).read().splitlines()
+    vb = open(ref_path).read().splitlines()
+    assert len(va) == len(vb)
+    for la, lb in zip(va, vb):
+        fa, fb = la.split(), lb.split()
+        assert len(fa) == len(fb)
+        for xa, xb in zip(fa, fb):
+            try:
+                xa, xb = float(xa), float(xb)
+            except ValueError:
+                assert xa == xb
+                continue
+            assert abs(xa - xb) <= tol * max(1.0, abs(xb)), (la, lb)
+
+
 def read_dat_frame(raw, n):
     N, step = struct.unpack("ii", raw[:8])
     assert N == n
@@ -72,39 +109,14 @@ def test_cli_matches_reference_program(name, prec, golden_dir, tmp_path):
         assert abs(e1 - e2) <= 1e-5 * abs(e2), (t1, e1, e2)
         assert abs(T1 - T2) <= 1e-5 * abs(T2), (t1, T1, T2)
         assert abs(f1 - f2) <= 0.011 * abs(f2), (t1, f1, f2)
-    # final energy table: same rows and labels; values to 1e-5 (1e-6 absolute
-    # floor), except the CPIM rows, functions of the solved charges and
-    # dipoles.  The reference's stop test bounds the relative residual by
-    # 1e-4, and on the field deck the reference's own solution is 3.8e-4 from
-    # the exact one (test_gpu_parity.py::test_bicgstab); the error that
-    # leaves in each CPIM term scales with the CPIM energy as a whole, not
-    # with the (sometimes small) term, so each CPIM row is held to 1e-4 of
-    # the summed magnitude of the CPIM rows.
-    ta, tb = energy_table(out), energy_table(ref)
-    assert [l for _, l in ta] == [l for _, l in tb]
-    cpim_scale = sum(abs(v) for v, l in tb if l.startswith("CPIM"))
-    for (va, la), (vb, _) in zip(ta, tb):
-        tol = 1e-4 * cpim_scale if la.startswith("CPIM") else max(1e-5 * abs(vb), 1e-6)
-        assert abs(va - vb) <= tol, (la, va, vb)
+    check_energy_table(out, ref)
     # traj.gro: the first frame byte for byte; the file has the same line count
     tg = open(os.path.join(tmp_path, "traj.gro")).read().splitlines()
     n = int(tg[1])
     ref0 = open(os.path.join(g, "ref_traj_frame0.gro")).read().splitlines()
     assert tg[:n + 3] == ref0
-    # vec_pq.txt: same layout; values to 1e-4 of the largest entry (solver tolerance)
-    va = open(os.path.join(tmp_path, "vec_pq.txt")).read().splitlines()
-    vb = open(os.path.join(g, "ref_vec_pq.txt")).read().splitlines()
-    assert len(va) == len(vb)
-    for la, lb in zip(va, vb):
-        fa, fb = la.split(), lb.split()
-        assert len(fa) == len(fb)
-        for xa, xb in zip(fa, fb):
-            try:
-                xa, xb = float(xa), float(xb)
-            except ValueError:
-                assert xa == xb
-                continue
-            assert abs(xa - xb) <= 2e-4 * max(1.0, abs(xb)), (la, lb)
+    # vec_pq.txt: same layout; values to 2e-4 (solver tolerance)
+    check_vec_pq(os.path.join(tmp_path, "vec_pq.txt"), os.path.join(g, "ref_vec_pq.txt"), 2e-4)
     # traj.dat: the last frame, positions to 1e-6 nm
     raw = open(os.path.join(tmp_path, "traj.dat"), "rb").read()
     rec = 8 + 24 + 24 * n
@@ -114,3 +126,72 @@ def test_cli_matches_reference_program(name, prec, golden_dir, tmp_path):
     assert sa == sb
     np.testing.assert_array_equal(ba, bb)
     assert np.abs(ra - rb).max() < 1e-6
+
+
+def em_steps(text):
+    return [(int(m.group(1)), float(m.group(2)), float(m.group(3)))
+            for m in map(EM_STEP.match, text.splitlines()) if m]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["au13_em", "2au13_cg"])
+def test_cli_minimiser_matches_reference_program(name, prec, golden_dir, tmp_path):
+    """au13_em: steepest descent that meets the convergence test at step 18;
+    2au13_cg: conjugate gradients with a field and fixed metal, 15 steps
+    without converging.  Every step moves the atoms by forces that carry the
+    solver tolerance, so positions are held to 1e-4 nm: the oracle, which
+    reproduces the reference program bit for bit, itself moves the final
+    positions by up to 4.1e-5 nm (energies by 5.4e-7 relative) when its
+    solves are tightened from the reference's 1e-8 to 1e-20."""
+    g = os.path.join(golden_dir, name)
+    out = run_cli(golden_dir, name, tmp_path, prec)
+    ref = open(os.path.join(g, "ref_stdout.txt")).read()
+    # step lines: the same steps; E (9 printed digits) to 1e-5 relative, Fmax
+    # (4 printed digits) to 1%.  The step length follows the sign of each
+    # energy change, so the same line count also says every step took the
+    # same branch.
+    a, b = em_steps(out), em_steps(ref)
+    assert len(a) == len(b) > 1, (len(a), len(b))
+    for (s1, f1, e1), (s2, f2, e2) in zip(a, b):
+        assert s1 == s2
+        assert abs(e1 - e2) <= 1e-5 * abs(e2), (s1, e1, e2)
+        assert abs(f1 - f2) <= 0.01 * abs(f2), (s1, f1, f2)
+    # the convergence report: present in both or neither, thresholds equal,
+    # values to the step-line tolerances (delta_E to 1e-5 of E)
+    ca = [m.groups() for m in map(CONVERGED.match, out.splitlines()) if m]
+    cb = [m.groups() for m in map(CONVERGED.match, ref.splitlines()) if m]
+    assert len(ca) == len(cb)
+    e_last = abs(b[-1][2])
+    for (la, xa, ta), (lb, xb, tb), tol in zip(ca, cb, (0.01 * abs(b[-1][1]), None, 1e-5 * e_last)):
+        assert la == lb and ta == tb
+        tol = tol if tol is not None else 0.01 * abs(float(xb))
+        assert abs(float(xa) - float(xb)) <= tol, (la, xa, xb)
+    assert ("Optimization converged" in out) == ("Optimization converged" in ref)
+    check_energy_table(out, ref)
+    # traj.gro: the first frame byte for byte, one frame per step, and the
+    # last frame's header, names and zero velocities exact, positions to
+    # 1e-4 nm
+    tg = open(os.path.join(tmp_path, "traj.gro")).read().splitlines()
+    n = int(tg[1])
+    assert tg[:n + 3] == open(os.path.join(g, "ref_traj_frame0.gro")).read().splitlines()
+    assert len(tg) == (n + 3) * len(b)
+    last_a, last_b = tg[-(n + 3):], open(os.path.join(g, "ref_traj_last.gro")).read().splitlines()
+    assert last_a[0] == last_b[0] and last_a[1] == last_b[1] and last_a[-1] == last_b[-1]
+    for la, lb in zip(last_a[2:-1], last_b[2:-1]):
+        assert la[:20] == lb[:20]
+        pa = [float(t) for t in la[20:].split()]
+        pb = [float(t) for t in lb[20:].split()]
+        assert pa[3:] == pb[3:] == [0.0, 0.0, 0.0]
+        assert max(abs(x - y) for x, y in zip(pa[:3], pb[:3])) <= 1e-4, (la, lb)
+    # vec_pq.txt every step: values to 1e-3 (solver tolerance carried
+    # through the minimiser's moves)
+    check_vec_pq(os.path.join(tmp_path, "vec_pq.txt"), os.path.join(g, "ref_vec_pq.txt"), 1e-3)
+    # traj.dat: the last frame, positions to 1e-4 nm
+    raw = open(os.path.join(tmp_path, "traj.dat"), "rb").read()
+    rec = 8 + 24 + 24 * n
+    assert len(raw) == rec * len(b)
+    sa, ba, ra = read_dat_frame(raw[-rec:], n)
+    sb, bb, rb = read_dat_frame(open(os.path.join(g, "ref_traj_last.dat"), "rb").read(), n)
+    assert sa == sb
+    np.testing.assert_array_equal(ba, bb)
+    assert np.abs(ra - rb).max() <= 1e-4

@@ -9,6 +9,9 @@
    terms vanish at the cut-off, the relay matrix is symmetric and tends to
    the bare dipole/Coulomb tensors at long range, BiCGSTAB agrees with a
    dense LU solve and keeps each nanoparticle neutral.
+3. The reference program's own output: the last traj.dat frame of a 10-step
+   MD run, and the step lines and last frame of steepest-descent and
+   conjugate-gradient minimisations.
 """
 import os
 
@@ -95,6 +98,53 @@ def test_reference_program_trajectory(golden_dir):
         assert o.md_step(s, deck.n_save) == 0
     r, *_ = o.get_state()
     assert same(r, pos)
+
+
+def oracle_minimise(deck, crit=None):
+    """The minimiser loop of mpi-main.c:706-865 over the oracle; returns the
+    oracle, the step lines (step, F_max, E) and whether it converged."""
+    o = Oracle(deck)
+    if crit is not None:
+        o.set_solve_crit(crit)
+    o.force()
+    o.md_begin()
+    obs = o.observables()
+    o.em_begin()
+    lines = [(0, obs["fmax"], obs["pot"][0])]
+    length, e, tol = deck.em_length, obs["pot"][0], deck.em_tol
+    for step in range(1, deck.em_steps + 1):
+        rc, pot, fmax, frms = o.em_step(length)
+        assert rc == 0, rc
+        delta = pot[0] - e
+        e = pot[0]
+        length *= 1.2 if delta <= 0.0 else 0.2
+        lines.append((step, fmax, pot[0]))
+        if fmax < tol and frms < tol * 0.5 and abs(delta) < tol * 0.1:
+            return o, lines, True
+        o.em_direction(deck.run_type == "cg")
+    return o, lines, False
+
+
+@pytest.mark.parametrize("name", ["au13_em", "2au13_cg"])
+def test_reference_program_minimiser(name, golden_dir):
+    """Steepest descent (au13_em, converges at step 18) and conjugate
+    gradients with a field and fixed metal (2au13_cg): every printed step line
+    and the convergence line as the reference program printed them, and its
+    last traj.dat frame bit for bit."""
+    g = os.path.join(golden_dir, name)
+    deck = lib.Deck.from_dir(g)
+    assert deck.run_type == name[-2:]
+    o, lines, converged = oracle_minimise(deck)
+    ref = open(os.path.join(g, "ref_stdout.txt")).read().splitlines()
+    ref_steps = [l for l in ref if l.startswith("    Step ")]
+    assert ["    Step %-5d Fmax=%10.3e  E=%15.8e" % t for t in lines] == ref_steps
+    assert converged == any("Optimization converged" in l for l in ref)
+    raw = open(os.path.join(g, "ref_traj_last.dat"), "rb").read()
+    n, step = np.frombuffer(raw[:8], np.int32)
+    assert step == lines[-1][0]
+    r, v, *_ = o.get_state()
+    assert same(r, np.frombuffer(raw[32:], np.float64))
+    assert not v.any()
 
 
 def test_reference_pair_count_wraps():
