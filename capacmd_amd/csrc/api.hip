@@ -1019,6 +1019,113 @@ extern "C" int capac_md_observables(capac_ctx* c, double* eksum, double* temp, d
     return 0;
 }
 
+// ------------------------------------------------------------------ checkpoint
+
+namespace {
+// What the MD loop carries from one step to the next, beyond what traj.gro
+// keeps (file.c:215-229 writes positions, velocities, vQ[0] and vP only).
+struct CkHeader {
+    char magic[8];                  // "CAPACCK1"
+    int32_t n_atoms, n_metal, n_mat, step;
+    int32_t solve_iters, rows_dirty, precision, reserved;
+    double box[3], rcut, skin_req, skin;
+    double first_temp, ext_temp, cur_ref_temp;
+    double pot[CAPAC_NPOT], vir[9];
+    double md[32];                  // kinetic energy, temperature, the whole NHC, F_max, F_rms, ...
+    double red_md[16];              // RATTLE_2 virial of the last step
+};
+// then r, v, f, r_build (3N each), x (n_mat)
+const char kCkMagic[8] = {'C', 'A', 'P', 'A', 'C', 'C', 'K', '1'};
+}  // namespace
+
+extern "C" size_t capac_checkpoint_bytes(const capac_ctx* c) {
+    if (!c) return 0;
+    return sizeof(CkHeader) + sizeof(double) * (12 * (size_t)c->bc.N + (size_t)c->n_mat);
+}
+
+extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t bytes) {
+    if (!c || !buf) return CAPAC_ERR_ARG;
+    if (bytes < capac_checkpoint_bytes(c)) {
+        capac_set_error("checkpoint buffer holds %zu bytes, %zu needed", bytes, capac_checkpoint_bytes(c));
+        return CAPAC_ERR_ARG;
+    }
+    const size_t n3 = 3 * (size_t)c->bc.N;
+    CkHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kCkMagic, 8);
+    h.n_atoms = c->bc.N; h.n_metal = c->bc.Nm; h.n_mat = c->n_mat; h.step = step;
+    h.solve_iters = c->solve_iters; h.rows_dirty = c->nb_dirty; h.precision = c->precision;
+    h.box[0] = c->bc.box[0]; h.box[1] = c->bc.box[1]; h.box[2] = c->bc.box[2];
+    h.rcut = c->bc.rc; h.skin_req = c->skin_req; h.skin = c->skin;
+    h.first_temp = c->first_temp; h.ext_temp = c->ext_temp; h.cur_ref_temp = c->cur_ref_temp;
+    memcpy(h.pot, c->pot, sizeof h.pot);
+    memcpy(h.vir, c->vir, sizeof h.vir);
+    double* a = (double*)((char*)buf + sizeof(CkHeader));
+    CHECK_HIP(hipMemcpyAsync(h.md, c->md, sizeof h.md, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(h.red_md, c->red + 64, sizeof h.red_md, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(a, c->r, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(a + n3, c->v, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(a + 2 * n3, c->f, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(a + 3 * n3, c->r_build, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (c->n_mat)
+        CHECK_HIP(hipMemcpyAsync(a + 4 * n3, c->x, c->n_mat * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    memcpy(buf, &h, sizeof h);
+    return 0;
+}
+
+extern "C" int capac_checkpoint_load(capac_ctx* c, const void* buf, size_t bytes, int* step) {
+    if (!c || !buf) return CAPAC_ERR_ARG;
+    CkHeader h;
+    if (bytes < sizeof h) { capac_set_error("checkpoint is truncated"); return CAPAC_ERR_ARG; }
+    memcpy(&h, buf, sizeof h);
+    if (memcmp(h.magic, kCkMagic, 8) != 0) { capac_set_error("not a capac checkpoint"); return CAPAC_ERR_ARG; }
+    if (h.n_atoms != c->bc.N || h.n_metal != c->bc.Nm || h.n_mat != c->n_mat || h.rcut != c->bc.rc ||
+        h.box[0] != c->bc.box[0] || h.box[1] != c->bc.box[1] || h.box[2] != c->bc.box[2]) {
+        capac_set_error("checkpoint of another system (%d atoms, %d metal, box %g %g %g)", h.n_atoms,
+                        h.n_metal, h.box[0], h.box[1], h.box[2]);
+        return CAPAC_ERR_ARG;
+    }
+    if (bytes < capac_checkpoint_bytes(c)) { capac_set_error("checkpoint is truncated"); return CAPAC_ERR_ARG; }
+    int rc;
+    if (h.skin_req != c->skin_req) {
+        c->skin_req = h.skin_req;
+        if ((rc = nb_regrid(c))) return rc;
+    }
+    const size_t n3 = 3 * (size_t)c->bc.N;
+    const double* a = (const double*)((const char*)buf + sizeof(CkHeader));
+    c->nb_dirty = 1;
+    if (!h.rows_dirty) {
+        // rebuild the neighbour rows at the positions they were built at, so
+        // the rows (and the order of every pair sum) are those of the saved run
+        CHECK_HIP(hipMemcpyAsync(c->r, a + 3 * n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        for (int attempt = 0;; ++attempt) {
+            if ((rc = nb_build(c))) return rc;
+            if ((rc = fetch_red(c))) return rc;
+            if (!nb_overflowed(c)) break;
+            if (attempt == 7) { capac_set_error("neighbour rows keep overflowing"); return CAPAC_ERR_HIP; }
+            if ((rc = grow_nbr(c))) return rc;
+        }
+        c->nb_dirty = 0;
+    }
+    CHECK_HIP(hipMemcpyAsync(c->r, a, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->v, a + n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->f, a + 2 * n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->n_mat)
+        CHECK_HIP(hipMemcpyAsync(c->x, a + 4 * n3, c->n_mat * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->md, h.md, sizeof h.md, hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->red + 64, h.red_md, sizeof h.red_md, hipMemcpyHostToDevice, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    c->solve_iters = h.solve_iters;
+    c->first_temp = h.first_temp; c->ext_temp = h.ext_temp; c->cur_ref_temp = h.cur_ref_temp;
+    memcpy(c->pot, h.pot, sizeof h.pot);
+    memcpy(c->vir, h.vir, sizeof h.vir);
+    if (step) *step = h.step;
+    return fetch_md(c);
+}
+
 extern "C" int capac_set_thermostat(capac_ctx* c, const double* vq, double vp) {
     (void)vp;
     if (!c || !vq) return CAPAC_ERR_ARG;

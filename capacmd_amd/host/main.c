@@ -2,6 +2,7 @@
  * (reference mpi-main.c:41-1219, one GPU in place of the MPI ranks):
  *
  *   capacmd -gro init.gro -par param.txt -mds mdset.txt [-prec fp64|fp32] [-dev N]
+ *           [-cpt out.cpt] [-rst in.cpt]
  *
  * Reads the three inputs with the reference's grammar, runs the force call
  * and then, by run_type, the MD loop (md: traj.gro and vec_pq.txt every 10
@@ -10,6 +11,13 @@
  * step) on the device, and writes the stdout step lines, final energy table
  * and phase timings in the reference's formats.  Failures print the
  * reference's message and exit(1), as the reference does.
+ *
+ * Restart (MD only): -cpt writes a checkpoint (capac_checkpoint_save) next
+ * to every traj.gro frame and after the last step; -rst continues the run
+ * it holds from the step after it, up to number_of_steps, writing the
+ * output files from there on.  The continued run repeats the saved run's
+ * later steps exactly, which a restart from traj.gro cannot (it keeps vQ[0]
+ * and vP of the ten-link chain and no CPIM warm start).
  */
 #include <math.h>
 #include <stdio.h>
@@ -32,11 +40,55 @@ static void die(const char* msg) {
     exit(1);
 }
 
+/* the checkpoint goes to a temporary file that replaces `path` when whole */
+static void save_checkpoint(capac_ctx* ctx, const char* path, int step) {
+    size_t n = capac_checkpoint_bytes(ctx);
+    void* buf = malloc(n);
+    if (!buf) die("Error: out of memory for the checkpoint");
+    if (capac_checkpoint_save(ctx, step, buf, n) != 0) die(capac_last_error());
+    char tmp[4096];
+    snprintf(tmp, sizeof tmp, "%s.tmp", path);
+    FILE* f = fopen(tmp, "wb");
+    if (!f || fwrite(buf, 1, n, f) != n || fclose(f) != 0) {
+        printf("Cannot write to %s!\n", path);
+        exit(1);
+    }
+    if (rename(tmp, path) != 0) {
+        printf("Cannot write to %s!\n", path);
+        exit(1);
+    }
+    free(buf);
+}
+
+static void load_checkpoint(capac_ctx* ctx, const char* path, int* step) {
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        printf("Cannot read %s!\n", path);
+        exit(1);
+    }
+    fseek(f, 0, SEEK_END);
+    long n = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    void* buf = malloc(n > 0 ? (size_t)n : 1);
+    if (!buf || n <= 0 || fread(buf, 1, (size_t)n, f) != (size_t)n) {
+        printf("Cannot read %s!\n", path);
+        exit(1);
+    }
+    fclose(f);
+    if (capac_checkpoint_load(ctx, buf, (size_t)n, step) != 0) {
+        printf("Error: %s: %s\n", path, capac_last_error());
+        exit(1);
+    }
+    free(buf);
+}
+
 int main(int argc, char** argv) {
     const char* gro = "init.gro";
     const char* par = "param.txt";
     const char* mds = "mdset.txt";
     int precision = CAPAC_PREC_FP64, device = 0;
+    const char* cpt_out = NULL;
+    const char* rst_in = NULL;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "-gro") && i + 1 < argc) gro = argv[++i];
         else if (!strcmp(argv[i], "-par") && i + 1 < argc) par = argv[++i];
@@ -45,6 +97,8 @@ int main(int argc, char** argv) {
             ++i;
             precision = !strcmp(argv[i], "fp32") ? CAPAC_PREC_FP32_REFINED : CAPAC_PREC_FP64;
         } else if (!strcmp(argv[i], "-dev") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-cpt") && i + 1 < argc) cpt_out = argv[++i];
+        else if (!strcmp(argv[i], "-rst") && i + 1 < argc) rst_in = argv[++i];
     }
     time_t start_t = time(NULL);
     double t_start = wall();
@@ -82,6 +136,10 @@ int main(int argc, char** argv) {
         printf("Error: unknown run_type %s (md, em or cg)\n", d->run_type);
         return 1;
     }
+    if (minimise && (cpt_out || rst_in)) {
+        printf("Error: -cpt and -rst continue MD runs (run_type md)\n");
+        return 1;
+    }
 
     capac_ctx* ctx = NULL;
     if (capac_create(p, precision, device, &ctx) != 0) {
@@ -98,10 +156,16 @@ int main(int argc, char** argv) {
     capac_set_thermostat(ctx, vq, d->vp);
 
     double pot[CAPAC_NPOT], vir[9];
-    int rc = capac_force(ctx, pot, vir, NULL);
-    if (rc == CAPAC_ERR_SOLVER) die(capac_last_error());
-    if (rc != 0) { printf("Error: %s\n", capac_last_error()); return 1; }
-    if (capac_md_begin(ctx) != 0) die(capac_last_error());
+    int rc = 0, first_step = 1;
+    if (rst_in) {
+        load_checkpoint(ctx, rst_in, &first_step);
+        ++first_step;
+    } else {
+        rc = capac_force(ctx, pot, vir, NULL);
+        if (rc == CAPAC_ERR_SOLVER) die(capac_last_error());
+        if (rc != 0) { printf("Error: %s\n", capac_last_error()); return 1; }
+        if (capac_md_begin(ctx) != 0) die(capac_last_error());
+    }
 
     FILE* fgro = fopen("traj.gro", "w");
     if (!fgro) die("Cannot write to traj.gro!");
@@ -113,9 +177,12 @@ int main(int argc, char** argv) {
     double ek, temp, reft, fmax, frms;
     capac_md_observables(ctx, &ek, &temp, &reft, vq, &fmax, &frms, pot, vir);
     capac_get_state(ctx, r, v, x);
-    capac_write_gro(fgro, d, r, v, vq[0], d->vp, 0);
-    capac_write_binary(fdat, d, r, 0);
-    if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, 0);
+    if (!rst_in) {
+        capac_write_gro(fgro, d, r, v, vq[0], d->vp, 0);
+        capac_write_binary(fdat, d, r, 0);
+        if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, 0);
+        if (cpt_out) save_checkpoint(ctx, cpt_out, 0);
+    }
     printf("              .---------------- start MD calculation ---------------.\n\n");
     if (minimise) {
         printf("    Step %-5d Fmax=%10.3e  E=%15.8e\n", 0, fmax, pot[0]);
@@ -150,8 +217,8 @@ int main(int argc, char** argv) {
     }
 
     int nsave = d->n_save > 0 ? d->n_save : 1;
-    int step = 1;
-    if (!minimise) {
+    int step = first_step;
+    if (!minimise && !rst_in) {
         printf("  %10.3f  Fmax=%.2e  E=%.6e  T=%.3e\n", 0.0, fmax, pot[0], temp);
         fflush(stdout);
     }
@@ -174,10 +241,12 @@ int main(int argc, char** argv) {
             if (last % (nsave * 10) == 0) {
                 capac_write_gro(fgro, d, r, v, vq[0], d->vp, last);
                 if (p->n_metal > 0) capac_write_vec_pq(fpq, d, x, last);
+                if (cpt_out) save_checkpoint(ctx, cpt_out, last);
             }
         }
         if (done == 0) break;
     }
+    if (!minimise && cpt_out && step > first_step) save_checkpoint(ctx, cpt_out, step - 1);
     fclose(fdat);
     fclose(fpq);
     fclose(fgro);

@@ -172,6 +172,7 @@ GPU_EXPORTS = [
     "capac_time_kernel", "capac_set_skin", "capac_skin", "capac_set_solve_tolerance",
     "capac_set_theta", "capac_get_theta", "capac_fit_frame",
     "capac_em_begin", "capac_em_step", "capac_em_direction",
+    "capac_checkpoint_bytes", "capac_checkpoint_save", "capac_checkpoint_load",
 ]
 NTHETA = 6
 
@@ -221,6 +222,10 @@ def gpu_lib() -> C.CDLL:
         lib.capac_em_begin.argtypes = [C.c_void_p]
         lib.capac_em_step.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
         lib.capac_em_direction.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_checkpoint_bytes.argtypes = [C.c_void_p]
+        lib.capac_checkpoint_bytes.restype = C.c_size_t
+        lib.capac_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+        lib.capac_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, _ip]
         _gpu = lib
     return _gpu
 
@@ -309,6 +314,21 @@ class Context:
 
     def em_direction(self, cg: bool):
         _check(self.lib.capac_em_direction(self.h, int(bool(cg))), "capac_em_direction")
+
+    # restart checkpoint
+    def checkpoint(self, step: int) -> bytes:
+        """Everything the MD loop carries between steps, as bytes
+        (capac_checkpoint_save)."""
+        n = self.lib.capac_checkpoint_bytes(self.h)
+        buf = C.create_string_buffer(n)
+        _check(self.lib.capac_checkpoint_save(self.h, int(step), buf, n), "capac_checkpoint_save")
+        return buf.raw
+
+    def restore(self, blob: bytes) -> int:
+        """Loads a checkpoint (capac_checkpoint_load); returns its step."""
+        step = C.c_int()
+        _check(self.lib.capac_checkpoint_load(self.h, blob, len(blob), C.byref(step)), "capac_checkpoint_load")
+        return step.value
 
     def minimise(self, cg: bool, steps: int, length: float, tol: float, on_step=None):
         """The reference's minimiser loop (mpi-main.c:706-865) after the first

@@ -174,6 +174,23 @@ double capac_skin(const capac_ctx* ctx);
  * studies can solve both precisions far below the reference's tolerance. */
 int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 
+/* ---- restart checkpoint (no counterpart in the reference) ----
+ * The reference restarts from the last traj.gro frame, which keeps
+ * positions, velocities, vQ[0] and vP only (file.c:152-160, 215-229): the
+ * other nine Nose-Hoover links and the CPIM warm start vec_pq are lost, so
+ * a restarted run leaves the trajectory it continues.  A checkpoint holds
+ * everything the MD loop carries between steps (positions, velocities,
+ * forces, vec_pq, the whole chain, the heating ramp, the neighbour-row
+ * build positions) so that capac_checkpoint_load followed by capac_md_run
+ * continues the saved run step for step, bit for bit on the same device and
+ * precision.  Save between capac_md_run calls; `step` is stored as given.
+ * Loading checks that the checkpoint belongs to this system (atom counts,
+ * box, cut-off) and takes over its Verlet skin.  Minimiser state is not
+ * included. */
+size_t capac_checkpoint_bytes(const capac_ctx* ctx);
+int capac_checkpoint_save(capac_ctx* ctx, int step, void* buf, size_t bytes);
+int capac_checkpoint_load(capac_ctx* ctx, const void* buf, size_t bytes, int* step);
+
 /* ---- energy minimisation, run_type em / cg (mpi-main.c:706-865) ----
  * capac_em_begin after the first capac_force: direction s = F, F_old = F.
  * capac_em_step: r_old = r; r += s / F_max * length (metal stays with
