@@ -353,6 +353,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     // cell list and neighbour rows (list radius rc + skin)
     c->skin_req = CAPAC_DEFAULT_SKIN;
     c->solve_crit = CAPAC_DEFAULT_SOLVE_CRIT;
+    c->pred_order = precision == CAPAC_PREC_FP32_REFINED ? CAPAC_DEFAULT_PREDICTOR_FP32 : 0;
     {
         // fitting parameters start at the deck's values, widths derived as the reference does
         const double pi = 3.14159265358979323846;
@@ -421,6 +422,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         TRY(upload(&c->b, (const double*)NULL, c->n_mat));
         TRY(upload(&c->x, (const double*)NULL, c->n_mat));
         TRY(upload(&c->x_backup, (const double*)NULL, c->n_mat));
+        for (int k = 0; k < 3; ++k) TRY(upload(&c->x_hist[k], (const double*)NULL, c->n_mat));
         TRY(upload(&c->diag, (const double*)NULL, c->n_mat));
         c->relay_ld = ((Nm + 63) / 64) * 64;
         if (precision == CAPAC_PREC_FP32_REFINED) {
@@ -448,7 +450,8 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->nb_overflow, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32, c->md, c->md_status};
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32, c->md, c->md_status,
+                    c->x_hist[0], c->x_hist[1], c->x_hist[2]};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     if (c->red_host) (void)hipHostFree(c->red_host);
@@ -476,6 +479,7 @@ extern "C" int capac_set_state(capac_ctx* c, const double* r, const double* v, c
         CHECK_HIP(hipMemcpyAsync(c->x, x, c->n_mat * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     if (r) c->nb_dirty = 1;
+    if (r || x) c->pred_count = 0;      // the stored solutions no longer describe this state
     return 0;
 }
 
@@ -507,6 +511,7 @@ extern "C" int capac_set_box(capac_ctx* c, const double box[3]) {
         c->bc.box[d] = box[d];
         c->bc.half[d] = 0.5 * box[d];
     }
+    c->pred_count = 0;
     return nb_regrid(c);
 }
 
@@ -556,6 +561,18 @@ extern "C" int capac_fit_frame(capac_ctx* c, const double* f_ref, double e_ref, 
     return fit_frame_loss_grad(c, f_ref, e_ref, w_e, loss, grad);
 }
 
+extern "C" int capac_set_predictor(capac_ctx* c, int order) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (order < 0 || order > 2) {
+        capac_set_error("predictor order must be 0, 1 or 2");
+        return CAPAC_ERR_ARG;
+    }
+    c->pred_order = order;
+    return 0;
+}
+
+extern "C" int capac_predictor(const capac_ctx* c) { return c ? c->pred_order : -1; }
+
 extern "C" int capac_set_solve_tolerance(capac_ctx* c, double rtr_over_btb) {
     if (!c) return CAPAC_ERR_ARG;
     if (!(rtr_over_btb > 0.0) || !(rtr_over_btb <= 1.0)) {
@@ -590,13 +607,17 @@ static void mark(capac_ctx* c, int k) {
     if (c->timing_on) (void)hipEventRecord(c->ev[k], c->stream);
 }
 
-// grid of the pair kernel while it runs beside the solve: half the compute
-// units for boxes of up to ~20k atoms (the kernel then still ends before the
-// solve), growing with N so that larger boxes, whose pair work outlasts a
-// capped grid, get more
+// grid of the pair kernel while it runs beside the solve.  Below 1024 metal
+// atoms the solve is a chain of short, latency-bound launches: the pair
+// kernel gets one workgroup per compute unit, so that the solver's
+// workgroups always find free slots (a full grid fills every CU and delays
+// each of them).  Above, the solve streams the relay at HBM speed and the
+// pair work of the larger box is the longer of the two: the full grid lets
+// it finish early.  Measured on MI355X (ms per MD step): Au561 FP32 with the
+// quadratic predictor 1.012 at 261 workgroups, 1.018 at 2048; FP64 1.725 at
+// 130, 1.739 at 261, 1.81 at 2048; Au2057 FP32 3.42 at 634, 3.21 at 2048.
 static int side_pair_blocks(const capac_ctx* c) {
-    int b = c->bc.N / 160;
-    return b < 128 ? 128 : b;
+    return c->bc.Nm < 1024 ? 256 : 0;
 }
 
 // pair and Sutton-Chen forces on the side stream, after the rows are built
@@ -947,6 +968,9 @@ extern "C" int capac_md_begin(capac_ctx* c) {
     c->first_temp = c->red_host[260 + 1];
     c->ext_temp = c->ref_temp0;
     c->cur_ref_temp = c->ref_temp0;
+    // the first force call's solution starts the predictor's history
+    c->pred_count = 0;
+    solver_push_history(c);
     return 0;
 }
 
@@ -966,8 +990,10 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (!rc) rc = md_drift(c);
         if (!rc) rc = md_rattle1(c);
         if (rc) return rc;
+        solver_predict(c);
         rc = force_compute(c);
         if (rc) return rc;
+        solver_push_history(c);
         mark(c, 9);
         if (!rc) rc = md_kick(c);
         if (!rc) rc = md_rattle2(c);
@@ -1027,20 +1053,21 @@ namespace {
 struct CkHeader {
     char magic[8];                  // "CAPACCK1"
     int32_t n_atoms, n_metal, n_mat, step;
-    int32_t solve_iters, rows_dirty, precision, reserved;
+    int32_t solve_iters, rows_dirty, precision, pred_order, pred_count, reserved[3];
     double box[3], rcut, skin_req, skin;
     double first_temp, ext_temp, cur_ref_temp;
     double pot[CAPAC_NPOT], vir[9];
     double md[32];                  // kinetic energy, temperature, the whole NHC, F_max, F_rms, ...
     double red_md[16];              // RATTLE_2 virial of the last step
 };
-// then r, v, f, r_build (3N each), x (n_mat)
+// then r, v, f, r_build (3N each), x and the predictor's three stored
+// solutions, newest first (n_mat each)
 const char kCkMagic[8] = {'C', 'A', 'P', 'A', 'C', 'C', 'K', '1'};
 }  // namespace
 
 extern "C" size_t capac_checkpoint_bytes(const capac_ctx* c) {
     if (!c) return 0;
-    return sizeof(CkHeader) + sizeof(double) * (12 * (size_t)c->bc.N + (size_t)c->n_mat);
+    return sizeof(CkHeader) + sizeof(double) * (12 * (size_t)c->bc.N + 4 * (size_t)c->n_mat);
 }
 
 extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t bytes) {
@@ -1055,6 +1082,7 @@ extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t b
     memcpy(h.magic, kCkMagic, 8);
     h.n_atoms = c->bc.N; h.n_metal = c->bc.Nm; h.n_mat = c->n_mat; h.step = step;
     h.solve_iters = c->solve_iters; h.rows_dirty = c->nb_dirty; h.precision = c->precision;
+    h.pred_order = c->pred_order; h.pred_count = c->pred_count;
     h.box[0] = c->bc.box[0]; h.box[1] = c->bc.box[1]; h.box[2] = c->bc.box[2];
     h.rcut = c->bc.rc; h.skin_req = c->skin_req; h.skin = c->skin;
     h.first_temp = c->first_temp; h.ext_temp = c->ext_temp; h.cur_ref_temp = c->cur_ref_temp;
@@ -1067,9 +1095,9 @@ extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t b
     CHECK_HIP(hipMemcpyAsync(a + n3, c->v, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipMemcpyAsync(a + 2 * n3, c->f, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipMemcpyAsync(a + 3 * n3, c->r_build, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (c->n_mat)
-        CHECK_HIP(hipMemcpyAsync(a + 4 * n3, c->x, c->n_mat * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
+    for (int k = 0; k < 4 && c->n_mat; ++k)
+        CHECK_HIP(hipMemcpyAsync(a + 4 * n3 + (size_t)k * c->n_mat, k ? c->x_hist[k - 1] : c->x,
+                                 c->n_mat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     memcpy(buf, &h, sizeof h);
     return 0;
@@ -1112,13 +1140,15 @@ extern "C" int capac_checkpoint_load(capac_ctx* c, const void* buf, size_t bytes
     CHECK_HIP(hipMemcpyAsync(c->r, a, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->v, a + n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->f, a + 2 * n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (c->n_mat)
-        CHECK_HIP(hipMemcpyAsync(c->x, a + 4 * n3, c->n_mat * sizeof(double), hipMemcpyHostToDevice,
-                                 c->stream));
+    for (int k = 0; k < 4 && c->n_mat; ++k)
+        CHECK_HIP(hipMemcpyAsync(k ? c->x_hist[k - 1] : c->x, a + 4 * n3 + (size_t)k * c->n_mat,
+                                 c->n_mat * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->md, h.md, sizeof h.md, hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->red + 64, h.red_md, sizeof h.red_md, hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     c->solve_iters = h.solve_iters;
+    c->pred_order = h.pred_order;
+    c->pred_count = h.pred_count;
     c->first_temp = h.first_temp; c->ext_temp = h.ext_temp; c->cur_ref_temp = h.cur_ref_temp;
     memcpy(c->pot, h.pot, sizeof h.pot);
     memcpy(c->vir, h.vir, sizeof h.vir);

@@ -120,6 +120,10 @@ struct capac_ctx {
     double* isd;
     // CPIM
     double *b, *x, *x_backup, *diag, *relay;
+    // solutions of the last MD steps, newest first, for the warm-start
+    // predictor (order 0: the previous solution, as the reference)
+    double* x_hist[3];
+    int pred_order, pred_count;
     float* relay32;
     int relay_ld;
     size_t relay_bytes;
@@ -208,6 +212,8 @@ void solver_destroy(capac_ctx* c);
 int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
 double* solver_scratch(capac_ctx* c);
+void solver_predict(capac_ctx* c);          // x from the history (md steps)
+void solver_push_history(capac_ctx* c);     // the solution just found joins it
 int cpim_mm_only(capac_ctx* c);
 // em.hip
 void em_destroy(capac_ctx* c);

@@ -2,7 +2,7 @@
  * (reference mpi-main.c:41-1219, one GPU in place of the MPI ranks):
  *
  *   capacmd -gro init.gro -par param.txt -mds mdset.txt [-prec fp64|fp32] [-dev N]
- *           [-cpt out.cpt] [-rst in.cpt]
+ *           [-cpt out.cpt] [-rst in.cpt] [-pred 0|1|2]
  *
  * Reads the three inputs with the reference's grammar, runs the force call
  * and then, by run_type, the MD loop (md: traj.gro and vec_pq.txt every 10
@@ -11,6 +11,10 @@
  * step) on the device, and writes the stdout step lines, final energy table
  * and phase timings in the reference's formats.  Failures print the
  * reference's message and exit(1), as the reference does.
+ *
+ * -pred sets the warm start of the MD solves (capac_set_predictor): 0 the
+ * previous solution, as the reference; 1, 2 linear and quadratic
+ * extrapolation.  The default is 0 for fp64 and 2 for fp32.
  *
  * Restart (MD only): -cpt writes a checkpoint (capac_checkpoint_save) next
  * to every traj.gro frame and after the last step; -rst continues the run
@@ -89,6 +93,7 @@ int main(int argc, char** argv) {
     int precision = CAPAC_PREC_FP64, device = 0;
     const char* cpt_out = NULL;
     const char* rst_in = NULL;
+    int pred = -1;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "-gro") && i + 1 < argc) gro = argv[++i];
         else if (!strcmp(argv[i], "-par") && i + 1 < argc) par = argv[++i];
@@ -99,6 +104,7 @@ int main(int argc, char** argv) {
         } else if (!strcmp(argv[i], "-dev") && i + 1 < argc) device = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-cpt") && i + 1 < argc) cpt_out = argv[++i];
         else if (!strcmp(argv[i], "-rst") && i + 1 < argc) rst_in = argv[++i];
+        else if (!strcmp(argv[i], "-pred") && i + 1 < argc) pred = atoi(argv[++i]);
     }
     time_t start_t = time(NULL);
     double t_start = wall();
@@ -147,6 +153,10 @@ int main(int argc, char** argv) {
         return 1;
     }
     capac_enable_timing(ctx, 1);
+    if (pred >= 0 && capac_set_predictor(ctx, pred) != 0) {
+        printf("Error: %s\n", capac_last_error());
+        return 1;
+    }
     double* x = (double*)calloc(n_mat > 0 ? n_mat : 1, sizeof(double));
     double* r = (double*)malloc(sizeof(double) * 3 * (size_t)N);
     double* v = (double*)malloc(sizeof(double) * 3 * (size_t)N);

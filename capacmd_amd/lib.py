@@ -173,6 +173,7 @@ GPU_EXPORTS = [
     "capac_set_theta", "capac_get_theta", "capac_fit_frame",
     "capac_em_begin", "capac_em_step", "capac_em_direction",
     "capac_checkpoint_bytes", "capac_checkpoint_save", "capac_checkpoint_load",
+    "capac_set_predictor", "capac_predictor",
 ]
 NTHETA = 6
 
@@ -222,6 +223,8 @@ def gpu_lib() -> C.CDLL:
         lib.capac_em_begin.argtypes = [C.c_void_p]
         lib.capac_em_step.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
         lib.capac_em_direction.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_set_predictor.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_predictor.argtypes = [C.c_void_p]
         lib.capac_checkpoint_bytes.argtypes = [C.c_void_p]
         lib.capac_checkpoint_bytes.restype = C.c_size_t
         lib.capac_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
@@ -290,6 +293,15 @@ class Context:
     def set_solve_tolerance(self, crit: float):
         """BiCGSTAB stop test rTr < crit bTb (the reference's crit is 1e-8)."""
         _check(self.lib.capac_set_solve_tolerance(self.h, float(crit)), "capac_set_solve_tolerance")
+
+    def set_predictor(self, order: int):
+        """Warm start of the MD solves: 0 previous solution (the reference's),
+        1 linear, 2 quadratic extrapolation (capac_set_predictor)."""
+        _check(self.lib.capac_set_predictor(self.h, int(order)), "capac_set_predictor")
+
+    @property
+    def predictor(self) -> int:
+        return int(self.lib.capac_predictor(self.h))
 
     def set_theta(self, theta):
         """Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC); see capac_set_theta."""

@@ -174,6 +174,18 @@ double capac_skin(const capac_ctx* ctx);
  * studies can solve both precisions far below the reference's tolerance. */
 int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 
+/* Warm start of each MD step's CPIM solve (capac_md_run).  Order 0 starts
+ * from the previous step's solution, as the reference does
+ * (bicg_stab.c:303-306); order 1 extrapolates the last two solutions
+ * linearly, order 2 the last three quadratically, which takes far fewer
+ * iterations to the same stop test.  Defaults: 0 for CAPAC_PREC_FP64 (the
+ * reference's iteration path), CAPAC_DEFAULT_PREDICTOR_FP32 for
+ * CAPAC_PREC_FP32_REFINED.  capac_md_begin starts the stored solutions;
+ * capac_set_state with positions or a solution, and capac_set_box, clear
+ * them.  No counterpart in the reference. */
+int capac_set_predictor(capac_ctx* ctx, int order);
+int capac_predictor(const capac_ctx* ctx);
+
 /* ---- restart checkpoint (no counterpart in the reference) ----
  * The reference restarts from the last traj.gro frame, which keeps
  * positions, velocities, vQ[0] and vP only (file.c:152-160, 215-229): the
@@ -181,7 +193,8 @@ int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
  * a restarted run leaves the trajectory it continues.  A checkpoint holds
  * everything the MD loop carries between steps (positions, velocities,
  * forces, vec_pq, the whole chain, the heating ramp, the neighbour-row
- * build positions) so that capac_checkpoint_load followed by capac_md_run
+ * build positions, the predictor's stored solutions and order) so that
+ * capac_checkpoint_load followed by capac_md_run
  * continues the saved run step for step, bit for bit on the same device and
  * precision.  Save between capac_md_run calls; `step` is stored as given.
  * Loading checks that the checkpoint belongs to this system (atom counts,
@@ -212,6 +225,7 @@ int capac_em_direction(capac_ctx* ctx, int cg);
  * the Sutton-Chen energy scale and embedding constant.  Setting theta
  * changes every later force call. */
 #define CAPAC_NTHETA 6
+#define CAPAC_DEFAULT_PREDICTOR_FP32 2
 int capac_set_theta(capac_ctx* ctx, const double theta[CAPAC_NTHETA]);
 int capac_get_theta(capac_ctx* ctx, double theta[CAPAC_NTHETA]);
 
