@@ -1187,9 +1187,11 @@ extern "C" int capac_time_kernel(capac_ctx* c, int kernel, int reps, double* ms)
             case CAPAC_KERNEL_CPIM_MM: rc = cpim_mm_only(c); break;
             case CAPAC_KERNEL_APPLY64: rc = cpim_apply64(c, c->x, NULL, scratch, NULL, NULL); break;
             case CAPAC_KERNEL_SOLVE:
-                // the last force call's system from its warm start (kept in
-                // x_backup); the result lands in x as it did then
-                rc = copy_d(c, c->x_backup, c->x, c->n_mat);
+                // the last force call's system from the reference's warm
+                // start: after MD steps the previous step's solution (second
+                // newest stored), else the call's own start (x_backup); the
+                // result lands in x as it did then
+                rc = copy_d(c, c->pred_count >= 2 ? c->x_hist[1] : c->x_backup, c->x, c->n_mat);
                 if (!rc) {
                     rc = solver_run(c, c->b, c->x);
                     if (rc == CAPAC_ERR_SOLVER) rc = 0;

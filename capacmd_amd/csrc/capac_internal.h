@@ -177,10 +177,27 @@ __device__ __forceinline__ double dist2_exact(double x, double y, double z) {
 }
 
 // wavefront sum of a double, fixed butterfly order (deterministic)
+// Wavefront sum through DPP lane moves (VALU) instead of LDS-crossbar
+// shuffles: quad sums, 8- and 16-lane sums by mirrors, then the row
+// broadcasts gather the four rows into lane 63, which every lane reads.
+// The order of the additions is fixed, so results are reproducible.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, CAPAC_WAVE);
-    return v;
+    v += dpp_move<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_move<0x141, 0xF>(v);   // row_half_mirror: 8-lane sums
+    v += dpp_move<0x140, 0xF>(v);   // row_mirror: 16-lane sums
+    v += dpp_move<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // ------------------------------------------------------------------ launchers

@@ -322,8 +322,11 @@ int capac_set_thermostat(capac_ctx* ctx, const double* vq, double vp);
 #define CAPAC_KERNEL_NEIGHBOUR 3
 #define CAPAC_KERNEL_CPIM_MM   4
 #define CAPAC_KERNEL_APPLY64   5   /* FP64 relay product recomputed on the fly (refinement residual) */
-#define CAPAC_KERNEL_SOLVE     6   /* the whole CPIM solve of the last force call, from its warm start,
-                                      nothing else running: the reference's timer category 7 */
+#define CAPAC_KERNEL_SOLVE     6   /* the whole CPIM solve of the last force call, nothing else
+                                      running: the reference's timer category 7.  After MD steps it
+                                      starts from the previous step's solution (the reference's warm
+                                      start) whatever the predictor, so both precisions time the
+                                      same start; otherwise from the last call's own start */
 int capac_time_kernel(capac_ctx* ctx, int kernel, int reps, double* ms_per_launch);
 
 int capac_get_timing(capac_ctx* ctx, capac_timing* t);
