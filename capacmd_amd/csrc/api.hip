@@ -609,15 +609,18 @@ static void mark(capac_ctx* c, int k) {
 
 // grid of the pair kernel while it runs beside the solve.  Below 1024 metal
 // atoms the solve is a chain of short, latency-bound launches: the pair
-// kernel gets one workgroup per compute unit, so that the solver's
-// workgroups always find free slots (a full grid fills every CU and delays
-// each of them).  Above, the solve streams the relay at HBM speed and the
-// pair work of the larger box is the longer of the two: the full grid lets
-// it finish early.  Measured on MI355X (ms per MD step): Au561 FP32 with the
-// quadratic predictor 1.012 at 261 workgroups, 1.018 at 2048; FP64 1.725 at
-// 130, 1.739 at 261, 1.81 at 2048; Au2057 FP32 3.42 at 634, 3.21 at 2048.
+// kernel gets one or one and a half workgroups per compute unit, so that the
+// solver's workgroups always find free slots (fuller grids delay each of
+// them), the larger share when the warm-start predictor has made the solve
+// short.  Above, the solve streams the relay at HBM speed and the pair work
+// of the larger box is the longer of the two: the full grid lets it finish
+// early.  Measured on MI355X (ms per MD step): Au561 FP32-refined with the
+// quadratic predictor 0.877 at 256 workgroups, 0.827 at 384, 0.965 at 512,
+// 0.925 at 768; FP64 1.589 at 256, 1.603 at 384, 1.758 at 512; Au2057
+// FP32-refined 3.42 at 634, 3.21 at 2048.
 static int side_pair_blocks(const capac_ctx* c) {
-    return c->bc.Nm < 1024 ? 256 : 0;
+    if (c->bc.Nm >= 1024) return 0;
+    return c->pred_order > 0 ? 384 : 256;
 }
 
 // pair and Sutton-Chen forces on the side stream, after the rows are built

@@ -9,6 +9,7 @@
 // both ends, so pair energies and the virial carry a factor 1/2.
 // FP64-VALU bound (erfc + exp per charged pair).
 #include "capac_internal.h"
+#include "erfcx_table.h"
 
 namespace {
 
@@ -21,8 +22,25 @@ struct PairOut {
 // The reference's expressions with each division by r, r^2, r^6 or the erf
 // width replaced by a product with the reciprocal, computed once per pair
 // (one FP64 division instead of up to five; results move by rounding only).
+// erfc(x) from exp(-x^2), already needed by the damped-shifted-force term:
+// erfc = exp(-x^2) erfcx(x), erfcx from the piecewise polynomials of
+// erfcx_table.h on [0, 6) (within 1 ulp of 60-digit values), the library
+// erfc elsewhere.  With exp(-alpha^2 r^2) in place of exp(-(alpha r)^2) the
+// exponent differs by rounding only (a relative 1e-15 in erfc).
+__device__ __forceinline__ double erfc_from_exp(double x, double ex, const double* __restrict__ coef) {
+    if (!(x >= 0.0 && x < kErfcxHi)) return erfc(x);
+    const int k = (int)(x * 4.0);
+    const double t = (x - (k + 0.5) * 0.25) * 8.0;
+    const double* c = coef + k * kErfcxTerms;
+    double p = c[kErfcxTerms - 1];
+#pragma unroll
+    for (int i = kErfcxTerms - 2; i >= 0; --i) p = fma(p, t, c[i]);
+    return ex * p;
+}
+
 __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
-                                          double dx, double dy, double dz, double r2, PairOut& o) {
+                                          double dx, double dy, double dz, double r2, PairOut& o,
+                                          const double* __restrict__ erfcx_coef) {
     double flj = 0.0, fqq = 0.0;
     const double rij = sqrt(r2);
     const double inv_r = 1.0 / rij;
@@ -47,8 +65,8 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
             double qq = K_FQQ * qi * qj;
             if (bc.use_coulomb == 1) {
                 // damped shifted force (force.c:787-795)
-                double erfc_a = erfc(bc.alpha * rij);
                 double ex = exp(-bc.alpha * bc.alpha * r2);
+                double erfc_a = erfc_from_exp(bc.alpha * rij, ex, erfcx_coef);
                 fqq = qq * ((erfc_a * inv_r2 + bc.a2_sqrtPI * ex * inv_r) - bc.wolfConst) * inv_r;
                 o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
             } else {
@@ -83,8 +101,11 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
        double* __restrict__ f, int accumulate, double* __restrict__ partial) {
     __shared__ PairParam s_tab[64];
     __shared__ double s_red[kWavesPerBlock][11];
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     const int N = bc.N, nt = bc.n_types;
     for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) s_tab[k] = table[k];
+    for (int k = threadIdx.x; k < kErfcxPieces * kErfcxTerms; k += blockDim.x)
+        s_erfcx[k] = kErfcxCoef[k / kErfcxTerms][k % kErfcxTerms];
     __syncthreads();
     const int lane = threadIdx.x & (CAPAC_WAVE - 1);
     const int wave = threadIdx.x / CAPAC_WAVE;
@@ -103,7 +124,7 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
             if (!(r2 < bc.rc2)) continue;     // rows reach rc + skin
-            pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o);
+            pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o, s_erfcx);
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {

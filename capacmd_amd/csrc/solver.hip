@@ -514,7 +514,10 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
     spmv(c, S->y, S->v, 1, S->part1, S->sc);
     k_vec_s<<<vb, kVecThreads, 0, st>>>(n, 1, S->n_groups, S->part1, vb, S->part4, S->r, S->v, S->K,
                                         S->r0, S->s, S->z, S->part2, S->sc);
-    int it = 1, batch = first_batch > 1 ? first_batch : 2, launched = 0;
+    // later batches start small: a first batch that falls short usually
+    // needs only a few more iterations, and every unit launched after the
+    // stop is a no-op that still costs four launches
+    int it = 1, batch = first_batch > 1 ? first_batch : 2, launched = 0, next = 2;
     for (;;) {
         for (int u = 0; u < batch; ++u, ++it) {
             spmv(c, S->z, S->t, 2, S->part3, S->sc);
@@ -540,8 +543,8 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
             capac_set_error("BiCGSTAB did not report completion after %d iterations", launched);
             return CAPAC_ERR_SOLVER;
         }
-        batch = batch < 16 ? 2 * batch : 16;
-        if (batch < 2) batch = 2;
+        batch = next;
+        next = next < 16 ? 2 * next : 16;
     }
     out->iters = S->sc_host->iter;
     out->kind = S->sc_host->kind;
