@@ -378,7 +378,6 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     TRY(upload(&c->cell_tmp, (const int*)NULL, N));
     TRY(upload(&c->n_front, (const int*)NULL, N));
     TRY(upload(&c->n_back, (const int*)NULL, N));
-    TRY(upload(&c->nb_overflow, (const int*)NULL, 4));
     {
         // room for the densest liquid or metal around a sphere of radius rc + skin
         double rl = b.rc + c->skin;
@@ -404,13 +403,19 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->n_partial = total;
         c->n_pending = 0;
         TRY(upload(&c->partial, (const double*)NULL, (size_t)total * 16));
-        TRY(upload(&c->red, (const double*)NULL, 256));
-        TRY(hipHostMalloc(&c->red_host, 320 * sizeof(double), hipHostMallocDefault) == hipSuccess
+        // one buffer for everything the host reads back, laid out as its
+        // host copy so one transfer fetches it: [0, 256) reduction outputs,
+        // [256, 258) the neighbour-row overflow flags, [260, 292) the MD
+        // scalars, [292, 294) the RATTLE failure flags
+        TRY(upload(&c->red, (const double*)NULL, kRedWords));
+        TRY(hipMemset(c->red, 0, kRedWords * sizeof(double)) == hipSuccess ? 0 : CAPAC_ERR_HIP);
+        TRY(hipHostMalloc(&c->red_host, kRedWords * sizeof(double), hipHostMallocDefault) == hipSuccess
                 ? 0 : CAPAC_ERR_HIP);
+        c->nb_overflow = (int*)(c->red + 256);
+        c->md = c->red + 260;
+        c->md_status = (int*)(c->red + 292);
     }
     TRY(upload(&c->isd, (const double*)NULL, Nm));
-    TRY(upload(&c->md, (const double*)NULL, 32));
-    TRY(upload(&c->md_status, (const int*)NULL, 4));
     {
         double qm = (double)p->ndf * (K_BOLTZ * p->ref_temp) * p->tau_t * p->tau_t;  // mpi-main.c:552
         double h[32] = {0};
@@ -449,8 +454,8 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->cstr_i, c->cstr_j, c->cstr_r, c->np_first, c->np_last, c->np_of_metal,
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
-                    c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->nb_overflow, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32, c->md, c->md_status,
+                    c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32,
                     c->x_hist[0], c->x_hist[1], c->x_hist[2]};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
@@ -588,9 +593,8 @@ extern "C" int capac_set_solve_tolerance(capac_ctx* c, double rtr_over_btb) {
 static int fetch_red(capac_ctx* c) {
     int rc = flush_reductions(c);
     if (rc) return rc;
-    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, 256 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CHECK_HIP(hipMemcpyAsync(c->red_host + 256, c->nb_overflow, 4 * sizeof(int), hipMemcpyDeviceToHost,
-                             c->stream));
+    // the reduction outputs and the overflow flags in one transfer
+    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, 260 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -911,10 +915,9 @@ extern "C" int capac_solve_stats(capac_ctx* c, int* iters, double* rTr, double* 
 static int fetch_md(capac_ctx* c) {
     int rc = flush_reductions(c);
     if (rc) return rc;
-    CHECK_HIP(hipMemcpyAsync(c->red_host + 260, c->md, 32 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CHECK_HIP(hipMemcpyAsync(c->red_host + 292, c->md_status, 4 * sizeof(int), hipMemcpyDeviceToHost,
-                             c->stream));
-    CHECK_HIP(hipMemcpyAsync(c->red_host + 300, c->red + 64, 16 * sizeof(double), hipMemcpyDeviceToHost,
+    // the MD scalars, RATTLE flags and the RATTLE_2 virial (red[64..73)),
+    // with everything between them, in one transfer
+    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, kRedWords * sizeof(double), hipMemcpyDeviceToHost,
                              c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     return 0;
@@ -961,8 +964,7 @@ extern "C" int capac_em_direction(capac_ctx* c, int cg) {
 
 extern "C" int capac_md_begin(capac_ctx* c) {
     if (!c) return CAPAC_ERR_ARG;
-    int rc = md_remove_com(c);
-    if (!rc) rc = md_kinetic(c);
+    int rc = md_remove_com_kinetic(c);
     if (!rc) rc = md_fmax(c);
     if (!rc) rc = fill_i(c, c->md_status, 4, 0);
     if (!rc) rc = clear_red(c, 64, 16);
@@ -988,9 +990,7 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         double kT = K_BOLTZ * ref_temp;
         int rc = 0;
         mark(c, 8);
-        if (c->nvt) rc = md_nhc(c, kT);
-        if (!rc) rc = md_kick(c);
-        if (!rc) rc = md_drift(c);
+        rc = md_first_half(c, kT);
         if (!rc) rc = md_rattle1(c);
         if (rc) return rc;
         solver_predict(c);
@@ -1000,10 +1000,7 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         mark(c, 9);
         if (!rc) rc = md_kick(c);
         if (!rc) rc = md_rattle2(c);
-        if (!rc) rc = md_kinetic(c);
-        if (!rc && c->nvt) rc = md_nhc(c, kT);
-        if (!rc) rc = md_remove_com(c);
-        if (!rc) rc = md_kinetic(c);
+        if (!rc) rc = md_second_half(c, kT);
         int save = n_save > 0 && step % n_save == 0;
         if (!rc && save) rc = md_wrap(c);
         if (!rc && save) rc = md_fmax(c);
@@ -1044,7 +1041,7 @@ extern "C" int capac_md_observables(capac_ctx* c, double* eksum, double* temp, d
         for (int k = 1; k < CAPAC_NPOT; ++k) pot[0] += pot[k];    // sum_potential (force.c:322-331)
     }
     if (vir)
-        for (int k = 0; k < 9; ++k) vir[k] = c->vir[k] + c->red_host[300 + k];
+        for (int k = 0; k < 9; ++k) vir[k] = c->vir[k] + c->red_host[64 + k];
     return 0;
 }
 

@@ -71,6 +71,8 @@ struct SolveScalars {
 
 struct Solver;              // solver.hip
 
+constexpr int kRedWords = 296;
+
 struct capac_ctx {
     int device, precision;
     BoxConst bc;
@@ -113,8 +115,9 @@ struct capac_ctx {
     int seg_off[8], seg_cap[8];  // segment offsets and capacities, in blocks of 16 doubles
     int n_pending;               // queued reductions, launched together by flush_reductions
     int pend_seg[8], pend_nblk[8], pend_nval[8], pend_red[8];
-    double* red;                // reduction results (64 doubles)
-    double* red_host;           // pinned mirror
+    double* red;                // kRedWords: reduction results [0, 256), overflow flags at 256,
+                                // MD scalars at 260 (md), RATTLE flags at 292 (md_status)
+    double* red_host;           // pinned mirror, same layout
 
     // Sutton-Chen
     double* isd;
@@ -244,12 +247,11 @@ int fit_frame_loss_grad(capac_ctx* c, const double* f_ref, double e_ref, double 
                         double* grad);
 // md.hip
 int md_kick(capac_ctx* c);
-int md_drift(capac_ctx* c);
 int md_rattle1(capac_ctx* c);
 int md_rattle2(capac_ctx* c);
-int md_kinetic(capac_ctx* c);
-int md_nhc(capac_ctx* c, double kT);
-int md_remove_com(capac_ctx* c);
+int md_first_half(capac_ctx* c, double kT);    // chain, scale, half kick, drift
+int md_second_half(capac_ctx* c, double kT);   // kinetic energy, chain, scale, COM removal
+int md_remove_com_kinetic(capac_ctx* c);   // after the first force call
 int md_wrap(capac_ctx* c);
 int md_fmax(capac_ctx* c);
 int ext_field(capac_ctx* c);

@@ -41,15 +41,28 @@ __global__ void k_kick(int N, int Nm, int fix_metal, double dt_2, const double* 
     v[2 * N + i] += dt_2 * f[2 * N + i] * im;
 }
 
-__global__ void k_drift(int N, int Nm, int fix_metal, double dt, double* __restrict__ r,
-                        const double* __restrict__ v, double* __restrict__ old_r) {
+// First half of a step in one pass: the thermostat's velocity scale (nvt:
+// every atom, as k_scale_v), then, for every atom that moves, the half kick
+// and the drift of k_kick and k_drift; the same operations in the same order.
+__global__ void k_scale_kick_drift(int N, int Nm, int fix_metal, int scale, double dt_2, double dt,
+                                   const double* __restrict__ md, const double* __restrict__ f,
+                                   const double* __restrict__ inv_mass, double* __restrict__ v,
+                                   double* __restrict__ r, double* __restrict__ old_r) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (scale) {
+        const double s = md[22];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c * N + i] *= s;
+    }
     if (fix_metal == 1 && i < Nm) return;
+    double im = inv_mass[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        old_r[c * N + i] = r[c * N + i];
-        r[c * N + i] += dt * v[c * N + i];
+        const size_t k = (size_t)c * N + i;
+        v[k] += dt_2 * f[k] * im;
+        old_r[k] = r[k];
+        r[k] += dt * v[k];
     }
 }
 
@@ -286,16 +299,29 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
 
 // Momentum / kinetic-energy partial sums per block of 256 atoms:
 // [sum m vx, m vy, m vz, m, m vx^2, m vy^2, m vz^2] over atoms with m > 0
-// (kinetic_energy thermostat.c:34-63, remove_comm thermostat.c:69-108)
+// (kinetic_energy thermostat.c:34-63, remove_comm thermostat.c:69-108).
+// pre 1 first scales every velocity by md[22] (the thermostat, as
+// k_scale_v); pre 2 first subtracts the centre-of-mass velocity from the
+// sums in `com` (as k_com_apply); the velocities are written back.
 __global__ void __launch_bounds__(256)
-k_mom_partial(int N, const double* __restrict__ mass, const double* __restrict__ v,
-              double* __restrict__ partial) {
+k_mom_partial(int N, const double* __restrict__ mass, double* __restrict__ v,
+              double* __restrict__ partial, int pre, const double* __restrict__ md,
+              const double* __restrict__ com) {
     __shared__ double s_red[4][7];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i < N) {
         double m = mass[i];
+        if (pre == 1) {
+            const double s = md[22];
+            v[i] *= s; v[N + i] *= s; v[2 * N + i] *= s;
+        } else if (pre == 2 && m > 0.0) {
+            double ms = com[3];
+            v[i] -= com[0] / ms;
+            v[N + i] -= com[1] / ms;
+            v[2 * N + i] -= com[2] / ms;
+        }
         if (m > 0.0) {
             double vx = v[i], vy = v[N + i], vz = v[2 * N + i];
             a[0] = vx * m; a[1] = vy * m; a[2] = vz * m; a[3] = m;
@@ -324,63 +350,101 @@ __global__ void k_ke_finish(const double* __restrict__ sums, int ndf, double* __
     md[1] = ek / (K_BOLTZ * ndf);
 }
 
-// subtract the centre-of-mass velocity from every atom with m > 0
-__global__ void k_com_apply(int N, const double* __restrict__ mass, const double* __restrict__ sums,
-                            double* __restrict__ v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || !(mass[i] > 0.0)) return;
-    double ms = sums[3];
-    v[i] -= sums[0] / ms;
-    v[N + i] -= sums[1] / ms;
-    v[2 * N + i] -= sums[2] / ms;
+// Taylor polynomial of exp of degree 6 for |x| < 2^-10 (truncation below
+// 2e-25, far under an ulp), evaluated as a short chain in x^2 (Estrin)
+__device__ __forceinline__ double exp_small(double x) {
+    const double x2 = x * x;
+    const double a = 0.5 + x * (1.0 / 6.0);
+    const double b = (1.0 / 24.0) + x * (1.0 / 120.0);
+    const double c = b + x2 * (1.0 / 720.0);
+    return (1.0 + x) + x2 * (a + x2 * c);
+}
+
+// The chain's sweeps (thermostat.c:135-186).  SMALL uses exp_small for every
+// exponential and returns false as soon as an argument leaves |x| < 2^-10
+// (the caller then starts over with the library exp); along a trajectory the
+// arguments, wdt/40 times a thermostat velocity, are of order 1e-4.  The
+// chain is one thread's serial dependency, so each exp's latency is paid in
+// full: the short polynomial takes a fraction of the library's.
+template <bool SMALL>
+__device__ __forceinline__ bool nhc_sweeps(double vQ[CAPAC_NHC], double aQ[CAPAC_NHC], double dt, double kT,
+                                           double NkT, double qMass, double inv_q, double eKsum,
+                                           double& scale_out) {
+    constexpr int Nc = 5, Nys = 5, M = CAPAC_NHC;
+    const double w1 = 0.41449077179437571194, w3 = -0.65796308717750284778;
+    // the step fractions of each weight, as the reference forms them
+    const double c8[2] = {-w1 * dt / (8 * Nc), -w3 * dt / (8 * Nc)};
+    const double c4[2] = {w1 * dt / (4 * Nc), w3 * dt / (4 * Nc)};
+    const double c2[2] = {-w1 * dt / (2 * Nc), -w3 * dt / (2 * Nc)};
+    double aa, x, scale = 1.0;
+    bool ok = true;
+#define NHC_EXP(arg) (x = (arg), SMALL ? (ok = ok && fabs(x) < 0x1p-10, exp_small(x)) : exp(x))
+    for (int ic = 0; ic < Nc; ++ic) {
+#pragma unroll
+        for (int iys = 0; iys < Nys; ++iys) {
+            const int w = iys == 2 ? 1 : 0;
+            const double wv = w ? w3 : w1;
+            vQ[M - 1] += aQ[M - 1] * wv * dt / (4 * Nc);
+#pragma unroll
+            for (int im = 0; im < M - 1; ++im) {
+                aa = NHC_EXP(c8[w] * vQ[M - (im + 1)]);
+                vQ[M - 1 - (im + 1)] = vQ[M - 1 - (im + 1)] * aa * aa + c4[w] * aQ[M - 1 - (im + 1)] * aa;
+            }
+            aa = NHC_EXP(c2[w] * vQ[0]);
+            scale *= aa;
+            aQ[0] = (eKsum * scale * scale - NkT) * inv_q;
+#pragma unroll
+            for (int im = 0; im < M - 1; ++im) {
+                aa = NHC_EXP(c8[w] * vQ[im + 1]);
+                vQ[im] = vQ[im] * aa * aa + c4[w] * aQ[im] * aa;
+                aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) * inv_q;
+            }
+            vQ[M - 1] += aQ[M - 1] * wv * dt / (4 * Nc);
+        }
+        if (SMALL && !ok) return false;
+    }
+#undef NHC_EXP
+    scale_out = scale;
+    return true;
 }
 
 // nose_hoover_chain (thermostat.c:117-191); the velocity scale goes to md[22].
 // The chain lives in registers (all indices are compile-time after unrolling).
-__global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf) {
+// With sums, eKsum and the temperature come from the momentum sums first
+// (as k_ke_finish).  The divisions by qMass are products with its
+// reciprocal, and small exponentials use exp_small: both differ from the
+// reference's operations by rounding only.
+__global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, const double* __restrict__ sums) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    constexpr int Nc = 5, Nys = 5, M = CAPAC_NHC;
-    const double w1 = 0.41449077179437571194, w3 = -0.65796308717750284778;
+    constexpr int M = CAPAC_NHC;
+    if (sums) {
+        const double ek = sums[4] + sums[5] + sums[6];
+        md[0] = ek;
+        md[1] = ek / (K_BOLTZ * ndf);
+    }
     double vQ[M], aQ[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) { vQ[k] = md[2 + k]; aQ[k] = md[12 + k]; }
+    for (int k = 0; k < M; ++k) vQ[k] = md[2 + k];
     const double qMass = md[25], eKsum = md[0];
+    const double inv_q = 1.0 / qMass;
     const double NkT = kT * ndf;
-    double aa, scale = 1.0;
-    aQ[0] = (eKsum - NkT) / qMass;
+    aQ[0] = (eKsum - NkT) * inv_q;
 #pragma unroll
-    for (int im = 0; im < M - 1; ++im) aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) / qMass;
-    for (int ic = 0; ic < Nc; ++ic)
+    for (int im = 0; im < M - 1; ++im) aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) * inv_q;
+    double vQ0[M], aQ0[M];
 #pragma unroll
-        for (int iys = 0; iys < Nys; ++iys) {
-            const double wdt = (iys == 2 ? w3 : w1) * dt;
-            vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
+    for (int k = 0; k < M; ++k) { vQ0[k] = vQ[k]; aQ0[k] = aQ[k]; }
+    double scale = 1.0;
+    if (!nhc_sweeps<true>(vQ, aQ, dt, kT, NkT, qMass, inv_q, eKsum, scale)) {
 #pragma unroll
-            for (int im = 0; im < M - 1; ++im) {
-                aa = exp(-wdt / (8 * Nc) * vQ[M - (im + 1)]);
-                vQ[M - 1 - (im + 1)] = vQ[M - 1 - (im + 1)] * aa * aa + wdt / (4 * Nc) * aQ[M - 1 - (im + 1)] * aa;
-            }
-            aa = exp(-wdt / (2 * Nc) * vQ[0]);
-            scale *= aa;
-            aQ[0] = (eKsum * scale * scale - NkT) / qMass;
-#pragma unroll
-            for (int im = 0; im < M - 1; ++im) {
-                aa = exp(-wdt / (8 * Nc) * vQ[im + 1]);
-                vQ[im] = vQ[im] * aa * aa + wdt / (4 * Nc) * aQ[im] * aa;
-                aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) / qMass;
-            }
-            vQ[M - 1] += aQ[M - 1] * wdt / (4 * Nc);
-        }
+        for (int k = 0; k < M; ++k) { vQ[k] = vQ0[k]; aQ[k] = aQ0[k]; }
+        nhc_sweeps<false>(vQ, aQ, dt, kT, NkT, qMass, inv_q, eKsum, scale);
+    }
 #pragma unroll
     for (int k = 0; k < M; ++k) { md[2 + k] = vQ[k]; md[12 + k] = aQ[k]; }
     md[22] = scale;
     md[0] = eKsum * (scale * scale);
     md[1] *= scale * scale;
-}
-
-__global__ void k_scale_v(int n3, const double* __restrict__ md, double* __restrict__ v) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n3) v[k] *= md[22];
 }
 
 // apply_pbc (force.c:127-165): wrap the middle atom of each molecule into the
@@ -458,13 +522,6 @@ int md_kick(capac_ctx* c) {
     return 0;
 }
 
-int md_drift(capac_ctx* c) {
-    k_drift<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->bc.Nm, c->fix_metal, c->dt, c->r,
-                                                       c->v, c->old_r);
-    CHECK_HIP(hipGetLastError());
-    return 0;
-}
-
 int md_rattle1(capac_ctx* c) {
     if (c->n_cstr == 0) return 0;
     k_rattle1<<<grid(c->bc.n_mols, 256), 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt,
@@ -489,36 +546,63 @@ int md_rattle2(capac_ctx* c) {
     return reduce_partials(c, SEG_RATTLE, nblk, 9, 64);
 }
 
-// red[96..103) = momentum / kinetic sums
-static int mom_sums(capac_ctx* c) {
+// red[96..103) = momentum / kinetic sums, after the optional pre-pass of
+// k_mom_partial (1: thermostat scale, 2: centre-of-mass removal with the
+// sums already in red[96..103))
+static int mom_sums(capac_ctx* c, int pre = 0) {
     int nblk = grid(c->bc.N, 256);
-    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM));
+    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM), pre, c->md,
+                                               c->red + 96);
     CHECK_HIP(hipGetLastError());
     int rc = reduce_partials(c, SEG_MOM, nblk, 7, 96);
     return rc ? rc : flush_reductions(c);      // read on the device right after
 }
 
-int md_kinetic(capac_ctx* c) {
-    int rc = mom_sums(c);
+// centre-of-mass removal on the sums already in red[96..103), then the
+// kinetic energy and temperature (remove_comm + kinetic_energy)
+static int com_kinetic(capac_ctx* c) {
+    int rc = mom_sums(c, 2);
     if (rc) return rc;
     k_ke_finish<<<1, 64, 0, c->stream>>>(c->red + 96, c->ndf, c->md);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
 
-int md_nhc(capac_ctx* c, double kT) {
-    k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf);
-    k_scale_v<<<grid(3 * c->bc.N, 256), 256, 0, c->stream>>>(3 * c->bc.N, c->md, c->v);
+// remove_comm and kinetic_energy after the first force call (mpi-main.c:632-633)
+int md_remove_com_kinetic(capac_ctx* c) {
+    int rc = mom_sums(c);
+    return rc ? rc : com_kinetic(c);
+}
+
+// First half of a step (mpi-main.c:885-925): nvt chain, scale, half kick
+// and drift, in two launches (k_nhc, k_scale_kick_drift)
+int md_first_half(capac_ctx* c, double kT) {
+    if (c->nvt) {
+        k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, NULL);
+        CHECK_HIP(hipGetLastError());
+    }
+    k_scale_kick_drift<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->bc.Nm, c->fix_metal, c->nvt,
+                                                                  0.5 * c->dt, c->dt, c->md, c->f,
+                                                                  c->inv_mass, c->v, c->r, c->old_r);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
 
-int md_remove_com(capac_ctx* c) {
-    int rc = mom_sums(c);
-    if (rc) return rc;
-    k_com_apply<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->mass, c->red + 96, c->v);
-    CHECK_HIP(hipGetLastError());
-    return 0;
+// Second half after RATTLE_2 (mpi-main.c:955-973): kinetic energy, nvt
+// chain and scale, centre-of-mass removal, kinetic energy.  The scale and
+// the removal are folded into the momentum-sum passes, and the chain takes
+// eKsum straight from the sums.
+int md_second_half(capac_ctx* c, double kT) {
+    int rc;
+    if (c->nvt) {
+        if ((rc = mom_sums(c))) return rc;
+        k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, c->red + 96);
+        CHECK_HIP(hipGetLastError());
+        if ((rc = mom_sums(c, 1))) return rc;
+    } else if ((rc = mom_sums(c))) {
+        return rc;
+    }
+    return com_kinetic(c);
 }
 
 int md_wrap(capac_ctx* c) {
