@@ -221,6 +221,11 @@ static int validate(const capac_params* p) {
                             p->nb_funct[k]);
             return CAPAC_ERR_UNSUPPORTED;
         }
+    if (p->n_mols >= (1 << 26)) {
+        // the neighbour build packs molecule * 32 + index into one int
+        capac_set_error("%d molecules: at most 2^26 are supported", p->n_mols);
+        return CAPAC_ERR_UNSUPPORTED;
+    }
     for (int i = 0; i < p->n_atoms; ++i) {
         if (p->atomtype[i] < 0 || p->atomtype[i] >= p->n_types) {
             capac_set_error("atom %d has type %d outside the pair table", i, p->atomtype[i]);

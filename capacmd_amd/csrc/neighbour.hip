@@ -179,8 +179,19 @@ struct Stencil {
 // candidates per pass, compacting the accepted ones with a ballot.
 constexpr int kCap = 1024;        // staged candidates per pass
 constexpr int kMaxHome = 512;     // home atoms per pass
+// eight wavefronts per cell: the candidate loop is latency-bound, and more
+// waves per compute unit hide it (Au561: 383 us per build at four, 259 at
+// eight, 280 at sixteen)
+constexpr int kBuildThreads = 512;
 
-__global__ void __launch_bounds__(256)
+// a staged candidate, read with two 16-byte LDS loads; ml = molecule * 32 +
+// index in the molecule (the exclusion masks are 32-bit)
+struct alignas(16) Cand {
+    double x, y, z;
+    int id, ml;
+};
+
+__global__ void __launch_bounds__(kBuildThreads)
 k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
            const double* __restrict__ sz, const int* __restrict__ smol,
            const int* __restrict__ sloc, const int* __restrict__ cell_atoms,
@@ -188,9 +199,8 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
            int n0, int n1, int n2, const Stencil* __restrict__ st, int stride,
            int* __restrict__ n_front, int* __restrict__ n_back, int* __restrict__ nbr,
            int* __restrict__ overflow, const unsigned long long* __restrict__ need) {
-    __shared__ double c_x[kCap], c_y[kCap], c_z[kCap];
+    __shared__ Cand c_c[kCap];
     if (!need[1]) return;
-    __shared__ int c_id[kCap], c_mol[kCap], c_loc[kCap];
     __shared__ int h_nf[kMaxHome], h_nb[kMaxHome];
     __shared__ int seg_lo[125], seg_off[126];
     const int Nm = bc.Nm;
@@ -231,11 +241,10 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
                     if (seg_off[mid] <= g) lo = mid; else hi = mid - 1;
                 }
                 const int k = seg_lo[lo] + (g - seg_off[lo]);
-                c_x[f] = sx[k]; c_y[f] = sy[k]; c_z[f] = sz[k];
-                c_id[f] = cell_atoms[k]; c_mol[f] = smol[k]; c_loc[f] = sloc[k];
+                c_c[f] = Cand{sx[k], sy[k], sz[k], cell_atoms[k], smol[k] * 32 + sloc[k]};
             }
             __syncthreads();
-            for (int h = wave; h < nh; h += 4) {
+            for (int h = wave; h < nh; h += kBuildThreads / 64) {
                 const int hs = hbase + h;
                 const int i = cell_atoms[hs];
                 const double xi = sx[hs], yi = sy[hs], zi = sz[hs];
@@ -249,10 +258,11 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
                     bool acc = false;
                     int j = -1;
                     if (q < total) {
-                        j = c_id[q];
-                        bool excluded = (c_mol[q] == moli) && ((exi >> c_loc[q]) & 1u);
+                        const Cand cd = c_c[q];
+                        j = cd.id;
+                        bool excluded = ((cd.ml >> 5) == moli) && ((exi >> (cd.ml & 31)) & 1u);
                         if (j != i && !excluded) {
-                            double dx = c_x[q] - xi, dy = c_y[q] - yi, dz = c_z[q] - zi;
+                            double dx = cd.x - xi, dy = cd.y - yi, dz = cd.z - zi;
                             pbc12(bc, dx, dy, dz);
                             double r2 = dist2_exact(dx, dy, dz);
                             acc = r2 < bc.rl2;
@@ -407,7 +417,7 @@ static int nb_launch(capac_ctx* c) {
     k_cell_sort<<<c->n_cell, 256, 0, s>>>(c->cell_start, c->cell_tmp, c->r, c->mol_id, c->local_idx, N,
                                           c->cell_atoms, c->sorted_r, c->sorted_r + N,
                                           c->sorted_r + 2 * N, c->sorted_mol, c->sorted_loc, need);
-    k_nb_build<<<c->n_cell, 256, 0, s>>>(c->sorted_r, c->sorted_r + N, c->sorted_r + 2 * N,
+    k_nb_build<<<c->n_cell, kBuildThreads, 0, s>>>(c->sorted_r, c->sorted_r + N, c->sorted_r + 2 * N,
                                          c->sorted_mol, c->sorted_loc, c->cell_atoms, c->cell_start,
                                          c->excl, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
                                          (const Stencil*)c->stencil, c->nb_stride, c->n_front,
