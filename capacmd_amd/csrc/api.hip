@@ -401,24 +401,21 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->seg_cap[SEG_MW] = metal_blocks;
         c->seg_cap[SEG_MM] = metal_blocks;
         c->seg_cap[SEG_EXT] = (N + 255) / 256 + 1;
-        c->seg_cap[SEG_RATTLE] = (p->n_mols + 255) / 256 + 1;
+        c->seg_cap[SEG_RATTLE] = (p->n_mols + 63) / 64 + 1;      // workgroups of 64 or more
         c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;
         int total = 0;
         for (int s = 0; s < SEG_COUNT; ++s) { c->seg_off[s] = total; total += c->seg_cap[s]; }
         c->n_partial = total;
         c->n_pending = 0;
         TRY(upload(&c->partial, (const double*)NULL, (size_t)total * 16));
-        // one buffer for everything the host reads back, laid out as its
-        // host copy so one transfer fetches it: [0, 256) reduction outputs,
-        // [256, 258) the neighbour-row overflow flags, [260, 292) the MD
-        // scalars, [292, 294) the RATTLE failure flags
+        // one buffer for everything the host reads back (layout at kRedWords)
         TRY(upload(&c->red, (const double*)NULL, kRedWords));
         TRY(hipMemset(c->red, 0, kRedWords * sizeof(double)) == hipSuccess ? 0 : CAPAC_ERR_HIP);
         TRY(hipHostMalloc(&c->red_host, kRedWords * sizeof(double), hipHostMallocDefault) == hipSuccess
                 ? 0 : CAPAC_ERR_HIP);
-        c->nb_overflow = (int*)(c->red + 256);
-        c->md = c->red + 260;
-        c->md_status = (int*)(c->red + 292);
+        c->nb_overflow = (int*)(c->red + kRedFlags);
+        c->md = c->red + kRedMd;
+        c->md_status = (int*)(c->red + kRedStatus);
     }
     TRY(upload(&c->isd, (const double*)NULL, Nm));
     {
@@ -599,12 +596,12 @@ static int fetch_red(capac_ctx* c) {
     int rc = flush_reductions(c);
     if (rc) return rc;
     // the reduction outputs and the overflow flags in one transfer
-    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, 260 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, kRedMd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 
-static int nb_overflowed(capac_ctx* c) { return ((int*)(c->red_host + 256))[0] != 0; }
+static int nb_overflowed(capac_ctx* c) { return ((int*)(c->red_host + kRedFlags))[0] != 0; }
 
 static int grow_nbr(capac_ctx* c) {
     int stride = c->nb_stride * 2;
@@ -654,9 +651,16 @@ static int side_forces(capac_ctx* c) {
     return 0;
 }
 
-// one mpi_force: forces into c->f, pot/vir into c->pot/c->vir
-static int force_compute(capac_ctx* c) {
+static void take_force_results(capac_ctx* c);
+
+// one mpi_force: forces into c->f, pot/vir into c->pot/c->vir.
+// deferred (MD steps with CPIM unknowns): the call does not synchronise.  The
+// neighbour-row flags come back with the solver's last synchronisation (they
+// share its transfer), and the caller collects the energies with
+// take_force_results after its own readback at the end of the step.
+static int force_compute(capac_ctx* c, bool deferred = false) {
     int rc;
+    deferred = deferred && c->n_mat > 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
         mark(c, 0);
         if ((rc = clear_red(c, 0, 256))) return rc;
@@ -679,6 +683,16 @@ static int force_compute(capac_ctx* c) {
         } else {
             mark(c, 4); mark(c, 5);
         }
+        if (deferred && nb_overflowed(c)) {
+            // a neighbour row overflowed (rare): let the side stream's readers
+            // of the rows finish, then grow, restore the warm start and redo
+            CHECK_HIP(hipStreamSynchronize(c->side));
+            c->nb_dirty = 1;
+            if ((rc = grow_nbr(c))) return rc;
+            if ((rc = copy_d(c, c->x_backup, c->x, c->n_mat))) return rc;
+            if (attempt == 7) { capac_set_error("neighbour rows keep overflowing"); return CAPAC_ERR_HIP; }
+            continue;
+        }
         mark(c, 6);
         CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_side, 0));
         if (c->n_mat && (rc = cpim_forces(c))) return rc;
@@ -686,6 +700,7 @@ static int force_compute(capac_ctx* c) {
         int rc2 = ext_field(c);
         if (rc2) return rc2;
         mark(c, 7);
+        if (deferred) break;
         rc2 = fetch_red(c);
         if (rc2) return rc2;
         if (!nb_overflowed(c)) break;
@@ -695,6 +710,13 @@ static int force_compute(capac_ctx* c) {
         if (c->n_mat && (rc2 = copy_d(c, c->x_backup, c->x, c->n_mat))) return rc2;
         if (attempt == 7) { capac_set_error("neighbour rows keep overflowing"); return CAPAC_ERR_HIP; }
     }
+    if (!deferred) take_force_results(c);
+    return rc;
+}
+
+// energies, virial and phase timings of the last force call, from the
+// readback buffer (after a synchronisation that covers the call)
+static void take_force_results(capac_ctx* c) {
     const double* h = c->red_host;
     memset(c->pot, 0, sizeof c->pot);
     memset(c->vir, 0, sizeof c->vir);
@@ -719,7 +741,6 @@ static int force_compute(capac_ctx* c) {
                 c->timing.t[spans[k][2]] += 1e-3 * ms;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[7]) == hipSuccess) c->timing.t[0] += 1e-3 * ms;
     }
-    return rc;
 }
 
 extern "C" int capac_force(capac_ctx* c, double pot[CAPAC_NPOT], double vir[9], double* f) {
@@ -949,7 +970,7 @@ extern "C" int capac_em_step(capac_ctx* c, double length, double pot[CAPAC_NPOT]
     rc = md_fmax(c);
     if (!rc) rc = fetch_md(c);
     if (rc) return rc;
-    const int* st = (const int*)(c->red_host + 292);
+    const int* st = (const int*)(c->red_host + kRedStatus);
     if (st[0]) { capac_set_error("RATTLE_1 not converged!"); return CAPAC_ERR_RATTLE1; }
     if (pot) {
         memcpy(pot, c->pot, sizeof c->pot);
@@ -957,8 +978,8 @@ extern "C" int capac_em_step(capac_ctx* c, double length, double pot[CAPAC_NPOT]
         for (int k = 1; k < CAPAC_NPOT; ++k) e += pot[k];     // sum_potential (force.c:322-331)
         pot[0] = e;
     }
-    if (fmax) *fmax = c->red_host[260 + 23];
-    if (frms) *frms = c->red_host[260 + 24];
+    if (fmax) *fmax = c->red_host[kRedMd + 23];
+    if (frms) *frms = c->red_host[kRedMd + 24];
     return frc;
 }
 
@@ -975,7 +996,7 @@ extern "C" int capac_md_begin(capac_ctx* c) {
     if (!rc) rc = clear_red(c, 64, 16);
     if (!rc) rc = fetch_md(c);
     if (rc) return rc;
-    c->first_temp = c->red_host[260 + 1];
+    c->first_temp = c->red_host[kRedMd + 1];
     c->ext_temp = c->ref_temp0;
     c->cur_ref_temp = c->ref_temp0;
     // the first force call's solution starts the predictor's history
@@ -999,7 +1020,7 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (!rc) rc = md_rattle1(c);
         if (rc) return rc;
         solver_predict(c);
-        rc = force_compute(c);
+        rc = force_compute(c, true);
         if (rc) return rc;
         solver_push_history(c);
         mark(c, 9);
@@ -1012,13 +1033,14 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         mark(c, 10);
         if (!rc) rc = fetch_md(c);
         if (rc) return rc;
+        if (c->n_mat) take_force_results(c);
         if (c->timing_on) {
             float a, b2;
             if (hipEventElapsedTime(&a, c->ev[8], c->ev[0]) == hipSuccess &&
                 hipEventElapsedTime(&b2, c->ev[9], c->ev[10]) == hipSuccess)
                 c->timing.t[12] += 1e-3 * (a + b2);
         }
-        const int* st = (const int*)(c->red_host + 292);
+        const int* st = (const int*)(c->red_host + kRedStatus);
         if (st[0]) { capac_set_error("RATTLE_1 not converged!"); return CAPAC_ERR_RATTLE1; }
         if (st[1]) { capac_set_error("RATTLE_2 not converged!"); return CAPAC_ERR_RATTLE2; }
         ++done;
@@ -1033,7 +1055,7 @@ extern "C" int capac_md_observables(capac_ctx* c, double* eksum, double* temp, d
     if (!c) return CAPAC_ERR_ARG;
     int rc = fetch_md(c);
     if (rc) return rc;
-    const double* m = c->red_host + 260;
+    const double* m = c->red_host + kRedMd;
     if (eksum) *eksum = m[0];
     if (temp) *temp = m[1];
     if (ref_temp) *ref_temp = c->cur_ref_temp;

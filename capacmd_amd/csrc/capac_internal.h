@@ -71,7 +71,17 @@ struct SolveScalars {
 
 struct Solver;              // solver.hip
 
-constexpr int kRedWords = 296;
+// The device buffer `red` and its pinned host mirror share one layout, so a
+// single transfer brings back everything the host reads:
+//   [0, 256)    reduction outputs (energies, virials, momentum sums)
+//   [256, 258)  neighbour-row flags (4 ints: overflow, spare)
+//   [260, 292)  MD scalars (md)
+//   [292, 294)  RATTLE failure flags (4 ints, md_status)
+//   [296, 312)  solver scalars (SolveScalars)
+//   [312, 316)  solver's FP64 residual norm
+constexpr int kRedFlags = 256, kRedMd = 260, kRedStatus = 292, kRedSc = 296, kRedRes = 312;
+constexpr int kRedWords = 316;
+static_assert(sizeof(SolveScalars) <= (kRedRes - kRedSc) * sizeof(double), "solver scalars overflow their slot");
 
 struct capac_ctx {
     int device, precision;
@@ -115,8 +125,7 @@ struct capac_ctx {
     int seg_off[8], seg_cap[8];  // segment offsets and capacities, in blocks of 16 doubles
     int n_pending;               // queued reductions, launched together by flush_reductions
     int pend_seg[8], pend_nblk[8], pend_nval[8], pend_red[8];
-    double* red;                // kRedWords: reduction results [0, 256), overflow flags at 256,
-                                // MD scalars at 260 (md), RATTLE flags at 292 (md_status)
+    double* red;                // kRedWords, laid out as described at kRedWords
     double* red_host;           // pinned mirror, same layout
 
     // Sutton-Chen

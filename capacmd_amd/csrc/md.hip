@@ -142,9 +142,14 @@ __device__ __forceinline__ void load_site(Site& s, int i, int N, const double* _
     s.im = inv_mass[i];
 }
 
+// RATTLE: one thread per molecule, and each thread a long serial chain of
+// FP64 operations; small workgroups spread the chains over more SIMDs, so
+// that fewer of them share a SIMD's issue slots
+constexpr int kRattleThreads = 64;
+
 // One thread per molecule; Gauss-Seidel sweeps over its constraints in file
 // order with the reference's arithmetic (no contraction, see the file pragma).
-__global__ void __launch_bounds__(256) k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
+__global__ void __launch_bounds__(kRattleThreads) k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
                           const int* __restrict__ mol_first, const int* __restrict__ mol_last,
                           const int* __restrict__ ci, const int* __restrict__ cj,
                           const double* __restrict__ cr, const double* __restrict__ inv_mass,
@@ -215,13 +220,13 @@ __global__ void __launch_bounds__(256) k_rattle1(int n_mols, int N, BoxConst bc,
     if (!done) status[0] = 1;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRattleThreads)
 k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
           const int* __restrict__ mol_first, const int* __restrict__ mol_last,
           const int* __restrict__ ci, const int* __restrict__ cj, const double* __restrict__ cr,
           const double* __restrict__ inv_mass, const double* __restrict__ r, double* __restrict__ v,
           int* __restrict__ status, double* __restrict__ partial) {
-    __shared__ double s_red[4][9];
+    __shared__ double s_red[kRattleThreads / 64][9];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -292,7 +297,7 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
     if (threadIdx.x < 16) {
         double s = 0.0;
         if (threadIdx.x < 9)
-            for (int q = 0; q < 4; ++q) s += s_red[q][threadIdx.x];
+            for (int q = 0; q < kRattleThreads / 64; ++q) s += s_red[q][threadIdx.x];
         partial[blockIdx.x * 16 + threadIdx.x] = s;
     }
 }
@@ -524,7 +529,7 @@ int md_kick(capac_ctx* c) {
 
 int md_rattle1(capac_ctx* c) {
     if (c->n_cstr == 0) return 0;
-    k_rattle1<<<grid(c->bc.n_mols, 256), 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt,
+    k_rattle1<<<grid(c->bc.n_mols, kRattleThreads), kRattleThreads, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt,
                                                               c->mol_cstr, c->mol_first, c->mol_last,
                                                               c->cstr_i, c->cstr_j,
                                                               c->cstr_r, c->inv_mass, c->r, c->v,
@@ -538,8 +543,8 @@ int md_rattle2(capac_ctx* c) {
     if (c->n_cstr == 0) {
         return fill_zero_d(c, c->red + 64, 16);
     }
-    int nblk = grid(c->bc.n_mols, 256);
-    k_rattle2<<<nblk, 256, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt, c->mol_cstr,
+    int nblk = grid(c->bc.n_mols, kRattleThreads);
+    k_rattle2<<<nblk, kRattleThreads, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt, c->mol_cstr,
                                            c->mol_first, c->mol_last, c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
                                            c->md_status, partial_seg(c, SEG_RATTLE));
     CHECK_HIP(hipGetLastError());

@@ -49,10 +49,12 @@ struct Solver {
     double* part2;          // n_vb: (r0, s)
     double* part4;          // n_vb: (r, r)
     double *d, *rr, *part_res;   // refinement: correction, FP64 residual, its partials
+    // inside the context's readback buffer, so that the synchronisation at
+    // the end of a batch also brings back the neighbour-row flags
     double* res;            // device: [0] (rr, rr)
     SolveScalars* sc;       // device
-    SolveScalars* sc_host;  // pinned
-    double* res_host;       // pinned
+    SolveScalars* sc_host;  // pinned mirror
+    double* res_host;       // pinned mirror
 };
 
 namespace {
@@ -442,11 +444,10 @@ int solver_create(capac_ctx* c) {
     CHECK_HIP(hipMalloc(&S->part_res, ng * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part2, nvb * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part4, nvb * sizeof(double)));
-    CHECK_HIP(hipMalloc(&S->res, 4 * sizeof(double)));
-    CHECK_HIP(hipMalloc(&S->sc, sizeof(SolveScalars)));
-    CHECK_HIP(hipMemset(S->sc, 0, sizeof(SolveScalars)));
-    CHECK_HIP(hipHostMalloc(&S->sc_host, sizeof(SolveScalars), hipHostMallocDefault));
-    CHECK_HIP(hipHostMalloc(&S->res_host, 4 * sizeof(double), hipHostMallocDefault));
+    S->sc = (SolveScalars*)(c->red + kRedSc);
+    S->sc_host = (SolveScalars*)(c->red_host + kRedSc);
+    S->res = c->red + kRedRes;
+    S->res_host = c->red_host + kRedRes;
     return 0;
 }
 
@@ -454,12 +455,9 @@ void solver_destroy(capac_ctx* c) {
     Solver* S = c->solver;
     if (!S) return;
     double* vecs[] = {S->r0, S->r, S->p, S->v, S->s, S->t, S->y, S->z, S->K, S->Ax, S->d, S->rr,
-                      S->part1, S->part3, S->part_res, S->part2, S->part4, S->res};
+                      S->part1, S->part3, S->part_res, S->part2, S->part4};
     for (size_t k = 0; k < sizeof vecs / sizeof vecs[0]; ++k)
         if (vecs[k]) (void)hipFree(vecs[k]);
-    if (S->sc) (void)hipFree(S->sc);
-    if (S->sc_host) (void)hipHostFree(S->sc_host);
-    if (S->res_host) (void)hipHostFree(S->res_host);
     free(S);
     c->solver = NULL;
 }
@@ -533,10 +531,11 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
             int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
             if (rc) return rc;
             k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
-            CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
         }
         CHECK_HIP(hipGetLastError());
-        CHECK_HIP(hipMemcpyAsync(S->sc_host, S->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st));
+        // the scalars, the FP64 residual and the neighbour-row flags in one transfer
+        CHECK_HIP(hipMemcpyAsync(c->red_host + kRedFlags, c->red + kRedFlags,
+                                 (kRedWords - kRedFlags) * sizeof(double), hipMemcpyDeviceToHost, st));
         CHECK_HIP(hipStreamSynchronize(st));
         if (S->sc_host->done) break;
         if (launched > 1010) {
