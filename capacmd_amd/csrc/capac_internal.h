@@ -189,6 +189,13 @@ __device__ __forceinline__ double dist2_exact(double x, double y, double z) {
 }
 
 // wavefront sum of a double, fixed butterfly order (deterministic)
+// Kernels on the main stream that run while the pair kernel occupies the
+// same compute units (the CPIM right-hand side, the assembly and the solve,
+// a chain of short latency-bound launches) raise their waves' issue
+// priority, so that a SIMD shared with pair-kernel waves serves theirs first.
+// This is the waves' priority inside a kernel; the streams keep the default.
+__device__ __forceinline__ void main_stream_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // Wavefront sum through DPP lane moves (VALU) instead of LDS-crossbar
 // shuffles: quad sums, 8- and 16-lane sums by mirrors, then the row
 // broadcasts gather the four rows into lane 63, which every lane reads.

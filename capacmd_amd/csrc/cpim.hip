@@ -24,6 +24,7 @@ __global__ void __launch_bounds__(256)
 k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
       const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
       const double* __restrict__ np_charge, double* __restrict__ b) {
+    main_stream_priority();
     const int N = bc.N, Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int i = blockIdx.x * 4 + wave;
@@ -122,6 +123,7 @@ __device__ __forceinline__ void relay_block64(const double* __restrict__ r, cons
 // one thread per ordered pair (i, j): grid (ceil(Nm/256), Nm)
 __global__ void __launch_bounds__(256)
 k_assemble64(const double* __restrict__ r, BoxConst bc, int ld, double* __restrict__ relay) {
+    main_stream_priority();
     const int Nm = bc.Nm;
     int i = blockIdx.y;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,6 +170,7 @@ __device__ __forceinline__ TCoef32 t_coef32(float r2, float iRpp, float iRpq, fl
 // One thread per ordered pair: grid (ceil(Nm/256), Nm).
 __global__ void __launch_bounds__(256)
 k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restrict__ relay) {
+    main_stream_priority();
     const int N = bc.N, Nm = bc.Nm;
     int i = blockIdx.y;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,6 +201,7 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
           const int* __restrict__ np_first, const int* __restrict__ np_last,
           const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ out,
           double* __restrict__ part, const SolveScalars* __restrict__ guard) {
+    main_stream_priority();
     __shared__ double s_w[4][4];
     if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
         return;
@@ -255,6 +259,7 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
 }
 
 __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
+    main_stream_priority();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_mat) return;
     int Nm = bc.Nm;

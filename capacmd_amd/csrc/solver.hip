@@ -155,6 +155,7 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
        const int* __restrict__ np_last, const double* __restrict__ x, double* __restrict__ y, int mode,
        const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
        const SolveScalars* __restrict__ sc) {
+    main_stream_priority();
     constexpr int GPB = 256 / TPG;                 // groups per block
     constexpr int WPG = TPG / 64;                  // waves per group
     __shared__ double s_w[4][4];
@@ -237,6 +238,7 @@ k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __res
             const double* __restrict__ diag, double* __restrict__ r0, double* __restrict__ r,
             double* __restrict__ p, double* __restrict__ v, double* __restrict__ K,
             double* __restrict__ y, SolveScalars* sc, double crit_factor) {
+    main_stream_priority();
     __shared__ double s_w[16];
     double bb = 0.0, rr = 0.0;
     for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
@@ -282,6 +284,7 @@ k_vec_s(int n_mat, int n, int n_groups, const double* __restrict__ part1, int n_
         const double* __restrict__ part4, const double* __restrict__ r, const double* __restrict__ v,
         const double* __restrict__ K, const double* __restrict__ r0, double* __restrict__ s,
         double* __restrict__ z, double* __restrict__ part2, SolveScalars* sc) {
+    main_stream_priority();
     __shared__ double s_w[16 * 2];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
@@ -341,6 +344,7 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
          const double* __restrict__ z, const double* __restrict__ s, const double* __restrict__ t,
          double* __restrict__ r, double* __restrict__ p, const double* __restrict__ v,
          const double* __restrict__ K, double* __restrict__ part4, SolveScalars* sc) {
+    main_stream_priority();
     __shared__ double s_w[16 * 4];
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
@@ -392,6 +396,7 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
 __global__ void __launch_bounds__(256)
 k_res_finish(const double* __restrict__ part, int n, double* __restrict__ res,
              const SolveScalars* __restrict__ guard) {
+    main_stream_priority();
     __shared__ double s_w[16];
     if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
         return;
