@@ -398,8 +398,8 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         const int metal_blocks = (Nm + 3) / 4 + 1;
         c->seg_cap[SEG_PAIR] = pair_blocks(N);
         c->seg_cap[SEG_SC] = metal_blocks;
-        c->seg_cap[SEG_MW] = metal_blocks;
-        c->seg_cap[SEG_MM] = metal_blocks;
+        c->seg_cap[SEG_MW] = Nm + 1;        // one workgroup per metal atom
+        c->seg_cap[SEG_MM] = Nm + 1;
         c->seg_cap[SEG_EXT] = (N + 255) / 256 + 1;
         c->seg_cap[SEG_RATTLE] = (p->n_mols + 63) / 64 + 1;      // workgroups of 64 or more
         c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;

@@ -348,15 +348,34 @@ __device__ __forceinline__ void mw_pair(const BoxConst& bc, double dx, double dy
     *e11 += -K_FQQ * qa * pr * const_a0;
 }
 
-// metal side: one wavefront per metal atom over its charged non-metal neighbours
+// The metal atoms' kernels give each metal atom a whole workgroup, its four
+// wavefronts splitting the atom's partners: with one wavefront per atom the
+// few hundred metal atoms leave most SIMDs with a single wavefront, whose
+// transcendental latency nothing hides.  The force on the atom is the
+// fixed-order sum of the four wavefront sums.
+__device__ __forceinline__ void block_force_add(double fx, double fy, double fz, double (*s_f)[3], int i, int N,
+                                                double* __restrict__ f) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) { s_f[wave][0] = fx; s_f[wave][1] = fy; s_f[wave][2] = fz; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += s_f[w][threadIdx.x];
+        f[(size_t)threadIdx.x * N + i] += t;
+    }
+}
+
+// metal side: one workgroup per metal atom over its charged non-metal neighbours
 __global__ void __launch_bounds__(256)
 k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
                 const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
                 const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
     __shared__ double s_red[4][11];
+    __shared__ double s_f[4][3];
     const int N = bc.N, Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int i = blockIdx.x * 4 + wave;
+    const int i = blockIdx.x;
     double e10 = 0.0, e11 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < Nm) {
         double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
@@ -365,7 +384,7 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
         int nf = n_front[i];
         const int* row = nbr + (size_t)i * stride;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int k = lane; k < nf; k += 64) {
+        for (int k = threadIdx.x; k < nf; k += blockDim.x) {
             int a = row[k];
             double qa = charge[a];
             if (qa == 0.0) continue;
@@ -382,8 +401,7 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
 #pragma unroll
                 for (int b2 = 0; b2 < 3; ++b2) v[a2 * 3 + b2] -= rv[a2] * fp[b2];
         }
-        fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
-        if (lane == 0) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
+        block_force_add(fx, fy, fz, s_f, i, N, f);
     }
     double vals[11] = {e10, e11, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
 #pragma unroll
@@ -438,9 +456,10 @@ __global__ void __launch_bounds__(256)
 k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
           double* __restrict__ f, double* __restrict__ partial) {
     __shared__ double s_red[4][13];
+    __shared__ double s_f[4][3];
     const int N = bc.N, Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int i = blockIdx.x * 4 + wave;
+    const int i = blockIdx.x;
     double e9 = 0.0, e12 = 0.0, e13 = 0.0, e14 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < Nm) {
         double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
@@ -450,7 +469,7 @@ k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ 
         const double iRpq3 = iRpq * iRpq * iRpq, iRpp3 = iRpp * iRpp * iRpp;
         const double iRpp5 = iRpp3 * iRpp * iRpp;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int j = lane; j < Nm; j += 64) {
+        for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
             if (j == i) continue;
             double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
             pbc12(bc, dx, dy, dz);
@@ -508,8 +527,7 @@ k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ 
 #pragma unroll
                 for (int b = 0; b < 3; ++b) v[a * 3 + b] -= 0.5 * rv[a] * g[b];
         }
-        fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             // self terms (inv_capac, inv_polar in atomic units, FQQ in MD units)
             e12 += 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
             e14 += 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
@@ -517,8 +535,8 @@ k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ 
             fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
             e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
             e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
-            f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz;
         }
+        block_force_add(fx, fy, fz, s_f, i, N, f);
     }
     double vals[13] = {e9, e12, e13, e14, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
 #pragma unroll
@@ -590,7 +608,7 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 int cpim_forces_x(capac_ctx* c, const double* x) {
     int Nm = c->bc.Nm, N = c->bc.N;
     if (Nm == 0) return 0;
-    int nblk = (Nm + 3) / 4;
+    int nblk = Nm;                      // one workgroup per metal atom
     k_cpim_mw_metal<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
                                                  c->nb_stride, x, c->f, partial_seg(c, SEG_MW));
     CHECK_HIP(hipGetLastError());
@@ -610,7 +628,7 @@ int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
 int cpim_mm_only(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
-    k_cpim_mm<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM));
+    k_cpim_mm<<<Nm, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM));
     CHECK_HIP(hipGetLastError());
     return 0;
 }
