@@ -15,25 +15,31 @@
 // The Lagrange rows and columns of the charge-neutrality constraint are
 // applied analytically by the solver.
 #include "capac_internal.h"
+#include "erfc_fast.h"
 
 namespace {
 
 // ------------------------------------------------------------------ RHS
-// One wavefront per metal atom over the non-metal front of its row.
+// One workgroup per metal atom over the non-metal front of its row (four
+// wavefronts split the row, as in the metal-side force kernels).
 __global__ void __launch_bounds__(256)
 k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
       const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
       const double* __restrict__ np_charge, double* __restrict__ b) {
     main_stream_priority();
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ double s_w[4][4];
     const int N = bc.N, Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int i = blockIdx.x * 4 + wave;
+    const int i = blockIdx.x;
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
     if (i >= Nm) return;
     double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
     int nf = n_front[i];
     const int* row = nbr + (size_t)i * stride;
     double ex = 0.0, ey = 0.0, ez = 0.0, pm = 0.0;
-    for (int k = lane; k < nf; k += 64) {
+    for (int k = threadIdx.x; k < nf; k += blockDim.x) {
         int a = row[k];
         double q = charge[a];
         if (q == 0.0) continue;
@@ -45,8 +51,8 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
         double rij = sqrt(r2);
         double e_field, v_field;
         if (bc.use_coulomb == 1) {
-            double erfc_a = erfc(bc.alpha * rij);
             double e2 = exp(-bc.alpha * bc.alpha * r2);
+            double erfc_a = erfc_from_exp(bc.alpha * rij, e2, s_erfcx);
             e_field = K_FQQ * q * (erfc_a / r2 + bc.a2_sqrtPI * e2 / rij - bc.wolfConst) / rij;
             v_field = K_FQQ * q * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
         } else {
@@ -58,7 +64,13 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
         pm += v_field * K_CONV_V;
     }
     ex = wave_sum(ex); ey = wave_sum(ey); ez = wave_sum(ez); pm = wave_sum(pm);
-    if (lane == 0) {
+    if (lane == 0) { s_w[wave][0] = ex; s_w[wave][1] = ey; s_w[wave][2] = ez; s_w[wave][3] = pm; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ex = s_w[0][0] + s_w[1][0] + s_w[2][0] + s_w[3][0];
+        ey = s_w[0][1] + s_w[1][1] + s_w[2][1] + s_w[3][1];
+        ez = s_w[0][2] + s_w[1][2] + s_w[2][2] + s_w[3][2];
+        pm = s_w[0][3] + s_w[1][3] + s_w[2][3] + s_w[3][3];
         b[3 * i + 0] = ex + bc.efield[0] * K_CONV_E;
         b[3 * i + 1] = ey + bc.efield[1] * K_CONV_E;
         b[3 * i + 2] = ez + bc.efield[2] * K_CONV_E;
@@ -563,7 +575,7 @@ int cpim_rhs(capac_ctx* c) {
         capac_set_error("more nanoparticles (%d) than metal atoms (%d)", c->bc.n_np, Nm);
         return CAPAC_ERR_ARG;
     }
-    k_rhs<<<(Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
+    k_rhs<<<Nm, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
                                                c->nb_stride, c->np_charge, c->b);
     CHECK_HIP(hipGetLastError());
     return 0;

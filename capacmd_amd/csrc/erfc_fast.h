@@ -1,0 +1,26 @@
+// erfc(x) from exp(-x^2), which the damped-shifted-force terms need anyway:
+// erfc = exp(-x^2) erfcx(x), erfcx from the piecewise polynomials of
+// erfcx_table.h on [0, 6) (within 1 ulp of 60-digit values), the library
+// erfc elsewhere.  With exp(-alpha^2 r^2) in place of exp(-(alpha r)^2) the
+// exponent differs by rounding only (a relative 1e-15 in erfc).  coef is the
+// table staged in LDS by load_erfcx_table.
+#pragma once
+#include "erfcx_table.h"
+
+__device__ __forceinline__ double erfc_from_exp(double x, double ex, const double* __restrict__ coef) {
+    if (!(x >= 0.0 && x < kErfcxHi)) return erfc(x);
+    const int k = (int)(x * 4.0);
+    const double t = (x - (k + 0.5) * 0.25) * 8.0;
+    const double* c = coef + k * kErfcxTerms;
+    double p = c[kErfcxTerms - 1];
+#pragma unroll
+    for (int i = kErfcxTerms - 2; i >= 0; --i) p = fma(p, t, c[i]);
+    return ex * p;
+}
+
+// the whole workgroup copies the table into s (kErfcxPieces * kErfcxTerms
+// doubles of LDS); the caller synchronises before use
+__device__ __forceinline__ void load_erfcx_table(double* s) {
+    for (int k = threadIdx.x; k < kErfcxPieces * kErfcxTerms; k += blockDim.x)
+        s[k] = kErfcxCoef[k / kErfcxTerms][k % kErfcxTerms];
+}

@@ -126,7 +126,7 @@ def main():
         "constexpr int kErfcxPieces = %d;" % N_PIECES,
         "constexpr int kErfcxTerms = %d;" % (DEG + 1),
         "constexpr double kErfcxHi = %d.0;" % (N_PIECES // 4),
-        "__constant__ const double kErfcxCoef[kErfcxPieces][kErfcxTerms] = {",
+        "static __constant__ const double kErfcxCoef[kErfcxPieces][kErfcxTerms] = {",
     ]
     for k, (mid, coef) in enumerate(pieces):
         lines.append("    {" + ", ".join("%.17g" % c for c in coef) + "},")

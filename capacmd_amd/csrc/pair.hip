@@ -9,7 +9,7 @@
 // both ends, so pair energies and the virial carry a factor 1/2.
 // FP64-VALU bound (erfc + exp per charged pair).
 #include "capac_internal.h"
-#include "erfcx_table.h"
+#include "erfc_fast.h"
 
 namespace {
 
@@ -22,22 +22,6 @@ struct PairOut {
 // The reference's expressions with each division by r, r^2, r^6 or the erf
 // width replaced by a product with the reciprocal, computed once per pair
 // (one FP64 division instead of up to five; results move by rounding only).
-// erfc(x) from exp(-x^2), already needed by the damped-shifted-force term:
-// erfc = exp(-x^2) erfcx(x), erfcx from the piecewise polynomials of
-// erfcx_table.h on [0, 6) (within 1 ulp of 60-digit values), the library
-// erfc elsewhere.  With exp(-alpha^2 r^2) in place of exp(-(alpha r)^2) the
-// exponent differs by rounding only (a relative 1e-15 in erfc).
-__device__ __forceinline__ double erfc_from_exp(double x, double ex, const double* __restrict__ coef) {
-    if (!(x >= 0.0 && x < kErfcxHi)) return erfc(x);
-    const int k = (int)(x * 4.0);
-    const double t = (x - (k + 0.5) * 0.25) * 8.0;
-    const double* c = coef + k * kErfcxTerms;
-    double p = c[kErfcxTerms - 1];
-#pragma unroll
-    for (int i = kErfcxTerms - 2; i >= 0; --i) p = fma(p, t, c[i]);
-    return ex * p;
-}
-
 __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
                                           double dx, double dy, double dz, double r2, PairOut& o,
                                           const double* __restrict__ erfcx_coef) {
@@ -104,8 +88,7 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     const int N = bc.N, nt = bc.n_types;
     for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) s_tab[k] = table[k];
-    for (int k = threadIdx.x; k < kErfcxPieces * kErfcxTerms; k += blockDim.x)
-        s_erfcx[k] = kErfcxCoef[k / kErfcxTerms][k % kErfcxTerms];
+    load_erfcx_table(s_erfcx);
     __syncthreads();
     const int lane = threadIdx.x & (CAPAC_WAVE - 1);
     const int wave = threadIdx.x / CAPAC_WAVE;
