@@ -360,86 +360,142 @@ __device__ __forceinline__ void mw_pair(const BoxConst& bc, double dx, double dy
     *e11 += -K_FQQ * qa * pr * const_a0;
 }
 
-// The metal atoms' kernels give each metal atom a whole workgroup, its four
-// wavefronts splitting the atom's partners: with one wavefront per atom the
-// few hundred metal atoms leave most SIMDs with a single wavefront, whose
-// transcendental latency nothing hides.  The force on the atom is the
-// fixed-order sum of the four wavefront sums.
-__device__ __forceinline__ void block_force_add(double fx, double fy, double fz, double (*s_f)[3], int i, int N,
-                                                double* __restrict__ f) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
-    if (lane == 0) { s_f[wave][0] = fx; s_f[wave][1] = fy; s_f[wave][2] = fz; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        double t = 0.0;
-        for (int w = 0; w < 4; ++w) t += s_f[w][threadIdx.x];
-        f[(size_t)threadIdx.x * N + i] += t;
-    }
-}
-
-// metal side: one workgroup per metal atom over its charged non-metal neighbours
-__global__ void __launch_bounds__(256)
-k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
-                const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
-                const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
-    __shared__ double s_red[4][11];
-    __shared__ double s_f[4][3];
+// ---------------------------------------------------------- force terms
+// metal-water terms of metal atom i over its charged non-metal neighbours,
+// the row split over 256 threads (t = 0..255); vals = e10, e11, virial
+__device__ __forceinline__ void mw_metal_terms(const double* __restrict__ r, const double* __restrict__ charge,
+                                               const BoxConst& bc, const int* __restrict__ n_front,
+                                               const int* __restrict__ nbr, int stride,
+                                               const double* __restrict__ x, int i, int t,
+                                               double& fx, double& fy, double& fz, double vals[13]) {
     const int N = bc.N, Nm = bc.Nm;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x;
+    const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+    const double qm = x[3 * Nm + i];
+    const double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    const int nf = n_front[i];
+    const int* row = nbr + (size_t)i * stride;
     double e10 = 0.0, e11 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < Nm) {
-        double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
-        double qm = x[3 * Nm + i];
-        double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-        int nf = n_front[i];
-        const int* row = nbr + (size_t)i * stride;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int k = threadIdx.x; k < nf; k += blockDim.x) {
-            int a = row[k];
-            double qa = charge[a];
-            if (qa == 0.0) continue;
-            double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
-            pbc12(bc, dx, dy, dz);
-            double r2 = dist2_exact(dx, dy, dz);
-            if (!(r2 < bc.rc2)) continue;         // rows reach rc + skin
-            double fp[3];
-            mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11);
-            fx += fp[0]; fy += fp[1]; fz += fp[2];
-            double rv[3] = {dx, dy, dz};
+    for (int k = t; k < nf; k += 256) {
+        int a = row[k];
+        double qa = charge[a];
+        if (qa == 0.0) continue;
+        double dx = xi - r[a], dy = yi - r[N + a], dz = zi - r[2 * N + a];
+        pbc12(bc, dx, dy, dz);
+        double r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < bc.rc2)) continue;         // rows reach rc + skin
+        double fp[3];
+        mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11);
+        fx += fp[0]; fy += fp[1]; fz += fp[2];
+        double rv[3] = {dx, dy, dz};
 #pragma unroll
-            for (int a2 = 0; a2 < 3; ++a2)
+        for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
-                for (int b2 = 0; b2 < 3; ++b2) v[a2 * 3 + b2] -= rv[a2] * fp[b2];
-        }
-        block_force_add(fx, fy, fz, s_f, i, N, f);
+            for (int b2 = 0; b2 < 3; ++b2) v[a2 * 3 + b2] -= rv[a2] * fp[b2];
     }
-    double vals[11] = {e10, e11, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+    vals[0] = e10; vals[1] = e11;
 #pragma unroll
-    for (int c = 0; c < 11; ++c) {
-        double s = wave_sum(vals[c]);
-        if (lane == 0) s_red[wave][c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 16) {
-        double s = 0.0;
-        if (threadIdx.x < 11)
-            for (int w = 0; w < 4; ++w) s += s_red[w][threadIdx.x];
-        partial[blockIdx.x * 16 + threadIdx.x] = s;
-    }
+    for (int c = 0; c < 9; ++c) vals[2 + c] = v[c];
 }
 
-// atom side: one wavefront per non-metal atom over the metal tail of its row;
-// the reaction force of the same pair term (the reference's f_atom -= f)
-__global__ void __launch_bounds__(256)
-k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
-               const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
-               const double* __restrict__ x, double* __restrict__ f) {
+// Metal-metal terms, all pairs, force on i only (cpff.c:762-927), plus the
+// self terms (cpff.c:727-742) and the external field on the induced charge
+// and dipole (cpff.c:930-948); j split over 256 threads (t = 0..255).  Each
+// erf comes from the Gaussian the same term needs anyway (erf_from_exp, with
+// the erfcx table staged in LDS), not from a library erf of its own.
+// vals = e9, e12, e13, e14, virial
+__device__ __forceinline__ void mm_terms(const double* __restrict__ r, const BoxConst& bc,
+                                         const double* __restrict__ x, const double* __restrict__ erfcx,
+                                         int i, int t,
+                                         double& fx, double& fy, double& fz, double vals[13]) {
     const int N = bc.N, Nm = bc.Nm;
-    const int lane = threadIdx.x & 63;
-    int a = Nm + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= N) return;
+    double e9 = 0.0, e12 = 0.0, e13 = 0.0, e14 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+    const double qi = x[3 * Nm + i];
+    const double pi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    const double iRqq = bc.inv_R_qq, iRpq = bc.inv_R_pq, iRpp = bc.inv_R_pp;
+    const double iRpq3 = iRpq * iRpq * iRpq, iRpp3 = iRpp * iRpp * iRpp;
+    const double iRpp5 = iRpp3 * iRpp * iRpp;
+    for (int j = t; j < Nm; j += 256) {
+        if (j == i) continue;
+        double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+        pbc12(bc, dx, dy, dz);
+        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+        double r2 = dist2_exact(dx, dy, dz);
+        double rij = sqrt(r2);
+        double inv_r = 1.0 / rij, inv_r2 = inv_r * inv_r;
+        double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
+        double qj = x[3 * Nm + j];
+        double pj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+        double rv[3] = {dx, dy, dz};
+        double g[3] = {0.0, 0.0, 0.0};
+        // charge-charge, T0
+        double e_qq = exp(-(r2 * iRqq * iRqq));
+        double erf_qq = erf_from_exp(rij * iRqq, e_qq, erfcx);
+        double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * iRqq * rij * e_qq);
+        double sqq = A0 * K_FQQ_AU2 * qi * qj;
+        g[0] += sqq * dx; g[1] += sqq * dy; g[2] += sqq * dz;
+        e12 += 0.5 * K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
+        // charge-dipole, T1 both directions
+        double e_pq = exp(-(r2 * iRpq * iRpq));
+        double br_pq = erf_from_exp(rij * iRpq, e_pq, erfcx) - 2.0 * K_INV_SQRT_PI * iRpq * rij * e_pq;
+        double Apq0 = inv_r3 * br_pq;
+        double A1 = inv_r5 * br_pq;
+        double B1 = 4.0 * K_INV_SQRT_PI * iRpq3 * inv_r2 * e_pq;
+        // w = qj p_i - qi p_j:  f_b += sum_a (A1 (3 r_a r_b - d r^2) - B1 r_a r_b) au2 w_a
+        double w[3] = {qj * pi[0] - qi * pj[0], qj * pi[1] - qi * pj[1], qj * pi[2] - qi * pj[2]};
+        double wr = w[0] * dx + w[1] * dy + w[2] * dz;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            g[b] += (A1 * (3.0 * wr * rv[b] - w[b] * r2) - B1 * wr * rv[b]) * K_FQQ_AU2;
+        double pjr = pj[0] * dx + pj[1] * dy + pj[2] * dz;
+        double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
+        e13 += 0.5 * K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
+        // dipole-dipole, T2
+        double e_pp = exp(-(r2 * iRpp * iRpp));
+        double erf_pp = erf_from_exp(rij * iRpp, e_pp, erfcx);
+        double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * iRpp * rij * e_pp);
+        double Bpp1 = 4.0 * K_INV_SQRT_PI * iRpp3 * inv_r2 * e_pp;
+        double A2 = inv_r7 * (3.0 * erf_pp - 6.0 * K_INV_SQRT_PI * iRpp * rij * e_pp -
+                              4.0 * r2 * rij * K_INV_SQRT_PI * iRpp3 * e_pp);
+        double B2 = 8.0 * K_INV_SQRT_PI * iRpp5 * inv_r2 * e_pp;
+        double pipj = pi[0] * pj[0] + pi[1] * pj[1] + pi[2] * pj[2];
+        // sum_ab p_ia p_jb [5 r_a r_b r_c - r^2 (r_a d_bc + r_b d_ca + r_c d_ab)]
+        //   = 5 (pi.r)(pj.r) r_c - r^2 ((pi.r) pj_c + (pj.r) pi_c + (pi.pj) r_c)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double t5 = 5.0 * pir * pjr * rv[c] - r2 * (pir * pj[c] + pjr * pi[c] + pipj * rv[c]);
+            g[c] -= (A2 * t5 - B2 * pir * pjr * rv[c]) * K_FQQ_AU2;
+        }
+        e14 -= 0.5 * K_FQQ_AU1 * ((3.0 * pir * pjr - pipj * r2) * App1 - pir * pjr * Bpp1);
+        fx -= g[0]; fy -= g[1]; fz -= g[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) v[a * 3 + b] -= 0.5 * rv[a] * g[b];
+    }
+    if (t == 0) {
+        // self terms (inv_capac, inv_polar in atomic units, FQQ in MD units)
+        e12 += 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
+        e14 += 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
+        // external field on the induced charge (force) and charge + dipole (energy)
+        fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
+        e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+        e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
+    }
+    vals[0] = e9; vals[1] = e12; vals[2] = e13; vals[3] = e14;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) vals[4 + c] = v[c];
+}
+
+// atom side of the metal-water terms: non-metal atom a over the metal tail of
+// its row, one wavefront; the reaction force of the same pair term (the
+// reference's f_atom -= f)
+__device__ __forceinline__ void mw_atom_terms(const double* __restrict__ r, const double* __restrict__ charge,
+                                              const BoxConst& bc, const int* __restrict__ n_back,
+                                              const int* __restrict__ nbr, int stride,
+                                              const double* __restrict__ x, int a, int lane,
+                                              double* __restrict__ f) {
+    const int N = bc.N, Nm = bc.Nm;
     double qa = charge[a];
     int nb = n_back[a];
     if (qa == 0.0 || nb == 0) return;
@@ -461,108 +517,87 @@ k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, 
     if (lane == 0) { f[a] -= fx; f[N + a] -= fy; f[2 * N + a] -= fz; }
 }
 
-// Metal-metal terms, all pairs, force on i only (cpff.c:762-927), plus the
-// self terms (cpff.c:727-742) and the external field on the induced charge
-// and dipole (cpff.c:930-948).  One wavefront per metal atom i, lanes over j.
-__global__ void __launch_bounds__(256)
-k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
-          double* __restrict__ f, double* __restrict__ partial) {
-    __shared__ double s_red[4][13];
-    __shared__ double s_f[4][3];
-    const int N = bc.N, Nm = bc.Nm;
+// The metal atoms' kernels give each metal atom a whole workgroup, its four
+// wavefronts splitting the atom's partners: with one wavefront per atom the
+// few hundred metal atoms leave most SIMDs with a single wavefront, whose
+// transcendental latency nothing hides.  The force on the atom is the
+// fixed-order sum of the four wavefront sums.
+__device__ __forceinline__ void block_force_add(double fx, double fy, double fz, double (*s_f)[3], int i, int N,
+                                                double* __restrict__ f) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x;
-    double e9 = 0.0, e12 = 0.0, e13 = 0.0, e14 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < Nm) {
-        double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
-        double qi = x[3 * Nm + i];
-        double pi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-        const double iRqq = bc.inv_R_qq, iRpq = bc.inv_R_pq, iRpp = bc.inv_R_pp;
-        const double iRpq3 = iRpq * iRpq * iRpq, iRpp3 = iRpp * iRpp * iRpp;
-        const double iRpp5 = iRpp3 * iRpp * iRpp;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
-            if (j == i) continue;
-            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
-            pbc12(bc, dx, dy, dz);
-            dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
-            double r2 = dist2_exact(dx, dy, dz);
-            double rij = sqrt(r2);
-            double inv_r = 1.0 / rij, inv_r2 = inv_r * inv_r;
-            double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
-            double qj = x[3 * Nm + j];
-            double pj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            double rv[3] = {dx, dy, dz};
-            double g[3] = {0.0, 0.0, 0.0};
-            // charge-charge, T0
-            double e_qq = exp(-(r2 * iRqq * iRqq));
-            double erf_qq = erf(rij * iRqq);
-            double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * iRqq * rij * e_qq);
-            double sqq = A0 * K_FQQ_AU2 * qi * qj;
-            g[0] += sqq * dx; g[1] += sqq * dy; g[2] += sqq * dz;
-            e12 += 0.5 * K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
-            // charge-dipole, T1 both directions
-            double e_pq = exp(-(r2 * iRpq * iRpq));
-            double br_pq = erf(rij * iRpq) - 2.0 * K_INV_SQRT_PI * iRpq * rij * e_pq;
-            double Apq0 = inv_r3 * br_pq;
-            double A1 = inv_r5 * br_pq;
-            double B1 = 4.0 * K_INV_SQRT_PI * iRpq3 * inv_r2 * e_pq;
-            // w = qj p_i - qi p_j:  f_b += sum_a (A1 (3 r_a r_b - d r^2) - B1 r_a r_b) au2 w_a
-            double w[3] = {qj * pi[0] - qi * pj[0], qj * pi[1] - qi * pj[1], qj * pi[2] - qi * pj[2]};
-            double wr = w[0] * dx + w[1] * dy + w[2] * dz;
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-                g[b] += (A1 * (3.0 * wr * rv[b] - w[b] * r2) - B1 * wr * rv[b]) * K_FQQ_AU2;
-            double pjr = pj[0] * dx + pj[1] * dy + pj[2] * dz;
-            double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
-            e13 += 0.5 * K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
-            // dipole-dipole, T2
-            double e_pp = exp(-(r2 * iRpp * iRpp));
-            double erf_pp = erf(rij * iRpp);
-            double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * iRpp * rij * e_pp);
-            double Bpp1 = 4.0 * K_INV_SQRT_PI * iRpp3 * inv_r2 * e_pp;
-            double A2 = inv_r7 * (3.0 * erf_pp - 6.0 * K_INV_SQRT_PI * iRpp * rij * e_pp -
-                                  4.0 * r2 * rij * K_INV_SQRT_PI * iRpp3 * e_pp);
-            double B2 = 8.0 * K_INV_SQRT_PI * iRpp5 * inv_r2 * e_pp;
-            double pipj = pi[0] * pj[0] + pi[1] * pj[1] + pi[2] * pj[2];
-            // sum_ab p_ia p_jb [5 r_a r_b r_c - r^2 (r_a d_bc + r_b d_ca + r_c d_ab)]
-            //   = 5 (pi.r)(pj.r) r_c - r^2 ((pi.r) pj_c + (pj.r) pi_c + (pi.pj) r_c)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double t5 = 5.0 * pir * pjr * rv[c] - r2 * (pir * pj[c] + pjr * pi[c] + pipj * rv[c]);
-                g[c] -= (A2 * t5 - B2 * pir * pjr * rv[c]) * K_FQQ_AU2;
-            }
-            e14 -= 0.5 * K_FQQ_AU1 * ((3.0 * pir * pjr - pipj * r2) * App1 - pir * pjr * Bpp1);
-            fx -= g[0]; fy -= g[1]; fz -= g[2];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) v[a * 3 + b] -= 0.5 * rv[a] * g[b];
-        }
-        if (threadIdx.x == 0) {
-            // self terms (inv_capac, inv_polar in atomic units, FQQ in MD units)
-            e12 += 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
-            e14 += 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
-            // external field on the induced charge (force) and charge + dipole (energy)
-            fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
-            e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
-            e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
-        }
-        block_force_add(fx, fy, fz, s_f, i, N, f);
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) { s_f[wave][0] = fx; s_f[wave][1] = fy; s_f[wave][2] = fz; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += s_f[w][threadIdx.x];
+        f[(size_t)threadIdx.x * N + i] += t;
     }
-    double vals[13] = {e9, e12, e13, e14, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+}
+
+// per-block partials: the fixed-order sum over the four wavefronts of each
+// of the nval values
+template <int NVAL>
+__device__ __forceinline__ void block_partials(const double (&vals)[13], double (*s_red)[13],
+                                               double* __restrict__ partial) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = 0; c < 13; ++c) {
+    for (int c = 0; c < NVAL; ++c) {
         double s = wave_sum(vals[c]);
         if (lane == 0) s_red[wave][c] = s;
     }
     __syncthreads();
     if (threadIdx.x < 16) {
         double s = 0.0;
-        if (threadIdx.x < 13)
+        if (threadIdx.x < NVAL)
             for (int w = 0; w < 4; ++w) s += s_red[w][threadIdx.x];
         partial[blockIdx.x * 16 + threadIdx.x] = s;
     }
+}
+
+// metal side of the metal-water terms: one workgroup per metal atom;
+// partial = e10, e11, virial
+__global__ void __launch_bounds__(256)
+k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+                const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+                const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
+    __shared__ double s_red[4][13];
+    __shared__ double s_f[4][3];
+    const int i = blockIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0, vals[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) vals[c] = 0.0;
+    mw_metal_terms(r, charge, bc, n_front, nbr, stride, x, i, threadIdx.x, fx, fy, fz, vals);
+    block_force_add(fx, fy, fz, s_f, i, bc.N, f);
+    block_partials<11>(vals, s_red, partial);
+}
+
+// atom side of the metal-water terms: one wavefront per non-metal atom
+__global__ void __launch_bounds__(256)
+k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+               const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
+               const double* __restrict__ x, double* __restrict__ f) {
+    const int a = bc.Nm + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a < bc.N) mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, a, threadIdx.x & 63, f);
+}
+
+// metal-metal terms: one workgroup per metal atom;
+// partial = e9, e12, e13, e14, virial
+__global__ void __launch_bounds__(256)
+k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
+          double* __restrict__ f, double* __restrict__ partial) {
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ double s_red[4][13];
+    __shared__ double s_f[4][3];
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    const int i = blockIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0, vals[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) vals[c] = 0.0;
+    mm_terms(r, bc, x, s_erfcx, i, threadIdx.x, fx, fy, fz, vals);
+    block_force_add(fx, fy, fz, s_f, i, bc.N, f);
+    block_partials<13>(vals, s_red, partial);
 }
 
 }  // namespace
@@ -636,7 +671,7 @@ int cpim_forces_x(capac_ctx* c, const double* x) {
 
 int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
 
-// metal-metal CPIM forces alone (for kernel timing)
+// the metal-metal CPIM force kernel alone, adding to c->f (for kernel timing)
 int cpim_mm_only(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;

@@ -162,9 +162,10 @@ k_gradx(const double* __restrict__ r, const double* __restrict__ charge, BoxCons
 }
 
 // Derivatives along one CPIM parameter (seed 0..3: alpha, c, R_q, R_p) at
-// fixed positions and x: of the metal-metal force on metal i (mirroring
-// k_cpim_mm), of the CPIM energy (pair + self terms of pot[12..14]) and of
-// the relay rows of i applied to x.  One wavefront per metal atom.
+// fixed positions and x: of the metal-metal force on metal i (mirroring the
+// metal-metal terms of k_cpim_mm), of the CPIM energy (pair + self terms
+// of pot[12..14]) and of the relay rows of i applied to x.  One wavefront per
+// metal atom.
 __global__ void __launch_bounds__(256)
 k_theta_cpim(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int seed,
              double alpha, double capac, double Rq, double Rp, double* __restrict__ dF,

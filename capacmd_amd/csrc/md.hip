@@ -355,29 +355,33 @@ __global__ void k_ke_finish(const double* __restrict__ sums, int ndf, double* __
     md[1] = ek / (K_BOLTZ * ndf);
 }
 
-// Taylor polynomial of exp of degree 6 for |x| < 2^-10 (truncation below
-// 2e-25, far under an ulp), evaluated as a short chain in x^2 (Estrin)
+// Taylor polynomial of exp of degree 4 for |x| < 2^-10: the first omitted
+// term, x^5/120, stays below 7.4e-18 (a fifteenth of an ulp of 1).  Three
+// dependent fused operations after x.
 __device__ __forceinline__ double exp_small(double x) {
     const double x2 = x * x;
-    const double a = 0.5 + x * (1.0 / 6.0);
-    const double b = (1.0 / 24.0) + x * (1.0 / 120.0);
-    const double c = b + x2 * (1.0 / 720.0);
-    return (1.0 + x) + x2 * (a + x2 * c);
+    const double a = fma(x, 1.0 / 6.0, 0.5);
+    const double inner = fma(x2, 1.0 / 24.0, a);
+    return fma(x2, inner, 1.0 + x);
 }
 
 // The chain's sweeps (thermostat.c:135-186).  SMALL uses exp_small for every
 // exponential and returns false as soon as an argument leaves |x| < 2^-10
 // (the caller then starts over with the library exp); along a trajectory the
 // arguments, wdt/40 times a thermostat velocity, are of order 1e-4.  The
-// chain is one thread's serial dependency, so each exp's latency is paid in
-// full: the short polynomial takes a fraction of the library's.
+// chain is one thread's serial dependency, so every operation on it is paid
+// at full latency.  The updates are written to keep it short: each step
+// fraction (w dt / 8 Nc and the like) is formed once, v aa^2 + c aQ aa is
+// fma(v aa, aa, (c aQ) aa), the forward pass's exponentials (of velocities
+// the backward pass already fixed) leave the chain, and qMass v^2 - kT is one
+// fma.  The results differ from the reference's operations by rounding only.
 template <bool SMALL>
 __device__ __forceinline__ bool nhc_sweeps(double vQ[CAPAC_NHC], double aQ[CAPAC_NHC], double dt, double kT,
                                            double NkT, double qMass, double inv_q, double eKsum,
                                            double& scale_out) {
     constexpr int Nc = 5, Nys = 5, M = CAPAC_NHC;
     const double w1 = 0.41449077179437571194, w3 = -0.65796308717750284778;
-    // the step fractions of each weight, as the reference forms them
+    // the step fractions of each weight
     const double c8[2] = {-w1 * dt / (8 * Nc), -w3 * dt / (8 * Nc)};
     const double c4[2] = {w1 * dt / (4 * Nc), w3 * dt / (4 * Nc)};
     const double c2[2] = {-w1 * dt / (2 * Nc), -w3 * dt / (2 * Nc)};
@@ -388,23 +392,25 @@ __device__ __forceinline__ bool nhc_sweeps(double vQ[CAPAC_NHC], double aQ[CAPAC
 #pragma unroll
         for (int iys = 0; iys < Nys; ++iys) {
             const int w = iys == 2 ? 1 : 0;
-            const double wv = w ? w3 : w1;
-            vQ[M - 1] += aQ[M - 1] * wv * dt / (4 * Nc);
+            vQ[M - 1] = fma(aQ[M - 1], c4[w], vQ[M - 1]);
 #pragma unroll
             for (int im = 0; im < M - 1; ++im) {
-                aa = NHC_EXP(c8[w] * vQ[M - (im + 1)]);
-                vQ[M - 1 - (im + 1)] = vQ[M - 1 - (im + 1)] * aa * aa + c4[w] * aQ[M - 1 - (im + 1)] * aa;
+                const int k = M - 1 - (im + 1);
+                aa = NHC_EXP(c8[w] * vQ[k + 1]);
+                vQ[k] = fma(vQ[k] * aa, aa, (c4[w] * aQ[k]) * aa);
             }
             aa = NHC_EXP(c2[w] * vQ[0]);
             scale *= aa;
-            aQ[0] = (eKsum * scale * scale - NkT) * inv_q;
+            aQ[0] = fma(eKsum * scale, scale, -NkT) * inv_q;
+            double ef[M - 1];
+#pragma unroll
+            for (int im = 0; im < M - 1; ++im) ef[im] = NHC_EXP(c8[w] * vQ[im + 1]);
 #pragma unroll
             for (int im = 0; im < M - 1; ++im) {
-                aa = NHC_EXP(c8[w] * vQ[im + 1]);
-                vQ[im] = vQ[im] * aa * aa + c4[w] * aQ[im] * aa;
-                aQ[im + 1] = (qMass * vQ[im] * vQ[im] - kT) * inv_q;
+                vQ[im] = fma(vQ[im] * ef[im], ef[im], (c4[w] * ef[im]) * aQ[im]);
+                aQ[im + 1] = fma(qMass * vQ[im], vQ[im], -kT) * inv_q;
             }
-            vQ[M - 1] += aQ[M - 1] * wv * dt / (4 * Nc);
+            vQ[M - 1] = fma(aQ[M - 1], c4[w], vQ[M - 1]);
         }
         if (SMALL && !ok) return false;
     }

@@ -30,6 +30,8 @@
 #ifndef CAPAC_GPU_H
 #define CAPAC_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
