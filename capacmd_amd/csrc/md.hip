@@ -74,8 +74,9 @@ __device__ __forceinline__ double pbc1(double dx, double L) {
 
 // Three-site molecules whose constraints are (0,1), (0,2), (1,2) in file
 // order (rigid water) run a copy of the Gauss-Seidel loop with compile-time
-// atom indices, so the whole molecule stays in registers; operations and
-// their order are those of the general loop, so results are identical.
+// atom indices, so the whole molecule stays in registers; the loop-invariant
+// parts leave the loop (Cstr3 below) and positions and velocities are
+// identical to the general loop's.
 __device__ __forceinline__ bool is_rigid3(int f0, int nat, int c0, int nc, const int* __restrict__ ci,
                                           const int* __restrict__ cj) {
     return nat == 3 && nc == 3 && ci[c0] == f0 && cj[c0] == f0 + 1 && ci[c0 + 1] == f0 &&
@@ -86,23 +87,46 @@ struct Site {
     double x, y, z, vx, vy, vz, ox, oy, oz, im;
 };
 
-// one RATTLE_1 correction of constraint (a, b) (force.c:1949-1988)
-__device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, double rab, const BoxConst& bc, double dt) {
+// Rigid three-site molecules: the reference's Gauss-Seidel sweeps with what
+// does not change inside the loop computed once per constraint (a, b): the
+// old-position vector q (RATTLE_1) or the position vector p (RATTLE_2), the
+// factor 2 (1/m_a + 1/m_b) or (1/m_a + 1/m_b) r_ab^2, and the tolerance
+// bound, each formed as the reference forms it, so positions and velocities
+// stay bitwise the reference's.  (A constraint's tolerance test can flip on a
+// last-bit change, and one sweep more or less moves a hydrogen by ~1e-6 nm.)
+// Only the RATTLE_2 virial, which feeds nothing back, is deferred: it is
+// linear in the multipliers g of a constraint whose p is fixed, so the sweeps
+// carry the sum G and the virial is formed once at the end (rounding-level
+// differences from the reference's per-sweep sums).
+struct Cstr3 {
+    double vx, vy, vz;     // q (RATTLE_1) or p (RATTLE_2)
+    double rab2, bound, den;
+};
+
+__device__ __forceinline__ Cstr3 cstr1_setup(const Site& a, const Site& b, double rab, const BoxConst& bc) {
     const double tol2 = 2.0e-05;
+    Cstr3 c;
+    c.vx = pbc1(a.ox - b.ox, bc.box[0]);
+    c.vy = pbc1(a.oy - b.oy, bc.box[1]);
+    c.vz = pbc1(a.oz - b.oz, bc.box[2]);
+    c.rab2 = rab * rab;
+    c.bound = c.rab2 * tol2;
+    c.den = 2.0 * (a.im + b.im);
+    return c;
+}
+
+// one RATTLE_1 correction of constraint (a, b) (force.c:1949-1988)
+__device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, const Cstr3& c, const BoxConst& bc, double dt) {
     double px = pbc1(a.x - b.x, bc.box[0]);
     double py = pbc1(a.y - b.y, bc.box[1]);
     double pz = pbc1(a.z - b.z, bc.box[2]);
     double pab2 = px * px + py * py + pz * pz;
-    double rab2 = rab * rab;
-    double diff = rab2 - pab2;
-    if (!(fabs(diff) > rab2 * tol2)) return false;
-    double qx = pbc1(a.ox - b.ox, bc.box[0]);
-    double qy = pbc1(a.oy - b.oy, bc.box[1]);
-    double qz = pbc1(a.oz - b.oz, bc.box[2]);
-    double rp = qx * px + qy * py + qz * pz;
-    double ma = a.im, mb = b.im;
-    double g = diff / (2.0 * (ma + mb) * rp);
-    double dx = qx * g, dy = qy * g, dz = qz * g;
+    double diff = c.rab2 - pab2;
+    if (!(fabs(diff) > c.bound)) return false;
+    double rp = c.vx * px + c.vy * py + c.vz * pz;
+    double g = diff / (c.den * rp);
+    double dx = c.vx * g, dy = c.vy * g, dz = c.vz * g;
+    const double ma = a.im, mb = b.im;
     a.x += ma * dx; a.y += ma * dy; a.z += ma * dz;
     b.x -= mb * dx; b.y -= mb * dy; b.z -= mb * dz;
     dx /= dt; dy /= dt; dz /= dt;
@@ -111,26 +135,40 @@ __device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, double rab, const
     return true;
 }
 
+__device__ __forceinline__ Cstr3 cstr2_setup(const Site& a, const Site& b, double rab, const BoxConst& bc) {
+    Cstr3 c;
+    c.vx = pbc1(a.x - b.x, bc.box[0]);
+    c.vy = pbc1(a.y - b.y, bc.box[1]);
+    c.vz = pbc1(a.z - b.z, bc.box[2]);
+    c.rab2 = rab * rab;
+    c.bound = 1.0e-05;
+    c.den = (a.im + b.im) * c.rab2;
+    return c;
+}
+
 // one RATTLE_2 correction of constraint (a, b) (force.c:2046-2081)
-__device__ __forceinline__ bool rattle2_pair(Site& a, Site& b, double rab, const BoxConst& bc,
-                                             double half_dt, double w[9]) {
-    const double tol = 1.0e-05;
+__device__ __forceinline__ bool rattle2_pair(Site& a, Site& b, const Cstr3& c, double& G) {
     double ux = a.vx - b.vx, uy = a.vy - b.vy, uz = a.vz - b.vz;
-    double px = pbc1(a.x - b.x, bc.box[0]);
-    double py = pbc1(a.y - b.y, bc.box[1]);
-    double pz = pbc1(a.z - b.z, bc.box[2]);
-    double rv = px * ux + py * uy + pz * uz;
-    double ma = a.im, mb = b.im;
-    double g = -rv / ((ma + mb) * (rab * rab));
-    if (!(fabs(g) > tol)) return false;
-    double dx = px * g, dy = py * g, dz = pz * g;
+    double rv = c.vx * ux + c.vy * uy + c.vz * uz;
+    double g = -rv / c.den;
+    if (!(fabs(g) > c.bound)) return false;
+    double dx = c.vx * g, dy = c.vy * g, dz = c.vz * g;
+    const double ma = a.im, mb = b.im;
     a.vx += ma * dx; a.vy += ma * dy; a.vz += ma * dz;
     b.vx -= mb * dx; b.vy -= mb * dy; b.vz -= mb * dz;
-    dx /= half_dt; dy /= half_dt; dz /= half_dt;
-    w[0] -= px * dx; w[1] -= px * dy; w[2] -= px * dz;
-    w[3] -= py * dx; w[4] -= py * dy; w[5] -= py * dz;
-    w[6] -= pz * dx; w[7] -= pz * dy; w[8] -= pz * dz;
+    G += g;
     return true;
+}
+
+// the constraint virial of (a, b) for the multiplier sum G: w -= p (p G / (dt/2))
+__device__ __forceinline__ void rattle2_virial(const Cstr3& c, double G, double half_dt, double w[9]) {
+    const double h = G / half_dt;
+    const double d[3] = {c.vx * h, c.vy * h, c.vz * h};
+    const double pv[3] = {c.vx, c.vy, c.vz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) w[a * 3 + b] -= pv[a] * d[b];
 }
 
 __device__ __forceinline__ void load_site(Site& s, int i, int N, const double* __restrict__ r,
@@ -148,7 +186,8 @@ __device__ __forceinline__ void load_site(Site& s, int i, int N, const double* _
 constexpr int kRattleThreads = 64;
 
 // One thread per molecule; Gauss-Seidel sweeps over its constraints in file
-// order with the reference's arithmetic (no contraction, see the file pragma).
+// order.  The general loop keeps the reference's arithmetic (no contraction,
+// see the file pragma); rigid three-site molecules take the register path.
 __global__ void __launch_bounds__(kRattleThreads) k_rattle1(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol_cstr,
                           const int* __restrict__ mol_first, const int* __restrict__ mol_last,
                           const int* __restrict__ ci, const int* __restrict__ cj,
@@ -165,13 +204,14 @@ __global__ void __launch_bounds__(kRattleThreads) k_rattle1(int n_mols, int N, B
         load_site(s0, f0, N, r, v, old_r, inv_mass);
         load_site(s1, f0 + 1, N, r, v, old_r, inv_mass);
         load_site(s2, f0 + 2, N, r, v, old_r, inv_mass);
-        const double r01 = cr[c0], r02 = cr[c0 + 1], r12 = cr[c0 + 2];
+        const Cstr3 k01 = cstr1_setup(s0, s1, cr[c0], bc), k02 = cstr1_setup(s0, s2, cr[c0 + 1], bc),
+                    k12 = cstr1_setup(s1, s2, cr[c0 + 2], bc);
         int iter = 0, done = 0;
         while (!done) {
             done = 1;
-            if (rattle1_pair(s0, s1, r01, bc, dt)) done = 0;
-            if (rattle1_pair(s0, s2, r02, bc, dt)) done = 0;
-            if (rattle1_pair(s1, s2, r12, bc, dt)) done = 0;
+            if (rattle1_pair(s0, s1, k01, bc, dt)) done = 0;
+            if (rattle1_pair(s0, s2, k02, bc, dt)) done = 0;
+            if (rattle1_pair(s1, s2, k12, bc, dt)) done = 0;
             if (++iter > 1000) break;
         }
         const Site* st[3] = {&s0, &s1, &s2};
@@ -239,15 +279,20 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
         load_site(s0, f0, N, r, v, NULL, inv_mass);
         load_site(s1, f0 + 1, N, r, v, NULL, inv_mass);
         load_site(s2, f0 + 2, N, r, v, NULL, inv_mass);
-        const double r01 = cr[c0m], r02 = cr[c0m + 1], r12 = cr[c0m + 2];
+        const Cstr3 k01 = cstr2_setup(s0, s1, cr[c0m], bc), k02 = cstr2_setup(s0, s2, cr[c0m + 1], bc),
+                    k12 = cstr2_setup(s1, s2, cr[c0m + 2], bc);
+        double g01 = 0.0, g02 = 0.0, g12 = 0.0;
         int iter = 0, done = 0;
         while (!done) {
             done = 1;
-            if (rattle2_pair(s0, s1, r01, bc, half_dt, w)) done = 0;
-            if (rattle2_pair(s0, s2, r02, bc, half_dt, w)) done = 0;
-            if (rattle2_pair(s1, s2, r12, bc, half_dt, w)) done = 0;
+            if (rattle2_pair(s0, s1, k01, g01)) done = 0;
+            if (rattle2_pair(s0, s2, k02, g02)) done = 0;
+            if (rattle2_pair(s1, s2, k12, g12)) done = 0;
             if (++iter > 1000) break;
         }
+        rattle2_virial(k01, g01, half_dt, w);
+        rattle2_virial(k02, g02, half_dt, w);
+        rattle2_virial(k12, g12, half_dt, w);
         const Site* st[3] = {&s0, &s1, &s2};
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
