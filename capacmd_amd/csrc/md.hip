@@ -115,8 +115,19 @@ __device__ __forceinline__ Cstr3 cstr1_setup(const Site& a, const Site& b, doubl
     return c;
 }
 
-// one RATTLE_1 correction of constraint (a, b) (force.c:1949-1988)
-__device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, const Cstr3& c, const BoxConst& bc, double dt) {
+// x / d for a fixed divisor d, bitwise the IEEE quotient: with inv_d = RN(1/d)
+// and q within an ulp of x/d, RN(q + (x - q d) inv_d) is the correctly
+// rounded quotient (Markstein's theorem; x - q d is exact in one fma).  The
+// fixed-divisor identity was also checked on 2e8 random quotients by ten
+// step sizes on the host.  Three dependent operations instead of a division.
+__device__ __forceinline__ double div_fixed(double x, double d, double inv_d) {
+    const double q = x * inv_d;
+    return fma(fma(-q, d, x), inv_d, q);
+}
+
+// one RATTLE_1 correction of constraint (a, b) (force.c:1949-1988); inv_dt = 1/dt
+__device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, const Cstr3& c, const BoxConst& bc, double dt,
+                                             double inv_dt) {
     double px = pbc1(a.x - b.x, bc.box[0]);
     double py = pbc1(a.y - b.y, bc.box[1]);
     double pz = pbc1(a.z - b.z, bc.box[2]);
@@ -129,7 +140,7 @@ __device__ __forceinline__ bool rattle1_pair(Site& a, Site& b, const Cstr3& c, c
     const double ma = a.im, mb = b.im;
     a.x += ma * dx; a.y += ma * dy; a.z += ma * dz;
     b.x -= mb * dx; b.y -= mb * dy; b.z -= mb * dz;
-    dx /= dt; dy /= dt; dz /= dt;
+    dx = div_fixed(dx, dt, inv_dt); dy = div_fixed(dy, dt, inv_dt); dz = div_fixed(dz, dt, inv_dt);
     a.vx += ma * dx; a.vy += ma * dy; a.vz += ma * dz;
     b.vx -= mb * dx; b.vy -= mb * dy; b.vz -= mb * dz;
     return true;
@@ -206,12 +217,13 @@ __global__ void __launch_bounds__(kRattleThreads) k_rattle1(int n_mols, int N, B
         load_site(s2, f0 + 2, N, r, v, old_r, inv_mass);
         const Cstr3 k01 = cstr1_setup(s0, s1, cr[c0], bc), k02 = cstr1_setup(s0, s2, cr[c0 + 1], bc),
                     k12 = cstr1_setup(s1, s2, cr[c0 + 2], bc);
+        const double inv_dt = 1.0 / dt;
         int iter = 0, done = 0;
         while (!done) {
             done = 1;
-            if (rattle1_pair(s0, s1, k01, bc, dt)) done = 0;
-            if (rattle1_pair(s0, s2, k02, bc, dt)) done = 0;
-            if (rattle1_pair(s1, s2, k12, bc, dt)) done = 0;
+            if (rattle1_pair(s0, s1, k01, bc, dt, inv_dt)) done = 0;
+            if (rattle1_pair(s0, s2, k02, bc, dt, inv_dt)) done = 0;
+            if (rattle1_pair(s1, s2, k12, bc, dt, inv_dt)) done = 0;
             if (++iter > 1000) break;
         }
         const Site* st[3] = {&s0, &s1, &s2};

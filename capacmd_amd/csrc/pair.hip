@@ -15,8 +15,11 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
+// what a lane accumulates: the force on the current atom i, and (over all
+// of the lane's pairs) the two energies and the virial, symmetric for central
+// pair forces, so six components: xx, xy, xz, yy, yz, zz
 struct PairOut {
-    double fx, fy, fz, e7, e8, v[9];
+    double fx, fy, fz, e7, e8, v[6];
 };
 
 // The reference's expressions with each division by r, r^2, r^6 or the erf
@@ -74,11 +77,15 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
     double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
     o.fx -= fjx; o.fy -= fjy; o.fz -= fjz;
     o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
-    o.v[3] -= dy * fjx; o.v[4] -= dy * fjy; o.v[5] -= dy * fjz;
-    o.v[6] -= dz * fjx; o.v[7] -= dz * fjy; o.v[8] -= dz * fjz;
+    o.v[3] -= dy * fjy; o.v[4] -= dy * fjz; o.v[5] -= dz * fjz;
 }
 
-__global__ void __launch_bounds__(256)
+// Registers decide this kernel's speed: at 2 wavefronts per SIMD (207
+// VGPRs) the transcendental chains were exposed; per-lane energy and virial
+// totals (no per-atom copies), a symmetric 6-component virial, the atom's
+// data in scalar registers and a cap of 3 wavefronts per SIMD (two VGPRs
+// spill) took the full-grid launch from 241 to 174 us on Au561 + water.
+__global__ void __launch_bounds__(256, 3)
 k_pair(const double* __restrict__ r, const double* __restrict__ charge, const int* __restrict__ type,
        const PairParam* __restrict__ table, BoxConst bc, const int* __restrict__ n_front,
        const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
@@ -91,35 +98,49 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
     load_erfcx_table(s_erfcx);
     __syncthreads();
     const int lane = threadIdx.x & (CAPAC_WAVE - 1);
-    const int wave = threadIdx.x / CAPAC_WAVE;
-    PairOut acc = {};
-    for (int i = blockIdx.x * kWavesPerBlock + wave; i < N; i += gridDim.x * kWavesPerBlock) {
+    // the atom index is the same on every lane: keep it (and what is loaded
+    // through it) in scalar registers
+    const int first = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / CAPAC_WAVE);
+    PairOut o = {};
+    for (int i = first; i < N; i += gridDim.x * kWavesPerBlock) {
         double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
         double qi = charge[i];
         int ti = type[i] * nt;
         int nf = n_front[i], nb = n_back[i];
         const int* row = nbr + (size_t)i * stride;
-        PairOut o = {};
-        int total = nf + nb;
-        for (int k = lane; k < total; k += CAPAC_WAVE) {
-            int j = k < nf ? row[k] : row[stride - nb + (k - nf)];
-            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+        o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
+        const int total = nf + nb;
+        // two-stage software pipeline over the lane's pairs: the neighbour
+        // index two rounds ahead and the neighbour's data one round ahead are
+        // in flight while the current pair is evaluated, so a wavefront does
+        // not wait out two dependent loads every round (past the row's end
+        // the index is i itself, a valid address whose data go unused)
+        auto nbr_at = [&](int kk) { return kk < total ? (kk < nf ? row[kk] : row[stride - nb + (kk - nf)]) : i; };
+        int k = lane;
+        const int j0 = nbr_at(k);
+        int j1 = nbr_at(k + CAPAC_WAVE);
+        double x0 = r[j0], y0 = r[N + j0], z0 = r[2 * N + j0], q0 = charge[j0];
+        int t0 = type[j0];
+        for (; k < total; k += CAPAC_WAVE) {
+            const int j2 = nbr_at(k + 2 * CAPAC_WAVE);
+            const double x1 = r[j1], y1 = r[N + j1], z1 = r[2 * N + j1], q1 = charge[j1];
+            const int t1 = type[j1];
+            double dx = x0 - xi, dy = y0 - yi, dz = z0 - zi;
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
-            if (!(r2 < bc.rc2)) continue;     // rows reach rc + skin
-            pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o, s_erfcx);
+            if (r2 < bc.rc2)                  // rows reach rc + skin
+                pair_term(bc, s_tab[ti + t0], qi, q0, dx, dy, dz, r2, o, s_erfcx);
+            j1 = j2; x0 = x1; y0 = y1; z0 = z1; q0 = q1; t0 = t1;
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
             if (accumulate) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
             else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
         }
-        acc.e7 += o.e7; acc.e8 += o.e8;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) acc.v[c] += o.v[c];
     }
-    double vals[11] = {acc.e7, acc.e8, acc.v[0], acc.v[1], acc.v[2], acc.v[3], acc.v[4],
-                       acc.v[5], acc.v[6], acc.v[7], acc.v[8]};
+    const int wave = threadIdx.x / CAPAC_WAVE;
+    double vals[11] = {o.e7, o.e8, o.v[0], o.v[1], o.v[2], o.v[1], o.v[3],
+                       o.v[4], o.v[2], o.v[4], o.v[5]};
 #pragma unroll
     for (int c = 0; c < 11; ++c) {
         double s = wave_sum(vals[c]);
