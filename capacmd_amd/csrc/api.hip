@@ -216,9 +216,8 @@ static int validate(const capac_params* p) {
         return CAPAC_ERR_UNSUPPORTED;
     }
     for (int k = 0; k < p->n_types * p->n_types; ++k)
-        if (p->nb_funct[k] != 1 && p->nb_funct[k] != 4) {
-            capac_set_error("vdW funct %d (Buckingham/Morse) is not supported by this build",
-                            p->nb_funct[k]);
+        if (p->nb_funct[k] < 1 || p->nb_funct[k] > 4) {
+            capac_set_error("Error: unsupported vdW type: %d", p->nb_funct[k]);
             return CAPAC_ERR_UNSUPPORTED;
         }
     if (p->n_mols >= (1 << 26)) {
@@ -254,6 +253,56 @@ static int validate(const capac_params* p) {
             if (p->np_first[k] <= p->np_last[l] && p->np_first[l] <= p->np_last[k]) {
                 capac_set_error("nanoparticles %d and %d overlap", l, k);
                 return CAPAC_ERR_ARG;
+            }
+    }
+    {
+        // bonded terms: each term's atoms inside its own molecule (the bonded
+        // kernels give a molecule one thread and let it write its atoms)
+        const int cnt[5] = {p->n_bonds, p->n_pairs14, p->n_angles, p->n_dihedrals, p->n_vsites};
+        const int* atoms[5] = {p->bond_atoms, p->pair14_atoms, p->angle_atoms, p->dihedral_atoms,
+                               p->vsite_atoms};
+        const int width[5] = {2, 2, 3, 4, 4};
+        const void* par[5] = {p->bond_par, p->pair14_par, p->angle_par, p->dihedral_par, p->vsite_par};
+        const char* what[5] = {"bond", "1-4 pair", "angle", "dihedral", "virtual site"};
+        const int total = cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4];
+        for (int t = 0; t < 5; ++t)
+            if (cnt[t] < 0 || (cnt[t] > 0 && (!atoms[t] || !par[t]))) {
+                capac_set_error("%s list: count %d without arrays", what[t], cnt[t]);
+                return CAPAC_ERR_ARG;
+            }
+        if (total > 0 && !p->mol_terms) { capac_set_error("bonded terms without mol_terms"); return CAPAC_ERR_ARG; }
+        if ((p->n_bonds > 0 && !p->bond_funct) || (p->n_angles > 0 && !p->angle_funct)) {
+            capac_set_error("bonded terms without their funct arrays");
+            return CAPAC_ERR_ARG;
+        }
+        const int M1 = p->n_mols + 1;
+        for (int t = 0; total > 0 && t < 5; ++t) {
+            const int* rg = p->mol_terms + (size_t)t * M1;
+            if (rg[0] != 0 || rg[p->n_mols] != cnt[t]) {
+                capac_set_error("%s ranges do not cover the %d terms", what[t], cnt[t]);
+                return CAPAC_ERR_ARG;
+            }
+            for (int m = 0; m < p->n_mols; ++m) {
+                if (rg[m + 1] < rg[m]) { capac_set_error("%s ranges decrease at molecule %d", what[t], m); return CAPAC_ERR_ARG; }
+                for (int q = rg[m]; q < rg[m + 1]; ++q)
+                    for (int u = 0; u < width[t]; ++u) {
+                        const int a = atoms[t][(size_t)width[t] * q + u];
+                        if (a < p->mol_first[m] || a > p->mol_last[m]) {
+                            capac_set_error("%s %d: atom %d outside its molecule %d", what[t], q, a, m);
+                            return CAPAC_ERR_ARG;
+                        }
+                    }
+            }
+        }
+        for (int q = 0; q < p->n_bonds; ++q)
+            if (p->bond_funct[q] != 1 && p->bond_funct[q] != 3) {
+                capac_set_error("Error: unsupported bond function type: %d", p->bond_funct[q]);
+                return CAPAC_ERR_UNSUPPORTED;
+            }
+        for (int q = 0; q < p->n_angles; ++q)
+            if (p->angle_funct[q] != 1 && p->angle_funct[q] != 5) {
+                capac_set_error("Error: unsupported angle function type: %d", p->angle_funct[q]);
+                return CAPAC_ERR_UNSUPPORTED;
             }
     }
     for (int d = 0; d < 3; ++d)
@@ -332,6 +381,8 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
             t[k].c12 = p->nb_c12[k];
             t[k].width = p->nb_a[k];
             t[k].inv_width = p->nb_a[k] != 0.0 ? 1.0 / p->nb_a[k] : 0.0;
+            t[k].a = p->nb_a[k];
+            t[k].b = p->nb_b[k];
         }
         rc = upload(&c->pair, (const PairParam*)t, nt * nt);
         free(t);
@@ -340,6 +391,23 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     TRY(upload(&c->mol_first, p->mol_first, p->n_mols));
     TRY(upload(&c->mol_last, p->mol_last, p->n_mols));
     TRY(upload(&c->mol_cstr, p->mol_cstr, p->n_mols + 1));
+    c->n_bonds = p->n_bonds; c->n_pairs14 = p->n_pairs14; c->n_angles = p->n_angles;
+    c->n_dihedrals = p->n_dihedrals; c->n_vsites = p->n_vsites;
+    if (p->n_bonds + p->n_pairs14 + p->n_angles + p->n_dihedrals + p->n_vsites > 0) {
+        TRY(upload(&c->mol_terms, p->mol_terms, 5 * ((size_t)p->n_mols + 1)));
+        TRY(upload(&c->bond_atoms, p->bond_atoms, 2 * (size_t)p->n_bonds));
+        TRY(upload(&c->bond_funct, p->bond_funct, (size_t)p->n_bonds));
+        TRY(upload(&c->bond_par, p->bond_par, 3 * (size_t)p->n_bonds));
+        TRY(upload(&c->pair14_atoms, p->pair14_atoms, 2 * (size_t)p->n_pairs14));
+        TRY(upload(&c->pair14_par, p->pair14_par, 6 * (size_t)p->n_pairs14));
+        TRY(upload(&c->angle_atoms, p->angle_atoms, 3 * (size_t)p->n_angles));
+        TRY(upload(&c->angle_funct, p->angle_funct, (size_t)p->n_angles));
+        TRY(upload(&c->angle_par, p->angle_par, 4 * (size_t)p->n_angles));
+        TRY(upload(&c->dihedral_atoms, p->dihedral_atoms, 4 * (size_t)p->n_dihedrals));
+        TRY(upload(&c->dihedral_par, p->dihedral_par, 6 * (size_t)p->n_dihedrals));
+        TRY(upload(&c->vsite_atoms, p->vsite_atoms, 4 * (size_t)p->n_vsites));
+        TRY(upload(&c->vsite_par, p->vsite_par, 3 * (size_t)p->n_vsites));
+    }
     TRY(upload(&c->cstr_i, p->cstr_i, p->n_cstr));
     TRY(upload(&c->cstr_j, p->cstr_j, p->n_cstr));
     TRY(upload(&c->cstr_r, p->cstr_r, p->n_cstr));
@@ -403,6 +471,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->seg_cap[SEG_EXT] = (N + 255) / 256 + 1;
         c->seg_cap[SEG_RATTLE] = (p->n_mols + 63) / 64 + 1;      // workgroups of 64 or more
         c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;
+        c->seg_cap[SEG_BONDED] = bonded_blocks(p->n_mols) + 1;
         int total = 0;
         for (int s = 0; s < SEG_COUNT; ++s) { c->seg_off[s] = total; total += c->seg_cap[s]; }
         c->n_partial = total;
@@ -458,7 +527,10 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
                     c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32,
-                    c->x_hist[0], c->x_hist[1], c->x_hist[2]};
+                    c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
+                    c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
+                    c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,
+                    c->vsite_par};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     if (c->red_host) (void)hipHostFree(c->red_host);
@@ -641,7 +713,15 @@ static int side_forces(capac_ctx* c) {
     hipStream_t main_stream = c->stream;
     c->stream = c->side;
     mark(c, 11);                     // events 11-13: the side stream's phases
-    rc = pair_nonbonded(c, NULL, side_pair_blocks(c));
+    if (has_bonded(c)) {
+        // the reference computes the bonded forces first (force.c:2163-2168);
+        // the pair forces then add to them
+        rc = fill_d(c, c->f, 3 * (size_t)c->bc.N, 0.0);
+        if (!rc) rc = bonded_forces(c);
+        if (!rc) rc = pair_nonbonded(c, c->f, side_pair_blocks(c));
+    } else {
+        rc = pair_nonbonded(c, NULL, side_pair_blocks(c));
+    }
     mark(c, 12);
     if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
     mark(c, 13);
@@ -663,6 +743,8 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
     deferred = deferred && c->n_mat > 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
         mark(c, 0);
+        // virtual sites from their atoms first (force.c:2128-2134)
+        if ((rc = vsite_build(c))) return rc;
         if ((rc = clear_red(c, 0, 256))) return rc;
         if (c->n_mat && (rc = copy_d(c, c->x, c->x_backup, c->n_mat))) return rc;
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
@@ -698,6 +780,9 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         if (c->n_mat && (rc = cpim_forces(c))) return rc;
         rc = solve_rc;
         int rc2 = ext_field(c);
+        // forces on virtual sites go to their atoms once everything is in
+        // (force.c:2300-2306)
+        if (!rc2) rc2 = vsite_spread(c, c->f);
         if (rc2) return rc2;
         mark(c, 7);
         if (deferred) break;
@@ -730,6 +815,14 @@ static void take_force_results(capac_ctx* c) {
     c->pot[13] = h[50];
     c->pot[14] = h[51];
     for (int k = 0; k < 9; ++k) c->vir[k] = h[2 + k] + h[17 + k] + h[34 + k] + h[52 + k];
+    if (has_bonded(c)) {
+        c->pot[2] = h[112];
+        c->pot[3] = h[113];
+        c->pot[4] = h[114];
+        c->pot[7] += h[115];
+        c->pot[8] += h[116];
+        for (int k = 0; k < 9; ++k) c->vir[k] += h[117 + k];
+    }
     c->timing.n_force_calls++;
     if (c->timing_on) {
         float ms;
@@ -828,6 +921,37 @@ extern "C" int capac_nonbonded(capac_ctx* c, double* f, double pot[CAPAC_NPOT], 
     const int pk[2] = {7, 8}, hk[2] = {0, 1};
     pot_out(c, pot, vir, pk, hk, 2, 2);
     return f ? capac_get_forces(c, f) : 0;
+}
+
+extern "C" int capac_bonded(capac_ctx* c, double* f, double pot[CAPAC_NPOT], double vir[9]) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = zero_outputs(c);
+    if (!rc) rc = bonded_forces(c);
+    if (!rc) rc = fetch_red(c);
+    if (rc) return rc;
+    const int pk[5] = {2, 3, 4, 7, 8}, hk[5] = {112, 113, 114, 115, 116};
+    pot_out(c, pot, vir, pk, hk, 5, 117);
+    return f ? capac_get_forces(c, f) : 0;
+}
+
+extern "C" int capac_vsite_build(capac_ctx* c, double* r) {
+    if (!c) return CAPAC_ERR_ARG;
+    int rc = vsite_build(c);
+    if (rc) return rc;
+    c->nb_dirty = 1;                 // positions changed outside the MD path
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (r) CHECK_HIP(hipMemcpy(r, c->r, sizeof(double) * 3 * c->bc.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int capac_vsite_spread(capac_ctx* c, const double* f_in, double* f_out) {
+    if (!c || !f_in || !f_out) return CAPAC_ERR_ARG;
+    const size_t n3 = 3 * (size_t)c->bc.N;
+    CHECK_HIP(hipMemcpyAsync(c->f, f_in, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+    int rc = vsite_spread(c, c->f);
+    if (rc) return rc;
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return capac_get_forces(c, f_out);
 }
 
 extern "C" int capac_sutton_chen(capac_ctx* c, double* f, double pot[CAPAC_NPOT], double vir[9],

@@ -58,8 +58,9 @@ struct BoxConst {
 };
 
 struct PairParam {          // one entry of the n_types x n_types table
-    int funct;
+    int funct;                  // 1 LJ + Coulomb, 2 Buckingham, 3 Morse, 4 erf vdW
     double c6, c12, width, inv_width;
+    double a, b;                // Buckingham A, B or Morse D, a (R in c6)
 };
 
 // BiCGSTAB scalars, resident on the device (bicg_stab.c:289-292)
@@ -101,6 +102,11 @@ struct capac_ctx {
     double* cstr_r;
     int *np_first, *np_last, *np_of_metal;
     double* np_charge;
+    // bonded terms and virtual sites, absolute atom indices (capac_params)
+    int n_bonds, n_pairs14, n_angles, n_dihedrals, n_vsites;
+    int *mol_terms, *bond_atoms, *bond_funct, *pair14_atoms, *angle_atoms, *angle_funct;
+    int *dihedral_atoms, *vsite_atoms;
+    double *bond_par, *pair14_par, *angle_par, *dihedral_par, *vsite_par;
 
     // cell list
     int ncell[3], n_cell;
@@ -251,6 +257,12 @@ double* solver_scratch(capac_ctx* c);
 void solver_predict(capac_ctx* c);          // x from the history (md steps)
 void solver_push_history(capac_ctx* c);     // the solution just found joins it
 int cpim_mm_only(capac_ctx* c);
+// bonded.hip
+bool has_bonded(const capac_ctx* c);        // bonds, 1-4 pairs, angles or dihedrals
+int bonded_blocks(int n_mols);              // workgroups of the bonded kernel
+int bonded_forces(capac_ctx* c);            // adds them to c->f; red[112..126)
+int vsite_build(capac_ctx* c);              // site positions from their atoms
+int vsite_spread(capac_ctx* c, double* f);  // site forces onto their atoms
 // em.hip
 void em_destroy(capac_ctx* c);
 int em_begin(capac_ctx* c);
@@ -276,7 +288,7 @@ int ext_field(capac_ctx* c);
 // segment of c->partial; reduce_partials queues their fixed-order reduction
 // into red[red_offset..+nval), and flush_reductions launches every queued one
 // in a single kernel (before anything reads red).
-enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_EXT, SEG_RATTLE, SEG_MOM, SEG_COUNT };
+enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_EXT, SEG_RATTLE, SEG_MOM, SEG_BONDED, SEG_COUNT };
 double* partial_seg(capac_ctx* c, int seg);
 int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int red_offset);
 int flush_reductions(capac_ctx* c);

@@ -1,6 +1,8 @@
 // Non-bonded pair forces: shifted-force LJ + damped-shifted-force Coulomb
-// (funct 1) and erf-damped vdW (funct 4), the reference's mpi_nonb_pair +
-// compute_nonbonded + compute_erf_vdw (force.c:1402-1540, 710-816, 591-632).
+// (funct 1), Buckingham (2), Morse (3) and erf-damped vdW (4), the
+// reference's mpi_nonb_pair + compute_nonbonded + compute_buckingham +
+// compute_morse + compute_erf_vdw (force.c:1402-1540, 710-816, 639-703,
+// 591-632).
 //
 // One wavefront per atom i over its full neighbour row (front and back
 // segments); lanes take consecutive neighbours, so the index stream is
@@ -72,6 +74,19 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
             flj = ((12.0 * cr12 - 6.0 * cr6) * inv_r * erf_rR -
                    (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * exp(-rR * rR) * pp.inv_width) * inv_r;
         }
+    } else if (pp.funct == 2) {
+        // Buckingham (force.c:674-703): A e^{-B r} - C6 / r^6, no Coulomb term
+        const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+        const double e = exp(-pp.b * rij);
+        flj = pp.a * e * pp.b * inv_r - 6.0 * pp.c6 * (inv_r6 * inv_r2);
+        o.e8 += pp.a * e - pp.c6 * inv_r6;
+    } else if (pp.funct == 3) {
+        // Morse (force.c:639-668): D (e^{-2a(r-R)} - 2 e^{-a(r-R)}), D in a,
+        // the width a in b, R in c6; no Coulomb term
+        const double e = exp(-pp.b * (rij - pp.c6));
+        const double e2 = e * e;
+        flj = 2.0 * pp.a * pp.b * (e2 - e) * inv_r;
+        o.e8 += pp.a * (e2 - 2.0 * e);
     }
     double fs = flj + fqq;               // f_j = fs * r_ij, f_i = -f_j
     double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;

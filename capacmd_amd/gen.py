@@ -431,6 +431,186 @@ def write_deck(deck: Deck, directory: str) -> Tuple[str, str, str]:
     return gro, par, mds
 
 
+# ------------------------------------------------- general topologies
+#
+# A deck that exercises every bonded term, virtual sites and every pair
+# type the reference reads (file.c:614-1000): a flexible six-atom chain
+# molecule with harmonic and Morse bonds, 1-4 pairs of both kinds, harmonic and Urey-Bradley angles (one with kb_ub = 0), a
+# Ryckaert-Bellemans dihedral and periodic dihedrals (funct 1, 4, 9; n = 1-4;
+# phi0 = 0 and 180); TIP5P-type water with two virtual sites; Buckingham and
+# Morse pair types beside LJ and erf-damped vdW; an Au13 cluster with CPIM
+# and an external field.
+
+TIP5P_ROH, TIP5P_HOH_DEG = 0.09572, 104.52
+TIP5P_A, TIP5P_C = -0.344908262, 6.4437903      # vsite a = b, +-c (nm^-1)
+TIP5P_QH = 0.241
+FLX_Q = (-0.2, 0.1, 0.1, -0.2, 0.1, 0.1)
+FLX_TYPE = (1, 1, 1, 1, 2, 2)
+FLX_M = (12.011, 12.011, 12.011, 12.011, 1.008, 1.008)
+
+
+def _place(parent: np.ndarray, grand: np.ndarray, great: np.ndarray, bond: float,
+           angle_deg: float, dihedral_deg: float) -> np.ndarray:
+    """z-matrix placement: the new atom at `bond` from parent, `angle` with
+    grand, `dihedral` with great."""
+    b = parent - grand
+    b /= np.linalg.norm(b)
+    n = np.cross(grand - great, b)
+    n /= np.linalg.norm(n)
+    m = np.cross(n, b)
+    a, d = math.radians(angle_deg), math.radians(dihedral_deg)
+    local = np.array([-bond * math.cos(a), bond * math.sin(a) * math.cos(d), bond * math.sin(a) * math.sin(d)])
+    return parent + local[0] * b + local[1] * m + local[2] * n
+
+
+def _flx_template() -> np.ndarray:
+    c0 = np.zeros(3)
+    c1 = np.array([0.153, 0.0, 0.0])
+    c2 = c1 + 0.153 * np.array([math.cos(math.radians(69.0)), math.sin(math.radians(69.0)), 0.0])
+    c3 = _place(c2, c1, c0, 0.153, 112.0, 180.0)
+    h4 = _place(c3, c2, c1, 0.109, 109.5, 60.0)
+    h5 = _place(c0, c1, c2, 0.109, 109.5, -60.0)
+    x = np.array([c0, c1, c2, c3, h4, h5])
+    return x - x.mean(axis=0)
+
+
+def _tip5p(o: np.ndarray, rot: np.ndarray) -> np.ndarray:
+    half = math.radians(TIP5P_HOH_DEG) / 2.0
+    h1 = o + rot @ np.array([TIP5P_ROH * math.sin(half), TIP5P_ROH * math.cos(half), 0.0])
+    h2 = o + rot @ np.array([-TIP5P_ROH * math.sin(half), TIP5P_ROH * math.cos(half), 0.0])
+    rij, rik = h1 - o, h2 - o
+    cr = np.cross(rij, rik)
+    l1 = o + TIP5P_A * rij + TIP5P_A * rik - TIP5P_C * cr
+    l2 = o + TIP5P_A * rij + TIP5P_A * rik + TIP5P_C * cr
+    return np.array([o, h1, h2, l1, l2])
+
+
+def write_topology_deck(directory: str, seed: int = 106, n_steps: int = 10) -> Tuple[str, str, str]:
+    """The general-topology deck above (3.0 nm box, 13 Au, 8 chain molecules,
+    40 TIP5P-type waters: 261 atoms)."""
+    rng = np.random.default_rng(3000 + seed)
+    L = 3.0
+    box = np.array([L, L, L])
+    metal = mackay_icosahedron(2) + L / 2
+    sites = []
+    g = np.arange(4) * (L / 4) + L / 8
+    for x in g:
+        for y in g:
+            for z in g:
+                p = np.array([x, y, z])
+                d = p - L / 2
+                if np.linalg.norm(d) > 0.62:
+                    sites.append(p)
+    rng.shuffle(sites)
+    n_flx, n_w = 8, 40
+    assert len(sites) >= n_flx + n_w
+    tmpl = _flx_template()
+    rots = random_rotations(rng, n_flx + n_w)
+    # bond lengths and angles off their minima, so every term pulls at step 0
+    flx = [sites[k] + (tmpl + rng.normal(0.0, 0.006, tmpl.shape)) @ rots[k].T for k in range(n_flx)]
+    wat = [_tip5p(sites[n_flx + k] + rng.normal(0.0, 0.01, 3), rots[n_flx + k]) for k in range(n_w)]
+    pos = np.concatenate([metal] + flx + wat)
+    masses = np.concatenate([np.full(len(metal), M_AU)] + [np.array(FLX_M)] * n_flx +
+                            [np.array([M_O, M_H, M_H, 0.0, 0.0])] * n_w)
+    vel = np.zeros_like(pos)
+    real = masses > 0
+    vel[real] = maxwell_boltzmann(masses[real], 300.0, np.random.default_rng(4000 + seed))
+    os.makedirs(directory, exist_ok=True)
+    gro = os.path.join(directory, "init.gro")
+    lines = ["step  0  vQ  0.000000  vP  0.000000", "%5d" % len(pos)]
+    res, i = 0, 0
+    for m in range(len(metal)):
+        res += 1
+        lines.append("%5d%-5s%5s%5d%14.9f%14.9f%14.9f%14.9f%14.9f%14.9f" % (res, "AU", "Au", i + 1, *pos[i], *vel[i]))
+        i += 1
+    for names, count, rn in ((("C0", "C1", "C2", "C3", "H4", "H5"), n_flx, "FLX"),
+                             (("OW", "HW1", "HW2", "LP1", "LP2"), n_w, "T5P")):
+        for _ in range(count):
+            res += 1
+            for nm in names:
+                lines.append("%5d%-5s%5s%5d%14.9f%14.9f%14.9f%14.9f%14.9f%14.9f" % (res, rn, nm, i + 1, *pos[i], *vel[i]))
+                i += 1
+    lines.append("%10.5f%10.5f%10.5f" % tuple(box))
+    with open(gro, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+    def lj(sig, eps):
+        return 4.0 * eps * sig ** 6, 4.0 * eps * sig ** 12
+
+    out = ["SCALE_LJ_QQ    0.50000000     0.83333333", "",
+           "MOLECULES  3", "gold       %d" % len(metal), "FLX        %d" % n_flx, "T5P        %d" % n_w, "",
+           "gold_ATOMS  1", "  0.000000  %.6f  0" % M_AU, "",
+           "FLX_ATOMS  6"]
+    out += ["  %.6f  %.6f  %d" % (FLX_Q[a], FLX_M[a], FLX_TYPE[a]) for a in range(6)]
+    out += ["", "T5P_ATOMS  5", "  0.000000  %.6f  3" % M_O, "  %.6f  %.6f  4" % (TIP5P_QH, M_H),
+            "  %.6f  %.6f  4" % (TIP5P_QH, M_H), "  %.6f  0.000000  4" % -TIP5P_QH,
+            "  %.6f  0.000000  4" % -TIP5P_QH, ""]
+    cc, cw, hh, ow = lj(0.35, 0.3), lj(0.33, 0.4), lj(0.25, 0.1), lj(0.312, 0.669)
+    out += ["NONBONDED  5  15",
+            " 1  0.0  0.0",                                           # Au-Au (Sutton-Chen)
+            " 4  %.12g  %.12g  %.12g" % AU_O_ERFVDW,                  # Au-C
+            " 2  2500.0  32.0  1.0e-4",                               # Au-H  Buckingham
+            " 4  %.12g  %.12g  %.12g" % AU_O_ERFVDW,                  # Au-O
+            " 1  0.0  0.0",                                           # Au-HW/LP
+            " 1  %.8e  %.8e" % cc,                                    # C-C
+            " 3  0.5  15.0  0.30",                                    # C-H   Morse
+            " 1  %.8e  %.8e" % cw,                                    # C-O
+            " 1  0.0  0.0",                                           # C-HW/LP
+            " 1  %.8e  %.8e" % hh,                                    # H-H
+            " 3  0.8  20.0  0.32",                                    # H-O   Morse
+            " 1  0.0  0.0",                                           # H-HW/LP
+            " 1  %.8e  %.8e" % ow,                                    # O-O
+            " 1  0.0  0.0",                                           # O-HW/LP
+            " 1  0.0  0.0", ""]                                       # HW/LP
+    out += ["gold_BONDS  0", "gold_PAIRS  0", "gold_ANGLES  0", "gold_DIHEDRALS  0", "gold_VSITES  0",
+            "gold_CSTRS  0", ""]
+    # no exclusion-only bonds (funct 5): the reference's mpi_bonded adds an
+    # uninitialised force pair for them (force.c:1710-1742), which the
+    # reader leaves without atoms (file.c:752-756)
+    out += ["FLX_BONDS  5",
+            "0 1 1  0.153  250000.0",
+            "1 2 3  0.153  350.0  19.0",
+            "2 3 1  0.153  250000.0",
+            "3 4 1  0.109  300000.0",
+            "0 5 1  0.109  300000.0",
+            "FLX_PAIRS  2",
+            "0 3 1",
+            "5 2 2  0.5  0.1  0.1  0.25  0.1",
+            "FLX_ANGLES  4",
+            "0 1 2 1  111.0  460.0",
+            "1 2 3 5  112.0  400.0  0.252  20000.0",
+            "2 3 4 1  109.5  300.0",
+            "5 0 1 5  109.5  300.0  0.25  0.0",
+            "FLX_DIHEDRALS  6",
+            "0 1 2 3 3  9.28  12.16  -13.12  -3.06  26.24  -31.5",
+            "1 2 3 4 9  0.0  2.0  3",
+            "5 0 1 2 1  180.0  1.5  2",
+            "0 1 2 3 9  0.0  0.5  4",
+            "4 3 2 1 4  180.0  0.8  1",
+            "5 0 1 2 9  180.0  0.6  4",
+            "FLX_VSITES  0", "FLX_CSTRS  0", ""]
+    out += ["T5P_BONDS  0", "T5P_PAIRS  0", "T5P_ANGLES  0", "T5P_DIHEDRALS  0",
+            "T5P_VSITES  2",
+            "0 1 2 3 4  %.9f  %.9f  %.7f" % (TIP5P_A, TIP5P_A, -TIP5P_C),
+            "0 1 2 4 4  %.9f  %.9f  %.7f" % (TIP5P_A, TIP5P_A, TIP5P_C),
+            "T5P_CSTRS  3",
+            "0 1  %.10f" % TIP5P_ROH, "0 2  %.10f" % TIP5P_ROH,
+            "1 2  %.10f" % (2.0 * TIP5P_ROH * math.sin(math.radians(TIP5P_HOH_DEG) / 2.0)), ""]
+    out += ["MM_METAL  0  %d" % (len(metal) - 1),
+            "QSC_METAL  %d  %d  %.10f  %.6f  %.6f" % (SC_N, SC_M, SC_EPS, SC_C, SC_A),
+            "CPFF_METAL  %.10f  %.10f   ;  atomic units" % (CPIM_POLAR, CPIM_CAPAC), "",
+            "NANOPARTICLES  1", "START_END_CHG  0  %d  0.000000" % (len(metal) - 1)]
+    par = os.path.join(directory, "param.txt")
+    with open(par, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    deck = Deck(name="topo_au13", box=box, metal=metal, np_ranges=[(0, len(metal) - 1, 0.0)],
+                water=np.zeros((0, 3, 3)), vel=vel, n_steps=n_steps, save_step=1, dt=0.0005,
+                efield=(0.0, 0.0, 0.3))
+    mds = os.path.join(directory, "mdset.txt")
+    write_mdset(deck, mds)
+    return gro, par, mds
+
+
 def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])

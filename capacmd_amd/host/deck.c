@@ -107,14 +107,52 @@ static int read_mdset(capac_deck* d, const char* path, char* err, int errlen) {
 
 typedef struct { int i, j; double r; } cstr_t;
 
+/* one bonded term or virtual site of a molecule type, local atom indices */
+typedef struct { int a[4]; int funct; double par[6]; } term_t;
+
 typedef struct {
     int n_atoms;
     double* q; double* m; int* type;
     unsigned excl[32];          /* excl[a] bit b: a and b excluded */
     int n_cstr; cstr_t* cstr;
-    int n_bonded_terms;         /* bonds (not funct 5), pairs, angles, dihedrals */
+    int n_term[5];              /* CAPAC_TERM_*: bonds (funct 5 bonds only
+                                   exclude and are not kept), 1-4 pairs,
+                                   angles, dihedrals, virtual sites */
+    term_t* term[5];
     int n_vsites;
 } moltype;
+
+static void moltype_free(moltype* mt, int n) {
+    for (int k = 0; mt && k < n; ++k) {
+        free(mt[k].q); free(mt[k].m); free(mt[k].type); free(mt[k].cstr);
+        for (int t = 0; t < 5; ++t) free(mt[k].term[t]);
+    }
+    free(mt);
+}
+
+/* periodic dihedral (funct 1, 4, 9) to the Ryckaert-Bellemans coefficients
+ * that compute_dihedral_9 passes on (force.c:1236-1347), the same operations
+ * on the same values; -2 for an n it rejects, -1 for a phi0 it rejects */
+static int periodic_to_rb(double phi0, double kphi, int n, double c[6]) {
+    for (int k = 0; k < 6; ++k) c[k] = 0.0;
+    if (n < 1 || n > 4) return -2;
+    if (phi0 != 0.0 && phi0 != 180.0) return -1;
+    const int flip = phi0 == 180.0;
+    if (n == 1) { c[0] = kphi; c[1] = flip ? kphi * (-1.0) : kphi; }
+    else if (n == 2) { if (flip) { c[0] = kphi * 2.0; c[2] = kphi * (-2.0); } else c[2] = kphi * 2.0; }
+    else if (n == 3) {
+        c[0] = kphi;
+        if (flip) { c[1] = kphi * 3.0; c[3] = kphi * (-4.0); }
+        else { c[1] = kphi * (-3.0); c[3] = kphi * 4.0; }
+    } else if (n == 4) {
+        if (flip) { c[2] = kphi * 8.0; c[4] = kphi * (-8.0); }
+        else { c[0] = kphi * 2.0; c[2] = kphi * (-8.0); c[4] = kphi * 8.0; }
+    } else return -1;
+    c[1] *= -1.0;
+    c[3] *= -1.0;
+    c[5] *= -1.0;
+    return 0;
+}
 
 static void excl_pair(moltype* mt, int a, int b) {
     if (a >= 0 && b >= 0 && a < 32 && b < 32) {
@@ -198,30 +236,87 @@ static int read_param(capac_deck* d, const char* path, moltype** out_mt, char* e
             snprintf(name, sizeof name, "%s_%s", d->mol_name[k], secs[s]);
             if (section(&rd, name, &n)) goto done;
             if (s == 5) { mt[k].n_cstr = n; mt[k].cstr = xcalloc(n, sizeof(cstr_t)); }
+            else mt[k].term[s] = xcalloc(n, sizeof(term_t));
             for (int t = 0; t < n; ++t) {
                 int ai = 0, aj = 0, ak = 0, al = 0, f = 0;
                 if (next_line(&rd)) goto done;
+                term_t tm;
+                memset(&tm, 0, sizeof tm);
                 if (s == 0) {               /* bonds: i j funct ... (file.c:721-765) */
                     sscanf(rd.line, "%d%d%d", &ai, &aj, &f);
+                    if (f == 1) sscanf(rd.line, "%*d%*d%*d%le%le", &tm.par[0], &tm.par[1]);
+                    else if (f == 3) sscanf(rd.line, "%*d%*d%*d%le%le%le", &tm.par[0], &tm.par[1], &tm.par[2]);
+                    else if (f != 5) {
+                        fail(&rd, "Error: unsupported bond function type: %d in line:\n%s\n", f, rd.line);
+                        goto done;
+                    }
                     excl_pair(&mt[k], ai, aj);
-                    if (f != 5) mt[k].n_bonded_terms++;
-                } else if (s == 1) {        /* 1-4 pairs */
-                    sscanf(rd.line, "%d%d", &ai, &aj);
+                    tm.a[0] = ai; tm.a[1] = aj; tm.funct = f;
+                    if (f != 5) mt[k].term[0][mt[k].n_term[0]++] = tm;
+                } else if (s == 1) {        /* 1-4 pairs (file.c:771-809) */
+                    sscanf(rd.line, "%d%d%d", &ai, &aj, &f);
+                    if (f == 2)             /* scaleQQ q_i q_j V (sigma) W (epsilon) */
+                        sscanf(rd.line, "%*d%*d%*d%lf%lf%lf%le%le", &tm.par[0], &tm.par[1], &tm.par[2],
+                               &tm.par[3], &tm.par[4]);
+                    else if (f != 1) {
+                        fail(&rd, "Error: unsupported pair function type: %d in line:\n%s\n", f, rd.line);
+                        goto done;
+                    }
                     excl_pair(&mt[k], ai, aj);
-                    mt[k].n_bonded_terms++;
-                } else if (s == 2) {        /* angles exclude all three pairs */
-                    sscanf(rd.line, "%d%d%d", &ai, &aj, &ak);
+                    tm.a[0] = ai; tm.a[1] = aj; tm.funct = f;
+                    mt[k].term[1][mt[k].n_term[1]++] = tm;
+                } else if (s == 2) {        /* angles exclude all three pairs (file.c:815-859) */
+                    sscanf(rd.line, "%d%d%d%d", &ai, &aj, &ak, &f);
+                    if (f == 1) sscanf(rd.line, "%*d%*d%*d%*d%le%le", &tm.par[0], &tm.par[1]);
+                    else if (f == 5)
+                        sscanf(rd.line, "%*d%*d%*d%*d%le%le%le%le", &tm.par[0], &tm.par[1], &tm.par[2],
+                               &tm.par[3]);
+                    else {
+                        fail(&rd, "Error: unsupported angle function type: %d in line:\n%s\n", f, rd.line);
+                        goto done;
+                    }
                     excl_pair(&mt[k], ai, aj); excl_pair(&mt[k], ai, ak); excl_pair(&mt[k], aj, ak);
-                    mt[k].n_bonded_terms++;
-                } else if (s == 3) {        /* dihedrals exclude all six pairs */
-                    sscanf(rd.line, "%d%d%d%d", &ai, &aj, &ak, &al);
+                    tm.a[0] = ai; tm.a[1] = aj; tm.a[2] = ak; tm.funct = f;
+                    mt[k].term[2][mt[k].n_term[2]++] = tm;
+                } else if (s == 3) {        /* dihedrals exclude all six pairs (file.c:865-925) */
+                    sscanf(rd.line, "%d%d%d%d%d", &ai, &aj, &ak, &al, &f);
+                    if (f == 3) {
+                        sscanf(rd.line, "%*d%*d%*d%*d%*d%lf%lf%lf%lf%lf%lf", &tm.par[0], &tm.par[1],
+                               &tm.par[2], &tm.par[3], &tm.par[4], &tm.par[5]);
+                    } else if (f == 1 || f == 4 || f == 9) {
+                        double phi0 = 0.0, kphi = 0.0;
+                        int nn = 0;
+                        sscanf(rd.line, "%*d%*d%*d%*d%*d%lf%lf%d", &phi0, &kphi, &nn);
+                        int bad = periodic_to_rb(phi0, kphi, nn, tm.par);
+                        if (bad) {
+                            fail(&rd, bad == -2 ? "<> Error: n is not equal to 1, 2, 3 or 4!\n"
+                                                  "    phi0=%f, kphi=%f, n=%d\n"
+                                                : "<> Error: phi0 is not equal to 0 or pi!\n"
+                                                  "    phi0=%f, kphi=%f, n=%d\n",
+                                 phi0, kphi, nn);
+                            goto done;
+                        }
+                    } else {
+                        fail(&rd, "Error: unsupported dihedral function type: %d in line:\n%s\n", f,
+                             rd.line);
+                        goto done;
+                    }
                     excl_pair(&mt[k], ai, aj); excl_pair(&mt[k], ai, ak); excl_pair(&mt[k], ai, al);
                     excl_pair(&mt[k], aj, ak); excl_pair(&mt[k], aj, al); excl_pair(&mt[k], ak, al);
-                    mt[k].n_bonded_terms++;
+                    tm.a[0] = ai; tm.a[1] = aj; tm.a[2] = ak; tm.a[3] = al; tm.funct = 3;
+                    mt[k].term[3][mt[k].n_term[3]++] = tm;
                 } else if (s == 4) {        /* TIP5P-type virtual site s excluded from i, j, k */
                     int as = 0;
-                    sscanf(rd.line, "%d%d%d%d", &ai, &aj, &ak, &as);
+                    sscanf(rd.line, "%d%d%d%d%d", &ai, &aj, &ak, &as, &f);
+                    if (f != 4) {
+                        fail(&rd, "Error: unsupported virtual_sites function type: %d in line:\n%s\n",
+                             f, rd.line);
+                        goto done;
+                    }
+                    sscanf(rd.line, "%*d%*d%*d%*d%*d%lf%lf%lf", &tm.par[0], &tm.par[1], &tm.par[2]);
                     excl_pair(&mt[k], as, ai); excl_pair(&mt[k], as, aj); excl_pair(&mt[k], as, ak);
+                    tm.a[0] = ai; tm.a[1] = aj; tm.a[2] = ak; tm.a[3] = as; tm.funct = 4;
+                    mt[k].term[4][mt[k].n_term[4]++] = tm;
                     mt[k].n_vsites++;
                 } else {                    /* constraints i j r (file.c:977-989) */
                     double r = 0;
@@ -324,13 +419,13 @@ int capac_deck_load(const char* gro, const char* par, const char* mds, capac_dec
     if (rc) goto fail_free;
 
     /* atoms and molecules in param order (file.c:1031-1099) */
-    int N = 0, M = 0, C = 0, nvs = 0, nbonded = 0;
+    int N = 0, M = 0, C = 0, nvs = 0, nterm[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k < d->mol_types; ++k) {
         N += d->mol_count[k] * mt[k].n_atoms;
         M += d->mol_count[k];
         C += d->mol_count[k] * mt[k].n_cstr;
         nvs += d->mol_count[k] * mt[k].n_vsites;
-        nbonded += d->mol_count[k] * mt[k].n_bonded_terms;
+        for (int t = 0; t < 5; ++t) nterm[t] += d->mol_count[k] * mt[k].n_term[t];
     }
     p->n_atoms = N; p->n_mols = M; p->n_cstr = C;
     d->n_vsites = nvs;
@@ -401,10 +496,80 @@ int capac_deck_load(const char* gro, const char* par, const char* mds, capac_dec
                      k, d->np_first[k], d->np_last[k]);
             rc = -1; goto fail_free;
         }
-    if (nbonded > 0 || nvs > 0) {
-        snprintf(err, errlen, "Error: bonded terms and virtual sites are not supported by this "
-                 "build (rigid molecules and metal only)\n");
-        rc = CAPAC_ERR_UNSUPPORTED; goto fail_free;
+    if (nterm[0] + nterm[1] + nterm[2] + nterm[3] + nterm[4] > 0) {
+        /* expand the molecule types' terms to every molecule (mpi_bonded
+           and build_vsites walk molecules in order, force.c:1582-1904) */
+        d->mol_terms = xcalloc(5 * ((size_t)M + 1), sizeof(int));
+        d->bond_atoms = xcalloc(2 * (size_t)nterm[0], sizeof(int));
+        d->bond_funct = xcalloc(nterm[0], sizeof(int));
+        d->bond_par = xcalloc(3 * (size_t)nterm[0], sizeof(double));
+        d->pair14_atoms = xcalloc(2 * (size_t)nterm[1], sizeof(int));
+        d->pair14_par = xcalloc(6 * (size_t)nterm[1], sizeof(double));
+        d->angle_atoms = xcalloc(3 * (size_t)nterm[2], sizeof(int));
+        d->angle_funct = xcalloc(nterm[2], sizeof(int));
+        d->angle_par = xcalloc(4 * (size_t)nterm[2], sizeof(double));
+        d->dihedral_atoms = xcalloc(4 * (size_t)nterm[3], sizeof(int));
+        d->dihedral_par = xcalloc(6 * (size_t)nterm[3], sizeof(double));
+        d->vsite_atoms = xcalloc(4 * (size_t)nterm[4], sizeof(int));
+        d->vsite_par = xcalloc(3 * (size_t)nterm[4], sizeof(double));
+        int at[5] = {0, 0, 0, 0, 0}, im = 0;
+        const int nt = p->n_types;
+        for (int k = 0; k < d->mol_types; ++k)
+            for (int c = 0; c < d->mol_count[k]; ++c, ++im) {
+                const int f0 = d->mol_first[im];
+                for (int t = 0; t < 5; ++t) d->mol_terms[t * ((size_t)M + 1) + im] = at[t];
+                for (int q = 0; q < mt[k].n_term[0]; ++q, ++at[0]) {
+                    const term_t* tm = &mt[k].term[0][q];
+                    d->bond_atoms[2 * at[0]] = f0 + tm->a[0];
+                    d->bond_atoms[2 * at[0] + 1] = f0 + tm->a[1];
+                    d->bond_funct[at[0]] = tm->funct;
+                    for (int u = 0; u < 3; ++u) d->bond_par[3 * at[0] + u] = tm->par[u];
+                }
+                for (int q = 0; q < mt[k].n_term[1]; ++q, ++at[1]) {
+                    const term_t* tm = &mt[k].term[1][q];
+                    const int i = f0 + tm->a[0], j = f0 + tm->a[1];
+                    double* pp = d->pair14_par + 6 * at[1];
+                    d->pair14_atoms[2 * at[1]] = i;
+                    d->pair14_atoms[2 * at[1] + 1] = j;
+                    if (tm->funct == 1) {   /* force.c:1752-1765 */
+                        const int ij = d->atomtype[i] * nt + d->atomtype[j];
+                        pp[0] = d->nb_c6[ij]; pp[1] = d->nb_c12[ij];
+                        pp[2] = d->charge[i]; pp[3] = d->charge[j];
+                        pp[4] = d->scale_lj; pp[5] = d->scale_qq;
+                    } else {                /* user-defined pair (force.c:1768-1782) */
+                        const double sig_6 = pow(tm->par[3], 6.0);
+                        const double eps_4 = 4.0 * tm->par[4];
+                        pp[0] = eps_4 * sig_6; pp[1] = pp[0] * sig_6;
+                        pp[2] = tm->par[1]; pp[3] = tm->par[2];
+                        pp[4] = 1.0; pp[5] = tm->par[0];
+                    }
+                }
+                for (int q = 0; q < mt[k].n_term[2]; ++q, ++at[2]) {
+                    const term_t* tm = &mt[k].term[2][q];
+                    for (int u = 0; u < 3; ++u) d->angle_atoms[3 * at[2] + u] = f0 + tm->a[u];
+                    d->angle_funct[at[2]] = tm->funct;
+                    for (int u = 0; u < 4; ++u) d->angle_par[4 * at[2] + u] = tm->par[u];
+                }
+                for (int q = 0; q < mt[k].n_term[3]; ++q, ++at[3]) {
+                    const term_t* tm = &mt[k].term[3][q];
+                    for (int u = 0; u < 4; ++u) d->dihedral_atoms[4 * at[3] + u] = f0 + tm->a[u];
+                    for (int u = 0; u < 6; ++u) d->dihedral_par[6 * at[3] + u] = tm->par[u];
+                }
+                for (int q = 0; q < mt[k].n_term[4]; ++q, ++at[4]) {
+                    const term_t* tm = &mt[k].term[4][q];
+                    for (int u = 0; u < 4; ++u) d->vsite_atoms[4 * at[4] + u] = f0 + tm->a[u];
+                    for (int u = 0; u < 3; ++u) d->vsite_par[3 * at[4] + u] = tm->par[u];
+                }
+            }
+        for (int t = 0; t < 5; ++t) d->mol_terms[t * ((size_t)M + 1) + M] = at[t];
+        p->n_bonds = nterm[0]; p->n_pairs14 = nterm[1]; p->n_angles = nterm[2];
+        p->n_dihedrals = nterm[3]; p->n_vsites = nterm[4];
+        p->mol_terms = d->mol_terms;
+        p->bond_atoms = d->bond_atoms; p->bond_funct = d->bond_funct; p->bond_par = d->bond_par;
+        p->pair14_atoms = d->pair14_atoms; p->pair14_par = d->pair14_par;
+        p->angle_atoms = d->angle_atoms; p->angle_funct = d->angle_funct; p->angle_par = d->angle_par;
+        p->dihedral_atoms = d->dihedral_atoms; p->dihedral_par = d->dihedral_par;
+        p->vsite_atoms = d->vsite_atoms; p->vsite_par = d->vsite_par;
     }
     p->ndf = 3 * (N - nvs) - 3 - C;            /* mpi-main.c:548 */
     rc = read_gro(d, gro, err, errlen);
@@ -417,17 +582,11 @@ int capac_deck_load(const char* gro, const char* par, const char* mds, capac_dec
     p->np_first = d->np_first; p->np_last = d->np_last; p->np_charge = d->np_charge;
     p->nb_funct = d->nb_funct; p->nb_c6 = d->nb_c6; p->nb_c12 = d->nb_c12;
     p->nb_a = d->nb_a; p->nb_b = d->nb_b;
-    for (int k = 0; mt && k < d->mol_types; ++k) {
-        free(mt[k].q); free(mt[k].m); free(mt[k].type); free(mt[k].cstr);
-    }
-    free(mt);
+    moltype_free(mt, d->mol_types);
     *out = d;
     return 0;
 fail_free:
-    for (int k = 0; mt && k < d->mol_types; ++k) {
-        free(mt[k].q); free(mt[k].m); free(mt[k].type); free(mt[k].cstr);
-    }
-    free(mt);
+    moltype_free(mt, d ? d->mol_types : 0);
     capac_deck_free(d);
     return rc ? rc : -1;
 }
@@ -441,6 +600,10 @@ void capac_deck_free(capac_deck* d) {
     free(d->np_first); free(d->np_last); free(d->np_charge);
     free(d->nb_funct); free(d->nb_c6); free(d->nb_c12); free(d->nb_a); free(d->nb_b);
     free(d->r); free(d->v); free(d->res_id); free(d->res_name); free(d->atom_name);
+    free(d->mol_terms); free(d->bond_atoms); free(d->bond_funct); free(d->bond_par);
+    free(d->pair14_atoms); free(d->pair14_par); free(d->angle_atoms); free(d->angle_funct);
+    free(d->angle_par); free(d->dihedral_atoms); free(d->dihedral_par); free(d->vsite_atoms);
+    free(d->vsite_par);
     free(d);
 }
 

@@ -49,6 +49,13 @@ class CapacParams(C.Structure):
         ("dt", C.c_double), ("fix_metal", C.c_int), ("nvt", C.c_int),
         ("ref_temp", C.c_double), ("tau_t", C.c_double), ("n_heating", C.c_int),
         ("ndf", C.c_int),
+        ("n_bonds", C.c_int), ("n_pairs14", C.c_int), ("n_angles", C.c_int),
+        ("n_dihedrals", C.c_int), ("n_vsites", C.c_int),
+        ("mol_terms", _ip), ("bond_atoms", _ip), ("bond_funct", _ip), ("bond_par", _dp),
+        ("pair14_atoms", _ip), ("pair14_par", _dp),
+        ("angle_atoms", _ip), ("angle_funct", _ip), ("angle_par", _dp),
+        ("dihedral_atoms", _ip), ("dihedral_par", _dp),
+        ("vsite_atoms", _ip), ("vsite_par", _dp),
     ]
 
 
@@ -174,6 +181,7 @@ GPU_EXPORTS = [
     "capac_em_begin", "capac_em_step", "capac_em_direction",
     "capac_checkpoint_bytes", "capac_checkpoint_save", "capac_checkpoint_load",
     "capac_set_predictor", "capac_predictor",
+    "capac_bonded", "capac_vsite_build", "capac_vsite_spread",
 ]
 NTHETA = 6
 
@@ -199,6 +207,9 @@ def gpu_lib() -> C.CDLL:
         lib.capac_force.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.capac_neighbours.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
         lib.capac_nonbonded.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_bonded.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.capac_vsite_build.argtypes = [C.c_void_p, _dp]
+        lib.capac_vsite_spread.argtypes = [C.c_void_p, _dp, _dp]
         lib.capac_sutton_chen.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         lib.capac_cpim_rhs.argtypes = [C.c_void_p, _dp]
         lib.capac_cpim_assemble.argtypes = [C.c_void_p, _dp, _dp]
@@ -399,6 +410,25 @@ class Context:
         f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
         _check(self.lib.capac_nonbonded(self.h, dptr(f), dptr(pot), dptr(vir)), "capac_nonbonded")
         return f, pot, vir
+
+    def bonded(self):
+        """mpi_bonded at the resident positions: f, pot (2, 3, 4, 7, 8), vir"""
+        f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
+        _check(self.lib.capac_bonded(self.h, dptr(f), dptr(pot), dptr(vir)), "capac_bonded")
+        return f, pot, vir
+
+    def vsite_build(self):
+        """build_vsites on the resident positions; returns them (3N)"""
+        r = np.empty(3 * self.N)
+        _check(self.lib.capac_vsite_build(self.h, dptr(r)), "capac_vsite_build")
+        return r
+
+    def vsite_spread(self, f):
+        """mpi_vsites: the forces on virtual sites moved to their atoms"""
+        f_in = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.empty(3 * self.N)
+        _check(self.lib.capac_vsite_spread(self.h, dptr(f_in), dptr(out)), "capac_vsite_spread")
+        return out
 
     def sutton_chen(self):
         f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)

@@ -102,7 +102,10 @@ typedef struct capac_params {
     const int*      np_last;
     const double*   np_charge;
     /* pair table, n_types x n_types row-major, symmetric (file.c:618-695);
-       funct 1 = LJ (C6, C12), 4 = erf-damped vdW (C6, C12, width in nb_a) */
+       funct 1 = LJ (C6, C12) + Coulomb, 2 = Buckingham (A, B in nb_a, nb_b;
+       C6), 3 = Morse (D, a in nb_a, nb_b; R in nb_c6), 4 = erf-damped vdW
+       (C6, C12, width in nb_a); funct 2-4 carry no Coulomb term
+       (force.c:1476-1525) */
     const int*      nb_funct;
     const double*   nb_c6;
     const double*   nb_c12;
@@ -129,7 +132,39 @@ typedef struct capac_params {
     double tau_t;               /* ps                                        */
     int    n_heating;
     int    ndf;                 /* 3(N - n_vsites) - 3 - n_cstr (mpi-main.c:548) */
+
+    /* Bonded terms and virtual sites (mpi_bonded force.c:1694-1904,
+     * build_vsites / mpi_vsites force.c:1582-1692), expanded to every
+     * molecule with absolute atom indices; within a molecule each list keeps
+     * param.txt order.  mol_terms holds five ranges of n_mols + 1 entries,
+     * [CAPAC_TERM_BOND .. CAPAC_TERM_VSITE], term t of molecule m running
+     * over [mol_terms[t (n_mols + 1) + m], mol_terms[t (n_mols + 1) + m + 1]).
+     * All counts 0 and pointers NULL for rigid molecules and metal only. */
+    int n_bonds, n_pairs14, n_angles, n_dihedrals, n_vsites;
+    const int*    mol_terms;
+    const int*    bond_atoms;     /* i, j                                         */
+    const int*    bond_funct;     /* 1 harmonic, 3 Morse (funct 5 bonds only exclude) */
+    const double* bond_par;       /* b0, kb (funct 1) or b0, D, beta (funct 3)    */
+    const int*    pair14_atoms;   /* i, j                                         */
+    const double* pair14_par;     /* c6, c12, q_i, q_j, scale_lj, scale_qq: funct 1
+                                     takes the pair table and the SCALE_LJ_QQ
+                                     line, funct 2 its own line (force.c:1752-1790) */
+    const int*    angle_atoms;    /* i, j, k                                      */
+    const int*    angle_funct;    /* 1 harmonic, 5 Urey-Bradley                   */
+    const double* angle_par;      /* a0 (degrees), ka, b0_ub, kb_ub               */
+    const int*    dihedral_atoms; /* i, j, k, l                                   */
+    const double* dihedral_par;   /* Ryckaert-Bellemans c0..c5; periodic dihedrals
+                                     (funct 1, 4, 9) arrive converted as
+                                     compute_dihedral_9 converts them (force.c:1236-1347) */
+    const int*    vsite_atoms;    /* i, j, k, s (the site)                        */
+    const double* vsite_par;      /* a, b, c of the TIP5P-type construction       */
 } capac_params;
+
+#define CAPAC_TERM_BOND     0
+#define CAPAC_TERM_PAIR14   1
+#define CAPAC_TERM_ANGLE    2
+#define CAPAC_TERM_DIHEDRAL 3
+#define CAPAC_TERM_VSITE    4
 
 /* per-phase wall times in seconds, the reference's timer categories
  * (mpi-main.c:119-128): [1] SC density [2] SC force [4] nonbonded
@@ -262,6 +297,19 @@ int capac_neighbours(capac_ctx* ctx, int* count, int* list, int max_per_atom);
 /* mpi_nonb_pair (force.c:1402-1540): f, pot[7], pot[8], vir (each nullable;
  * values are written, not accumulated) */
 int capac_nonbonded(capac_ctx* ctx, double* f, double pot[CAPAC_NPOT], double vir[9]);
+
+/* mpi_bonded (force.c:1694-1904) at the resident positions: bonds, 1-4
+ * pairs, angles and dihedrals.  f, pot[2], pot[3], pot[4], and the 1-4
+ * pairs' pot[7], pot[8], vir (each nullable; written, not accumulated) */
+int capac_bonded(capac_ctx* ctx, double* f, double pot[CAPAC_NPOT], double vir[9]);
+
+/* build_vsites (force.c:1582-1629) on the resident positions, which it
+ * updates; r (3N, nullable) receives them */
+int capac_vsite_build(capac_ctx* ctx, double* r);
+
+/* mpi_vsites (force.c:1636-1692): moves the forces f_in (3N) on the virtual
+ * sites to their constructing atoms at the resident positions; f_out (3N) */
+int capac_vsite_spread(capac_ctx* ctx, const double* f_in, double* f_out);
 
 /* mpi_qsc_dens + mpi_qsc_force (force.c:385-542) */
 int capac_sutton_chen(capac_ctx* ctx, double* f, double pot[CAPAC_NPOT], double vir[9],

@@ -58,6 +58,10 @@ typedef struct capac_deck {
     unsigned *excl_mask;
     double *nb_c6, *nb_c12, *nb_a, *nb_b;
     int    metal_min, metal_max;
+    /* bonded terms and virtual sites, expanded (see capac_params) */
+    int    *mol_terms, *bond_atoms, *bond_funct, *pair14_atoms, *angle_atoms, *angle_funct;
+    int    *dihedral_atoms, *vsite_atoms;
+    double *bond_par, *pair14_par, *angle_par, *dihedral_par, *vsite_par;
 } capac_deck;
 
 /* Reads the three input files.  On failure returns < 0 and writes the

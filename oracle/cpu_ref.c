@@ -255,6 +255,27 @@ static inline double pair_erf_vdw(double rij2, double c6, double c12, double wid
     return fij;
 }
 
+/* compute_morse (force.c:639-668) */
+static inline double pair_morse(double rij2, double D, double a, double R, double* pot) {
+    double rij = sqrt(rij2);
+    double e = exp(-a * (rij - R));
+    double e2 = e * e;
+    double fij = 2.0 * D * a * (e2 - e) / rij;
+    pot[8] += D * (e2 - 2.0 * e);
+    return fij;
+}
+
+/* compute_buckingham (force.c:674-703) */
+static inline double pair_buckingham(double rij2, double A, double B, double c6, double* pot) {
+    double rij = sqrt(rij2);
+    double rij6 = rij2 * rij2 * rij2;
+    double rij8 = rij6 * rij2;
+    double e = exp(-B * rij);
+    double fij = A * e * B / rij - 6.0 * c6 / rij8;
+    pot[8] += (A * e - c6 / rij6);
+    return fij;
+}
+
 /* one accepted pair of mpi_nonb_pair, forces and virial (force.c:1476-1537) */
 static inline void nonb_one(const oref* o, int i, int j, double rx, double ry, double rz,
                             double rij2, double* f, double* pot, double* vir) {
@@ -265,6 +286,10 @@ static inline void nonb_one(const oref* o, int i, int j, double rx, double ry, d
     double flj = 0.0, fqq = 0.0;
     if (funct == 4) {
         flj = pair_erf_vdw(rij2, p->nb_c6[ij], p->nb_c12[ij], p->nb_a[ij], pot);
+    } else if (funct == 3) {
+        flj = pair_morse(rij2, p->nb_a[ij], p->nb_b[ij], p->nb_c6[ij], pot);
+    } else if (funct == 2) {
+        flj = pair_buckingham(rij2, p->nb_a[ij], p->nb_b[ij], p->nb_c6[ij], pot);
     } else if (funct == 1) {
         pair_lj_dsf(o, rij2, p->nb_c6[ij], p->nb_c12[ij], p->charge[i], p->charge[j], &flj, &fqq, pot);
     }
@@ -337,6 +362,238 @@ void oref_nonb(oref* o, const double* r, double* f, double* pot, double* vir) {
     }
 #endif
     o->timers[4] += now() - t0;
+}
+
+/* ------------------------------------------------------------------ bonded */
+
+typedef struct { double x, y, z; } ovec;
+
+static inline ovec ov_pbc(const oref* o, double dx, double dy, double dz) {
+    ovec v = {pbc1(dx, o->box[0]), pbc1(dy, o->box[1]), pbc1(dz, o->box[2])};
+    return v;
+}
+static inline double ov_dot(ovec a, ovec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline ovec ov_scale(double s, ovec v) { ovec r = {s * v.x, s * v.y, s * v.z}; return r; }
+static inline ovec ov_cross(ovec a, ovec b) {
+    ovec r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+/* virial_rvec_fvec_full (force.c:84-100) */
+static inline void ov_virial(ovec r, ovec f, double* vir) {
+    vir[0] -= r.x * f.x; vir[1] -= r.x * f.y; vir[2] -= r.x * f.z;
+    vir[3] -= r.y * f.x; vir[4] -= r.y * f.y; vir[5] -= r.y * f.z;
+    vir[6] -= r.z * f.x; vir[7] -= r.z * f.y; vir[8] -= r.z * f.z;
+}
+static inline void ov_add(double* f, int N, int i, ovec v) {
+    f[i] += v.x; f[N + i] += v.y; f[2 * N + i] += v.z;
+}
+
+/* compute_bond_1 / compute_bond_3 (force.c:822-909): returns fj, fi = -fj */
+static ovec bond_term(const oref* o, const double* r, int i, int j, int funct, const double* bp,
+                      double* pot, double* vir) {
+    int N = o->N;
+    ovec d = ov_pbc(o, r[j] - r[i], r[N + j] - r[N + i], r[2 * N + j] - r[2 * N + i]);
+    double rij = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    double fij;
+    if (funct == 1) {
+        double b0 = bp[0], kb = bp[1];
+        fij = kb * (b0 - rij) / rij;
+        pot[2] += 0.5 * kb * (b0 - rij) * (b0 - rij);
+    } else {
+        double b0 = bp[0], D = bp[1], beta = bp[2];
+        double e = exp(-beta * (rij - b0));
+        fij = -2.0 * D * beta * e * (1.0 - e) / rij;
+        pot[2] += D * pow(1.0 - e, 2.0);
+    }
+    ovec fj = {fij * d.x, fij * d.y, fij * d.z};
+    vir[0] += -d.x * fj.x; vir[1] += -d.x * fj.y; vir[2] += -d.x * fj.z;
+    vir[3] += -d.y * fj.x; vir[4] += -d.y * fj.y; vir[5] += -d.y * fj.z;
+    vir[6] += -d.z * fj.x; vir[7] += -d.z * fj.y; vir[8] += -d.z * fj.z;
+    return fj;
+}
+
+/* mpi_bonded (force.c:1694-1904) over all molecules, one rank */
+void oref_bonded(oref* o, const double* r, double* f, double* pot, double* vir) {
+    const capac_params* p = &o->p;
+    int N = o->N, M = p->n_mols, M1 = M + 1;
+    if (!p->mol_terms) return;
+    const int* rg = p->mol_terms;
+    for (int m = 0; m < M; ++m) {
+        /* bonds */
+        for (int t = rg[m]; t < rg[m + 1]; ++t) {
+            int i = p->bond_atoms[2 * t], j = p->bond_atoms[2 * t + 1];
+            ovec fj = bond_term(o, r, i, j, p->bond_funct[t], p->bond_par + 3 * t, pot, vir);
+            f[i] += -fj.x; f[N + i] += -fj.y; f[2 * N + i] += -fj.z;
+            ov_add(f, N, j, fj);
+        }
+        /* 1-4 pairs: compute_nonbonded with is_14_pair = 1 (force.c:710-816) */
+        for (int t = rg[M1 + m]; t < rg[M1 + m + 1]; ++t) {
+            int i = p->pair14_atoms[2 * t], j = p->pair14_atoms[2 * t + 1];
+            const double* pp = p->pair14_par + 6 * t;
+            double c6 = pp[0], c12 = pp[1], qi = pp[2], qj = pp[3], sLJ = pp[4], sQQ = pp[5];
+            double rx = pbc1(r[j] - r[i], o->box[0]);
+            double ry = pbc1(r[N + j] - r[N + i], o->box[1]);
+            double rz = pbc1(r[2 * N + j] - r[2 * N + i], o->box[2]);
+            double rij2 = rx * rx + ry * ry + rz * rz;
+            double rij = sqrt(rij2);
+            ovec lj = {0.0, 0.0, 0.0}, qq = {0.0, 0.0, 0.0};
+            if (c6 != 0.0 || c12 != 0.0) {
+                double rij6 = rij2 * rij2 * rij2;
+                double rij12 = rij6 * rij6;
+                double fij = (c12 * 2.0 / rij12 - c6 / rij6) * 6.0 / rij2 * sLJ;
+                pot[8] += (c12 / rij12 - c6 / rij6) * sLJ;
+                lj.x = fij * rx; lj.y = fij * ry; lj.z = fij * rz;
+            }
+            if (qi != 0.0 && qj != 0.0) {
+                double fij = sQQ * R_FQQ * qi * qj / (rij2 * rij);
+                pot[7] += sQQ * R_FQQ * qi * qj / rij;
+                qq.x = fij * rx; qq.y = fij * ry; qq.z = fij * rz;
+            }
+            double fx = lj.x + qq.x, fy = lj.y + qq.y, fz = lj.z + qq.z;
+            vir[0] -= rx * fx; vir[1] -= rx * fy; vir[2] -= rx * fz;
+            vir[3] -= ry * fx; vir[4] -= ry * fy; vir[5] -= ry * fz;
+            vir[6] -= rz * fx; vir[7] -= rz * fy; vir[8] -= rz * fz;
+            f[i] += -lj.x + -qq.x; f[N + i] += -lj.y + -qq.y; f[2 * N + i] += -lj.z + -qq.z;
+            f[j] += lj.x + qq.x; f[N + j] += lj.y + qq.y; f[2 * N + j] += lj.z + qq.z;
+        }
+        /* angles: compute_angle_1 / compute_angle_5 (force.c:956-1052) */
+        for (int t = rg[2 * M1 + m]; t < rg[2 * M1 + m + 1]; ++t) {
+            int i = p->angle_atoms[3 * t], j = p->angle_atoms[3 * t + 1], k = p->angle_atoms[3 * t + 2];
+            const double* ap = p->angle_par + 4 * t;
+            double a0 = ap[0], ka = ap[1];
+            ovec r21 = ov_pbc(o, r[i] - r[j], r[N + i] - r[N + j], r[2 * N + i] - r[2 * N + j]);
+            double r21_d = sqrt(ov_dot(r21, r21));
+            ovec u21 = ov_scale(1.0 / r21_d, r21);
+            ovec r32 = ov_pbc(o, r[j] - r[k], r[N + j] - r[N + k], r[2 * N + j] - r[2 * N + k]);
+            double r32_d = sqrt(ov_dot(r32, r32));
+            ovec u32 = ov_scale(1.0 / r32_d, r32);
+            double cos123 = -ov_dot(u21, u32);
+            double theta123 = acos(cos123);
+            double sin123 = sin(theta123);
+            double delta_a = a0 * R_PI / 180.0 - theta123;
+            double fij = ka * delta_a;
+            ovec fi, fk, fj;
+            fi.x = (cos123 * u21.x + u32.x) / (sin123 * r21_d) * fij;
+            fi.y = (cos123 * u21.y + u32.y) / (sin123 * r21_d) * fij;
+            fi.z = (cos123 * u21.z + u32.z) / (sin123 * r21_d) * fij;
+            fk.x = -(cos123 * u32.x + u21.x) / (sin123 * r32_d) * fij;
+            fk.y = -(cos123 * u32.y + u21.y) / (sin123 * r32_d) * fij;
+            fk.z = -(cos123 * u32.z + u21.z) / (sin123 * r32_d) * fij;
+            fj.x = -fi.x - fk.x; fj.y = -fi.y - fk.y; fj.z = -fi.z - fk.z;
+            pot[3] += 0.5 * ka * delta_a * delta_a;
+            ov_virial(r21, fi, vir);
+            ovec m32 = {-r32.x, -r32.y, -r32.z};
+            ov_virial(m32, fk, vir);
+            if (p->angle_funct[t] == 5 && ap[3] > 0.0) {
+                ovec bj = bond_term(o, r, i, k, 1, ap + 2, pot, vir);
+                fi.x += -bj.x; fi.y += -bj.y; fi.z += -bj.z;
+                fk.x += bj.x; fk.y += bj.y; fk.z += bj.z;
+            }
+            ov_add(f, N, i, fi);
+            ov_add(f, N, j, fj);
+            ov_add(f, N, k, fk);
+        }
+        /* dihedrals: compute_dihedral_3 (force.c:1100-1230) */
+        for (int t = rg[3 * M1 + m]; t < rg[3 * M1 + m + 1]; ++t) {
+            const int* q = p->dihedral_atoms + 4 * t;
+            const double* c = p->dihedral_par + 6 * t;
+            int i = q[0], j = q[1], k = q[2], l = q[3];
+            ovec r21 = ov_pbc(o, r[i] - r[j], r[N + i] - r[N + j], r[2 * N + i] - r[2 * N + j]);
+            double r21_d = sqrt(ov_dot(r21, r21));
+            ovec u21 = ov_scale(1.0 / r21_d, r21);
+            ovec r32 = ov_pbc(o, r[j] - r[k], r[N + j] - r[N + k], r[2 * N + j] - r[2 * N + k]);
+            double r32_d = sqrt(ov_dot(r32, r32));
+            ovec u32 = ov_scale(1.0 / r32_d, r32);
+            ovec r43 = ov_pbc(o, r[k] - r[l], r[N + k] - r[N + l], r[2 * N + k] - r[2 * N + l]);
+            double r43_d = sqrt(ov_dot(r43, r43));
+            ovec u43 = ov_scale(1.0 / r43_d, r43);
+            double cos123 = -ov_dot(u21, u32);
+            double theta123 = acos(cos123);
+            double sin123 = sin(theta123);
+            double cos234 = -ov_dot(u32, u43);
+            double theta234 = acos(cos234);
+            double sin234 = sin(theta234);
+            double cos_phi = (cos123 * cos234 - ov_dot(u21, u43)) / (sin123 * sin234);
+            pot[4] += (c[0] - c[1] * cos_phi + c[2] * pow(cos_phi, 2.0) - c[3] * pow(cos_phi, 3.0) +
+                       c[4] * pow(cos_phi, 4.0) - c[5] * pow(cos_phi, 5.0));
+            double fij = -(-c[1] + c[2] * 2.0 * cos_phi - c[3] * 3.0 * pow(cos_phi, 2.0) +
+                           c[4] * 4.0 * pow(cos_phi, 3.0) - c[5] * 5.0 * pow(cos_phi, 4.0));
+            ovec a = ov_cross(u21, u32), b = ov_cross(u43, u32);
+            double const_1 = ov_dot(u43, a) / (r21_d * pow(sin123, 3.0) * sin234);
+            double const_4 = ov_dot(u43, a) / (r43_d * pow(sin234, 3.0) * sin123);
+            double c123 = r21_d * cos123 / r32_d - 1.0;
+            double b432 = r43_d * cos234 / r32_d;
+            ovec fi = {-const_1 * a.x * fij, -const_1 * a.y * fij, -const_1 * a.z * fij};
+            ovec fl = {-const_4 * b.x * fij, -const_4 * b.y * fij, -const_4 * b.z * fij};
+            ovec fj = {c123 * fi.x - b432 * fl.x, c123 * fi.y - b432 * fl.y, c123 * fi.z - b432 * fl.z};
+            ovec fk = {-fi.x - fj.x - fl.x, -fi.y - fj.y - fl.y, -fi.z - fj.z - fl.z};
+            ovec rv = {r[i], r[N + i], r[2 * N + i]};
+            ov_virial(rv, fi, vir);
+            rv.x -= r21.x; rv.y -= r21.y; rv.z -= r21.z;
+            ov_virial(rv, fj, vir);
+            rv.x -= r32.x; rv.y -= r32.y; rv.z -= r32.z;
+            ov_virial(rv, fk, vir);
+            rv.x -= r43.x; rv.y -= r43.y; rv.z -= r43.z;
+            ov_virial(rv, fl, vir);
+            ov_add(f, N, i, fi);
+            ov_add(f, N, j, fj);
+            ov_add(f, N, k, fk);
+            ov_add(f, N, l, fl);
+        }
+    }
+}
+
+/* build_vsites (force.c:1582-1629) */
+void oref_vsite_build(oref* o, double* r) {
+    const capac_params* p = &o->p;
+    int N = o->N, M = p->n_mols;
+    if (!p->mol_terms) return;
+    const int* rg = p->mol_terms + 4 * (M + 1);
+    for (int m = 0; m < M; ++m)
+        for (int t = rg[m]; t < rg[m + 1]; ++t) {
+            const int* q = p->vsite_atoms + 4 * t;
+            int i = q[0], j = q[1], k = q[2], s = q[3];
+            double a = p->vsite_par[3 * t], b = p->vsite_par[3 * t + 1], c = p->vsite_par[3 * t + 2];
+            ovec rij = ov_pbc(o, r[j] - r[i], r[N + j] - r[N + i], r[2 * N + j] - r[2 * N + i]);
+            ovec rik = ov_pbc(o, r[k] - r[i], r[N + k] - r[N + i], r[2 * N + k] - r[2 * N + i]);
+            ovec x = ov_cross(rij, rik);
+            r[s] = r[i] + a * rij.x + b * rik.x + c * x.x;
+            r[N + s] = r[N + i] + a * rij.y + b * rik.y + c * x.y;
+            r[2 * N + s] = r[2 * N + i] + a * rij.z + b * rik.z + c * x.z;
+        }
+}
+
+/* mpi_vsites with compute_vsite_4 (force.c:1060-1090, 1636-1692) */
+void oref_vsite_spread(oref* o, const double* r, double* f) {
+    const capac_params* p = &o->p;
+    int N = o->N, M = p->n_mols;
+    if (!p->mol_terms) return;
+    const int* rg = p->mol_terms + 4 * (M + 1);
+    for (int m = 0; m < M; ++m)
+        for (int t = rg[m]; t < rg[m + 1]; ++t) {
+            const int* q = p->vsite_atoms + 4 * t;
+            int i = q[0], j = q[1], k = q[2], s = q[3];
+            double a = p->vsite_par[3 * t], b = p->vsite_par[3 * t + 1], c = p->vsite_par[3 * t + 2];
+            double xij = pbc1(r[j] - r[i], o->box[0]), yij = pbc1(r[N + j] - r[N + i], o->box[1]);
+            double zij = pbc1(r[2 * N + j] - r[2 * N + i], o->box[2]);
+            double xik = pbc1(r[k] - r[i], o->box[0]), yik = pbc1(r[N + k] - r[N + i], o->box[1]);
+            double zik = pbc1(r[2 * N + k] - r[2 * N + i], o->box[2]);
+            double fxs = f[s], fys = f[N + s], fzs = f[2 * N + s];
+            ovec fj, fk, fi;
+            fj.x = a * fxs - c * zik * fys + c * yik * fzs;
+            fj.y = c * zik * fxs + a * fys - c * xik * fzs;
+            fj.z = -c * yik * fxs + c * xik * fys + a * fzs;
+            fk.x = b * fxs + c * zij * fys - c * yij * fzs;
+            fk.y = -c * zij * fxs + b * fys + c * xij * fzs;
+            fk.z = c * yij * fxs - c * xij * fys + b * fzs;
+            fi.x = fxs - fj.x - fk.x;
+            fi.y = fys - fj.y - fk.y;
+            fi.z = fzs - fj.z - fk.z;
+            ov_add(f, N, i, fi);
+            ov_add(f, N, j, fj);
+            ov_add(f, N, k, fk);
+            f[s] = 0.0; f[N + s] = 0.0; f[2 * N + s] = 0.0;
+        }
 }
 
 /* ------------------------------------------------------------- Sutton-Chen */
@@ -811,9 +1068,11 @@ int oref_force(oref* o) {
     const capac_params* p = &o->p;
     int N = o->N;
     double t0 = now();
+    oref_vsite_build(o, o->rr);
     memset(o->ff, 0, 3 * (size_t)N * sizeof(double));
     memset(o->pot, 0, sizeof o->pot);
     memset(o->vir, 0, sizeof o->vir);
+    oref_bonded(o, o->rr, o->ff, o->pot, o->vir);
     oref_nonb(o, o->rr, o->ff, o->pot, o->vir);
     int rc = 0;
     if (o->Nm > 0) {
@@ -834,6 +1093,7 @@ int oref_force(oref* o) {
         o->pot[9] -= p->efield[0] * qi * o->rr[i] + p->efield[1] * qi * o->rr[N + i] +
                      p->efield[2] * qi * o->rr[2 * N + i];
     }
+    oref_vsite_spread(o, o->rr, o->ff);
     o->timers[0] += now() - t0;
     return rc;
 }

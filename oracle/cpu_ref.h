@@ -35,6 +35,11 @@ long oref_reference_pair_end(int n_atoms);
 /* phases; f/pot/vir accumulate (callers zero them) */
 void oref_nonb(oref* o, const double* r, double* f, double* pot, double* vir);
 void oref_qsc(oref* o, const double* r, double* f, double* pot, double* vir, double* inv_sqrt_dens);
+/* mpi_bonded (force.c:1694-1904): adds to f, pot[2,3,4,7,8], vir */
+void oref_bonded(oref* o, const double* r, double* f, double* pot, double* vir);
+/* build_vsites (force.c:1582-1629) on r; mpi_vsites (force.c:1636-1692) on f */
+void oref_vsite_build(oref* o, double* r);
+void oref_vsite_spread(oref* o, const double* r, double* f);
 void oref_vec_ext(oref* o, const double* r, double* b);
 long oref_relay(oref* o, const double* r);            /* returns nnz */
 void oref_relay_coo(const oref* o, long* row, long* col, double* val, double* diag);

@@ -38,6 +38,9 @@ def lib(omp: bool = False) -> C.CDLL:
         L.oref_reference_pair_end.restype = C.c_long
         L.oref_nonb.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.oref_qsc.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.oref_bonded.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.oref_vsite_build.argtypes = [C.c_void_p, _dp]
+        L.oref_vsite_spread.argtypes = [C.c_void_p, _dp, _dp]
         L.oref_vec_ext.argtypes = [C.c_void_p, _dp, _dp]
         L.oref_relay.argtypes = [C.c_void_p, _dp]
         L.oref_relay.restype = C.c_long
@@ -112,6 +115,24 @@ class Oracle:
         f = np.zeros(3 * self.N); pot = np.zeros(15); vir = np.zeros(9)
         self.L.oref_nonb(self.h, _d(np.ascontiguousarray(r)), _d(f), _d(pot), _d(vir))
         return f, pot, vir
+
+    def bonded(self, r):
+        """mpi_bonded (force.c:1694-1904)"""
+        f = np.zeros(3 * self.N); pot = np.zeros(15); vir = np.zeros(9)
+        self.L.oref_bonded(self.h, _d(np.ascontiguousarray(r)), _d(f), _d(pot), _d(vir))
+        return f, pot, vir
+
+    def vsite_build(self, r):
+        """build_vsites (force.c:1582-1629) on a copy of r"""
+        out = np.array(r, dtype=np.float64)
+        self.L.oref_vsite_build(self.h, _d(out))
+        return out
+
+    def vsite_spread(self, r, f):
+        """mpi_vsites (force.c:1636-1692) on a copy of f"""
+        out = np.array(f, dtype=np.float64)
+        self.L.oref_vsite_spread(self.h, _d(np.ascontiguousarray(r)), _d(out))
+        return out
 
     def qsc(self, r):
         f = np.zeros(3 * self.N); pot = np.zeros(15); vir = np.zeros(9)

@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "typedef.h"
@@ -36,6 +37,14 @@ void mpi_nonb_pair(Topol* p_topol, Task* p_task, Atom_Info* atom_info, RunSet* p
                    System* p_system, int my_id);
 void zero_force_pot_vir(int nAtoms, double* fx, double* fy, double* fz, double* potential,
                         double** virial);
+void mpi_bonded(Task* p_task, Mol_Info* mol_info, Atom_Info* atom_info, Topol* p_topol,
+                RunSet* p_runset, System* p_system, int my_id);
+void build_vsites(int mol_start, int mol_end, Mol_Info* mol_info, Atom_Info* atom_info,
+                  int* nVSites, int** vsite_funct, VSite_4** vsite_4, double* rx, double* ry,
+                  double* rz, double* box);
+void mpi_vsites(int mol_start, int mol_end, Mol_Info* mol_info, Atom_Info* atom_info,
+                int* nVSites, int** vsite_funct, VSite_4** vsite_4, double* rx, double* ry,
+                double* rz, double* box, double* fx, double* fy, double* fz);
 
 static std::string g_out;
 
@@ -224,6 +233,29 @@ static void phases(Ref& R) {
   zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);
   mpi_nonb_pair(R.top, R.task, R.ai, R.rs, s, 0);
   dump_fpv(R, "nonb");
+
+  Topol* t = R.top;
+  int n_terms = 0;
+  for (int mol = 0; mol < t->mol_types; ++mol)
+    n_terms += t->n_bonds[mol] + t->n_pairs[mol] + t->n_angles[mol] + t->n_dihedrals[mol] +
+               t->n_vsites[mol];
+  if (n_terms > 0) {
+    // the nonbonded forces of the positions as read, moved off the virtual sites
+    mpi_vsites(0, t->n_mols - 1, R.mi, R.ai, t->n_vsites, t->vsite_funct, t->vsite_4, s->rx,
+               s->ry, s->rz, s->box, s->fx, s->fy, s->fz);
+    npy_d("vsite_f", s->fx, 3 * (size_t)nA);
+    // bonded terms alone at the positions as read
+    zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);
+    mpi_bonded(R.task, R.mi, R.ai, t, R.rs, s, 0);
+    dump_fpv(R, "bonded");
+    // virtual sites built from the positions as read; the state keeps the
+    // positions as read for the phases below
+    std::vector<double> keep(s->rx, s->rx + 3 * (size_t)nA);
+    build_vsites(0, t->n_mols - 1, R.mi, R.ai, t->n_vsites, t->vsite_funct, t->vsite_4, s->rx,
+                 s->ry, s->rz, s->box);
+    npy_d("vsite_pos", s->rx, 3 * (size_t)nA);
+    std::copy(keep.begin(), keep.end(), s->rx);
+  }
 
   if (m->min >= 0 && m->num > 0) {
     zero_force_pot_vir(nA, s->fx, s->fy, s->fz, s->potential, s->virial);

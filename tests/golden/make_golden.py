@@ -42,6 +42,7 @@ DECKS = {
     "water_only": (2, False, False),
     "au13_em": (None, False, True),
     "2au13_cg": (None, False, True),
+    "topo_au13": (3, False, False),
 }
 
 
@@ -99,12 +100,17 @@ def main() -> int:
     for name, (nsteps, keep_coo, run_prog) in DECKS.items():
         out = os.path.join(HERE, name)
         os.makedirs(out, exist_ok=True)
-        deck = gen.small(name)
-        gro, par, mds = gen.write_deck(deck, out)
+        if name == "topo_au13":         # general topologies (bonded terms, virtual sites)
+            gro, par, mds = gen.write_topology_deck(out)
+            minimiser = False
+        else:
+            deck = gen.small(name)
+            gro, par, mds = gen.write_deck(deck, out)
+            minimiser = deck.run_type != "md"
         if nsteps is not None:
             dump_phases(gro, par, mds, out, nsteps, keep_coo)
         if run_prog:
-            run_program(gro, par, mds, out, deck.run_type != "md")
+            run_program(gro, par, mds, out, minimiser)
         print(name, "->", out)
     return 0
 

@@ -21,7 +21,7 @@ import pytest
 from capacmd_amd import gen, lib
 from oracle.oracle import Oracle
 
-DECKS = ["au13_w", "2au13_field", "au147_vac", "water_only"]
+DECKS = ["au13_w", "2au13_field", "au147_vac", "water_only", "topo_au13"]
 
 
 def load(golden_dir, name):
@@ -41,6 +41,13 @@ def test_oracle_phases_bitwise(name, golden_dir):
     r = deck.r0
     f, pot, vir = o.nonb(r)
     assert same(f, ref["nonb_f"]) and same(pot, ref["nonb_pot"]) and same(vir, ref["nonb_vir"])
+    if "bonded_f" in ref:
+        # general topologies: virtual-site spread of the nonbonded forces,
+        # bonded terms, virtual-site construction (force.c:1582-1904)
+        assert same(o.vsite_spread(r, f), ref["vsite_f"])
+        fb, pb, vb = o.bonded(r)
+        assert same(fb, ref["bonded_f"]) and same(pb, ref["bonded_pot"]) and same(vb, ref["bonded_vir"])
+        assert same(o.vsite_build(r), ref["vsite_pos"])
     if deck.n_metal:
         f, pot, vir, isd = o.qsc(r)
         assert same(isd, ref["sc_inv_sqrt_dens"])
@@ -64,7 +71,7 @@ def test_oracle_phases_bitwise(name, golden_dir):
     assert same(f, ref["force_f"]) and same(pot, ref["force_pot"]) and same(vir, ref["force_vir"])
 
 
-@pytest.mark.parametrize("name", ["au13_w", "2au13_field", "water_only"])
+@pytest.mark.parametrize("name", ["au13_w", "2au13_field", "water_only", "topo_au13"])
 def test_oracle_md_bitwise(name, golden_dir):
     deck, ref = load(golden_dir, name)
     o = Oracle(deck)
@@ -390,3 +397,116 @@ def test_cpim_energy_stationarity(golden_dir):
     lam = x[4 * Nm]
     np.testing.assert_allclose(g[3 * Nm:] - 2 * au1 * mtp, au1 * lam, rtol=1e-6)
     assert np.abs(mtp).max() > 1e-4          # the discrepancy is real, not zero
+
+
+# ------------------------------------------------------- general topologies
+
+def test_bonded_force_is_minus_gradient(golden_dir):
+    """Bonds (harmonic, Morse), 1-4 pairs (both kinds), angles (harmonic,
+    Urey-Bradley) and dihedrals (Ryckaert-Bellemans and periodic): the
+    forces are minus the gradient of pot[2] + pot[3] + pot[4] + pot[7] +
+    pot[8] by central differences.  A dropped term, a wrong sign or a
+    swapped atom in any of them fails here."""
+    deck, _ = load(golden_dir, "topo_au13")
+    o = Oracle(deck)
+    r = deck.r0.copy()
+    f, _, _ = o.bonded(r)
+
+    def energy(rr):
+        _, p, _ = o.bonded(rr)
+        return p[2] + p[3] + p[4] + p[7] + p[8]
+
+    N = deck.n_atoms
+    first = deck.n_metal                      # the first chain molecule
+    for a in range(first, first + 12):        # two chain molecules, all six atoms each
+        for c in range(3):
+            idx = c * N + a
+            g = central_diff(energy, r, idx, h=1e-6)
+            assert abs(-g - f[idx]) < 1e-5 * max(10.0, abs(f[idx])), (a, c, -g, f[idx])
+
+
+def test_bonded_energies_by_hand(golden_dir):
+    """pot[2], pot[3], pot[4] against textbook forms of every term type
+    (harmonic and Morse bonds, Urey-Bradley, harmonic angles, the
+    Ryckaert-Bellemans series and periodic dihedrals), written independently
+    of the reference's force-based expressions."""
+    import math
+    deck, _ = load(golden_dir, "topo_au13")
+    o = Oracle(deck)
+    r = deck.r0.reshape(3, -1)
+    a0 = deck.n_metal
+
+    def dist(i, j):
+        d = r[:, i] - r[:, j]
+        return float(np.sqrt(d @ d))
+
+    def angle(i, j, k):
+        u, v = r[:, i] - r[:, j], r[:, k] - r[:, j]
+        return math.acos(float(u @ v) / math.sqrt(float(u @ u) * float(v @ v)))
+
+    def phi(i, j, k, l):
+        b1, b2, b3 = r[:, j] - r[:, i], r[:, k] - r[:, j], r[:, l] - r[:, k]
+        n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+        return math.atan2(float(np.cross(n1, n2) @ b2) / math.sqrt(float(b2 @ b2)), float(n1 @ n2))
+
+    # closed forms, summed over the eight chain molecules: bonds in file
+    # order (0-1 harmonic, 1-2 Morse, 2-3, 3-4, 0-5 harmonic) plus the
+    # Urey-Bradley 1-3 bond; angles; the RB dihedral in psi = phi - 180
+    # (cos psi = -cos phi) and k (1 + cos(n phi - phi0)) for the periodic ones
+    rb = [9.28, 12.16, -13.12, -3.06, 26.24, -31.5]
+    n_mol = 8
+    tot_b = tot_a = tot_d = 0.0
+    for m in range(n_mol):
+        s_ = a0 + 6 * m
+        eb = 0.5 * 250000.0 * (dist(s_, s_ + 1) - 0.153) ** 2
+        eb += 350.0 * (1.0 - math.exp(-19.0 * (dist(s_ + 1, s_ + 2) - 0.153))) ** 2
+        eb += 0.5 * 250000.0 * (dist(s_ + 2, s_ + 3) - 0.153) ** 2
+        eb += 0.5 * 300000.0 * (dist(s_ + 3, s_ + 4) - 0.109) ** 2
+        eb += 0.5 * 300000.0 * (dist(s_, s_ + 5) - 0.109) ** 2
+        eb += 0.5 * 20000.0 * (dist(s_ + 1, s_ + 3) - 0.252) ** 2
+        ea = 0.5 * 460.0 * (math.radians(111.0) - angle(s_, s_ + 1, s_ + 2)) ** 2
+        ea += 0.5 * 400.0 * (math.radians(112.0) - angle(s_ + 1, s_ + 2, s_ + 3)) ** 2
+        ea += 0.5 * 300.0 * (math.radians(109.5) - angle(s_ + 2, s_ + 3, s_ + 4)) ** 2
+        ea += 0.5 * 300.0 * (math.radians(109.5) - angle(s_ + 5, s_, s_ + 1)) ** 2
+        p = phi(s_, s_ + 1, s_ + 2, s_ + 3)
+        ed = sum(c * (-math.cos(p)) ** n for n, c in enumerate(rb))
+        ed += 2.0 * (1.0 + math.cos(3 * phi(s_ + 1, s_ + 2, s_ + 3, s_ + 4)))
+        ed += 1.5 * (1.0 + math.cos(2 * phi(s_ + 5, s_, s_ + 1, s_ + 2) - math.pi))
+        ed += 0.5 * (1.0 + math.cos(4 * p))
+        ed += 0.8 * (1.0 + math.cos(phi(s_ + 4, s_ + 3, s_ + 2, s_ + 1) - math.pi))
+        ed += 0.6 * (1.0 + math.cos(4 * phi(s_ + 5, s_, s_ + 1, s_ + 2) - math.pi))
+        tot_b += eb; tot_a += ea; tot_d += ed
+    _, pot, _ = o.bonded(deck.r0)
+    assert abs(pot[2] - tot_b) < 1e-9 * max(1.0, abs(tot_b)), (pot[2], tot_b)
+    assert abs(pot[3] - tot_a) < 1e-9 * max(1.0, abs(tot_a)), (pot[3], tot_a)
+    assert abs(pot[4] - tot_d) < 1e-9 * max(1.0, abs(tot_d)), (pot[4], tot_d)
+
+
+def test_vsite_spread_is_the_chain_rule(golden_dir):
+    """mpi_vsites moves a force on a virtual site to its atoms as the chain
+    rule through the site's construction dictates: for E = -F . r_s the
+    spread forces equal -dE/dr of the constructing atoms (central
+    differences of build_vsites), they sum to F, and the site keeps none."""
+    deck, _ = load(golden_dir, "topo_au13")
+    o = Oracle(deck)
+    N = deck.n_atoms
+    r = o.vsite_build(deck.r0)
+    rng = np.random.default_rng(5)
+    s = N - 2                                     # a lone pair of the last water
+    i, j, k = N - 5, N - 4, N - 3
+    F = rng.normal(0.0, 100.0, 3)
+    f = np.zeros(3 * N)
+    f[[s, N + s, 2 * N + s]] = F
+    out = o.vsite_spread(r, f)
+
+    def energy(rr):
+        b = o.vsite_build(rr)
+        return -float(F @ b[[s, N + s, 2 * N + s]])
+
+    for a in (i, j, k):
+        for c in range(3):
+            g = central_diff(energy, r, c * N + a, h=1e-6)
+            assert abs(-g - out[c * N + a]) < 1e-6 * max(1.0, abs(out[c * N + a]))
+    tot = out.reshape(3, N)[:, [i, j, k]].sum(axis=1)
+    np.testing.assert_allclose(tot, F, rtol=1e-13, atol=1e-12)
+    assert not out.reshape(3, N)[:, s].any()
