@@ -42,7 +42,7 @@ DECKS = {
     "water_only": (2, False, False),
     "au13_em": (None, False, True),
     "2au13_cg": (None, False, True),
-    "topo_au13": (3, False, False),
+    "topo_au13": (3, False, True),
 }
 
 

@@ -3,7 +3,8 @@ outputs for the same deck (tests/golden/*/ref_*, written by the reference
 built from its sources): the stdout step lines and final energy table
 (mpi-main.c:993-995, force.c:337-354), traj.gro (file.c:212-229),
 vec_pq.txt (file.c:236-260) and the last traj.dat frame (file.c:269-278),
-for MD (run_type md) and for the minimiser (run_type em and cg,
+for MD (run_type md, including the general-topology deck topo_au13 with
+bonded terms and virtual sites) and for the minimiser (run_type em and cg,
 mpi-main.c:706-865: step lines, convergence report, a frame every step).
 
 traj.gro's first frame is compared byte for byte.  The rest passes through
@@ -95,7 +96,7 @@ def read_dat_frame(raw, n):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("name", ["au13_w", "2au13_field"])
+@pytest.mark.parametrize("name", ["au13_w", "2au13_field", "topo_au13"])
 def test_cli_matches_reference_program(name, prec, golden_dir, tmp_path):
     g = os.path.join(golden_dir, name)
     out = run_cli(golden_dir, name, tmp_path, prec)
