@@ -361,29 +361,16 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
 
 // Momentum / kinetic-energy partial sums per block of 256 atoms:
 // [sum m vx, m vy, m vz, m, m vx^2, m vy^2, m vz^2] over atoms with m > 0
-// (kinetic_energy thermostat.c:34-63, remove_comm thermostat.c:69-108).
-// pre 1 first scales every velocity by md[22] (the thermostat, as
-// k_scale_v); pre 2 first subtracts the centre-of-mass velocity from the
-// sums in `com` (as k_com_apply); the velocities are written back.
+// (kinetic_energy thermostat.c:34-63, remove_comm thermostat.c:69-108)
 __global__ void __launch_bounds__(256)
-k_mom_partial(int N, const double* __restrict__ mass, double* __restrict__ v,
-              double* __restrict__ partial, int pre, const double* __restrict__ md,
-              const double* __restrict__ com) {
+k_mom_partial(int N, const double* __restrict__ mass, const double* __restrict__ v,
+              double* __restrict__ partial) {
     __shared__ double s_red[4][7];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i < N) {
         double m = mass[i];
-        if (pre == 1) {
-            const double s = md[22];
-            v[i] *= s; v[N + i] *= s; v[2 * N + i] *= s;
-        } else if (pre == 2 && m > 0.0) {
-            double ms = com[3];
-            v[i] -= com[0] / ms;
-            v[N + i] -= com[1] / ms;
-            v[2 * N + i] -= com[2] / ms;
-        }
         if (m > 0.0) {
             double vx = v[i], vy = v[N + i], vz = v[2 * N + i];
             a[0] = vx * m; a[1] = vy * m; a[2] = vz * m; a[3] = m;
@@ -402,14 +389,6 @@ k_mom_partial(int N, const double* __restrict__ mass, double* __restrict__ v,
             for (int q = 0; q < 4; ++q) sm += s_red[q][threadIdx.x];
         partial[blockIdx.x * 16 + threadIdx.x] = sm;
     }
-}
-
-// eKsum and the instantaneous temperature from the reduced sums
-__global__ void k_ke_finish(const double* __restrict__ sums, int ndf, double* __restrict__ md) {
-    if (threadIdx.x != 0) return;
-    double ek = sums[4] + sums[5] + sums[6];
-    md[0] = ek;
-    md[1] = ek / (K_BOLTZ * ndf);
 }
 
 // Taylor polynomial of exp of degree 4 for |x| < 2^-10: the first omitted
@@ -476,20 +455,41 @@ __device__ __forceinline__ bool nhc_sweeps(double vQ[CAPAC_NHC], double aQ[CAPAC
     return true;
 }
 
+// The centre-of-mass velocity and the kinetic energy after the velocities
+// are scaled by s and the centre-of-mass motion is removed (remove_comm +
+// kinetic_energy, thermostat.c:34-108), from the sums of the unscaled
+// velocities: with P = sum m v and S = sum m v^2 per component over the atoms
+// with mass M, the removal subtracts c = s P / M and leaves
+// sum m (s v - c)^2 = s^2 (S - P^2 / M) per component.  One pass over the
+// atoms (k_scale_com) then applies s and c, where the reference makes three
+// (scale, momentum sum and removal, kinetic sum).  The values differ from
+// the reference's sums by rounding only: P is tiny once the motion has been
+// removed at the previous step, so the difference S - P^2/M cancels nothing.
+// md[26..28] = c, md[0] = eKsum, md[1] = temperature.
+__device__ __forceinline__ void com_kinetic_from_sums(double* __restrict__ md, const double* __restrict__ sums,
+                                                      double scale, int ndf) {
+    const double msum = sums[3];
+    double ek = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double pc = sums[c];
+        md[26 + c] = scale * pc / msum;
+        ek += scale * scale * (sums[4 + c] - pc * pc / msum);
+    }
+    md[0] = ek;
+    md[1] = ek / (K_BOLTZ * ndf);
+}
+
 // nose_hoover_chain (thermostat.c:117-191); the velocity scale goes to md[22].
 // The chain lives in registers (all indices are compile-time after unrolling).
-// With sums, eKsum and the temperature come from the momentum sums first
-// (as k_ke_finish).  The divisions by qMass are products with its
-// reciprocal, and small exponentials use exp_small: both differ from the
-// reference's operations by rounding only.
+// With sums (the step's second half), eKsum comes from the momentum sums and
+// the kinetic energy after the scale and the centre-of-mass removal is
+// formed as com_kinetic_from_sums describes.  The divisions by qMass are
+// products with its reciprocal, and small exponentials use exp_small: both
+// differ from the reference's operations by rounding only.
 __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, const double* __restrict__ sums) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     constexpr int M = CAPAC_NHC;
-    if (sums) {
-        const double ek = sums[4] + sums[5] + sums[6];
-        md[0] = ek;
-        md[1] = ek / (K_BOLTZ * ndf);
-    }
+    if (sums) md[0] = sums[4] + sums[5] + sums[6];
     double vQ[M], aQ[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) vQ[k] = md[2 + k];
@@ -511,8 +511,37 @@ __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, co
 #pragma unroll
     for (int k = 0; k < M; ++k) { md[2 + k] = vQ[k]; md[12 + k] = aQ[k]; }
     md[22] = scale;
-    md[0] = eKsum * (scale * scale);
-    md[1] *= scale * scale;
+    if (sums) {
+        com_kinetic_from_sums(md, sums, scale, ndf);
+    } else {
+        md[0] = eKsum * (scale * scale);
+        md[1] *= scale * scale;
+    }
+}
+
+// the same without a thermostat (scale 1)
+__global__ void k_com_finish(double* __restrict__ md, const double* __restrict__ sums, int ndf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    com_kinetic_from_sums(md, sums, 1.0, ndf);
+}
+
+// v = s v - c for atoms with mass, v = s v for virtual sites (the
+// thermostat scales every atom, the removal skips massless ones); s = md[22]
+// when scale is set, else 1; c = md[26..28]
+__global__ void __launch_bounds__(256)
+k_scale_com(int N, const double* __restrict__ mass, double* __restrict__ v, const double* __restrict__ md,
+            int scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double s = scale ? md[22] : 1.0;
+    const bool has_mass = mass[i] > 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t k = (size_t)c * N + i;
+        double vk = scale ? v[k] * s : v[k];
+        if (has_mass) vk -= md[26 + c];
+        v[k] = vk;
+    }
 }
 
 // apply_pbc (force.c:127-165): wrap the middle atom of each molecule into the
@@ -614,32 +643,31 @@ int md_rattle2(capac_ctx* c) {
     return reduce_partials(c, SEG_RATTLE, nblk, 9, 64);
 }
 
-// red[96..103) = momentum / kinetic sums, after the optional pre-pass of
-// k_mom_partial (1: thermostat scale, 2: centre-of-mass removal with the
-// sums already in red[96..103))
-static int mom_sums(capac_ctx* c, int pre = 0) {
+// red[96..103) = momentum / kinetic sums of the current velocities
+static int mom_sums(capac_ctx* c) {
     int nblk = grid(c->bc.N, 256);
-    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM), pre, c->md,
-                                               c->red + 96);
+    k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM));
     CHECK_HIP(hipGetLastError());
     int rc = reduce_partials(c, SEG_MOM, nblk, 7, 96);
     return rc ? rc : flush_reductions(c);      // read on the device right after
 }
 
-// centre-of-mass removal on the sums already in red[96..103), then the
-// kinetic energy and temperature (remove_comm + kinetic_energy)
-static int com_kinetic(capac_ctx* c) {
-    int rc = mom_sums(c, 2);
-    if (rc) return rc;
-    k_ke_finish<<<1, 64, 0, c->stream>>>(c->red + 96, c->ndf, c->md);
+// applies the scale (md[22], when `scale`) and the centre-of-mass removal
+// that k_nhc or k_com_finish formed from the sums
+static int scale_com(capac_ctx* c, int scale) {
+    k_scale_com<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, c->md, scale);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
 
-// remove_comm and kinetic_energy after the first force call (mpi-main.c:632-633)
+// remove_comm and kinetic_energy (mpi-main.c:632-633 after the first force
+// call; the end of a step without thermostat)
 int md_remove_com_kinetic(capac_ctx* c) {
     int rc = mom_sums(c);
-    return rc ? rc : com_kinetic(c);
+    if (rc) return rc;
+    k_com_finish<<<1, 64, 0, c->stream>>>(c->md, c->red + 96, c->ndf);
+    CHECK_HIP(hipGetLastError());
+    return scale_com(c, 0);
 }
 
 // First half of a step (mpi-main.c:885-925): nvt chain, scale, half kick
@@ -657,20 +685,17 @@ int md_first_half(capac_ctx* c, double kT) {
 }
 
 // Second half after RATTLE_2 (mpi-main.c:955-973): kinetic energy, nvt
-// chain and scale, centre-of-mass removal, kinetic energy.  The scale and
-// the removal are folded into the momentum-sum passes, and the chain takes
-// eKsum straight from the sums.
+// chain and scale, centre-of-mass removal, kinetic energy, in three
+// launches after the sums: one momentum-sum pass, the chain (which also forms
+// the centre-of-mass velocity and the final kinetic energy from the sums)
+// and one pass applying the scale and the removal.
 int md_second_half(capac_ctx* c, double kT) {
-    int rc;
-    if (c->nvt) {
-        if ((rc = mom_sums(c))) return rc;
-        k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, c->red + 96);
-        CHECK_HIP(hipGetLastError());
-        if ((rc = mom_sums(c, 1))) return rc;
-    } else if ((rc = mom_sums(c))) {
-        return rc;
-    }
-    return com_kinetic(c);
+    if (!c->nvt) return md_remove_com_kinetic(c);
+    int rc = mom_sums(c);
+    if (rc) return rc;
+    k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, c->red + 96);
+    CHECK_HIP(hipGetLastError());
+    return scale_com(c, 1);
 }
 
 int md_wrap(capac_ctx* c) {
