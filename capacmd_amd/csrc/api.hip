@@ -451,6 +451,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     TRY(upload(&c->cell_tmp, (const int*)NULL, N));
     TRY(upload(&c->n_front, (const int*)NULL, N));
     TRY(upload(&c->n_back, (const int*)NULL, N));
+    TRY(upload(&c->tail_list, (const int*)NULL, (size_t)(N - p->n_metal) + 1));
     {
         // room for the densest liquid or metal around a sphere of radius rc + skin
         double rl = b.rc + c->skin;
@@ -530,7 +531,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,
-                    c->vsite_par};
+                    c->vsite_par, c->tail_list};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     if (c->red_host) (void)hipHostFree(c->red_host);

@@ -117,6 +117,7 @@ struct capac_ctx {
     // neighbour rows
     int nb_stride;
     int *n_front, *n_back, *nbr, *nb_overflow;
+    int* tail_list;             // [0] count, then the non-metal atoms with metal neighbours
     // Verlet skin: rows hold pairs within rc + skin; the device rebuilds them
     // when an atom has moved more than skin/2 since the last build
     double skin, skin_req;      // in use / requested (nm)
@@ -138,6 +139,8 @@ struct capac_ctx {
     double* isd;
     // CPIM
     double *b, *x, *x_backup, *diag, *relay;
+    int diag_set;               // diag holds the values for diag_polar / diag_capac
+    double diag_polar, diag_capac;
     // solutions of the last MD steps, newest first, for the warm-start
     // predictor (order 0: the previous solution, as the reference)
     double* x_hist[3];

@@ -572,13 +572,24 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
     block_partials<11>(vals, s_red, partial);
 }
 
-// atom side of the metal-water terms: one wavefront per non-metal atom
+// atom side of the metal-water terms: the wavefronts stride over the
+// non-metal atoms that have metal neighbours (listed by the neighbour build;
+// most atoms have none), one atom per wavefront at a time
 __global__ void __launch_bounds__(256)
 k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
                const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
-               const double* __restrict__ x, double* __restrict__ f) {
-    const int a = bc.Nm + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a < bc.N) mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, a, threadIdx.x & 63, f);
+               const int* __restrict__ tail_list, const double* __restrict__ x, double* __restrict__ f) {
+    const int count = tail_list[0];
+    const int nw = gridDim.x * 4;
+    for (int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)); q < count; q += nw)
+        mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, tail_list[1 + q], threadIdx.x & 63, f);
+}
+
+// workgroups of the atom-side kernel: enough wavefronts to give each listed
+// atom its own in one round on this deck size, capped
+inline unsigned mw_atom_blocks(int n_nonmetal) {
+    const int nb = (n_nonmetal + 3) / 4;
+    return (unsigned)(nb < 1 ? 1 : (nb < 2048 ? nb : 2048));
 }
 
 // metal-metal terms: one workgroup per metal atom;
@@ -624,7 +635,14 @@ int cpim_assemble(capac_ctx* c) {
         k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
     else
         k_assemble64<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay);
-    k_diag<<<(c->n_mat + 255) / 256, 256, 0, c->stream>>>(c->bc, c->n_mat, c->diag);
+    // the Jacobi diagonal holds constants only (1/alpha, -1/c, 1): formed once
+    // and again only when those constants change (fitting parameters)
+    if (!c->diag_set || c->diag_polar != c->bc.inv_polar || c->diag_capac != c->bc.inv_capac) {
+        k_diag<<<(c->n_mat + 255) / 256, 256, 0, c->stream>>>(c->bc, c->n_mat, c->diag);
+        c->diag_set = 1;
+        c->diag_polar = c->bc.inv_polar;
+        c->diag_capac = c->bc.inv_capac;
+    }
     CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -662,8 +680,9 @@ int cpim_forces_x(capac_ctx* c, const double* x) {
     int rc = reduce_partials(c, SEG_MW, nblk, 11, 32);
     if (rc) return rc;
     if (N > Nm)
-        k_cpim_mw_atom<<<(N - Nm + 3) / 4, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
-                                                                    c->nbr, c->nb_stride, x, c->f);
+        k_cpim_mw_atom<<<mw_atom_blocks(N - Nm), 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
+                                                                        c->nbr, c->nb_stride, c->tail_list, x,
+                                                                        c->f);
     k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, partial_seg(c, SEG_MM));
     CHECK_HIP(hipGetLastError());
     return reduce_partials(c, SEG_MM, nblk, 13, 48);

@@ -198,7 +198,8 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
            const int* __restrict__ cell_start, const unsigned* __restrict__ excl, BoxConst bc,
            int n0, int n1, int n2, const Stencil* __restrict__ st, int stride,
            int* __restrict__ n_front, int* __restrict__ n_back, int* __restrict__ nbr,
-           int* __restrict__ overflow, const unsigned long long* __restrict__ need) {
+           int* __restrict__ overflow, int* __restrict__ tail_list,
+           const unsigned long long* __restrict__ need) {
     __shared__ Cand c_c[kCap];
     if (!need[1]) return;
     __shared__ int h_nf[kMaxHome], h_nb[kMaxHome];
@@ -289,8 +290,13 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
         __syncthreads();
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             const int i = cell_atoms[hbase + h];
+            const int nbv = h_nf[h] < 0 ? 0 : h_nb[h];
             n_front[i] = h_nf[h] < 0 ? 0 : h_nf[h];
-            n_back[i] = h_nf[h] < 0 ? 0 : h_nb[h];
+            n_back[i] = nbv;
+            // non-metal atoms with metal neighbours: the atom-side CPIM
+            // force kernel visits only these (their order does not matter,
+            // each is summed on its own)
+            if (i >= Nm && nbv > 0) tail_list[1 + atomicAdd(&tail_list[0], 1)] = i;
         }
         __syncthreads();
     }
@@ -316,7 +322,7 @@ k_disp(const double* __restrict__ r, const double* __restrict__ r_build, int N,
 // rebuild flag from the displacement; when set, also clears the counters the
 // build accumulates into
 __global__ void k_decide(unsigned long long* __restrict__ ctl, double limit2, int* __restrict__ cell_count,
-                         int n_cell, int* __restrict__ overflow) {
+                         int n_cell, int* __restrict__ overflow, int* __restrict__ tail_list) {
     __shared__ int s_go;
     if (threadIdx.x == 0) {
         double d2 = __longlong_as_double((long long)ctl[0]);
@@ -329,6 +335,7 @@ __global__ void k_decide(unsigned long long* __restrict__ ctl, double limit2, in
     if (!s_go) return;
     for (int k = threadIdx.x; k <= n_cell; k += blockDim.x) cell_count[k] = 0;
     if (threadIdx.x < 2) overflow[threadIdx.x] = 0;
+    if (threadIdx.x == 0) tail_list[0] = 0;
 }
 
 // after a build: remember the positions and clear the flag
@@ -421,7 +428,7 @@ static int nb_launch(capac_ctx* c) {
                                          c->sorted_mol, c->sorted_loc, c->cell_atoms, c->cell_start,
                                          c->excl, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
                                          (const Stencil*)c->stencil, c->nb_stride, c->n_front,
-                                         c->n_back, c->nbr, c->nb_overflow, need);
+                                         c->n_back, c->nbr, c->nb_overflow, c->tail_list, need);
     k_mark_built<<<(3 * N + 255) / 256, 256, 0, s>>>(c->r, c->r_build, 3 * N, c->nb_ctl);
     k_clear_flag<<<1, 64, 0, s>>>(c->nb_ctl);
     CHECK_HIP(hipGetLastError());
@@ -431,7 +438,7 @@ static int nb_launch(capac_ctx* c) {
 // unconditional build
 int nb_build(capac_ctx* c) {
     k_set_flag<<<1, 64, 0, c->stream>>>(c->nb_ctl);
-    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, 0.0, c->cell_count, c->n_cell, c->nb_overflow);
+    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, 0.0, c->cell_count, c->n_cell, c->nb_overflow, c->tail_list);
     return nb_launch(c);
 }
 
@@ -444,6 +451,7 @@ int nb_build_if_needed(capac_ctx* c) {
     // half the skin, less a relative 1e-6 so rounding in the distance tests
     // cannot matter at the boundary
     double half = 0.5 * c->skin * (1.0 - 1e-6);
-    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, half * half, c->cell_count, c->n_cell, c->nb_overflow);
+    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, half * half, c->cell_count, c->n_cell, c->nb_overflow,
+                                       c->tail_list);
     return nb_launch(c);
 }
