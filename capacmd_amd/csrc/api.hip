@@ -559,7 +559,7 @@ extern "C" int capac_set_state(capac_ctx* c, const double* r, const double* v, c
         CHECK_HIP(hipMemcpyAsync(c->x, x, c->n_mat * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     if (r) c->nb_dirty = 1;
-    if (r || x) c->pred_count = 0;      // the stored solutions no longer describe this state
+    if (r || x) { c->pred_count = 0; c->pred_pending = 0; }   // the stored solutions no longer describe this state
     return 0;
 }
 
@@ -591,7 +591,7 @@ extern "C" int capac_set_box(capac_ctx* c, const double box[3]) {
         c->bc.box[d] = box[d];
         c->bc.half[d] = 0.5 * box[d];
     }
-    c->pred_count = 0;
+    c->pred_count = 0; c->pred_pending = 0;
     return nb_regrid(c);
 }
 
@@ -746,8 +746,8 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         mark(c, 0);
         // virtual sites from their atoms first (force.c:2128-2134)
         if ((rc = vsite_build(c))) return rc;
-        if ((rc = clear_red(c, 0, 256))) return rc;
-        if (c->n_mat && (rc = copy_d(c, c->x, c->x_backup, c->n_mat))) return rc;
+        if ((rc = flush_reductions(c))) return rc;     // nothing queued may land after the clear
+        if ((rc = force_prologue(c, 256))) return rc;
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
         c->nb_dirty = 0;
         mark(c, 1);
@@ -1125,7 +1125,7 @@ extern "C" int capac_md_begin(capac_ctx* c) {
     c->ext_temp = c->ref_temp0;
     c->cur_ref_temp = c->ref_temp0;
     // the first force call's solution starts the predictor's history
-    c->pred_count = 0;
+    c->pred_count = 0; c->pred_pending = 0;
     solver_push_history(c);
     return 0;
 }
@@ -1300,7 +1300,7 @@ extern "C" int capac_checkpoint_load(capac_ctx* c, const void* buf, size_t bytes
     CHECK_HIP(hipStreamSynchronize(c->stream));
     c->solve_iters = h.solve_iters;
     c->pred_order = h.pred_order;
-    c->pred_count = h.pred_count;
+    c->pred_count = h.pred_count; c->pred_pending = 0;
     c->first_temp = h.first_temp; c->ext_temp = h.ext_temp; c->cur_ref_temp = h.cur_ref_temp;
     memcpy(c->pot, h.pot, sizeof h.pot);
     memcpy(c->vir, h.vir, sizeof h.vir);

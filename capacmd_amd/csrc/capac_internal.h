@@ -145,6 +145,7 @@ struct capac_ctx {
     // predictor (order 0: the previous solution, as the reference)
     double* x_hist[3];
     int pred_order, pred_count;
+    int pred_pending;           // extrapolation order the next force call applies
     float* relay32;
     int relay_ld;
     size_t relay_bytes;
@@ -257,7 +258,8 @@ void solver_destroy(capac_ctx* c);
 int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
 double* solver_scratch(capac_ctx* c);
-void solver_predict(capac_ctx* c);          // x from the history (md steps)
+void solver_predict(capac_ctx* c);          // x from the history (md steps), at the next force call
+int force_prologue(capac_ctx* c, int n_red); // red cleared, pending extrapolation, x_backup
 void solver_push_history(capac_ctx* c);     // the solution just found joins it
 int cpim_mm_only(capac_ctx* c);
 // bonded.hip

@@ -414,15 +414,22 @@ __global__ void __launch_bounds__(256) k_zero(int n, double* __restrict__ x) {
     if (k < n) x[k] = 0.0;
 }
 
-// warm start extrapolated from the last solutions (h0 newest): linear
-// 2 h0 - h1, quadratic 3 h0 - 3 h1 + h2
+// The start of a force call in one launch: red[0..n_red) cleared, the warm
+// start extrapolated from the last solutions when an MD step asked for it
+// (h0 newest; order 1: 2 h0 - h1, order 2: 3 h0 - 3 h1 + h2), and x kept in
+// x_backup for a call that has to be repeated
 __global__ void __launch_bounds__(256)
-k_predict(int n, int order, const double* __restrict__ h0, const double* __restrict__ h1,
-          const double* __restrict__ h2, double* __restrict__ x) {
+k_force_prologue(int n, int order, const double* __restrict__ h0, const double* __restrict__ h1,
+                 const double* __restrict__ h2, double* __restrict__ x, double* __restrict__ x_backup,
+                 double* __restrict__ red, int n_red) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_red) red[k] = 0.0;
     if (k >= n) return;
-    if (order == 1) x[k] = 2.0 * h0[k] - h1[k];
-    else x[k] = 3.0 * (h0[k] - h1[k]) + h2[k];
+    double xk;
+    if (order == 1) { xk = 2.0 * h0[k] - h1[k]; x[k] = xk; }
+    else if (order == 2) { xk = 3.0 * (h0[k] - h1[k]) + h2[k]; x[k] = xk; }
+    else xk = x[k];
+    x_backup[k] = xk;
 }
 
 }  // namespace
@@ -627,12 +634,21 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
 // (quadratic).  The stop test, and so the accuracy of every solution, is
 // unchanged.  With fewer stored solutions than the order needs, the order
 // drops to what is there.
+// The extrapolation runs in the next force call's prologue (force_prologue).
 void solver_predict(capac_ctx* c) {
     const int order = c->pred_order < c->pred_count - 1 ? c->pred_order : c->pred_count - 1;
-    if (order <= 0 || c->n_mat == 0) return;
+    c->pred_pending = (order > 0 && c->n_mat > 0) ? order : 0;
+}
+
+// clears red[0..n_red), applies a pending extrapolation and saves x in x_backup
+int force_prologue(capac_ctx* c, int n_red) {
     const int n = c->n_mat;
-    k_predict<<<(n + 255) / 256, 256, 0, c->stream>>>(n, order, c->x_hist[0], c->x_hist[1], c->x_hist[2],
-                                                      c->x);
+    const int work = n > n_red ? n : n_red;
+    k_force_prologue<<<(work + 255) / 256, 256, 0, c->stream>>>(n, c->pred_pending, c->x_hist[0], c->x_hist[1],
+                                                                c->x_hist[2], c->x, c->x_backup, c->red, n_red);
+    c->pred_pending = 0;
+    CHECK_HIP(hipGetLastError());
+    return 0;
 }
 
 void solver_push_history(capac_ctx* c) {
