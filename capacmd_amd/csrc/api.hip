@@ -469,7 +469,6 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->seg_cap[SEG_SC] = metal_blocks;
         c->seg_cap[SEG_MW] = Nm + 1;        // one workgroup per metal atom
         c->seg_cap[SEG_MM] = Nm + 1;
-        c->seg_cap[SEG_EXT] = (N + 255) / 256 + 1;
         c->seg_cap[SEG_RATTLE] = (p->n_mols + 63) / 64 + 1;      // workgroups of 64 or more
         c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;
         c->seg_cap[SEG_BONDED] = bonded_blocks(p->n_mols) + 1;
@@ -719,9 +718,9 @@ static int side_forces(capac_ctx* c) {
         // the pair forces then add to them
         rc = fill_d(c, c->f, 3 * (size_t)c->bc.N, 0.0);
         if (!rc) rc = bonded_forces(c);
-        if (!rc) rc = pair_nonbonded(c, c->f, side_pair_blocks(c));
+        if (!rc) rc = pair_nonbonded(c, c->f, side_pair_blocks(c), 1);
     } else {
-        rc = pair_nonbonded(c, NULL, side_pair_blocks(c));
+        rc = pair_nonbonded(c, NULL, side_pair_blocks(c), 1);
     }
     mark(c, 12);
     if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
@@ -780,10 +779,10 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_side, 0));
         if (c->n_mat && (rc = cpim_forces(c))) return rc;
         rc = solve_rc;
-        int rc2 = ext_field(c);
+        // (the external field was added with the pair forces)
         // forces on virtual sites go to their atoms once everything is in
         // (force.c:2300-2306)
-        if (!rc2) rc2 = vsite_spread(c, c->f);
+        int rc2 = vsite_spread(c, c->f);
         if (rc2) return rc2;
         mark(c, 7);
         if (deferred) break;
@@ -811,7 +810,7 @@ static void take_force_results(capac_ctx* c) {
     c->pot[1] = h[16];
     c->pot[10] = h[32];
     c->pot[11] = h[33];
-    c->pot[9] = h[48] + h[80];
+    c->pot[9] = h[48] + h[11];
     c->pot[12] = h[49];
     c->pot[13] = h[50];
     c->pot[14] = h[51];

@@ -237,7 +237,7 @@ int nb_build(capac_ctx* c);                 // unconditional build at rc + skin
 int nb_build_if_needed(capac_ctx* c);       // device-side decision from displacements
 int nb_set_radius(capac_ctx* c, double skin);
 // pair.hip
-int pair_nonbonded(capac_ctx* c, double* f_out_accumulate, int max_blocks = 0);
+int pair_nonbonded(capac_ctx* c, double* f_out_accumulate, int max_blocks = 0, int field = 0);
 // sc.hip
 int sc_forces(capac_ctx* c);
 // cpim.hip
@@ -287,13 +287,12 @@ int md_second_half(capac_ctx* c, double kT);   // kinetic energy, chain, scale, 
 int md_remove_com_kinetic(capac_ctx* c);   // after the first force call
 int md_wrap(capac_ctx* c);
 int md_fmax(capac_ctx* c);
-int ext_field(capac_ctx* c);
 // reductions (api.hip): sum per-block partials [nblk x 16] into red[0..nval)
 // Per-block partial sums [nblk x 16] are written into the producer's own
 // segment of c->partial; reduce_partials queues their fixed-order reduction
 // into red[red_offset..+nval), and flush_reductions launches every queued one
 // in a single kernel (before anything reads red).
-enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_EXT, SEG_RATTLE, SEG_MOM, SEG_BONDED, SEG_COUNT };
+enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_RATTLE, SEG_MOM, SEG_BONDED, SEG_COUNT };
 double* partial_seg(capac_ctx* c, int seg);
 int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int red_offset);
 int flush_reductions(capac_ctx* c);

@@ -587,27 +587,6 @@ k_fmax(int N, const double* __restrict__ f, double* __restrict__ md) {
     }
 }
 
-// external field on all charges (force.c:2251-2260); pot[9] partials
-__global__ void __launch_bounds__(256)
-k_ext(int N, BoxConst bc, const double* __restrict__ charge, const double* __restrict__ r,
-      double* __restrict__ f, double* __restrict__ partial) {
-    __shared__ double s_red[4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double e = 0.0;
-    if (i < N) {
-        double q = charge[i];
-        f[i] += bc.efield[0] * q;
-        f[N + i] += bc.efield[1] * q;
-        f[2 * N + i] += bc.efield[2] * q;
-        e = -(bc.efield[0] * q * r[i] + bc.efield[1] * q * r[N + i] + bc.efield[2] * q * r[2 * N + i]);
-    }
-    e = wave_sum(e);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = e;
-    __syncthreads();
-    if (threadIdx.x < 16) partial[blockIdx.x * 16 + threadIdx.x] =
-        threadIdx.x == 0 ? s_red[0] + s_red[1] + s_red[2] + s_red[3] : 0.0;
-}
-
 }  // namespace
 
 static inline int grid(int n, int b) { return (n + b - 1) / b; }
@@ -711,10 +690,3 @@ int md_fmax(capac_ctx* c) {
     return 0;
 }
 
-// red[80] = pot9 from the charges
-int ext_field(capac_ctx* c) {
-    int nblk = grid(c->bc.N, 256);
-    k_ext<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->bc, c->charge, c->r, c->f, partial_seg(c, SEG_EXT));
-    CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_EXT, nblk, 1, 80);
-}

@@ -21,7 +21,7 @@ constexpr int kWavesPerBlock = 4;
 // of the lane's pairs) the two energies and the virial, symmetric for central
 // pair forces, so six components: xx, xy, xz, yy, yz, zz
 struct PairOut {
-    double fx, fy, fz, e7, e8, v[6];
+    double fx, fy, fz, e7, e8, e9, v[6];
 };
 
 // The reference's expressions with each division by r, r^2, r^6 or the erf
@@ -104,9 +104,9 @@ __global__ void __launch_bounds__(256, 3)
 k_pair(const double* __restrict__ r, const double* __restrict__ charge, const int* __restrict__ type,
        const PairParam* __restrict__ table, BoxConst bc, const int* __restrict__ n_front,
        const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
-       double* __restrict__ f, int accumulate, double* __restrict__ partial) {
+       double* __restrict__ f, int accumulate, int field, double* __restrict__ partial) {
     __shared__ PairParam s_tab[64];
-    __shared__ double s_red[kWavesPerBlock][11];
+    __shared__ double s_red[kWavesPerBlock][12];
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     const int N = bc.N, nt = bc.n_types;
     for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) s_tab[k] = table[k];
@@ -149,25 +149,32 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
+            if (field) {
+                // the external field on the atom's charge, and its energy
+                // (force.c:2251-2260), in the same write
+                fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
+                o.e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+            }
             if (accumulate) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
             else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
         }
     }
     const int wave = threadIdx.x / CAPAC_WAVE;
-    double vals[11] = {o.e7, o.e8, o.v[0], o.v[1], o.v[2], o.v[1], o.v[3],
-                       o.v[4], o.v[2], o.v[4], o.v[5]};
+    double vals[12] = {o.e7, o.e8, o.v[0], o.v[1], o.v[2], o.v[1], o.v[3],
+                       o.v[4], o.v[2], o.v[4], o.v[5], o.e9};
 #pragma unroll
-    for (int c = 0; c < 11; ++c) {
+    for (int c = 0; c < 12; ++c) {
         double s = wave_sum(vals[c]);
         if (lane == 0) s_red[wave][c] = s;
     }
     __syncthreads();
     if (threadIdx.x < 16) {
         double s = 0.0;
-        if (threadIdx.x < 11)
+        if (threadIdx.x < 12)
             for (int w = 0; w < kWavesPerBlock; ++w) s += s_red[w][threadIdx.x];
-        // each ordered pair was visited from both atoms
-        partial[blockIdx.x * 16 + threadIdx.x] = 0.5 * s;
+        // each ordered pair was visited from both atoms (the field energy,
+        // slot 11, once per atom)
+        partial[blockIdx.x * 16 + threadIdx.x] = threadIdx.x == 11 ? s : 0.5 * s;
     }
 }
 
@@ -178,16 +185,18 @@ int pair_blocks(int N) {
     return nb < 2048 ? nb : 2048;
 }
 
-// writes (or adds to) f; red[0..11) = pot7, pot8, vir[9]; max_blocks > 0
-// caps the grid (the kernel strides over atoms), leaving compute units to
-// work that runs beside it
-int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks) {
+// writes (or adds to) f; red[0..11) = pot7, pot8, vir[9], and with `field`
+// the external field's force on every charge added in the same write and
+// red[11] = its energy pot9 (force.c:2251-2260); max_blocks > 0 caps the
+// grid (the kernel strides over atoms), leaving compute units to work that
+// runs beside it
+int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks, int field) {
     int nblk = pair_blocks(c->bc.N);
     if (max_blocks > 0 && nblk > max_blocks) nblk = max_blocks;
     double* f = f_accumulate ? f_accumulate : c->f;
     k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
                                         c->n_back, c->nbr, c->nb_stride, f,
-                                        f_accumulate ? 1 : 0, partial_seg(c, SEG_PAIR));
+                                        f_accumulate ? 1 : 0, field, partial_seg(c, SEG_PAIR));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_PAIR, nblk, 11, 0);
+    return reduce_partials(c, SEG_PAIR, nblk, 12, 0);
 }
