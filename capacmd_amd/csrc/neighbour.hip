@@ -38,6 +38,9 @@ namespace {
 constexpr int kHalf = 2;                 // stencil half-width in cells
 constexpr int kHistLds = 4096;           // cells handled by the LDS histogram
 
+// a coordinate wrapped into [0, L] (L itself possible by rounding)
+__device__ __forceinline__ double wrap_coord(double x, double L) { return x - L * floor(x / L); }
+
 __device__ __forceinline__ int cell_of_pos(const BoxConst& bc, double x, double y, double z,
                                            int n0, int n1, int n2) {
     int n[3] = {n0, n1, n2};
@@ -46,7 +49,7 @@ __device__ __forceinline__ int cell_of_pos(const BoxConst& bc, double x, double 
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double L = bc.box[d];
-        double xw = p[d] - L * floor(p[d] / L);
+        double xw = wrap_coord(p[d], L);
         double t = xw * n[d] / L;
         // clamp before the conversion (also maps a non-finite coordinate to 0)
         if (!(t >= 0.0)) t = 0.0;
@@ -142,7 +145,7 @@ __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict
                             const double* __restrict__ r, const int* __restrict__ mol_id,
                             const int* __restrict__ local_idx, int N, int* __restrict__ cell_atoms,
                             double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
-                            int* __restrict__ smol, int* __restrict__ sloc,
+                            int* __restrict__ smol, int* __restrict__ sloc, double3 box,
                             const unsigned long long* __restrict__ need) {
     __shared__ int s_ids[SORT_LDS];
     if (!need[1]) return;
@@ -159,9 +162,11 @@ __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict
         for (int m = 0; m < n; ++m) rank += src[m] < v;
         int slot = lo + rank;
         cell_atoms[slot] = v;
-        sx[slot] = r[v];
-        sy[slot] = r[N + v];
-        sz[slot] = r[2 * N + v];
+        // coordinates wrapped into the box as the cell assignment wraps
+        // them, so a stencil cell's periodic image is one shift for all its atoms
+        sx[slot] = wrap_coord(r[v], box[0]);
+        sy[slot] = wrap_coord(r[N + v], box[1]);
+        sz[slot] = wrap_coord(r[2 * N + v], box[2]);
         smol[slot] = mol_id[v];
         sloc[slot] = local_idx[v];
     }
@@ -191,6 +196,14 @@ struct alignas(16) Cand {
     int id, ml;
 };
 
+// Shifted: every box edge holds at least 2 kHalf + 1 cells, so each stencil
+// cell is a distinct cell and its periodic image relative to the home cell is
+// fixed by the offset's wrap; the staged candidates carry that image (wrapped
+// coordinate plus the shift), and the candidate loop needs no periodic
+// correction.  Membership can differ from the corrected difference only by
+// rounding at the list radius, which the rebuild test's 1e-6 margin covers.
+// Otherwise (a stencil that lists every cell once) the loop applies pbc12.
+template <bool Shifted>
 __global__ void __launch_bounds__(kBuildThreads)
 k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
            const double* __restrict__ sz, const int* __restrict__ smol,
@@ -204,6 +217,7 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
     if (!need[1]) return;
     __shared__ int h_nf[kMaxHome], h_nb[kMaxHome];
     __shared__ int seg_lo[125], seg_off[126];
+    __shared__ double seg_sh[125][3];
     const int Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -216,6 +230,11 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
     if (threadIdx.x < ns) {
         int t = threadIdx.x;
         int ox = cx + st->off[t][0], oy = cy + st->off[t][1], oz = cz + st->off[t][2];
+        if (Shifted) {
+            seg_sh[t][0] = ox < 0 ? -bc.box[0] : (ox >= n0 ? bc.box[0] : 0.0);
+            seg_sh[t][1] = oy < 0 ? -bc.box[1] : (oy >= n1 ? bc.box[1] : 0.0);
+            seg_sh[t][2] = oz < 0 ? -bc.box[2] : (oz >= n2 ? bc.box[2] : 0.0);
+        }
         ox = (ox % n0 + n0) % n0; oy = (oy % n1 + n1) % n1; oz = (oz % n2 + n2) % n2;
         const int oc = (ox * n1 + oy) * n2 + oz;
         seg_lo[t] = cell_start[oc];
@@ -233,7 +252,12 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
         for (int h = threadIdx.x; h < nh; h += blockDim.x) { h_nf[h] = 0; h_nb[h] = 0; }
         for (int w0 = 0; w0 < total_all; w0 += kCap) {
             const int total = min(kCap, total_all - w0);
+            // padded to whole passes of 64 with far-away entries that no test
+            // accepts, so the candidate loop needs no bounds test
+            const int padded = (total + 63) & ~63;
             __syncthreads();
+            for (int f = total + threadIdx.x; f < padded; f += blockDim.x)
+                c_c[f] = Cand{1e30, 1e30, 1e30, -1, -1};
             for (int f = threadIdx.x; f < total; f += blockDim.x) {
                 const int g = w0 + f;
                 int lo = 0, hi = ns - 1;             // segment holding flat index g
@@ -242,33 +266,35 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
                     if (seg_off[mid] <= g) lo = mid; else hi = mid - 1;
                 }
                 const int k = seg_lo[lo] + (g - seg_off[lo]);
-                c_c[f] = Cand{sx[k], sy[k], sz[k], cell_atoms[k], smol[k] * 32 + sloc[k]};
+                if (Shifted)
+                    c_c[f] = Cand{sx[k] + seg_sh[lo][0], sy[k] + seg_sh[lo][1], sz[k] + seg_sh[lo][2],
+                                  cell_atoms[k], smol[k] * 32 + sloc[k]};
+                else
+                    c_c[f] = Cand{sx[k], sy[k], sz[k], cell_atoms[k], smol[k] * 32 + sloc[k]};
             }
             __syncthreads();
-            for (int h = wave; h < nh; h += kBuildThreads / 64) {
+            // the home atom is the same on every lane: its index, data and
+            // row counts live in scalar registers, so the row-capacity test
+            // below is a scalar branch
+            for (int h = __builtin_amdgcn_readfirstlane(wave); h < nh; h += kBuildThreads / 64) {
                 const int hs = hbase + h;
                 const int i = cell_atoms[hs];
                 const double xi = sx[hs], yi = sy[hs], zi = sz[hs];
                 const int moli = smol[hs];
                 const unsigned exi = excl[i];
                 int* row = nbr + (size_t)i * stride;
-                int nf = h_nf[h], nb = h_nb[h];
+                int nf = __builtin_amdgcn_readfirstlane(h_nf[h]), nb = __builtin_amdgcn_readfirstlane(h_nb[h]);
                 if (nf < 0) continue;                 // this row already overflowed
-                for (int base = 0; base < total; base += 64) {
-                    const int q = base + lane;
-                    bool acc = false;
-                    int j = -1;
-                    if (q < total) {
-                        const Cand cd = c_c[q];
-                        j = cd.id;
-                        bool excluded = ((cd.ml >> 5) == moli) && ((exi >> (cd.ml & 31)) & 1u);
-                        if (j != i && !excluded) {
-                            double dx = cd.x - xi, dy = cd.y - yi, dz = cd.z - zi;
-                            pbc12(bc, dx, dy, dz);
-                            double r2 = dist2_exact(dx, dy, dz);
-                            acc = r2 < bc.rl2;
-                        }
-                    }
+                for (int base = 0; base < padded; base += 64) {
+                    // every lane evaluates its candidate without branching;
+                    // the tests combine into one predicate
+                    const Cand cd = c_c[base + lane];
+                    const int j = cd.id;
+                    const bool excluded = ((cd.ml >> 5) == moli) && ((exi >> (cd.ml & 31)) & 1u);
+                    double dx = cd.x - xi, dy = cd.y - yi, dz = cd.z - zi;
+                    if (!Shifted) pbc12(bc, dx, dy, dz);
+                    const double r2 = dist2_exact(dx, dy, dz);
+                    const bool acc = (j != i) & !excluded & (r2 < bc.rl2);
                     const unsigned long long mf = __ballot(acc && j >= Nm);
                     const unsigned long long mb = __ballot(acc && j < Nm);
                     const int cf = __popcll(mf), cb = __popcll(mb);
@@ -277,10 +303,9 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
                         nf = -1;
                         break;
                     }
-                    if (acc) {
-                        if (j >= Nm) row[nf + __popcll(mf & below)] = j;
-                        else row[stride - 1 - (nb + __popcll(mb & below))] = j;
-                    }
+                    const int pf = nf + __popcll(mf & below);
+                    const int pb = stride - 1 - (nb + __popcll(mb & below));
+                    if (acc) row[j >= Nm ? pf : pb] = j;
                     nf += cf;
                     nb += cb;
                 }
@@ -421,14 +446,18 @@ static int nb_launch(capac_ctx* c) {
                                     c->cell_of, c->cell_rank, c->cell_count, need);
     k_cell_scan<<<1, 1024, 0, s>>>(c->cell_count, c->n_cell, c->cell_start, c->cell_fill, need);
     k_cell_fill<<<gb, tb, 0, s>>>(c->cell_of, c->cell_rank, N, c->n_cell, c->cell_fill, c->cell_tmp, need);
+    const double3 box = make_double3(c->bc.box[0], c->bc.box[1], c->bc.box[2]);
     k_cell_sort<<<c->n_cell, 256, 0, s>>>(c->cell_start, c->cell_tmp, c->r, c->mol_id, c->local_idx, N,
                                           c->cell_atoms, c->sorted_r, c->sorted_r + N,
-                                          c->sorted_r + 2 * N, c->sorted_mol, c->sorted_loc, need);
-    k_nb_build<<<c->n_cell, kBuildThreads, 0, s>>>(c->sorted_r, c->sorted_r + N, c->sorted_r + 2 * N,
-                                         c->sorted_mol, c->sorted_loc, c->cell_atoms, c->cell_start,
-                                         c->excl, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
-                                         (const Stencil*)c->stencil, c->nb_stride, c->n_front,
-                                         c->n_back, c->nbr, c->nb_overflow, c->tail_list, need);
+                                          c->sorted_r + 2 * N, c->sorted_mol, c->sorted_loc, box, need);
+    bool shifted = true;
+    for (int d = 0; d < 3; ++d) shifted = shifted && c->ncell[d] >= 2 * kHalf + 1;
+    auto build = shifted ? k_nb_build<true> : k_nb_build<false>;
+    build<<<c->n_cell, kBuildThreads, 0, s>>>(c->sorted_r, c->sorted_r + N, c->sorted_r + 2 * N,
+                                            c->sorted_mol, c->sorted_loc, c->cell_atoms, c->cell_start,
+                                            c->excl, c->bc, c->ncell[0], c->ncell[1], c->ncell[2],
+                                            (const Stencil*)c->stencil, c->nb_stride, c->n_front,
+                                            c->n_back, c->nbr, c->nb_overflow, c->tail_list, need);
     k_mark_built<<<(3 * N + 255) / 256, 256, 0, s>>>(c->r, c->r_build, 3 * N, c->nb_ctl);
     k_clear_flag<<<1, 64, 0, s>>>(c->nb_ctl);
     CHECK_HIP(hipGetLastError());
