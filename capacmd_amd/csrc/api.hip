@@ -441,7 +441,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     TRY(nb_set_radius(c, c->skin_req));
     TRY(alloc_cells(c));
     TRY(upload(&c->r_build, (const double*)NULL, 3 * (size_t)N));
-    TRY(upload(&c->nb_ctl, (const unsigned long long*)NULL, 4));
+    TRY(upload(&c->nb_ctl, (const unsigned long long*)NULL, 8));
     TRY(upload(&c->cell_of, (const int*)NULL, N));
     TRY(upload(&c->cell_rank, (const int*)NULL, N));
     TRY(upload(&c->sorted_r, (const double*)NULL, 3 * (size_t)N));

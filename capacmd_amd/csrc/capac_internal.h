@@ -124,7 +124,8 @@ struct capac_ctx {
     int nb_dirty;               // positions or box replaced: build unconditionally
     long long nb_builds_base;   // build counter at the last timing reset
     double* r_build;            // 3N positions at the last build
-    unsigned long long* nb_ctl; // [0] max displacement^2 bits, [1] rebuild flag, [2] builds
+    unsigned long long* nb_ctl; // [0] max displacement^2 bits, [1] rebuild flag, [2] builds,
+                                // [3], [4] workgroup completion counts (zero between launches)
 
     // reductions: per-block partials and a small result area
     double* partial;            // per-block partials, one segment per producer (PartialSeg)

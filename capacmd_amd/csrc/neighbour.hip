@@ -328,10 +328,15 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
 }
 
 // largest squared raw displacement since the last build (atomicMax on the
-// bit pattern: non-negative doubles order like their bits); decides the rebuild
+// bit pattern: non-negative doubles order like their bits) and, in the last
+// workgroup of the launch to finish (a completion count in ctl[3]), the
+// rebuild decision and the clearing of the counters the build accumulates
+// into: one launch where a displacement pass and a decision kernel were two
 __global__ void __launch_bounds__(256)
-k_disp(const double* __restrict__ r, const double* __restrict__ r_build, int N,
-       unsigned long long* __restrict__ ctl) {
+k_disp_decide(const double* __restrict__ r, const double* __restrict__ r_build, int N,
+              unsigned long long* __restrict__ ctl, double limit2, int* __restrict__ cell_count,
+              int n_cell, int* __restrict__ overflow, int* __restrict__ tail_list) {
+    __shared__ int s_flag;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
     if (i < N) {
@@ -341,7 +346,30 @@ k_disp(const double* __restrict__ r, const double* __restrict__ r_build, int N,
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&ctl[0], (unsigned long long)__double_as_longlong(d2));
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&ctl[0], (unsigned long long)__double_as_longlong(d2));
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_flag = atomicAdd(&ctl[3], 1ull) == (unsigned long long)(gridDim.x - 1);
+    __syncthreads();
+    if (!s_flag) return;
+    // the last workgroup: every other one has added its maximum
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const double m2 = __longlong_as_double((long long)atomicMax(&ctl[0], 0ull));
+        const int go = ctl[1] != 0ull || m2 > limit2;
+        if (go) ctl[1] = 1ull;
+        ctl[0] = 0ull;
+        ctl[3] = 0ull;
+        s_flag = go;
+    }
+    __syncthreads();
+    if (!s_flag) return;
+    for (int k = threadIdx.x; k <= n_cell; k += blockDim.x) cell_count[k] = 0;
+    if (threadIdx.x < 2) overflow[threadIdx.x] = 0;
+    if (threadIdx.x == 0) tail_list[0] = 0;
 }
 
 // rebuild flag from the displacement; when set, also clears the counters the
@@ -363,17 +391,26 @@ __global__ void k_decide(unsigned long long* __restrict__ ctl, double limit2, in
     if (threadIdx.x == 0) tail_list[0] = 0;
 }
 
-// after a build: remember the positions and clear the flag
+// after a build: remember the positions; the last workgroup to finish (a
+// completion count in ctl[4]) clears the flag, once every workgroup has read it
 __global__ void __launch_bounds__(256)
 k_mark_built(const double* __restrict__ r, double* __restrict__ r_build, int n3,
              unsigned long long* __restrict__ ctl) {
-    if (!ctl[1]) return;
+    __shared__ int s_flag;
+    if (threadIdx.x == 0) s_flag = ctl[1] != 0ull;
+    __syncthreads();
+    if (!s_flag) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n3) r_build[k] = r[k];
-}
-
-__global__ void k_clear_flag(unsigned long long* __restrict__ ctl) {
-    if (threadIdx.x == 0 && ctl[1]) { ctl[1] = 0ull; ctl[2] += 1ull; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctl[4], 1ull) == (unsigned long long)(gridDim.x - 1)) {
+            ctl[4] = 0ull;
+            ctl[1] = 0ull;
+            ctl[2] += 1ull;
+        }
+    }
 }
 
 __global__ void k_set_flag(unsigned long long* __restrict__ ctl) {
@@ -459,7 +496,6 @@ static int nb_launch(capac_ctx* c) {
                                             (const Stencil*)c->stencil, c->nb_stride, c->n_front,
                                             c->n_back, c->nbr, c->nb_overflow, c->tail_list, need);
     k_mark_built<<<(3 * N + 255) / 256, 256, 0, s>>>(c->r, c->r_build, 3 * N, c->nb_ctl);
-    k_clear_flag<<<1, 64, 0, s>>>(c->nb_ctl);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -476,11 +512,11 @@ int nb_build(capac_ctx* c) {
 int nb_build_if_needed(capac_ctx* c) {
     if (c->skin <= 0.0) return nb_build(c);
     const int N = c->bc.N;
-    k_disp<<<(N + 255) / 256, 256, 0, c->stream>>>(c->r, c->r_build, N, c->nb_ctl);
     // half the skin, less a relative 1e-6 so rounding in the distance tests
     // cannot matter at the boundary
     double half = 0.5 * c->skin * (1.0 - 1e-6);
-    k_decide<<<1, 256, 0, c->stream>>>(c->nb_ctl, half * half, c->cell_count, c->n_cell, c->nb_overflow,
-                                       c->tail_list);
+    k_disp_decide<<<(N + 255) / 256, 256, 0, c->stream>>>(c->r, c->r_build, N, c->nb_ctl, half * half,
+                                                          c->cell_count, c->n_cell, c->nb_overflow,
+                                                          c->tail_list);
     return nb_launch(c);
 }
