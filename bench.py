@@ -66,17 +66,21 @@ def spmv32_bytes(nm: int, n_mat: int) -> float:
     return 28.0 * nm * nm + 16.0 * n_mat
 
 
-def traffic_from_profiles(kernel: str):
+def traffic_from_profiles(kernel: str, algorithmic_bytes: float):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), as committed under
-    profiles/; None when no such summary exists."""
+    profiles/; None when no such summary exists or when it was measured on
+    another workload (its algorithmic bytes per launch differ)."""
     path = os.path.join(PROFILES, "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as fh:
             data = json.load(fh)
-        return data.get(kernel, {}).get("bytes_per_launch")
+        entry = data.get(kernel, {})
+        if int(entry.get("algorithmic_bytes", -1)) != int(algorithmic_bytes):
+            return None
+        return entry.get("bytes_per_launch")
     except Exception:
         return None
 
@@ -296,7 +300,7 @@ def main() -> int:
     roofline = {"kernel": "relay SpMV, " + ("k_spmv<RelayF64>" if best == "fp64" else "k_spmv<RelayC32>"),
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles("spmv_" + best),
+                "traffic": traffic_from_profiles("spmv_" + best, nbytes),
                 "algorithmic_bytes_per_launch": nbytes,
                 "ms_per_launch": spmv_ms}
     for name, p in paths.items():
