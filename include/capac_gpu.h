@@ -346,7 +346,13 @@ int capac_md_begin(capac_ctx* ctx);
  * step, velocity Verlet with RATTLE, force, NHC, centre-of-mass removal, and
  * the molecule wrap every n_save steps (force.c:127-165).  Stops early after
  * a step that is a multiple of n_save when stop_at_save is set, so the host
- * can write output; returns the number of steps done, or < 0. */
+ * can write output; returns the number of steps done, or < 0.
+ * Energies and observables (capac_md_observables) are those of the last step
+ * of the call or of a save step.  Other steps end without waiting for the
+ * device, so a RATTLE failure in such a step is reported (as the reference's
+ * error) during the next step's force call, before that step's second half,
+ * instead of at the end of the failing step; with no CPIM unknowns, or with
+ * the per-phase timers on, every step still ends with a synchronisation. */
 int capac_md_run(capac_ctx* ctx, int first_step, int n_steps, int n_save, int stop_at_save);
 
 /* Thermostat and step observables: eKsum, instantaneous T, the reference
