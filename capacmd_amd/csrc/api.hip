@@ -1318,6 +1318,7 @@ extern "C" int capac_checkpoint_load(capac_ctx* c, const void* buf, size_t bytes
     CHECK_HIP(hipMemcpyAsync(c->red + 64, h.red_md, sizeof h.red_md, hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     c->solve_iters = h.solve_iters;
+    c->batch_main = h.solve_iters;
     c->pred_order = h.pred_order;
     c->pred_count = h.pred_count; c->pred_pending = 0;
     c->first_temp = h.first_temp; c->ext_temp = h.ext_temp; c->cur_ref_temp = h.cur_ref_temp;

@@ -152,6 +152,8 @@ struct capac_ctx {
     size_t relay_bytes;
     Solver* solver;
     int solve_iters, solve_kind, solve_refine_rounds;
+    int batch_main, batch_main2, batch_corr;  // iterations of the last two main solves and the last
+                                              // correction solve (first batch sizes)
     double solve_crit;          // stop when rTr < solve_crit bTb (the reference: 1e-8)
 
     // fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC) and work space

@@ -591,11 +591,22 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
     Solver* S = c->solver;
     // first batch: the previous solve's iteration count (warm starts change
     // slowly), so few launches are wasted as no-ops
-    const int first = c->solve_iters > 1 ? c->solve_iters + 1 : 2;
+    // (the main solve's count alone: a refinement's correction solve has its
+    // own count, and adding it would launch that many no-op units).  In MD
+    // with the predictor the count swings from step to step (12, 18, 12, 18,
+    // ... on Au561), so the mean of the last two counts, plus one, sizes the
+    // batch: on a recorded 117-step sequence that left 1.7 no-op units and
+    // 0.4 extra synchronisations per solve, against 2.6 and 0.55 for the last
+    // count plus one
+    int first = 2;
+    if (c->batch_main > 1)
+        first = (c->batch_main2 > 1 ? (c->batch_main + c->batch_main2) / 2 : c->batch_main) + 1;
     const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
     SolveOut o;
     int rc = bicgstab(c, b, x, first, refine, &o);
     if (rc) return rc;
+    c->batch_main2 = c->batch_main;
+    c->batch_main = o.iters;
     c->solve_refine_rounds = 0;
     if (refine && (o.kind == CAPAC_SOLVE_CONVERGED || o.kind == CAPAC_SOLVE_LOOSENED)) {
         // FP64 refinement: the reference's criterion on the true residual,
@@ -607,7 +618,9 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
             const int n = S->n_mat;
             k_zero<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d);
             SolveOut od;
-            if ((rc = bicgstab(c, S->rr, S->d, 4, false, &od))) return rc;
+            const int first_corr = c->batch_corr > 0 ? c->batch_corr + 1 : 4;
+            if ((rc = bicgstab(c, S->rr, S->d, first_corr, false, &od))) return rc;
+            c->batch_corr = od.iters;
             total += od.iters;
             k_axpy1<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d, x);
             if ((rc = residual64(c, b, x, &rTr))) return rc;
