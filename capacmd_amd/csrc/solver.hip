@@ -589,10 +589,10 @@ static void record(capac_ctx* c, const SolveOut& o) {
 // returns CAPAC_ERR_SOLVER when the reference would exit (bicg_stab.c:563-569).
 int solver_run(capac_ctx* c, const double* b, double* x) {
     Solver* S = c->solver;
-    // first batch: the previous solve's iteration count (warm starts change
-    // slowly), so few launches are wasted as no-ops
-    // (the main solve's count alone: a refinement's correction solve has its
-    // own count, and adding it would launch that many no-op units).  In MD
+    // first batch: from the previous solves' iteration counts (warm starts
+    // change slowly), so few launches are wasted as no-ops; the main solves'
+    // counts alone (a refinement's correction solve has its own count, and
+    // adding it would launch that many no-op units).  In MD
     // with the predictor the count swings from step to step (12, 18, 12, 18,
     // ... on Au561), so the mean of the last two counts, plus one, sizes the
     // batch: on a recorded 117-step sequence that left 1.7 no-op units and
