@@ -695,10 +695,13 @@ static void mark(capac_ctx* c, int k) {
 // early.  Measured on MI355X (ms per MD step): Au561 FP32-refined with the
 // quadratic predictor 0.877 at 256 workgroups, 0.827 at 384, 0.965 at 512,
 // 0.925 at 768; FP64 1.589 at 256, 1.603 at 384, 1.758 at 512; Au2057
-// FP32-refined 3.42 at 634, 3.21 at 2048.
+// FP32-refined 3.42 at 634, 3.21 at 2048.  Once the solve had shortened
+// further (0.33 ms per step), 384 workgroups left the pair kernel ~12 us
+// behind it; 448 let it finish in time: 0.562-0.571 against 0.580-0.581 ms
+// per Au561 step on one box (320: 0.584-0.592).
 static int side_pair_blocks(const capac_ctx* c) {
     if (c->bc.Nm >= 1024) return 0;
-    return c->pred_order > 0 ? 384 : 256;
+    return c->pred_order > 0 ? 448 : 256;
 }
 
 // pair and Sutton-Chen forces on the side stream, after the rows are built
