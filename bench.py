@@ -1,37 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark: ns/day and CPIM solve ms/step of the MI355X force + CPIM path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
-                    [--precision fp64|fp32|both] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3]
+                    [--equil E] [--precision fp64|fp32|both] [--no-cpu] [--no-error]
 
-Workload (default): Au561 Mackay icosahedron (6 shells) in a 6 nm box with
-SPC/E water at 1.0 g/cm^3 (the seeded config-1 deck of capacmd_amd.gen),
-Sutton-Chen gold, DSF water electrostatics (alpha 2 nm^-1, rc 1.2 nm), CPIM
-on the gold (2,245 unknowns), Nose-Hoover NVT, dt = 1 fs.  A "step" is one
-full MD step on the device: NHC, velocity Verlet + RATTLE, neighbour build,
-pair/Sutton-Chen/CPIM forces with the BiCGSTAB solve, the molecule wrap at
-the save cadence.  Each path runs with its default warm start: the previous
-solution for FP64 (the reference's), quadratic extrapolation of the last
-three solutions for FP32-refined (same stop test, fewer iterations); the
-FP32-refined path is also timed with the previous solution as its start.  Inputs are resident in HBM before the timed region; the
-library returns only after each step's work has finished (it synchronises on
-the solver's convergence flag), so wall time around the K steps is device
-time.  ns/day = 0.0864 / (seconds per step) at dt = 1 fs.
+Workload (default): Au2057 Mackay icosahedron (9 shells) in a 10 nm box with
+SPC/E water at 1.0 g/cm^3 (the seeded config-3 deck of capacmd_amd.gen:
+98,189 atoms, 8,229 CPIM unknowns), Sutton-Chen gold, DSF water
+electrostatics (alpha 2 nm^-1, rc 1.2 nm), CPIM on the gold, Nose-Hoover NVT
+at 300 K, dt = 1 fs.  A "step" is one full MD step on the device: NHC,
+velocity Verlet + RATTLE, neighbour build, pair/Sutton-Chen/CPIM forces with
+the BiCGSTAB solve, the molecule wrap at the save cadence.
 
-Prints one JSON line (rank 0; there is one GPU).  The roofline object is for
-the relay SpMV, the kernel inside the solve (timed live with HIP events on
-the library's stream); cpu_baseline times the reference program built from
-its own sources (oracle/_ref, one MPI rank on one core) on a one-step sample
-of the same deck, or the oracle restatement where that binary cannot run.
+Before anything is timed the deck is equilibrated for E MD steps
+(FP32-refined path) so that the timed window runs near the thermostat's
+300 K; both precisions then start from that state, thermostat included.
+Each path runs with its default warm start: the previous solution for FP64
+(the reference's), quadratic extrapolation of the last three solutions for
+FP32-refined (same stop test, fewer iterations); the FP32-refined path is
+also timed with the previous solution as its start.  Inputs are resident in
+HBM before the timed region; the library returns only after the last step's
+work has finished, so wall time around the K steps is device time.
+ns/day = 0.0864 / (seconds per step) at dt = 1 fs.
+
+Prints one JSON line (there is one GPU).  The roofline object is for the
+relay SpMV, the kernel that takes the largest share of the MD step's kernel
+time, timed live with HIP events on the library's stream.  cpu_baseline
+times the oracle restatement (OpenMP over the reference's decomposition
+units, every core the process may use) on two MD steps from the same
+equilibrated state and warm start.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import re
 import shutil
-import subprocess
 import sys
 import tempfile
 import time
@@ -52,18 +56,10 @@ def log(*a):
 
 
 def spmv_bytes(nm: int, n_mat: int, w: int) -> float:
-    """Algorithmic bytes of one FP64 relay SpMV: the unique entries of the
-    symmetric relay, w (8 N_m^2 - 3 N_m), plus 16 n_mat for the vectors."""
+    """Algorithmic bytes of one relay SpMV: the unique entries of the
+    symmetric relay, w (8 N_m^2 - 3 N_m), plus 16 n_mat for the vectors
+    (w = 8 FP64, 4 FP32)."""
     return w * (8.0 * nm * nm - 3.0 * nm) + 16.0 * n_mat
-
-
-def spmv32_bytes(nm: int, n_mat: int) -> float:
-    """Bytes one FP32-refined SpMV must read: its relay format, 7 FP32
-    coefficients per ordered metal pair (28 N_m^2 with the zero diagonal
-    slots), plus 16 n_mat for the vectors.  This is below the unique-entry
-    count of the FP32 matrix, 4 (8 N_m^2 - 3 N_m), because the format stores
-    the block's generators rather than its 16 entries."""
-    return 28.0 * nm * nm + 16.0 * n_mat
 
 
 def traffic_from_profiles(kernel: str, algorithmic_bytes: float):
@@ -85,17 +81,60 @@ def traffic_from_profiles(kernel: str, algorithmic_bytes: float):
         return None
 
 
-def run_gpu(deck, precision: int, steps: int, warmup: int, predictor=None, kernels=True):
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def equilibrate(deck, steps: int):
+    """MD on the FP32-refined path from the deck's start; returns the state
+    (r, v, vec_pq), the thermostat chain and the temperature at the end."""
+    ctx = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
+    ctx.force()
+    ctx.md_begin()
+    n_save = max(deck.n_save, 1)
+    t0 = time.perf_counter()
+    if steps:
+        done = ctx.md_run(1, steps, n_save)
+        assert done == steps, done
+    t1 = time.perf_counter()
+    r, v, x = ctx.get_state()
+    obs = ctx.observables()
+    ctx.close()
+    log(f"equilibration: {steps} steps in {t1 - t0:.1f} s, T = {obs['temp']:.1f} K")
+    return {"r": r, "v": v, "x": x, "vq": obs["vq"], "temp": obs["temp"], "steps": steps}
+
+
+def start_context(deck, precision: int, eq, predictor=None):
     ctx = lib.Context(deck, precision=precision)
+    ctx.set_state(r=eq["r"], v=eq["v"], x=eq["x"])
+    ctx.set_thermostat(eq["vq"])
     if predictor is not None:
         ctx.set_predictor(predictor)
     ctx.force()
     ctx.md_begin()
+    return ctx
+
+
+def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, kernels=True):
+    ctx = start_context(deck, precision, eq, predictor)
     n_save = max(deck.n_save, 1)
     step = 1
     if warmup:
-        done = ctx.md_run(step, warmup, n_save)
-        step += done
+        step += ctx.md_run(step, warmup, n_save)
+    t_start = ctx.observables()["temp"]
     # the timed steps run as the product runs them (no per-phase timers, so
     # unobserved steps end without a host synchronisation)
     builds0 = ctx.nb_builds()
@@ -105,6 +144,7 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, predictor=None, kerne
     assert done == steps, done
     builds = ctx.nb_builds() - builds0
     step += done
+    t_end = ctx.observables()["temp"]
     # the phase breakdown from as many further steps with the library's
     # HIP-event timers on (they synchronise at every step's end)
     ctx.reset_timing()
@@ -112,7 +152,6 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, predictor=None, kerne
     done = ctx.md_run(step, steps, n_save)
     assert done == steps, done
     t, calls, iters = ctx.timing()
-    obs = ctx.observables()
     ctx.enable_timing(False)
     sec = (t1 - t0) / steps
     out = {
@@ -130,132 +169,109 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, predictor=None, kerne
         "nb_builds_per_step": builds / steps,
         "nb_skin_nm": ctx.skin,
         "warm_start_predictor": ctx.predictor,
-        "final_temp": obs["temp"],
-        "final_energy": float(obs["pot"][0]),
+        "temp_start_of_timed_window": t_start,
+        "temp_end_of_timed_window": t_end,
     }
     if not kernels:
         ctx.close()
         return out
     kern = {}
-    kernels = [("spmv", lib.KERNEL_SPMV), ("assemble", lib.KERNEL_ASSEMBLE),
-               ("pair", lib.KERNEL_PAIR), ("neighbour", lib.KERNEL_NEIGHBOUR),
-               ("cpim_mm", lib.KERNEL_CPIM_MM)]
+    klist = [("spmv", lib.KERNEL_SPMV), ("assemble", lib.KERNEL_ASSEMBLE),
+             ("pair", lib.KERNEL_PAIR), ("neighbour", lib.KERNEL_NEIGHBOUR),
+             ("cpim_mm", lib.KERNEL_CPIM_MM)]
     if precision == lib.PREC_FP32_REFINED:
-        kernels.append(("apply64", lib.KERNEL_APPLY64))
-    kernels.append(("solve", lib.KERNEL_SOLVE))       # last: from the last MD step's warm start
-    for name, kid in kernels:
+        klist.append(("apply64", lib.KERNEL_APPLY64))
+    klist.append(("solve", lib.KERNEL_SOLVE))       # last: from the last MD step's warm start
+    for name, kid in klist:
         kern[name] = ctx.time_kernel(kid, 10 if kid == lib.KERNEL_SOLVE else 20)
     out["kernel_ms"] = kern
     ctx.close()
     return out
 
 
-def fp32_error(deck, states: int = 3, stride: int = 20, crit: float = 1e-8):
-    """Per-step error of the FP32-refined path against the FP64 path.
+def rel(a, b):
+    den = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (den if den > 0 else 1.0)
 
-    The FP64 path runs MD; at `states` points `stride` steps apart, the
-    FP32-refined path evaluates one force call from the identical state and
-    the identical warm start (the FP64 solution of the step before).  Both
-    solves stop at the reference's criterion (rTr < 1e-8 bTb), so part of
-    the difference is the solver tolerance itself; the FP64 path agrees with
-    the CPU oracle to 1e-10 at a given solution (tests/test_gpu_parity.py).
-    Returns the largest relative errors (2-norms) of the induced charges,
-    dipoles, all forces, metal forces and the potential energy."""
-    c64 = lib.Context(deck, precision=lib.PREC_FP64)
-    c32 = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
-    c64.set_solve_tolerance(crit)
-    c32.set_solve_tolerance(crit)
-    nm, n = deck.n_metal, deck.n_atoms
-    worst = {"q": 0.0, "p": 0.0, "f": 0.0, "f_metal": 0.0, "epot": 0.0}
 
-    def rel(a, b):
-        den = float(np.linalg.norm(b))
-        return float(np.linalg.norm(a - b)) / (den if den > 0 else 1.0)
-
+def error_vs_oracle(deck, eq, steps: int = 20):
+    """Error of one force call of each GPU path against the FP64 oracle
+    (OpenMP build) from an identical state and warm start: the state after
+    `steps` FP64 MD steps from the equilibrated start, warm start the
+    solution of the step before.  All three solves stop at the reference's
+    criterion rTr < 1e-8 bTb, so part of each error is the solver tolerance
+    itself (two valid solves may stop that far apart)."""
+    from oracle.oracle import Oracle
+    c64 = start_context(deck, lib.PREC_FP64, eq)
+    n_save = max(deck.n_save, 1)
     try:
-        c64.force()
-        c64.md_begin()
-        step = 1
-        for _ in range(states):
-            if stride > 1:
-                c64.md_run(step, stride - 1, 10 ** 9)
-                step += stride - 1
-            r_prev, v_prev, x_prev = c64.get_state()
-            c64.md_run(step, 1, 10 ** 9)
-            step += 1
-            r, v, x64 = c64.get_state()
-            f64 = c64.forces()
-            e64 = float(np.sum(c64.observables()["pot"][1:]))
-            c32.set_state(r=r, v=v, x=x_prev)
-            f32, pot32, _ = c32.force()
-            _, _, x32 = c32.get_state()
-            e32 = float(np.sum(pot32[1:]))
-            F64, F32 = f64.reshape(3, n), f32.reshape(3, n)
-            errs = {"q": rel(x32[3 * nm:4 * nm], x64[3 * nm:4 * nm]),
-                    "p": rel(x32[:3 * nm], x64[:3 * nm]),
-                    "f": rel(f32, f64),
-                    "f_metal": rel(F32[:, :nm], F64[:, :nm]),
-                    "epot": abs(e32 - e64) / max(abs(e64), 1e-300)}
-            for k, e in errs.items():
-                worst[k] = max(worst[k], e)
+        if steps > 1:
+            c64.md_run(1, steps - 1, n_save)
+        _, _, x_prev = c64.get_state()          # the solution of step `steps` - 1
+        c64.md_run(steps, 1, n_save)
+        r, v, _ = c64.get_state()               # the positions of step `steps`
     finally:
         c64.close()
-        c32.close()
-    return {k: float("%.3e" % v) for k, v in worst.items()} | {
-        "states": states, "stride_steps": stride, "solve_crit": crit}
-
-
-def cpu_reference(deck_dir: str, steps: int = 1):
-    """The reference program on the host: one MPI rank, `steps` MD steps."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "capacmd_ref")
-    if not os.path.exists(exe):
-        return None
-    with tempfile.TemporaryDirectory() as tmp:
-        for f in ("init.gro", "param.txt", "mdset.txt"):
-            shutil.copy(os.path.join(deck_dir, f), tmp)
-        mds = os.path.join(tmp, "mdset.txt")
-        txt = open(mds).read()
-        txt = re.sub(r"number_of_steps\s+\d+", f"number_of_steps   {steps}", txt)
-        txt = re.sub(r"save_step\s+\d+", "save_step         1000000", txt)
-        open(mds, "w").write(txt)
-        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
-        try:
-            res = subprocess.run([exe, "-gro", "init.gro", "-par", "param.txt", "-mds", "mdset.txt"],
-                                 cwd=tmp, env=env, capture_output=True, text=True, timeout=300)
-        except Exception as e:  # noqa: BLE001
-            log("reference program did not run:", e)
-            return None
-        if res.returncode != 0:
-            log("reference program failed:", res.stdout[-500:], res.stderr[-500:])
-            return None
-        m = re.search(r"Total time used in force:\s+([0-9.]+)", res.stdout)
-        s = re.search(r"Time used in CPIM-solve:\s+([0-9.]+)", res.stdout)
-        if not m:
-            return None
-        calls = steps + 1                       # the initial force call + one per step
-        sec = float(m.group(1)) / calls
-        return {"value": 0.0864 / sec, "unit": "ns/day", "cores": 1, "kind": "reference",
-                "ms_per_step": 1e3 * sec,
-                "solve_ms_per_step": 1e3 * float(s.group(1)) / calls if s else None,
-                "sample": f"reference program (oracle/_ref/capacmd_ref, 1 MPI rank), same deck, "
-                          f"{steps} MD step(s) = {calls} force calls incl. the first solve from zero"}
-
-
-def cpu_port(deck, steps: int = 1):
-    """The oracle restatement on one core, same sample."""
-    from oracle.oracle import Oracle
-    o = Oracle(deck)
+    nm, n = deck.n_metal, deck.n_atoms
+    o = Oracle(deck, omp=True)
+    o.set_state(r, v, x_prev)
     t0 = time.perf_counter()
-    o.force()
+    rc, fo, po, _, _ = o.force()
+    t_or = time.perf_counter() - t0
+    _, _, xo, _ = o.get_state()
+    it_o = o.solve_stats()[0]
+    o.close()
+    eo = float(np.sum(po[1:]))
+    Fo = fo.reshape(3, n)
+    out = {"note": "one force call from an identical state and warm start; errors are relative "
+                   "2-norms against the FP64 oracle (OpenMP build, every pair: the corrected "
+                   "pair count), all solves at the reference's stop test rTr < 1e-8 bTb",
+           "oracle_iterations": it_o, "oracle_seconds": round(t_or, 2)}
+    for name, prec in (("fp32_refined", lib.PREC_FP32_REFINED), ("fp64", lib.PREC_FP64)):
+        c = lib.Context(deck, precision=prec)
+        try:
+            c.set_state(r=r, v=v, x=x_prev)
+            f, pot, _ = c.force()
+            _, _, x = c.get_state()
+            it = c.solve_stats()[0]
+        finally:
+            c.close()
+        F = f.reshape(3, n)
+        out[name] = {"q": float("%.3e" % rel(x[3 * nm:4 * nm], xo[3 * nm:4 * nm])),
+                     "p": float("%.3e" % rel(x[:3 * nm], xo[:3 * nm])),
+                     "f": float("%.3e" % rel(f, fo)),
+                     "f_metal": float("%.3e" % rel(F[:, :nm], Fo[:, :nm])),
+                     "epot": float("%.3e" % (abs(float(np.sum(pot[1:])) - eo) / max(abs(eo), 1e-300))),
+                     "iterations": it}
+    return out
+
+
+def cpu_port(deck, eq, steps: int = 2):
+    """The oracle restatement with OpenMP over the reference's own
+    decomposition units (i atoms, metal rows, row-groups), every usable core,
+    `steps` MD steps from the equilibrated state with its warm start."""
+    from oracle.oracle import Oracle
+    info = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", info["usable"] or 1))
+    o = Oracle(deck, omp=True)
+    o.set_state(eq["r"], eq["v"], eq["x"])
+    o.set_thermostat(eq["vq"])
+    o.force()                       # the start state's forces (not timed), as capac_md_begin needs
     o.md_begin()
+    t_base = o.timers()
+    t0 = time.perf_counter()
     for s in range(1, steps + 1):
         o.md_step(s, 10 ** 9)
-    sec = (time.perf_counter() - t0) / (steps + 1)
-    t = o.timers()
-    return {"value": 0.0864 / sec, "unit": "ns/day", "cores": 1, "kind": "port",
-            "ms_per_step": 1e3 * sec, "solve_ms_per_step": 1e3 * t[7] / (steps + 1),
-            "sample": f"oracle restatement, 1 core, same deck, {steps} MD step(s) = "
-                      f"{steps + 1} force calls"}
+    sec = (time.perf_counter() - t0) / steps
+    t = o.timers() - t_base
+    o.close()
+    return {"value": 0.0864 / sec, "unit": "ns/day", "cores": threads, "kind": "port",
+            "ms_per_step": 1e3 * sec, "solve_ms_per_step": 1e3 * t[7] / steps,
+            "nonbonded_ms_per_step": 1e3 * t[4] / steps,
+            "nproc": info["nproc"], "cpu_model": info["model"],
+            "sample": f"oracle restatement (oracle/cpu_ref.c, OpenMP build, {threads} threads, "
+                      f"brute-force pair scan with every pair counted), {steps} MD steps of the same "
+                      f"deck from the equilibrated state, warm start the previous solution"}
 
 
 def main() -> int:
@@ -263,9 +279,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--equil", type=int, default=1500,
+                    help="untimed MD steps before the paths start (thermal equilibration)")
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -276,13 +295,14 @@ def main() -> int:
     gen.write_deck(deck_spec, tmp)
     deck = lib.Deck.from_dir(tmp)
     log(f"deck {deck_spec.name}: N={deck.n_atoms} Nm={deck.n_metal} n_mat={deck.n_mat}")
+    eq = equilibrate(deck, a.equil)
 
     paths = {}
     precs = {"fp64": lib.PREC_FP64, "fp32": lib.PREC_FP32_REFINED}
     wanted = ["fp64", "fp32"] if a.precision == "both" else [a.precision]
     for name in wanted:
         try:
-            paths[name] = run_gpu(deck, precs[name], a.steps, a.warmup)
+            paths[name] = run_gpu(deck, precs[name], a.steps, a.warmup, eq)
             log(name, json.dumps(paths[name]))
         except lib.CapacError as e:
             log(f"{name} path unavailable: {e}")
@@ -294,7 +314,7 @@ def main() -> int:
         # for the predictor's share of the speed
         try:
             paths["fp32_previous_solution_start"] = run_gpu(deck, lib.PREC_FP32_REFINED, a.steps, a.warmup,
-                                                            predictor=0, kernels=False)
+                                                            eq, predictor=0, kernels=False)
             log("fp32 with the previous solution as start", json.dumps(paths["fp32_previous_solution_start"]))
         except lib.CapacError as e:
             log(f"fp32 run with predictor 0 failed: {e}")
@@ -302,43 +322,43 @@ def main() -> int:
     main_path = paths[best]
 
     nm, n_mat = deck.n_metal, deck.n_mat
+    w = 8 if best == "fp64" else 4
     spmv_ms = main_path["kernel_ms"]["spmv"]
-    nbytes = spmv_bytes(nm, n_mat, 8) if best == "fp64" else spmv32_bytes(nm, n_mat)
+    nbytes = spmv_bytes(nm, n_mat, w)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"kernel": "relay SpMV, " + ("k_spmv<RelayF64>" if best == "fp64" else "k_spmv<RelayC32>"),
+    roofline = {"kernel": "relay SpMV (" + ("FP64" if best == "fp64" else "FP32") + " relay, one "
+                          "BiCGSTAB half-iteration)",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles("spmv_" + best, nbytes),
                 "algorithmic_bytes_per_launch": nbytes,
+                "algorithmic_bytes_formula": f"{w} (8 N_m^2 - 3 N_m) + 16 n_mat, N_m = {nm}, n_mat = {n_mat}",
                 "ms_per_launch": spmv_ms}
     for name, p in paths.items():
-        if "kernel_ms" not in p:
-            continue
-        b = spmv_bytes(nm, n_mat, 8) if name == "fp64" else spmv32_bytes(nm, n_mat)
-        p["spmv_GBps"] = round(b / (p["kernel_ms"]["spmv"] * 1e-3) / 1e9, 1)
+        if "kernel_ms" in p:
+            b = spmv_bytes(nm, n_mat, 8 if name == "fp64" else 4)
+            p["spmv_GBps"] = round(b / (p["kernel_ms"]["spmv"] * 1e-3) / 1e9, 1)
     speedup = None
-    err32 = None
     if "fp32" in paths and "fp64" in paths:
         # the solve alone (the reference's timer category 7), from the last MD
         # step's system and warm start; in MD the pair forces overlap it
         speedup = round(paths["fp64"]["kernel_ms"]["solve"] / paths["fp32"]["kernel_ms"]["solve"], 3)
-        # at the reference's stop test, and with both solves taken far below
-        # it, so that what remains is the FP32 relay's own error
-        err32 = {"note": "FP32-refined vs FP64 path from identical states and warm starts, max "
-                         "over states; solve_crit is the stop test rTr < crit bTb",
-                 "reference_tolerance": None, "tight_tolerance": None}
-        for key, crit in (("reference_tolerance", 1e-8), ("tight_tolerance", 1e-11)):
-            try:
-                err32[key] = fp32_error(deck, crit=crit)
-            except lib.CapacError as e:
-                log(f"fp32 error measurement at crit {crit} failed: {e}")
-        log("fp32-refined error vs fp64:", json.dumps(err32))
+
+    err = None
+    if not a.no_error and "fp32" in paths:
+        try:
+            err = error_vs_oracle(deck, eq)
+            log("error vs the FP64 oracle:", json.dumps(err))
+        except (lib.CapacError, RuntimeError, OSError) as e:
+            log(f"error measurement failed: {e}")
 
     cpu = None
-    if not a.no_cpu and deck.n_atoms <= 46341:
-        cpu = cpu_reference(tmp, 1)
-        if cpu is None:
-            cpu = cpu_port(deck, 1)
+    if not a.no_cpu:
+        try:
+            cpu = cpu_port(deck, eq)
+            log("cpu baseline:", json.dumps(cpu))
+        except (RuntimeError, OSError) as e:
+            log(f"cpu baseline failed: {e}")
     shutil.rmtree(tmp, ignore_errors=True)
 
     line = {
@@ -353,14 +373,18 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "f32-matrix/f64-refined",
-        "data": "synthetic (seeded generator: Mackay icosahedron + SPC/E lattice water)",
+        "data": "synthetic (seeded generator: Mackay icosahedron + SPC/E lattice water), "
+                f"equilibrated {eq['steps']} MD steps before timing",
         "config": {"workload": f"{deck_spec.name}: Au{nm} + {deck_spec.n_water} SPC/E waters, "
                                f"{deck_spec.box[0]:g} nm box, {n_mat} CPIM unknowns, MD step dt 1 fs",
-                   "atoms": deck.n_atoms, "path": best},
+                   "atoms": deck.n_atoms, "path": best, "equilibration_steps": eq["steps"],
+                   "temp_after_equilibration": round(eq["temp"], 2)},
         "solve_ms_per_step": round(main_path["solve_ms_per_step"], 4),
         "solve_ms_alone": round(main_path["kernel_ms"]["solve"], 4),
         "solve_speedup_fp32_refined_vs_fp64": speedup,
-        "fp32_refined_error": err32,
+        "temp_timed_window": [round(main_path["temp_start_of_timed_window"], 2),
+                              round(main_path["temp_end_of_timed_window"], 2)],
+        "error_vs_fp64_oracle": err,
         "paths": paths,
         "roofline": roofline,
         "cpu_baseline": cpu,

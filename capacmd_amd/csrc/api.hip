@@ -753,13 +753,21 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
         c->nb_dirty = 0;
         mark(c, 1);
-        if ((rc = side_forces(c))) return rc;
+        // A full-grid pair launch takes every compute unit for its whole
+        // duration, and launches queued behind it on the main stream wait
+        // for free slots: at Au2057 the 14 us RHS launch measured 450 us on
+        // average.  So the full-grid pair kernel starts once the RHS and the
+        // assembly are done and overlaps only the solve; a capped grid
+        // leaves slots free and starts at once.
+        const bool side_after_matrix = c->n_mat && side_pair_blocks(c) == 0;
+        if (!side_after_matrix && (rc = side_forces(c))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {
             if ((rc = cpim_rhs(c))) return rc;
             mark(c, 4);
             if ((rc = cpim_assemble(c))) return rc;
             mark(c, 5);
+            if (side_after_matrix && (rc = side_forces(c))) return rc;
             solve_rc = solver_run(c, c->b, c->x);
             if (solve_rc && solve_rc != CAPAC_ERR_SOLVER) {
                 (void)hipStreamSynchronize(c->side);

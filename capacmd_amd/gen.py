@@ -54,6 +54,7 @@ CPIM_POLAR = 1.5 * (NM2BOHR / 10.0) ** 3
 CPIM_CAPAC = 2.0 * (NM2BOHR / 10.0)
 
 AU_NN = 0.288          # nm, nearest-neighbour distance along icosahedron edges
+CONTACTS = (0.18, 0.16)  # nm, smallest intermolecular H-H and O-H distances of the benchmark decks
 WATER_DENSITY = 1.0    # g/cm^3
 AVOGADRO = 6.02214129e23
 
@@ -185,10 +186,53 @@ class Deck:
                                np.tile([M_O, M_H, M_H], self.n_water)])
 
 
+def _untangle(centres: np.ndarray, tmpl: np.ndarray, R: np.ndarray, box: np.ndarray,
+              rng: np.random.Generator, hh_min: float, oh_min: float,
+              max_rounds: int = 400) -> np.ndarray:
+    """Redraws the orientation of molecules whose atoms come closer to another
+    molecule's than hh_min (H-H) or oh_min (O-H), periodic images included,
+    until no such contact is left.  Of each close pair the molecule with the
+    larger index turns, so the result depends on the seed only."""
+    from scipy.spatial import cKDTree
+    n = len(centres)
+    mol = np.repeat(np.arange(n), 3)
+    is_h = np.tile([False, True, True], n)
+    reach = max(hh_min, oh_min)
+    moved = None                        # atoms to check: all at first, then the turned molecules'
+    for _ in range(max_rounds):
+        mols = centres[:, None, :] + np.einsum("nij,aj->nai", R, tmpl)
+        w = np.mod(mols, box).reshape(-1, 3)
+        tree = cKDTree(w, boxsize=box)
+        if moved is None:
+            pairs = tree.query_pairs(reach, output_type="ndarray")
+            a, b = pairs[:, 0], pairs[:, 1]
+        else:
+            hits = tree.query_ball_point(w[moved], reach)
+            a = np.repeat(moved, [len(h) for h in hits])
+            b = np.fromiter((j for h in hits for j in h), dtype=np.int64, count=len(a))
+        d = w[a] - w[b]
+        d -= box * np.round(d / box)
+        r = np.sqrt(np.einsum("ij,ij->i", d, d))
+        hh = is_h[a] & is_h[b]
+        oh = is_h[a] != is_h[b]
+        close = (mol[a] != mol[b]) & ((hh & (r < hh_min)) | (oh & (r < oh_min)))
+        if not close.any():
+            return mols
+        idx = np.unique(np.maximum(mol[a[close]], mol[b[close]]))
+        R[idx] = random_rotations(rng, len(idx))
+        moved = (3 * idx[:, None] + np.arange(3)[None, :]).ravel()
+    raise RuntimeError("water orientations: close contacts remain after %d rounds" % max_rounds)
+
+
 def fill_water(box: Sequence[float], metal: np.ndarray, rng: np.random.Generator,
-               exclusion: float = 0.25) -> np.ndarray:
+               exclusion: float = 0.25, hh_min: float = 0.0, oh_min: float = 0.0) -> np.ndarray:
     """SPC/E waters on a cubic lattice at 1.0 g/cm^3 with random orientations.
 
+    With hh_min / oh_min > 0 the orientations are redrawn until no two
+    molecules have H-H closer than hh_min or O-H closer than oh_min (nm):
+    lattice water with independent random orientations puts some hydrogens
+    of neighbouring molecules ~0.11 nm apart, and those contacts heat the
+    box far above the thermostat's temperature in the first steps.
     Molecules with any atom within `exclusion` nm of a metal atom are removed.
     """
     box = np.asarray(box, dtype=float)
@@ -209,17 +253,16 @@ def fill_water(box: Sequence[float], metal: np.ndarray, rng: np.random.Generator
     com = (tmpl * np.array([M_O, M_H, M_H])[:, None]).sum(0) / mw
     tmpl = tmpl - com
     R = random_rotations(rng, len(centres))
-    mols = centres[:, None, :] + np.einsum("nij,aj->nai", R, tmpl)
+    if hh_min > 0.0 or oh_min > 0.0:
+        mols = _untangle(centres, tmpl, R, box, rng, hh_min, oh_min)
+    else:
+        mols = centres[:, None, :] + np.einsum("nij,aj->nai", R, tmpl)
     if len(metal):
-        keep = np.ones(len(mols), dtype=bool)
-        flat = mols.reshape(-1, 3)
-        for start in range(0, len(flat), 4096):
-            blk = flat[start:start + 4096]
-            d = blk[:, None, :] - metal[None, :, :]
-            d -= box * np.round(d / box)
-            close = (np.einsum("ijk,ijk->ij", d, d) < exclusion ** 2).any(axis=1)
-            idx = np.nonzero(close)[0] + start
-            keep[idx // 3] = False
+        from scipy.spatial import cKDTree
+        flat = np.mod(mols.reshape(-1, 3), box)
+        dist, _ = cKDTree(np.mod(metal, box), boxsize=box).query(flat, distance_upper_bound=exclusion)
+        close = dist < exclusion
+        keep = ~close.reshape(-1, 3).any(axis=1)
         mols = mols[keep]
     return mols
 
@@ -233,8 +276,11 @@ def maxwell_boltzmann(masses: np.ndarray, temp: float, rng: np.random.Generator)
 
 def make_deck(name: str, box, clusters: Sequence[Tuple[int, Sequence[float]]],
               n_steps: int, seed: int, save_step: int = 10,
-              efield=(0.0, 0.0, 0.0), with_water: bool = True, **kw) -> Deck:
-    """clusters: list of (shells, centre) for each gold nanoparticle."""
+              efield=(0.0, 0.0, 0.0), with_water: bool = True,
+              contacts: Tuple[float, float] = (0.0, 0.0), **kw) -> Deck:
+    """clusters: list of (shells, centre) for each gold nanoparticle;
+    contacts: the (H-H, O-H) distances below which water orientations are
+    redrawn (fill_water), 0 for none."""
     box = np.asarray(box, dtype=float)
     metal_blocks, ranges = [], []
     first = 0
@@ -245,7 +291,8 @@ def make_deck(name: str, box, clusters: Sequence[Tuple[int, Sequence[float]]],
         first += len(pts)
     metal = np.concatenate(metal_blocks) if metal_blocks else np.zeros((0, 3))
     rng = np.random.default_rng(1000 + seed)
-    water = fill_water(box, metal, rng) if with_water else np.zeros((0, 3, 3))
+    water = (fill_water(box, metal, rng, hh_min=contacts[0], oh_min=contacts[1])
+             if with_water else np.zeros((0, 3, 3)))
     deck = Deck(name=name, box=box, metal=metal, np_ranges=ranges, water=water,
                 vel=np.zeros((0, 3)), n_steps=n_steps, save_step=save_step,
                 efield=tuple(efield), **kw)
@@ -263,24 +310,27 @@ def config(index: int) -> Deck:
        0.5 V/nm field along x, 500 steps
     3: Au2057 in a 10 nm box, ~33,000 waters, 1,000 steps
     4: Au309 in a 4.8 nm box, ~4,000 waters, 3,200 steps (frames every 50)
+
+    Water orientations are redrawn until no intermolecular H-H contact is
+    below 0.18 nm and no O-H contact below 0.16 nm (CONTACTS).
     """
     if index == 0:
-        return make_deck("cfg0_au147", [3.6] * 3, [(4, [1.8] * 3)], 100, 0)
+        return make_deck("cfg0_au147", [3.6] * 3, [(4, [1.8] * 3)], 100, 0, contacts=CONTACTS)
     if index == 1:
-        return make_deck("cfg1_au561", [6.0] * 3, [(6, [3.0] * 3)], 500, 1)
+        return make_deck("cfg1_au561", [6.0] * 3, [(6, [3.0] * 3)], 500, 1, contacts=CONTACTS)
     if index == 2:
         # Au309 vertex radius: 4 * 0.288 * 0.9511 nm; surface gap 1.0 nm
         rad = 4 * AU_NN * 0.9510565162951535
         half = rad + 0.5
         return make_deck("cfg2_2au309_field", [8.0, 5.0, 5.0],
                          [(5, [4.0 - half, 2.5, 2.5]), (5, [4.0 + half, 2.5, 2.5])],
-                         500, 2, efield=(0.5, 0.0, 0.0))
+                         500, 2, efield=(0.5, 0.0, 0.0), contacts=CONTACTS)
     if index == 3:
         return make_deck("cfg3_au2057", [10.0] * 3, [(9, [5.0] * 3)], 1000, 3,
-                         save_step=100)
+                         save_step=100, contacts=CONTACTS)
     if index == 4:
         return make_deck("cfg4_au309", [4.8] * 3, [(5, [2.4] * 3)], 3200, 4,
-                         save_step=50)
+                         save_step=50, contacts=CONTACTS)
     raise ValueError(f"unknown config {index}")
 
 

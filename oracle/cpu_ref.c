@@ -51,10 +51,13 @@ struct oref {
     double box[6];
     double rc, rc2, alpha, a2_sqrtPI, wolfConst, erfc_arCut, inv_rc12, inv_rc6;
     double inv_polar, inv_capac, inv_R_pp, inv_R_pq, inv_R_qq;
-    /* COO relay matrix in reference order */
+    /* COO relay matrix in reference order; group_start[i] is the first
+       entry of metal row-group i (its rows 3i..3i+2, 3Nm+i and its Lagrange
+       entries), group_start[Nm] the first of the nanoparticle rows */
     long nnz, cap;
     long *row, *col;
     double *val, *diag;
+    long* group_start;
     double *vec_ext, *vec_pq, *isd;
     double *Ax, *r0, *r, *pp, *v, *s, *t, *y, *z, *Kt, *K;
     /* state, 3N SoA */
@@ -148,7 +151,7 @@ oref* oref_create(const capac_params* p) {
 
 void oref_destroy(oref* o) {
     if (!o) return;
-    free(o->row); free(o->col); free(o->val); free(o->diag);
+    free(o->row); free(o->col); free(o->val); free(o->diag); free(o->group_start);
     free(o->vec_ext); free(o->vec_pq); free(o->isd);
     double* vecs[11] = {o->Ax, o->r0, o->r, o->pp, o->v, o->s, o->t, o->y, o->z, o->Kt, o->K};
     for (int i = 0; i < 11; ++i) free(vecs[i]);
@@ -720,7 +723,13 @@ void oref_vec_ext(oref* o, const double* r, double* b) {
 
 /* ------------------------------------------------------------ relay matrix */
 
-static inline void coo_push(oref* o, long row, long col, double v) {
+typedef struct {
+    long nnz, cap;
+    long *row, *col;
+    double* val;
+} coo_buf;
+
+static inline void coo_push(coo_buf* o, long row, long col, double v) {
     if (o->nnz >= o->cap) {
         long nc = o->cap ? 2 * o->cap : 1024;
         o->row = realloc(o->row, nc * sizeof(long));
@@ -735,80 +744,133 @@ static inline void coo_push(oref* o, long row, long col, double v) {
     ++o->nnz;
 }
 
-/* mpi_cpff_mat_relay_COO (cpff.c:205-519) at one rank, same element order */
-long oref_relay(oref* o, const double* r) {
+/* the entries of metal row-group i of mpi_cpff_mat_relay_COO (cpff.c:245-478),
+ * in the reference's push order */
+static void relay_group(oref* o, const double* r, int i, coo_buf* out) {
     const capac_params* p = &o->p;
     int N = o->N, Nm = o->Nm;
     const double *rx = r, *ry = r + N, *rz = r + 2 * N;
     static const double delta[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
-    double t0 = now();
-    o->nnz = 0;
-    for (int i = 0; i < Nm; ++i) {
-        for (int j = 0; j < Nm; ++j) {
-            double rv[3] = {0, 0, 0}, rij2 = 0, rij = 0;
-            double c1 = 0, c2 = 0, c3 = 0;
-            if (j != i) {
-                double x = rx[j] - rx[i], y = ry[j] - ry[i], z = rz[j] - rz[i];
-                pbc12(o->box, &x, &y, &z);
-                x *= R_NM2BOHR; y *= R_NM2BOHR; z *= R_NM2BOHR;
-                rij2 = dist2(x, y, z);
-                rij = sqrt(rij2);
-                rv[0] = x; rv[1] = y; rv[2] = z;
-                double e_pp = exp(-(rij2 * o->inv_R_pp * o->inv_R_pp));
-                double inv_Rpp3 = pow(o->inv_R_pp, 3.0);
-                c1 = 1.0 / pow(rij, 5.0) *
-                     (erf(rij * o->inv_R_pp) - 2.0 * R_INV_SQRT_PI * o->inv_R_pp * rij * e_pp);
-                c2 = 4.0 * R_INV_SQRT_PI * inv_Rpp3 / rij2 * e_pp;
-                double e_pq = exp(-(rij2 * o->inv_R_pq * o->inv_R_pq));
-                c3 = 1.0 / pow(rij, 3.0) *
-                     (erf(rij * o->inv_R_pq) - 2.0 * R_INV_SQRT_PI * o->inv_R_pq * rij * e_pq);
-            }
-            for (int a = 0; a < 3; ++a) {
-                long irow = i * 3 + a;
-                for (int b = 0; b < 3; ++b) {
-                    long icol = j * 3 + b;
-                    if (j == i) {
-                        if (b == a) {
-                            double el = o->inv_polar;
-                            o->diag[irow] = el;
-                            if (fabs(el) > R_THRSHD) coo_push(o, irow, icol, el);
-                        }
-                    } else {
-                        double el = -(c1 * (3.0 * rv[a] * rv[b] - delta[a][b] * rij2) - c2 * rv[a] * rv[b]);
-                        if (fabs(el) > R_THRSHD) coo_push(o, irow, icol, el);
-                    }
-                }
-                if (j != i) {
-                    double el = c3 * rv[a];
-                    if (fabs(el) > R_THRSHD) coo_push(o, irow, (long)Nm * 3 + j, el);
-                }
-            }
-            long irow = (long)Nm * 3 + i;
+    for (int j = 0; j < Nm; ++j) {
+        double rv[3] = {0, 0, 0}, rij2 = 0, rij = 0;
+        double c1 = 0, c2 = 0, c3 = 0;
+        if (j != i) {
+            double x = rx[j] - rx[i], y = ry[j] - ry[i], z = rz[j] - rz[i];
+            pbc12(o->box, &x, &y, &z);
+            x *= R_NM2BOHR; y *= R_NM2BOHR; z *= R_NM2BOHR;
+            rij2 = dist2(x, y, z);
+            rij = sqrt(rij2);
+            rv[0] = x; rv[1] = y; rv[2] = z;
+            double e_pp = exp(-(rij2 * o->inv_R_pp * o->inv_R_pp));
+            double inv_Rpp3 = pow(o->inv_R_pp, 3.0);
+            c1 = 1.0 / pow(rij, 5.0) *
+                 (erf(rij * o->inv_R_pp) - 2.0 * R_INV_SQRT_PI * o->inv_R_pp * rij * e_pp);
+            c2 = 4.0 * R_INV_SQRT_PI * inv_Rpp3 / rij2 * e_pp;
+            double e_pq = exp(-(rij2 * o->inv_R_pq * o->inv_R_pq));
+            c3 = 1.0 / pow(rij, 3.0) *
+                 (erf(rij * o->inv_R_pq) - 2.0 * R_INV_SQRT_PI * o->inv_R_pq * rij * e_pq);
+        }
+        for (int a = 0; a < 3; ++a) {
+            long irow = i * 3 + a;
             for (int b = 0; b < 3; ++b) {
-                if (j != i) {
-                    double el = -c3 * rv[b];
-                    if (fabs(el) > R_THRSHD) coo_push(o, irow, j * 3 + b, el);
+                long icol = j * 3 + b;
+                if (j == i) {
+                    if (b == a) {
+                        double el = o->inv_polar;
+                        o->diag[irow] = el;
+                        if (fabs(el) > R_THRSHD) coo_push(out, irow, icol, el);
+                    }
+                } else {
+                    double el = -(c1 * (3.0 * rv[a] * rv[b] - delta[a][b] * rij2) - c2 * rv[a] * rv[b]);
+                    if (fabs(el) > R_THRSHD) coo_push(out, irow, icol, el);
                 }
             }
-            if (j == i) {
-                double el = -o->inv_capac;
-                o->diag[irow] = el;
-                if (fabs(el) > R_THRSHD) coo_push(o, irow, (long)Nm * 3 + j, el);
-            } else {
-                double el = -erf(rij * o->inv_R_qq) / rij;
-                if (fabs(el) > R_THRSHD) coo_push(o, irow, (long)Nm * 3 + j, el);
-            }
-            if (j == 0) {
-                for (int k = 0; k < o->n_np; ++k)
-                    if (i >= p->np_first[k] && i <= p->np_last[k])
-                        coo_push(o, irow, (long)Nm * 4 + k, 1.0);
+            if (j != i) {
+                double el = c3 * rv[a];
+                if (fabs(el) > R_THRSHD) coo_push(out, irow, (long)Nm * 3 + j, el);
             }
         }
+        long irow = (long)Nm * 3 + i;
+        for (int b = 0; b < 3; ++b) {
+            if (j != i) {
+                double el = -c3 * rv[b];
+                if (fabs(el) > R_THRSHD) coo_push(out, irow, j * 3 + b, el);
+            }
+        }
+        if (j == i) {
+            double el = -o->inv_capac;
+            o->diag[irow] = el;
+            if (fabs(el) > R_THRSHD) coo_push(out, irow, (long)Nm * 3 + j, el);
+        } else {
+            double el = -erf(rij * o->inv_R_qq) / rij;
+            if (fabs(el) > R_THRSHD) coo_push(out, irow, (long)Nm * 3 + j, el);
+        }
+        if (j == 0) {
+            for (int k = 0; k < o->n_np; ++k)
+                if (i >= p->np_first[k] && i <= p->np_last[k])
+                    coo_push(out, irow, (long)Nm * 4 + k, 1.0);
+        }
     }
+}
+
+/* mpi_cpff_mat_relay_COO (cpff.c:205-519) at one rank, same element order.
+ * The OpenMP build gives each thread a contiguous range of row-groups
+ * (static schedule) and concatenates the threads' entries in thread order,
+ * which is the sequential order. */
+long oref_relay(oref* o, const double* r) {
+    const capac_params* p = &o->p;
+    int Nm = o->Nm;
+    double t0 = now();
+    coo_buf main = {0, o->cap, o->row, o->col, o->val};
+    if (!o->group_start) o->group_start = zalloc((size_t)Nm + 1, sizeof(long));
+#ifdef ORACLE_OMP
+    int nth = omp_get_max_threads();
+    coo_buf* tb = zalloc(nth, sizeof(coo_buf));
+    long* cnt = zalloc((size_t)Nm + 1, sizeof(long));
+#pragma omp parallel
+    {
+        coo_buf* b = &tb[omp_get_thread_num()];
+#pragma omp for schedule(static)
+        for (int i = 0; i < Nm; ++i) {
+            long before = b->nnz;
+            relay_group(o, r, i, b);
+            cnt[i] = b->nnz - before;
+        }
+    }
+    long total = 0;
+    for (int t = 0; t < nth; ++t) total += tb[t].nnz;
+    if (total > main.cap) {
+        main.row = realloc(main.row, total * sizeof(long));
+        main.col = realloc(main.col, total * sizeof(long));
+        main.val = realloc(main.val, total * sizeof(double));
+        if (!main.row || !main.col || !main.val) { fprintf(stderr, "oracle: out of memory\n"); exit(1); }
+        main.cap = total;
+    }
+    for (int t = 0; t < nth; ++t) {
+        memcpy(main.row + main.nnz, tb[t].row, tb[t].nnz * sizeof(long));
+        memcpy(main.col + main.nnz, tb[t].col, tb[t].nnz * sizeof(long));
+        memcpy(main.val + main.nnz, tb[t].val, tb[t].nnz * sizeof(double));
+        main.nnz += tb[t].nnz;
+        free(tb[t].row); free(tb[t].col); free(tb[t].val);
+    }
+    free(tb);
+    long acc = 0;
+    for (int i = 0; i < Nm; ++i) { o->group_start[i] = acc; acc += cnt[i]; }
+    o->group_start[Nm] = acc;
+    free(cnt);
+#else
+    for (int i = 0; i < Nm; ++i) {
+        o->group_start[i] = main.nnz;
+        relay_group(o, r, i, &main);
+    }
+    o->group_start[Nm] = main.nnz;
+#endif
     for (int k = 0; k < o->n_np; ++k)
         for (int i = 0; i < Nm; ++i)
             if (i >= p->np_first[k] && i <= p->np_last[k])
-                coo_push(o, (long)Nm * 4 + k, (long)Nm * 3 + i, 1.0);
+                coo_push(&main, (long)Nm * 4 + k, (long)Nm * 3 + i, 1.0);
+    o->nnz = main.nnz; o->cap = main.cap;
+    o->row = main.row; o->col = main.col; o->val = main.val;
     o->timers[5] += now() - t0;
     return o->nnz;
 }
@@ -828,10 +890,20 @@ void oref_relay_dense(const oref* o, double* dense) {
 
 /* ------------------------------------------------------------------ solver */
 
-/* mpi_mat_vec_COO (bicg_stab.c:46-63) */
+/* mpi_mat_vec_COO (bicg_stab.c:46-63).  A row-group's entries touch only its
+ * own rows, so the OpenMP build runs row-groups in parallel, each over its
+ * entries in COO order: every row sums in the sequential order. */
 static void spmv_coo(const oref* o, const double* x, double* Ax) {
     for (long i = 0; i < o->n_mat; ++i) Ax[i] = 0.0;
+#ifdef ORACLE_OMP
+    const long* gs = o->group_start;
+#pragma omp parallel for schedule(static)
+    for (int g = 0; g < o->Nm; ++g)
+        for (long k = gs[g]; k < gs[g + 1]; ++k) Ax[o->row[k]] += o->val[k] * x[o->col[k]];
+    for (long k = gs[o->Nm]; k < o->nnz; ++k) Ax[o->row[k]] += o->val[k] * x[o->col[k]];
+#else
     for (long k = 0; k < o->nnz; ++k) Ax[o->row[k]] += o->val[k] * x[o->col[k]];
+#endif
 }
 
 /* mpi_vec_vec (bicg_stab.c:121-178) at one rank */
@@ -915,8 +987,10 @@ out:
 /* -------------------------------------------------------------- CPIM force */
 
 /* mpi_cpff_force (cpff.c:526-950) at one rank */
-void oref_cpff_force(oref* o, const double* r, const double* vec_pq, double* f, double* pot,
-                     double* vir) {
+/* the terms of metal atom i in mpi_cpff_force (cpff.c:560-948): metal-water,
+ * self, metal-metal and external field */
+static void cpff_one(const oref* o, const double* r, const double* vec_pq, int i, double* f,
+                     double* pot, double* vir) {
     const capac_params* p = &o->p;
     int N = o->N, Nm = o->Nm;
     const double *rx = r, *ry = r + N, *rz = r + 2 * N;
@@ -924,8 +998,7 @@ void oref_cpff_force(oref* o, const double* r, const double* vec_pq, double* f, 
     double alpha = o->alpha;
     double alpha3 = pow(alpha, 3.0);
     double inv_Rpp3 = pow(o->inv_R_pp, 3.0);
-    double t0 = now();
-    for (int i = 0; i < Nm; ++i) {
+    {
         for (int a_at = 0; a_at < N; ++a_at) {
             if (a_at < Nm) continue;
             if (0.0 == p->charge[a_at]) continue;
@@ -1058,6 +1131,35 @@ void oref_cpff_force(oref* o, const double* r, const double* vec_pq, double* f, 
         double pv[3] = {vec_pq[i * 3], vec_pq[i * 3 + 1], vec_pq[i * 3 + 2]};
         pot[9] -= p->efield[0] * pv[0] + p->efield[1] * pv[1] + p->efield[2] * pv[2];
     }
+}
+
+/* mpi_cpff_force (cpff.c:526-950) at one rank.  The OpenMP build runs metal
+ * atoms in parallel with per-thread force, energy and virial accumulators
+ * (summation order changes: timing only). */
+void oref_cpff_force(oref* o, const double* r, const double* vec_pq, double* f, double* pot,
+                     double* vir) {
+    int Nm = o->Nm;
+    double t0 = now();
+#ifdef ORACLE_OMP
+    int N = o->N, nth = omp_get_max_threads();
+    double* fb = zalloc((size_t)nth * 3 * N, sizeof(double));
+    double* pb = zalloc((size_t)nth * 24, sizeof(double));
+#pragma omp parallel
+    {
+        int tid = omp_get_thread_num();
+#pragma omp for schedule(dynamic, 8)
+        for (int i = 0; i < Nm; ++i)
+            cpff_one(o, r, vec_pq, i, fb + (size_t)tid * 3 * N, pb + (size_t)tid * 24, pb + (size_t)tid * 24 + 15);
+    }
+    for (int t = 0; t < nth; ++t) {
+        for (size_t k = 0; k < 3 * (size_t)N; ++k) f[k] += fb[(size_t)t * 3 * N + k];
+        for (int k = 9; k < 15; ++k) pot[k] += pb[t * 24 + k];
+        for (int k = 0; k < 9; ++k) vir[k] += pb[t * 24 + 15 + k];
+    }
+    free(fb); free(pb);
+#else
+    for (int i = 0; i < Nm; ++i) cpff_one(o, r, vec_pq, i, f, pot, vir);
+#endif
     o->timers[8] += now() - t0;
 }
 
