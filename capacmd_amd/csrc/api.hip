@@ -500,13 +500,14 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         TRY(upload(&c->x_backup, (const double*)NULL, c->n_mat));
         for (int k = 0; k < 3; ++k) TRY(upload(&c->x_hist[k], (const double*)NULL, c->n_mat));
         TRY(upload(&c->diag, (const double*)NULL, c->n_mat));
+        // the FP64 path stores no relay (its product forms the blocks)
         c->relay_ld = ((Nm + 63) / 64) * 64;
         if (precision == CAPAC_PREC_FP32_REFINED) {
-            c->relay_bytes = (size_t)Nm * 7 * c->relay_ld * sizeof(float);
+            // rows padded with zero coefficients (the SpMV reads whole row blocks)
+            const size_t rows = ((size_t)Nm + kRelay32RowPad - 1) / kRelay32RowPad * kRelay32RowPad;
+            c->relay_bytes = rows * kRelay32Planes * c->relay_ld * sizeof(float);
             CHECK_HIP(hipMalloc(&c->relay32, c->relay_bytes));
-        } else {
-            c->relay_bytes = (size_t)Nm * 16 * c->relay_ld * sizeof(double);
-            CHECK_HIP(hipMalloc(&c->relay, c->relay_bytes));
+            CHECK_HIP(hipMemset(c->relay32, 0, c->relay_bytes));
         }
         TRY(solver_create(c));
     }
@@ -526,7 +527,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay, c->relay32,
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32,
                     c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,

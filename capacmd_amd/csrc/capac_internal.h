@@ -7,12 +7,13 @@
 //   neighbour rows        nbr[i * nb_stride + k]: non-metal neighbours from the
 //                         front (k < n_front[i]), metal neighbours from the back
 //                         (k >= nb_stride - n_back[i]); int32
-//   relay blocks          FP64: Nm x Nm 4x4 blocks [A(3x3) M(3x1); M^T(1x3) C],
-//                         element k of block (i,j) at relay[(i*16 + k)*relay_ld + j]
-//   relay (FP32-refined)  7 FP32 coefficient planes per metal row-group i,
-//                         plane k of pair (i,j) at relay32[(i*7 + k)*relay_ld + j]:
-//                         r_x, r_y, r_z (bohr), g_a = c2 - 3 c1, g_b = c1 r^2, c3, cq;
-//                         the diagonal blocks are applied analytically
+//   relay (FP32-refined)  4 FP32 coefficient planes per metal row-group i,
+//                         plane k of pair (i,j) at relay32[(i*4 + k)*relay_ld + j]:
+//                         g_a = c2 - 3 c1, g_b = c1 r^2, c3, cq (relay.h); the
+//                         separation is formed from the positions where a block
+//                         is applied, and the diagonal blocks analytically.  The
+//                         FP64 path stores no relay: its product forms each block
+//                         from the positions.
 //   CPIM vector           x = [p (3Nm) | q (Nm) | lambda (n_np)], atomic units
 #pragma once
 
@@ -56,6 +57,12 @@ struct BoxConst {
     double inv_polar, inv_capac, inv_R_pp, inv_R_pq, inv_R_qq;
     double efield[3];
 };
+
+// coefficient planes per ordered metal pair of the FP32 relay (relay.h);
+// its rows are allocated up to a multiple of kRelay32RowPad, the padding rows
+// zero (the SpMV reads up to that many row-groups at once)
+constexpr int kRelay32Planes = 4;
+constexpr int kRelay32RowPad = 8;
 
 struct PairParam {          // one entry of the n_types x n_types table
     int funct;                  // 1 LJ + Coulomb, 2 Buckingham, 3 Morse, 4 erf vdW
@@ -139,7 +146,7 @@ struct capac_ctx {
     // Sutton-Chen
     double* isd;
     // CPIM
-    double *b, *x, *x_backup, *diag, *relay;
+    double *b, *x, *x_backup, *diag;
     int diag_set;               // diag holds the values for diag_polar / diag_capac
     double diag_polar, diag_capac;
     // solutions of the last MD steps, newest first, for the warm-start

@@ -3,19 +3,17 @@
 //   cpim_assemble  mpi_cpff_mat_relay_COO  (cpff.c:205-519)
 //   cpim_forces    mpi_cpff_force          (cpff.c:526-950)
 //
-// Relay layout (FP64 path): every ordered metal pair (i, j) owns one 4x4 block
-//     [ A_ab (3x3)   M_a = c3 r_a ]        rows: dipole p_i (a), charge q_i
-//     [ -c3 r_b      C            ]        cols: dipole p_j (b), charge q_j
-// with r = (r_j - r_i) in bohr, the Gaussian-smeared T-tensors of the
-// reference, and the diagonal blocks diag(1/alpha) and -1/c.  The relay has
-// no cut-off (every metal pair), so each row-block i holds Nm blocks, stored
-// component-major: element k (row-major index in the block) of block (i, j)
-// is relay[(i*16 + k)*ld + j], ld = Nm rounded up to 64.  Lanes that take
-// consecutive j then read and write 512 contiguous bytes per instruction.
-// The Lagrange rows and columns of the charge-neutrality constraint are
-// applied analytically by the solver.
+// The relay has no cut-off (every metal pair), and its blocks are given in
+// relay.h.  The FP64 path stores no matrix: its SpMV recomputes each block
+// from the positions (the entries the reference assembles, term by term).
+// The FP32-refined path stores four FP32 coefficients per ordered pair,
+// component-major (plane k of pair (i, j) at relay32[(i*4 + k)*ld + j], ld =
+// Nm rounded up to 64), so lanes that take consecutive j read and write 256
+// contiguous bytes per instruction.  The Lagrange rows and columns of the
+// charge-neutrality constraint are applied analytically by the solver.
 #include "capac_internal.h"
 #include "erfc_fast.h"
+#include "relay.h"
 
 namespace {
 
@@ -81,133 +79,37 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
 }
 
 // ------------------------------------------------------------------ relay
-// Gaussian-smeared T-tensor coefficients of one metal pair (cpff.c:266-287):
-//   c1 = [erf(r/R_pp) - 2/sqrt(pi) (r/R_pp) e^{-r^2/R_pp^2}] / r^5
-//   c2 = 4/sqrt(pi) R_pp^-3 e^{-r^2/R_pp^2} / r^2
-//   c3 = [erf(r/R_pq) - 2/sqrt(pi) (r/R_pq) e^{-r^2/R_pq^2}] / r^3
-//   cq = -erf(r/R_qq) / r
-struct TCoef {
-    double c1, c2, c3, cq;
-};
-
-__device__ __forceinline__ TCoef t_coef(const BoxConst& bc, double r2) {
-    TCoef t;
-    double rij = sqrt(r2);
-    double inv_r = 1.0 / rij;
-    double inv_r2 = inv_r * inv_r;
-    double inv_r3 = inv_r2 * inv_r;
-    double e_pp = exp(-(r2 * bc.inv_R_pp * bc.inv_R_pp));
-    double inv_Rpp3 = bc.inv_R_pp * bc.inv_R_pp * bc.inv_R_pp;
-    t.c1 = inv_r3 * inv_r2 * (erf(rij * bc.inv_R_pp) - 2.0 * K_INV_SQRT_PI * bc.inv_R_pp * rij * e_pp);
-    t.c2 = 4.0 * K_INV_SQRT_PI * inv_Rpp3 * inv_r2 * e_pp;
-    double e_pq = exp(-(r2 * bc.inv_R_pq * bc.inv_R_pq));
-    t.c3 = inv_r3 * (erf(rij * bc.inv_R_pq) - 2.0 * K_INV_SQRT_PI * bc.inv_R_pq * rij * e_pq);
-    t.cq = -erf(rij * bc.inv_R_qq) * inv_r;
-    return t;
-}
-
-// relay block (i, j), i != j, with r = (r_j - r_i) NM2BOHR (cpff.c:259-263):
-//   A_ab = -(c1 (3 r_a r_b - delta_ab r^2) - c2 r_a r_b)      (cpff.c:326-328)
-//   dipole row / charge column  c3 r_a                        (cpff.c:353)
-//   charge row / dipole column  -c3 r_b                       (cpff.c:386)
-//   charge / charge             cq                            (cpff.c:428)
-__device__ __forceinline__ void relay_block64(const double* __restrict__ r, const BoxConst& bc, int i,
-                                              int j, double blk[16]) {
-    const int N = bc.N;
-    double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
-    pbc12(bc, dx, dy, dz);
-    dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
-    double r2 = dist2_exact(dx, dy, dz);
-    TCoef t = t_coef(bc, r2);
-    double rv[3] = {dx, dy, dz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-            blk[a * 4 + bb] = -(t.c1 * (3.0 * rv[a] * rv[bb] - (a == bb ? r2 : 0.0)) -
-                                t.c2 * rv[a] * rv[bb]);
-        blk[a * 4 + 3] = t.c3 * rv[a];
-        blk[12 + a] = -t.c3 * rv[a];
-    }
-    blk[15] = t.cq;
-}
-
-// one thread per ordered pair (i, j): grid (ceil(Nm/256), Nm)
+// FP32-refined relay: the four coefficient planes of relay.h, T-tensors in
+// FP32 from the separation formed in FP64 with the reference's pbc_12 and
+// rounded once to FP32.  The diagonal pairs hold zeros (their blocks are
+// applied analytically).  One thread per ordered pair: grid (ceil(Nm/256), Nm).
 __global__ void __launch_bounds__(256)
-k_assemble64(const double* __restrict__ r, BoxConst bc, int ld, double* __restrict__ relay) {
+k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restrict__ relay) {
     main_stream_priority();
     const int Nm = bc.Nm;
     int i = blockIdx.y;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Nm) return;
-    double blk[16];
-    if (i == j) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) blk[k] = 0.0;
-        blk[0] = blk[5] = blk[10] = bc.inv_polar;     // cpff.c:305
-        blk[15] = -bc.inv_capac;                      // cpff.c:408
-    } else {
-        relay_block64(r, bc, i, j, blk);
-    }
-    double* dst = relay + (size_t)i * 16 * ld + j;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) dst[(size_t)k * ld] = blk[k];
-}
-
-// FP32 T-tensor coefficients: the same expressions as t_coef, evaluated in
-// single precision
-struct TCoef32 {
-    float c1, c2, c3, cq;
-};
-
-__device__ __forceinline__ TCoef32 t_coef32(float r2, float iRpp, float iRpq, float iRqq) {
-    const float isp = (float)K_INV_SQRT_PI;
-    TCoef32 t;
-    float rij = sqrtf(r2);
-    float inv_r = 1.0f / rij;
-    float inv_r2 = inv_r * inv_r;
-    float inv_r3 = inv_r2 * inv_r;
-    float e_pp = expf(-(r2 * iRpp * iRpp));
-    t.c1 = inv_r3 * inv_r2 * (erff(rij * iRpp) - 2.0f * isp * iRpp * rij * e_pp);
-    t.c2 = 4.0f * isp * (iRpp * iRpp * iRpp) * inv_r2 * e_pp;
-    float e_pq = expf(-(r2 * iRpq * iRpq));
-    t.c3 = inv_r3 * (erff(rij * iRpq) - 2.0f * isp * iRpq * rij * e_pq);
-    t.cq = -erff(rij * iRqq) * inv_r;
-    return t;
-}
-
-// FP32-refined relay: 7 coefficient planes per row-group (the layout and
-// block algebra are described with RelayC32 in solver.hip); the separation is
-// formed in FP64 with the reference's pbc_12 and rounded once to FP32.
-// One thread per ordered pair: grid (ceil(Nm/256), Nm).
-__global__ void __launch_bounds__(256)
-k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restrict__ relay) {
-    main_stream_priority();
-    const int N = bc.N, Nm = bc.Nm;
-    int i = blockIdx.y;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Nm) return;
-    float cf[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float cf[kRelay32Planes] = {0.f, 0.f, 0.f, 0.f};
     if (i != j) {
-        double dx = r[j] - r[i], dy = r[N + j] - r[N + i], dz = r[2 * N + j] - r[2 * N + i];
-        pbc12(bc, dx, dy, dz);
-        float rx = (float)(dx * K_NM2BOHR), ry = (float)(dy * K_NM2BOHR), rz = (float)(dz * K_NM2BOHR);
+        double dx, dy, dz;
+        metal_sep(r, bc, i, j, dx, dy, dz);
+        float rx = (float)dx, ry = (float)dy, rz = (float)dz;
         float r2 = (rx * rx + ry * ry) + rz * rz;
         TCoef32 t = t_coef32(r2, (float)bc.inv_R_pp, (float)bc.inv_R_pq, (float)bc.inv_R_qq);
-        cf[0] = rx; cf[1] = ry; cf[2] = rz;
-        cf[3] = t.c2 - 3.0f * t.c1;
-        cf[4] = t.c1 * r2;
-        cf[5] = t.c3;
-        cf[6] = t.cq;
+        cf[0] = t.c2 - 3.0f * t.c1;
+        cf[1] = t.c1 * r2;
+        cf[2] = t.c3;
+        cf[3] = t.cq;
     }
-    float* dst = relay + (size_t)i * 7 * ld + j;
+    float* dst = relay + (size_t)i * kRelay32Planes * ld + j;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) dst[(size_t)k * ld] = cf[k];
+    for (int k = 0; k < kRelay32Planes; ++k) dst[(size_t)k * ld] = cf[k];
 }
 
-// y = A x (or b - A x) with the FP64 relay recomputed pair by pair, the
-// entries bitwise those of k_assemble64; one workgroup per row-group (metal
-// i, then the Lagrange rows); part[g] = (out, out) over the group's rows
+// y = A x (or b - A x) with the FP64 relay recomputed pair by pair
+// (relay_block64); one workgroup per row-group (metal i, then the Lagrange
+// rows); part[g] = (out, out) over the group's rows
 __global__ void __launch_bounds__(256)
 k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_of,
           const int* __restrict__ np_first, const int* __restrict__ np_last,
@@ -278,8 +180,10 @@ __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
     diag[k] = k < 3 * Nm ? bc.inv_polar : (k < 4 * Nm ? -bc.inv_capac : 1.0);
 }
 
-// dense 4Nm x 4Nm metal block from the FP32 coefficients (tests)
-__global__ void k_dense32(const float* __restrict__ relay, BoxConst bc, int ld, double* __restrict__ dense) {
+// dense 4Nm x 4Nm metal block of either relay (tests): FP32 from the stored
+// coefficients and the separation, FP64 from the positions as the SpMV forms it
+__global__ void k_dense(const float* __restrict__ relay, const double* __restrict__ r, BoxConst bc, int ld,
+                        double* __restrict__ dense) {
     const int Nm = bc.Nm;
     long n4 = 4L * Nm;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -292,28 +196,21 @@ __global__ void k_dense32(const float* __restrict__ relay, BoxConst bc, int ld, 
     double v;
     if (i == j) {
         v = (a != b) ? 0.0 : (a < 3 ? bc.inv_polar : -bc.inv_capac);
-    } else {
-        const float* cp = relay + (size_t)i * 7 * ld + j;
-        double rv[3] = {cp[0], cp[ld], cp[2 * ld]};
-        double ga = cp[3 * ld], gb = cp[4 * ld], c3 = cp[5 * ld], cq = cp[6 * ld];
+    } else if (relay) {
+        const float* cp = relay + (size_t)i * kRelay32Planes * ld + j;
+        double rv[3];
+        metal_sep(r, bc, i, j, rv[0], rv[1], rv[2]);
+        double ga = cp[0], gb = cp[ld], c3 = cp[2 * ld], cq = cp[3 * ld];
         if (a < 3 && b < 3) v = ga * rv[a] * rv[b] + (a == b ? gb : 0.0);
         else if (a < 3) v = c3 * rv[a];
         else if (b < 3) v = -c3 * rv[b];
         else v = cq;
+    } else {
+        double blk[16];
+        relay_block64(r, bc, i, j, blk);
+        v = blk[a * 4 + b];
     }
     dense[idx] = v;
-}
-
-__global__ void k_dense(const double* __restrict__ relay, int Nm, int ld, double* __restrict__ dense) {
-    long n4 = 4L * Nm;
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n4 * n4) return;
-    long row = idx / n4, col = idx % n4;
-    int i = row < 3L * Nm ? (int)(row / 3) : (int)(row - 3L * Nm);
-    int a = row < 3L * Nm ? (int)(row % 3) : 3;
-    int j = col < 3L * Nm ? (int)(col / 3) : (int)(col - 3L * Nm);
-    int b = col < 3L * Nm ? (int)(col % 3) : 3;
-    dense[idx] = relay[((size_t)i * 16 + a * 4 + b) * ld + j];
 }
 
 // ------------------------------------------------------------------ forces
@@ -630,11 +527,10 @@ int cpim_rhs(capac_ctx* c) {
 int cpim_assemble(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
+    // the FP64 path stores nothing: its SpMV forms each block from the positions
     dim3 grid((Nm + 255) / 256, Nm);
     if (c->precision == CAPAC_PREC_FP32_REFINED)
         k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
-    else
-        k_assemble64<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay);
     // the Jacobi diagonal holds constants only (1/alpha, -1/c, 1): formed once
     // and again only when those constants change (fitting parameters)
     if (!c->diag_set || c->diag_polar != c->bc.inv_polar || c->diag_capac != c->bc.inv_capac) {
@@ -650,10 +546,8 @@ int cpim_assemble(capac_ctx* c) {
 int cpim_dense_expand(capac_ctx* c, double* dense) {
     long n4 = 4L * c->bc.Nm;
     unsigned blocks = (unsigned)((n4 * n4 + 255) / 256);
-    if (c->precision == CAPAC_PREC_FP32_REFINED)
-        k_dense32<<<blocks, 256, 0, c->stream>>>(c->relay32, c->bc, c->relay_ld, dense);
-    else
-        k_dense<<<blocks, 256, 0, c->stream>>>(c->relay, c->bc.Nm, c->relay_ld, dense);
+    k_dense<<<blocks, 256, 0, c->stream>>>(c->precision == CAPAC_PREC_FP32_REFINED ? c->relay32 : NULL,
+                                           c->r, c->bc, c->relay_ld, dense);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

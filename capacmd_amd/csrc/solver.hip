@@ -30,14 +30,17 @@
 // and the host synchronises once per batch.
 //
 // Two relay formats share these kernels (template parameter):
-//   RelayF64  the reference's 4x4 blocks, FP64, diagonal blocks stored
-//   RelayC32  7 FP32 coefficients per metal pair; products accumulate in
-//             FP64 with FP64 vectors; diagonal blocks applied analytically
-// The FP32-refined solve runs BiCGSTAB on RelayC32, then checks the true
-// residual b - A x with the FP64 relay recomputed on the fly (no FP64 matrix
-// is stored) and, while the reference's criterion fails on it, solves for
-// the correction on RelayC32 and adds it (iterative refinement).
+//   RelayM64  FP64, no stored matrix: each block is formed from the
+//             positions inside the product (the reference's entries)
+//   RelayG32  four FP32 coefficients per ordered pair plus the separation
+//             formed from the FP64 positions; products accumulate in FP64
+//             with FP64 vectors; diagonal blocks applied analytically
+// The FP32-refined solve runs BiCGSTAB on RelayG32, then checks the true
+// residual b - A x with the FP64 relay recomputed on the fly and, while the
+// reference's criterion fails on it, solves for the correction on RelayG32
+// and adds it (iterative refinement).
 #include "capac_internal.h"
+#include "relay.h"
 
 #include <stdlib.h>
 
@@ -62,48 +65,130 @@ namespace {
 constexpr int kVecThreads = 256;       // vector kernels: one row per thread
 constexpr int kInitThreads = 1024;
 
-// 4 x 4 FP64 blocks, component-major (element k of block (i,j) at
-// a[(i*16 + k)*ld + j]); diagonal blocks are part of the storage
-struct RelayF64 {
-    const double* a;
-    int ld;
-    __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
-                                        double o[4]) const {
-        const double* bp = a + (size_t)i * 16 * ld + j;
-        o[0] += bp[0 * ld] * x0 + bp[1 * ld] * x1 + bp[2 * ld] * x2 + bp[3 * ld] * x3;
-        o[1] += bp[4 * ld] * x0 + bp[5 * ld] * x1 + bp[6 * ld] * x2 + bp[7 * ld] * x3;
-        o[2] += bp[8 * ld] * x0 + bp[9 * ld] * x1 + bp[10 * ld] * x2 + bp[11 * ld] * x3;
-        o[3] += bp[12 * ld] * x0 + bp[13 * ld] * x1 + bp[14 * ld] * x2 + bp[15 * ld] * x3;
+// FP64 blocks formed from the positions in the product (VALU work, one
+// row-group per block: R = 1); acc skips j == i (diag applies that block)
+struct RelayM64 {
+    const double* r;
+    BoxConst bc;
+    template <int R>
+    struct Rows {
+        const double* r;
+        const BoxConst* bc;
+        int i, N;
+        double xi, yi, zi;
+        template <int U>
+        __device__ __forceinline__ void acc_n(int j, int Nm, const double* __restrict__ x, double (&o)[R][4]) const {
+            static_assert(U == 1, "one column per lane and pass");
+            acc(j, x[3 * j], x[3 * j + 1], x[3 * j + 2], x[3 * Nm + j], o);
+        }
+        __device__ __forceinline__ void acc(int j, double x0, double x1, double x2, double x3,
+                                            double (&o)[R][4]) const {
+            static_assert(R == 1, "the FP64 product takes one row-group per block");
+            if (j == i) return;
+            double blk[16];
+            relay_block64_sep(r[j] - xi, r[N + j] - yi, r[2 * N + j] - zi, *bc, blk);
+            o[0][0] += blk[0] * x0 + blk[1] * x1 + blk[2] * x2 + blk[3] * x3;
+            o[0][1] += blk[4] * x0 + blk[5] * x1 + blk[6] * x2 + blk[7] * x3;
+            o[0][2] += blk[8] * x0 + blk[9] * x1 + blk[10] * x2 + blk[11] * x3;
+            o[0][3] += blk[12] * x0 + blk[13] * x1 + blk[14] * x2 + blk[15] * x3;
+        }
+    };
+    template <int R>
+    __device__ __forceinline__ Rows<R> rows(int i0) const {
+        const int N = bc.N;
+        return Rows<R>{r, &bc, i0, N, r[i0], r[N + i0], r[2 * N + i0]};
     }
-    __device__ __forceinline__ void diag(double, double, double, double, double[4]) const {}
+    static constexpr int kUnroll = 1;      // columns per lane and pass
+    __device__ __forceinline__ void diag(double x0, double x1, double x2, double x3, double o[4]) const {
+        o[0] += bc.inv_polar * x0;
+        o[1] += bc.inv_polar * x1;
+        o[2] += bc.inv_polar * x2;
+        o[3] += -bc.inv_capac * x3;
+    }
 };
 
-// 7 FP32 coefficients per ordered pair (i != j), r = r_j - r_i in bohr:
-//   A_ab = g_a r_a r_b + g_b delta_ab,  M_a = c3 r_a,  M^T_b = -c3 r_b,  C = cq
-// so one block times (p_j, q_j) is
-//   dipole rows  (g_a (r.p) + c3 q) r + g_b p,   charge row  cq q - c3 (r.p)
-struct RelayC32 {
+#ifndef CAPAC_G32_UNROLL
+#define CAPAC_G32_UNROLL 2
+#endif
+
+// FP32 coefficients g_a, g_b, c3, cq per ordered pair (relay.h) for R
+// consecutive row-groups at once: column j's vector entries and position are
+// loaded once for all R rows, each row adds its block from its own four
+// coefficients and the separation formed from the FP64 positions.  The
+// relay holds zero coefficient rows up to a multiple of kRelay32RowPad (R
+// must not exceed it), so
+// a block's last rows need no test (their position index is clamped to a
+// metal atom, and zero coefficients add exactly zero); the diagonal pair's
+// coefficients are zero too, so j == i needs no test either.
+struct RelayG32 {
     const float* a;
     int ld;
-    double inv_polar, inv_capac;
-    __device__ __forceinline__ void acc(int i, int j, double x0, double x1, double x2, double x3,
-                                        double o[4]) const {
-        const float* cp = a + (size_t)i * 7 * ld + j;
-        double rx = cp[0], ry = cp[ld], rz = cp[2 * ld];
-        double ga = cp[3 * ld], gb = cp[4 * ld], c3 = cp[5 * ld], cq = cp[6 * ld];
-        double rp = rx * x0 + ry * x1 + rz * x2;
-        double gr = ga * rp + c3 * x3;
-        o[0] += gr * rx + gb * x0;
-        o[1] += gr * ry + gb * x1;
-        o[2] += gr * rz + gb * x2;
-        o[3] += cq * x3 - c3 * rp;
+    const double* r;
+    BoxConst bc;
+    template <int R>
+    struct Rows {
+        const float* cp;            // plane 0 of row-group i0
+        int ld, N;
+        const double* r;
+        const BoxConst* bc;
+        double xi[R], yi[R], zi[R];
+        // U columns j, j + 256, ... per lane: every load of the U columns is
+        // issued before the first block is applied, so a wavefront keeps U
+        // columns' worth of loads in flight.  A column past the end is read
+        // at Nm - 1 with zero vector entries (its blocks add exactly zero).
+        template <int U>
+        __device__ __forceinline__ void acc_n(int j0, int Nm, const double* __restrict__ x,
+                                              double (&o)[R][4]) const {
+            float c[U][R][kRelay32Planes];
+            double xv[U][4], pj[U][3];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + 256 * u;
+                const bool in = j < Nm;
+                const int jj = in ? j : Nm - 1;
+#pragma unroll
+                for (int q = 0; q < R; ++q)
+#pragma unroll
+                    for (int k = 0; k < kRelay32Planes; ++k)
+                        c[u][q][k] = cp[(size_t)(q * kRelay32Planes + k) * ld + jj];
+                xv[u][0] = x[3 * jj]; xv[u][1] = x[3 * jj + 1]; xv[u][2] = x[3 * jj + 2];
+                xv[u][3] = x[3 * Nm + jj];
+                if (!in) xv[u][0] = xv[u][1] = xv[u][2] = xv[u][3] = 0.0;
+                pj[u][0] = r[jj]; pj[u][1] = r[N + jj]; pj[u][2] = r[2 * N + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    double dx = pj[u][0] - xi[q], dy = pj[u][1] - yi[q], dz = pj[u][2] - zi[q];
+                    pbc12(*bc, dx, dy, dz);
+                    dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+                    block32_apply(dx, dy, dz, c[u][q][0], c[u][q][1], c[u][q][2], c[u][q][3], xv[u][0], xv[u][1],
+                                  xv[u][2], xv[u][3], o[q]);
+                }
+        }
+    };
+    template <int R>
+    __device__ __forceinline__ Rows<R> rows(int i0) const {
+        static_assert(R <= kRelay32RowPad, "row block larger than the relay's row padding");
+        const int N = bc.N, Nm = bc.Nm;
+        Rows<R> w;
+        w.cp = a + (size_t)i0 * kRelay32Planes * ld;
+        w.ld = ld; w.N = N; w.r = r; w.bc = &bc;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = i0 + q < Nm ? i0 + q : Nm - 1;
+            w.xi[q] = r[i]; w.yi[q] = r[N + i]; w.zi[q] = r[2 * N + i];
+        }
+        return w;
     }
     __device__ __forceinline__ void diag(double x0, double x1, double x2, double x3, double o[4]) const {
-        o[0] += inv_polar * x0;
-        o[1] += inv_polar * x1;
-        o[2] += inv_polar * x2;
-        o[3] -= inv_capac * x3;
+        o[0] += bc.inv_polar * x0;
+        o[1] += bc.inv_polar * x1;
+        o[2] += bc.inv_polar * x2;
+        o[3] -= bc.inv_capac * x3;
     }
+    static constexpr int kUnroll = CAPAC_G32_UNROLL;
 };
 
 // fixed-order block sum (blockDim a multiple of 64, <= 1024)
@@ -143,53 +228,13 @@ __device__ void block_sum_n(double v[NV], double* s_w) {
     }
 }
 
-// y = A x over row-groups: group g < Nm is metal i's four rows (3 dipole, 1
-// charge), group Nm + k the Lagrange row of nanoparticle k.
-//   mode 0: y only
-//   mode 1: part[g] = (r0, y)_g
-//   mode 2: part[3g..3g+2] = (K y, x)_g, (K y, K y)_g, (r0, y)_g
-// TPG threads per row-group (64: one wavefront, 256: the whole workgroup)
-template <class M, int TPG>
-__global__ void __launch_bounds__(256)
-k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restrict__ np_first,
-       const int* __restrict__ np_last, const double* __restrict__ x, double* __restrict__ y, int mode,
-       const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
-       const SolveScalars* __restrict__ sc) {
-    main_stream_priority();
-    constexpr int GPB = 256 / TPG;                 // groups per block
-    constexpr int WPG = TPG / 64;                  // waves per group
-    __shared__ double s_w[4][4];
-    if (sc && sc->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int t = threadIdx.x % TPG;
-    const int g = blockIdx.x * GPB + threadIdx.x / TPG;
-    const int n_groups = Nm + n_np;
-    // a group spans whole waves, and the barrier below is used only when a
-    // group is the whole block, so returning here never splits a barrier
-    if (g >= n_groups) return;
-    double o[4] = {0.0, 0.0, 0.0, 0.0};
-    if (g >= Nm) {
-        const int k = g - Nm;
-        for (int mi = np_first[k] + t; mi <= np_last[k]; mi += TPG) o[3] += x[3 * Nm + mi];
-    } else {
-        const int i = g;
-        for (int j = t; j < Nm; j += TPG)
-            m.acc(i, j, x[3 * j], x[3 * j + 1], x[3 * j + 2], x[3 * Nm + j], o);
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) o[c] = wave_sum(o[c]);
-    if (WPG > 1) {
-        if (lane == 0)
-            for (int c = 0; c < 4; ++c) s_w[wave][c] = o[c];
-        __syncthreads();
-        if (t == 0)
-            for (int c = 0; c < 4; ++c) {
-                double acc = 0.0;
-                for (int w = 0; w < WPG; ++w) acc += s_w[w][c];
-                o[c] = acc;
-            }
-    }
-    if (t != 0) return;
+// the outputs of row-group g from its four row sums o (see k_spmv)
+template <class M>
+__device__ __forceinline__ void spmv_group_out(const M& m, int g, int Nm, const int* __restrict__ np_of,
+                                               const double* __restrict__ x, double* __restrict__ y,
+                                               int mode, const double* __restrict__ r0,
+                                               const double* __restrict__ K, double* __restrict__ part,
+                                               double o[4]) {
     if (g >= Nm) {
         const int row = 4 * Nm + (g - Nm);
         const double val = o[3];
@@ -228,6 +273,61 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
         part[3 * g + 1] = d1;
         part[3 * g + 2] = d2;
     }
+}
+
+// y = A x over row-groups: group g < Nm is metal i's four rows (3 dipole, 1
+// charge), group Nm + k the Lagrange row of nanoparticle k.
+//   mode 0: y only
+//   mode 1: part[g] = (r0, y)_g
+//   mode 2: part[3g..3g+2] = (K y, x)_g, (K y, K y)_g, (r0, y)_g
+// One workgroup of 256 threads per R consecutive metal row-groups (the
+// threads stride over the columns j), then one per Lagrange row; the row
+// sums are fixed-order wavefront and block reductions.
+template <class M, int R>
+__global__ void __launch_bounds__(256)
+k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restrict__ np_first,
+       const int* __restrict__ np_last, const double* __restrict__ x, double* __restrict__ y, int mode,
+       const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
+       const SolveScalars* __restrict__ sc) {
+    main_stream_priority();
+    __shared__ double s_w[4][4 * R];
+    if (sc && sc->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n_mblk = (Nm + R - 1) / R;
+    double o[R][4];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[q][c] = 0.0;
+    const bool lagrange = (int)blockIdx.x >= n_mblk;
+    const int i0 = blockIdx.x * R;
+    if (lagrange) {
+        const int k = blockIdx.x - n_mblk;
+        for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += 256) o[0][3] += x[3 * Nm + mi];
+    } else {
+        const auto rows = m.template rows<R>(i0);
+        constexpr int U = M::kUnroll;
+        for (int j = threadIdx.x; j < Nm; j += 256 * U) rows.template acc_n<U>(j, Nm, x, o);
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double v = wave_sum(o[q][c]);
+            if (lane == 0) s_w[wave][4 * q + c] = v;
+        }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (lagrange) {
+        if (q != 0) return;
+    } else if (q >= R || i0 + q >= Nm) {
+        return;
+    }
+    double oq[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) oq[c] = s_w[0][4 * q + c] + s_w[1][4 * q + c] + s_w[2][4 * q + c] + s_w[3][4 * q + c];
+    const int g = lagrange ? Nm + (blockIdx.x - n_mblk) : i0 + q;
+    spmv_group_out(m, g, Nm, np_of, x, y, mode, r0, K, part, oq);
 }
 
 // start of BiCGSTAB (bicg_stab.c:294-404): bTb, criteria, r0 = r = b - Ax,
@@ -404,9 +504,14 @@ k_res_finish(const double* __restrict__ part, int n, double* __restrict__ res,
     if (threadIdx.x == 0) res[0] = v;
 }
 
-__global__ void __launch_bounds__(256) k_axpy1(int n, const double* __restrict__ d, double* __restrict__ x) {
+__global__ void __launch_bounds__(256) k_axpy(int n, double a, const double* __restrict__ d, double* __restrict__ x) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) x[k] += d[k];
+    if (k < n) x[k] += a * d[k];
+}
+
+__global__ void __launch_bounds__(256) k_scale(int n, double a, double* __restrict__ x) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] *= a;
 }
 
 __global__ void __launch_bounds__(256) k_zero(int n, double* __restrict__ x) {
@@ -474,29 +579,44 @@ void solver_destroy(capac_ctx* c) {
     c->solver = NULL;
 }
 
-template <class M, int TPG>
+template <class M, int R>
 static void spmv_t(capac_ctx* c, const M& m, const double* x, double* y, int mode, double* part,
                    const SolveScalars* sc) {
     Solver* S = c->solver;
-    constexpr int GPB = 256 / TPG;
-    const int blocks = (S->n_groups + GPB - 1) / GPB;
-    k_spmv<M, TPG><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
-                                                  c->np_last, x, y, mode, S->r0, S->K, part, sc);
+    const int blocks = (S->Nm + R - 1) / R + S->n_np;
+    k_spmv<M, R><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
+                                                c->np_last, x, y, mode, S->r0, S->K, part, sc);
 }
 
-// Row-group width: the FP64 rows (128 N_m bytes each) always get a whole
-// workgroup; the compact FP32 rows (28 N_m bytes) get a workgroup while
-// N_m is small enough that one wavefront per group would leave most SIMDs
-// idle, a wavefront beyond that.
+// Rows per workgroup: the FP64 product is VALU work (every block formed
+// from the positions) and takes one row-group per workgroup.  The FP32
+// product streams 16 bytes per pair; several row-groups per workgroup share
+// each column's vector entries and position, which the lanes otherwise load
+// once per pair, while small decks keep one row-group per workgroup so that
+// there are enough workgroups to fill the chip.
+#ifndef CAPAC_G32_ROWS_LARGE
+#define CAPAC_G32_ROWS_LARGE 2
+#endif
+#ifndef CAPAC_G32_ROWS_MID
+#define CAPAC_G32_ROWS_MID 2
+#endif
+#ifndef CAPAC_G32_LARGE_MIN
+#define CAPAC_G32_LARGE_MIN 1536
+#endif
+#ifndef CAPAC_G32_MID_MIN
+#define CAPAC_G32_MID_MIN 768
+#endif
 static void spmv(capac_ctx* c, const double* x, double* y, int mode, double* part,
                  const SolveScalars* sc) {
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
-        RelayC32 m{c->relay32, c->relay_ld, c->bc.inv_polar, c->bc.inv_capac};
-        if (c->bc.Nm < 1024) spmv_t<RelayC32, 256>(c, m, x, y, mode, part, sc);
-        else spmv_t<RelayC32, 64>(c, m, x, y, mode, part, sc);
+        RelayG32 m{c->relay32, c->relay_ld, c->r, c->bc};
+        const int nm = c->bc.Nm;
+        if (nm >= CAPAC_G32_LARGE_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_LARGE>(c, m, x, y, mode, part, sc);
+        else if (nm >= CAPAC_G32_MID_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_MID>(c, m, x, y, mode, part, sc);
+        else spmv_t<RelayG32, 1>(c, m, x, y, mode, part, sc);
     } else {
-        RelayF64 m{c->relay, c->relay_ld};
-        spmv_t<RelayF64, 256>(c, m, x, y, mode, part, sc);
+        RelayM64 m{c->r, c->bc};
+        spmv_t<RelayM64, 1>(c, m, x, y, mode, part, sc);
     }
 }
 
@@ -615,14 +735,20 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
         double rTr = S->res_host[0];
         int total = o.iters, rounds = 0;
         while (!(rTr < crit) && rounds < 4) {
+            // the correction solves A d = rr / |rr| and x += |rr| d: with the
+            // residual scaled to unit norm, the solver's absolute early exit
+            // (bTb < 1e-10, meant for the system itself) cannot end a
+            // correction whose residual is merely small
             const int n = S->n_mat;
+            const double norm = sqrt(rTr);
             k_zero<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d);
+            k_scale<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 1.0 / norm, S->rr);
             SolveOut od;
             const int first_corr = c->batch_corr > 0 ? c->batch_corr + 1 : 4;
             if ((rc = bicgstab(c, S->rr, S->d, first_corr, false, &od))) return rc;
             c->batch_corr = od.iters;
             total += od.iters;
-            k_axpy1<<<(n + 255) / 256, 256, 0, c->stream>>>(n, S->d, x);
+            k_axpy<<<(n + 255) / 256, 256, 0, c->stream>>>(n, norm, S->d, x);
             if ((rc = residual64(c, b, x, &rTr))) return rc;
             ++rounds;
         }

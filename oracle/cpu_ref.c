@@ -906,6 +906,9 @@ static void spmv_coo(const oref* o, const double* x, double* Ax) {
 #endif
 }
 
+/* y = A x with the assembled COO relay (oref_relay first) */
+void oref_relay_apply(const oref* o, const double* x, double* y) { spmv_coo(o, x, y); }
+
 /* mpi_vec_vec (bicg_stab.c:121-178) at one rank */
 static double vdot(const oref* o, const double* a, const double* b) {
     int Nm = o->Nm;

@@ -44,6 +44,7 @@ void oref_vec_ext(oref* o, const double* r, double* b);
 long oref_relay(oref* o, const double* r);            /* returns nnz */
 void oref_relay_coo(const oref* o, long* row, long* col, double* val, double* diag);
 void oref_relay_dense(const oref* o, double* dense);  /* n_mat x n_mat incl. Lagrange rows */
+void oref_relay_apply(const oref* o, const double* x, double* y);   /* y = A x, COO order */
 int  oref_solve(oref* o, const double* b, double* x, int* iters, double* rTr, double* bTb);
 void oref_cpff_force(oref* o, const double* r, const double* x, double* f, double* pot, double* vir);
 

@@ -46,6 +46,7 @@ def lib(omp: bool = False) -> C.CDLL:
         L.oref_relay.restype = C.c_long
         L.oref_relay_coo.argtypes = [C.c_void_p, _lp, _lp, _dp, _dp]
         L.oref_relay_dense.argtypes = [C.c_void_p, _dp]
+        L.oref_relay_apply.argtypes = [C.c_void_p, _dp, _dp]
         L.oref_solve.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp]
         L.oref_cpff_force.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oref_force.argtypes = [C.c_void_p]
@@ -156,6 +157,12 @@ class Oracle:
         a = np.empty(self.n_mat * self.n_mat)
         self.L.oref_relay_dense(self.h, _d(a))
         return a.reshape(self.n_mat, self.n_mat)
+
+    def relay_apply(self, x):
+        """A x with the relay assembled by the last relay() call."""
+        y = np.empty(self.n_mat)
+        self.L.oref_relay_apply(self.h, _d(np.ascontiguousarray(x, dtype=np.float64)), _d(y))
+        return y
 
     def solve(self, b, x0):
         x = np.ascontiguousarray(x0, dtype=np.float64).copy()

@@ -1,0 +1,313 @@
+"""Parity at the sizes of the five benchmark configurations (capacmd_amd.gen
+config 0-4), through the C ABI, against the oracle.
+
+The golden-deck tests (test_gpu_parity.py, test_gpu_fp32.py) pin every
+kernel on systems of at most 147 metal atoms.  These run the same bars on the
+decks the benchmark is quoted on, where other code paths run: the FP32
+product with several row-groups per workgroup (N_m >= 1536), the pair kernel
+on its full grid beside the solve (N_m >= 1024), neighbour rows of 20k-98k
+atoms, and solves of 589-8,229 unknowns.
+
+  cfg0 Au147 (4,434 atoms, 589 unknowns), cfg1 Au561 (20,886 / 2,245),
+  cfg2 2 x Au309 + field (19,128 / 2,474): one force call per precision from
+      a cold start; every phase at the GPU's solution within 1e-10 of the
+      oracle; the solve within the solver tolerance (the reference's stop
+      test on the true FP64 residual, the distance to the dense-LU solution
+      at most 10x the oracle's own, per-particle neutrality).
+  cfg3 Au2057 (98,189 atoms, 8,229 unknowns): nonbonded, Sutton-Chen, RHS and
+      the CPIM force at a given solution within 1e-10 of the oracle (OpenMP
+      build, every pair: the corrected 64-bit pair count, not the
+      reference's 32-bit wrap above 46,341 atoms); the force call of each
+      precision with its solve checked on the oracle's FP64 COO residual.
+  cfg1 MD: three steps against the oracle with both sides solving to
+      rTr < 1e-20 bTb, so that the trajectories agree far inside the MD
+      parity bar.
+  cfg4 Au309 fitting: the adjoint gradient over two frames (50 MD steps
+      apart) against central differences of the oracle's loss with exact
+      dense solves, and the loss itself.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from capacmd_amd import fit, gen, lib
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def cfg(tmp_path_factory):
+    made = {}
+
+    def get(index):
+        if index not in made:
+            d = tmp_path_factory.mktemp(f"cfg{index}")
+            gen.write_deck(gen.config(index), str(d))
+            deck = lib.Deck.from_dir(str(d))
+            print(f"cfg{index}: N={deck.n_atoms} Nm={deck.n_metal} n_mat={deck.n_mat}", flush=True)
+            made[index] = deck
+        return made[index]
+    return get
+
+
+def field_terms(deck, r, f, pot):
+    """the external field on every charge (force.c:2251-2260)"""
+    N = deck.n_atoms
+    q = deck.array("charge", N)
+    E = np.array(deck.p.efield[:])
+    R = r.reshape(3, N)
+    for c in range(3):
+        f[c * N:(c + 1) * N] += E[c] * q
+    pot[9] -= np.sum(E[0] * q * R[0] + E[1] * q * R[1] + E[2] * q * R[2])
+
+
+def oracle_force_at(o, deck, r, x):
+    """mpi_force's phases at positions r and the solution x"""
+    f, pot, vir = o.nonb(r)
+    f2, p2, v2, _ = o.qsc(r)
+    f3, p3, v3 = o.cpff_force(r, x)
+    f, pot, vir = f + f2 + f3, pot + p2 + p3, vir + v2 + v3
+    field_terms(deck, r, f, pot)
+    return f, pot, vir
+
+
+def check_against(fg, pg, vg, f, pot, vir, what):
+    print(what, "force rel", rel(fg, f), "virial rel", rel(vg, vir), flush=True)
+    assert rel(fg, f) < 1e-10
+    for k in range(1, 15):
+        assert abs(pg[k] - pot[k]) <= 1e-10 * max(1.0, abs(pot[k])), (what, k, pg[k], pot[k])
+    assert rel(vg, vir) < 1e-10
+
+
+def check_neutral(deck, x, bnorm):
+    Nm = deck.n_metal
+    q = x[3 * Nm: 4 * Nm]
+    first = deck.array("np_first", deck.p.n_nps, np.int64)
+    last = deck.array("np_last", deck.p.n_nps, np.int64)
+    qtot = deck.array("np_charge", deck.p.n_nps)
+    for k in range(deck.p.n_nps):
+        # neutrality holds to the residual of the Lagrange row
+        assert abs(q[first[k]: last[k] + 1].sum() - qtot[k]) < 1.1e-4 * bnorm
+
+
+@pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("index", [0, 1, 2])
+def test_force_call_config(index, prec, cfg):
+    deck = cfg(index)
+    r = deck.r0
+    ctx = lib.Context(deck, precision=prec)
+    try:
+        ctx.set_state(r=r, x=np.zeros(deck.n_mat))
+        fg, pg, vg = ctx.force()
+        _, _, xg = ctx.get_state()
+        it, rtr, btb, kind = ctx.solve_stats()
+        if prec == lib.PREC_FP32_REFINED:
+            dense32, _ = ctx.cpim_assemble(dense=True)
+    finally:
+        ctx.close()
+    o = Oracle(deck, omp=True)
+    try:
+        b = o.vec_ext(r)
+        o.relay(r)
+        A = o.relay_dense()
+        xo, ito, _, _, kindo = o.solve(b, np.zeros(deck.n_mat))
+        assert kind == 0 and kindo == 0
+        res = b - A @ xg
+        bb = float(np.dot(b, b))
+        print(f"cfg{index}", "prec", prec, "iters", it, ito, "rTr/bTb", np.dot(res, res) / bb, flush=True)
+        assert np.dot(res, res) < 1.2e-8 * bb
+        xe = np.linalg.solve(A, b)
+        err_o = rel(xo, xe)
+        assert rel(xg, xe) <= max(10 * err_o, 1e-9), (rel(xg, xe), err_o)
+        check_neutral(deck, xg, np.sqrt(bb))
+        if prec == lib.PREC_FP64:
+            # same start, same stop test, FP64 on both sides; the iterates
+            # differ by summation order, which BiCGSTAB amplifies, and with
+            # thousands of rows that moves the stopping iteration by a few
+            # (measured: 70 against 65 on cfg1)
+            assert abs(it - ito) <= max(2, 0.15 * ito), (it, ito)
+        else:
+            n4 = 4 * deck.n_metal
+            err = np.abs(dense32 - A[:n4, :n4]).max() / np.abs(A[:n4, :n4]).max()
+            print(f"cfg{index}", "FP32 relay max rel entry error", err, flush=True)
+            assert err < 2e-6
+        f, pot, vir = oracle_force_at(o, deck, r, xg)
+    finally:
+        o.close()
+    check_against(fg, pg, vg, f, pot, vir, f"cfg{index} prec {prec}")
+
+
+@pytest.fixture(scope="module")
+def au2057(cfg):
+    """cfg3 with an FP32-refined force call's solution as the given x"""
+    deck = cfg(3)
+    o = Oracle(deck, omp=True)
+    ctx = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
+    ctx.set_state(r=deck.r0, x=np.zeros(deck.n_mat))
+    ctx.force()
+    _, _, x = ctx.get_state()
+    yield deck, o, ctx, x
+    ctx.close()
+    o.close()
+
+
+def test_au2057_phases(au2057):
+    deck, o, ctx, x = au2057
+    r = deck.r0
+    fo, po, vo = o.nonb(r)
+    fg, pg, vg = ctx.nonbonded()
+    print("cfg3 nonbonded rel", rel(fg, fo), pg[7], po[7], pg[8], po[8], flush=True)
+    assert rel(fg, fo) < 1e-10
+    assert abs(pg[7] - po[7]) <= 1e-10 * abs(po[7])
+    assert abs(pg[8] - po[8]) <= 1e-10 * abs(po[8])
+    assert rel(vg, vo) < 1e-10
+    fo, po, vo, io = o.qsc(r)
+    fg, pg, vg, ig = ctx.sutton_chen()
+    print("cfg3 Sutton-Chen rel", rel(fg, fo), rel(ig, io), flush=True)
+    assert rel(ig, io) < 1e-12
+    assert rel(fg, fo) < 1e-10
+    assert abs(pg[1] - po[1]) <= 1e-10 * abs(po[1])
+    assert rel(vg, vo) < 1e-10
+    bo = o.vec_ext(r)
+    bg = ctx.cpim_rhs()
+    print("cfg3 RHS rel", rel(bg, bo), flush=True)
+    assert rel(bg, bo) < 1e-10
+    fo, po, vo = o.cpff_force(r, x)
+    fg, pg, vg = ctx.cpim_force(x)
+    print("cfg3 CPIM force rel", rel(fg, fo), flush=True)
+    assert rel(fg, fo) < 1e-10
+    for k in range(9, 15):
+        assert abs(pg[k] - po[k]) <= 1e-10 * max(1e-3, abs(po[k])), k
+    assert rel(vg, vo) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
+def test_au2057_force_call(prec, au2057):
+    """The force call as MD runs it: pair kernel on its full grid beside the
+    solve, the FP32 product with several row-groups per workgroup; the solve
+    on the oracle's FP64 COO residual, everything else at 1e-10."""
+    deck, o, _, x32 = au2057
+    r = deck.r0
+    ctx = lib.Context(deck, precision=prec)
+    try:
+        ctx.set_state(r=r, x=np.zeros(deck.n_mat))
+        fg, pg, vg = ctx.force()
+        _, _, xg = ctx.get_state()
+        it, _, btb, kind = ctx.solve_stats()
+    finally:
+        ctx.close()
+    assert kind == 0
+    b = o.vec_ext(r)
+    o.relay(r)
+    res = b - o.relay_apply(xg)
+    bb = float(np.dot(b, b))
+    assert abs(btb - bb) <= 1e-9 * bb          # the GPU's own right-hand side (1e-10 above)
+    print("cfg3 prec", prec, "iters", it, "rTr/bTb", np.dot(res, res) / bb,
+          "x vs the FP32 solution", rel(xg, x32), flush=True)
+    assert np.dot(res, res) < 1.2e-8 * bb
+    check_neutral(deck, xg, np.sqrt(bb))
+    # two valid solves of the same system stop within the solver tolerance
+    assert rel(xg, x32) < 1e-2
+    f, pot, vir = oracle_force_at(o, deck, r, xg)
+    check_against(fg, pg, vg, f, pot, vir, f"cfg3 prec {prec}")
+
+
+@pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
+def test_md_config1_tight(prec, cfg):
+    """Three MD steps of the Au561 deck; with both sides solving to
+    rTr < 1e-20 bTb the trajectories agree to the round-off of the solves."""
+    deck = cfg(1)
+    o = Oracle(deck, omp=True)
+    o.set_solve_crit(1e-20)
+    o.force()
+    o.md_begin()
+    ctx = lib.Context(deck, precision=prec)
+    ctx.set_predictor(0)
+    ctx.set_solve_tolerance(1e-20)
+    ctx.force()
+    ctx.md_begin()
+    try:
+        for step in range(1, 4):
+            assert o.md_step(step, deck.n_save) == 0
+            assert ctx.md_run(step, 1, deck.n_save) == 1
+            ro, vo, xo, _ = o.get_state()
+            rg, vg, xg = ctx.get_state()
+            obo, obg = o.observables(), ctx.observables()
+            print("cfg1 prec", prec, "step", step, "r", np.abs(rg - ro).max(), "v rel", rel(vg, vo),
+                  "x rel", rel(xg, xo), flush=True)
+            assert np.abs(rg - ro).max() < 1e-9
+            assert rel(vg, vo) < 1e-7
+            assert abs(obg["temp"] - obo["temp"]) < 1e-7 * obo["temp"]
+            for k in range(1, 15):
+                assert abs(obg["pot"][k] - obo["pot"][k]) <= 1e-7 * max(1.0, abs(obo["pot"][k])), k
+    finally:
+        ctx.close()
+        o.close()
+
+
+PERTURB = np.array([0.03, -0.02, 0.04, -0.03, 0.02, -0.04])
+W_E = 1e-2
+
+
+def test_fit_gradient_config4(cfg):
+    """Adjoint gradient of the force-matching loss on two frames of the Au309
+    fitting deck against central differences (h = 1e-5 theta_k) of the
+    oracle's loss with exact dense solves: the FP64 reference gradient.  The
+    frames' nonbonded terms do not depend on theta and are computed once."""
+    deck = cfg(4)
+    frames = fit.make_frames(deck, n_frames=2, stride=50)
+    h = fit.Harness(deck, frames, w_e=W_E, solve_crit=1e-22)
+    try:
+        theta = h.theta0 * (1.0 + PERTURB)
+        loss, g = h.loss_grad(theta)
+    finally:
+        h.close()
+    N, Nm = deck.n_atoms, deck.n_metal
+    o = Oracle(deck, omp=True)
+    base = []
+    for fr in frames:
+        f, pot, _ = o.nonb(fr.r)
+        field_terms(deck, fr.r, f, pot)
+        base.append((f, pot))
+
+    def oracle_loss(th):
+        o.set_theta(th)
+        total = 0.0
+        for fr, (fn, pn) in zip(frames, base):
+            b = o.vec_ext(fr.r)
+            o.relay(fr.r)
+            x = np.linalg.solve(o.relay_dense(), b)
+            f2, p2, _, _ = o.qsc(fr.r)
+            f3, p3, _ = o.cpff_force(fr.r, x)
+            f, pot = fn + f2 + f3, pn + p2 + p3
+            fm = f.reshape(3, N)[:, :Nm].reshape(-1)
+            e = float(np.sum(pot[1:]))
+            total += float(np.sum((fm - fr.f_ref) ** 2)) + W_E * (e - fr.e_ref) ** 2
+        return total
+
+    try:
+        lo = oracle_loss(theta)
+        print("cfg4 loss gpu", loss, "oracle", lo, flush=True)
+        assert abs(loss - lo) <= 1e-8 * abs(lo)
+        g_fd = np.zeros(6)
+        for k in range(6):
+            hk = 1e-5 * theta[k]
+            tp = theta.copy(); tp[k] += hk
+            tm = theta.copy(); tm[k] -= hk
+            g_fd[k] = (oracle_loss(tp) - oracle_loss(tm)) / (2 * hk)
+    finally:
+        o.close()
+    floor = 1e-3 * np.abs(g_fd).max()
+    err = np.abs(g - g_fd) / np.maximum(np.abs(g_fd), floor)
+    for k, name in enumerate(fit.THETA_NAMES):
+        print(f"  cfg4 {name:12s} adjoint {g[k]: .10e} fd {g_fd[k]: .10e} rel {err[k]:.2e}", flush=True)
+    assert err.max() <= 1e-6, err
