@@ -510,6 +510,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
             CHECK_HIP(hipMemset(c->relay32, 0, c->relay_bytes));
         }
         TRY(solver_create(c));
+        TRY(solver_noalloc_init(device));
     }
     for (int k = 0; k < 16; ++k) TRY(hipEventCreate(&c->ev[k]) == hipSuccess ? 0 : CAPAC_ERR_HIP);
 #undef TRY
@@ -1023,6 +1024,28 @@ extern "C" int capac_cpim_solve(capac_ctx* c, const double* b, double* x, int* i
     int rc = cpim_assemble(c);
     if (rc) return rc;
     rc = solver_run(c, c->b, c->x);
+    if (rc && rc != CAPAC_ERR_SOLVER) return rc;
+    if (x) CHECK_HIP(hipMemcpyAsync(x, c->x, bytes, hipMemcpyDeviceToHost, c->stream));
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = c->solve_iters;
+    if (rTr) *rTr = c->solve_rTr;
+    if (bTb) *bTb = c->solve_bTb;
+    if (kind) *kind = c->solve_kind;
+    return rc;
+}
+
+// capac_cpim_solve with the solver's work space in the library's
+// file-scope device array instead of the context's allocations
+extern "C" int capac_cpim_solve_noalloc(capac_ctx* c, const double* b, double* x, int* iters, double* rTr,
+                                        double* bTb, int* kind) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (!c->n_mat) { capac_set_error("no CPIM unknowns (no metal or use_cpff 0)"); return CAPAC_ERR_ARG; }
+    size_t bytes = sizeof(double) * c->n_mat;
+    if (b) CHECK_HIP(hipMemcpyAsync(c->b, b, bytes, hipMemcpyHostToDevice, c->stream));
+    if (x) CHECK_HIP(hipMemcpyAsync(c->x, x, bytes, hipMemcpyHostToDevice, c->stream));
+    int rc = cpim_assemble(c);
+    if (rc) return rc;
+    rc = solver_run_noalloc(c, c->b, c->x);
     if (rc && rc != CAPAC_ERR_SOLVER) return rc;
     if (x) CHECK_HIP(hipMemcpyAsync(x, c->x, bytes, hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));

@@ -266,6 +266,8 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 int solver_create(capac_ctx* c);
 void solver_destroy(capac_ctx* c);
 int solver_run(capac_ctx* c, const double* b_dev, double* x_dev);
+int solver_noalloc_init(int device);        // device address of the allocation-free work space
+int solver_run_noalloc(capac_ctx* c, const double* b_dev, double* x_dev);
 int solver_spmv_only(capac_ctx* c, const double* x, double* y);
 double* solver_scratch(capac_ctx* c);
 void solver_predict(capac_ctx* c);          // x from the history (md steps), at the next force call

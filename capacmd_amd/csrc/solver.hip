@@ -43,6 +43,9 @@
 #include "relay.h"
 
 #include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
 
 struct Solver {
     int n_mat, Nm, n_np, n_groups, n_vb;
@@ -58,6 +61,7 @@ struct Solver {
     SolveScalars* sc;       // device
     SolveScalars* sc_host;  // pinned mirror
     double* res_host;       // pinned mirror
+    bool in_red;            // sc and res inside the context's readback buffer
 };
 
 namespace {
@@ -565,6 +569,7 @@ int solver_create(capac_ctx* c) {
     S->sc_host = (SolveScalars*)(c->red_host + kRedSc);
     S->res = c->red + kRedRes;
     S->res_host = c->red_host + kRedRes;
+    S->in_red = true;
     return 0;
 }
 
@@ -580,9 +585,8 @@ void solver_destroy(capac_ctx* c) {
 }
 
 template <class M, int R>
-static void spmv_t(capac_ctx* c, const M& m, const double* x, double* y, int mode, double* part,
-                   const SolveScalars* sc) {
-    Solver* S = c->solver;
+static void spmv_t(capac_ctx* c, const Solver* S, const M& m, const double* x, double* y, int mode,
+                   double* part, const SolveScalars* sc) {
     const int blocks = (S->Nm + R - 1) / R + S->n_np;
     k_spmv<M, R><<<blocks, 256, 0, c->stream>>>(m, S->Nm, S->n_np, c->np_of_metal, c->np_first,
                                                 c->np_last, x, y, mode, S->r0, S->K, part, sc);
@@ -606,17 +610,17 @@ static void spmv_t(capac_ctx* c, const M& m, const double* x, double* y, int mod
 #ifndef CAPAC_G32_MID_MIN
 #define CAPAC_G32_MID_MIN 768
 #endif
-static void spmv(capac_ctx* c, const double* x, double* y, int mode, double* part,
+static void spmv(capac_ctx* c, const Solver* S, const double* x, double* y, int mode, double* part,
                  const SolveScalars* sc) {
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
         RelayG32 m{c->relay32, c->relay_ld, c->r, c->bc};
         const int nm = c->bc.Nm;
-        if (nm >= CAPAC_G32_LARGE_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_LARGE>(c, m, x, y, mode, part, sc);
-        else if (nm >= CAPAC_G32_MID_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_MID>(c, m, x, y, mode, part, sc);
-        else spmv_t<RelayG32, 1>(c, m, x, y, mode, part, sc);
+        if (nm >= CAPAC_G32_LARGE_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_LARGE>(c, S, m, x, y, mode, part, sc);
+        else if (nm >= CAPAC_G32_MID_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_MID>(c, S, m, x, y, mode, part, sc);
+        else spmv_t<RelayG32, 1>(c, S, m, x, y, mode, part, sc);
     } else {
         RelayM64 m{c->r, c->bc};
-        spmv_t<RelayM64, 1>(c, m, x, y, mode, part, sc);
+        spmv_t<RelayM64, 1>(c, S, m, x, y, mode, part, sc);
     }
 }
 
@@ -630,18 +634,17 @@ struct SolveOut {
 // residual b - A x (guarded: it runs only once the solve has finished), so
 // the refinement test needs no extra synchronisation; (rr, rr) lands in
 // res_host[0].
-static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, bool check64,
+static int bicgstab(capac_ctx* c, Solver* S, const double* b, double* x, int first_batch, bool check64,
                     SolveOut* out) {
-    Solver* S = c->solver;
     const int n = S->n_mat, n4 = 4 * S->Nm;
     const int vb = S->n_vb;
     hipStream_t st = c->stream;
-    spmv(c, x, S->Ax, 0, NULL, NULL);
+    spmv(c, S, x, S->Ax, 0, NULL, NULL);
     k_bicg_init<<<1, kInitThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
                                             S->y, S->sc, c->solve_crit);
     // K1, K2 of iteration 1, then units (K3, K4 of n; K1, K2 of n+1): each
     // unit ends with a stop test, so a batch of units can end the solve
-    spmv(c, S->y, S->v, 1, S->part1, S->sc);
+    spmv(c, S, S->y, S->v, 1, S->part1, S->sc);
     k_vec_s<<<vb, kVecThreads, 0, st>>>(n, 1, S->n_groups, S->part1, vb, S->part4, S->r, S->v, S->K,
                                         S->r0, S->s, S->z, S->part2, S->sc);
     // later batches start small: a first batch that falls short usually
@@ -650,11 +653,11 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
     int it = 1, batch = first_batch > 1 ? first_batch : 2, launched = 0, next = 2;
     for (;;) {
         for (int u = 0; u < batch; ++u, ++it) {
-            spmv(c, S->z, S->t, 2, S->part3, S->sc);
+            spmv(c, S, S->z, S->t, 2, S->part3, S->sc);
             k_vec_xp<<<vb, kVecThreads, 0, st>>>(n, it, S->n_groups, S->part3, vb, S->part2, x, S->y,
                                                  S->z, S->s, S->t, S->r, S->p, S->v, S->K, S->part4,
                                                  S->sc);
-            spmv(c, S->y, S->v, 1, S->part1, S->sc);
+            spmv(c, S, S->y, S->v, 1, S->part1, S->sc);
             k_vec_s<<<vb, kVecThreads, 0, st>>>(n, it + 1, S->n_groups, S->part1, vb, S->part4, S->r,
                                                 S->v, S->K, S->r0, S->s, S->z, S->part2, S->sc);
         }
@@ -665,9 +668,14 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
             k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
         }
         CHECK_HIP(hipGetLastError());
-        // the scalars, the FP64 residual and the neighbour-row flags in one transfer
-        CHECK_HIP(hipMemcpyAsync(c->red_host + kRedFlags, c->red + kRedFlags,
-                                 (kRedWords - kRedFlags) * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (S->in_red) {
+            // the scalars, the FP64 residual and the neighbour-row flags in one transfer
+            CHECK_HIP(hipMemcpyAsync(c->red_host + kRedFlags, c->red + kRedFlags,
+                                     (kRedWords - kRedFlags) * sizeof(double), hipMemcpyDeviceToHost, st));
+        } else {
+            CHECK_HIP(hipMemcpyAsync(S->sc_host, S->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st));
+            CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
+        }
         CHECK_HIP(hipStreamSynchronize(st));
         if (S->sc_host->done) break;
         if (launched > 1010) {
@@ -685,8 +693,7 @@ static int bicgstab(capac_ctx* c, const double* b, double* x, int first_batch, b
 }
 
 // FP64 residual rr = b - A64 x with the relay recomputed on the fly; returns (rr, rr)
-static int residual64(capac_ctx* c, const double* b, const double* x, double* rTr) {
-    Solver* S = c->solver;
+static int residual64(capac_ctx* c, Solver* S, const double* b, const double* x, double* rTr) {
     int rc = cpim_apply64(c, x, b, S->rr, S->part_res, NULL);
     if (rc) return rc;
     k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, S->n_groups, S->res, NULL);
@@ -705,10 +712,10 @@ static void record(capac_ctx* c, const SolveOut& o) {
     c->timing.solve_iters += o.iters;
 }
 
-// Solves relay * x = b; x is the warm start and the result.  Fills c->solve_*;
-// returns CAPAC_ERR_SOLVER when the reference would exit (bicg_stab.c:563-569).
-int solver_run(capac_ctx* c, const double* b, double* x) {
-    Solver* S = c->solver;
+// Solves relay * x = b with the work vectors of S; x is the warm start and
+// the result.  Fills c->solve_*; returns CAPAC_ERR_SOLVER when the reference
+// would exit (bicg_stab.c:563-569).
+static int solver_run_with(capac_ctx* c, Solver* S, const double* b, double* x) {
     // first batch: from the previous solves' iteration counts (warm starts
     // change slowly), so few launches are wasted as no-ops; the main solves'
     // counts alone (a refinement's correction solve has its own count, and
@@ -723,7 +730,7 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
         first = (c->batch_main2 > 1 ? (c->batch_main + c->batch_main2) / 2 : c->batch_main) + 1;
     const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
     SolveOut o;
-    int rc = bicgstab(c, b, x, first, refine, &o);
+    int rc = bicgstab(c, S, b, x, first, refine, &o);
     if (rc) return rc;
     c->batch_main2 = c->batch_main;
     c->batch_main = o.iters;
@@ -745,11 +752,11 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
             k_scale<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 1.0 / norm, S->rr);
             SolveOut od;
             const int first_corr = c->batch_corr > 0 ? c->batch_corr + 1 : 4;
-            if ((rc = bicgstab(c, S->rr, S->d, first_corr, false, &od))) return rc;
+            if ((rc = bicgstab(c, S, S->rr, S->d, first_corr, false, &od))) return rc;
             c->batch_corr = od.iters;
             total += od.iters;
             k_axpy<<<(n + 255) / 256, 256, 0, c->stream>>>(n, norm, S->d, x);
-            if ((rc = residual64(c, b, x, &rTr))) return rc;
+            if ((rc = residual64(c, S, b, x, &rTr))) return rc;
             ++rounds;
         }
         o.iters = total;
@@ -763,6 +770,83 @@ int solver_run(capac_ctx* c, const double* b, double* x) {
         return CAPAC_ERR_SOLVER;
     }
     return 0;
+}
+
+int solver_run(capac_ctx* c, const double* b, double* x) { return solver_run_with(c, c->solver, b, x); }
+
+// ---- the solve without library-allocated work space ----
+// The work vectors, partial sums, solver scalars and residual of one solve
+// live in a file-scope device array instead of memory the context allocated,
+// so solver_run_noalloc calls no allocator.  Each device holds its own copy
+// of the array at its own address, looked up once per device at context
+// creation (solver_noalloc_init).  Every call on a device shares its array:
+// calls that overlap on one device race.
+constexpr int kNoAllocMaxUnknowns = 16384;
+constexpr size_t kNoAllocGroups = kNoAllocMaxUnknowns + 1;       // metal row-groups + Lagrange rows, bound
+constexpr size_t kNoAllocVecBlocks = (kNoAllocMaxUnknowns + kVecThreads - 1) / kVecThreads + 1;
+constexpr size_t kNoAllocScWords = 16;                             // SolveScalars, padded
+constexpr size_t kNoAllocWords = 12 * (size_t)kNoAllocMaxUnknowns + 5 * kNoAllocGroups +
+                                 2 * kNoAllocVecBlocks + kNoAllocScWords + 4;
+static_assert(sizeof(SolveScalars) <= kNoAllocScWords * sizeof(double), "solver scalars overflow their slot");
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocWords];
+
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on device dev (the current device)
+int solver_noalloc_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        capac_set_error("device %d is beyond the %d devices the allocation-free solve supports", dev, kMaxDevices);
+        return CAPAC_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void* p = NULL;
+    CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+    return 0;
+}
+
+int solver_run_noalloc(capac_ctx* c, const double* b, double* x) {
+    const int n = c->n_mat;
+    if (n > kNoAllocMaxUnknowns) {
+        capac_set_error("the allocation-free solve takes at most %d CPIM unknowns (this system has %d)",
+                        kNoAllocMaxUnknowns, n);
+        return CAPAC_ERR_ARG;
+    }
+    double* base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : NULL;
+    }
+    if (!base) {
+        capac_set_error("allocation-free work space of device %d was not set up", c->device);
+        return CAPAC_ERR_HIP;
+    }
+    const Solver* own = c->solver;
+    Solver V;
+    memset(&V, 0, sizeof V);
+    V.n_mat = n; V.Nm = own->Nm; V.n_np = own->n_np; V.n_groups = own->n_groups; V.n_vb = own->n_vb;
+    double* p = base;
+    double** vecs[12] = {&V.r0, &V.r, &V.p, &V.v, &V.s, &V.t, &V.y, &V.z, &V.K, &V.Ax, &V.d, &V.rr};
+    for (int k = 0; k < 12; ++k) { *vecs[k] = p; p += kNoAllocMaxUnknowns; }
+    V.part1 = p; p += kNoAllocGroups;
+    V.part3 = p; p += 3 * kNoAllocGroups;
+    V.part_res = p; p += kNoAllocGroups;
+    V.part2 = p; p += kNoAllocVecBlocks;
+    V.part4 = p; p += kNoAllocVecBlocks;
+    V.sc = (SolveScalars*)p; p += kNoAllocScWords;
+    V.res = p; p += 4;
+    V.sc_host = own->sc_host;
+    V.res_host = own->res_host;
+    V.in_red = false;
+    // the array keeps the previous call's contents: clear it as a new
+    // context's work space starts (a fill kernel, no memset node)
+    k_zero<<<(unsigned)((kNoAllocWords + 255) / 256), 256, 0, c->stream>>>((int)kNoAllocWords, base);
+    CHECK_HIP(hipGetLastError());
+    return solver_run_with(c, &V, b, x);
 }
 
 // Warm start of an MD step's solve.  The reference starts from the previous
@@ -805,7 +889,7 @@ double* solver_scratch(capac_ctx* c) { return c->solver->Ax; }
 
 // one plain SpMV over the resident matrix (for kernel timing)
 int solver_spmv_only(capac_ctx* c, const double* x, double* y) {
-    spmv(c, x, y, 0, NULL, NULL);
+    spmv(c, c->solver, x, y, 0, NULL, NULL);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

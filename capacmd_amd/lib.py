@@ -173,7 +173,8 @@ GPU_EXPORTS = [
     "capac_last_error", "capac_version", "capac_create", "capac_destroy",
     "capac_set_state", "capac_get_state", "capac_get_forces", "capac_set_box",
     "capac_force", "capac_neighbours", "capac_nonbonded", "capac_sutton_chen",
-    "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_force",
+    "capac_cpim_rhs", "capac_cpim_assemble", "capac_cpim_solve", "capac_cpim_solve_noalloc",
+    "capac_cpim_force",
     "capac_solve_stats", "capac_md_begin", "capac_md_run", "capac_md_observables",
     "capac_set_thermostat", "capac_get_timing", "capac_reset_timing", "capac_enable_timing",
     "capac_time_kernel", "capac_set_skin", "capac_skin", "capac_set_solve_tolerance",
@@ -214,6 +215,7 @@ def gpu_lib() -> C.CDLL:
         lib.capac_cpim_rhs.argtypes = [C.c_void_p, _dp]
         lib.capac_cpim_assemble.argtypes = [C.c_void_p, _dp, _dp]
         lib.capac_cpim_solve.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, _ip]
+        lib.capac_cpim_solve_noalloc.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, _ip]
         lib.capac_cpim_force.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         lib.capac_solve_stats.argtypes = [C.c_void_p, _ip, _dp, _dp, _ip]
         lib.capac_md_begin.argtypes = [C.c_void_p]
@@ -455,6 +457,16 @@ class Context:
         x = None if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
         _check(self.lib.capac_cpim_solve(self.h, dptr(b), dptr(x), C.byref(it), C.byref(rtr),
                                          C.byref(btb), C.byref(kind)), "capac_cpim_solve")
+        return x, it.value, rtr.value, btb.value, kind.value
+
+    def cpim_solve_noalloc(self, b=None, x=None):
+        """cpim_solve with the work space in the library's device array
+        (capac_cpim_solve_noalloc: the call allocates nothing)."""
+        it = C.c_int(); rtr = C.c_double(); btb = C.c_double(); kind = C.c_int()
+        b = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        x = None if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        _check(self.lib.capac_cpim_solve_noalloc(self.h, dptr(b), dptr(x), C.byref(it), C.byref(rtr),
+                                                 C.byref(btb), C.byref(kind)), "capac_cpim_solve_noalloc")
         return x, it.value, rtr.value, btb.value, kind.value
 
     def cpim_force(self, x=None):

@@ -329,6 +329,18 @@ int capac_cpim_assemble(capac_ctx* ctx, double* dense, double* diag);
 int capac_cpim_solve(capac_ctx* ctx, const double* b, double* x, int* iters,
                      double* rTr, double* bTb, int* kind);
 
+/* capac_cpim_solve without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  The solver's work vectors,
+ * partial sums and scalars live in the library's file-scope device array
+ * (about 2.2 MB per device, holding systems of at most 16384 CPIM unknowns;
+ * larger systems are refused with CAPAC_ERR_ARG) instead of the context's
+ * allocations; the call itself allocates nothing.  Same arguments, kernels
+ * and results as capac_cpim_solve.  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU race. */
+int capac_cpim_solve_noalloc(capac_ctx* ctx, const double* b, double* x, int* iters,
+                             double* rTr, double* bTb, int* kind);
+
 /* mpi_cpff_force (cpff.c:526-950) for the given x (nullable: resident) */
 int capac_cpim_force(capac_ctx* ctx, const double* x, double* f, double pot[CAPAC_NPOT],
                      double vir[9]);
