@@ -746,6 +746,7 @@ static void take_force_results(capac_ctx* c);
 static int force_compute(capac_ctx* c, bool deferred = false) {
     int rc;
     deferred = deferred && c->n_mat > 0;
+    c->timing.n_force_calls++;          // every call, also those whose results stay on the device
     for (int attempt = 0; attempt < 8; ++attempt) {
         mark(c, 0);
         // virtual sites from their atoms first (force.c:2128-2134)
@@ -836,7 +837,6 @@ static void take_force_results(capac_ctx* c) {
         c->pot[8] += h[116];
         for (int k = 0; k < 9; ++k) c->vir[k] += h[117 + k];
     }
-    c->timing.n_force_calls++;
     if (c->timing_on) {
         float ms;
         // (from, to, category): build, RHS, matrix, solve, CPIM force on the
@@ -1180,15 +1180,19 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (rc) return rc;
         solver_predict(c);
         rc = force_compute(c, true);
-        if (rc) return rc;
         if (c->n_mat) {
             // the solve's synchronisation brought the RATTLE flags back (this
             // step's RATTLE_1 and the previous step's RATTLE_2; they stay set
-            // until capac_md_begin)
+            // until capac_md_begin).  They are checked before an error of the
+            // force call is returned: the reference stops at the RATTLE
+            // failure, which comes first (force.c:1990-1994, 2083-2087).  A
+            // call that failed before its solve leaves the flags of an
+            // earlier synchronisation, which were clear or already reported.
             const int* st = (const int*)(c->red_host + kRedStatus);
             if (st[0]) { capac_set_error("RATTLE_1 not converged!"); return CAPAC_ERR_RATTLE1; }
             if (st[1]) { capac_set_error("RATTLE_2 not converged!"); return CAPAC_ERR_RATTLE2; }
         }
+        if (rc) return rc;
         solver_push_history(c);
         mark(c, 9);
         if (!rc) rc = md_kick(c);

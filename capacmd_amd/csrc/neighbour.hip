@@ -448,7 +448,12 @@ int nb_set_radius(capac_ctx* c, double skin) {
     for (int d = 0; d < 3; ++d)
         if (c->bc.box[d] < 2.0 * rl) { skin = 0.0; rl = c->bc.rc; }
     c->skin = skin;
-    c->bc.rl2 = skin > 0.0 ? rl * rl : c->bc.rc2;
+    // Without a skin the rows hold the pairs within rc itself.  The build's
+    // distance test works on wrapped and shifted coordinates, which round
+    // differently from the force kernels' pbc_12 differences, so its radius
+    // is rc widened by a relative 1e-12: a pair within rounding of rc stays
+    // in the row, and the force kernels apply the exact r^2 < rc^2 test.
+    c->bc.rl2 = skin > 0.0 ? rl * rl : c->bc.rc2 * (1.0 + 1e-12);
     return nb_grid_setup(c);
 }
 

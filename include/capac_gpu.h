@@ -50,7 +50,8 @@ extern "C" {
 
 /* precision of the CPIM relay matrix and inner solve */
 #define CAPAC_PREC_FP64          0  /* FP64 matrix, FP64 BiCGSTAB (the reference's arithmetic) */
-/* FP32 relay (T-tensors evaluated in FP32, 7 coefficients per metal pair),
+/* FP32 relay (T-tensors evaluated in FP32, 4 coefficients per metal pair,
+ * the separation formed from the positions where a block is applied),
  * BiCGSTAB on it with FP64 vectors and dots, and FP64 iterative refinement:
  * the reference's stop test is applied to the true residual b - A x with the
  * FP64 relay recomputed on the fly, and corrections are solved for until it
@@ -289,7 +290,9 @@ int capac_force(capac_ctx* ctx, double pot[CAPAC_NPOT], double vir[9], double* f
 
 /* Neighbour list (cell list + the reference's exact cut-off test):
  * per atom i, every j != i not excluded with |pbc_12(r_j - r_i)|^2 < rl^2,
- * rl = rc + capac_skin(ctx) (rc itself when the skin is 0), built now.
+ * rl = rc + capac_skin(ctx), built now.  When the skin is 0, rl^2 is rc^2
+ * widened by a relative 1e-12, so that no pair within rounding of rc is
+ * missed (every force term applies the exact r^2 < rc^2 test itself).
  * count[N]; list[N * max_per_atom] (j indices, unordered); returns the
  * largest row length, or < 0. */
 int capac_neighbours(capac_ctx* ctx, int* count, int* list, int max_per_atom);
