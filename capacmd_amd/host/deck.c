@@ -133,7 +133,7 @@ static void moltype_free(moltype* mt, int n) {
 /* periodic dihedral (funct 1, 4, 9) to the Ryckaert-Bellemans coefficients
  * that compute_dihedral_9 passes on (force.c:1236-1347), the same operations
  * on the same values; -2 for an n it rejects, -1 for a phi0 it rejects */
-static int periodic_to_rb(double phi0, double kphi, int n, double c[6]) {
+int capac_dihedral_periodic_to_rb(double phi0, double kphi, int n, double c[6]) {
     for (int k = 0; k < 6; ++k) c[k] = 0.0;
     if (n < 1 || n > 4) return -2;
     if (phi0 != 0.0 && phi0 != 180.0) return -1;
@@ -287,7 +287,7 @@ static int read_param(capac_deck* d, const char* path, moltype** out_mt, char* e
                         double phi0 = 0.0, kphi = 0.0;
                         int nn = 0;
                         sscanf(rd.line, "%*d%*d%*d%*d%*d%lf%lf%d", &phi0, &kphi, &nn);
-                        int bad = periodic_to_rb(phi0, kphi, nn, tm.par);
+                        int bad = capac_dihedral_periodic_to_rb(phi0, kphi, nn, tm.par);
                         if (bad) {
                             fail(&rd, bad == -2 ? "<> Error: n is not equal to 1, 2, 3 or 4!\n"
                                                   "    phi0=%f, kphi=%f, n=%d\n"

@@ -101,6 +101,13 @@ int  capac_write_binary_path(const char* path, const char* mode, const capac_dec
 /* The reference's energy table (force.c:337-354). */
 void capac_print_potential(FILE* fh, const double* pot);
 
+/* A periodic dihedral (funct 1, 4, 9: phi0 in degrees, kphi, n) as the
+ * Ryckaert-Bellemans coefficients rb[0..5] that the reference's
+ * compute_dihedral_9 forms from it (force.c:1236-1347); capac_params carries
+ * every dihedral in that form.  Returns -2 for an n other than 1-4 and -1 for
+ * a phi0 other than 0 or 180, which the reference rejects. */
+int  capac_dihedral_periodic_to_rb(double phi0, double kphi, int n, double rb[6]);
+
 #ifdef __cplusplus
 }
 #endif

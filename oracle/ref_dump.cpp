@@ -5,6 +5,10 @@
 // writing every intermediate array at full FP64 precision as .npy files:
 //
 //   ref_dump init.gro param.txt mdset.txt OUTDIR [nsteps]
+//   ref_dump init.gro param.txt mdset.txt OUTDIR params
+//
+// The second form writes the capac_params that the reference-side adapter
+// (capacmd_amd/host/ref_adapter.cpp) forms from the reference's structs.
 //
 // The set-up below restates what the reference's main() does before its
 // first force call (mpi-main.c:99-609) with the structs constructed properly;
@@ -27,6 +31,7 @@
 #include "cpff.h"
 #include "bicg_stab.h"
 #include "thermostat.h"
+#include "ref_adapter.h"
 
 // phase functions that force.c defines but its header does not declare
 void mpi_qsc_dens(Task* p_task, const double rCut2, Metal* p_metal, System* p_system,
@@ -360,15 +365,78 @@ static void md(Ref& R, int nsteps) {
   npy_d("md_vq", nhc.data(), nhc.size());
 }
 
+// the capac_params the reference-side adapter (capacmd_amd/host/ref_adapter.cpp)
+// forms from the reference's own structs after its set-up, one .npy per field
+static void dump_params(Ref& R) {
+  char err[512] = "";
+  capac_ref_params* h =
+      capac_params_from_reference(R.top, R.ai, R.mi, R.rs, R.metal, R.sys, err, sizeof err);
+  if (!h) { printf("%s", err); exit(1); }
+  const capac_params* p = capac_ref_params_get(h);
+  const int N = p->n_atoms, M = p->n_mols, nt = p->n_types, C = p->n_cstr;
+  int counts[6] = {N, p->n_metal, p->n_nps, nt, M, C};
+  npy("counts", counts, "<i4", 6, 4);
+  int terms[5] = {p->n_bonds, p->n_pairs14, p->n_angles, p->n_dihedrals, p->n_vsites};
+  npy("term_counts", terms, "<i4", 5, 4);
+  npy_d("charge", p->charge, N);
+  npy_d("mass", p->mass, N);
+  npy("atomtype", p->atomtype, "<i4", N, 4);
+  npy("mol_id", p->mol_id, "<i4", N, 4);
+  npy("local_idx", p->local_idx, "<i4", N, 4);
+  npy("excl_mask", p->excl_mask, "<u4", N, 4);
+  npy("mol_first", p->mol_first, "<i4", M, 4);
+  npy("mol_last", p->mol_last, "<i4", M, 4);
+  npy("mol_cstr", p->mol_cstr, "<i4", M + 1, 4);
+  npy("cstr_i", p->cstr_i, "<i4", C, 4);
+  npy("cstr_j", p->cstr_j, "<i4", C, 4);
+  npy_d("cstr_r", p->cstr_r, C);
+  npy("np_first", p->np_first, "<i4", p->n_nps, 4);
+  npy("np_last", p->np_last, "<i4", p->n_nps, 4);
+  npy_d("np_charge", p->np_charge, p->n_nps);
+  npy("nb_funct", p->nb_funct, "<i4", (size_t)nt * nt, 4);
+  npy_d("nb_c6", p->nb_c6, (size_t)nt * nt);
+  npy_d("nb_c12", p->nb_c12, (size_t)nt * nt);
+  npy_d("nb_a", p->nb_a, (size_t)nt * nt);
+  npy_d("nb_b", p->nb_b, (size_t)nt * nt);
+  double sc[21] = {p->box[0], p->box[1], p->box[2], p->rcut, p->dsf_alpha, (double)p->use_coulomb,
+                   (double)p->use_vdw, (double)p->qsc_n, (double)p->qsc_m, p->qsc_eps, p->qsc_c,
+                   p->qsc_a, (double)p->use_cpff, p->cpff_polar, p->cpff_capac, p->efield[0],
+                   p->efield[1], p->efield[2], p->dt, (double)p->fix_metal, (double)p->nvt};
+  npy_d("scalars", sc, 21);
+  double sc2[4] = {p->ref_temp, p->tau_t, (double)p->n_heating, (double)p->ndf};
+  npy_d("scalars2", sc2, 4);
+  if (p->mol_terms) {
+    npy("mol_terms", p->mol_terms, "<i4", 5 * ((size_t)M + 1), 4);
+    npy("bond_atoms", p->bond_atoms, "<i4", 2 * (size_t)p->n_bonds, 4);
+    npy("bond_funct", p->bond_funct, "<i4", p->n_bonds, 4);
+    npy_d("bond_par", p->bond_par, 3 * (size_t)p->n_bonds);
+    npy("pair14_atoms", p->pair14_atoms, "<i4", 2 * (size_t)p->n_pairs14, 4);
+    npy_d("pair14_par", p->pair14_par, 6 * (size_t)p->n_pairs14);
+    npy("angle_atoms", p->angle_atoms, "<i4", 3 * (size_t)p->n_angles, 4);
+    npy("angle_funct", p->angle_funct, "<i4", p->n_angles, 4);
+    npy_d("angle_par", p->angle_par, 4 * (size_t)p->n_angles);
+    npy("dihedral_atoms", p->dihedral_atoms, "<i4", 4 * (size_t)p->n_dihedrals, 4);
+    npy_d("dihedral_par", p->dihedral_par, 6 * (size_t)p->n_dihedrals);
+    npy("vsite_atoms", p->vsite_atoms, "<i4", 4 * (size_t)p->n_vsites, 4);
+    npy_d("vsite_par", p->vsite_par, 3 * (size_t)p->n_vsites);
+  }
+  capac_ref_params_free(h);
+}
+
 int main(int argc, char** argv) {
   if (argc < 5) {
-    fprintf(stderr, "usage: %s init.gro param.txt mdset.txt OUTDIR [nsteps]\n", argv[0]);
+    fprintf(stderr, "usage: %s init.gro param.txt mdset.txt OUTDIR [nsteps | params]\n", argv[0]);
     return 2;
   }
   MPI_Init(&argc, &argv);
   g_out = argv[4];
   Ref R;
   setup(R, argv[1], argv[2], argv[3]);
+  if (argc > 5 && !strcmp(argv[5], "params")) {
+    dump_params(R);
+    MPI_Finalize();
+    return 0;
+  }
   phases(R);
   if (argc > 5) md(R, atoi(argv[5]));
   MPI_Finalize();
