@@ -692,17 +692,23 @@ static void mark(capac_ctx* c, int k) {
 // kernel gets one or one and a half workgroups per compute unit, so that the
 // solver's workgroups always find free slots (fuller grids delay each of
 // them), the larger share when the warm-start predictor has made the solve
-// short.  Above, the solve streams the relay at HBM speed and the pair work
-// of the larger box is the longer of the two: the full grid lets it finish
-// early.  Measured on MI355X (ms per MD step): Au561 FP32-refined with the
+// short.  Measured on MI355X (ms per MD step): Au561 FP32-refined with the
 // quadratic predictor 0.877 at 256 workgroups, 0.827 at 384, 0.965 at 512,
-// 0.925 at 768; FP64 1.589 at 256, 1.603 at 384, 1.758 at 512; Au2057
-// FP32-refined 3.42 at 634, 3.21 at 2048.  Once the solve had shortened
-// further (0.33 ms per step), 384 workgroups left the pair kernel ~12 us
+// 0.925 at 768; FP64 1.589 at 256, 1.603 at 384, 1.758 at 512.  Once the
+// solve had shortened further (0.33 ms per step), 384 workgroups left the pair kernel ~12 us
 // behind it; 448 let it finish in time: 0.562-0.571 against 0.580-0.581 ms
 // per Au561 step on one box (320: 0.584-0.592).
+// From 1024 metal atoms the pair kernel starts after the assembly (see
+// force_compute) with two workgroups per compute unit: it then shares every
+// compute unit with the solve's launches instead of holding their slots.
+// Au2057 FP32-refined with the quadratic predictor, ms per MD step on one
+// box: 2.53 at 256 workgroups, 2.22 at 384, 2.01 at 512, 2.10 at 640, 2.23
+// at 768, 2.15 at 1024, 2.23 at 1536, 2.20 with the full grid (2048).
+#ifndef CAPAC_SIDE_BLOCKS_LARGE
+#define CAPAC_SIDE_BLOCKS_LARGE 512
+#endif
 static int side_pair_blocks(const capac_ctx* c) {
-    if (c->bc.Nm >= 1024) return 0;
+    if (c->bc.Nm >= 1024) return CAPAC_SIDE_BLOCKS_LARGE;
     return c->pred_order > 0 ? 448 : 256;
 }
 
@@ -756,13 +762,13 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         if ((rc = c->nb_dirty ? nb_build(c) : nb_build_if_needed(c))) return rc;
         c->nb_dirty = 0;
         mark(c, 1);
-        // A full-grid pair launch takes every compute unit for its whole
-        // duration, and launches queued behind it on the main stream wait
-        // for free slots: at Au2057 the 14 us RHS launch measured 450 us on
-        // average.  So the full-grid pair kernel starts once the RHS and the
-        // assembly are done and overlaps only the solve; a capped grid
-        // leaves slots free and starts at once.
-        const bool side_after_matrix = c->n_mat && side_pair_blocks(c) == 0;
+        // A pair launch over the large decks' grid takes every compute unit
+        // for its whole duration, and launches queued behind it on the main
+        // stream wait for free slots: at Au2057 the 14 us RHS launch measured
+        // 450 us on average.  So there the pair kernel starts once the RHS
+        // and the assembly are done and overlaps only the solve; the capped
+        // grid of the small decks leaves slots free and starts at once.
+        const bool side_after_matrix = c->n_mat && c->bc.Nm >= 1024;
         if (!side_after_matrix && (rc = side_forces(c))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {
