@@ -508,6 +508,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
             c->relay_bytes = rows * kRelay32Planes * c->relay_ld * sizeof(float);
             CHECK_HIP(hipMalloc(&c->relay32, c->relay_bytes));
             CHECK_HIP(hipMemset(c->relay32, 0, c->relay_bytes));
+            TRY(upload(&c->metal_u, (const double*)NULL, 4 * ((size_t)Nm + 1)));
         }
         TRY(solver_create(c));
         TRY(solver_noalloc_init(device));
@@ -528,7 +529,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32,
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32, c->metal_u,
                     c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,

@@ -155,6 +155,8 @@ struct capac_ctx {
     int pred_order, pred_count;
     int pred_pending;           // extrapolation order the next force call applies
     float* relay32;
+    double* metal_u;            // FP32 path: metal separations from metal atom 0 in bohr, 4 per atom
+                                // (x, y, z, 0), then a flag at [4 Nm]: 1 when the metal is compact
     int relay_ld;
     size_t relay_bytes;
     Solver* solver;

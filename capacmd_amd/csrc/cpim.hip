@@ -107,6 +107,51 @@ k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restric
     for (int k = 0; k < kRelay32Planes; ++k) dst[(size_t)k * ld] = cf[k];
 }
 
+// Metal separations from metal atom 0, d_j = pbc_12(r_j - r_0) in bohr, and
+// whether they are compact: when every component's spread max_j d_j -
+// min_j d_j is below half the box edge, every pair's minimum-image
+// separation is d_j - d_i (pbc_12 leaves a difference inside half the box
+// alone), and the FP32 product applies its blocks from d without the
+// per-pair image test.  One workgroup; the flag lands at u[4 Nm].
+__global__ void __launch_bounds__(1024)
+k_metal_unwrap(const double* __restrict__ r, BoxConst bc, double* __restrict__ u) {
+    main_stream_priority();
+    __shared__ double s_lo[3][16], s_hi[3][16];
+    const int N = bc.N, Nm = bc.Nm;
+    const double x0 = r[0], y0 = r[N], z0 = r[2 * N];
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
+        double d[3] = {r[j] - x0, r[N + j] - y0, r[2 * N + j] - z0};
+        pbc12(bc, d[0], d[1], d[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], d[k]);
+            hi[k] = fmax(hi[k], d[k]);
+            u[4 * j + k] = d[k] * K_NM2BOHR;
+        }
+        u[4 * j + 3] = 0.0;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], off, 64));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], off, 64));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) { s_lo[k][wave] = lo[k]; s_hi[k][wave] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool compact = true;
+        for (int k = 0; k < 3; ++k) {
+            double a = s_lo[k][0], b = s_hi[k][0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { a = fmin(a, s_lo[k][w]); b = fmax(b, s_hi[k][w]); }
+            compact = compact && (b - a) < bc.half[k] * (1.0 - 1e-9);
+        }
+        u[4 * Nm] = compact ? 1.0 : 0.0;
+    }
+}
+
 // y = A x (or b - A x) with the FP64 relay recomputed pair by pair
 // (relay_block64); one workgroup per row-group (metal i, then the Lagrange
 // rows); part[g] = (out, out) over the group's rows
@@ -529,8 +574,10 @@ int cpim_assemble(capac_ctx* c) {
     if (Nm == 0) return 0;
     // the FP64 path stores nothing: its SpMV forms each block from the positions
     dim3 grid((Nm + 255) / 256, Nm);
-    if (c->precision == CAPAC_PREC_FP32_REFINED)
+    if (c->precision == CAPAC_PREC_FP32_REFINED) {
         k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
+        k_metal_unwrap<<<1, 1024, 0, c->stream>>>(c->r, c->bc, c->metal_u);
+    }
     // the Jacobi diagonal holds constants only (1/alpha, -1/c, 1): formed once
     // and again only when those constants change (fitting parameters)
     if (!c->diag_set || c->diag_polar != c->bc.inv_polar || c->diag_capac != c->bc.inv_capac) {

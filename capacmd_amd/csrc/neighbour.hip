@@ -332,11 +332,13 @@ k_nb_build(const double* __restrict__ sx, const double* __restrict__ sy,
 // workgroup of the launch to finish (a completion count in ctl[3]), the
 // rebuild decision and the clearing of the counters the build accumulates
 // into: one launch where a displacement pass and a decision kernel were two
-__global__ void __launch_bounds__(256)
+constexpr int kDispThreads = 1024;
+__global__ void __launch_bounds__(kDispThreads)
 k_disp_decide(const double* __restrict__ r, const double* __restrict__ r_build, int N,
               unsigned long long* __restrict__ ctl, double limit2, int* __restrict__ cell_count,
               int n_cell, int* __restrict__ overflow, int* __restrict__ tail_list) {
     __shared__ int s_flag;
+    __shared__ double s_max[kDispThreads / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
     if (i < N) {
@@ -346,12 +348,16 @@ k_disp_decide(const double* __restrict__ r, const double* __restrict__ r_build, 
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    // one atomic and one fence per workgroup: the global maximum and the
+    // completion count are contended by every workgroup of the launch
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kDispThreads / 64; ++w) d2 = fmax(d2, s_max[w]);
         atomicMax(&ctl[0], (unsigned long long)__double_as_longlong(d2));
         __threadfence();
+        s_flag = atomicAdd(&ctl[3], 1ull) == (unsigned long long)(gridDim.x - 1);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_flag = atomicAdd(&ctl[3], 1ull) == (unsigned long long)(gridDim.x - 1);
     __syncthreads();
     if (!s_flag) return;
     // the last workgroup: every other one has added its maximum
@@ -520,7 +526,7 @@ int nb_build_if_needed(capac_ctx* c) {
     // half the skin, less a relative 1e-6 so rounding in the distance tests
     // cannot matter at the boundary
     double half = 0.5 * c->skin * (1.0 - 1e-6);
-    k_disp_decide<<<(N + 255) / 256, 256, 0, c->stream>>>(c->r, c->r_build, N, c->nb_ctl, half * half,
+    k_disp_decide<<<(N + kDispThreads - 1) / kDispThreads, kDispThreads, 0, c->stream>>>(c->r, c->r_build, N, c->nb_ctl, half * half,
                                                           c->cell_count, c->n_cell, c->nb_overflow,
                                                           c->tail_list);
     return nb_launch(c);

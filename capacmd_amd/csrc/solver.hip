@@ -67,7 +67,6 @@ struct Solver {
 namespace {
 
 constexpr int kVecThreads = 256;       // vector kernels: one row per thread
-constexpr int kInitThreads = 1024;
 
 // FP64 blocks formed from the positions in the product (VALU work, one
 // row-group per block: R = 1); acc skips j == i (diag applies that block)
@@ -84,6 +83,11 @@ struct RelayM64 {
         __device__ __forceinline__ void acc_n(int j, int Nm, const double* __restrict__ x, double (&o)[R][4]) const {
             static_assert(U == 1, "one column per lane and pass");
             acc(j, x[3 * j], x[3 * j + 1], x[3 * j + 2], x[3 * Nm + j], o);
+        }
+        template <int U>
+        __device__ __forceinline__ void acc_compact(int j, int Nm, const double* __restrict__ x,
+                                                    double (&o)[R][4]) const {
+            acc_n<U>(j, Nm, x, o);
         }
         __device__ __forceinline__ void acc(int j, double x0, double x1, double x2, double x3,
                                             double (&o)[R][4]) const {
@@ -103,6 +107,7 @@ struct RelayM64 {
         return Rows<R>{r, &bc, i0, N, r[i0], r[N + i0], r[2 * N + i0]};
     }
     static constexpr int kUnroll = 1;      // columns per lane and pass
+    __device__ __forceinline__ bool compact() const { return false; }
     __device__ __forceinline__ void diag(double x0, double x1, double x2, double x3, double o[4]) const {
         o[0] += bc.inv_polar * x0;
         o[1] += bc.inv_polar * x1;
@@ -128,14 +133,17 @@ struct RelayG32 {
     const float* a;
     int ld;
     const double* r;
+    const double* u;                // metal separations from metal atom 0 (k_metal_unwrap)
     BoxConst bc;
     template <int R>
     struct Rows {
         const float* cp;            // plane 0 of row-group i0
         int ld, N;
         const double* r;
+        const double* u;
         const BoxConst* bc;
         double xi[R], yi[R], zi[R];
+        double ux[R], uy[R], uz[R];
         // U columns j, j + 256, ... per lane: every load of the U columns is
         // issued before the first block is applied, so a wavefront keeps U
         // columns' worth of loads in flight.  A column past the end is read
@@ -171,6 +179,36 @@ struct RelayG32 {
                                   xv[u][2], xv[u][3], o[q]);
                 }
         }
+        // the same when the metal is compact: the separation is d_j - d_i
+        // from the stored separations in bohr, no image test per pair
+        template <int U>
+        __device__ __forceinline__ void acc_compact(int j0, int Nm, const double* __restrict__ x,
+                                                    double (&o)[R][4]) const {
+            float c[U][R][kRelay32Planes];
+            double xv[U][4], pj[U][3];
+#pragma unroll
+            for (int w = 0; w < U; ++w) {
+                const int j = j0 + 256 * w;
+                const bool in = j < Nm;
+                const int jj = in ? j : Nm - 1;
+#pragma unroll
+                for (int q = 0; q < R; ++q)
+#pragma unroll
+                    for (int k = 0; k < kRelay32Planes; ++k)
+                        c[w][q][k] = cp[(size_t)(q * kRelay32Planes + k) * ld + jj];
+                xv[w][0] = x[3 * jj]; xv[w][1] = x[3 * jj + 1]; xv[w][2] = x[3 * jj + 2];
+                xv[w][3] = x[3 * Nm + jj];
+                if (!in) xv[w][0] = xv[w][1] = xv[w][2] = xv[w][3] = 0.0;
+                const double4 d = reinterpret_cast<const double4*>(u)[jj];
+                pj[w][0] = d.x; pj[w][1] = d.y; pj[w][2] = d.z;
+            }
+#pragma unroll
+            for (int w = 0; w < U; ++w)
+#pragma unroll
+                for (int q = 0; q < R; ++q)
+                    block32_apply(pj[w][0] - ux[q], pj[w][1] - uy[q], pj[w][2] - uz[q], c[w][q][0], c[w][q][1],
+                                  c[w][q][2], c[w][q][3], xv[w][0], xv[w][1], xv[w][2], xv[w][3], o[q]);
+        }
     };
     template <int R>
     __device__ __forceinline__ Rows<R> rows(int i0) const {
@@ -178,14 +216,16 @@ struct RelayG32 {
         const int N = bc.N, Nm = bc.Nm;
         Rows<R> w;
         w.cp = a + (size_t)i0 * kRelay32Planes * ld;
-        w.ld = ld; w.N = N; w.r = r; w.bc = &bc;
+        w.ld = ld; w.N = N; w.r = r; w.u = u; w.bc = &bc;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int i = i0 + q < Nm ? i0 + q : Nm - 1;
             w.xi[q] = r[i]; w.yi[q] = r[N + i]; w.zi[q] = r[2 * N + i];
+            w.ux[q] = u[4 * i]; w.uy[q] = u[4 * i + 1]; w.uz[q] = u[4 * i + 2];
         }
         return w;
     }
+    __device__ __forceinline__ bool compact() const { return u[4 * bc.Nm] != 0.0; }
     __device__ __forceinline__ void diag(double x0, double x1, double x2, double x3, double o[4]) const {
         o[0] += bc.inv_polar * x0;
         o[1] += bc.inv_polar * x1;
@@ -311,7 +351,10 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
     } else {
         const auto rows = m.template rows<R>(i0);
         constexpr int U = M::kUnroll;
-        for (int j = threadIdx.x; j < Nm; j += 256 * U) rows.template acc_n<U>(j, Nm, x, o);
+        if (m.compact())
+            for (int j = threadIdx.x; j < Nm; j += 256 * U) rows.template acc_compact<U>(j, Nm, x, o);
+        else
+            for (int j = threadIdx.x; j < Nm; j += 256 * U) rows.template acc_n<U>(j, Nm, x, o);
     }
 #pragma unroll
     for (int q = 0; q < R; ++q)
@@ -334,38 +377,56 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
     spmv_group_out(m, g, Nm, np_of, x, y, mode, r0, K, part, oq);
 }
 
-// start of BiCGSTAB (bicg_stab.c:294-404): bTb, criteria, r0 = r = b - Ax,
-// v = 0, K = 1/diag, rho = (r0, r) with the two early exits, and the first
-// direction p = r + rho (0 - 0) = r, y = K p
-__global__ void __launch_bounds__(kInitThreads)
+// start of BiCGSTAB (bicg_stab.c:294-404) in two launches.  k_bicg_init,
+// one thread per row: r0 = r = b - Ax, v = 0, K = 1/diag, the first
+// direction p = r + rho (0 - 0) = r and y = K p, and per-block partial sums
+// of (b, b) and (r, r).  k_bicg_scalars, one workgroup: bTb, criteria, rho =
+// (r0, r) and the two early exits, which mark the solve done before any
+// iteration runs (x unchanged).
+__global__ void __launch_bounds__(kVecThreads)
 k_bicg_init(int n_mat, int n4, const double* __restrict__ b, const double* __restrict__ Ax,
             const double* __restrict__ diag, double* __restrict__ r0, double* __restrict__ r,
             double* __restrict__ p, double* __restrict__ v, double* __restrict__ K,
-            double* __restrict__ y, SolveScalars* sc, double crit_factor) {
+            double* __restrict__ y, double* __restrict__ part_bb, double* __restrict__ part_rr) {
     main_stream_priority();
-    __shared__ double s_w[16];
-    double bb = 0.0, rr = 0.0;
-    for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
-        double bk = b[k];
-        double rk = bk - Ax[k];
+    __shared__ double s_w[16 * 2];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0.0, 0.0};
+    if (k < n_mat) {
+        const double bk = b[k];
+        const double rk = bk - Ax[k];
+        const double Kk = k < n4 ? 1.0 / diag[k] : 1.0;
         r0[k] = rk;
         r[k] = rk;
         v[k] = 0.0;
-        K[k] = k < n4 ? 1.0 / diag[k] : 1.0;
-        bb += bk * bk;
-        rr += rk * rk;
+        K[k] = Kk;
+        p[k] = rk;
+        y[k] = rk * Kk;
+        acc[0] = bk * bk;
+        acc[1] = rk * rk;
     }
-    double bTb = block_sum(bb, s_w);
-    double rho = block_sum(rr, s_w);
-    const bool zero_rhs = bTb < 1.0e-10;
-    const bool zero_rho = !zero_rhs && rho < 1.0e-10;
-    if (!zero_rhs && !zero_rho)
-        for (int k = threadIdx.x; k < n_mat; k += blockDim.x) {
-            double pk = r[k];
-            p[k] = pk;
-            y[k] = pk * K[k];
-        }
+    block_sum_n<2>(acc, s_w);
     if (threadIdx.x == 0) {
+        part_bb[blockIdx.x] = acc[0];
+        part_rr[blockIdx.x] = acc[1];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_bicg_scalars(int n_vb, const double* __restrict__ part_bb, const double* __restrict__ part_rr,
+               SolveScalars* sc, double crit_factor) {
+    main_stream_priority();
+    __shared__ double s_w[16 * 2];
+    double red[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < n_vb; k += blockDim.x) {
+        red[0] += part_bb[k];
+        red[1] += part_rr[k];
+    }
+    block_sum_n<2>(red, s_w);
+    if (threadIdx.x == 0) {
+        const double bTb = red[0], rho = red[1];
+        const bool zero_rhs = bTb < 1.0e-10;
+        const bool zero_rho = !zero_rhs && rho < 1.0e-10;
         sc->bTb = bTb;
         sc->criteria = bTb * crit_factor;          // 1e-8 in the reference
         sc->rho_s[0] = 1.0;
@@ -613,7 +674,7 @@ static void spmv_t(capac_ctx* c, const Solver* S, const M& m, const double* x, d
 static void spmv(capac_ctx* c, const Solver* S, const double* x, double* y, int mode, double* part,
                  const SolveScalars* sc) {
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
-        RelayG32 m{c->relay32, c->relay_ld, c->r, c->bc};
+        RelayG32 m{c->relay32, c->relay_ld, c->r, c->metal_u, c->bc};
         const int nm = c->bc.Nm;
         if (nm >= CAPAC_G32_LARGE_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_LARGE>(c, S, m, x, y, mode, part, sc);
         else if (nm >= CAPAC_G32_MID_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_MID>(c, S, m, x, y, mode, part, sc);
@@ -640,8 +701,11 @@ static int bicgstab(capac_ctx* c, Solver* S, const double* b, double* x, int fir
     const int vb = S->n_vb;
     hipStream_t st = c->stream;
     spmv(c, S, x, S->Ax, 0, NULL, NULL);
-    k_bicg_init<<<1, kInitThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K,
-                                            S->y, S->sc, c->solve_crit);
+    // the partial sums of the start go where the iterations' (r0, s) and
+    // (r, r) partials go later; those are written before they are read
+    k_bicg_init<<<vb, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->r0, S->r, S->p, S->v, S->K, S->y,
+                                            S->part2, S->part4);
+    k_bicg_scalars<<<1, 256, 0, st>>>(vb, S->part2, S->part4, S->sc, c->solve_crit);
     // K1, K2 of iteration 1, then units (K3, K4 of n; K1, K2 of n+1): each
     // unit ends with a stop test, so a batch of units can end the solve
     spmv(c, S, S->y, S->v, 1, S->part1, S->sc);
