@@ -383,6 +383,10 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
             t[k].inv_width = p->nb_a[k] != 0.0 ? 1.0 / p->nb_a[k] : 0.0;
             t[k].a = p->nb_a[k];
             t[k].b = p->nb_b[k];
+            const double inv_rc12 = 1.0 / pow(p->rcut, 12.0), inv_rc6 = 1.0 / pow(p->rcut, 6.0);
+            t[k].lj_fc = 12.0 * t[k].c12 * inv_rc12 - 6.0 * t[k].c6 * inv_rc6;
+            t[k].lj_fc_rc = t[k].lj_fc / p->rcut;
+            t[k].lj_ec = t[k].c12 * inv_rc12 - t[k].c6 * inv_rc6;
         }
         rc = upload(&c->pair, (const PairParam*)t, nt * nt);
         free(t);

@@ -68,6 +68,9 @@ struct PairParam {          // one entry of the n_types x n_types table
     int funct;                  // 1 LJ + Coulomb, 2 Buckingham, 3 Morse, 4 erf vdW
     double c6, c12, width, inv_width;
     double a, b;                // Buckingham A, B or Morse D, a (R in c6)
+    // shifted-force LJ constants of the pair type (force.c:748-757):
+    // f_c = 12 C12 / rc^12 - 6 C6 / rc^6, f_c / rc, and C12 / rc^12 - C6 / rc^6
+    double lj_fc, lj_fc_rc, lj_ec;
 };
 
 // BiCGSTAB scalars, resident on the device (bicg_stab.c:289-292)

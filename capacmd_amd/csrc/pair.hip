@@ -25,14 +25,15 @@ struct PairOut {
 };
 
 // The reference's expressions with each division by r, r^2, r^6 or the erf
-// width replaced by a product with the reciprocal, computed once per pair
-// (one FP64 division instead of up to five; results move by rounding only).
+// width replaced by a product with 1/r from one reciprocal square root per
+// pair, and r = r^2 (1/r); the shifted-force LJ constants of the pair type
+// come from the table (results move by rounding only).
 __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
                                           double dx, double dy, double dz, double r2, PairOut& o,
                                           const double* __restrict__ erfcx_coef) {
     double flj = 0.0, fqq = 0.0;
-    const double rij = sqrt(r2);
-    const double inv_r = 1.0 / rij;
+    const double inv_r = rsqrt(r2);
+    const double rij = r2 * inv_r;
     const double inv_r2 = inv_r * inv_r;
     if (pp.funct == 1) {
         if (pp.c6 != 0.0 || pp.c12 != 0.0) {
@@ -41,10 +42,8 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
             if (bc.use_vdw == 1) {
                 // shifted force LJ (force.c:748-757)
                 double f_r = 12.0 * cr12 - 6.0 * cr6;
-                double f_c = 12.0 * pp.c12 * bc.inv_rc12 - 6.0 * pp.c6 * bc.inv_rc6;
-                flj = (f_r - f_c) * inv_r2;
-                o.e8 += (cr12 - cr6) + (rij - bc.rc) * f_c / bc.rc -
-                        (pp.c12 * bc.inv_rc12 - pp.c6 * bc.inv_rc6);
+                flj = (f_r - pp.lj_fc) * inv_r2;
+                o.e8 += (cr12 - cr6) + (rij - bc.rc) * pp.lj_fc_rc - pp.lj_ec;
             } else {
                 flj = (2.0 * cr12 - cr6) * 6.0 * inv_r2;
                 o.e8 += cr12 - cr6;
