@@ -62,6 +62,17 @@ def spmv_bytes(nm: int, n_mat: int, w: int) -> float:
     return w * (8.0 * nm * nm - 3.0 * nm) + 16.0 * n_mat
 
 
+def format_bytes(path: str, nm: int, n_mat: int):
+    """Bytes the SpMV's storage format moves per launch: the FP32 relay's four
+    coefficients per ordered metal pair over rows padded to a multiple of 64
+    plus the two vectors; None for the matrix-free FP64 product (it reads
+    positions and vectors only)."""
+    if path != "fp32":
+        return None
+    ld = (nm + 63) // 64 * 64
+    return 16.0 * nm * ld + 16.0 * n_mat
+
+
 def traffic_from_profiles(kernel: str, algorithmic_bytes: float):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), as committed under
@@ -183,8 +194,15 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, k
         klist.append(("apply64", lib.KERNEL_APPLY64))
     klist.append(("solve", lib.KERNEL_SOLVE))       # last: from the last MD step's warm start
     for name, kid in klist:
-        kern[name] = ctx.time_kernel(kid, 10 if kid == lib.KERNEL_SOLVE else 20)
+        # the SpMV, the roofline's kernel, over 100 launches so that the event
+        # pair's own overhead is small against them
+        kern[name] = ctx.time_kernel(kid, 10 if kid == lib.KERNEL_SOLVE else (100 if kid == lib.KERNEL_SPMV else 20))
     out["kernel_ms"] = kern
+    # the pair kernel's work: every entry of every full row (rc + skin) is
+    # one distance test, and each pair inside rc is evaluated from both ends
+    entries = int(ctx.neighbour_counts().astype(np.int64).sum())
+    out["pair_row_entries"] = entries
+    out["pair_Gentries_per_s"] = round(entries / (kern["pair"] * 1e-3) / 1e9, 2)
     ctx.close()
     return out
 
@@ -340,6 +358,14 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": nbytes,
                 "algorithmic_bytes_formula": f"{w} (8 N_m^2 - 3 N_m) + 16 n_mat, N_m = {nm}, n_mat = {n_mat}",
                 "ms_per_launch": spmv_ms}
+    fmt = format_bytes(best, nm, n_mat)
+    if fmt:
+        # what the kernel's format moves per launch, and the fraction of HBM
+        # peak on those bytes: the 4-coefficient FP32 relay stores 16 B per
+        # ordered pair where the unique FP32 entries take 32 B per pair, so
+        # the fraction on the algorithmic bytes can exceed 1
+        roofline["format_bytes_per_launch"] = fmt
+        roofline["format_frac"] = round(fmt / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     for name, p in paths.items():
         if "kernel_ms" in p:
             b = spmv_bytes(nm, n_mat, 8 if name == "fp64" else 4)

@@ -408,6 +408,12 @@ class Context:
                    "capac_neighbours")
         return cnt, lst.reshape(self.N, max_per_atom), m
 
+    def neighbour_counts(self):
+        """entries of each atom's full neighbour row (rc + skin), no rows"""
+        cnt = np.zeros(self.N, dtype=np.int32)
+        _check(self.lib.capac_neighbours(self.h, iptr(cnt), None, 0), "capac_neighbours")
+        return cnt
+
     def nonbonded(self):
         f = np.empty(3 * self.N); pot = np.zeros(NPOT); vir = np.zeros(9)
         _check(self.lib.capac_nonbonded(self.h, dptr(f), dptr(pot), dptr(vir)), "capac_nonbonded")
