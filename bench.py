@@ -62,13 +62,18 @@ def spmv_bytes(nm: int, n_mat: int, w: int) -> float:
     return w * (8.0 * nm * nm - 3.0 * nm) + 16.0 * n_mat
 
 
-def format_bytes(path: str, nm: int, n_mat: int):
-    """Bytes the SpMV's storage format moves per launch: the FP32 relay's four
-    coefficients per ordered metal pair over rows padded to a multiple of 64
-    plus the two vectors; None for the matrix-free FP64 product (it reads
-    positions and vectors only)."""
+def format_bytes(path: str, layout, nm: int, n_mat: int):
+    """Bytes the SpMV's storage format moves per launch, plus 16 n_mat for
+    the vectors: the FP32 relay's four coefficients (16 B) per ordered metal
+    pair over rows padded to a multiple of 64 (ROWS), or per slot of the
+    Nb (Nb + 1) / 2 tiles of 64 x 64 (TILES, Nb = ceil(N_m / 64)) plus its
+    partial sums written and read once (2 x 2 KB per block pair); None for
+    the matrix-free FP64 product (it reads positions and vectors only)."""
     if path != "fp32":
         return None
+    if layout == "tiles":
+        nb = (nm + 63) // 64
+        return 16.0 * 4096 * nb * (nb + 1) / 2 + 4096.0 * nb * nb + 16.0 * n_mat
     ld = (nm + 63) // 64 * 64
     return 16.0 * nm * ld + 16.0 * n_mat
 
@@ -128,8 +133,13 @@ def equilibrate(deck, steps: int):
     return {"r": r, "v": v, "x": x, "vq": obs["vq"], "temp": obs["temp"], "steps": steps}
 
 
+FP32_LAYOUT = lib.LAYOUT_AUTO        # --fp32-layout
+
+
 def start_context(deck, precision: int, eq, predictor=None):
     ctx = lib.Context(deck, precision=precision)
+    if precision == lib.PREC_FP32_REFINED and FP32_LAYOUT != lib.LAYOUT_AUTO:
+        ctx.set_fp32_layout(FP32_LAYOUT)
     ctx.set_state(r=eq["r"], v=eq["v"], x=eq["x"])
     ctx.set_thermostat(eq["vq"])
     if predictor is not None:
@@ -180,6 +190,7 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, k
         "nb_builds_per_step": builds / steps,
         "nb_skin_nm": ctx.skin,
         "warm_start_predictor": ctx.predictor,
+        "fp32_layout": {1: "rows", 2: "tiles"}.get(ctx.fp32_layout),
         "temp_start_of_timed_window": t_start,
         "temp_end_of_timed_window": t_end,
     }
@@ -307,6 +318,8 @@ def main() -> int:
     ap.add_argument("--equil", type=int, default=1500,
                     help="untimed MD steps before the paths start (thermal equilibration)")
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--fp32-layout", choices=["auto", "rows", "tiles"], default="auto",
+                    help="storage layout of the FP32 relay (capac_set_fp32_layout)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     a = ap.parse_args()
@@ -314,6 +327,8 @@ def main() -> int:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the whole box lives on one MI355X)", file=sys.stderr)
         return 2
+    global FP32_LAYOUT
+    FP32_LAYOUT = {"auto": lib.LAYOUT_AUTO, "rows": lib.LAYOUT_ROWS, "tiles": lib.LAYOUT_TILES}[a.fp32_layout]
     deck_spec = gen.config(a.config)
     tmp = tempfile.mkdtemp(prefix="capac_bench_")
     gen.write_deck(deck_spec, tmp)
@@ -350,20 +365,22 @@ def main() -> int:
     spmv_ms = main_path["kernel_ms"]["spmv"]
     nbytes = spmv_bytes(nm, n_mat, w)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"kernel": "relay SpMV (" + ("FP64" if best == "fp64" else "FP32") + " relay, one "
-                          "BiCGSTAB half-iteration)",
+    layout = main_path.get("fp32_layout")
+    roofline = {"kernel": "relay SpMV (" + ("FP64 matrix-free" if best == "fp64" else f"FP32 {layout} layout")
+                          + " relay, one BiCGSTAB half-iteration)",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles("spmv_" + best, nbytes),
+                "traffic": traffic_from_profiles("spmv_" + best + (f"_{layout}" if best == "fp32" else ""), nbytes),
                 "algorithmic_bytes_per_launch": nbytes,
                 "algorithmic_bytes_formula": f"{w} (8 N_m^2 - 3 N_m) + 16 n_mat, N_m = {nm}, n_mat = {n_mat}",
                 "ms_per_launch": spmv_ms}
-    fmt = format_bytes(best, nm, n_mat)
+    fmt = format_bytes(best, main_path.get("fp32_layout"), nm, n_mat)
     if fmt:
         # what the kernel's format moves per launch, and the fraction of HBM
         # peak on those bytes: the 4-coefficient FP32 relay stores 16 B per
-        # ordered pair where the unique FP32 entries take 32 B per pair, so
-        # the fraction on the algorithmic bytes can exceed 1
+        # ordered pair (ROWS) or per unordered pair (TILES) where the unique
+        # FP32 entries take 32 B per unordered pair, so the fraction on the
+        # algorithmic bytes can exceed 1
         roofline["format_bytes_per_launch"] = fmt
         roofline["format_frac"] = round(fmt / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     for name, p in paths.items():

@@ -2,6 +2,7 @@
 // call that replaces the reference's mpi_force (force.c:2097-2320), the
 // sub-operator entry points and the device MD loop (mpi-main.c:874-1013).
 #include "capac_internal.h"
+#include "relay_tile.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -156,6 +157,35 @@ static int upload(T** dst, const T* src, size_t n) {
     CHECK_HIP(hipMalloc(dst, bytes));
     if (src && n) CHECK_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
     else CHECK_HIP(hipMemset(*dst, 0, bytes));
+    return 0;
+}
+
+// (re)allocates the FP32 relay in `layout` (AUTO: tiles from
+// CAPAC_TILES_MIN_METAL metal atoms on), zeroed: ROWS padded with zero
+// coefficient rows up to a multiple of kRelay32RowPad (the SpMV reads whole
+// row blocks), TILES with its partial-sum buffer.  The contents are formed by
+// the next assembly.
+static int relay32_alloc(capac_ctx* c, int layout) {
+    const int Nm = c->bc.Nm;
+    if (layout == CAPAC_LAYOUT_AUTO) layout = Nm >= CAPAC_TILES_MIN_METAL ? CAPAC_LAYOUT_TILES : CAPAC_LAYOUT_ROWS;
+    if (c->relay32) { (void)hipFree(c->relay32); c->relay32 = NULL; }
+    if (c->relay_part) { (void)hipFree(c->relay_part); c->relay_part = NULL; }
+    if (c->relay_cnt) { (void)hipFree(c->relay_cnt); c->relay_cnt = NULL; }
+    c->relay_layout = layout;
+    c->relay_nb = tile_blocks(Nm);
+    if (layout == CAPAC_LAYOUT_TILES) {
+        c->relay_bytes = (size_t)tile_count(c->relay_nb) * kTileSlots * 4 * sizeof(float);
+        const size_t part = (size_t)c->relay_nb * c->relay_nb * 4 * kTileEdge * sizeof(double);
+        CHECK_HIP(hipMalloc(&c->relay_part, part));
+        CHECK_HIP(hipMemset(c->relay_part, 0, part));
+        CHECK_HIP(hipMalloc(&c->relay_cnt, sizeof(int) * c->relay_nb));
+        CHECK_HIP(hipMemset(c->relay_cnt, 0, sizeof(int) * c->relay_nb));
+    } else {
+        const size_t rows = ((size_t)Nm + kRelay32RowPad - 1) / kRelay32RowPad * kRelay32RowPad;
+        c->relay_bytes = rows * kRelay32Planes * c->relay_ld * sizeof(float);
+    }
+    CHECK_HIP(hipMalloc(&c->relay32, c->relay_bytes));
+    CHECK_HIP(hipMemset(c->relay32, 0, c->relay_bytes));
     return 0;
 }
 
@@ -507,11 +537,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         // the FP64 path stores no relay (its product forms the blocks)
         c->relay_ld = ((Nm + 63) / 64) * 64;
         if (precision == CAPAC_PREC_FP32_REFINED) {
-            // rows padded with zero coefficients (the SpMV reads whole row blocks)
-            const size_t rows = ((size_t)Nm + kRelay32RowPad - 1) / kRelay32RowPad * kRelay32RowPad;
-            c->relay_bytes = rows * kRelay32Planes * c->relay_ld * sizeof(float);
-            CHECK_HIP(hipMalloc(&c->relay32, c->relay_bytes));
-            CHECK_HIP(hipMemset(c->relay32, 0, c->relay_bytes));
+            TRY(relay32_alloc(c, CAPAC_LAYOUT_AUTO));
             TRY(upload(&c->metal_u, (const double*)NULL, 4 * ((size_t)Nm + 1)));
         }
         TRY(solver_create(c));
@@ -533,8 +559,8 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32, c->metal_u,
-                    c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
+                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32, c->relay_part,
+                    c->relay_cnt, c->metal_u, c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,
                     c->vsite_par, c->tail_list};
@@ -645,6 +671,25 @@ extern "C" int capac_fit_frame(capac_ctx* c, const double* f_ref, double e_ref, 
                                double grad[CAPAC_NTHETA]) {
     if (!c || !f_ref) return CAPAC_ERR_ARG;
     return fit_frame_loss_grad(c, f_ref, e_ref, w_e, loss, grad);
+}
+
+extern "C" int capac_set_fp32_layout(capac_ctx* c, int layout) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (c->precision != CAPAC_PREC_FP32_REFINED || !c->n_mat) {
+        capac_set_error("the relay layout applies to the FP32-refined path with metal atoms only");
+        return CAPAC_ERR_ARG;
+    }
+    if (layout != CAPAC_LAYOUT_AUTO && layout != CAPAC_LAYOUT_ROWS && layout != CAPAC_LAYOUT_TILES) {
+        capac_set_error("unknown relay layout %d", layout);
+        return CAPAC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    return relay32_alloc(c, layout);
+}
+
+extern "C" int capac_fp32_layout(const capac_ctx* c) {
+    return c && c->precision == CAPAC_PREC_FP32_REFINED && c->n_mat ? c->relay_layout : -1;
 }
 
 extern "C" int capac_set_predictor(capac_ctx* c, int order) {

@@ -7,13 +7,18 @@
 //   neighbour rows        nbr[i * nb_stride + k]: non-metal neighbours from the
 //                         front (k < n_front[i]), metal neighbours from the back
 //                         (k >= nb_stride - n_back[i]); int32
-//   relay (FP32-refined)  4 FP32 coefficient planes per metal row-group i,
-//                         plane k of pair (i,j) at relay32[(i*4 + k)*relay_ld + j]:
-//                         g_a = c2 - 3 c1, g_b = c1 r^2, c3, cq (relay.h); the
-//                         separation is formed from the positions where a block
-//                         is applied, and the diagonal blocks analytically.  The
-//                         FP64 path stores no relay: its product forms each block
-//                         from the positions.
+//   relay (FP32-refined)  4 FP32 coefficients per metal pair: g_a = c2 - 3 c1,
+//                         g_b = c1 r^2, c3, cq (relay.h); the separation is formed
+//                         from the positions where a block is applied, and the
+//                         diagonal blocks analytically.  Two layouts:
+//                         rows   plane k of ordered pair (i,j) at
+//                                relay32[(i*4 + k)*relay_ld + j];
+//                         tiles  each unordered pair once: 64 x 64 tiles (I <= J)
+//                                of metal blocks, tile t's pair (I*64 + l,
+//                                J*64 + ((l + k) & 63)) as float4 at
+//                                relay32[4 (t*4096 + k*64 + l)] (relay_tile.h).
+//                         The FP64 path stores no relay: its product forms each
+//                         block from the positions.
 //   CPIM vector           x = [p (3Nm) | q (Nm) | lambda (n_np)], atomic units
 #pragma once
 
@@ -158,6 +163,10 @@ struct capac_ctx {
     int pred_order, pred_count;
     int pred_pending;           // extrapolation order the next force call applies
     float* relay32;
+    int relay_layout;           // CAPAC_LAYOUT_ROWS or CAPAC_LAYOUT_TILES (FP32 path)
+    int relay_nb;               // tiles: metal blocks of 64 per edge
+    double* relay_part;         // tiles: the product's partial sums, Nb x Nb x 4 x 64
+    int* relay_cnt;             // tiles: partials delivered per row block in the running product (Nb)
     double* metal_u;            // FP32 path: metal separations from metal atom 0 in bohr, 4 per atom
                                 // (x, y, z, 0), then a flag at [4 Nm]: 1 when the metal is compact
     int relay_ld;

@@ -14,6 +14,7 @@
 #include "capac_internal.h"
 #include "erfc_fast.h"
 #include "relay.h"
+#include "relay_tile.h"
 
 namespace {
 
@@ -105,6 +106,34 @@ k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restric
     float* dst = relay + (size_t)i * kRelay32Planes * ld + j;
 #pragma unroll
     for (int k = 0; k < kRelay32Planes; ++k) dst[(size_t)k * ld] = cf[k];
+}
+
+// The same coefficients in the TILES layout (relay_tile.h): one workgroup
+// per tile, the threads over its 4096 slots; each unordered pair is formed
+// once, from metal_sep(i, j) with i in the tile's row block (the reversed
+// separation is its exact negative, and the coefficients depend on r^2
+// only).  Slot k = 0 of a diagonal tile (i = j) and pairs past Nm hold zeros.
+__global__ void __launch_bounds__(256)
+k_assemble32_tiles(const double* __restrict__ r, BoxConst bc, int Nb, float4* __restrict__ tiles) {
+    main_stream_priority();
+    const int Nm = bc.Nm;
+    int I, J;
+    tile_coords(blockIdx.x, Nb, I, J);
+    float4* dst = tiles + (size_t)blockIdx.x * kTileSlots;
+    for (int e = threadIdx.x; e < kTileSlots; e += blockDim.x) {
+        const int k = e / kTileEdge, l = e % kTileEdge;
+        const int i = I * kTileEdge + l, j = J * kTileEdge + ((l + k) & (kTileEdge - 1));
+        float4 cf = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i != j && i < Nm && j < Nm) {
+            double dx, dy, dz;
+            metal_sep(r, bc, i, j, dx, dy, dz);
+            float rx = (float)dx, ry = (float)dy, rz = (float)dz;
+            float r2 = (rx * rx + ry * ry) + rz * rz;
+            TCoef32 t = t_coef32(r2, (float)bc.inv_R_pp, (float)bc.inv_R_pq, (float)bc.inv_R_qq);
+            cf = make_float4(t.c2 - 3.0f * t.c1, t.c1 * r2, t.c3, t.cq);
+        }
+        dst[e] = cf;
+    }
 }
 
 // Metal separations from metal atom 0, d_j = pbc_12(r_j - r_0) in bohr, and
@@ -227,8 +256,8 @@ __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
 
 // dense 4Nm x 4Nm metal block of either relay (tests): FP32 from the stored
 // coefficients and the separation, FP64 from the positions as the SpMV forms it
-__global__ void k_dense(const float* __restrict__ relay, const double* __restrict__ r, BoxConst bc, int ld,
-                        double* __restrict__ dense) {
+__global__ void k_dense(const float* __restrict__ relay, int tiles, int Nb, const double* __restrict__ r,
+                        BoxConst bc, int ld, double* __restrict__ dense) {
     const int Nm = bc.Nm;
     long n4 = 4L * Nm;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,10 +271,16 @@ __global__ void k_dense(const float* __restrict__ relay, const double* __restric
     if (i == j) {
         v = (a != b) ? 0.0 : (a < 3 ? bc.inv_polar : -bc.inv_capac);
     } else if (relay) {
-        const float* cp = relay + (size_t)i * kRelay32Planes * ld + j;
+        double ga, gb, c3, cq;
+        if (tiles) {
+            const float4 cf = reinterpret_cast<const float4*>(relay)[tile_slot(i, j, Nb)];
+            ga = cf.x; gb = cf.y; c3 = cf.z; cq = cf.w;
+        } else {
+            const float* cp = relay + (size_t)i * kRelay32Planes * ld + j;
+            ga = cp[0]; gb = cp[ld]; c3 = cp[2 * ld]; cq = cp[3 * ld];
+        }
         double rv[3];
         metal_sep(r, bc, i, j, rv[0], rv[1], rv[2]);
-        double ga = cp[0], gb = cp[ld], c3 = cp[2 * ld], cq = cp[3 * ld];
         if (a < 3 && b < 3) v = ga * rv[a] * rv[b] + (a == b ? gb : 0.0);
         else if (a < 3) v = c3 * rv[a];
         else if (b < 3) v = -c3 * rv[b];
@@ -575,7 +610,11 @@ int cpim_assemble(capac_ctx* c) {
     // the FP64 path stores nothing: its SpMV forms each block from the positions
     dim3 grid((Nm + 255) / 256, Nm);
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
-        k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
+        if (c->relay_layout == CAPAC_LAYOUT_TILES)
+            k_assemble32_tiles<<<(unsigned)tile_count(c->relay_nb), 256, 0, c->stream>>>(
+                c->r, c->bc, c->relay_nb, reinterpret_cast<float4*>(c->relay32));
+        else
+            k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
         k_metal_unwrap<<<1, 1024, 0, c->stream>>>(c->r, c->bc, c->metal_u);
     }
     // the Jacobi diagonal holds constants only (1/alpha, -1/c, 1): formed once
@@ -594,7 +633,8 @@ int cpim_dense_expand(capac_ctx* c, double* dense) {
     long n4 = 4L * c->bc.Nm;
     unsigned blocks = (unsigned)((n4 * n4 + 255) / 256);
     k_dense<<<blocks, 256, 0, c->stream>>>(c->precision == CAPAC_PREC_FP32_REFINED ? c->relay32 : NULL,
-                                           c->r, c->bc, c->relay_ld, dense);
+                                           c->relay_layout == CAPAC_LAYOUT_TILES, c->relay_nb, c->r, c->bc,
+                                           c->relay_ld, dense);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

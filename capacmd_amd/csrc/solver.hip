@@ -41,6 +41,7 @@
 // and adds it (iterative refinement).
 #include "capac_internal.h"
 #include "relay.h"
+#include "relay_tile.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -377,6 +378,277 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
     spmv_group_out(m, g, Nm, np_of, x, y, mode, r0, K, part, oq);
 }
 
+// Row block b's outputs from its Nb partials P[b][q] (relay_tile.h):
+// thread (g, l) sums the four components of row l over the source blocks
+// q = g, g + G, ... (G = blockDim / 64 groups), all of its loads issued
+// before the first add, so the partials arrive in about one memory round
+// trip; the G groups are added in order, then row l's diagonal block,
+// Lagrange column and outputs follow as k_spmv writes them.  Every thread of
+// the workgroup calls it (one barrier inside); s_o holds G x 4 x 64 doubles.
+constexpr int kSumBatch = 4;            // source blocks per thread and round trip
+__device__ void tile_block_out(int b, int Nb, int Nm, const int* __restrict__ np_of,
+                               const double* __restrict__ P, const BoxConst& bc, const double* __restrict__ x,
+                               double* __restrict__ y, int mode, const double* __restrict__ r0,
+                               const double* __restrict__ K, double* __restrict__ part,
+                               double (*s_o)[4][kTileEdge]) {
+    const int G = blockDim.x >> 6, g = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const double* src = P + (size_t)b * Nb * (4 * kTileEdge) + l;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q0 = g; q0 < Nb; q0 += G * kSumBatch) {
+        double v[kSumBatch][4];
+#pragma unroll
+        for (int t = 0; t < kSumBatch; ++t) {
+            const int q = q0 + G * t;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                v[t][c] = q < Nb ? src[(size_t)q * (4 * kTileEdge) + c * kTileEdge] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kSumBatch; ++t)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] += v[t][c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_o[g][c][l] = acc[c];
+    __syncthreads();
+    const int i = b * kTileEdge + threadIdx.x;
+    if (threadIdx.x >= kTileEdge || i >= Nm) return;
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int h = 0; h < G; ++h)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] += s_o[h][c][threadIdx.x];
+    RelayG32 m{NULL, 0, NULL, NULL, bc};
+    spmv_group_out(m, i, Nm, np_of, x, y, mode, r0, K, part, o);
+}
+
+// y = A x on the TILES layout (relay_tile.h), first of two launches.  One
+// workgroup per tile (I, J): the lanes are the rows i of block I, and at
+// diagonal k lane l meets column j = J*64 + ((l + k) & 63), so at every k the
+// wavefront covers 64 distinct pairs.  Each pair is applied to row i (from
+// x_j) and to row j (from x_i, the block transposed: r -> -r flips the
+// charge-dipole terms).  The row sums stay in the lane; the column sums go
+// to the wavefront's own column array in LDS through FP64 LDS atomic adds
+// that return nothing (the lane does not wait for them): at one k the 64
+// lanes add to 64 distinct columns, and one wavefront's LDS operations
+// complete in program order, so every column's sum is formed in a fixed
+// order.  The kTileWaves wavefronts take consecutive runs of diagonals.
+// Their sums are added in a fixed order and written as partials
+// P[dst][src][c][l] (64 rows of block dst, component c, from block src):
+// P[I][J] the row sums, P[J][I] the column sums, P[I][I] the row sums of a
+// diagonal tile.  Workgroups past the tiles form the Lagrange rows.
+// Wavefronts per tile: 4 (16 diagonals each, at most 168 VGPRs) lets three
+// workgroups share a compute unit, so the 561 tiles of Au2057 all start at
+// once.  Measured at Au2057 (product alone / MD step): 4 waves 15.7 us /
+// 1.91 ms; 8 waves (at most 128 VGPRs) 16.1 us / 2.21 ms, their larger
+// workgroups finding fewer free slots beside the pair kernel.
+#ifndef CAPAC_TILE_WAVES
+#define CAPAC_TILE_WAVES 4
+#endif
+constexpr int kTileWaves = CAPAC_TILE_WAVES;            // wavefronts per tile
+constexpr int kTileThreads = kTileWaves * CAPAC_WAVE;
+constexpr int kTileSteps = kTileEdge / kTileWaves;      // diagonals per wavefront
+
+// the wavefront's diagonals k0 .. k0 + kTileSteps - 1, their coefficients cf
+// already loaded.  Column j's entries for the next diagonal are read before
+// this diagonal's column adds are issued: the adds then never stand between
+// a read and its use (LDS operations complete in order).
+template <bool Compact>
+__device__ __forceinline__ void tile_apply(const float4 (&cf)[kTileSteps], int k0, const BoxConst& bc,
+                                           const double (*s_x)[kTileEdge], const double (*s_p)[kTileEdge],
+                                           const double xi[4], const double pi[3], int lane, double orow[4],
+                                           double (*s_col)[kTileEdge]) {
+    int jl = (lane + k0) & (kTileEdge - 1);
+    double xj[4] = {s_x[0][jl], s_x[1][jl], s_x[2][jl], s_x[3][jl]};
+    double pj[3] = {s_p[0][jl], s_p[1][jl], s_p[2][jl]};
+#pragma unroll
+    for (int s = 0; s < kTileSteps; ++s) {
+        double dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
+        if (!Compact) {
+            // pbc_12 with both candidates formed and selected (same results;
+            // its branches inside this unrolled loop double the kernel's
+            // registers)
+            const double ax = dx + bc.box[0], bx = dx - bc.box[0];
+            const double ay = dy + bc.box[1], by = dy - bc.box[1];
+            const double az = dz + bc.box[2], bz = dz - bc.box[2];
+            dx = dx < -bc.half[0] ? ax : (dx > bc.half[0] ? bx : dx);
+            dy = dy < -bc.half[1] ? ay : (dy > bc.half[1] ? by : dy);
+            dz = dz < -bc.half[2] ? az : (dz > bc.half[2] ? bz : dz);
+            dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+        }
+        const double ga = cf[s].x, gb = cf[s].y, c3 = cf[s].z, cq = cf[s].w;
+        // row i from column j (block (i, j) of relay.h)
+        const double rpj = dx * xj[0] + dy * xj[1] + dz * xj[2];
+        const double gr = ga * rpj + c3 * xj[3];
+        orow[0] += gr * dx + gb * xj[0];
+        orow[1] += gr * dy + gb * xj[1];
+        orow[2] += gr * dz + gb * xj[2];
+        orow[3] += cq * xj[3] - c3 * rpj;
+        // row j from column i: block (j, i) has separation -r
+        const double rpi = dx * xi[0] + dy * xi[1] + dz * xi[2];
+        const double gc = ga * rpi - c3 * xi[3];
+        const double oc[4] = {gc * dx + gb * xi[0], gc * dy + gb * xi[1], gc * dz + gb * xi[2],
+                              cq * xi[3] + c3 * rpi};
+        const int jc = jl;
+        if (s + 1 < kTileSteps) {
+            jl = (jl + 1) & (kTileEdge - 1);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) xj[c] = s_x[c][jl];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pj[c] = s_p[c][jl];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            (void)__hip_atomic_fetch_add(&s_col[c][jc], oc[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+#ifndef CAPAC_TILE_MINW
+#define CAPAC_TILE_MINW 3
+#endif
+// CAPAC_TILE_FUSED=1 forms the row blocks' outputs inside the tile launch:
+// the workgroup that delivers a block's last partial sums it, after a
+// device-scope release fence, a counter add and an acquire fence.  It saves
+// the second launch but measured 68 us per product at Au2057 against
+// 16.3 us for the two launches (every workgroup's release fence writes back
+// its XCD's L2), so it is off; it stays buildable as that measurement.
+#ifndef CAPAC_TILE_FUSED
+#define CAPAC_TILE_FUSED 0
+#endif
+__global__ void __launch_bounds__(kTileThreads, CAPAC_TILE_MINW)
+k_spmv_tiles(const float4* __restrict__ tiles, int Nb, int n_tiles, int Nm, int n_np,
+             const int* __restrict__ np_of, const int* __restrict__ np_first, const int* __restrict__ np_last,
+             const double* __restrict__ r, const double* __restrict__ u, BoxConst bc,
+             const double* __restrict__ x, double* __restrict__ y, double* __restrict__ P, int* __restrict__ cnt,
+             int mode, const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
+             const SolveScalars* __restrict__ sc) {
+    main_stream_priority();
+    // component-major, so that the lanes' reads at consecutive rows hit
+    // consecutive banks
+    __shared__ double s_x[2][4][kTileEdge];
+    __shared__ double s_p[2][3][kTileEdge];
+    __shared__ double s_acc[2][kTileWaves][4][kTileEdge];    // [row/column][wave][component][lane]
+    if (sc && sc->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if ((int)blockIdx.x >= n_tiles) {
+        // Lagrange row of nanoparticle k: the sum of its metal charges
+        const int k = blockIdx.x - n_tiles;
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += kTileThreads) o[3] += x[3 * Nm + mi];
+        double v = wave_sum(o[3]);
+        if (lane == 0) s_acc[0][0][0][wave] = v;
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        o[3] = 0.0;
+        for (int w = 0; w < kTileWaves; ++w) o[3] += s_acc[0][0][0][w];
+        RelayG32 m{NULL, 0, r, u, bc};
+        spmv_group_out(m, Nm + k, Nm, np_of, x, y, mode, r0, K, part, o);
+        return;
+    }
+    int I, J;
+    tile_coords(blockIdx.x, Nb, I, J);
+    // the wavefront's coefficients first: their HBM stream then overlaps the
+    // staging of the vectors below
+    const int k0 = kTileSteps * wave;
+    float4 cf[kTileSteps];
+    {
+        const float4* cp = tiles + (size_t)blockIdx.x * kTileSlots + (size_t)k0 * kTileEdge + lane;
+#pragma unroll
+        for (int s = 0; s < kTileSteps; ++s) cf[s] = cp[s * kTileEdge];
+    }
+    // the compact flag and both position forms are loaded together, so the
+    // staging costs one memory round trip
+    const double flag = u[4 * Nm];
+    if (threadIdx.x < 2 * kTileEdge) {
+        // x and positions of blocks I (side 0) and J (side 1); zeros past Nm
+        const int side = threadIdx.x >> 6;
+        const int m = (side ? J : I) * kTileEdge + lane;
+        double xv[4] = {0.0, 0.0, 0.0, 0.0}, pv[3] = {0.0, 0.0, 0.0};
+        if (m < Nm) {
+            xv[0] = x[3 * m]; xv[1] = x[3 * m + 1]; xv[2] = x[3 * m + 2]; xv[3] = x[3 * Nm + m];
+            const double4 uv = reinterpret_cast<const double4*>(u)[m];
+            const double rx = r[m], ry = r[bc.N + m], rz = r[2 * bc.N + m];
+            const bool cmp = flag != 0.0;
+            pv[0] = cmp ? uv.x : rx; pv[1] = cmp ? uv.y : ry; pv[2] = cmp ? uv.z : rz;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s_x[side][c][lane] = xv[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s_p[side][c][lane] = pv[c];
+    }
+    __syncthreads();
+    const bool diag = I == J, compact = flag != 0.0;
+    const double xi[4] = {s_x[0][0][lane], s_x[0][1][lane], s_x[0][2][lane], s_x[0][3][lane]};
+    const double pi[3] = {s_p[0][0][lane], s_p[0][1][lane], s_p[0][2][lane]};
+    double orow[4] = {0.0, 0.0, 0.0, 0.0};
+    // this wavefront's column sums (its LDS operations run in order, so the
+    // zeros land before its first add)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_acc[1][wave][c][lane] = 0.0;
+    if (compact) tile_apply<true>(cf, k0, bc, s_x[1], s_p[1], xi, pi, lane, orow, s_acc[1][wave]);
+    else tile_apply<false>(cf, k0, bc, s_x[1], s_p[1], xi, pi, lane, orow, s_acc[1][wave]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_acc[0][wave][c][lane] = orow[c];
+    __syncthreads();
+    if (threadIdx.x < 4 * kTileEdge) {
+        const int c = threadIdx.x >> 6;            // component; lane = row in the block
+        double rs = 0.0, cs = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTileWaves; ++w) {
+            rs += s_acc[0][w][c][lane];
+            cs += s_acc[1][w][c][lane];
+        }
+        const size_t blk = 4 * kTileEdge;
+        if (diag) {
+            // every ordered pair of the block is a slot of its own here: the
+            // row sums are complete and the column sums (formed alongside, as
+            // in the other tiles) are not used
+            P[((size_t)I * Nb + I) * blk + c * kTileEdge + lane] = rs;
+        } else {
+            P[((size_t)I * Nb + J) * blk + c * kTileEdge + lane] = rs;
+            P[((size_t)J * Nb + I) * blk + c * kTileEdge + lane] = cs;
+        }
+    }
+#if CAPAC_TILE_FUSED
+    // The workgroup that delivers a row block's last partial forms that
+    // block's outputs (each block receives exactly Nb partials per launch):
+    // release fence, one counter add per block written, and the last one
+    // in acquires the others' partials.  The sums are taken in the same
+    // fixed order whichever workgroup takes them; the counter is cleared for
+    // the next launch.
+    __shared__ int s_last[2];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_last[0] = atomicAdd(&cnt[I], 1) == Nb - 1;
+        s_last[1] = !diag && atomicAdd(&cnt[J], 1) == Nb - 1;
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+        if (!s_last[side]) continue;
+        const int b = side ? J : I;
+        __threadfence();
+        // s_acc is free again: its first 8 KB hold the block sums
+        tile_block_out(b, Nb, Nm, np_of, P, bc, x, y, mode, r0, K, part,
+                       reinterpret_cast<double (*)[4][kTileEdge]>(&s_acc[0][0][0][0]));
+        if (threadIdx.x == 0) cnt[b] = 0;
+        __syncthreads();
+    }
+#endif
+}
+
+// Second launch (without CAPAC_TILE_FUSED): one workgroup per block of 64
+// metal atoms forms its rows' outputs from the partials.
+constexpr int kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads)
+k_spmv_tsum(int Nb, int Nm, const int* __restrict__ np_of, const double* __restrict__ P, BoxConst bc,
+            const double* __restrict__ x, double* __restrict__ y, int mode, const double* __restrict__ r0,
+            const double* __restrict__ K, double* __restrict__ part, const SolveScalars* __restrict__ sc) {
+    main_stream_priority();
+    __shared__ double s_o[kSumThreads / 64][4][kTileEdge];        // [group][component][row]
+    if (sc && sc->done) return;
+    tile_block_out(blockIdx.x, Nb, Nm, np_of, P, bc, x, y, mode, r0, K, part, s_o);
+}
+
 // start of BiCGSTAB (bicg_stab.c:294-404) in two launches.  k_bicg_init,
 // one thread per row: r0 = r = b - Ax, v = 0, K = 1/diag, the first
 // direction p = r + rho (0 - 0) = r and y = K p, and per-block partial sums
@@ -673,7 +945,16 @@ static void spmv_t(capac_ctx* c, const Solver* S, const M& m, const double* x, d
 #endif
 static void spmv(capac_ctx* c, const Solver* S, const double* x, double* y, int mode, double* part,
                  const SolveScalars* sc) {
-    if (c->precision == CAPAC_PREC_FP32_REFINED) {
+    if (c->precision == CAPAC_PREC_FP32_REFINED && c->relay_layout == CAPAC_LAYOUT_TILES) {
+        const int nb = c->relay_nb, nt = (int)tile_count(nb);
+        k_spmv_tiles<<<nt + S->n_np, kTileThreads, 0, c->stream>>>(
+            reinterpret_cast<const float4*>(c->relay32), nb, nt, S->Nm, S->n_np, c->np_of_metal, c->np_first,
+            c->np_last, c->r, c->metal_u, c->bc, x, y, c->relay_part, c->relay_cnt, mode, S->r0, S->K, part,
+            sc);
+        if (!CAPAC_TILE_FUSED)
+            k_spmv_tsum<<<nb, kSumThreads, 0, c->stream>>>(nb, S->Nm, c->np_of_metal, c->relay_part, c->bc, x,
+                                                           y, mode, S->r0, S->K, part, sc);
+    } else if (c->precision == CAPAC_PREC_FP32_REFINED) {
         RelayG32 m{c->relay32, c->relay_ld, c->r, c->metal_u, c->bc};
         const int nm = c->bc.Nm;
         if (nm >= CAPAC_G32_LARGE_MIN) spmv_t<RelayG32, CAPAC_G32_ROWS_LARGE>(c, S, m, x, y, mode, part, sc);

@@ -181,11 +181,12 @@ GPU_EXPORTS = [
     "capac_set_theta", "capac_get_theta", "capac_fit_frame",
     "capac_em_begin", "capac_em_step", "capac_em_direction",
     "capac_checkpoint_bytes", "capac_checkpoint_save", "capac_checkpoint_load",
-    "capac_set_predictor", "capac_predictor",
+    "capac_set_predictor", "capac_predictor", "capac_set_fp32_layout", "capac_fp32_layout",
     "capac_bonded", "capac_vsite_build", "capac_vsite_spread",
 ]
 NTHETA = 6
 
+LAYOUT_AUTO, LAYOUT_ROWS, LAYOUT_TILES = 0, 1, 2      # CAPAC_LAYOUT_*
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64, KERNEL_SOLVE = range(7)
 DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
 
@@ -238,6 +239,8 @@ def gpu_lib() -> C.CDLL:
         lib.capac_em_direction.argtypes = [C.c_void_p, C.c_int]
         lib.capac_set_predictor.argtypes = [C.c_void_p, C.c_int]
         lib.capac_predictor.argtypes = [C.c_void_p]
+        lib.capac_set_fp32_layout.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_fp32_layout.argtypes = [C.c_void_p]
         lib.capac_checkpoint_bytes.argtypes = [C.c_void_p]
         lib.capac_checkpoint_bytes.restype = C.c_size_t
         lib.capac_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
@@ -315,6 +318,15 @@ class Context:
     @property
     def predictor(self) -> int:
         return int(self.lib.capac_predictor(self.h))
+
+    def set_fp32_layout(self, layout: int):
+        """Storage layout of the FP32 relay: LAYOUT_AUTO, LAYOUT_ROWS or
+        LAYOUT_TILES (capac_set_fp32_layout)."""
+        _check(self.lib.capac_set_fp32_layout(self.h, int(layout)), "capac_set_fp32_layout")
+
+    @property
+    def fp32_layout(self) -> int:
+        return int(self.lib.capac_fp32_layout(self.h))
 
     def set_theta(self, theta):
         """Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC); see capac_set_theta."""

@@ -224,6 +224,24 @@ int capac_set_solve_tolerance(capac_ctx* ctx, double crit);
 int capac_set_predictor(capac_ctx* ctx, int order);
 int capac_predictor(const capac_ctx* ctx);
 
+/* Storage layout of the FP32 relay (CAPAC_PREC_FP32_REFINED only).  ROWS
+ * keeps the 4 coefficients of every ordered metal pair in row-major planes
+ * (16 B per ordered pair); TILES keeps each unordered pair once in 64 x 64
+ * tiles of metal blocks (8 B per ordered pair) and its product applies every
+ * stored pair to both of its rows, with fixed-order partial sums, so it
+ * moves half the bytes.  AUTO (the default at capac_create) takes TILES from
+ * CAPAC_TILES_MIN_METAL metal atoms on, where the tiles fill the chip.  Both
+ * hold the same FP32 coefficients; the products differ by FP64 summation
+ * order only.  Changing the layout reallocates the relay; the next force
+ * call assembles it.  capac_fp32_layout returns the layout in use (-1 on
+ * the FP64 path).  No counterpart in the reference. */
+#define CAPAC_LAYOUT_AUTO   0
+#define CAPAC_LAYOUT_ROWS   1
+#define CAPAC_LAYOUT_TILES  2
+#define CAPAC_TILES_MIN_METAL 1024
+int capac_set_fp32_layout(capac_ctx* ctx, int layout);
+int capac_fp32_layout(const capac_ctx* ctx);
+
 /* ---- restart checkpoint (no counterpart in the reference) ----
  * The reference restarts from the last traj.gro frame, which keeps
  * positions, velocities, vQ[0] and vP only (file.c:152-160, 215-229): the

@@ -99,12 +99,27 @@ def check_neutral(deck, x, bnorm):
         assert abs(q[first[k]: last[k] + 1].sum() - qtot[k]) < 1.1e-4 * bnorm
 
 
-@pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
+# (precision, FP32 relay layout): the default layout of these decks is ROWS
+# (fewer metal atoms than CAPAC_TILES_MIN_METAL), so TILES is forced as well
+PATHS = {"fp64": (lib.PREC_FP64, None), "fp32": (lib.PREC_FP32_REFINED, None),
+         "fp32_tiles": (lib.PREC_FP32_REFINED, lib.LAYOUT_TILES)}
+
+
+def context(deck, path):
+    prec, layout = PATHS[path]
+    ctx = lib.Context(deck, precision=prec)
+    if layout is not None:
+        ctx.set_fp32_layout(layout)
+    return ctx
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("index", [0, 1, 2])
-def test_force_call_config(index, prec, cfg):
+def test_force_call_config(index, path, cfg):
     deck = cfg(index)
     r = deck.r0
-    ctx = lib.Context(deck, precision=prec)
+    prec = PATHS[path][0]
+    ctx = context(deck, path)
     try:
         ctx.set_state(r=r, x=np.zeros(deck.n_mat))
         fg, pg, vg = ctx.force()
@@ -123,7 +138,7 @@ def test_force_call_config(index, prec, cfg):
         assert kind == 0 and kindo == 0
         res = b - A @ xg
         bb = float(np.dot(b, b))
-        print(f"cfg{index}", "prec", prec, "iters", it, ito, "rTr/bTb", np.dot(res, res) / bb, flush=True)
+        print(f"cfg{index}", path, "iters", it, ito, "rTr/bTb", np.dot(res, res) / bb, flush=True)
         assert np.dot(res, res) < 1.2e-8 * bb
         xe = np.linalg.solve(A, b)
         err_o = rel(xo, xe)
@@ -143,7 +158,7 @@ def test_force_call_config(index, prec, cfg):
         f, pot, vir = oracle_force_at(o, deck, r, xg)
     finally:
         o.close()
-    check_against(fg, pg, vg, f, pot, vir, f"cfg{index} prec {prec}")
+    check_against(fg, pg, vg, f, pot, vir, f"cfg{index} {path}")
 
 
 @pytest.fixture(scope="module")
@@ -190,14 +205,20 @@ def test_au2057_phases(au2057):
     assert rel(vg, vo) < 1e-10
 
 
-@pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
-def test_au2057_force_call(prec, au2057):
-    """The force call as MD runs it: pair kernel on its full grid beside the
-    solve, the FP32 product with several row-groups per workgroup; the solve
-    on the oracle's FP64 COO residual, everything else at 1e-10."""
+@pytest.mark.parametrize("path", ["fp64", "fp32", "fp32_rows"])
+def test_au2057_force_call(path, au2057):
+    """The force call as MD runs it: pair kernel beside the solve, the FP32
+    product in its default layout here (TILES) and in ROWS with several
+    row-groups per workgroup; the solve on the oracle's FP64 COO residual,
+    everything else at 1e-10."""
     deck, o, _, x32 = au2057
     r = deck.r0
+    prec = lib.PREC_FP64 if path == "fp64" else lib.PREC_FP32_REFINED
     ctx = lib.Context(deck, precision=prec)
+    if path == "fp32":
+        assert ctx.fp32_layout == lib.LAYOUT_TILES
+    if path == "fp32_rows":
+        ctx.set_fp32_layout(lib.LAYOUT_ROWS)
     try:
         ctx.set_state(r=r, x=np.zeros(deck.n_mat))
         fg, pg, vg = ctx.force()
@@ -211,14 +232,14 @@ def test_au2057_force_call(prec, au2057):
     res = b - o.relay_apply(xg)
     bb = float(np.dot(b, b))
     assert abs(btb - bb) <= 1e-9 * bb          # the GPU's own right-hand side (1e-10 above)
-    print("cfg3 prec", prec, "iters", it, "rTr/bTb", np.dot(res, res) / bb,
+    print("cfg3", path, "iters", it, "rTr/bTb", np.dot(res, res) / bb,
           "x vs the FP32 solution", rel(xg, x32), flush=True)
     assert np.dot(res, res) < 1.2e-8 * bb
     check_neutral(deck, xg, np.sqrt(bb))
     # two valid solves of the same system stop within the solver tolerance
     assert rel(xg, x32) < 1e-2
     f, pot, vir = oracle_force_at(o, deck, r, xg)
-    check_against(fg, pg, vg, f, pot, vir, f"cfg3 prec {prec}")
+    check_against(fg, pg, vg, f, pot, vir, f"cfg3 {path}")
 
 
 @pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])

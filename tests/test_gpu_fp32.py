@@ -1,10 +1,12 @@
 """FP32-refined CPIM path (CAPAC_PREC_FP32_REFINED) against the FP64 oracle.
 
-The relay is stored as 7 FP32 coefficients per metal pair, with its
-T-tensors evaluated in FP32.  BiCGSTAB runs on it with FP64 vectors and
-dots.  FP64 iterative refinement then applies the reference's stop test to
+The relay is stored as 4 FP32 coefficients per metal pair, with its
+T-tensors evaluated in FP32, in either layout: ROWS (every ordered pair) or
+TILES (each unordered pair once, applied to both of its rows); every test
+below runs on both.  BiCGSTAB runs on it with FP64 vectors and dots.  FP64 iterative refinement then applies the reference's stop test to
 the true residual, with the FP64 relay recomputed on the fly.  Bars:
   - relay entries: max |A32 - A64| / max |A64| < 2e-6 (FP32 evaluation);
+    the two layouts expand to bitwise the same matrix (same coefficients);
   - solve: the reference criterion rTr < 1e-8 bTb holds for the FP64
     residual b - A64 x (A64 from the oracle, dense); the distance to the
     exact FP64 solution is at most 10x the oracle's own; neutrality holds
@@ -48,12 +50,50 @@ def decks(golden_dir):
     return out
 
 
-@pytest.fixture(scope="module")
-def ctxs(decks):
-    out = {name: lib.Context(d, precision=lib.PREC_FP32_REFINED) for name, d in decks.items()}
+LAYOUTS = {"rows": lib.LAYOUT_ROWS, "tiles": lib.LAYOUT_TILES}
+
+
+def fp32_context(deck, layout):
+    c = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
+    c.set_fp32_layout(LAYOUTS[layout])
+    assert c.fp32_layout == LAYOUTS[layout]
+    return c
+
+
+@pytest.fixture(scope="module", params=list(LAYOUTS))
+def ctxs(decks, request):
+    out = {name: fp32_context(d, request.param) for name, d in decks.items()}
     yield out
     for c in out.values():
         c.close()
+
+
+@pytest.mark.parametrize("name", METAL_DECKS)
+def test_layouts_same_relay(name, decks):
+    """Both layouts hold the same FP32 coefficients: their dense expansions
+    are bitwise equal.  Their products differ by FP64 summation order only,
+    so two solves taken far below the reference's tolerance (rTr < 1e-18
+    bTb) agree to round-off; at the reference's own tolerance two valid
+    solves may differ by up to about 1e-5."""
+    deck = decks[name]
+    cr, ct = fp32_context(deck, "rows"), fp32_context(deck, "tiles")
+    try:
+        ar, _ = cr.cpim_assemble(dense=True)
+        at, _ = ct.cpim_assemble(dense=True)
+        np.testing.assert_array_equal(ar, at)
+        o = Oracle(deck)
+        b = o.vec_ext(deck.r0)
+        x0 = np.zeros(deck.n_mat)
+        for c in (cr, ct):
+            c.set_solve_tolerance(1e-18)
+        xr, itr, _, _, kr = cr.cpim_solve(b=b, x=x0)
+        xt, itt, _, _, kt = ct.cpim_solve(b=b, x=x0)
+        print(name, "iterations rows", itr, "tiles", itt, "x rel", rel(xt, xr))
+        assert kr == kt
+        assert rel(xt, xr) < 1e-8
+    finally:
+        cr.close()
+        ct.close()
 
 
 def check_refined_solution(deck, o, b, xg, kindg, xo, kindo):
@@ -160,14 +200,15 @@ def test_force_call_fp32_refined(name, decks, ctxs):
 MD_TOL = {"au13_w": (1e-7, 1e-5), "2au13_field": (1e-7, 1e-5)}
 
 
+@pytest.mark.parametrize("layout", list(LAYOUTS))
 @pytest.mark.parametrize("name", ["au13_w", "2au13_field"])
-def test_md_three_steps_fp32_refined(name, decks):
+def test_md_three_steps_fp32_refined(name, layout, decks):
     deck = decks[name]
     tol_r, tol_v = MD_TOL[name]
     o = Oracle(deck)
     o.force()
     o.md_begin()
-    ctx = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
+    ctx = fp32_context(deck, layout)
     try:
         ctx.force()
         ctx.md_begin()
