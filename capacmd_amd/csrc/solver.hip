@@ -380,12 +380,13 @@ k_spmv(M m, int Nm, int n_np, const int* __restrict__ np_of, const int* __restri
 
 // Row block b's outputs from its Nb partials P[b][q] (relay_tile.h):
 // thread (g, l) sums the four components of row l over the source blocks
-// q = g, g + G, ... (G = blockDim / 64 groups), all of its loads issued
-// before the first add, so the partials arrive in about one memory round
-// trip; the G groups are added in order, then row l's diagonal block,
+// q = g, g + G, ... (G = blockDim / 64 groups), kSumBatch blocks' loads
+// issued before their first add, so the partials arrive in one memory round
+// trip per G kSumBatch source blocks; the G groups are added in order, then
+// row l's diagonal block,
 // Lagrange column and outputs follow as k_spmv writes them.  Every thread of
 // the workgroup calls it (one barrier inside); s_o holds G x 4 x 64 doubles.
-constexpr int kSumBatch = 4;            // source blocks per thread and round trip
+constexpr int kSumBatch = 8;            // source blocks per thread and round trip
 __device__ void tile_block_out(int b, int Nb, int Nm, const int* __restrict__ np_of,
                                const double* __restrict__ P, const BoxConst& bc, const double* __restrict__ x,
                                double* __restrict__ y, int mode, const double* __restrict__ r0,
@@ -638,7 +639,11 @@ k_spmv_tiles(const float4* __restrict__ tiles, int Nb, int n_tiles, int Nm, int 
 
 // Second launch (without CAPAC_TILE_FUSED): one workgroup per block of 64
 // metal atoms forms its rows' outputs from the partials.
-constexpr int kSumThreads = 1024;
+// 256 threads (four groups of source blocks): in MD this launch runs beside
+// the pair kernel, and a larger workgroup can find no compute unit with room
+// for it until the pair kernel drains (a 1024-thread version waited 775 us
+// behind it at Au2057)
+constexpr int kSumThreads = 256;
 __global__ void __launch_bounds__(kSumThreads)
 k_spmv_tsum(int Nb, int Nm, const int* __restrict__ np_of, const double* __restrict__ P, BoxConst bc,
             const double* __restrict__ x, double* __restrict__ y, int mode, const double* __restrict__ r0,
