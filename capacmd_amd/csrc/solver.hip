@@ -29,16 +29,18 @@
 // device; a `done` flag turns the remaining launches of a batch into no-ops
 // and the host synchronises once per batch.
 //
-// Two relay formats share these kernels (template parameter):
+// Relay formats:
 //   RelayM64  FP64, no stored matrix: each block is formed from the
 //             positions inside the product (the reference's entries)
 //   RelayG32  four FP32 coefficients per ordered pair plus the separation
 //             formed from the FP64 positions; products accumulate in FP64
 //             with FP64 vectors; diagonal blocks applied analytically
-// The FP32-refined solve runs BiCGSTAB on RelayG32, then checks the true
+//   tiles     the same coefficients, each unordered pair once (relay_tile.h)
+// The FP64 path runs the reference's BiCGSTAB on RelayM64.  The FP32-refined
+// path runs GMRES on its FP32 relay (section below), then checks the true
 // residual b - A x with the FP64 relay recomputed on the fly and, while the
-// reference's criterion fails on it, solves for the correction on RelayG32
-// and adds it (iterative refinement).
+// reference's criterion fails on it, solves for the correction on the FP32
+// relay and adds it (iterative refinement).
 #include "capac_internal.h"
 #include "relay.h"
 #include "relay_tile.h"
@@ -48,6 +50,21 @@
 
 #include <mutex>
 
+// GMRES restart length: the columns kept before the solve restarts from its
+// updated x (Au2057 with the quadratic warm start needs ~20-30)
+constexpr int kGmM = 40;
+
+// GMRES state on the device (kernels below)
+struct GmState {
+    double nrm[kGmM + 1];           // |q_i|; the basis vector v_i = q_i / nrm[i]
+    double H[kGmM][kGmM + 1];       // column j: h_0j .. h_jj as formed, rotated in place into R
+    double cs[kGmM], sn[kGmM];      // Givens rotations
+    double g[kGmM + 1];             // rotated right-hand side |r0| e_0
+    double y[kGmM];                 // the update's coefficients once a solve or restart is due
+    int ny;                         // columns in y (0: no update pending)
+    int pad;
+};
+
 struct Solver {
     int n_mat, Nm, n_np, n_groups, n_vb;
     double *r0, *r, *p, *v, *s, *t, *y, *z, *K, *Ax;
@@ -56,6 +73,11 @@ struct Solver {
     double* part2;          // n_vb: (r0, s)
     double* part4;          // n_vb: (r, r)
     double *d, *rr, *part_res;   // refinement: correction, FP64 residual, its partials
+    // GMRES (the FP32-refined path's inner solver): the basis, the per-block
+    // partials of its dot products, the Hessenberg column state
+    double* Q;              // (kGmM + 1) n_mat: unnormalised basis vectors q_i
+    double* part_h;         // n_vb (kGmM + 1)
+    GmState* gm;            // device
     // inside the context's readback buffer, so that the synchronisation at
     // the end of a batch also brings back the neighbour-row flags
     double* res;            // device: [0] (rr, rr)
@@ -833,6 +855,263 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
     }
 }
 
+// ------------------------------------------------------------------ GMRES
+// The FP32-refined path's inner solver (the reference's BiCGSTAB stays the
+// FP64 path's): GMRES(kGmM) right-preconditioned with the same Jacobi K, so
+// it minimises |b - A (x0 + K V y)| over the Krylov space of A K.  The relay
+// is symmetric but indefinite, and its signed Jacobi preconditioner is what
+// makes it converge fast; GMRES keeps that preconditioner and needs one
+// product per column where BiCGSTAB needs two per iteration (Au561 and Au2057
+// decks from the quadratic warm start, dense oracle relay: 14 against 23 and
+// 19 against 29 products to rTr < 1e-8 bTb).  Column j is one unit:
+//   SpMV   w = A u, u = K q_j (q_j unnormalised)
+//   dots   |q_j| from the partials that formed q_j; w /= |q_j|; partials of
+//          (w, v_i), i <= j (a second grid dimension over groups of basis
+//          vectors); one thread also rotates column j - 1 and takes the stop
+//          test (so one product past the stop is spent, as BiCGSTAB's
+//          deferred test does)
+//   update v_j = q_j / |q_j| in place; h_i from the partials; q_{j+1} = w -
+//          sum h_i v_i (classical Gram-Schmidt), u = K q_{j+1}, partials of
+//          |q_{j+1}|^2
+// x is updated once, by k_gm_finish, from y = R^-1 g.  Every block reduces
+// the partials itself in the same fixed order, as the vector kernels do.
+// The stop test is the reference's rTr < crit bTb on GMRES's residual
+// estimate |g_{j+1}|^2; the 1000-iteration rule counts columns.
+
+// start: r0 = b - Ax (Ax formed by the product before), q_0 = r0, K, u = K r0,
+// partials of (b, b) and (r0, r0)
+__global__ void __launch_bounds__(kVecThreads)
+k_gm_init(int n_mat, int n4, const double* __restrict__ b, const double* __restrict__ Ax,
+          const double* __restrict__ diag, double* __restrict__ q0, double* __restrict__ u,
+          double* __restrict__ K, double* __restrict__ part_bb, double* __restrict__ part_qq) {
+    main_stream_priority();
+    __shared__ double s_w[16 * 2];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0.0, 0.0};
+    if (k < n_mat) {
+        const double bk = b[k];
+        const double rk = bk - Ax[k];
+        const double Kk = k < n4 ? 1.0 / diag[k] : 1.0;
+        q0[k] = rk;
+        K[k] = Kk;
+        u[k] = Kk * rk;
+        acc[0] = bk * bk;
+        acc[1] = rk * rk;
+    }
+    block_sum_n<2>(acc, s_w);
+    if (threadIdx.x == 0) {
+        part_bb[blockIdx.x] = acc[0];
+        part_qq[blockIdx.x] = acc[1];
+    }
+}
+
+// one workgroup: bTb, criteria, |r0|; the early exits of a first start (x
+// unchanged): bTb < 1e-10, and (r0, r0) < 1e-10 (BiCGSTAB's first rho); a
+// restart whose residual already meets the test ends converged
+__global__ void __launch_bounds__(256)
+k_gm_start(int n_vb, const double* __restrict__ part_bb, const double* __restrict__ part_qq,
+           SolveScalars* sc, GmState* gm, double crit_factor, int first) {
+    main_stream_priority();
+    __shared__ double s_w[16 * 2];
+    double red[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < n_vb; k += blockDim.x) {
+        red[0] += part_bb[k];
+        red[1] += part_qq[k];
+    }
+    block_sum_n<2>(red, s_w);
+    if (threadIdx.x != 0) return;
+    const double bTb = red[0], rr = red[1];
+    sc->bTb = bTb;
+    sc->criteria = bTb * crit_factor;
+    sc->rTr = rr;
+    if (first) sc->iter = 0;
+    gm->nrm[0] = sqrt(rr);
+    gm->g[0] = sqrt(rr);
+    gm->ny = 0;
+    if (first && bTb < 1.0e-10) {
+        sc->done = 1; sc->kind = CAPAC_SOLVE_ZERO_RHS;
+    } else if (first && rr < 1.0e-10) {
+        sc->done = 1; sc->kind = CAPAC_SOLVE_ZERO_RHO;
+    } else if (!first && rr < sc->criteria) {
+        sc->done = 1; sc->kind = CAPAC_SOLVE_CONVERGED;
+    } else {
+        sc->done = 0; sc->kind = CAPAC_SOLVE_CONVERGED;
+    }
+}
+
+// kind of a solve that stopped to restart from its updated x (host-side only)
+constexpr int kSolveRestart = 5;
+
+// one thread: rotate column jj (its h_{jj+1,jj} = hnext), the residual
+// estimate and the stop test; when the solve stops or the basis is full, the
+// update's coefficients y = R^-1 g
+__device__ void gm_column(GmState* gm, SolveScalars* sc, int jj, double hnext) {
+    double* col = gm->H[jj];
+    for (int i = 0; i < jj; ++i) {
+        const double t = gm->cs[i] * col[i] + gm->sn[i] * col[i + 1];
+        col[i + 1] = -gm->sn[i] * col[i] + gm->cs[i] * col[i + 1];
+        col[i] = t;
+    }
+    const double a = col[jj];
+    const double rad = sqrt(a * a + hnext * hnext);
+    const double c = rad > 0.0 ? a / rad : 1.0, sn = rad > 0.0 ? hnext / rad : 0.0;
+    gm->cs[jj] = c;
+    gm->sn[jj] = sn;
+    col[jj] = rad;
+    gm->g[jj + 1] = -sn * gm->g[jj];
+    gm->g[jj] = c * gm->g[jj];
+    const double res2 = gm->g[jj + 1] * gm->g[jj + 1];
+    sc->rTr = res2;
+    sc->iter += 1;
+    const bool converged = res2 < sc->criteria;
+    const bool over = sc->iter > 1000;
+    if (!(converged || over || jj + 1 == kGmM)) return;
+    for (int i = jj; i >= 0; --i) {
+        double t = gm->g[i];
+        for (int l = i + 1; l <= jj; ++l) t -= gm->H[l][i] * gm->y[l];
+        gm->y[i] = gm->H[i][i] != 0.0 ? t / gm->H[i][i] : 0.0;
+    }
+    gm->ny = jj + 1;
+    sc->done = 1;
+    if (converged) sc->kind = CAPAC_SOLVE_CONVERGED;
+    else if (over) sc->kind = res2 < 100.0 * sc->criteria ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
+    else sc->kind = kSolveRestart;
+}
+
+// column j's dot products (see the block comment above), grid (row blocks,
+// groups of kGmDots basis vectors): q_0 .. q_{j-1} are stored normalised (the
+// update launch normalises q_j when it has |q_j|), q_j is scaled here; j ==
+// kGmM only rotates the last column
+constexpr int kGmDots = 8;
+__global__ void __launch_bounds__(kVecThreads)
+k_gm_dots(int n_mat, int j, double* __restrict__ w, const double* __restrict__ Q,
+          const double* __restrict__ part_qq, int n_vb, GmState* gm, SolveScalars* sc,
+          double* __restrict__ part_h) {
+    main_stream_priority();
+    __shared__ double s_w[16 * kGmDots];
+    __shared__ int s_skip;
+    if (threadIdx.x == 0) s_skip = sc->done;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = blockIdx.y * kGmDots;
+    const bool row = k < n_mat && j < kGmM;
+    // every load of the launch before the first wait: the row, its basis
+    // entries and the partials of |q_j|^2
+    const double wraw = row ? w[k] : 0.0;
+    double qv[kGmDots];
+#pragma unroll
+    for (int t = 0; t < kGmDots; ++t) qv[t] = row && i0 + t <= j ? Q[(size_t)(i0 + t) * n_mat + k] : 0.0;
+    double pq = 0.0;
+    for (int b = threadIdx.x; b < n_vb; b += blockDim.x) pq += part_qq[b];
+    __syncthreads();
+    if (s_skip) return;
+    const double nq = block_sum(pq, s_w);
+    const double nrm_j = sqrt(nq);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        gm->nrm[j] = nrm_j;
+        if (j >= 1) gm_column(gm, sc, j - 1, nrm_j);
+    }
+    if (j >= kGmM) return;
+    const double inv = 1.0 / nrm_j;
+    const double wk = wraw * inv;
+    if (row && blockIdx.y == 0) w[k] = wk;
+    double d[kGmDots];
+#pragma unroll
+    for (int t = 0; t < kGmDots; ++t) d[t] = wk * (i0 + t == j ? qv[t] * inv : qv[t]);
+    __syncthreads();                      // s_w is reused
+    block_sum_n<kGmDots>(d, s_w);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int t = 0; t < kGmDots; ++t)
+            if (i0 + t <= j) part_h[(size_t)blockIdx.x * (kGmM + 1) + i0 + t] = d[t];
+}
+
+// column j's Gram-Schmidt step: q_j normalised in place, h from the partials,
+// q_{j+1} = w - sum h_i v_i, u = K q_{j+1}, partials of |q_{j+1}|^2; block 0
+// keeps the column's h
+__global__ void __launch_bounds__(kVecThreads)
+k_gm_update(int n_mat, int j, const double* __restrict__ w, double* __restrict__ Q,
+            const double* __restrict__ K, const double* __restrict__ part_h, int n_vb, GmState* gm,
+            const SolveScalars* __restrict__ sc, double* __restrict__ u, double* __restrict__ part_qq) {
+    main_stream_priority();
+    __shared__ double s_w[16];
+    __shared__ double s_h[kGmM + 1];
+    __shared__ double s_g[kVecThreads / (kGmM + 1)][kGmM + 1];
+    __shared__ int s_skip;
+    if (threadIdx.x == 0) s_skip = sc->done;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double wk = 0.0, Kk = 0.0, qj = 0.0;
+    if (k < n_mat) { wk = w[k]; Kk = K[k]; qj = Q[(size_t)j * n_mat + k]; }
+    __syncthreads();
+    if (s_skip) return;
+    // h_i = sum over the blocks' partials: thread (g, i) adds blocks g, g + G,
+    // ... (G = 6 groups, loads in flight together), then the groups in order
+    {
+        constexpr int G = kVecThreads / (kGmM + 1);
+        const int i = threadIdx.x % (kGmM + 1), g = threadIdx.x / (kGmM + 1);
+        double t = 0.0;
+        if (g < G && i <= j)
+            for (int b0 = g; b0 < n_vb; b0 += 8 * G) {
+                double pv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    pv[u] = b0 + u * G < n_vb ? part_h[(size_t)(b0 + u * G) * (kGmM + 1) + i] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t += pv[u];
+            }
+        if (g < G) s_g[g][i] = t;
+        __syncthreads();
+        if (threadIdx.x <= j) {
+            double h = 0.0;
+            for (int gg = 0; gg < G; ++gg) h += s_g[gg][threadIdx.x];
+            s_h[threadIdx.x] = h;
+        }
+        __syncthreads();
+    }
+    double acc = 0.0;
+    if (k < n_mat) {
+        const double vj = qj * (1.0 / gm->nrm[j]);
+        Q[(size_t)j * n_mat + k] = vj;
+        double q = wk;
+        // v_0 .. v_{j-1} in order, eight loads in flight at a time
+        for (int i0 = 0; i0 < j; i0 += 8) {
+            double vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = i0 + u < j ? Q[(size_t)(i0 + u) * n_mat + k] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (i0 + u < j) q -= s_h[i0 + u] * vv[u];
+        }
+        q -= s_h[j] * vj;
+        Q[(size_t)(j + 1) * n_mat + k] = q;
+        u[k] = Kk * q;
+        acc = q * q;
+    }
+    const double qq = block_sum(acc, s_w);
+    if (threadIdx.x == 0) part_qq[blockIdx.x] = qq;
+    if (blockIdx.x == 0 && threadIdx.x <= j) gm->H[j][threadIdx.x] = s_h[threadIdx.x];
+}
+
+// x += K sum_i y_i v_i once the solve (or a restart) is due; no-op otherwise
+__global__ void __launch_bounds__(kVecThreads)
+k_gm_finish(int n_mat, const double* __restrict__ Q, const double* __restrict__ K, const GmState* gm,
+            const SolveScalars* __restrict__ sc, double* __restrict__ x) {
+    main_stream_priority();
+    if (!sc->done) return;
+    const int ny = gm->ny;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ny == 0 || k >= n_mat) return;
+    double t = 0.0;
+    for (int i0 = 0; i0 < ny; i0 += 8) {
+        double vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vv[u] = i0 + u < ny ? Q[(size_t)(i0 + u) * n_mat + k] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u < ny) t += gm->y[i0 + u] * vv[u];
+    }
+    x[k] += K[k] * t;
+}
+
 // (rr, rr) of the FP64 residual from its per-group partials (with a guard:
 // only after a solve that finished converged or loosened)
 __global__ void __launch_bounds__(256)
@@ -903,6 +1182,13 @@ int solver_create(capac_ctx* c) {
     CHECK_HIP(hipMalloc(&S->part_res, ng * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part2, nvb * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part4, nvb * sizeof(double)));
+    if (c->precision == CAPAC_PREC_FP32_REFINED) {
+        CHECK_HIP(hipMalloc(&S->Q, (size_t)(kGmM + 1) * n * sizeof(double)));
+        CHECK_HIP(hipMemset(S->Q, 0, (size_t)(kGmM + 1) * n * sizeof(double)));
+        CHECK_HIP(hipMalloc(&S->part_h, nvb * (kGmM + 1) * sizeof(double)));
+        CHECK_HIP(hipMalloc(&S->gm, sizeof(GmState)));
+        CHECK_HIP(hipMemset(S->gm, 0, sizeof(GmState)));
+    }
     S->sc = (SolveScalars*)(c->red + kRedSc);
     S->sc_host = (SolveScalars*)(c->red_host + kRedSc);
     S->res = c->red + kRedRes;
@@ -914,8 +1200,8 @@ int solver_create(capac_ctx* c) {
 void solver_destroy(capac_ctx* c) {
     Solver* S = c->solver;
     if (!S) return;
-    double* vecs[] = {S->r0, S->r, S->p, S->v, S->s, S->t, S->y, S->z, S->K, S->Ax, S->d, S->rr,
-                      S->part1, S->part3, S->part_res, S->part2, S->part4};
+    void* vecs[] = {S->r0, S->r, S->p, S->v, S->s, S->t, S->y, S->z, S->K, S->Ax, S->d, S->rr,
+                    S->part1, S->part3, S->part_res, S->part2, S->part4, S->Q, S->part_h, S->gm};
     for (size_t k = 0; k < sizeof vecs / sizeof vecs[0]; ++k)
         if (vecs[k]) (void)hipFree(vecs[k]);
     free(S);
@@ -1042,6 +1328,73 @@ static int bicgstab(capac_ctx* c, Solver* S, const double* b, double* x, int fir
     return 0;
 }
 
+// GMRES on the FP32 relay for A x = b (kernels above), x the warm start and
+// the result; batches of columns with the done flag, one synchronisation per
+// batch, and the FP64 residual of the finished solve in the same transfer
+// (check64), as bicgstab does.  A full basis restarts from the updated x.
+static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_batch, bool check64,
+                 SolveOut* out) {
+    const int n = S->n_mat, n4 = 4 * S->Nm;
+    const int vb = S->n_vb;
+    hipStream_t st = c->stream;
+    bool first = true;
+    int launched = 0;
+    for (;;) {
+        spmv(c, S, x, S->Ax, 0, NULL, NULL);
+        k_gm_init<<<vb, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->Q, S->p, S->K, S->part2, S->part4);
+        k_gm_start<<<1, 256, 0, st>>>(vb, S->part2, S->part4, S->sc, S->gm, c->solve_crit, first ? 1 : 0);
+        first = false;
+        // later batches start small (see bicgstab)
+        int j = 0, batch = first_batch > 1 ? first_batch : 2, next = 2;
+        for (;;) {
+            for (int u = 0; u < batch && j <= kGmM; ++u, ++j) {
+                if (j < kGmM) spmv(c, S, S->p, S->v, 0, NULL, S->sc);
+                const dim3 grid(j < kGmM ? vb : 1, j < kGmM ? (j + kGmDots) / kGmDots : 1);
+                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, vb, S->gm, S->sc, S->part_h);
+                if (j < kGmM)
+                    k_gm_update<<<vb, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->K, S->part_h, vb, S->gm, S->sc,
+                                                            S->p, S->part4);
+            }
+            launched += batch;
+            k_gm_finish<<<vb, kVecThreads, 0, st>>>(n, S->Q, S->K, S->gm, S->sc, x);
+            if (check64) {
+                int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
+                if (rc) return rc;
+                k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
+            }
+            CHECK_HIP(hipGetLastError());
+            if (S->in_red) {
+                CHECK_HIP(hipMemcpyAsync(c->red_host + kRedFlags, c->red + kRedFlags,
+                                         (kRedWords - kRedFlags) * sizeof(double), hipMemcpyDeviceToHost, st));
+            } else {
+                CHECK_HIP(hipMemcpyAsync(S->sc_host, S->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st));
+                CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+            CHECK_HIP(hipStreamSynchronize(st));
+            if (S->sc_host->done) break;
+            if (j > kGmM || launched > 1100) {
+                capac_set_error("GMRES did not report completion after %d columns", launched);
+                return CAPAC_ERR_SOLVER;
+            }
+            batch = next;
+            next = next < 16 ? 2 * next : 16;
+        }
+        if (S->sc_host->kind != kSolveRestart) break;
+    }
+    out->iters = S->sc_host->iter;
+    out->kind = S->sc_host->kind;
+    out->rTr = S->sc_host->rTr;
+    out->bTb = S->sc_host->bTb;
+    return 0;
+}
+
+// the inner solver: the reference's BiCGSTAB on the FP64 relay, GMRES on the FP32 one
+static int inner_solve(capac_ctx* c, Solver* S, const double* b, double* x, int first_batch, bool check64,
+                       SolveOut* out) {
+    if (c->precision == CAPAC_PREC_FP32_REFINED) return gmres(c, S, b, x, first_batch, check64, out);
+    return bicgstab(c, S, b, x, first_batch, check64, out);
+}
+
 // FP64 residual rr = b - A64 x with the relay recomputed on the fly; returns (rr, rr)
 static int residual64(capac_ctx* c, Solver* S, const double* b, const double* x, double* rTr) {
     int rc = cpim_apply64(c, x, b, S->rr, S->part_res, NULL);
@@ -1080,7 +1433,7 @@ static int solver_run_with(capac_ctx* c, Solver* S, const double* b, double* x) 
         first = (c->batch_main2 > 1 ? (c->batch_main + c->batch_main2) / 2 : c->batch_main) + 1;
     const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
     SolveOut o;
-    int rc = bicgstab(c, S, b, x, first, refine, &o);
+    int rc = inner_solve(c, S, b, x, first, refine, &o);
     if (rc) return rc;
     c->batch_main2 = c->batch_main;
     c->batch_main = o.iters;
@@ -1102,7 +1455,7 @@ static int solver_run_with(capac_ctx* c, Solver* S, const double* b, double* x) 
             k_scale<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 1.0 / norm, S->rr);
             SolveOut od;
             const int first_corr = c->batch_corr > 0 ? c->batch_corr + 1 : 4;
-            if ((rc = bicgstab(c, S, S->rr, S->d, first_corr, false, &od))) return rc;
+            if ((rc = inner_solve(c, S, S->rr, S->d, first_corr, false, &od))) return rc;
             c->batch_corr = od.iters;
             total += od.iters;
             k_axpy<<<(n + 255) / 256, 256, 0, c->stream>>>(n, norm, S->d, x);
@@ -1135,8 +1488,11 @@ constexpr int kNoAllocMaxUnknowns = 16384;
 constexpr size_t kNoAllocGroups = kNoAllocMaxUnknowns + 1;       // metal row-groups + Lagrange rows, bound
 constexpr size_t kNoAllocVecBlocks = (kNoAllocMaxUnknowns + kVecThreads - 1) / kVecThreads + 1;
 constexpr size_t kNoAllocScWords = 16;                             // SolveScalars, padded
+constexpr size_t kNoAllocGmWords = (sizeof(GmState) + 255) / 256 * 32;   // GmState, padded to 256 B
 constexpr size_t kNoAllocWords = 12 * (size_t)kNoAllocMaxUnknowns + 5 * kNoAllocGroups +
-                                 2 * kNoAllocVecBlocks + kNoAllocScWords + 4;
+                                 2 * kNoAllocVecBlocks + kNoAllocScWords + 4 +
+                                 (size_t)(kGmM + 1) * kNoAllocMaxUnknowns + kNoAllocVecBlocks * (kGmM + 1) +
+                                 kNoAllocGmWords;
 static_assert(sizeof(SolveScalars) <= kNoAllocScWords * sizeof(double), "solver scalars overflow their slot");
 constexpr int kMaxDevices = 64;
 
@@ -1189,6 +1545,9 @@ int solver_run_noalloc(capac_ctx* c, const double* b, double* x) {
     V.part4 = p; p += kNoAllocVecBlocks;
     V.sc = (SolveScalars*)p; p += kNoAllocScWords;
     V.res = p; p += 4;
+    V.Q = p; p += (size_t)(kGmM + 1) * kNoAllocMaxUnknowns;
+    V.part_h = p; p += kNoAllocVecBlocks * (kGmM + 1);
+    V.gm = (GmState*)p; p += kNoAllocGmWords;
     V.sc_host = own->sc_host;
     V.res_host = own->res_host;
     V.in_red = false;

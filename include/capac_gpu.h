@@ -52,10 +52,13 @@ extern "C" {
 #define CAPAC_PREC_FP64          0  /* FP64 matrix, FP64 BiCGSTAB (the reference's arithmetic) */
 /* FP32 relay (T-tensors evaluated in FP32, 4 coefficients per metal pair,
  * the separation formed from the positions where a block is applied),
- * BiCGSTAB on it with FP64 vectors and dots, and FP64 iterative refinement:
- * the reference's stop test is applied to the true residual b - A x with the
- * FP64 relay recomputed on the fly, and corrections are solved for until it
- * holds.  No counterpart in the reference (its arithmetic is FP64 only). */
+ * GMRES on it (restarted every 40 columns, right-preconditioned with the
+ * reference's Jacobi preconditioner) with FP64 vectors and dots, and FP64
+ * iterative refinement: the reference's stop test is applied to the true
+ * residual b - A x with the FP64 relay recomputed on the fly, and
+ * corrections are solved for until it holds.  Iteration counts of this path
+ * are GMRES columns (one product each; a BiCGSTAB iteration takes two).  No
+ * counterpart in the reference (its arithmetic is FP64 only). */
 #define CAPAC_PREC_FP32_REFINED  1
 
 /* Verlet skin of the neighbour rows (nm), see capac_set_skin */

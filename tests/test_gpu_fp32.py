@@ -3,7 +3,8 @@
 The relay is stored as 4 FP32 coefficients per metal pair, with its
 T-tensors evaluated in FP32, in either layout: ROWS (every ordered pair) or
 TILES (each unordered pair once, applied to both of its rows); every test
-below runs on both.  BiCGSTAB runs on it with FP64 vectors and dots.  FP64 iterative refinement then applies the reference's stop test to
+below runs on both.  GMRES (right-preconditioned with the reference's Jacobi
+preconditioner) runs on it with FP64 vectors and dots.  FP64 iterative refinement then applies the reference's stop test to
 the true residual, with the FP64 relay recomputed on the fly.  Bars:
   - relay entries: max |A32 - A64| / max |A64| < 2e-6 (FP32 evaluation);
     the two layouts expand to bitwise the same matrix (same coefficients);
