@@ -865,14 +865,14 @@ k_vec_xp(int n_mat, int n, int n_groups, const double* __restrict__ part3, int n
 // decks from the quadratic warm start, dense oracle relay: 14 against 23 and
 // 19 against 29 products to rTr < 1e-8 bTb).  Column j is one unit:
 //   SpMV   w = A u, u = K q_j (q_j unnormalised)
-//   dots   |q_j| from the partials that formed q_j; w /= |q_j|; partials of
-//          (w, v_i), i <= j (a second grid dimension over groups of basis
-//          vectors); one thread also rotates column j - 1 and takes the stop
+//   dots   partials of (w, q_i), i <= j (a second grid dimension over groups
+//          of basis vectors); one workgroup also forms |q_j| from the
+//          partials that formed q_j, rotates column j - 1 and takes the stop
 //          test (so one product past the stop is spent, as BiCGSTAB's
 //          deferred test does)
-//   update v_j = q_j / |q_j| in place; h_i from the partials; q_{j+1} = w -
-//          sum h_i v_i (classical Gram-Schmidt), u = K q_{j+1}, partials of
-//          |q_{j+1}|^2
+//   update w /= |q_j| and v_j = q_j / |q_j| in place; h_i from the partials;
+//          q_{j+1} = w - sum h_i v_i (classical Gram-Schmidt), u = K q_{j+1},
+//          partials of |q_{j+1}|^2
 // x is updated once, by k_gm_finish, from y = R^-1 g.  Every block reduces
 // the partials itself in the same fixed order, as the vector kernels do.
 // The stop test is the reference's rTr < crit bTb on GMRES's residual
@@ -979,12 +979,14 @@ __device__ void gm_column(GmState* gm, SolveScalars* sc, int jj, double hnext) {
 }
 
 // column j's dot products (see the block comment above), grid (row blocks,
-// groups of kGmDots basis vectors): q_0 .. q_{j-1} are stored normalised (the
-// update launch normalises q_j when it has |q_j|), q_j is scaled here; j ==
-// kGmM only rotates the last column
+// groups of kGmDots basis vectors): partials of (w, q_i) for the product as it
+// came, with q_0 .. q_{j-1} stored normalised (the update launch normalises
+// q_j when it has |q_j|); the update launch applies the 1 / |q_j| scales, so
+// no workgroup here waits for |q_j| except workgroup (0, 0), which forms it,
+// keeps it and rotates column j - 1.  j == kGmM only rotates the last column.
 constexpr int kGmDots = 8;
 __global__ void __launch_bounds__(kVecThreads)
-k_gm_dots(int n_mat, int j, double* __restrict__ w, const double* __restrict__ Q,
+k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restrict__ Q,
           const double* __restrict__ part_qq, int n_vb, GmState* gm, SolveScalars* sc,
           double* __restrict__ part_h) {
     main_stream_priority();
@@ -994,30 +996,29 @@ k_gm_dots(int n_mat, int j, double* __restrict__ w, const double* __restrict__ Q
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int i0 = blockIdx.y * kGmDots;
     const bool row = k < n_mat && j < kGmM;
-    // every load of the launch before the first wait: the row, its basis
-    // entries and the partials of |q_j|^2
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
+    // every load of the launch before the first wait
     const double wraw = row ? w[k] : 0.0;
     double qv[kGmDots];
 #pragma unroll
     for (int t = 0; t < kGmDots; ++t) qv[t] = row && i0 + t <= j ? Q[(size_t)(i0 + t) * n_mat + k] : 0.0;
     double pq = 0.0;
-    for (int b = threadIdx.x; b < n_vb; b += blockDim.x) pq += part_qq[b];
+    if (lead)
+        for (int b = threadIdx.x; b < n_vb; b += blockDim.x) pq += part_qq[b];
     __syncthreads();
     if (s_skip) return;
-    const double nq = block_sum(pq, s_w);
-    const double nrm_j = sqrt(nq);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        gm->nrm[j] = nrm_j;
-        if (j >= 1) gm_column(gm, sc, j - 1, nrm_j);
+    if (lead) {
+        const double nrm_j = sqrt(block_sum(pq, s_w));
+        if (threadIdx.x == 0) {
+            gm->nrm[j] = nrm_j;
+            if (j >= 1) gm_column(gm, sc, j - 1, nrm_j);
+        }
+        __syncthreads();                  // s_w is reused
     }
     if (j >= kGmM) return;
-    const double inv = 1.0 / nrm_j;
-    const double wk = wraw * inv;
-    if (row && blockIdx.y == 0) w[k] = wk;
     double d[kGmDots];
 #pragma unroll
-    for (int t = 0; t < kGmDots; ++t) d[t] = wk * (i0 + t == j ? qv[t] * inv : qv[t]);
-    __syncthreads();                      // s_w is reused
+    for (int t = 0; t < kGmDots; ++t) d[t] = wraw * qv[t];
     block_sum_n<kGmDots>(d, s_w);
     if (threadIdx.x == 0)
 #pragma unroll
@@ -1025,11 +1026,13 @@ k_gm_dots(int n_mat, int j, double* __restrict__ w, const double* __restrict__ Q
             if (i0 + t <= j) part_h[(size_t)blockIdx.x * (kGmM + 1) + i0 + t] = d[t];
 }
 
-// column j's Gram-Schmidt step: q_j normalised in place, h from the partials,
+// column j's Gram-Schmidt step: w = w / |q_j| and v_j = q_j / |q_j| in place,
+// h_i from the partials (scaled by 1 / |q_j|, and 1 / |q_j|^2 for i = j),
 // q_{j+1} = w - sum h_i v_i, u = K q_{j+1}, partials of |q_{j+1}|^2; block 0
-// keeps the column's h
+// keeps the column's h.  The basis entries of the row are loaded before the
+// reduction of h, so the launch waits for memory once.
 __global__ void __launch_bounds__(kVecThreads)
-k_gm_update(int n_mat, int j, const double* __restrict__ w, double* __restrict__ Q,
+k_gm_update(int n_mat, int j, double* __restrict__ w, double* __restrict__ Q,
             const double* __restrict__ K, const double* __restrict__ part_h, int n_vb, GmState* gm,
             const SolveScalars* __restrict__ sc, double* __restrict__ u, double* __restrict__ part_qq) {
     main_stream_priority();
@@ -1040,47 +1043,45 @@ k_gm_update(int n_mat, int j, const double* __restrict__ w, double* __restrict__
     if (threadIdx.x == 0) s_skip = sc->done;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double wk = 0.0, Kk = 0.0, qj = 0.0;
+    double qv[kGmM];
     if (k < n_mat) { wk = w[k]; Kk = K[k]; qj = Q[(size_t)j * n_mat + k]; }
-    __syncthreads();
-    if (s_skip) return;
+#pragma unroll
+    for (int i = 0; i < kGmM; ++i) qv[i] = k < n_mat && i < j ? Q[(size_t)i * n_mat + k] : 0.0;
     // h_i = sum over the blocks' partials: thread (g, i) adds blocks g, g + G,
     // ... (G = 6 groups, loads in flight together), then the groups in order
-    {
-        constexpr int G = kVecThreads / (kGmM + 1);
-        const int i = threadIdx.x % (kGmM + 1), g = threadIdx.x / (kGmM + 1);
-        double t = 0.0;
-        if (g < G && i <= j)
-            for (int b0 = g; b0 < n_vb; b0 += 8 * G) {
-                double pv[8];
+    constexpr int G = kVecThreads / (kGmM + 1);
+    const int ih = threadIdx.x % (kGmM + 1), g = threadIdx.x / (kGmM + 1);
+    double t = 0.0;
+    if (g < G && ih <= j)
+        for (int b0 = g; b0 < n_vb; b0 += 8 * G) {
+            double pv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    pv[u] = b0 + u * G < n_vb ? part_h[(size_t)(b0 + u * G) * (kGmM + 1) + i] : 0.0;
+            for (int q = 0; q < 8; ++q)
+                pv[q] = b0 + q * G < n_vb ? part_h[(size_t)(b0 + q * G) * (kGmM + 1) + ih] : 0.0;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) t += pv[u];
-            }
-        if (g < G) s_g[g][i] = t;
-        __syncthreads();
-        if (threadIdx.x <= j) {
-            double h = 0.0;
-            for (int gg = 0; gg < G; ++gg) h += s_g[gg][threadIdx.x];
-            s_h[threadIdx.x] = h;
+            for (int q = 0; q < 8; ++q) t += pv[q];
         }
-        __syncthreads();
+    __syncthreads();
+    if (s_skip) return;
+    if (g < G) s_g[g][ih] = t;
+    __syncthreads();
+    const double inv = 1.0 / gm->nrm[j];
+    if (threadIdx.x <= j) {
+        double h = 0.0;
+        for (int gg = 0; gg < G; ++gg) h += s_g[gg][threadIdx.x];
+        s_h[threadIdx.x] = (int)threadIdx.x == j ? h * inv * inv : h * inv;
     }
+    __syncthreads();
     double acc = 0.0;
     if (k < n_mat) {
-        const double vj = qj * (1.0 / gm->nrm[j]);
+        const double vj = qj * inv;
         Q[(size_t)j * n_mat + k] = vj;
-        double q = wk;
-        // v_0 .. v_{j-1} in order, eight loads in flight at a time
-        for (int i0 = 0; i0 < j; i0 += 8) {
-            double vv[8];
+        const double wn = wk * inv;
+        w[k] = wn;
+        double q = wn;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) vv[u] = i0 + u < j ? Q[(size_t)(i0 + u) * n_mat + k] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (i0 + u < j) q -= s_h[i0 + u] * vv[u];
-        }
+        for (int i = 0; i < kGmM; ++i)
+            if (i < j) q -= s_h[i] * qv[i];
         q -= s_h[j] * vj;
         Q[(size_t)(j + 1) * n_mat + k] = q;
         u[k] = Kk * q;
