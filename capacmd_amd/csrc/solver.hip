@@ -942,40 +942,72 @@ k_gm_start(int n_vb, const double* __restrict__ part_bb, const double* __restric
 // kind of a solve that stopped to restart from its updated x (host-side only)
 constexpr int kSolveRestart = 5;
 
-// one thread: rotate column jj (its h_{jj+1,jj} = hnext), the residual
-// estimate and the stop test; when the solve stops or the basis is full, the
-// update's coefficients y = R^-1 g
-__device__ void gm_column(GmState* gm, SolveScalars* sc, int jj, double hnext) {
-    double* col = gm->H[jj];
-    for (int i = 0; i < jj; ++i) {
-        const double t = gm->cs[i] * col[i] + gm->sn[i] * col[i + 1];
-        col[i + 1] = -gm->sn[i] * col[i] + gm->cs[i] * col[i + 1];
-        col[i] = t;
+// one workgroup (every thread calls it): rotate column jj (its h_{jj+1,jj} =
+// hnext), the residual estimate and the stop test; when the solve stops or
+// the basis is full, the update's coefficients y = R^-1 g.  The column, the
+// rotations and R are staged in LDS, so the serial work of thread 0 touches
+// no global memory.
+struct GmLds {
+    double col[kGmM + 1], cs[kGmM], sn[kGmM], g[kGmM + 1], y[kGmM];
+    double R[kGmM][kGmM];       // column l of R in R[l][0..l]
+    int stop;
+};
+__device__ void gm_column(GmState* gm, SolveScalars* sc, int jj, double hnext, GmLds& L) {
+    const int t = threadIdx.x;
+    if (t <= jj) L.col[t] = gm->H[jj][t];
+    if (t < jj) { L.cs[t] = gm->cs[t]; L.sn[t] = gm->sn[t]; }
+    if (t == 0) L.g[jj] = gm->g[jj];
+    __syncthreads();
+    if (t == 0) {
+        double* col = L.col;
+        for (int i = 0; i < jj; ++i) {
+            const double u = L.cs[i] * col[i] + L.sn[i] * col[i + 1];
+            col[i + 1] = -L.sn[i] * col[i] + L.cs[i] * col[i + 1];
+            col[i] = u;
+        }
+        const double a = col[jj];
+        const double rad = sqrt(a * a + hnext * hnext);
+        const double c = rad > 0.0 ? a / rad : 1.0, sn = rad > 0.0 ? hnext / rad : 0.0;
+        col[jj] = rad;
+        const double gj = L.g[jj];
+        const double g1 = -sn * gj;
+        gm->cs[jj] = c;
+        gm->sn[jj] = sn;
+        gm->g[jj + 1] = g1;
+        gm->g[jj] = c * gj;
+        const double res2 = g1 * g1;
+        sc->rTr = res2;
+        sc->iter += 1;
+        const bool converged = res2 < sc->criteria;
+        const bool over = sc->iter > 1000;
+        L.stop = converged || over || jj + 1 == kGmM;
+        if (L.stop) {
+            sc->done = 1;
+            if (converged) sc->kind = CAPAC_SOLVE_CONVERGED;
+            else if (over) sc->kind = res2 < 100.0 * sc->criteria ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
+            else sc->kind = kSolveRestart;
+        }
     }
-    const double a = col[jj];
-    const double rad = sqrt(a * a + hnext * hnext);
-    const double c = rad > 0.0 ? a / rad : 1.0, sn = rad > 0.0 ? hnext / rad : 0.0;
-    gm->cs[jj] = c;
-    gm->sn[jj] = sn;
-    col[jj] = rad;
-    gm->g[jj + 1] = -sn * gm->g[jj];
-    gm->g[jj] = c * gm->g[jj];
-    const double res2 = gm->g[jj + 1] * gm->g[jj + 1];
-    sc->rTr = res2;
-    sc->iter += 1;
-    const bool converged = res2 < sc->criteria;
-    const bool over = sc->iter > 1000;
-    if (!(converged || over || jj + 1 == kGmM)) return;
-    for (int i = jj; i >= 0; --i) {
-        double t = gm->g[i];
-        for (int l = i + 1; l <= jj; ++l) t -= gm->H[l][i] * gm->y[l];
-        gm->y[i] = gm->H[i][i] != 0.0 ? t / gm->H[i][i] : 0.0;
+    __syncthreads();
+    if (t <= jj) gm->H[jj][t] = L.col[t];
+    if (!L.stop) return;
+    // y = R^-1 g over columns 0 .. jj, R and g staged first
+    for (int e = t; e < (jj + 1) * (jj + 1); e += blockDim.x) {
+        const int l = e / (jj + 1), i = e % (jj + 1);
+        L.R[l][i] = l == jj ? L.col[i] : (i <= l ? gm->H[l][i] : 0.0);
     }
-    gm->ny = jj + 1;
-    sc->done = 1;
-    if (converged) sc->kind = CAPAC_SOLVE_CONVERGED;
-    else if (over) sc->kind = res2 < 100.0 * sc->criteria ? CAPAC_SOLVE_LOOSENED : CAPAC_SOLVE_FAILED;
-    else sc->kind = kSolveRestart;
+    if (t <= jj) L.g[t] = gm->g[t];     // thread 0's writes above are visible after the barrier
+    __syncthreads();
+    if (t == 0) {
+        for (int i = jj; i >= 0; --i) {
+            double u = L.g[i];
+            for (int l = i + 1; l <= jj; ++l) u -= L.R[l][i] * L.y[l];
+            L.y[i] = L.R[i][i] != 0.0 ? u / L.R[i][i] : 0.0;
+        }
+        gm->ny = jj + 1;
+    }
+    __syncthreads();
+    if (t <= jj) gm->y[t] = L.y[t];
 }
 
 // column j's dot products (see the block comment above), grid (row blocks,
@@ -991,6 +1023,7 @@ k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restri
           double* __restrict__ part_h) {
     main_stream_priority();
     __shared__ double s_w[16 * kGmDots];
+    __shared__ GmLds s_gm;
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = sc->done;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1009,10 +1042,8 @@ k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restri
     if (s_skip) return;
     if (lead) {
         const double nrm_j = sqrt(block_sum(pq, s_w));
-        if (threadIdx.x == 0) {
-            gm->nrm[j] = nrm_j;
-            if (j >= 1) gm_column(gm, sc, j - 1, nrm_j);
-        }
+        if (threadIdx.x == 0) gm->nrm[j] = nrm_j;
+        if (j >= 1) gm_column(gm, sc, j - 1, nrm_j, s_gm);
         __syncthreads();                  // s_w is reused
     }
     if (j >= kGmM) return;
