@@ -134,12 +134,15 @@ def equilibrate(deck, steps: int):
 
 
 FP32_LAYOUT = lib.LAYOUT_AUTO        # --fp32-layout
+BLOCK_PRECOND = None                  # --block-precond (None: the library's default)
 
 
 def start_context(deck, precision: int, eq, predictor=None):
     ctx = lib.Context(deck, precision=precision)
     if precision == lib.PREC_FP32_REFINED and FP32_LAYOUT != lib.LAYOUT_AUTO:
         ctx.set_fp32_layout(FP32_LAYOUT)
+    if precision == lib.PREC_FP32_REFINED and BLOCK_PRECOND is not None:
+        ctx.set_block_precond(BLOCK_PRECOND)
     ctx.set_state(r=eq["r"], v=eq["v"], x=eq["x"])
     ctx.set_thermostat(eq["vq"])
     if predictor is not None:
@@ -191,6 +194,7 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, k
         "nb_skin_nm": ctx.skin,
         "warm_start_predictor": ctx.predictor,
         "fp32_layout": {1: "rows", 2: "tiles"}.get(ctx.fp32_layout),
+        "block_precond_every": ctx.block_precond if ctx.block_precond >= 0 else None,
         "temp_start_of_timed_window": t_start,
         "temp_end_of_timed_window": t_end,
     }
@@ -320,6 +324,10 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
     ap.add_argument("--fp32-layout", choices=["auto", "rows", "tiles"], default="auto",
                     help="storage layout of the FP32 relay (capac_set_fp32_layout)")
+    ap.add_argument("--block-precond", type=int, default=None,
+                    help="FP32 path's GMRES preconditioner: block-Jacobi inverses re-formed every N "
+                         "assemblies, 0 for the Jacobi diagonal (capac_set_block_precond; default: the "
+                         "library's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     a = ap.parse_args()
@@ -327,7 +335,8 @@ def main() -> int:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the whole box lives on one MI355X)", file=sys.stderr)
         return 2
-    global FP32_LAYOUT
+    global FP32_LAYOUT, BLOCK_PRECOND
+    BLOCK_PRECOND = a.block_precond
     FP32_LAYOUT = {"auto": lib.LAYOUT_AUTO, "rows": lib.LAYOUT_ROWS, "tiles": lib.LAYOUT_TILES}[a.fp32_layout]
     deck_spec = gen.config(a.config)
     tmp = tempfile.mkdtemp(prefix="capac_bench_")

@@ -160,6 +160,10 @@ static int upload(T** dst, const T* src, size_t n) {
     return 0;
 }
 
+// (re)allocates the block-Jacobi inverses (none when bj_every is 0); the
+// next assembly forms them
+static int bj_alloc(capac_ctx* c) { return bj_alloc_dev(c); }
+
 // (re)allocates the FP32 relay in `layout` (AUTO: tiles from
 // CAPAC_TILES_MIN_METAL metal atoms on), zeroed: ROWS padded with zero
 // coefficient rows up to a multiple of kRelay32RowPad (the SpMV reads whole
@@ -171,6 +175,7 @@ static int relay32_alloc(capac_ctx* c, int layout) {
     if (c->relay32) { (void)hipFree(c->relay32); c->relay32 = NULL; }
     if (c->relay_part) { (void)hipFree(c->relay_part); c->relay_part = NULL; }
     if (c->relay_cnt) { (void)hipFree(c->relay_cnt); c->relay_cnt = NULL; }
+    if (int rc = bj_alloc(c)) return rc;
     c->relay_layout = layout;
     c->relay_nb = tile_blocks(Nm);
     if (layout == CAPAC_LAYOUT_TILES) {
@@ -537,6 +542,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         // the FP64 path stores no relay (its product forms the blocks)
         c->relay_ld = ((Nm + 63) / 64) * 64;
         if (precision == CAPAC_PREC_FP32_REFINED) {
+            c->bj_every = CAPAC_BJ_EVERY_DEFAULT;
             TRY(relay32_alloc(c, CAPAC_LAYOUT_AUTO));
             TRY(upload(&c->metal_u, (const double*)NULL, 4 * ((size_t)Nm + 1)));
         }
@@ -566,6 +572,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->vsite_par, c->tail_list};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
+    bj_free(c);
     if (c->red_host) (void)hipHostFree(c->red_host);
     solver_destroy(c);
     fit_destroy(c);
@@ -690,6 +697,46 @@ extern "C" int capac_set_fp32_layout(capac_ctx* c, int layout) {
 
 extern "C" int capac_fp32_layout(const capac_ctx* c) {
     return c && c->precision == CAPAC_PREC_FP32_REFINED && c->n_mat ? c->relay_layout : -1;
+}
+
+extern "C" int capac_set_block_precond(capac_ctx* c, int every) {
+    if (!c) return CAPAC_ERR_ARG;
+    if (c->precision != CAPAC_PREC_FP32_REFINED || !c->n_mat) {
+        capac_set_error("the block preconditioner applies to the FP32-refined path with metal atoms only");
+        return CAPAC_ERR_ARG;
+    }
+    if (every < 0) {
+        capac_set_error("block preconditioner interval must be >= 0 (0: Jacobi), got %d", every);
+        return CAPAC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    CHECK_HIP(hipStreamSynchronize(c->stream));
+    c->bj_every = every;
+    return bj_alloc(c);
+}
+
+extern "C" int capac_block_precond(const capac_ctx* c) {
+    return c && c->precision == CAPAC_PREC_FP32_REFINED && c->n_mat ? c->bj_every : -1;
+}
+
+extern "C" int capac_block_precond_inverse(capac_ctx* c, int block, float* out) {
+    if (!c || !out) return CAPAC_ERR_ARG;
+    if (c->precision != CAPAC_PREC_FP32_REFINED || !c->n_mat) {
+        capac_set_error("the block preconditioner applies to the FP32-refined path with metal atoms only");
+        return CAPAC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    return bj_copy_block(c, block, out);
+}
+
+extern "C" int capac_block_precond_coarse(capac_ctx* c, int* n, double* ainv, float* basis) {
+    if (!c || !n) return CAPAC_ERR_ARG;
+    if (c->precision != CAPAC_PREC_FP32_REFINED || !c->n_mat) {
+        capac_set_error("the block preconditioner applies to the FP32-refined path with metal atoms only");
+        return CAPAC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    return bj_copy_coarse(c, n, ainv, basis);
 }
 
 extern "C" int capac_set_predictor(capac_ctx* c, int order) {

@@ -171,6 +171,18 @@ struct capac_ctx {
                                 // (x, y, z, 0), then a flag at [4 Nm]: 1 when the metal is compact
     int relay_ld;
     size_t relay_bytes;
+    // FP32 path: block-Jacobi preconditioner of GMRES (precond.hip), the
+    // inverses of the relay's diagonal blocks of 64 metal atoms, re-formed
+    // every bj_every assemblies (0: the Jacobi K instead)
+    float* bj_inv;
+    int bj_every, bj_age, bj_valid;
+    // its coarse space (cg_n = 4 per block of 64 metal atoms, 0 when off):
+    // basis entries per metal atom (float4), active flags, Z^T q of the
+    // running application, Ac^-1 (cg_n x cg_n)
+    float* cg_w;
+    int* cg_act;
+    double *cg_s, *cg_A;
+    int cg_n;
     Solver* solver;
     int solve_iters, solve_kind, solve_refine_rounds;
     int batch_main, batch_main2, batch_corr;  // iterations of the last two main solves and the last
@@ -276,6 +288,19 @@ int cpim_dense_expand(capac_ctx* c, double* dense_dev);
 // finished with kind converged or loosened.
 int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part,
                  const SolveScalars* guard);
+// precond.hip
+size_t bj_bytes(int Nm);                    // the inverses' size
+int bj_alloc_dev(capac_ctx* c);             // (re)allocates for bj_every (none when 0)
+void bj_free(capac_ctx* c);
+int bj_build(capac_ctx* c);                 // inverts the diagonal blocks of the FP32 relay
+int bj_copy_block(capac_ctx* c, int b, float* out);   // block b's inverse, row-major, host
+int bj_copy_coarse(capac_ctx* c, int* n, double* ainv, float* basis);   // coarse space, host
+// The preconditioner M^-1 = B^-1 + Z Ac^-1 Z^T in two launches: bj_apply
+// sets out = B^-1 q (or adds it: add) and keeps Z^T q in cg_s; cg_apply adds
+// Z Ac^-1 cg_s to out (nothing without a coarse space).  guard 1: no-op
+// once sc reports done, 2: no-op until then.
+void bj_apply(capac_ctx* c, const double* q, double* out, bool add, const SolveScalars* sc, int guard);
+void cg_apply(capac_ctx* c, double* out, const SolveScalars* sc, int guard);
 // solver.hip
 int solver_create(capac_ctx* c);
 void solver_destroy(capac_ctx* c);

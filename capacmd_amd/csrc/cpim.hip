@@ -624,6 +624,16 @@ int cpim_assemble(capac_ctx* c) {
         c->diag_set = 1;
         c->diag_polar = c->bc.inv_polar;
         c->diag_capac = c->bc.inv_capac;
+        c->bj_valid = 0;                // the blocks' diagonal changed with them
+    }
+    // block-Jacobi inverses: every bj_every assemblies (the blocks change
+    // little from one MD step to the next, and the preconditioner only steers
+    // the iterations: the stop test is the same)
+    if (c->precision == CAPAC_PREC_FP32_REFINED && c->bj_inv && c->bj_every > 0 &&
+        (!c->bj_valid || ++c->bj_age >= c->bj_every)) {
+        if (int rc = bj_build(c)) return rc;
+        c->bj_valid = 1;
+        c->bj_age = 0;
     }
     CHECK_HIP(hipGetLastError());
     return 0;

@@ -43,11 +43,13 @@
 // relay and adds it (iterative refinement).
 #include "capac_internal.h"
 #include "relay.h"
+#include "precond.h"
 #include "relay_tile.h"
 
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 
 // GMRES restart length: the columns kept before the solve restarts from its
@@ -78,6 +80,7 @@ struct Solver {
     double* Q;              // (kGmM + 1) n_mat: unnormalised basis vectors q_i
     double* part_h;         // n_vb (kGmM + 1)
     GmState* gm;            // device
+    const float* bj;        // block-Jacobi inverses (precond.hip), or NULL: the Jacobi K
     // inside the context's readback buffer, so that the synchronisation at
     // the end of a batch also brings back the neighbour-row flags
     double* res;            // device: [0] (rr, rr)
@@ -525,6 +528,18 @@ __device__ __forceinline__ void tile_apply(const float4 (&cf)[kTileSteps], int k
     }
 }
 
+// The coarse correction the tiled product adds to its input (GMRES with the
+// block preconditioner): x's charges take w(i) . c_b, c = Ac^-1 s, s = Z^T q
+// as the update left it; the corrected vector goes to x2 for the sums launch.
+// nc = 0: none.  With one, the sums launch forms the Lagrange rows from x2.
+struct CoarseArgs {
+    const double* A;        // Ac^-1, nc x nc
+    const double* s;        // Z^T q, nc
+    const float4* w;        // basis entries per metal atom
+    double* x2;             // the corrected input (n_mat)
+    int nc;
+};
+
 #ifndef CAPAC_TILE_MINW
 #define CAPAC_TILE_MINW 3
 #endif
@@ -543,17 +558,19 @@ k_spmv_tiles(const float4* __restrict__ tiles, int Nb, int n_tiles, int Nm, int 
              const double* __restrict__ r, const double* __restrict__ u, BoxConst bc,
              const double* __restrict__ x, double* __restrict__ y, double* __restrict__ P, int* __restrict__ cnt,
              int mode, const double* __restrict__ r0, const double* __restrict__ K, double* __restrict__ part,
-             const SolveScalars* __restrict__ sc) {
+             const SolveScalars* __restrict__ sc, CoarseArgs cg) {
     main_stream_priority();
     // component-major, so that the lanes' reads at consecutive rows hit
     // consecutive banks
     __shared__ double s_x[2][4][kTileEdge];
     __shared__ double s_p[2][3][kTileEdge];
     __shared__ double s_acc[2][kTileWaves][4][kTileEdge];    // [row/column][wave][component][lane]
+    __shared__ double s_cc[2][4];
     if (sc && sc->done) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if ((int)blockIdx.x >= n_tiles) {
-        // Lagrange row of nanoparticle k: the sum of its metal charges
+        // Lagrange row of nanoparticle k: the sum of its metal charges (with
+        // a coarse correction the sums launch forms these rows instead)
         const int k = blockIdx.x - n_tiles;
         double o[4] = {0.0, 0.0, 0.0, 0.0};
         for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += kTileThreads) o[3] += x[3 * Nm + mi];
@@ -578,6 +595,37 @@ k_spmv_tiles(const float4* __restrict__ tiles, int Nb, int n_tiles, int Nm, int 
 #pragma unroll
         for (int s = 0; s < kTileSteps; ++s) cf[s] = cp[s * kTileEdge];
     }
+    // with a coarse correction, waves 2 and 3 form the coarse values of
+    // blocks I and J (rows 4I.. and 4J.. of Ac^-1 times Z^T q) while waves 0
+    // and 1 stage the vectors; the staged charges then take w(i) . c
+    float4 zw = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cg.nc) {
+        if (threadIdx.x >= 2 * kTileEdge) {
+            const int side = wave - 2, blk = side ? J : I;
+            double cc[4] = {0.0, 0.0, 0.0, 0.0};
+            // every load of the row segments issued before the first add
+            double av[kCgMax / kTileEdge][4], sv[kCgMax / kTileEdge];
+#pragma unroll
+            for (int it = 0; it < kCgMax / kTileEdge; ++it) {
+                const int m = lane + kTileEdge * it;
+                sv[it] = m < cg.nc ? cg.s[m] : 0.0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) av[it][a] = m < cg.nc ? cg.A[(size_t)(4 * blk + a) * cg.nc + m] : 0.0;
+            }
+#pragma unroll
+            for (int it = 0; it < kCgMax / kTileEdge; ++it)
+#pragma unroll
+                for (int a = 0; a < 4; ++a) cc[a] += av[it][a] * sv[it];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) cc[a] = wave_sum(cc[a]);
+            if (lane == 0)
+#pragma unroll
+                for (int a = 0; a < 4; ++a) s_cc[side][a] = cc[a];
+        } else {
+            const int m = ((threadIdx.x >> 6) ? J : I) * kTileEdge + lane;
+            if (m < Nm) zw = cg.w[m];
+        }
+    }
     // the compact flag and both position forms are loaded together, so the
     // staging costs one memory round trip
     const double flag = u[4 * Nm];
@@ -600,6 +648,25 @@ k_spmv_tiles(const float4* __restrict__ tiles, int Nb, int n_tiles, int Nm, int 
     }
     __syncthreads();
     const bool diag = I == J, compact = flag != 0.0;
+    if (cg.nc) {
+        if (threadIdx.x < 2 * kTileEdge) {
+            const int side = threadIdx.x >> 6;
+            const double* cc = s_cc[side];
+            s_x[side][3][lane] += ((zw.x * cc[0] + zw.y * cc[1]) + zw.z * cc[2]) + zw.w * cc[3];
+            // the diagonal tile keeps its block's corrected vector for the sums
+            // launch; tile 0 also copies the Lagrange unknowns there
+            const int m = I * kTileEdge + lane;
+            if (diag && side == 0 && m < Nm) {
+                cg.x2[3 * m] = s_x[0][0][lane];
+                cg.x2[3 * m + 1] = s_x[0][1][lane];
+                cg.x2[3 * m + 2] = s_x[0][2][lane];
+                cg.x2[3 * Nm + m] = s_x[0][3][lane];
+            }
+            if (blockIdx.x == 0 && side == 1)
+                for (int k = lane; k < n_np; k += kTileEdge) cg.x2[4 * Nm + k] = x[4 * Nm + k];
+        }
+        __syncthreads();
+    }
     const double xi[4] = {s_x[0][0][lane], s_x[0][1][lane], s_x[0][2][lane], s_x[0][3][lane]};
     const double pi[3] = {s_p[0][0][lane], s_p[0][1][lane], s_p[0][2][lane]};
     double orow[4] = {0.0, 0.0, 0.0, 0.0};
@@ -669,10 +736,26 @@ constexpr int kSumThreads = 256;
 __global__ void __launch_bounds__(kSumThreads)
 k_spmv_tsum(int Nb, int Nm, const int* __restrict__ np_of, const double* __restrict__ P, BoxConst bc,
             const double* __restrict__ x, double* __restrict__ y, int mode, const double* __restrict__ r0,
-            const double* __restrict__ K, double* __restrict__ part, const SolveScalars* __restrict__ sc) {
+            const double* __restrict__ K, double* __restrict__ part, const SolveScalars* __restrict__ sc,
+            const int* __restrict__ np_first, const int* __restrict__ np_last) {
     main_stream_priority();
     __shared__ double s_o[kSumThreads / 64][4][kTileEdge];        // [group][component][row]
     if (sc && sc->done) return;
+    if ((int)blockIdx.x >= Nb) {
+        // Lagrange row of nanoparticle k (when the tile launch leaves them here)
+        const int k = blockIdx.x - Nb, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += kSumThreads) o[3] += x[3 * Nm + mi];
+        const double v = wave_sum(o[3]);
+        if (lane == 0) s_o[0][0][wave] = v;
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        o[3] = 0.0;
+        for (int w = 0; w < kSumThreads / 64; ++w) o[3] += s_o[0][0][w];
+        RelayG32 m{NULL, 0, NULL, NULL, bc};
+        spmv_group_out(m, Nm + k, Nm, np_of, x, y, mode, r0, K, part, o);
+        return;
+    }
     tile_block_out(blockIdx.x, Nb, Nm, np_of, P, bc, x, y, mode, r0, K, part, s_o);
 }
 
@@ -1123,14 +1206,133 @@ k_gm_update(int n_mat, int j, double* __restrict__ w, double* __restrict__ Q,
     if (blockIdx.x == 0 && threadIdx.x <= j) gm->H[j][threadIdx.x] = s_h[threadIdx.x];
 }
 
-// x += K sum_i y_i v_i once the solve (or a restart) is due; no-op otherwise
+// Column j's Gram-Schmidt step with the block preconditioner, aligned to its
+// blocks (precond.h): workgroup (b, s) forms q_{j+1} on all 256 unknowns of
+// block b as k_gm_update forms it on its rows, writes quarter s of them (v_j,
+// w and q_{j+1}) with its partial of |q_{j+1}|^2, and sets u = B_b^-1 q_{j+1}
+// on the quarter from the block's inverse (k_bj_apply's sums); quarter 3
+// also keeps Z_b^T q_{j+1} in cs for the coarse correction, which the
+// product adds.  The last workgroup takes the Lagrange rows (preconditioner
+// 1).  Partials go to part_qq[g] for workgroup g (zero from idle ones).
+__global__ void __launch_bounds__(kVecThreads)
+k_gm_update_bj(int n_mat, int Nm, int Nb, int j, double* __restrict__ w, double* __restrict__ Q,
+               const double* __restrict__ part_h, int n_vb, GmState* gm, const SolveScalars* __restrict__ sc,
+               double* __restrict__ u, double* __restrict__ part_qq, const float* __restrict__ binv,
+               const float4* __restrict__ cw, double* __restrict__ cs) {
+    main_stream_priority();
+    __shared__ double s_w[16];
+    __shared__ double s_h[kGmM + 1];
+    __shared__ double s_g[kVecThreads / (kGmM + 1)][kGmM + 1];
+    __shared__ double s_q[kBjUnk];
+    __shared__ double s_red[4][kTileEdge];
+    __shared__ int s_skip;
+    const int g = blockIdx.x, t = threadIdx.x;
+    const bool lag = g == bj_groups(Nb);
+    int b = 0, s = 0;
+    if (!lag) bj_group(g, b, s);
+    if (!lag && b >= Nb) {
+        if (t == 0) part_qq[g] = 0.0;
+        return;
+    }
+    if (t == 0) s_skip = sc->done;
+    // this thread's unknown: rho = t of block b, or Lagrange row 4 Nm + t
+    const int row = lag ? (4 * Nm + t < n_mat ? 4 * Nm + t : -1) : bj_row(b, t, Nm);
+    const bool mine = row >= 0 && (lag || (t >> 6) == s);      // the rows this workgroup writes
+    double wk = 0.0, qj = 0.0;
+    double qv[kGmM];
+    if (row >= 0) { wk = w[row]; qj = Q[(size_t)j * n_mat + row]; }
+#pragma unroll
+    for (int i = 0; i < kGmM; ++i) qv[i] = row >= 0 && i < j ? Q[(size_t)i * n_mat + row] : 0.0;
+    constexpr int G = kVecThreads / (kGmM + 1);
+    const int ih = t % (kGmM + 1), gg0 = t / (kGmM + 1);
+    double hs = 0.0;
+    if (gg0 < G && ih <= j)
+        for (int b0 = gg0; b0 < n_vb; b0 += 8 * G) {
+            double pv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                pv[q] = b0 + q * G < n_vb ? part_h[(size_t)(b0 + q * G) * (kGmM + 1) + ih] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) hs += pv[q];
+        }
+    __syncthreads();
+    if (s_skip) return;
+    if (gg0 < G) s_g[gg0][ih] = hs;
+    __syncthreads();
+    const double inv = 1.0 / gm->nrm[j];
+    if (t <= j) {
+        double h = 0.0;
+        for (int q = 0; q < G; ++q) h += s_g[q][t];
+        s_h[t] = t == j ? h * inv * inv : h * inv;
+    }
+    __syncthreads();
+    double q = 0.0;
+    if (row >= 0) {
+        const double vj = qj * inv;
+        const double wn = wk * inv;
+        q = wn;
+#pragma unroll
+        for (int i = 0; i < kGmM; ++i)
+            if (i < j) q -= s_h[i] * qv[i];
+        q -= s_h[j] * vj;
+        if (mine) {
+            Q[(size_t)j * n_mat + row] = vj;
+            w[row] = wn;
+            Q[(size_t)(j + 1) * n_mat + row] = q;
+        }
+    }
+    if (g == 0 && t <= j) gm->H[j][t] = s_h[t];
+    if (lag) {
+        if (row >= 0) u[row] = q;
+        const double qq = block_sum(row >= 0 ? q * q : 0.0, s_w);
+        if (t == 0) part_qq[g] = qq;
+        return;
+    }
+    if ((t >> 6) == s) {
+        const double qq = wave_sum(mine ? q * q : 0.0);
+        if ((t & 63) == 0) part_qq[g] = qq;
+    }
+    s_q[t] = q;
+    if (cs && s == 3 && t >= 3 * kTileEdge) {
+        // the block's coarse components Z_b^T q (its charges are unknowns 192..255)
+        const float4 z = cw[b * kTileEdge + (t & 63)];
+        const double c0 = wave_sum(z.x * q), c1 = wave_sum(z.y * q), c2 = wave_sum(z.z * q), c3 = wave_sum(z.w * q);
+        if ((t & 63) == 0) {
+            cs[4 * b] = c0; cs[4 * b + 1] = c1; cs[4 * b + 2] = c2; cs[4 * b + 3] = c3;
+        }
+    }
+    __syncthreads();
+    // u = B_b^-1 q on quarter s: thread (ch, l) sums columns 64 ch .. 64 ch + 63 of row 64 s + l
+    const int ro = s * kTileEdge + (t & 63), ch = t >> 6;
+    const float4* mp = reinterpret_cast<const float4*>(binv + (size_t)b * kBjUnk * kBjUnk) + ro;
+    float4 mv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) mv[i] = mp[(size_t)(16 * ch + i) * kBjUnk];
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c0 = 4 * (16 * ch + i);
+        acc += (double)mv[i].x * s_q[c0] + (double)mv[i].y * s_q[c0 + 1] + (double)mv[i].z * s_q[c0 + 2] +
+               (double)mv[i].w * s_q[c0 + 3];
+    }
+    s_red[ch][t & 63] = acc;
+    __syncthreads();
+    if (t >= kTileEdge) return;
+    const int orow = bj_row(b, ro, Nm);
+    if (orow >= 0) u[orow] = ((s_red[0][t] + s_red[1][t]) + s_red[2][t]) + s_red[3][t];
+}
+
+// x += K sum_i y_i v_i once the solve (or a restart) is due; no-op otherwise.
+// With t_out, sum_i y_i v_i goes there instead (zeros when y is empty), for
+// the block preconditioner's application.
 __global__ void __launch_bounds__(kVecThreads)
 k_gm_finish(int n_mat, const double* __restrict__ Q, const double* __restrict__ K, const GmState* gm,
-            const SolveScalars* __restrict__ sc, double* __restrict__ x) {
+            const SolveScalars* __restrict__ sc, double* __restrict__ x, double* __restrict__ t_out) {
     main_stream_priority();
     if (!sc->done) return;
     const int ny = gm->ny;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t_out && ny == 0 && k < n_mat) t_out[k] = 0.0;
     if (ny == 0 || k >= n_mat) return;
     double t = 0.0;
     for (int i0 = 0; i0 < ny; i0 += 8) {
@@ -1141,7 +1343,8 @@ k_gm_finish(int n_mat, const double* __restrict__ Q, const double* __restrict__ 
         for (int u = 0; u < 8; ++u)
             if (i0 + u < ny) t += gm->y[i0 + u] * vv[u];
     }
-    x[k] += K[k] * t;
+    if (t_out) t_out[k] = t;
+    else x[k] += K[k] * t;
 }
 
 // (rr, rr) of the FP64 residual from its per-group partials (with a guard:
@@ -1213,7 +1416,9 @@ int solver_create(capac_ctx* c) {
     CHECK_HIP(hipMalloc(&S->part3, 3 * ng * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part_res, ng * sizeof(double)));
     CHECK_HIP(hipMalloc(&S->part2, nvb * sizeof(double)));
-    CHECK_HIP(hipMalloc(&S->part4, nvb * sizeof(double)));
+    // part4 also takes the block-aligned update's partials (one per workgroup)
+    const size_t n4p = std::max(nvb, (size_t)bj_groups(tile_blocks(S->Nm)) + 2);
+    CHECK_HIP(hipMalloc(&S->part4, n4p * sizeof(double)));
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
         CHECK_HIP(hipMalloc(&S->Q, (size_t)(kGmM + 1) * n * sizeof(double)));
         CHECK_HIP(hipMemset(S->Q, 0, (size_t)(kGmM + 1) * n * sizeof(double)));
@@ -1266,17 +1471,26 @@ static void spmv_t(capac_ctx* c, const Solver* S, const M& m, const double* x, d
 #ifndef CAPAC_G32_MID_MIN
 #define CAPAC_G32_MID_MIN 768
 #endif
+// coarse (tiles only): x is B^-1 q from the block-preconditioned update and
+// the product adds the coarse correction first (CoarseArgs), its corrected
+// input kept in S->t
 static void spmv(capac_ctx* c, const Solver* S, const double* x, double* y, int mode, double* part,
-                 const SolveScalars* sc) {
+                 const SolveScalars* sc, bool coarse = false) {
     if (c->precision == CAPAC_PREC_FP32_REFINED && c->relay_layout == CAPAC_LAYOUT_TILES) {
         const int nb = c->relay_nb, nt = (int)tile_count(nb);
-        k_spmv_tiles<<<nt + S->n_np, kTileThreads, 0, c->stream>>>(
+        CoarseArgs cg{NULL, NULL, NULL, NULL, 0};
+        if (coarse && c->cg_n) cg = CoarseArgs{c->cg_A, c->cg_s, reinterpret_cast<const float4*>(c->cg_w), S->t,
+                                               c->cg_n};
+        // the Lagrange rows: in the tile launch, or with a coarse correction in the sums launch
+        const int lag_t = cg.nc ? 0 : S->n_np, lag_s = cg.nc ? S->n_np : 0;
+        k_spmv_tiles<<<nt + lag_t, kTileThreads, 0, c->stream>>>(
             reinterpret_cast<const float4*>(c->relay32), nb, nt, S->Nm, S->n_np, c->np_of_metal, c->np_first,
             c->np_last, c->r, c->metal_u, c->bc, x, y, c->relay_part, c->relay_cnt, mode, S->r0, S->K, part,
-            sc);
+            sc, cg);
         if (!CAPAC_TILE_FUSED)
-            k_spmv_tsum<<<nb, kSumThreads, 0, c->stream>>>(nb, S->Nm, c->np_of_metal, c->relay_part, c->bc, x,
-                                                           y, mode, S->r0, S->K, part, sc);
+            k_spmv_tsum<<<nb + lag_s, kSumThreads, 0, c->stream>>>(nb, S->Nm, c->np_of_metal, c->relay_part, c->bc,
+                                                                   cg.nc ? cg.x2 : x, y, mode, S->r0, S->K, part,
+                                                                   sc, c->np_first, c->np_last);
     } else if (c->precision == CAPAC_PREC_FP32_REFINED) {
         RelayG32 m{c->relay32, c->relay_ld, c->r, c->metal_u, c->bc};
         const int nm = c->bc.Nm;
@@ -1371,24 +1585,49 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
     hipStream_t st = c->stream;
     bool first = true;
     int launched = 0;
+    // block preconditioner: the update applies B^-1 itself (block-aligned
+    // launch of ug workgroups); the coarse correction is added by the tiled
+    // product, or by its own launch with the rows layout
+    const bool bjf = S->bj != NULL;
+    const bool cg = bjf && c->cg_n > 0;
+    const bool cg_fused = cg && c->relay_layout == CAPAC_LAYOUT_TILES && !CAPAC_TILE_FUSED;
+    const int ug = bj_groups(tile_blocks(S->Nm)) + 1;
     for (;;) {
         spmv(c, S, x, S->Ax, 0, NULL, NULL);
         k_gm_init<<<vb, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->Q, S->p, S->K, S->part2, S->part4);
         k_gm_start<<<1, 256, 0, st>>>(vb, S->part2, S->part4, S->sc, S->gm, c->solve_crit, first ? 1 : 0);
+        if (bjf) {
+            bj_apply(c, S->Q, S->p, false, S->sc, 1);
+            if (cg && !cg_fused) cg_apply(c, S->p, S->sc, 1);
+        }
         first = false;
         // later batches start small (see bicgstab)
         int j = 0, batch = first_batch > 1 ? first_batch : 2, next = 2;
         for (;;) {
             for (int u = 0; u < batch && j <= kGmM; ++u, ++j) {
-                if (j < kGmM) spmv(c, S, S->p, S->v, 0, NULL, S->sc);
+                if (j < kGmM) spmv(c, S, S->p, S->v, 0, NULL, S->sc, cg_fused);
                 const dim3 grid(j < kGmM ? vb : 1, j < kGmM ? (j + kGmDots) / kGmDots : 1);
-                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, vb, S->gm, S->sc, S->part_h);
-                if (j < kGmM)
-                    k_gm_update<<<vb, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->K, S->part_h, vb, S->gm, S->sc,
-                                                            S->p, S->part4);
+                // |q_j|^2 partials: the start's (vb) for q_0, the update's after
+                const int nqq = j == 0 || !bjf ? vb : ug;
+                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, nqq, S->gm, S->sc, S->part_h);
+                if (j < kGmM) {
+                    if (bjf) {
+                        k_gm_update_bj<<<ug, kVecThreads, 0, st>>>(
+                            n, S->Nm, tile_blocks(S->Nm), j, S->v, S->Q, S->part_h, vb, S->gm, S->sc, S->p, S->part4,
+                            S->bj, reinterpret_cast<const float4*>(c->cg_w), cg ? c->cg_s : NULL);
+                        if (cg && !cg_fused) cg_apply(c, S->p, S->sc, 1);
+                    } else {
+                        k_gm_update<<<vb, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->K, S->part_h, vb, S->gm,
+                                                                S->sc, S->p, S->part4);
+                    }
+                }
             }
             launched += batch;
-            k_gm_finish<<<vb, kVecThreads, 0, st>>>(n, S->Q, S->K, S->gm, S->sc, x);
+            k_gm_finish<<<vb, kVecThreads, 0, st>>>(n, S->Q, S->K, S->gm, S->sc, x, bjf ? S->s : NULL);
+            if (bjf) {
+                bj_apply(c, S->s, x, true, S->sc, 2);
+                cg_apply(c, x, S->sc, 2);
+            }
             if (check64) {
                 int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
                 if (rc) return rc;
@@ -1464,6 +1703,8 @@ static int solver_run_with(capac_ctx* c, Solver* S, const double* b, double* x) 
     if (c->batch_main > 1)
         first = (c->batch_main2 > 1 ? (c->batch_main + c->batch_main2) / 2 : c->batch_main) + 1;
     const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
+    // (the block-aligned update takes the Lagrange rows in one workgroup)
+    S->bj = refine && c->bj_inv && c->bj_valid && S->n_np <= kVecThreads ? c->bj_inv : NULL;
     SolveOut o;
     int rc = inner_solve(c, S, b, x, first, refine, &o);
     if (rc) return rc;
@@ -1519,10 +1760,12 @@ int solver_run(capac_ctx* c, const double* b, double* x) { return solver_run_wit
 constexpr int kNoAllocMaxUnknowns = 16384;
 constexpr size_t kNoAllocGroups = kNoAllocMaxUnknowns + 1;       // metal row-groups + Lagrange rows, bound
 constexpr size_t kNoAllocVecBlocks = (kNoAllocMaxUnknowns + kVecThreads - 1) / kVecThreads + 1;
+// part4 also holds the block-aligned update's partials: at most 4096 metal atoms, 64 blocks
+constexpr size_t kNoAllocPart4 = std::max(kNoAllocVecBlocks, (size_t)bj_groups(kNoAllocMaxUnknowns / 4 / kTileEdge) + 2);
 constexpr size_t kNoAllocScWords = 16;                             // SolveScalars, padded
 constexpr size_t kNoAllocGmWords = (sizeof(GmState) + 255) / 256 * 32;   // GmState, padded to 256 B
 constexpr size_t kNoAllocWords = 12 * (size_t)kNoAllocMaxUnknowns + 5 * kNoAllocGroups +
-                                 2 * kNoAllocVecBlocks + kNoAllocScWords + 4 +
+                                 kNoAllocVecBlocks + kNoAllocPart4 + kNoAllocScWords + 4 +
                                  (size_t)(kGmM + 1) * kNoAllocMaxUnknowns + kNoAllocVecBlocks * (kGmM + 1) +
                                  kNoAllocGmWords;
 static_assert(sizeof(SolveScalars) <= kNoAllocScWords * sizeof(double), "solver scalars overflow their slot");
@@ -1574,7 +1817,7 @@ int solver_run_noalloc(capac_ctx* c, const double* b, double* x) {
     V.part3 = p; p += 3 * kNoAllocGroups;
     V.part_res = p; p += kNoAllocGroups;
     V.part2 = p; p += kNoAllocVecBlocks;
-    V.part4 = p; p += kNoAllocVecBlocks;
+    V.part4 = p; p += kNoAllocPart4;
     V.sc = (SolveScalars*)p; p += kNoAllocScWords;
     V.res = p; p += 4;
     V.Q = p; p += (size_t)(kGmM + 1) * kNoAllocMaxUnknowns;

@@ -182,11 +182,14 @@ GPU_EXPORTS = [
     "capac_em_begin", "capac_em_step", "capac_em_direction",
     "capac_checkpoint_bytes", "capac_checkpoint_save", "capac_checkpoint_load",
     "capac_set_predictor", "capac_predictor", "capac_set_fp32_layout", "capac_fp32_layout",
+    "capac_set_block_precond", "capac_block_precond", "capac_block_precond_inverse",
+    "capac_block_precond_coarse",
     "capac_bonded", "capac_vsite_build", "capac_vsite_spread",
 ]
 NTHETA = 6
 
 LAYOUT_AUTO, LAYOUT_ROWS, LAYOUT_TILES = 0, 1, 2      # CAPAC_LAYOUT_*
+BJ_EVERY_DEFAULT = 32       # CAPAC_BJ_EVERY_DEFAULT
 KERNEL_SPMV, KERNEL_ASSEMBLE, KERNEL_PAIR, KERNEL_NEIGHBOUR, KERNEL_CPIM_MM, KERNEL_APPLY64, KERNEL_SOLVE = range(7)
 DEFAULT_SKIN = 0.1          # nm, CAPAC_DEFAULT_SKIN
 
@@ -241,6 +244,10 @@ def gpu_lib() -> C.CDLL:
         lib.capac_predictor.argtypes = [C.c_void_p]
         lib.capac_set_fp32_layout.argtypes = [C.c_void_p, C.c_int]
         lib.capac_fp32_layout.argtypes = [C.c_void_p]
+        lib.capac_set_block_precond.argtypes = [C.c_void_p, C.c_int]
+        lib.capac_block_precond.argtypes = [C.c_void_p]
+        lib.capac_block_precond_inverse.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.capac_block_precond_coarse.argtypes = [C.c_void_p, _ip, _dp, C.POINTER(C.c_float)]
         lib.capac_checkpoint_bytes.argtypes = [C.c_void_p]
         lib.capac_checkpoint_bytes.restype = C.c_size_t
         lib.capac_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
@@ -327,6 +334,40 @@ class Context:
     @property
     def fp32_layout(self) -> int:
         return int(self.lib.capac_fp32_layout(self.h))
+
+    def set_block_precond(self, every: int):
+        """GMRES preconditioner of the FP32-refined path: block Jacobi over the
+        relay's diagonal blocks of 64 metal atoms, inverted every `every`
+        assemblies, or the Jacobi diagonal for 0 (capac_set_block_precond)."""
+        _check(self.lib.capac_set_block_precond(self.h, int(every)), "capac_set_block_precond")
+
+    @property
+    def block_precond(self) -> int:
+        return int(self.lib.capac_block_precond(self.h))
+
+    def block_precond_inverse(self, block: int) -> np.ndarray:
+        """Block `block`'s inverse as last formed, 256 x 256 FP32 (unknowns:
+        dipoles of its 64 atoms, then their charges)."""
+        out = np.zeros((256, 256), dtype=np.float32)
+        _check(self.lib.capac_block_precond_inverse(self.h, int(block), out.ctypes.data_as(C.POINTER(C.c_float))),
+               "capac_block_precond_inverse")
+        return out
+
+    def block_precond_coarse(self):
+        """The coarse space as last formed: (Ac^-1 as n x n, basis as
+        (ceil(Nm/64) * 64, 4) entries over the metal charges), or None when
+        it is off or not formed."""
+        n = C.c_int()
+        _check(self.lib.capac_block_precond_coarse(self.h, C.byref(n), None, None), "capac_block_precond_coarse")
+        if n.value == 0:
+            return None
+        nb = (self.n_metal + 63) // 64
+        ainv = np.zeros((n.value, n.value))
+        basis = np.zeros((nb * 64, 4), dtype=np.float32)
+        _check(self.lib.capac_block_precond_coarse(self.h, C.byref(n), dptr(ainv),
+                                                   basis.ctypes.data_as(C.POINTER(C.c_float))),
+               "capac_block_precond_coarse")
+        return ainv, basis
 
     def set_theta(self, theta):
         """Fitting parameters (alpha, c, R_q, R_p, eps_SC, c_SC); see capac_set_theta."""

@@ -245,6 +245,33 @@ int capac_predictor(const capac_ctx* ctx);
 int capac_set_fp32_layout(capac_ctx* ctx, int layout);
 int capac_fp32_layout(const capac_ctx* ctx);
 
+/* Preconditioner of the FP32-refined path's GMRES.  every > 0: block Jacobi,
+ * the exact inverses (FP32) of the relay's diagonal blocks of 64 metal
+ * atoms (256 unknowns each: their dipoles and charges), plus an additive
+ * coarse correction over four charge vectors per block (1, x, y, z over its
+ * atoms, orthonormalised; up to 64 blocks), all formed at the first
+ * assembly and again every `every` assemblies (an MD step assembles once);
+ * every = 0: the reference's Jacobi diagonal.  The preconditioner steers
+ * the iterations only: the stop test, and so the accuracy of a solution, is
+ * the same.  The inverses take 256 KiB per block of 64 metal atoms.
+ * capac_block_precond returns the interval in use (-1 on the FP64 path).
+ * No counterpart in the reference. */
+#define CAPAC_BJ_EVERY_DEFAULT 32
+int capac_set_block_precond(capac_ctx* ctx, int every);
+int capac_block_precond(const capac_ctx* ctx);
+/* Diagnostic: the inverse of diagonal block `block` (0 .. ceil(Nm/64) - 1)
+ * as last formed, row-major 256 x 256 into out (host).  Unknown rho < 192 is
+ * dipole component rho % 3 of metal atom 64 block + rho / 3, rho >= 192 the
+ * charge of metal atom 64 block + rho - 192; unknowns past Nm are identity
+ * rows.  No counterpart in the reference. */
+int capac_block_precond_inverse(capac_ctx* ctx, int block, float* out);
+/* Diagnostic: the coarse space as last formed.  *n = its size (4 per block;
+ * 0 when it is off or not formed yet); ainv (n x n, row-major) receives
+ * Ac^-1, basis (4 floats per atom slot, ceil(Nm/64) * 64 slots) the basis
+ * entries over the metal charges, a dropped vector's entries zero.  Either
+ * may be NULL.  No counterpart in the reference. */
+int capac_block_precond_coarse(capac_ctx* ctx, int* n, double* ainv, float* basis);
+
 /* ---- restart checkpoint (no counterpart in the reference) ----
  * The reference restarts from the last traj.gro frame, which keeps
  * positions, velocities, vQ[0] and vP only (file.c:152-160, 215-229): the

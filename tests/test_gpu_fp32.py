@@ -3,9 +3,12 @@
 The relay is stored as 4 FP32 coefficients per metal pair, with its
 T-tensors evaluated in FP32, in either layout: ROWS (every ordered pair) or
 TILES (each unordered pair once, applied to both of its rows); every test
-below runs on both.  GMRES (right-preconditioned with the reference's Jacobi
-preconditioner) runs on it with FP64 vectors and dots.  FP64 iterative refinement then applies the reference's stop test to
-the true residual, with the FP64 relay recomputed on the fly.  Bars:
+below runs on both.  GMRES runs on it with FP64 vectors and dots,
+right-preconditioned with the reference's Jacobi diagonal or with the
+block-Jacobi inverses of the relay's 64-atom diagonal blocks (the default);
+the solve tests run with both.  FP64 iterative refinement then applies the
+reference's stop test to the true residual, with the FP64 relay recomputed
+on the fly.  Bars:
   - relay entries: max |A32 - A64| / max |A64| < 2e-6 (FP32 evaluation);
     the two layouts expand to bitwise the same matrix (same coefficients);
   - solve: the reference criterion rTr < 1e-8 bTb holds for the FP64
@@ -54,16 +57,22 @@ def decks(golden_dir):
 LAYOUTS = {"rows": lib.LAYOUT_ROWS, "tiles": lib.LAYOUT_TILES}
 
 
-def fp32_context(deck, layout):
+PRECONDS = {"block": lib.BJ_EVERY_DEFAULT, "jacobi": 0}
+
+
+def fp32_context(deck, layout, precond="block"):
     c = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
     c.set_fp32_layout(LAYOUTS[layout])
     assert c.fp32_layout == LAYOUTS[layout]
+    c.set_block_precond(PRECONDS[precond])
+    assert c.block_precond == PRECONDS[precond]
     return c
 
 
-@pytest.fixture(scope="module", params=list(LAYOUTS))
+@pytest.fixture(scope="module", params=[(lay, pc) for lay in LAYOUTS for pc in PRECONDS],
+                ids=lambda p: f"{p[0]}-{p[1]}")
 def ctxs(decks, request):
-    out = {name: fp32_context(d, request.param) for name, d in decks.items()}
+    out = {name: fp32_context(d, *request.param) for name, d in decks.items()}
     yield out
     for c in out.values():
         c.close()
