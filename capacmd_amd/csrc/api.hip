@@ -1364,9 +1364,11 @@ namespace {
 // What the MD loop carries from one step to the next, beyond what traj.gro
 // keeps (file.c:215-229 writes positions, velocities, vQ[0] and vP only).
 struct CkHeader {
-    char magic[8];                  // "CAPACCK1"
+    char magic[8];                  // "CAPACCK2"
     int32_t n_atoms, n_metal, n_mat, step;
-    int32_t solve_iters, rows_dirty, precision, pred_order, pred_count, reserved[3];
+    int32_t solve_iters, rows_dirty, precision, pred_order, pred_count;
+    int32_t bj_valid, bj_age;       // block preconditioner: formed, assemblies since
+    int32_t reserved;
     double box[3], rcut, skin_req, skin;
     double first_temp, ext_temp, cur_ref_temp;
     double pot[CAPAC_NPOT], vir[9];
@@ -1374,13 +1376,14 @@ struct CkHeader {
     double red_md[16];              // RATTLE_2 virial of the last step
 };
 // then r, v, f, r_build (3N each), x and the predictor's three stored
-// solutions, newest first (n_mat each)
-const char kCkMagic[8] = {'C', 'A', 'P', 'A', 'C', 'C', 'K', '1'};
+// solutions, newest first (n_mat each), and the metal positions the block
+// preconditioner's inverses were formed at (3 Nm, zeros when none)
+const char kCkMagic[8] = {'C', 'A', 'P', 'A', 'C', 'C', 'K', '2'};
 }  // namespace
 
 extern "C" size_t capac_checkpoint_bytes(const capac_ctx* c) {
     if (!c) return 0;
-    return sizeof(CkHeader) + sizeof(double) * (12 * (size_t)c->bc.N + 4 * (size_t)c->n_mat);
+    return sizeof(CkHeader) + sizeof(double) * (12 * (size_t)c->bc.N + 4 * (size_t)c->n_mat + 3 * (size_t)c->bc.Nm);
 }
 
 extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t bytes) {
@@ -1396,6 +1399,7 @@ extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t b
     h.n_atoms = c->bc.N; h.n_metal = c->bc.Nm; h.n_mat = c->n_mat; h.step = step;
     h.solve_iters = c->solve_iters; h.rows_dirty = c->nb_dirty; h.precision = c->precision;
     h.pred_order = c->pred_order; h.pred_count = c->pred_count;
+    h.bj_valid = c->bj_r && c->bj_valid; h.bj_age = c->bj_age;
     h.box[0] = c->bc.box[0]; h.box[1] = c->bc.box[1]; h.box[2] = c->bc.box[2];
     h.rcut = c->bc.rc; h.skin_req = c->skin_req; h.skin = c->skin;
     h.first_temp = c->first_temp; h.ext_temp = c->ext_temp; h.cur_ref_temp = c->cur_ref_temp;
@@ -1411,6 +1415,12 @@ extern "C" int capac_checkpoint_save(capac_ctx* c, int step, void* buf, size_t b
     for (int k = 0; k < 4 && c->n_mat; ++k)
         CHECK_HIP(hipMemcpyAsync(a + 4 * n3 + (size_t)k * c->n_mat, k ? c->x_hist[k - 1] : c->x,
                                  c->n_mat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double* bj_r = a + 4 * n3 + 4 * (size_t)c->n_mat;
+    if (h.bj_valid)
+        CHECK_HIP(hipMemcpyAsync(bj_r, c->bj_r, 3 * (size_t)c->bc.Nm * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    else
+        memset(bj_r, 0, 3 * (size_t)c->bc.Nm * sizeof(double));
     CHECK_HIP(hipStreamSynchronize(c->stream));
     memcpy(buf, &h, sizeof h);
     return 0;
@@ -1449,6 +1459,21 @@ extern "C" int capac_checkpoint_load(capac_ctx* c, const void* buf, size_t bytes
             if ((rc = grow_nbr(c))) return rc;
         }
         c->nb_dirty = 0;
+    }
+    if (c->bj_inv) {
+        // re-form the block preconditioner at the metal positions the saved
+        // run formed it at, and take over its schedule
+        c->bj_valid = 0;
+        c->bj_age = 0;
+        if (h.bj_valid) {
+            const size_t Nm = (size_t)c->bc.Nm, N = (size_t)c->bc.N;
+            const double* bj_r = a + 4 * n3 + 4 * (size_t)c->n_mat;
+            for (int d = 0; d < 3; ++d)
+                CHECK_HIP(hipMemcpyAsync(c->r + d * N, bj_r + d * Nm, Nm * sizeof(double), hipMemcpyHostToDevice,
+                                         c->stream));
+            if ((rc = cpim_assemble(c))) return rc;
+            c->bj_age = h.bj_age;
+        }
     }
     CHECK_HIP(hipMemcpyAsync(c->r, a, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CHECK_HIP(hipMemcpyAsync(c->v, a + n3, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));

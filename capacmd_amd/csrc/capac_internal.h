@@ -176,6 +176,8 @@ struct capac_ctx {
     // every bj_every assemblies (0: the Jacobi K instead)
     float* bj_inv;
     int bj_every, bj_age, bj_valid;
+    double* bj_r;               // metal positions the inverses were formed at (x, y, z planes of Nm):
+                                // a checkpoint carries them, so a restart re-forms the same inverses
     // its coarse space (cg_n = 4 per block of 64 metal atoms, 0 when off):
     // basis entries per metal atom (float4), active flags, Z^T q of the
     // running application, Ac^-1 (cg_n x cg_n)

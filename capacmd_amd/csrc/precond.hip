@@ -108,11 +108,18 @@ __device__ __forceinline__ void gj256(float (&A)[8][8], float (*s_col)[kBjUnk], 
 
 __global__ void __launch_bounds__(kBjThreads)
 k_bj_build(const float* __restrict__ relay, int tiles, int Nb, int ld, const double* __restrict__ r,
-           const double* __restrict__ mu, BoxConst bc, float* __restrict__ inv) {
+           const double* __restrict__ mu, BoxConst bc, float* __restrict__ inv, double* __restrict__ r_at) {
     main_stream_priority();
     __shared__ float s_col[2][kBjUnk], s_row[2][kBjUnk];
     const int b = blockIdx.x, Nm = bc.Nm;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (threadIdx.x < kTileEdge && b * kTileEdge + (int)threadIdx.x < Nm) {
+        // the block's metal positions, kept with its inverse
+        const int gi = b * kTileEdge + threadIdx.x;
+        r_at[gi] = r[gi];
+        r_at[Nm + gi] = r[bc.N + gi];
+        r_at[2 * Nm + gi] = r[2 * bc.N + gi];
+    }
     const bool compact = mu[4 * Nm] != 0.0;
     float A[8][8];
 #pragma unroll
@@ -353,10 +360,10 @@ k_cg_apply(int Nm, int nc, const double* __restrict__ Ai, const double* __restri
 }  // namespace
 
 void bj_free(capac_ctx* c) {
-    void* p[] = {c->bj_inv, c->cg_w, c->cg_act, c->cg_s, c->cg_A};
+    void* p[] = {c->bj_inv, c->bj_r, c->cg_w, c->cg_act, c->cg_s, c->cg_A};
     for (void* q : p)
         if (q) (void)hipFree(q);
-    c->bj_inv = NULL; c->cg_w = NULL; c->cg_act = NULL; c->cg_s = NULL; c->cg_A = NULL;
+    c->bj_inv = NULL; c->bj_r = NULL; c->cg_w = NULL; c->cg_act = NULL; c->cg_s = NULL; c->cg_A = NULL;
     c->cg_n = 0;
     c->bj_valid = 0;
     c->bj_age = 0;
@@ -369,6 +376,7 @@ int bj_alloc_dev(capac_ctx* c) {
     if (c->bj_every <= 0 || Nm == 0) return 0;
     const int Nb = tile_blocks(Nm), nc = 4 * Nb;
     CHECK_HIP(hipMalloc(&c->bj_inv, (size_t)Nb * kBjUnk * kBjUnk * sizeof(float)));
+    CHECK_HIP(hipMalloc(&c->bj_r, 3 * (size_t)Nm * sizeof(double)));
     if (CAPAC_BJ_COARSE && nc <= kCgMax) {
         CHECK_HIP(hipMalloc(&c->cg_w, (size_t)Nb * kTileEdge * sizeof(float4)));
         CHECK_HIP(hipMalloc(&c->cg_act, (size_t)nc * sizeof(int)));
@@ -388,7 +396,7 @@ int bj_build(capac_ctx* c) {
     const int tiles = c->relay_layout == CAPAC_LAYOUT_TILES;
     const int Nb = tile_blocks(Nm);
     k_bj_build<<<Nb, kBjThreads, 0, c->stream>>>(c->relay32, tiles, Nb, c->relay_ld, c->r, c->metal_u, c->bc,
-                                                 c->bj_inv);
+                                                 c->bj_inv, c->bj_r);
     if (c->cg_n) {
         float4* w = reinterpret_cast<float4*>(c->cg_w);
         k_cg_basis<<<Nb, kTileEdge, 0, c->stream>>>(c->r, c->bc, w, c->cg_act);

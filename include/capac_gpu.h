@@ -279,7 +279,8 @@ int capac_block_precond_coarse(capac_ctx* ctx, int* n, double* ainv, float* basi
  * a restarted run leaves the trajectory it continues.  A checkpoint holds
  * everything the MD loop carries between steps (positions, velocities,
  * forces, vec_pq, the whole chain, the heating ramp, the neighbour-row
- * build positions, the predictor's stored solutions and order) so that
+ * build positions, the predictor's stored solutions and order, the metal
+ * positions the block preconditioner was formed at and its schedule) so that
  * capac_checkpoint_load followed by capac_md_run
  * continues the saved run step for step, bit for bit on the same device and
  * precision.  Save between capac_md_run calls; `step` is stored as given.
