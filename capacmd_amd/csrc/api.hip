@@ -23,6 +23,7 @@ extern "C" const char* capac_last_error(void) { return g_err; }
 extern "C" const char* capac_version(void) { return "capac_gpu 0.1 (gfx950, FP64 + FP32-refined CPIM)"; }
 
 int pair_blocks(int N);
+int pair_partial_blocks(int N);
 
 namespace {
 
@@ -426,6 +427,20 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         rc = upload(&c->pair, (const PairParam*)t, nt * nt);
         free(t);
         if (rc) { capac_destroy(c); return rc; }
+        // the water pair kernel: every pair of non-metal atom types funct 1,
+        // damped-shifted-force Coulomb and shifted-force LJ, alpha rc < 6
+        // (erfc from the erfcx table over the whole cut-off)
+        char* used = (char*)calloc(nt > 0 ? nt : 1, 1);
+        for (int i = p->n_metal; i < N; ++i)
+            if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
+        int water = p->use_coulomb == 1 && p->use_vdw == 1 && p->dsf_alpha >= 0.0 &&
+                    p->dsf_alpha * p->rcut < 5.99 && nt <= 8;
+        for (int a = 0; a < nt && water; ++a)
+            for (int b2 = 0; b2 < nt; ++b2)
+                if (used[a] && used[b2] && p->nb_funct[a * nt + b2] != 1) water = 0;
+        free(used);
+        c->pair_water = water;
+        if (water) TRY(upload(&c->xyzq, (const double*)NULL, 4 * (size_t)N));
     }
     TRY(upload(&c->mol_first, p->mol_first, p->n_mols));
     TRY(upload(&c->mol_last, p->mol_last, p->n_mols));
@@ -504,7 +519,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     // reductions: one partial segment per producer
     {
         const int metal_blocks = (Nm + 3) / 4 + 1;
-        c->seg_cap[SEG_PAIR] = pair_blocks(N);
+        c->seg_cap[SEG_PAIR] = pair_partial_blocks(N);
         c->seg_cap[SEG_SC] = metal_blocks;
         c->seg_cap[SEG_MW] = Nm + 1;        // one workgroup per metal atom
         c->seg_cap[SEG_MM] = Nm + 1;
@@ -569,7 +584,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->relay_cnt, c->metal_u, c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,
-                    c->vsite_par, c->tail_list};
+                    c->vsite_par, c->tail_list, c->xyzq};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k)
         if (ptrs[k]) (void)hipFree(ptrs[k]);
     bj_free(c);

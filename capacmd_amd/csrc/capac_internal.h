@@ -112,6 +112,8 @@ struct capac_ctx {
     int *type, *mol_id, *local_idx;
     unsigned* excl;
     PairParam* pair;
+    int pair_water;             // the water pair kernel applies (pair.hip); then xyzq holds
+    double* xyzq;               // its packed (x, y, z, q) copy, 4 doubles per atom
     // molecules / constraints / nanoparticles
     int *mol_first, *mol_last, *mol_cstr, *cstr_i, *cstr_j;
     double* cstr_r;

@@ -10,12 +10,28 @@
 // wavefront reduction written once (no atomics); each pair is evaluated from
 // both ends, so pair energies and the virial carry a factor 1/2.
 // FP64-VALU bound (erfc + exp per charged pair).
+//
+// Two kernels.  k_pair takes every pair type through the general term.
+// k_pair_water serves the decks whose non-metal atom types meet only in
+// funct 1 pairs under the damped-shifted-force Coulomb and shifted-force LJ
+// (use_coulomb = use_vdw = 1) with alpha rc < 6 -- the water of the CPIM
+// decks: the front segment of a non-metal atom (its non-metal neighbours,
+// ~95% of all row entries) runs a term with no type dispatch and no
+// branches: every pair takes the LJ terms (pairs without LJ have c6 = c12 = 0
+// and zero shift constants, so they add exact zeros), every pair takes the
+// Coulomb terms (a zero charge product adds zeros), exp(-alpha^2 r^2) comes
+// from a polynomial whose coefficients sit in scalar registers, and positions
+// and charges are read as two 16-byte loads from a packed (x, y, z, q) copy.
+// Metal atoms' rows and the back segments (metal neighbours) keep the general
+// term.  Results differ from k_pair's by rounding only.
 #include "capac_internal.h"
 #include "erfc_fast.h"
+#include "exp_poly.h"
 
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kPairRestBlocks = 512;    // workgroups of the water kernel's second launch
 
 // what a lane accumulates: the force on the current atom i, and (over all
 // of the lane's pairs) the two energies and the virial, symmetric for central
@@ -177,12 +193,233 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
     }
 }
 
+// ---------------------------------------------------------------- water path
+
+// the LJ constants of one pair type, as the water term reads them from LDS
+// (64 bytes: the entry's offset is a shift of the type index)
+struct PairLJ {
+    double c6, c12, lj_fc, lj_fc_rc, lj_ec, pad[3];
+};
+
+// 1/sqrt(s) for a positive finite s: the hardware estimate and one
+// third-order Newton step, y + y e (1/2 + 3/8 e), e = 1 - s y^2 (the
+// library rsqrt's own expansion without its zero and infinity cases)
+__device__ __forceinline__ double rsqrt_pos(double s) {
+    const double y = __builtin_amdgcn_rsq(s);
+    const double e = fma(-s * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// funct 1 under use_coulomb = use_vdw = 1 (pair_term's expressions), for
+// every pair without a test of the type or the charges; kqi = K_FQQ q_i,
+// na2 = -alpha^2
+__device__ __forceinline__ void water_term(const BoxConst& bc, const PairLJ& lj, double kqi, double qj, double na2,
+                                           double dx, double dy, double dz, double r2, PairOut& o,
+                                           const double* __restrict__ erfcx_coef) {
+    const double inv_r = rsqrt_pos(r2);
+    const double rij = r2 * inv_r;
+    const double inv_r2 = inv_r * inv_r;
+    // shifted-force LJ (force.c:748-757)
+    const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    const double cr12 = lj.c12 * (inv_r6 * inv_r6), cr6 = lj.c6 * inv_r6;
+    const double f_r = 12.0 * cr12 - 6.0 * cr6;
+    const double flj = (f_r - lj.lj_fc) * inv_r2;
+    o.e8 += (cr12 - cr6) + (rij - bc.rc) * lj.lj_fc_rc - lj.lj_ec;
+    // damped shifted force (force.c:787-795); alpha r < 6, so erfc comes from
+    // the erfcx table on its whole range: erfc = exp(-x^2) erfcx(x)
+    const double qq = kqi * qj;
+    const double ex = exp_nonpos(na2 * r2);
+    const double y = 4.0 * (bc.alpha * rij);
+    const int piece = (int)y;
+    const double t = fma(y - floor(y), 2.0, -1.0);
+    const double* c = erfcx_coef + piece * kErfcxTerms;
+    double p = c[kErfcxTerms - 1];
+#pragma unroll
+    for (int m = kErfcxTerms - 2; m >= 0; --m) p = fma(p, t, c[m]);
+    const double erfc_a = ex * p;
+    const double fqq = qq * ((erfc_a * inv_r2 + bc.a2_sqrtPI * ex * inv_r) - bc.wolfConst) * inv_r;
+    o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+    const double fs = flj + fqq;
+    const double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
+    o.fx -= fjx; o.fy -= fjy; o.fz -= fjz;
+    o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
+    o.v[3] -= dy * fjy; o.v[4] -= dy * fjz; o.v[5] -= dz * fjz;
+}
+
+// the front segment of non-metal atom i (cnt entries from seg) with the water
+// term; the same two-stage pipeline as k_pair
+__device__ __forceinline__ void water_segment(const double2* __restrict__ xyzq, const int* __restrict__ type,
+                                              const int* __restrict__ seg, int cnt, int i, int lane,
+                                              const BoxConst& bc, const PairLJ* s_lj, int ti, double xi, double yi,
+                                              double zi, double kqi, double na2, PairOut& o,
+                                              const double* s_erfcx) {
+    auto nbr_at = [&](int kk) { return kk < cnt ? seg[kk] : i; };
+    int k = lane;
+    const int j0 = nbr_at(k);
+    int j1 = nbr_at(k + CAPAC_WAVE);
+    double2 a0 = xyzq[2 * j0], b0 = xyzq[2 * j0 + 1];
+    int t0 = type[j0];
+    for (; k < cnt; k += CAPAC_WAVE) {
+        const int j2 = nbr_at(k + 2 * CAPAC_WAVE);
+        const double2 a1 = xyzq[2 * j1], b1 = xyzq[2 * j1 + 1];
+        const int t1 = type[j1];
+        double dx = a0.x - xi, dy = a0.y - yi, dz = b0.x - zi;
+        pbc12(bc, dx, dy, dz);
+        const double r2 = dist2_exact(dx, dy, dz);
+        if (r2 < bc.rc2)                   // rows reach rc + skin
+            water_term(bc, s_lj[ti + t0], kqi, b0.y, na2, dx, dy, dz, r2, o, s_erfcx);
+        j1 = j2; a0 = a1; b0 = b1; t0 = t1;
+    }
+}
+
+// the 12 partial sums of a workgroup (pot7, pot8, the virial, the field
+// energy), written to partial[slot * 16 ..]; each ordered pair was visited
+// from both atoms (the field energy, slot 11, once per atom)
+__device__ __forceinline__ void pair_block_partials(const PairOut& o, double (*s_red)[12], int lane,
+                                                    double* __restrict__ partial, int slot) {
+    const int wave = threadIdx.x / CAPAC_WAVE;
+    double vals[12] = {o.e7, o.e8, o.v[0], o.v[1], o.v[2], o.v[1], o.v[3],
+                       o.v[4], o.v[2], o.v[4], o.v[5], o.e9};
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        double s = wave_sum(vals[c]);
+        if (lane == 0) s_red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = 0.0;
+        if (threadIdx.x < 12)
+            for (int w = 0; w < kWavesPerBlock; ++w) s += s_red[w][threadIdx.x];
+        partial[slot * 16 + threadIdx.x] = threadIdx.x == 11 ? s : 0.5 * s;
+    }
+}
+
+// Water kernel, first of two launches: the front segments of the non-metal
+// atoms, one wavefront per atom; writes (or adds to) their forces
+__global__ void __launch_bounds__(256, 4)
+k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, const PairParam* __restrict__ table,
+             BoxConst bc, const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+             double* __restrict__ f, int accumulate, int field, double* __restrict__ partial) {
+    __shared__ PairLJ s_lj[64];
+    __shared__ double s_red[kWavesPerBlock][12];
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    const int N = bc.N, nt = bc.n_types;
+    for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) {
+        const PairParam& pp = table[k];
+        s_lj[k] = PairLJ{pp.c6, pp.c12, pp.lj_fc, pp.lj_fc_rc, pp.lj_ec, {0.0, 0.0, 0.0}};
+    }
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    const int lane = threadIdx.x & (CAPAC_WAVE - 1);
+    const double na2 = -bc.alpha * bc.alpha;
+    const int first = __builtin_amdgcn_readfirstlane(bc.Nm + blockIdx.x * kWavesPerBlock + threadIdx.x / CAPAC_WAVE);
+    PairOut o = {};
+    for (int i = first; i < N; i += gridDim.x * kWavesPerBlock) {
+        const double2 ai = xyzq[2 * i], bi = xyzq[2 * i + 1];
+        const double xi = ai.x, yi = ai.y, zi = bi.x, qi = bi.y;
+        const int ti = type[i] * nt;
+        o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
+        water_segment(xyzq, type, nbr + (size_t)i * stride, n_front[i], i, lane, bc, s_lj, ti, xi, yi, zi,
+                      K_FQQ * qi, na2, o, s_erfcx);
+        double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
+        if (lane == 0) {
+            if (field) {
+                // the external field on the atom's charge, and its energy
+                // (force.c:2251-2260), in the same write
+                fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
+                o.e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+            }
+            if (accumulate) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
+            else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
+        }
+    }
+    pair_block_partials(o, s_red, lane, partial, blockIdx.x);
+}
+
+// Water kernel, second launch: what the first leaves to the general term --
+// the whole rows of the metal atoms (their forces written, or added to) and
+// the back segments (metal neighbours) of the non-metal atoms (added to
+// their forces).  Wavefront w takes metal atoms w, w + W, ... (W wavefronts
+// in all); then workgroup g takes the 64-atom chunks g, g + G, ... of the
+// non-metal atoms, and of the chunk's atoms with a back segment, in index
+// order, its wavefront v takes those of rank v, v + 4, ...  The assignment
+// is fixed, so each workgroup's sums keep a fixed order.
+__global__ void __launch_bounds__(256, 3)
+k_pair_rest(const double* __restrict__ r, const double* __restrict__ charge, const int* __restrict__ type,
+            const PairParam* __restrict__ table, BoxConst bc, const int* __restrict__ n_front,
+            const int* __restrict__ n_back, const int* __restrict__ nbr, int stride, double* __restrict__ f,
+            int accumulate, int field, double* __restrict__ partial, int slot0) {
+    __shared__ PairParam s_tab[64];
+    __shared__ double s_red[kWavesPerBlock][12];
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    const int N = bc.N, Nm = bc.Nm, nt = bc.n_types;
+    for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) s_tab[k] = table[k];
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    const int lane = threadIdx.x & (CAPAC_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / CAPAC_WAVE);
+    const int n_waves = gridDim.x * kWavesPerBlock;
+    PairOut o = {};
+    // atom i's row from entry `skip` on (metal: all of it; others: the back segment)
+    auto do_atom = [&](int i, bool metal) {
+        const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
+        const double qi = charge[i];
+        const int ti = type[i] * nt;
+        const int nf = n_front[i], nb = n_back[i];
+        const int* row = nbr + (size_t)i * stride;
+        const int total = metal ? nf + nb : nb;
+        const int skip = metal ? 0 : nf;
+        o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
+        for (int k = lane; k < total; k += CAPAC_WAVE) {
+            const int kk = k + skip;
+            const int j = kk < nf ? row[kk] : row[stride - nb + (kk - nf)];
+            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+            pbc12(bc, dx, dy, dz);
+            const double r2 = dist2_exact(dx, dy, dz);
+            if (r2 < bc.rc2) pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o, s_erfcx);
+        }
+        double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
+        if (lane == 0) {
+            if (metal && field) {
+                fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
+                o.e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+            }
+            if (accumulate || !metal) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
+            else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
+        }
+    };
+    for (int i = blockIdx.x * kWavesPerBlock + wave; i < Nm; i += n_waves) do_atom(i, true);
+    for (int c0 = Nm + blockIdx.x * CAPAC_WAVE; c0 < N; c0 += gridDim.x * CAPAC_WAVE) {
+        const int mine = c0 + lane;
+        unsigned long long todo = __ballot(mine < N && n_back[mine] > 0);
+        for (int rank = 0; todo; ++rank) {
+            const int i = c0 + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            if ((rank & (kWavesPerBlock - 1)) == wave) do_atom(i, false);
+        }
+    }
+    pair_block_partials(o, s_red, lane, partial, slot0 + blockIdx.x);
+}
+
+// the packed (x, y, z, q) copy the water kernel reads: two double2 per atom
+__global__ void __launch_bounds__(256)
+k_pack_xyzq(const double* __restrict__ r, const double* __restrict__ charge, int N, double2* __restrict__ xyzq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    xyzq[2 * i] = make_double2(r[i], r[N + i]);
+    xyzq[2 * i + 1] = make_double2(r[2 * N + i], charge[i]);
+}
+
 }  // namespace
 
 int pair_blocks(int N) {
     int nb = (N + kWavesPerBlock - 1) / kWavesPerBlock;
     return nb < 2048 ? nb : 2048;
 }
+
+// partial-sum slots the pair launches may write: the water kernel's second
+// launch adds its workgroups after the first's
+int pair_partial_blocks(int N) { return pair_blocks(N) + kPairRestBlocks; }
 
 // writes (or adds to) f; red[0..11) = pot7, pot8, vir[9], and with `field`
 // the external field's force on every charge added in the same write and
@@ -193,9 +430,22 @@ int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks, int field
     int nblk = pair_blocks(c->bc.N);
     if (max_blocks > 0 && nblk > max_blocks) nblk = max_blocks;
     double* f = f_accumulate ? f_accumulate : c->f;
-    k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
-                                        c->n_back, c->nbr, c->nb_stride, f,
-                                        f_accumulate ? 1 : 0, field, partial_seg(c, SEG_PAIR));
+    if (c->pair_water) {
+        double2* xyzq = reinterpret_cast<double2*>(c->xyzq);
+        double* part = partial_seg(c, SEG_PAIR);
+        k_pack_xyzq<<<(c->bc.N + 255) / 256, 256, 0, c->stream>>>(c->r, c->charge, c->bc.N, xyzq);
+        k_pair_water<<<nblk, 256, 0, c->stream>>>(xyzq, c->type, c->pair, c->bc, c->n_front, c->nbr,
+                                                  c->nb_stride, f, f_accumulate ? 1 : 0, field, part);
+        k_pair_rest<<<kPairRestBlocks, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc,
+                                                            c->n_front, c->n_back, c->nbr, c->nb_stride, f,
+                                                            f_accumulate ? 1 : 0, field, part, nblk);
+        CHECK_HIP(hipGetLastError());
+        return reduce_partials(c, SEG_PAIR, nblk + kPairRestBlocks, 12, 0);
+    } else {
+        k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
+                                            c->n_back, c->nbr, c->nb_stride, f,
+                                            f_accumulate ? 1 : 0, field, partial_seg(c, SEG_PAIR));
+    }
     CHECK_HIP(hipGetLastError());
     return reduce_partials(c, SEG_PAIR, nblk, 12, 0);
 }
