@@ -438,8 +438,29 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         for (int a = 0; a < nt && water; ++a)
             for (int b2 = 0; b2 < nt; ++b2)
                 if (used[a] && used[b2] && p->nb_funct[a * nt + b2] != 1) water = 0;
+        int lj_types = 0;
+        for (int a = 0; a < nt && a < 31; ++a)
+            for (int b2 = 0; b2 < nt; ++b2)
+                if (used[b2] && p->nb_funct[a * nt + b2] == 1 &&
+                    (p->nb_c6[a * nt + b2] != 0.0 || p->nb_c12[a * nt + b2] != 0.0))
+                    lj_types |= 1 << a;
+        c->pair_lj_types = lj_types;
         free(used);
         c->pair_water = water;
+        // metal-metal pairs that the pair term would evaluate to exact zeros
+        char* mused = (char*)calloc(nt > 0 ? nt : 1, 1);
+        int inert = 1;
+        for (int i = 0; i < p->n_metal; ++i) {
+            if (p->charge[i] != 0.0) inert = 0;
+            if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) mused[p->atomtype[i]] = 1;
+        }
+        for (int a = 0; a < nt && inert; ++a)
+            for (int b2 = 0; b2 < nt; ++b2)
+                if (mused[a] && mused[b2] &&
+                    (p->nb_funct[a * nt + b2] != 1 || p->nb_c6[a * nt + b2] != 0.0 || p->nb_c12[a * nt + b2] != 0.0))
+                    inert = 0;
+        free(mused);
+        c->pair_metal_inert = inert;
         if (water) TRY(upload(&c->xyzq, (const double*)NULL, 4 * (size_t)N));
     }
     TRY(upload(&c->mol_first, p->mol_first, p->n_mols));
@@ -816,6 +837,9 @@ static void mark(capac_ctx* c, int k) {
 // Au2057 FP32-refined with the quadratic predictor, ms per MD step on one
 // box: 2.53 at 256 workgroups, 2.22 at 384, 2.01 at 512, 2.10 at 640, 2.23
 // at 768, 2.15 at 1024, 2.23 at 1536, 2.20 with the full grid (2048).
+#ifndef CAPAC_SIDE_EARLY
+#define CAPAC_SIDE_EARLY 0
+#endif
 #ifndef CAPAC_SIDE_BLOCKS_LARGE
 #define CAPAC_SIDE_BLOCKS_LARGE 512
 #endif
@@ -880,7 +904,7 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         // 450 us on average.  So there the pair kernel starts once the RHS
         // and the assembly are done and overlaps only the solve; the capped
         // grid of the small decks leaves slots free and starts at once.
-        const bool side_after_matrix = c->n_mat && c->bc.Nm >= 1024;
+        const bool side_after_matrix = !CAPAC_SIDE_EARLY && c->n_mat && c->bc.Nm >= 1024;
         if (!side_after_matrix && (rc = side_forces(c))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {

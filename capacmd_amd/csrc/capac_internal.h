@@ -112,6 +112,8 @@ struct capac_ctx {
     int *type, *mol_id, *local_idx;
     unsigned* excl;
     PairParam* pair;
+    int pair_metal_inert;       // metal-metal pairs add nothing (uncharged metal, funct 1 with c6 = c12 = 0)
+    int pair_lj_types;          // bit t: type t has funct-1 LJ (c6 or c12 nonzero) with a non-metal type
     int pair_water;             // the water pair kernel applies (pair.hip); then xyzq holds
     double* xyzq;               // its packed (x, y, z, q) copy, 4 doubles per atom
     // molecules / constraints / nanoparticles
@@ -223,9 +225,10 @@ struct capac_ctx {
 
 // pbc_12 (force.c:111-121)
 __device__ __forceinline__ void pbc12(const BoxConst& b, double& x, double& y, double& z) {
-    if (x < -b.half[0]) x += b.box[0]; else if (x > b.half[0]) x -= b.box[0];
-    if (y < -b.half[1]) y += b.box[1]; else if (y > b.half[1]) y -= b.box[1];
-    if (z < -b.half[2]) z += b.box[2]; else if (z > b.half[2]) z -= b.box[2];
+    // x < -half: x + L; x > half: x - L (one magnitude test per component)
+    if (fabs(x) > b.half[0]) x = x < 0.0 ? x + b.box[0] : x - b.box[0];
+    if (fabs(y) > b.half[1]) y = y < 0.0 ? y + b.box[1] : y - b.box[1];
+    if (fabs(z) > b.half[2]) z = z < 0.0 ? z + b.box[2] : z - b.box[2];
 }
 
 // dist_2 (force.c:41-44) evaluated exactly as the reference: (x*x + y*y) + z*z,

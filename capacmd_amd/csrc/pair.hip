@@ -43,7 +43,11 @@ struct PairOut {
 // The reference's expressions with each division by r, r^2, r^6 or the erf
 // width replaced by a product with 1/r from one reciprocal square root per
 // pair, and r = r^2 (1/r); the shifted-force LJ constants of the pair type
-// come from the table (results move by rounding only).
+// come from the table (results move by rounding only).  Poly: the
+// exponentials from exp_poly and the erf-damped vdW's erf from the erfcx
+// table (the water kernel's second launch, which has registers to spare);
+// otherwise the library exp and erf (k_pair, which would spill).
+template <bool Poly>
 __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& pp, double qi, double qj,
                                           double dx, double dy, double dz, double r2, PairOut& o,
                                           const double* __restrict__ erfcx_coef) {
@@ -69,7 +73,7 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
             double qq = K_FQQ * qi * qj;
             if (bc.use_coulomb == 1) {
                 // damped shifted force (force.c:787-795)
-                double ex = exp(-bc.alpha * bc.alpha * r2);
+                double ex = Poly ? exp_poly<false>(-bc.alpha * bc.alpha * r2) : exp(-bc.alpha * bc.alpha * r2);
                 double erfc_a = erfc_from_exp(bc.alpha * rij, ex, erfcx_coef);
                 fqq = qq * ((erfc_a * inv_r2 + bc.a2_sqrtPI * ex * inv_r) - bc.wolfConst) * inv_r;
                 o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
@@ -83,16 +87,19 @@ __device__ __forceinline__ void pair_term(const BoxConst& bc, const PairParam& p
             // erf-damped vdW (force.c:602-628)
             const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
             const double cr12 = pp.c12 * (inv_r6 * inv_r6), cr6 = pp.c6 * inv_r6;
+            // (erf and the Gaussian from one exponential: erf = 1 - e^{-x^2}
+            // erfcx(x) from the erfcx table for x >= 0.5)
             double rR = rij * pp.inv_width;
-            double erf_rR = erf(rR);
+            const double ex = Poly ? exp_poly<false>(-rR * rR) : exp(-rR * rR);
+            double erf_rR = Poly ? erf_from_exp(rR, ex, erfcx_coef) : erf(rR);
             o.e8 += (cr12 - cr6) * erf_rR;
             flj = ((12.0 * cr12 - 6.0 * cr6) * inv_r * erf_rR -
-                   (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * exp(-rR * rR) * pp.inv_width) * inv_r;
+                   (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * ex * pp.inv_width) * inv_r;
         }
     } else if (pp.funct == 2) {
         // Buckingham (force.c:674-703): A e^{-B r} - C6 / r^6, no Coulomb term
         const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-        const double e = exp(-pp.b * rij);
+        const double e = Poly ? exp_poly<false>(-pp.b * rij) : exp(-pp.b * rij);
         flj = pp.a * e * pp.b * inv_r - 6.0 * pp.c6 * (inv_r6 * inv_r2);
         o.e8 += pp.a * e - pp.c6 * inv_r6;
     } else if (pp.funct == 3) {
@@ -159,7 +166,7 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
             pbc12(bc, dx, dy, dz);
             double r2 = dist2_exact(dx, dy, dz);
             if (r2 < bc.rc2)                  // rows reach rc + skin
-                pair_term(bc, s_tab[ti + t0], qi, q0, dx, dy, dz, r2, o, s_erfcx);
+                pair_term<false>(bc, s_tab[ti + t0], qi, q0, dx, dy, dz, r2, o, s_erfcx);
             j1 = j2; x0 = x1; y0 = y1; z0 = z1; q0 = q1; t0 = t1;
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
@@ -212,23 +219,27 @@ __device__ __forceinline__ double rsqrt_pos(double s) {
 
 // funct 1 under use_coulomb = use_vdw = 1 (pair_term's expressions), for
 // every pair without a test of the type or the charges; kqi = K_FQQ q_i,
-// na2 = -alpha^2
-__device__ __forceinline__ void water_term(const BoxConst& bc, const PairLJ& lj, double kqi, double qj, double na2,
-                                           double dx, double dy, double dz, double r2, PairOut& o,
+// na2 = -alpha^2.  lj_row false (the same on every lane): atom i's type has
+// no LJ with any non-metal type, so the LJ terms (exact zeros) are left out
+__device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, const PairLJ& lj, double kqi, double qj,
+                                           double na2, double dx, double dy, double dz, double r2, PairOut& o,
                                            const double* __restrict__ erfcx_coef) {
     const double inv_r = rsqrt_pos(r2);
     const double rij = r2 * inv_r;
     const double inv_r2 = inv_r * inv_r;
     // shifted-force LJ (force.c:748-757)
-    const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-    const double cr12 = lj.c12 * (inv_r6 * inv_r6), cr6 = lj.c6 * inv_r6;
-    const double f_r = 12.0 * cr12 - 6.0 * cr6;
-    const double flj = (f_r - lj.lj_fc) * inv_r2;
-    o.e8 += (cr12 - cr6) + (rij - bc.rc) * lj.lj_fc_rc - lj.lj_ec;
+    double flj = 0.0;
+    if (lj_row) {
+        const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+        const double cr12 = lj.c12 * (inv_r6 * inv_r6), cr6 = lj.c6 * inv_r6;
+        const double f_r = 12.0 * cr12 - 6.0 * cr6;
+        flj = (f_r - lj.lj_fc) * inv_r2;
+        o.e8 += (cr12 - cr6) + (rij - bc.rc) * lj.lj_fc_rc - lj.lj_ec;
+    }
     // damped shifted force (force.c:787-795); alpha r < 6, so erfc comes from
     // the erfcx table on its whole range: erfc = exp(-x^2) erfcx(x)
     const double qq = kqi * qj;
-    const double ex = exp_nonpos(na2 * r2);
+    const double ex = exp_poly<true>(na2 * r2);
     const double y = 4.0 * (bc.alpha * rij);
     const int piece = (int)y;
     const double t = fma(y - floor(y), 2.0, -1.0);
@@ -247,28 +258,51 @@ __device__ __forceinline__ void water_term(const BoxConst& bc, const PairLJ& lj,
 }
 
 // the front segment of non-metal atom i (cnt entries from seg) with the water
-// term; the same two-stage pipeline as k_pair
-__device__ __forceinline__ void water_segment(const double2* __restrict__ xyzq, const int* __restrict__ type,
-                                              const int* __restrict__ seg, int cnt, int i, int lane,
-                                              const BoxConst& bc, const PairLJ* s_lj, int ti, double xi, double yi,
-                                              double zi, double kqi, double na2, PairOut& o,
+// term.  Software pipeline three rounds deep: while round s is evaluated, the
+// positions and charges of rounds s + 1 and s + 2 and the neighbour index of
+// round s + 3 are in flight (beside the solve the kernel has two wavefronts
+// per SIMD, too few to hide a gather's latency behind one round of work:
+// with two rounds in flight the Au2057 launch beside the solve took 0.79 ms
+// per MD step, with three 0.69 ms).  Unrolled three times, so that each
+// round's data keep their registers.
+__device__ __forceinline__ void water_segment(bool lj_row, const double2* __restrict__ xyzq,
+                                              const int* __restrict__ type, const int* __restrict__ seg, int cnt,
+                                              int i, int lane, const BoxConst& bc, const PairLJ* s_lj, int ti,
+                                              double xi, double yi, double zi, double kqi, double na2, PairOut& o,
                                               const double* s_erfcx) {
     auto nbr_at = [&](int kk) { return kk < cnt ? seg[kk] : i; };
-    int k = lane;
-    const int j0 = nbr_at(k);
-    int j1 = nbr_at(k + CAPAC_WAVE);
-    double2 a0 = xyzq[2 * j0], b0 = xyzq[2 * j0 + 1];
-    int t0 = type[j0];
-    for (; k < cnt; k += CAPAC_WAVE) {
-        const int j2 = nbr_at(k + 2 * CAPAC_WAVE);
-        const double2 a1 = xyzq[2 * j1], b1 = xyzq[2 * j1 + 1];
-        const int t1 = type[j1];
-        double dx = a0.x - xi, dy = a0.y - yi, dz = b0.x - zi;
+    auto round = [&](const double2& a, const double2& b, int t) {
+        double dx = a.x - xi, dy = a.y - yi, dz = b.x - zi;
         pbc12(bc, dx, dy, dz);
         const double r2 = dist2_exact(dx, dy, dz);
         if (r2 < bc.rc2)                   // rows reach rc + skin
-            water_term(bc, s_lj[ti + t0], kqi, b0.y, na2, dx, dy, dz, r2, o, s_erfcx);
-        j1 = j2; a0 = a1; b0 = b1; t0 = t1;
+            water_term(lj_row, bc, s_lj[ti + t], kqi, b.y, na2, dx, dy, dz, r2, o, s_erfcx);
+    };
+    constexpr int W = CAPAC_WAVE;
+    int k = lane;
+    int i0 = nbr_at(k), i1 = nbr_at(k + W), i2 = nbr_at(k + 2 * W);
+    double2 a0 = xyzq[2 * i0], b0 = xyzq[2 * i0 + 1];
+    int t0 = type[i0];
+    double2 a1 = xyzq[2 * i1], b1 = xyzq[2 * i1 + 1];
+    int t1 = type[i1];
+    double2 a2, b2;
+    int t2;
+    i0 = nbr_at(k + 3 * W);
+    while (k < cnt) {
+        a2 = xyzq[2 * i2]; b2 = xyzq[2 * i2 + 1]; t2 = type[i2];
+        i1 = nbr_at(k + 4 * W);
+        round(a0, b0, t0);
+        k += W;
+        if (k >= cnt) break;
+        a0 = xyzq[2 * i0]; b0 = xyzq[2 * i0 + 1]; t0 = type[i0];
+        i2 = nbr_at(k + 4 * W);
+        round(a1, b1, t1);
+        k += W;
+        if (k >= cnt) break;
+        a1 = xyzq[2 * i1]; b1 = xyzq[2 * i1 + 1]; t1 = type[i1];
+        i0 = nbr_at(k + 4 * W);
+        round(a2, b2, t2);
+        k += W;
     }
 }
 
@@ -296,10 +330,10 @@ __device__ __forceinline__ void pair_block_partials(const PairOut& o, double (*s
 
 // Water kernel, first of two launches: the front segments of the non-metal
 // atoms, one wavefront per atom; writes (or adds to) their forces
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(256, 3)
 k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, const PairParam* __restrict__ table,
              BoxConst bc, const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
-             double* __restrict__ f, int accumulate, int field, double* __restrict__ partial) {
+             double* __restrict__ f, int accumulate, int field, double* __restrict__ partial, int lj_types) {
     __shared__ PairLJ s_lj[64];
     __shared__ double s_red[kWavesPerBlock][12];
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
@@ -319,7 +353,8 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
         const double xi = ai.x, yi = ai.y, zi = bi.x, qi = bi.y;
         const int ti = type[i] * nt;
         o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
-        water_segment(xyzq, type, nbr + (size_t)i * stride, n_front[i], i, lane, bc, s_lj, ti, xi, yi, zi,
+        const bool lj_row = (lj_types >> type[i]) & 1;
+        water_segment(lj_row, xyzq, type, nbr + (size_t)i * stride, n_front[i], i, lane, bc, s_lj, ti, xi, yi, zi,
                       K_FQQ * qi, na2, o, s_erfcx);
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
@@ -337,18 +372,20 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
 }
 
 // Water kernel, second launch: what the first leaves to the general term --
-// the whole rows of the metal atoms (their forces written, or added to) and
+// the whole rows of the metal atoms (their forces written, or added to; only
+// the front segment when metal-metal pairs add nothing) and
 // the back segments (metal neighbours) of the non-metal atoms (added to
-// their forces).  Wavefront w takes metal atoms w, w + W, ... (W wavefronts
-// in all); then workgroup g takes the 64-atom chunks g, g + G, ... of the
-// non-metal atoms, and of the chunk's atoms with a back segment, in index
-// order, its wavefront v takes those of rank v, v + 4, ...  The assignment
-// is fixed, so each workgroup's sums keep a fixed order.
-__global__ void __launch_bounds__(256, 3)
+// their forces).  Wavefront w of W takes metal atoms w, w + W, ..., then
+// the non-metal atoms Nm + w + k W (k = 0, 1, ...) that have a back
+// segment: its lanes test 64 of them at once and it takes those found in
+// order.  The non-metal atoms near the metal are runs of consecutive
+// indices; the stride spreads them evenly over the wavefronts.  The
+// assignment is fixed, so each workgroup's sums keep a fixed order.
+__global__ void __launch_bounds__(256, 2)
 k_pair_rest(const double* __restrict__ r, const double* __restrict__ charge, const int* __restrict__ type,
             const PairParam* __restrict__ table, BoxConst bc, const int* __restrict__ n_front,
             const int* __restrict__ n_back, const int* __restrict__ nbr, int stride, double* __restrict__ f,
-            int accumulate, int field, double* __restrict__ partial, int slot0) {
+            int accumulate, int field, double* __restrict__ partial, int slot0, int metal_inert) {
     __shared__ PairParam s_tab[64];
     __shared__ double s_red[kWavesPerBlock][12];
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
@@ -367,7 +404,9 @@ k_pair_rest(const double* __restrict__ r, const double* __restrict__ charge, con
         const int ti = type[i] * nt;
         const int nf = n_front[i], nb = n_back[i];
         const int* row = nbr + (size_t)i * stride;
-        const int total = metal ? nf + nb : nb;
+        // (a metal atom's back segment, its metal neighbours, adds exact
+        // zeros when the metal is inert)
+        const int total = metal ? (metal_inert ? nf : nf + nb) : nb;
         const int skip = metal ? 0 : nf;
         o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
         for (int k = lane; k < total; k += CAPAC_WAVE) {
@@ -376,7 +415,7 @@ k_pair_rest(const double* __restrict__ r, const double* __restrict__ charge, con
             double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
             pbc12(bc, dx, dy, dz);
             const double r2 = dist2_exact(dx, dy, dz);
-            if (r2 < bc.rc2) pair_term(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o, s_erfcx);
+            if (r2 < bc.rc2) pair_term<true>(bc, s_tab[ti + type[j]], qi, charge[j], dx, dy, dz, r2, o, s_erfcx);
         }
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
@@ -388,14 +427,15 @@ k_pair_rest(const double* __restrict__ r, const double* __restrict__ charge, con
             else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
         }
     };
-    for (int i = blockIdx.x * kWavesPerBlock + wave; i < Nm; i += n_waves) do_atom(i, true);
-    for (int c0 = Nm + blockIdx.x * CAPAC_WAVE; c0 < N; c0 += gridDim.x * CAPAC_WAVE) {
-        const int mine = c0 + lane;
+    const int w = blockIdx.x * kWavesPerBlock + wave;
+    for (int i = w; i < Nm; i += n_waves) do_atom(i, true);
+    for (long base = (long)Nm + w; base < N; base += (long)CAPAC_WAVE * n_waves) {
+        const long mine = base + (long)lane * n_waves;
         unsigned long long todo = __ballot(mine < N && n_back[mine] > 0);
-        for (int rank = 0; todo; ++rank) {
-            const int i = c0 + __builtin_ctzll(todo);
+        while (todo) {
+            const int i = (int)(base + (long)__builtin_ctzll(todo) * n_waves);
             todo &= todo - 1;
-            if ((rank & (kWavesPerBlock - 1)) == wave) do_atom(i, false);
+            do_atom(i, false);
         }
     }
     pair_block_partials(o, s_red, lane, partial, slot0 + blockIdx.x);
@@ -435,10 +475,12 @@ int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks, int field
         double* part = partial_seg(c, SEG_PAIR);
         k_pack_xyzq<<<(c->bc.N + 255) / 256, 256, 0, c->stream>>>(c->r, c->charge, c->bc.N, xyzq);
         k_pair_water<<<nblk, 256, 0, c->stream>>>(xyzq, c->type, c->pair, c->bc, c->n_front, c->nbr,
-                                                  c->nb_stride, f, f_accumulate ? 1 : 0, field, part);
+                                                  c->nb_stride, f, f_accumulate ? 1 : 0, field, part,
+                                                  c->pair_lj_types);
         k_pair_rest<<<kPairRestBlocks, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc,
                                                             c->n_front, c->n_back, c->nbr, c->nb_stride, f,
-                                                            f_accumulate ? 1 : 0, field, part, nblk);
+                                                            f_accumulate ? 1 : 0, field, part, nblk,
+                                                            c->pair_metal_inert);
         CHECK_HIP(hipGetLastError());
         return reduce_partials(c, SEG_PAIR, nblk + kPairRestBlocks, 12, 0);
     } else {

@@ -1099,11 +1099,14 @@ __device__ void gm_column(GmState* gm, SolveScalars* sc, int jj, double hnext, G
 // q_j when it has |q_j|); the update launch applies the 1 / |q_j| scales, so
 // no workgroup here waits for |q_j| except workgroup (0, 0), which forms it,
 // keeps it and rotates column j - 1.  j == kGmM only rotates the last column.
+// With qcopy, the workgroups that load q_j also copy it there: the block
+// update launch reads q_j from the copy, since its workgroups normalise Q[j]
+// in place while others of the same block still read it.
 constexpr int kGmDots = 8;
 __global__ void __launch_bounds__(kVecThreads)
 k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restrict__ Q,
           const double* __restrict__ part_qq, int n_vb, GmState* gm, SolveScalars* sc,
-          double* __restrict__ part_h) {
+          double* __restrict__ part_h, double* __restrict__ qcopy) {
     main_stream_priority();
     __shared__ double s_w[16 * kGmDots];
     __shared__ GmLds s_gm;
@@ -1118,6 +1121,11 @@ k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restri
     double qv[kGmDots];
 #pragma unroll
     for (int t = 0; t < kGmDots; ++t) qv[t] = row && i0 + t <= j ? Q[(size_t)(i0 + t) * n_mat + k] : 0.0;
+    if (qcopy && row && i0 <= j && j < i0 + kGmDots) {
+#pragma unroll
+        for (int t = 0; t < kGmDots; ++t)
+            if (i0 + t == j) qcopy[k] = qv[t];
+    }
     double pq = 0.0;
     if (lead)
         for (int b = threadIdx.x; b < n_vb; b += blockDim.x) pq += part_qq[b];
@@ -1214,8 +1222,13 @@ k_gm_update(int n_mat, int j, double* __restrict__ w, double* __restrict__ Q,
 // also keeps Z_b^T q_{j+1} in cs for the coarse correction, which the
 // product adds.  The last workgroup takes the Lagrange rows (preconditioner
 // 1).  Partials go to part_qq[g] for workgroup g (zero from idle ones).
+// Every workgroup of a block reads w and q_j on all of the block's rows, and
+// the owners of the other quarters write Q[j] in the same launch, so q_j
+// comes from the dots launch's copy qj_in, and w (not read after this
+// launch) is left unnormalised.
 __global__ void __launch_bounds__(kVecThreads)
-k_gm_update_bj(int n_mat, int Nm, int Nb, int j, double* __restrict__ w, double* __restrict__ Q,
+k_gm_update_bj(int n_mat, int Nm, int Nb, int j, const double* __restrict__ w, const double* __restrict__ qj_in,
+               double* __restrict__ Q,
                const double* __restrict__ part_h, int n_vb, GmState* gm, const SolveScalars* __restrict__ sc,
                double* __restrict__ u, double* __restrict__ part_qq, const float* __restrict__ binv,
                const float4* __restrict__ cw, double* __restrict__ cs) {
@@ -1240,7 +1253,7 @@ k_gm_update_bj(int n_mat, int Nm, int Nb, int j, double* __restrict__ w, double*
     const bool mine = row >= 0 && (lag || (t >> 6) == s);      // the rows this workgroup writes
     double wk = 0.0, qj = 0.0;
     double qv[kGmM];
-    if (row >= 0) { wk = w[row]; qj = Q[(size_t)j * n_mat + row]; }
+    if (row >= 0) { wk = w[row]; qj = qj_in[row]; }
 #pragma unroll
     for (int i = 0; i < kGmM; ++i) qv[i] = row >= 0 && i < j ? Q[(size_t)i * n_mat + row] : 0.0;
     constexpr int G = kVecThreads / (kGmM + 1);
@@ -1277,7 +1290,6 @@ k_gm_update_bj(int n_mat, int Nm, int Nb, int j, double* __restrict__ w, double*
         q -= s_h[j] * vj;
         if (mine) {
             Q[(size_t)j * n_mat + row] = vj;
-            w[row] = wn;
             Q[(size_t)(j + 1) * n_mat + row] = q;
         }
     }
@@ -1609,12 +1621,13 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
                 const dim3 grid(j < kGmM ? vb : 1, j < kGmM ? (j + kGmDots) / kGmDots : 1);
                 // |q_j|^2 partials: the start's (vb) for q_0, the update's after
                 const int nqq = j == 0 || !bjf ? vb : ug;
-                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, nqq, S->gm, S->sc, S->part_h);
+                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, nqq, S->gm, S->sc, S->part_h,
+                                                        bjf ? S->z : NULL);
                 if (j < kGmM) {
                     if (bjf) {
                         k_gm_update_bj<<<ug, kVecThreads, 0, st>>>(
-                            n, S->Nm, tile_blocks(S->Nm), j, S->v, S->Q, S->part_h, vb, S->gm, S->sc, S->p, S->part4,
-                            S->bj, reinterpret_cast<const float4*>(c->cg_w), cg ? c->cg_s : NULL);
+                            n, S->Nm, tile_blocks(S->Nm), j, S->v, S->z, S->Q, S->part_h, vb, S->gm, S->sc, S->p,
+                            S->part4, S->bj, reinterpret_cast<const float4*>(c->cg_w), cg ? c->cg_s : NULL);
                         if (cg && !cg_fused) cg_apply(c, S->p, S->sc, 1);
                     } else {
                         k_gm_update<<<vb, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->K, S->part_h, vb, S->gm,

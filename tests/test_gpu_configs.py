@@ -242,6 +242,35 @@ def test_au2057_force_call(path, au2057):
     check_against(fg, pg, vg, f, pot, vir, f"cfg3 {path}")
 
 
+@pytest.mark.parametrize("index", [1, 3])
+def test_fp32_runs_repeat_bitwise(index, cfg):
+    """Three contexts from the deck's start give the same FP32-refined force
+    call and three MD steps bit for bit.  The pair launches beside the solve
+    change the solve's timing from run to run, never its arithmetic: every
+    reduction has a fixed order, and no workgroup reads what another writes
+    in the same launch (a block-preconditioned GMRES update that normalised
+    the basis in place while other workgroups of the block still read it
+    gave 74-80 products on the first Au2057 solve, depending on timing)."""
+    deck = cfg(index)
+    runs = []
+    for rep in range(3):
+        ctx = lib.Context(deck, precision=lib.PREC_FP32_REFINED)
+        f, pot, vir = ctx.force()
+        its = [ctx.solve_stats()[0]]
+        ctx.md_begin()
+        for step in range(1, 4):
+            assert ctx.md_run(step, 1, 1) == 1
+            its.append(ctx.solve_stats()[0])
+        r, v, x = ctx.get_state()
+        runs.append((f.copy(), r, v, x, its))
+        ctx.close()
+        print(f"cfg{index} run {rep}: products per solve {its}", flush=True)
+    for other in runs[1:]:
+        assert other[4] == runs[0][4]
+        for a, b in zip(runs[0][:4], other[:4]):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", [lib.PREC_FP64, lib.PREC_FP32_REFINED], ids=["fp64", "fp32"])
 def test_md_config1_tight(prec, cfg):
     """Three MD steps of the Au561 deck; with both sides solving to
