@@ -837,9 +837,6 @@ static void mark(capac_ctx* c, int k) {
 // Au2057 FP32-refined with the quadratic predictor, ms per MD step on one
 // box: 2.53 at 256 workgroups, 2.22 at 384, 2.01 at 512, 2.10 at 640, 2.23
 // at 768, 2.15 at 1024, 2.23 at 1536, 2.20 with the full grid (2048).
-#ifndef CAPAC_SIDE_EARLY
-#define CAPAC_SIDE_EARLY 0
-#endif
 #ifndef CAPAC_SIDE_BLOCKS_LARGE
 #define CAPAC_SIDE_BLOCKS_LARGE 512
 #endif
@@ -904,7 +901,7 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         // 450 us on average.  So there the pair kernel starts once the RHS
         // and the assembly are done and overlaps only the solve; the capped
         // grid of the small decks leaves slots free and starts at once.
-        const bool side_after_matrix = !CAPAC_SIDE_EARLY && c->n_mat && c->bc.Nm >= 1024;
+        const bool side_after_matrix = c->n_mat && c->bc.Nm >= 1024;
         if (!side_after_matrix && (rc = side_forces(c))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {

@@ -231,6 +231,15 @@ __device__ __forceinline__ void pbc12(const BoxConst& b, double& x, double& y, d
     if (fabs(z) > b.half[2]) z = z < 0.0 ? z + b.box[2] : z - b.box[2];
 }
 
+// 1/sqrt(s) for a positive finite s: the hardware estimate and one
+// third-order Newton step, y + y e (1/2 + 3/8 e), e = 1 - s y^2 (the
+// library rsqrt's own expansion without its zero and infinity cases)
+__device__ __forceinline__ double rsqrt_pos(double s) {
+    const double y = __builtin_amdgcn_rsq(s);
+    const double e = fma(-s * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // dist_2 (force.c:41-44) evaluated exactly as the reference: (x*x + y*y) + z*z,
 // each product and sum rounded once, no fused multiply-add
 __device__ __forceinline__ double dist2_exact(double x, double y, double z) {

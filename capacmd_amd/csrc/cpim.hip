@@ -50,7 +50,7 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
         double rij = sqrt(r2);
         double e_field, v_field;
         if (bc.use_coulomb == 1) {
-            double e2 = exp(-bc.alpha * bc.alpha * r2);
+            double e2 = exp_poly<false>(-bc.alpha * bc.alpha * r2);
             double erfc_a = erfc_from_exp(bc.alpha * rij, e2, s_erfcx);
             e_field = K_FQQ * q * (erfc_a / r2 + bc.a2_sqrtPI * e2 / rij - bc.wolfConst) / rij;
             v_field = K_FQQ * q * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
@@ -191,9 +191,12 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
           double* __restrict__ part, const SolveScalars* __restrict__ guard) {
     main_stream_priority();
     __shared__ double s_w[4][4];
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
         return;
-    const int Nm = bc.Nm;
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    const int Nm = bc.Nm, N = bc.N;
     const int g = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double o[4] = {0.0, 0.0, 0.0, 0.0};
@@ -201,7 +204,10 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
         const int k = g - Nm;
         for (int mi = np_first[k] + threadIdx.x; mi <= np_last[k]; mi += blockDim.x) o[3] += x[3 * Nm + mi];
     } else {
+        // block (i, j) as g_a r r^T + g_b 1, c3 r; -c3 r^T, cq (relay.h) from
+        // the FP64 coefficients of t_coef_fast, applied in that form
         const int i = g;
+        const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
         for (int j = threadIdx.x; j < Nm; j += blockDim.x) {
             const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2], x3 = x[3 * Nm + j];
             if (j == i) {
@@ -211,12 +217,12 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
                 o[3] += -bc.inv_capac * x3;
                 continue;
             }
-            double blk[16];
-            relay_block64(r, bc, i, j, blk);
-            o[0] += blk[0] * x0 + blk[1] * x1 + blk[2] * x2 + blk[3] * x3;
-            o[1] += blk[4] * x0 + blk[5] * x1 + blk[6] * x2 + blk[7] * x3;
-            o[2] += blk[8] * x0 + blk[9] * x1 + blk[10] * x2 + blk[11] * x3;
-            o[3] += blk[12] * x0 + blk[13] * x1 + blk[14] * x2 + blk[15] * x3;
+            double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
+            pbc12(bc, dx, dy, dz);
+            dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+            const double r2 = dist2_exact(dx, dy, dz);
+            const TCoef t = t_coef_fast(bc, r2, s_erfcx);
+            block32_apply(dx, dy, dz, t.c2 - 3.0 * t.c1, t.c1 * r2, t.c3, t.cq, x0, x1, x2, x3, o);
         }
     }
 #pragma unroll
@@ -298,13 +304,14 @@ __global__ void k_dense(const float* __restrict__ relay, int tiles, int Nb, cons
 // DSF charge(q_i)-charge(q_a) and induced dipole(p_i)-charge(q_a).
 __device__ __forceinline__ void mw_pair(const BoxConst& bc, double dx, double dy, double dz, double r2,
                                         double qa, double qm, const double* pm, double f[3],
-                                        double* e10, double* e11) {
+                                        double* e10, double* e11, const double* __restrict__ erfcx) {
     double rij = sqrt(r2);
     double rv[3] = {dx, dy, dz};
     double const_a0, erfc_a = 0.0, e2 = 0.0;
     if (bc.use_coulomb == 1) {
-        erfc_a = erfc(bc.alpha * rij);
-        e2 = exp(-bc.alpha * bc.alpha * r2);
+        // erfc from the Gaussian the term needs anyway (the erfcx table)
+        e2 = exp_poly<false>(-bc.alpha * bc.alpha * r2);
+        erfc_a = erfc_from_exp(bc.alpha * rij, e2, erfcx);
         const_a0 = (erfc_a / r2 + bc.a2_sqrtPI * e2 / rij - bc.wolfConst) / rij;
         *e10 += K_FQQ * qa * qm * (erfc_a / rij - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
     } else {
@@ -344,7 +351,8 @@ __device__ __forceinline__ void mw_metal_terms(const double* __restrict__ r, con
                                                const BoxConst& bc, const int* __restrict__ n_front,
                                                const int* __restrict__ nbr, int stride,
                                                const double* __restrict__ x, int i, int t,
-                                               double& fx, double& fy, double& fz, double vals[13]) {
+                                               double& fx, double& fy, double& fz, double vals[13],
+                                               const double* __restrict__ erfcx) {
     const int N = bc.N, Nm = bc.Nm;
     const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
     const double qm = x[3 * Nm + i];
@@ -361,7 +369,7 @@ __device__ __forceinline__ void mw_metal_terms(const double* __restrict__ r, con
         double r2 = dist2_exact(dx, dy, dz);
         if (!(r2 < bc.rc2)) continue;         // rows reach rc + skin
         double fp[3];
-        mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11);
+        mw_pair(bc, dx, dy, dz, r2, qa, qm, pm, fp, &e10, &e11, erfcx);
         fx += fp[0]; fy += fp[1]; fz += fp[2];
         double rv[3] = {dx, dy, dz};
 #pragma unroll
@@ -398,22 +406,22 @@ __device__ __forceinline__ void mm_terms(const double* __restrict__ r, const Box
         pbc12(bc, dx, dy, dz);
         dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
         double r2 = dist2_exact(dx, dy, dz);
-        double rij = sqrt(r2);
-        double inv_r = 1.0 / rij, inv_r2 = inv_r * inv_r;
+        double inv_r = rsqrt_pos(r2), inv_r2 = inv_r * inv_r;
+        double rij = r2 * inv_r;
         double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
         double qj = x[3 * Nm + j];
         double pj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
         double rv[3] = {dx, dy, dz};
         double g[3] = {0.0, 0.0, 0.0};
         // charge-charge, T0
-        double e_qq = exp(-(r2 * iRqq * iRqq));
+        double e_qq = exp_poly<false>(-(r2 * iRqq * iRqq));
         double erf_qq = erf_from_exp(rij * iRqq, e_qq, erfcx);
         double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * iRqq * rij * e_qq);
         double sqq = A0 * K_FQQ_AU2 * qi * qj;
         g[0] += sqq * dx; g[1] += sqq * dy; g[2] += sqq * dz;
         e12 += 0.5 * K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
         // charge-dipole, T1 both directions
-        double e_pq = exp(-(r2 * iRpq * iRpq));
+        double e_pq = exp_poly<false>(-(r2 * iRpq * iRpq));
         double br_pq = erf_from_exp(rij * iRpq, e_pq, erfcx) - 2.0 * K_INV_SQRT_PI * iRpq * rij * e_pq;
         double Apq0 = inv_r3 * br_pq;
         double A1 = inv_r5 * br_pq;
@@ -428,7 +436,7 @@ __device__ __forceinline__ void mm_terms(const double* __restrict__ r, const Box
         double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
         e13 += 0.5 * K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
         // dipole-dipole, T2
-        double e_pp = exp(-(r2 * iRpp * iRpp));
+        double e_pp = exp_poly<false>(-(r2 * iRpp * iRpp));
         double erf_pp = erf_from_exp(rij * iRpp, e_pp, erfcx);
         double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * iRpp * rij * e_pp);
         double Bpp1 = 4.0 * K_INV_SQRT_PI * iRpp3 * inv_r2 * e_pp;
@@ -471,7 +479,7 @@ __device__ __forceinline__ void mw_atom_terms(const double* __restrict__ r, cons
                                               const BoxConst& bc, const int* __restrict__ n_back,
                                               const int* __restrict__ nbr, int stride,
                                               const double* __restrict__ x, int a, int lane,
-                                              double* __restrict__ f) {
+                                              double* __restrict__ f, const double* __restrict__ erfcx) {
     const int N = bc.N, Nm = bc.Nm;
     double qa = charge[a];
     int nb = n_back[a];
@@ -487,7 +495,7 @@ __device__ __forceinline__ void mw_atom_terms(const double* __restrict__ r, cons
         if (!(r2 < bc.rc2)) continue;             // rows reach rc + skin
         double pm[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
         double fp[3];
-        mw_pair(bc, dx, dy, dz, r2, qa, x[3 * Nm + i], pm, fp, &e10, &e11);
+        mw_pair(bc, dx, dy, dz, r2, qa, x[3 * Nm + i], pm, fp, &e10, &e11, erfcx);
         fx += fp[0]; fy += fp[1]; fz += fp[2];
     }
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
@@ -540,11 +548,14 @@ k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge,
                 const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
     __shared__ double s_red[4][13];
     __shared__ double s_f[4][3];
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
     const int i = blockIdx.x;
     double fx = 0.0, fy = 0.0, fz = 0.0, vals[13];
 #pragma unroll
     for (int c = 0; c < 13; ++c) vals[c] = 0.0;
-    mw_metal_terms(r, charge, bc, n_front, nbr, stride, x, i, threadIdx.x, fx, fy, fz, vals);
+    mw_metal_terms(r, charge, bc, n_front, nbr, stride, x, i, threadIdx.x, fx, fy, fz, vals, s_erfcx);
     block_force_add(fx, fy, fz, s_f, i, bc.N, f);
     block_partials<11>(vals, s_red, partial);
 }
@@ -556,10 +567,13 @@ __global__ void __launch_bounds__(256)
 k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
                const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
                const int* __restrict__ tail_list, const double* __restrict__ x, double* __restrict__ f) {
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
     const int count = tail_list[0];
     const int nw = gridDim.x * 4;
     for (int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)); q < count; q += nw)
-        mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, tail_list[1 + q], threadIdx.x & 63, f);
+        mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, tail_list[1 + q], threadIdx.x & 63, f, s_erfcx);
 }
 
 // workgroups of the atom-side kernel: enough wavefronts to give each listed

@@ -208,15 +208,6 @@ struct PairLJ {
     double c6, c12, lj_fc, lj_fc_rc, lj_ec, pad[3];
 };
 
-// 1/sqrt(s) for a positive finite s: the hardware estimate and one
-// third-order Newton step, y + y e (1/2 + 3/8 e), e = 1 - s y^2 (the
-// library rsqrt's own expansion without its zero and infinity cases)
-__device__ __forceinline__ double rsqrt_pos(double s) {
-    const double y = __builtin_amdgcn_rsq(s);
-    const double e = fma(-s * y, y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
-}
-
 // funct 1 under use_coulomb = use_vdw = 1 (pair_term's expressions), for
 // every pair without a test of the type or the charges; kqi = K_FQQ q_i,
 // na2 = -alpha^2.  lj_row false (the same on every lane): atom i's type has
