@@ -78,7 +78,8 @@ struct Solver {
     // GMRES (the FP32-refined path's inner solver): the basis, the per-block
     // partials of its dot products, the Hessenberg column state
     double* Q;              // (kGmM + 1) n_mat: unnormalised basis vectors q_i
-    double* part_h;         // n_vb (kGmM + 1)
+    double* part_h;         // ph_rows (kGmM + 1)
+    int ph_rows;            // rows of part_h: n_vb, or the fused column's Nb + n_np when more
     GmState* gm;            // device
     const float* bj;        // block-Jacobi inverses (precond.hip), or NULL: the Jacobi K
     // inside the context's readback buffer, so that the synchronisation at
@@ -1148,6 +1149,122 @@ k_gm_dots(int n_mat, int j, const double* __restrict__ w, const double* __restri
             if (i0 + t <= j) part_h[(size_t)blockIdx.x * (kGmM + 1) + i0 + t] = d[t];
 }
 
+// Column j of the block-preconditioned GMRES on the TILES layout: the
+// product's sums launch and the dots launch in one.  Workgroup b < Nb forms
+// row block b of w = A u from the tile partials (as k_spmv_tsum) and then
+// the partial dots (w, q_i), i <= j, over its 256 rows into part_h[b], and
+// copies q_j's rows into qcopy (the update launch reads q_j there);
+// workgroup Nb + k takes the Lagrange row of nanoparticle k (formed here
+// when the tile launch left it to the sums: lag_here) and its dots into
+// part_h[Nb + k]; the last workgroup does what the dots launch's workgroup
+// (0, 0) does besides its dots (|q_j| from part_qq, the rotation of column
+// j - 1, the stop test).  The update launch then reduces Nb + n_np rows of
+// part_h.
+__global__ void __launch_bounds__(kSumThreads)
+k_gm_tsum_dots(int Nb, int Nm, int n_np, int n_mat, int j, const int* __restrict__ np_of,
+               const int* __restrict__ np_first, const int* __restrict__ np_last, const double* __restrict__ P,
+               BoxConst bc, const double* __restrict__ x, int lag_here, double* __restrict__ y,
+               const double* __restrict__ Q, double* __restrict__ qcopy, const double* __restrict__ part_qq,
+               int n_qq, GmState* gm, SolveScalars* sc, double* __restrict__ part_h) {
+    main_stream_priority();
+    __shared__ double s_o[kSumThreads / 64][4][kTileEdge];       // [group][component][row]
+    __shared__ double s_y[4][kTileEdge];                         // the block's outputs
+    __shared__ double s_w[(kSumThreads / 64) * kGmDots];
+    __shared__ GmLds s_gm;
+    if (sc->done) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int b = blockIdx.x;
+    if (b == Nb + n_np) {
+        double pq = 0.0;
+        for (int k = t; k < n_qq; k += blockDim.x) pq += part_qq[k];
+        const double nrm_j = sqrt(block_sum(pq, s_w));
+        if (t == 0) gm->nrm[j] = nrm_j;
+        if (j >= 1) gm_column(gm, sc, j - 1, nrm_j, s_gm);
+        return;
+    }
+    int row = -1;
+    double wr = 0.0;
+    if (b < Nb) {
+        // the row sums over the source blocks, as tile_block_out
+        const int G = kSumThreads >> 6, g = wave;
+        const double* src = P + (size_t)b * Nb * (4 * kTileEdge) + lane;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q0 = g; q0 < Nb; q0 += G * kSumBatch) {
+            double v[kSumBatch][4];
+#pragma unroll
+            for (int u = 0; u < kSumBatch; ++u) {
+                const int q = q0 + G * u;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    v[u][c] = q < Nb ? src[(size_t)q * (4 * kTileEdge) + c * kTileEdge] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kSumBatch; ++u)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] += v[u][c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s_o[g][c][lane] = acc[c];
+        __syncthreads();
+        const int ia = b * kTileEdge + lane;
+        if (t < kTileEdge) {
+            double o[4] = {0.0, 0.0, 0.0, 0.0};
+            if (ia < Nm) {
+                for (int h = 0; h < G; ++h)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) o[c] += s_o[h][c][t];
+                RelayG32 m{NULL, 0, NULL, NULL, bc};
+                spmv_group_out(m, ia, Nm, np_of, x, y, 0, NULL, NULL, NULL, o);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_y[c][t] = o[c];
+        }
+        __syncthreads();
+        if (ia < Nm) {
+            const int c = wave;
+            row = c < 3 ? 3 * ia + c : 3 * Nm + ia;
+            wr = s_y[c][lane];
+        }
+    } else {
+        // Lagrange row of nanoparticle k
+        const int k = b - Nb;
+        if (lag_here) {
+            double v = 0.0;
+            for (int mi = np_first[k] + t; mi <= np_last[k]; mi += kSumThreads) v += x[3 * Nm + mi];
+            v = block_sum(v, s_w);
+            __syncthreads();                 // s_w is reused below
+            if (t == 0) y[4 * Nm + k] = v;
+            wr = v;
+        } else {
+            wr = y[4 * Nm + k];
+        }
+        if (t == 0) row = 4 * Nm + k;
+    }
+    // the partial dots, kGmDots basis vectors per round
+    for (int i0 = 0; i0 <= j; i0 += kGmDots) {
+        double d[kGmDots];
+#pragma unroll
+        for (int u = 0; u < kGmDots; ++u) {
+            const int i = i0 + u;
+            const double qv = row >= 0 && i <= j ? Q[(size_t)i * n_mat + row] : 0.0;
+            if (i == j && row >= 0) qcopy[row] = qv;
+            d[u] = wr * qv;
+        }
+#pragma unroll
+        for (int u = 0; u < kGmDots; ++u) d[u] = wave_sum(d[u]);
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < kGmDots; ++u) s_w[wave * kGmDots + u] = d[u];
+        __syncthreads();
+        if (t < kGmDots && i0 + t <= j) {
+            double v = 0.0;
+            for (int w = 0; w < kSumThreads / 64; ++w) v += s_w[w * kGmDots + t];
+            part_h[(size_t)b * (kGmM + 1) + i0 + t] = v;
+        }
+        __syncthreads();                     // s_w is reused by the next round
+    }
+}
+
 // column j's Gram-Schmidt step: w = w / |q_j| and v_j = q_j / |q_j| in place,
 // h_i from the partials (scaled by 1 / |q_j|, and 1 / |q_j|^2 for i = j),
 // q_{j+1} = w - sum h_i v_i, u = K q_{j+1}, partials of |q_{j+1}|^2; block 0
@@ -1434,7 +1551,8 @@ int solver_create(capac_ctx* c) {
     if (c->precision == CAPAC_PREC_FP32_REFINED) {
         CHECK_HIP(hipMalloc(&S->Q, (size_t)(kGmM + 1) * n * sizeof(double)));
         CHECK_HIP(hipMemset(S->Q, 0, (size_t)(kGmM + 1) * n * sizeof(double)));
-        CHECK_HIP(hipMalloc(&S->part_h, nvb * (kGmM + 1) * sizeof(double)));
+        S->ph_rows = (int)std::max(nvb, (size_t)tile_blocks(S->Nm) + (size_t)S->n_np);
+        CHECK_HIP(hipMalloc(&S->part_h, (size_t)S->ph_rows * (kGmM + 1) * sizeof(double)));
         CHECK_HIP(hipMalloc(&S->gm, sizeof(GmState)));
         CHECK_HIP(hipMemset(S->gm, 0, sizeof(GmState)));
     }
@@ -1513,6 +1631,23 @@ static void spmv(capac_ctx* c, const Solver* S, const double* x, double* y, int 
         RelayM64 m{c->r, c->bc};
         spmv_t<RelayM64, 1>(c, S, m, x, y, mode, part, sc);
     }
+}
+
+// the TILES product's tile launch alone (the fused GMRES column forms the
+// sums itself); returns the input the sums take (x, or the corrected input
+// with a coarse correction) and whether the sums form the Lagrange rows
+static const double* spmv_tiles_launch(capac_ctx* c, const Solver* S, const double* x, double* y,
+                                       const SolveScalars* sc, bool coarse, int* lag_in_sums) {
+    const int nb = c->relay_nb, nt = (int)tile_count(nb);
+    CoarseArgs cg{NULL, NULL, NULL, NULL, 0};
+    if (coarse && c->cg_n) cg = CoarseArgs{c->cg_A, c->cg_s, reinterpret_cast<const float4*>(c->cg_w), S->t,
+                                           c->cg_n};
+    const int lag_t = cg.nc ? 0 : S->n_np;
+    k_spmv_tiles<<<nt + lag_t, kTileThreads, 0, c->stream>>>(
+        reinterpret_cast<const float4*>(c->relay32), nb, nt, S->Nm, S->n_np, c->np_of_metal, c->np_first,
+        c->np_last, c->r, c->metal_u, c->bc, x, y, c->relay_part, c->relay_cnt, 0, S->r0, S->K, NULL, sc, cg);
+    *lag_in_sums = cg.nc ? 1 : 0;
+    return cg.nc ? cg.x2 : x;
 }
 
 struct SolveOut {
@@ -1604,6 +1739,11 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
     const bool cg = bjf && c->cg_n > 0;
     const bool cg_fused = cg && c->relay_layout == CAPAC_LAYOUT_TILES && !CAPAC_TILE_FUSED;
     const int ug = bj_groups(tile_blocks(S->Nm)) + 1;
+    // block preconditioner on the TILES layout: a column is three launches
+    // (tiles, sums with the dots, update) instead of four
+    const int nbk = tile_blocks(S->Nm);
+    const bool fused = bjf && c->relay_layout == CAPAC_LAYOUT_TILES && !CAPAC_TILE_FUSED &&
+                       nbk + S->n_np <= S->ph_rows;
     for (;;) {
         spmv(c, S, x, S->Ax, 0, NULL, NULL);
         k_gm_init<<<vb, kVecThreads, 0, st>>>(n, n4, b, S->Ax, c->diag, S->Q, S->p, S->K, S->part2, S->part4);
@@ -1617,16 +1757,27 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
         int j = 0, batch = first_batch > 1 ? first_batch : 2, next = 2;
         for (;;) {
             for (int u = 0; u < batch && j <= kGmM; ++u, ++j) {
-                if (j < kGmM) spmv(c, S, S->p, S->v, 0, NULL, S->sc, cg_fused);
-                const dim3 grid(j < kGmM ? vb : 1, j < kGmM ? (j + kGmDots) / kGmDots : 1);
                 // |q_j|^2 partials: the start's (vb) for q_0, the update's after
                 const int nqq = j == 0 || !bjf ? vb : ug;
-                k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, nqq, S->gm, S->sc, S->part_h,
-                                                        bjf ? S->z : NULL);
+                int nph = vb;               // rows of part_h the update reduces
+                if (j < kGmM && fused) {
+                    // the product's tile launch, then its sums and the dots in one launch
+                    int lag_here = 0;
+                    const double* xin = spmv_tiles_launch(c, S, S->p, S->v, S->sc, cg_fused, &lag_here);
+                    k_gm_tsum_dots<<<nbk + S->n_np + 1, kSumThreads, 0, st>>>(
+                        nbk, S->Nm, S->n_np, n, j, c->np_of_metal, c->np_first, c->np_last, c->relay_part, c->bc,
+                        xin, lag_here, S->v, S->Q, S->z, S->part4, nqq, S->gm, S->sc, S->part_h);
+                    nph = nbk + S->n_np;
+                } else {
+                    if (j < kGmM) spmv(c, S, S->p, S->v, 0, NULL, S->sc, cg_fused);
+                    const dim3 grid(j < kGmM ? vb : 1, j < kGmM ? (j + kGmDots) / kGmDots : 1);
+                    k_gm_dots<<<grid, kVecThreads, 0, st>>>(n, j, S->v, S->Q, S->part4, nqq, S->gm, S->sc,
+                                                            S->part_h, bjf ? S->z : NULL);
+                }
                 if (j < kGmM) {
                     if (bjf) {
                         k_gm_update_bj<<<ug, kVecThreads, 0, st>>>(
-                            n, S->Nm, tile_blocks(S->Nm), j, S->v, S->z, S->Q, S->part_h, vb, S->gm, S->sc, S->p,
+                            n, S->Nm, tile_blocks(S->Nm), j, S->v, S->z, S->Q, S->part_h, nph, S->gm, S->sc, S->p,
                             S->part4, S->bj, reinterpret_cast<const float4*>(c->cg_w), cg ? c->cg_s : NULL);
                         if (cg && !cg_fused) cg_apply(c, S->p, S->sc, 1);
                     } else {
@@ -1835,6 +1986,7 @@ int solver_run_noalloc(capac_ctx* c, const double* b, double* x) {
     V.res = p; p += 4;
     V.Q = p; p += (size_t)(kGmM + 1) * kNoAllocMaxUnknowns;
     V.part_h = p; p += kNoAllocVecBlocks * (kGmM + 1);
+    V.ph_rows = (int)kNoAllocVecBlocks;
     V.gm = (GmState*)p; p += kNoAllocGmWords;
     V.sc_host = own->sc_host;
     V.res_host = own->res_host;
