@@ -162,11 +162,13 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, k
     # the timed steps run as the product runs them (no per-phase timers, so
     # unobserved steps end without a host synchronisation)
     builds0 = ctx.nb_builds()
+    syncs0 = ctx.solve_syncs()
     t0 = time.perf_counter()
     done = ctx.md_run(step, steps, n_save)
     t1 = time.perf_counter()
     assert done == steps, done
     builds = ctx.nb_builds() - builds0
+    syncs = ctx.solve_syncs() - syncs0
     step += done
     t_end = ctx.observables()["temp"]
     # the phase breakdown from as many further steps with the library's
@@ -191,6 +193,7 @@ def run_gpu(deck, precision: int, steps: int, warmup: int, eq, predictor=None, k
         "integrator_ms_per_step": 1e3 * t[12] / steps,
         "solve_iters_per_step": iters / steps,
         "nb_builds_per_step": builds / steps,
+        "solve_syncs_per_step": syncs / steps,
         "nb_skin_nm": ctx.skin,
         "warm_start_predictor": ctx.predictor,
         "fp32_layout": {1: "rows", 2: "tiles"}.get(ctx.fp32_layout),

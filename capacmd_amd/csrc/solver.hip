@@ -1706,6 +1706,7 @@ static int bicgstab(capac_ctx* c, Solver* S, const double* b, double* x, int fir
             CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
         }
         CHECK_HIP(hipStreamSynchronize(st));
+        ++c->timing.solve_syncs;
         if (S->sc_host->done) break;
         if (launched > 1010) {
             capac_set_error("BiCGSTAB did not report completion after %d iterations", launched);
@@ -1806,6 +1807,7 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
                 CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, st));
             }
             CHECK_HIP(hipStreamSynchronize(st));
+            ++c->timing.solve_syncs;
             if (S->sc_host->done) break;
             if (j > kGmM || launched > 1100) {
                 capac_set_error("GMRES did not report completion after %d columns", launched);
@@ -1838,6 +1840,7 @@ static int residual64(capac_ctx* c, Solver* S, const double* b, const double* x,
     CHECK_HIP(hipGetLastError());
     CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
+    ++c->timing.solve_syncs;
     *rTr = S->res_host[0];
     return 0;
 }
@@ -1857,15 +1860,17 @@ static int solver_run_with(capac_ctx* c, Solver* S, const double* b, double* x) 
     // first batch: from the previous solves' iteration counts (warm starts
     // change slowly), so few launches are wasted as no-ops; the main solves'
     // counts alone (a refinement's correction solve has its own count, and
-    // adding it would launch that many no-op units).  In MD
-    // with the predictor the count swings from step to step (12, 18, 12, 18,
-    // ... on Au561), so the mean of the last two counts, plus one, sizes the
-    // batch: on a recorded 117-step sequence that left 1.7 no-op units and
-    // 0.4 extra synchronisations per solve, against 2.6 and 0.55 for the last
-    // count plus one
+    // adding it would launch that many no-op units).  In MD with the
+    // predictor the count swings from step to step, so the larger of the last
+    // two counts, plus one, sizes the batch: a no-op column costs three short
+    // launches, a batch that ends short a synchronisation and the batch-end
+    // launches.  Au2057 FP32-refined, 300 MD steps (ms per step /
+    // synchronisations per solve): mean of the last two plus one 1.231 /
+    // 1.60, plus two 1.228 / 1.35, plus three 1.225 / 1.14, larger plus one
+    // 1.223 / 1.18.
     int first = 2;
     if (c->batch_main > 1)
-        first = (c->batch_main2 > 1 ? (c->batch_main + c->batch_main2) / 2 : c->batch_main) + 1;
+        first = (c->batch_main2 > c->batch_main ? c->batch_main2 : c->batch_main) + 1;
     const bool refine = c->precision == CAPAC_PREC_FP32_REFINED;
     // (the block-aligned update takes the Lagrange rows in one workgroup)
     S->bj = refine && c->bj_inv && c->bj_valid && S->n_np <= kVecThreads ? c->bj_inv : NULL;

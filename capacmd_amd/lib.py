@@ -61,7 +61,8 @@ class CapacParams(C.Structure):
 
 class CapacTiming(C.Structure):
     _fields_ = [("t", C.c_double * NPOT), ("n_force_calls", C.c_longlong),
-                ("solve_iters", C.c_longlong), ("nb_builds", C.c_longlong)]
+                ("solve_iters", C.c_longlong), ("nb_builds", C.c_longlong),
+                ("solve_syncs", C.c_longlong)]
 
 
 def dptr(a: Optional[np.ndarray]):
@@ -567,6 +568,12 @@ class Context:
         t = CapacTiming()
         _check(self.lib.capac_get_timing(self.h, C.byref(t)), "capac_get_timing")
         return np.array(t.t[:]), t.n_force_calls, t.solve_iters
+
+    def solve_syncs(self) -> int:
+        """Host synchronisations inside the solves since the last reset_timing."""
+        t = CapacTiming()
+        _check(self.lib.capac_get_timing(self.h, C.byref(t)), "capac_get_timing")
+        return int(t.solve_syncs)
 
     def nb_builds(self) -> int:
         """Neighbour-row builds since the last reset_timing."""

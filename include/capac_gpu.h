@@ -179,6 +179,7 @@ typedef struct capac_timing {
     long long n_force_calls;
     long long solve_iters;      /* BiCGSTAB iterations, summed               */
     long long nb_builds;        /* neighbour-row builds (see capac_set_skin) */
+    long long solve_syncs;      /* host synchronisations inside the solves   */
 } capac_timing;
 
 typedef struct capac_ctx capac_ctx;

@@ -54,7 +54,7 @@ def test_struct_layouts():
     h = lib.host_lib()
     assert h.capac_params_size() == C.sizeof(lib.CapacParams)
     # capac_timing: 15 doubles + 3 long long
-    assert C.sizeof(lib.CapacTiming) == 15 * 8 + 3 * 8
+    assert C.sizeof(lib.CapacTiming) == 15 * 8 + 4 * 8
 
 
 def copy_deck(golden_dir, name, dst):
