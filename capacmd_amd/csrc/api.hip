@@ -836,9 +836,14 @@ static void mark(capac_ctx* c, int k) {
 // compute unit with the solve's launches instead of holding their slots.
 // Au2057 FP32-refined with the quadratic predictor, ms per MD step on one
 // box: 2.53 at 256 workgroups, 2.22 at 384, 2.01 at 512, 2.10 at 640, 2.23
-// at 768, 2.15 at 1024, 2.23 at 1536, 2.20 with the full grid (2048).
+// at 768, 2.15 at 1024, 2.23 at 1536, 2.20 with the full grid (2048).  With
+// the water pair kernel (two launches, the first at most 132 VGPRs) and the
+// three-launch GMRES column: 1.300 at 384, 1.236 at 416, 1.242 at 432,
+// 1.199-1.205 at 448, 1.214 at 456, 1.256 at 464, 1.288 at 480, 1.223-1.227
+// at 512, 1.317 at 640 (the pair launches then end 0.03-0.05 ms after the
+// solve at 448, and the solve takes 0.75 ms per step against 0.81 at 512).
 #ifndef CAPAC_SIDE_BLOCKS_LARGE
-#define CAPAC_SIDE_BLOCKS_LARGE 512
+#define CAPAC_SIDE_BLOCKS_LARGE 448
 #endif
 static int side_pair_blocks(const capac_ctx* c) {
     if (c->bc.Nm >= 1024) return CAPAC_SIDE_BLOCKS_LARGE;
