@@ -10,7 +10,8 @@ SPC/E water at 1.0 g/cm^3 (the seeded config-3 deck of capacmd_amd.gen:
 electrostatics (alpha 2 nm^-1, rc 1.2 nm), CPIM on the gold, Nose-Hoover NVT
 at 300 K, dt = 1 fs.  A "step" is one full MD step on the device: NHC,
 velocity Verlet + RATTLE, neighbour build, pair/Sutton-Chen/CPIM forces with
-the BiCGSTAB solve, the molecule wrap at the save cadence.
+the CPIM solve (GMRES on the FP32 relay with FP64 refinement; the reference's
+BiCGSTAB on the FP64 path), the molecule wrap at the save cadence.
 
 Before anything is timed the deck is equilibrated for E MD steps
 (FP32-refined path) so that the timed window runs near the thermostat's
@@ -24,8 +25,11 @@ work has finished, so wall time around the K steps is device time.
 ns/day = 0.0864 / (seconds per step) at dt = 1 fs.
 
 Prints one JSON line (there is one GPU).  The roofline object is for the
-relay SpMV, the kernel that takes the largest share of the MD step's kernel
-time, timed live with HIP events on the library's stream.  cpu_baseline
+relay SpMV, the HBM stream of the solve, timed live with HIP events on the
+library's stream.  pair_valu gives the pair launches, which take the largest
+share of the MD step's kernel time and are bound by FP64 VALU issue: row
+entries and VALU instructions per second against the FP64 issue peak.
+cpu_baseline
 times the oracle restatement (OpenMP over the reference's decomposition
 units, every core the process may use) on two MD steps from the same
 equilibrated state and warm start.
@@ -95,6 +99,40 @@ def traffic_from_profiles(kernel: str, algorithmic_bytes: float):
         return entry.get("bytes_per_launch")
     except Exception:
         return None
+
+
+FP64_ISSUE_PEAK_G = 256 * 4 * 2.4 / 4        # wavefront FP64 VALU instructions per ns: 614.4 G/s
+
+
+def pair_valu_roofline(path, deck):
+    """The pair launches' FP64-VALU figures (they are VALU-bound, not
+    HBM-bound): row entries per second and VALU instructions per second
+    against the FP64 issue peak (one wavefront FP64 instruction per SIMD every
+    4 cycles), the instruction count per launch from the committed rocprofv3
+    summary of the same workload (None for another workload), and the HBM
+    rate on the algorithmic bytes 4 P_list + 60 N."""
+    ms = path.get("kernel_ms", {}).get("pair")
+    entries = path.get("pair_row_entries")
+    if not ms or not entries:
+        return None
+    out = {"kernels": "one force call's pair launches, alone on the full grid (k_pack_xyzq, k_pair_water, "
+                      "k_pair_rest on water decks)",
+           "bound": "fp64-valu", "ms_per_launch": ms, "row_entries": entries,
+           "G_entries_per_s": round(entries / (ms * 1e-3) / 1e9, 1),
+           "algorithmic_bytes": 4 * entries + 60 * deck.n_atoms,
+           "hbm_GBps_algorithmic": round((4 * entries + 60 * deck.n_atoms) / (ms * 1e-3) / 1e9, 1),
+           "peak_G_instr_per_s": FP64_ISSUE_PEAK_G}
+    try:
+        with open(os.path.join(PROFILES, "valu.json")) as fh:
+            v = json.load(fh)
+        if int(v.get("atoms", -1)) == deck.n_atoms and "k_pair_water" in v and "k_pair_rest" in v:
+            instr = v["k_pair_water"]["valu_instr"] + v["k_pair_rest"]["valu_instr"]
+            out["valu_instr_per_launch"] = instr
+            out["achieved_G_instr_per_s"] = round(instr / (ms * 1e-3) / 1e9, 1)
+            out["frac"] = round(out["achieved_G_instr_per_s"] / FP64_ISSUE_PEAK_G, 4)
+    except (OSError, ValueError, KeyError):
+        pass
+    return out
 
 
 def cpu_info():
@@ -253,8 +291,12 @@ def error_vs_oracle(deck, eq, steps: int = 20, crits=(1e-8, 1e-14)):
     nm, n = deck.n_metal, deck.n_atoms
     out = {"note": "one force call from an identical state and warm start; errors are relative "
                    "2-norms against the FP64 oracle (OpenMP build, every pair: the corrected "
-                   "pair count); each entry holds all solves at the stop test rTr < crit bTb"}
-    for key, crit in zip(("reference_tolerance", "tight_tolerance"), crits):
+                   "pair count); each entry holds all solves at the stop test rTr < crit bTb; "
+                   "at the reference's stop test, *_vs_converged compare with the oracle's "
+                   "tight-tolerance solution (the converged answer), for the GPU paths and for "
+                   "the oracle's own reference-tolerance solve"}
+    converged = None
+    for key, crit in sorted(zip(("reference_tolerance", "tight_tolerance"), crits), key=lambda kc: kc[1]):
         o = Oracle(deck, omp=True)
         o.set_solve_crit(crit)
         o.set_state(r, v, x_prev)
@@ -267,6 +309,17 @@ def error_vs_oracle(deck, eq, steps: int = 20, crits=(1e-8, 1e-14)):
         eo = float(np.sum(po[1:]))
         Fo = fo.reshape(3, n)
         res = {"solve_crit": crit, "oracle_iterations": it_o, "oracle_seconds": round(t_or, 2)}
+
+        def vs_converged(f, pot):
+            if converged is None:
+                return {}
+            fc, ec = converged
+            Fc = fc.reshape(3, n)
+            F_ = f.reshape(3, n)
+            return {"f_vs_converged": float("%.3e" % rel(f, fc)),
+                    "f_metal_vs_converged": float("%.3e" % rel(F_[:, :nm], Fc[:, :nm])),
+                    "epot_vs_converged": float("%.3e" % (abs(float(np.sum(pot[1:])) - ec) / max(abs(ec), 1e-300)))}
+        res["oracle"] = vs_converged(fo, po)
         for name, prec in (("fp32_refined", lib.PREC_FP32_REFINED), ("fp64", lib.PREC_FP64)):
             c = lib.Context(deck, precision=prec)
             try:
@@ -284,6 +337,10 @@ def error_vs_oracle(deck, eq, steps: int = 20, crits=(1e-8, 1e-14)):
                          "f_metal": float("%.3e" % rel(F[:, :nm], Fo[:, :nm])),
                          "epot": float("%.3e" % (abs(float(np.sum(pot[1:])) - eo) / max(abs(eo), 1e-300))),
                          "iterations": it}
+            res[name].update(vs_converged(f, pot))
+        if converged is None:
+            converged = (fo.copy(), eo)
+            del res["oracle"]
         out[key] = res
     return out
 
@@ -378,8 +435,9 @@ def main() -> int:
     nbytes = spmv_bytes(nm, n_mat, w)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
     layout = main_path.get("fp32_layout")
-    roofline = {"kernel": "relay SpMV (" + ("FP64 matrix-free" if best == "fp64" else f"FP32 {layout} layout")
-                          + " relay, one BiCGSTAB half-iteration)",
+    roofline = {"kernel": "relay SpMV (" + ("FP64 matrix-free relay, one BiCGSTAB half-iteration"
+                                             if best == "fp64" else f"FP32 {layout} layout relay, one GMRES column")
+                          + "'s product)",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles("spmv_" + best + (f"_{layout}" if best == "fp32" else ""), nbytes),
@@ -399,6 +457,7 @@ def main() -> int:
         if "kernel_ms" in p:
             b = spmv_bytes(nm, n_mat, 8 if name == "fp64" else 4)
             p["spmv_GBps"] = round(b / (p["kernel_ms"]["spmv"] * 1e-3) / 1e9, 1)
+    pair_valu = pair_valu_roofline(main_path, deck)
     speedup = None
     if "fp32" in paths and "fp64" in paths:
         # the solve alone (the reference's timer category 7), from the last MD
@@ -448,6 +507,7 @@ def main() -> int:
         "error_vs_fp64_oracle": err,
         "paths": paths,
         "roofline": roofline,
+        "pair_valu": pair_valu,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
