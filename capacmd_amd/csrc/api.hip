@@ -365,6 +365,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     if (!c) { capac_set_error("out of host memory"); return CAPAC_ERR_ARG; }
     c->device = device;
     c->precision = precision;
+    c->need_energy = 1;
     const int N = p->n_atoms, Nm = p->n_metal;
     BoxConst& b = c->bc;
     b.N = N; b.Nm = Nm; b.n_np = Nm > 0 ? p->n_nps : 0; b.n_types = p->n_types; b.n_mols = p->n_mols;
@@ -429,7 +430,8 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         if (rc) { capac_destroy(c); return rc; }
         // the water pair kernel: every pair of non-metal atom types funct 1,
         // damped-shifted-force Coulomb and shifted-force LJ, alpha rc < 6
-        // (erfc from the erfcx table over the whole cut-off)
+        // (erfc from the erfcx table over the whole cut-off); with the metal
+        // types' pairs funct 1 too, it takes every row (pair_water = 2)
         char* used = (char*)calloc(nt > 0 ? nt : 1, 1);
         for (int i = p->n_metal; i < N; ++i)
             if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
@@ -438,6 +440,20 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         for (int a = 0; a < nt && water; ++a)
             for (int b2 = 0; b2 < nt; ++b2)
                 if (used[a] && used[b2] && p->nb_funct[a * nt + b2] != 1) water = 0;
+        if (water) {
+            for (int i = 0; i < p->n_metal; ++i)
+                if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
+            int all = 1;
+            for (int a = 0; a < nt && all; ++a)
+                for (int b2 = 0; b2 < nt; ++b2)
+                    if (used[a] && used[b2] && p->nb_funct[a * nt + b2] != 1) all = 0;
+            if (all) water = 2;
+            else                        // the LJ bits below over the non-metal partners only
+                for (int i = 0; i < p->n_metal; ++i)
+                    if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 0;
+            for (int i = p->n_metal; i < N; ++i)
+                if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
+        }
         int lj_types = 0;
         for (int a = 0; a < nt && a < 31; ++a)
             for (int b2 = 0; b2 < nt; ++b2)
@@ -1311,6 +1327,11 @@ extern "C" int capac_md_begin(capac_ctx* c) {
 extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_save, int stop_at_save) {
     if (!c) return CAPAC_ERR_ARG;
     int done = 0;
+    // every other entry point forms the energies
+    struct EnergyReset {
+        capac_ctx* c;
+        ~EnergyReset() { c->need_energy = 1; }
+    } energy_reset{c};
     for (int step = first_step; step < first_step + n_steps; ++step) {
         double ref_temp = step < c->n_heating
                               ? c->first_temp + (c->ext_temp - c->first_temp) * step / c->n_heating
@@ -1323,6 +1344,14 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (!rc) rc = md_rattle1(c);
         if (rc) return rc;
         solver_predict(c);
+        // A step that ends without a synchronisation (below) has its
+        // energies and virial read by nobody: the reference prints and writes
+        // them at save steps only (mpi-main.c:987-1000), and the integrator
+        // uses forces and kinetic energies alone.  Its pair launch forms
+        // forces only.
+        const bool save = n_save > 0 && step % n_save == 0;
+        const bool last = step + 1 == first_step + n_steps;
+        c->need_energy = (save || last || c->timing_on || !c->n_mat) ? 1 : 0;
         rc = force_compute(c, true);
         if (c->n_mat) {
             // the solve's synchronisation brought the RATTLE flags back (this
@@ -1342,7 +1371,6 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (!rc) rc = md_kick(c);
         if (!rc) rc = md_rattle2(c);
         if (!rc) rc = md_second_half(c, kT);
-        int save = n_save > 0 && step % n_save == 0;
         if (!rc && save) rc = md_wrap(c);
         if (!rc && save) rc = md_fmax(c);
         mark(c, 10);
@@ -1353,8 +1381,7 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         // timing (read per step), and a force call that synchronised in its
         // solve, which also reports the RATTLE flags one step later at most.
         // The potential energies of such a step are not brought back.
-        const bool last = step + 1 == first_step + n_steps;
-        if (!save && !last && !c->timing_on && c->n_mat) {
+        if (!c->need_energy) {
             ++done;
             continue;
         }

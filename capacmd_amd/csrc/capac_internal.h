@@ -113,8 +113,9 @@ struct capac_ctx {
     unsigned* excl;
     PairParam* pair;
     int pair_metal_inert;       // metal-metal pairs add nothing (uncharged metal, funct 1 with c6 = c12 = 0)
-    int pair_lj_types;          // bit t: type t has funct-1 LJ (c6 or c12 nonzero) with a non-metal type
-    int pair_water;             // the water pair kernel applies (pair.hip); then xyzq holds
+    int pair_lj_types;          // bit t: type t has funct-1 LJ (c6 or c12 nonzero) with a type the water kernel meets
+    int pair_water;             // the water pair kernel applies (pair.hip): 1 to the non-metal atoms' front
+                                // segments, 2 to every row (all pair types funct 1); then xyzq holds
     double* xyzq;               // its packed (x, y, z, q) copy, 4 doubles per atom
     // molecules / constraints / nanoparticles
     int *mol_first, *mol_last, *mol_cstr, *cstr_i, *cstr_j;
@@ -210,6 +211,9 @@ struct capac_ctx {
 
     // results of the last force call
     double pot[CAPAC_NPOT], vir[9];
+    // 0: the force call may leave the pair energies and virial unformed (an
+    // MD step whose energies nobody reads); 1 otherwise (the default)
+    int need_energy;
     double pot_dev_ofs;         // unused
     // timing
     int timing_on;

@@ -23,7 +23,13 @@
 // from a polynomial whose coefficients sit in scalar registers, and positions
 // and charges are read as two 16-byte loads from a packed (x, y, z, q) copy.
 // Metal atoms' rows and the back segments (metal neighbours) keep the general
-// term.  Results differ from k_pair's by rounding only.
+// term, in a second launch, unless every pair type of the deck (the metal's
+// included) is funct 1: then the water kernel takes every row.  Results
+// differ from k_pair's by rounding only.
+//
+// An MD step whose energies nobody reads (no save step, no synchronisation)
+// runs the water kernel without its energies and virial: the forces are the
+// same bits, and the reduction launch is left out.
 #include "capac_internal.h"
 #include "erfc_fast.h"
 #include "exp_poly.h"
@@ -212,6 +218,7 @@ struct PairLJ {
 // every pair without a test of the type or the charges; kqi = K_FQQ q_i,
 // na2 = -alpha^2.  lj_row false (the same on every lane): atom i's type has
 // no LJ with any non-metal type, so the LJ terms (exact zeros) are left out
+template <bool Energy>
 __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, const PairLJ& lj, double kqi, double qj,
                                            double na2, double dx, double dy, double dz, double r2, PairOut& o,
                                            const double* __restrict__ erfcx_coef) {
@@ -225,7 +232,7 @@ __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, cons
         const double cr12 = lj.c12 * (inv_r6 * inv_r6), cr6 = lj.c6 * inv_r6;
         const double f_r = 12.0 * cr12 - 6.0 * cr6;
         flj = (f_r - lj.lj_fc) * inv_r2;
-        o.e8 += (cr12 - cr6) + (rij - bc.rc) * lj.lj_fc_rc - lj.lj_ec;
+        if (Energy) o.e8 += (cr12 - cr6) + (rij - bc.rc) * lj.lj_fc_rc - lj.lj_ec;
     }
     // damped shifted force (force.c:787-795); alpha r < 6, so erfc comes from
     // the erfcx table on its whole range: erfc = exp(-x^2) erfcx(x)
@@ -240,34 +247,41 @@ __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, cons
     for (int m = kErfcxTerms - 2; m >= 0; --m) p = fma(p, t, c[m]);
     const double erfc_a = ex * p;
     const double fqq = qq * ((erfc_a * inv_r2 + bc.a2_sqrtPI * ex * inv_r) - bc.wolfConst) * inv_r;
-    o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
+    if (Energy) o.e7 += qq * (erfc_a * inv_r - bc.erfc_arCut + bc.wolfConst * (rij - bc.rc));
     const double fs = flj + fqq;
-    const double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
-    o.fx -= fjx; o.fy -= fjy; o.fz -= fjz;
-    o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
-    o.v[3] -= dy * fjy; o.v[4] -= dy * fjz; o.v[5] -= dz * fjz;
+    // explicit FMAs: the force takes the same instructions with or without
+    // the energies beside it
+    o.fx = fma(-fs, dx, o.fx); o.fy = fma(-fs, dy, o.fy); o.fz = fma(-fs, dz, o.fz);
+    if (Energy) {
+        const double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
+        o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
+        o.v[3] -= dy * fjy; o.v[4] -= dy * fjz; o.v[5] -= dz * fjz;
+    }
 }
 
-// the front segment of non-metal atom i (cnt entries from seg) with the water
-// term.  Software pipeline three rounds deep: while round s is evaluated, the
+// atom i's row with the water term: the nf entries of its front segment
+// (row), then cnt - nf of its back segment (back).  Software pipeline three rounds deep: while round s is evaluated, the
 // positions and charges of rounds s + 1 and s + 2 and the neighbour index of
 // round s + 3 are in flight (beside the solve the kernel has two wavefronts
 // per SIMD, too few to hide a gather's latency behind one round of work:
 // with two rounds in flight the Au2057 launch beside the solve took 0.79 ms
 // per MD step, with three 0.69 ms).  Unrolled three times, so that each
 // round's data keep their registers.
+template <bool Energy>
 __device__ __forceinline__ void water_segment(bool lj_row, const double2* __restrict__ xyzq,
-                                              const int* __restrict__ type, const int* __restrict__ seg, int cnt,
+                                              const int* __restrict__ type, const int* __restrict__ row, int nf,
+                                              const int* __restrict__ back, int cnt,
                                               int i, int lane, const BoxConst& bc, const PairLJ* s_lj, int ti,
                                               double xi, double yi, double zi, double kqi, double na2, PairOut& o,
                                               const double* s_erfcx) {
-    auto nbr_at = [&](int kk) { return kk < cnt ? seg[kk] : i; };
+    // past the row: the atom itself (a valid address for the look-ahead loads)
+    auto nbr_at = [&](int kk) { return kk < nf ? row[kk] : (kk < cnt ? back[kk - nf] : i); };
     auto round = [&](const double2& a, const double2& b, int t) {
         double dx = a.x - xi, dy = a.y - yi, dz = b.x - zi;
         pbc12(bc, dx, dy, dz);
         const double r2 = dist2_exact(dx, dy, dz);
         if (r2 < bc.rc2)                   // rows reach rc + skin
-            water_term(lj_row, bc, s_lj[ti + t], kqi, b.y, na2, dx, dy, dz, r2, o, s_erfcx);
+            water_term<Energy>(lj_row, bc, s_lj[ti + t], kqi, b.y, na2, dx, dy, dz, r2, o, s_erfcx);
     };
     constexpr int W = CAPAC_WAVE;
     int k = lane;
@@ -319,12 +333,18 @@ __device__ __forceinline__ void pair_block_partials(const PairOut& o, double (*s
     }
 }
 
-// Water kernel, first of two launches: the front segments of the non-metal
-// atoms, one wavefront per atom; writes (or adds to) their forces
+// Water kernel, one wavefront per atom; writes (or adds to) the forces of
+// the atoms it takes.  all_rows = 0: the front segments of the non-metal
+// atoms, the first of two launches.  all_rows = 1 (every pair type funct 1):
+// the whole rows of all atoms, the metal atoms' back segments (their metal
+// neighbours) left out when metal_inert says they add exact zeros.  Energy =
+// false: forces only (no energies, virial or partial sums).
+template <bool Energy>
 __global__ void __launch_bounds__(256, 3)
 k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, const PairParam* __restrict__ table,
-             BoxConst bc, const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
-             double* __restrict__ f, int accumulate, int field, double* __restrict__ partial, int lj_types) {
+             BoxConst bc, const int* __restrict__ n_front, const int* __restrict__ n_back,
+             const int* __restrict__ nbr, int stride, double* __restrict__ f, int accumulate, int field,
+             double* __restrict__ partial, int lj_types, int all_rows, int metal_inert) {
     __shared__ PairLJ s_lj[64];
     __shared__ double s_red[kWavesPerBlock][12];
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
@@ -337,7 +357,8 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
     __syncthreads();
     const int lane = threadIdx.x & (CAPAC_WAVE - 1);
     const double na2 = -bc.alpha * bc.alpha;
-    const int first = __builtin_amdgcn_readfirstlane(bc.Nm + blockIdx.x * kWavesPerBlock + threadIdx.x / CAPAC_WAVE);
+    const int first = __builtin_amdgcn_readfirstlane((all_rows ? 0 : bc.Nm) + blockIdx.x * kWavesPerBlock +
+                                                     threadIdx.x / CAPAC_WAVE);
     PairOut o = {};
     for (int i = first; i < N; i += gridDim.x * kWavesPerBlock) {
         const double2 ai = xyzq[2 * i], bi = xyzq[2 * i + 1];
@@ -345,21 +366,24 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
         const int ti = type[i] * nt;
         o.fx = 0.0; o.fy = 0.0; o.fz = 0.0;
         const bool lj_row = (lj_types >> type[i]) & 1;
-        water_segment(lj_row, xyzq, type, nbr + (size_t)i * stride, n_front[i], i, lane, bc, s_lj, ti, xi, yi, zi,
-                      K_FQQ * qi, na2, o, s_erfcx);
+        const int* row = nbr + (size_t)i * stride;
+        const int nf = n_front[i];
+        const int nb = all_rows && !(i < bc.Nm && metal_inert) ? n_back[i] : 0;
+        water_segment<Energy>(lj_row, xyzq, type, row, nf, row + stride - nb, nf + nb, i, lane, bc, s_lj, ti,
+                              xi, yi, zi, K_FQQ * qi, na2, o, s_erfcx);
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
             if (field) {
                 // the external field on the atom's charge, and its energy
                 // (force.c:2251-2260), in the same write
                 fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
-                o.e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
+                if (Energy) o.e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
             }
             if (accumulate) { f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz; }
             else { f[i] = fx; f[N + i] = fy; f[2 * N + i] = fz; }
         }
     }
-    pair_block_partials(o, s_red, lane, partial, blockIdx.x);
+    if (Energy) pair_block_partials(o, s_red, lane, partial, blockIdx.x);
 }
 
 // Water kernel, second launch: what the first leaves to the general term --
@@ -464,16 +488,28 @@ int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks, int field
     if (c->pair_water) {
         double2* xyzq = reinterpret_cast<double2*>(c->xyzq);
         double* part = partial_seg(c, SEG_PAIR);
+        const int all_rows = c->pair_water == 2;
         k_pack_xyzq<<<(c->bc.N + 255) / 256, 256, 0, c->stream>>>(c->r, c->charge, c->bc.N, xyzq);
-        k_pair_water<<<nblk, 256, 0, c->stream>>>(xyzq, c->type, c->pair, c->bc, c->n_front, c->nbr,
-                                                  c->nb_stride, f, f_accumulate ? 1 : 0, field, part,
-                                                  c->pair_lj_types);
-        k_pair_rest<<<kPairRestBlocks, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc,
-                                                            c->n_front, c->n_back, c->nbr, c->nb_stride, f,
-                                                            f_accumulate ? 1 : 0, field, part, nblk,
-                                                            c->pair_metal_inert);
+        if (c->need_energy)
+            k_pair_water<true><<<nblk, 256, 0, c->stream>>>(xyzq, c->type, c->pair, c->bc, c->n_front, c->n_back,
+                                                            c->nbr, c->nb_stride, f, f_accumulate ? 1 : 0, field,
+                                                            part, c->pair_lj_types, all_rows, c->pair_metal_inert);
+        else
+            k_pair_water<false><<<nblk, 256, 0, c->stream>>>(xyzq, c->type, c->pair, c->bc, c->n_front, c->n_back,
+                                                             c->nbr, c->nb_stride, f, f_accumulate ? 1 : 0, field,
+                                                             part, c->pair_lj_types, all_rows, c->pair_metal_inert);
+        int nparts = nblk;
+        if (!all_rows) {
+            k_pair_rest<<<kPairRestBlocks, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc,
+                                                                c->n_front, c->n_back, c->nbr, c->nb_stride, f,
+                                                                f_accumulate ? 1 : 0, field, part, nblk,
+                                                                c->pair_metal_inert);
+            nparts += kPairRestBlocks;
+        }
         CHECK_HIP(hipGetLastError());
-        return reduce_partials(c, SEG_PAIR, nblk + kPairRestBlocks, 12, 0);
+        // without energies red[0..12) keeps an earlier call's sums, which
+        // nothing reads
+        return c->need_energy ? reduce_partials(c, SEG_PAIR, nparts, 12, 0) : 0;
     } else {
         k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
                                             c->n_back, c->nbr, c->nb_stride, f,
