@@ -431,7 +431,8 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         // the water pair kernel: every pair of non-metal atom types funct 1,
         // damped-shifted-force Coulomb and shifted-force LJ, alpha rc < 6
         // (erfc from the erfcx table over the whole cut-off); with the metal
-        // types' pairs funct 1 too, it takes every row (pair_water = 2)
+        // types' pairs funct 1 or 4 (erf-damped vdW, width > 0) it takes
+        // every row (pair_water = 2)
         char* used = (char*)calloc(nt > 0 ? nt : 1, 1);
         for (int i = p->n_metal; i < N; ++i)
             if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
@@ -445,8 +446,10 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
                 if (p->atomtype[i] >= 0 && p->atomtype[i] < nt) used[p->atomtype[i]] = 1;
             int all = 1;
             for (int a = 0; a < nt && all; ++a)
-                for (int b2 = 0; b2 < nt; ++b2)
-                    if (used[a] && used[b2] && p->nb_funct[a * nt + b2] != 1) all = 0;
+                for (int b2 = 0; b2 < nt; ++b2) {
+                    const int k = a * nt + b2, fn = p->nb_funct[k];
+                    if (used[a] && used[b2] && fn != 1 && !(fn == 4 && p->nb_a[k] > 0.0)) all = 0;
+                }
             if (all) water = 2;
             else                        // the LJ bits below over the non-metal partners only
                 for (int i = 0; i < p->n_metal; ++i)

@@ -18,14 +18,24 @@ __device__ __forceinline__ double erfc_from_exp(double x, double ex, const doubl
     return ex * p;
 }
 
+// erf(x) for 0 <= x < 1/2 from its Maclaurin series (kErfSmall)
+__device__ __forceinline__ double erf_small(double x) {
+    const double x2 = x * x;
+    double p = kErfSmall[kErfSmallTerms - 1];
+#pragma unroll
+    for (int i = kErfSmallTerms - 2; i >= 0; --i) p = fma(p, x2, kErfSmall[i]);
+    return x * p;
+}
+
 // erf(x) = 1 - exp(-x^2) erfcx(x) for 0.5 <= x < 6, from the exponential
 // the caller already holds (as erfc_from_exp); there erfc <= 0.48, so the
 // difference loses nothing: within 2 ulp of the library erf over a 2e5-point
 // scan of [0.5, 6).  Beyond 6
 // erfc < 2.2e-17 and the result is 1.0, as the library's.  Below 0.5 (where
-// 1 - erfc would cancel) and for negative or NaN x, the library erf.
+// 1 - erfc would cancel) the Maclaurin series; for negative or NaN x the
+// library erf.
 __device__ __forceinline__ double erf_from_exp(double x, double ex, const double* __restrict__ coef) {
-    if (!(x >= 0.5)) return erf(x);
+    if (!(x >= 0.5)) return x >= 0.0 ? erf_small(x) : erf(x);
     if (!(x < kErfcxHi)) return 1.0;
     const int k = (int)(x * 4.0);
     const double t = (x - (k + 0.5) * 0.25) * 8.0;

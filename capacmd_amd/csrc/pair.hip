@@ -208,16 +208,49 @@ k_pair(const double* __restrict__ r, const double* __restrict__ charge, const in
 
 // ---------------------------------------------------------------- water path
 
-// the LJ constants of one pair type, as the water term reads them from LDS
-// (64 bytes: the entry's offset is a shift of the type index)
+// the constants of one pair type, as the water term reads them from LDS
+// (64 bytes: the entry's offset is a shift of the type index); erf4 = 1 for
+// a funct-4 type (erf-damped vdW, 1/width in inv_width)
 struct PairLJ {
-    double c6, c12, lj_fc, lj_fc_rc, lj_ec, pad[3];
+    double c6, c12, lj_fc, lj_fc_rc, lj_ec, inv_width, erf4, pad;
 };
+
+// erf-damped vdW (force.c:602-628, funct 4; pair_term's expressions) without
+// a test of c6 and c12 (zeros give zeros); erf from its Maclaurin series
+// below 1/2, from the Gaussian and the erfcx table up to 6, 1 beyond.
+// Returns the force over r.
+template <bool Energy>
+__device__ __forceinline__ double water_erf_vdw(const PairLJ& lj, double inv_r, double inv_r2, double rij,
+                                                PairOut& o, const double* __restrict__ erfcx_coef) {
+    const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    const double cr12 = lj.c12 * (inv_r6 * inv_r6), cr6 = lj.c6 * inv_r6;
+    const double rR = rij * lj.inv_width;
+    const double ex = exp_poly<false>(-rR * rR);
+    double erf_rR;
+    if (rR < 0.5) {
+        erf_rR = erf_small(rR);
+    } else if (rR < kErfcxHi) {
+        const double y = 4.0 * rR;
+        const double* c = erfcx_coef + (int)y * kErfcxTerms;
+        const double t = fma(y - floor(y), 2.0, -1.0);
+        double p = c[kErfcxTerms - 1];
+#pragma unroll
+        for (int m = kErfcxTerms - 2; m >= 0; --m) p = fma(p, t, c[m]);
+        erf_rR = 1.0 - ex * p;
+    } else {
+        erf_rR = 1.0;
+    }
+    if (Energy) o.e8 += (cr12 - cr6) * erf_rR;
+    return ((12.0 * cr12 - 6.0 * cr6) * inv_r * erf_rR - (cr12 - cr6) * 2.0 * K_INV_SQRT_PI * ex * lj.inv_width) *
+           inv_r;
+}
 
 // funct 1 under use_coulomb = use_vdw = 1 (pair_term's expressions), for
 // every pair without a test of the type or the charges; kqi = K_FQQ q_i,
 // na2 = -alpha^2.  lj_row false (the same on every lane): atom i's type has
-// no LJ with any non-metal type, so the LJ terms (exact zeros) are left out
+// no LJ with any type it meets here, so the LJ terms (exact zeros) are left
+// out.  A funct-4 pair type (lj.erf4) takes water_erf_vdw instead (no
+// Coulomb term, as the reference's compute_erf_vdw).
 template <bool Energy>
 __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, const PairLJ& lj, double kqi, double qj,
                                            double na2, double dx, double dy, double dz, double r2, PairOut& o,
@@ -225,6 +258,16 @@ __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, cons
     const double inv_r = rsqrt_pos(r2);
     const double rij = r2 * inv_r;
     const double inv_r2 = inv_r * inv_r;
+    if (lj.erf4 != 0.0) {
+        const double fs = water_erf_vdw<Energy>(lj, inv_r, inv_r2, rij, o, erfcx_coef);
+        o.fx = fma(-fs, dx, o.fx); o.fy = fma(-fs, dy, o.fy); o.fz = fma(-fs, dz, o.fz);
+        if (Energy) {
+            const double fjx = fs * dx, fjy = fs * dy, fjz = fs * dz;
+            o.v[0] -= dx * fjx; o.v[1] -= dx * fjy; o.v[2] -= dx * fjz;
+            o.v[3] -= dy * fjy; o.v[4] -= dy * fjz; o.v[5] -= dz * fjz;
+        }
+        return;
+    }
     // shifted-force LJ (force.c:748-757)
     double flj = 0.0;
     if (lj_row) {
@@ -351,7 +394,8 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
     const int N = bc.N, nt = bc.n_types;
     for (int k = threadIdx.x; k < nt * nt && k < 64; k += blockDim.x) {
         const PairParam& pp = table[k];
-        s_lj[k] = PairLJ{pp.c6, pp.c12, pp.lj_fc, pp.lj_fc_rc, pp.lj_ec, {0.0, 0.0, 0.0}};
+        const bool e4 = pp.funct == 4;
+        s_lj[k] = PairLJ{pp.c6, pp.c12, pp.lj_fc, pp.lj_fc_rc, pp.lj_ec, e4 ? pp.inv_width : 0.0, e4 ? 1.0 : 0.0, 0.0};
     }
     load_erfcx_table(s_erfcx);
     __syncthreads();
