@@ -581,6 +581,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->md_status = (int*)(c->red + kRedStatus);
     }
     TRY(upload(&c->isd, (const double*)NULL, Nm));
+    TRY(upload(&c->f_sc, (const double*)NULL, 3 * (size_t)Nm));
     {
         double qm = (double)p->ndf * (K_BOLTZ * p->ref_temp) * p->tau_t * p->tau_t;  // mpi-main.c:552
         double h[32] = {0};
@@ -620,7 +621,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->b, c->x, c->x_backup, c->diag, c->relay32, c->relay_part,
+                    c->partial, c->red, c->isd, c->f_sc, c->b, c->x, c->x_backup, c->diag, c->relay32, c->relay_part,
                     c->relay_cnt, c->metal_u, c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,
@@ -869,10 +870,10 @@ static int side_pair_blocks(const capac_ctx* c) {
     return c->pred_order > 0 ? 448 : 256;
 }
 
-// pair and Sutton-Chen forces on the side stream, after the rows are built
-// (event ev_rows); they write f first, so everything that accumulates into f
-// afterwards waits for ev_side.  Their reductions are queued here and
-// launched on the main stream once it has waited.
+// pair forces (and Sutton-Chen without CPIM unknowns) on the side stream,
+// after the rows are built (event ev_rows); they write f first, so everything
+// that accumulates into f afterwards waits for ev_side.  Their reductions are
+// queued here and launched on the main stream once it has waited.
 static int side_forces(capac_ctx* c) {
     int rc = flush_reductions(c);
     if (rc) return rc;
@@ -880,7 +881,7 @@ static int side_forces(capac_ctx* c) {
     CHECK_HIP(hipStreamWaitEvent(c->side, c->ev_rows, 0));
     hipStream_t main_stream = c->stream;
     c->stream = c->side;
-    mark(c, 11);                     // events 11-13: the side stream's phases
+    mark(c, 11);                     // events 11-12: the side stream's pair phase
     if (has_bonded(c)) {
         // the reference computes the bonded forces first (force.c:2163-2168);
         // the pair forces then add to them
@@ -891,8 +892,8 @@ static int side_forces(capac_ctx* c) {
         rc = pair_nonbonded(c, NULL, side_pair_blocks(c), 1);
     }
     mark(c, 12);
-    if (!rc && c->bc.Nm > 0) rc = sc_forces(c);
-    mark(c, 13);
+    // (with CPIM unknowns the Sutton-Chen force runs on the main stream)
+    if (!rc && c->bc.Nm > 0 && !c->n_mat) rc = sc_forces(c);
     c->stream = main_stream;
     if (rc) return rc;
     CHECK_HIP(hipEventRecord(c->ev_side, c->side));
@@ -926,14 +927,25 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         // and the assembly are done and overlaps only the solve; the capped
         // grid of the small decks leaves slots free and starts at once.
         const bool side_after_matrix = c->n_mat && c->bc.Nm >= 1024;
-        if (!side_after_matrix && (rc = side_forces(c))) return rc;
+        // With CPIM unknowns the Sutton-Chen force goes to f_sc on the main
+        // stream, right after the side stream has started (it needs only the
+        // rows), and the metal-metal CPIM launch adds it: the side stream,
+        // which ends after the solve at Au2057, loses two launches.
+        auto sc_main = [&]() -> int {
+            if (!c->n_mat) return 0;
+            mark(c, 2);
+            int r2 = sc_forces(c, c->f_sc);
+            mark(c, 3);
+            return r2;
+        };
+        if (!side_after_matrix && ((rc = side_forces(c)) || (rc = sc_main()))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {
             if ((rc = cpim_rhs(c))) return rc;
             mark(c, 4);
             if ((rc = cpim_assemble(c))) return rc;
             mark(c, 5);
-            if (side_after_matrix && (rc = side_forces(c))) return rc;
+            if (side_after_matrix && ((rc = side_forces(c)) || (rc = sc_main()))) return rc;
             solve_rc = solver_run(c, c->b, c->x);
             if (solve_rc && solve_rc != CAPAC_ERR_SOLVER) {
                 (void)hipStreamSynchronize(c->side);
@@ -1002,9 +1014,10 @@ static void take_force_results(capac_ctx* c) {
     }
     if (c->timing_on) {
         float ms;
-        // (from, to, category): build, RHS, matrix, solve, CPIM force on the
-        // main stream; pair and SC on the side stream, overlapping the solve
-        const int spans[7][3] = {{0, 1, 10}, {1, 4, 6}, {4, 5, 5}, {5, 6, 7}, {6, 7, 8}, {11, 12, 4}, {12, 13, 2}};
+        // (from, to, category): build, RHS, matrix, solve, CPIM force and SC
+        // (with CPIM unknowns) on the main stream; pair on the side stream,
+        // overlapping the solve
+        const int spans[7][3] = {{0, 1, 10}, {1, 4, 6}, {4, 5, 5}, {5, 6, 7}, {6, 7, 8}, {11, 12, 4}, {2, 3, 2}};
         for (int k = 0; k < 7; ++k)
             if (hipEventElapsedTime(&ms, c->ev[spans[k][0]], c->ev[spans[k][1]]) == hipSuccess)
                 c->timing.t[spans[k][2]] += 1e-3 * ms;

@@ -158,6 +158,7 @@ struct capac_ctx {
 
     // Sutton-Chen
     double* isd;
+    double* f_sc;               // its forces on the metal atoms (3 x N_m) when the CPIM force adds them
     // CPIM
     double *b, *x, *x_backup, *diag;
     int diag_set;               // diag holds the values for diag_polar / diag_capac
@@ -295,12 +296,14 @@ int nb_set_radius(capac_ctx* c, double skin);
 // pair.hip
 int pair_nonbonded(capac_ctx* c, double* f_out_accumulate, int max_blocks = 0, int field = 0);
 // sc.hip
-int sc_forces(capac_ctx* c);
+// f_sc NULL: adds the forces to c->f; else writes them to f_sc (3 x N_m)
+int sc_forces(capac_ctx* c, double* f_sc = NULL);
 // cpim.hip
 int cpim_rhs(capac_ctx* c);
 int cpim_assemble(capac_ctx* c);
 int cpim_forces(capac_ctx* c);
-int cpim_forces_x(capac_ctx* c, const double* x);
+// f_sc (3 x N_m, or NULL): forces the metal-metal launch adds with its own
+int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc = NULL);
 int cpim_dense_expand(capac_ctx* c, double* dense_dev);
 // out = b - A x (b non-NULL) or A x, with the FP64 relay recomputed on the fly
 // (no stored matrix); part[g] = (out, out) over the rows of row-group g.

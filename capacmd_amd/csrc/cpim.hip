@@ -507,8 +507,10 @@ __device__ __forceinline__ void mw_atom_terms(const double* __restrict__ r, cons
 // few hundred metal atoms leave most SIMDs with a single wavefront, whose
 // transcendental latency nothing hides.  The force on the atom is the
 // fixed-order sum of the four wavefront sums.
+// extra (3 x Nm, or NULL): a force the same write adds (the Sutton-Chen force)
 __device__ __forceinline__ void block_force_add(double fx, double fy, double fz, double (*s_f)[3], int i, int N,
-                                                double* __restrict__ f) {
+                                                double* __restrict__ f, const double* __restrict__ extra = NULL,
+                                                int Nm = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
     if (lane == 0) { s_f[wave][0] = fx; s_f[wave][1] = fy; s_f[wave][2] = fz; }
@@ -516,6 +518,7 @@ __device__ __forceinline__ void block_force_add(double fx, double fy, double fz,
     if (threadIdx.x < 3) {
         double t = 0.0;
         for (int w = 0; w < 4; ++w) t += s_f[w][threadIdx.x];
+        if (extra) t += extra[(size_t)threadIdx.x * Nm + i];
         f[(size_t)threadIdx.x * N + i] += t;
     }
 }
@@ -587,7 +590,7 @@ inline unsigned mw_atom_blocks(int n_nonmetal) {
 // partial = e9, e12, e13, e14, virial
 __global__ void __launch_bounds__(256)
 k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
-          double* __restrict__ f, double* __restrict__ partial) {
+          double* __restrict__ f, double* __restrict__ partial, const double* __restrict__ f_sc) {
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     __shared__ double s_red[4][13];
     __shared__ double s_f[4][3];
@@ -598,7 +601,7 @@ k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ 
 #pragma unroll
     for (int c = 0; c < 13; ++c) vals[c] = 0.0;
     mm_terms(r, bc, x, s_erfcx, i, threadIdx.x, fx, fy, fz, vals);
-    block_force_add(fx, fy, fz, s_f, i, bc.N, f);
+    block_force_add(fx, fy, fz, s_f, i, bc.N, f, f_sc, bc.Nm);
     block_partials<13>(vals, s_red, partial);
 }
 
@@ -675,7 +678,7 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 
 // adds the CPIM forces for x (device) to c->f;
 // red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
-int cpim_forces_x(capac_ctx* c, const double* x) {
+int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
     int Nm = c->bc.Nm, N = c->bc.N;
     if (Nm == 0) return 0;
     int nblk = Nm;                      // one workgroup per metal atom
@@ -688,18 +691,19 @@ int cpim_forces_x(capac_ctx* c, const double* x) {
         k_cpim_mw_atom<<<mw_atom_blocks(N - Nm), 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
                                                                         c->nbr, c->nb_stride, c->tail_list, x,
                                                                         c->f);
-    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, partial_seg(c, SEG_MM));
+    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, partial_seg(c, SEG_MM), f_sc);
     CHECK_HIP(hipGetLastError());
     return reduce_partials(c, SEG_MM, nblk, 13, 48);
 }
 
-int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x); }
+// (in a force call the Sutton-Chen force waits in f_sc)
+int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x, c->f_sc); }
 
 // the metal-metal CPIM force kernel alone, adding to c->f (for kernel timing)
 int cpim_mm_only(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
-    k_cpim_mm<<<Nm, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM));
+    k_cpim_mm<<<Nm, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM), NULL);
     CHECK_HIP(hipGetLastError());
     return 0;
 }

@@ -48,7 +48,7 @@ k_sc_dens(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_b
 __global__ void __launch_bounds__(256)
 k_sc_force(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_back,
            const int* __restrict__ nbr, int stride, const double* __restrict__ isd,
-           double* __restrict__ f, double* __restrict__ partial) {
+           double* __restrict__ f, int f_stride, int add, double* __restrict__ partial) {
     __shared__ double s_red[4][10];
     const int N = bc.N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -81,7 +81,8 @@ k_sc_force(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_
         }
         fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
         if (lane == 0) {
-            f[i] += fx; f[N + i] += fy; f[2 * N + i] += fz;
+            if (add) { f[i] += fx; f[f_stride + i] += fy; f[2 * f_stride + i] += fz; }
+            else { f[i] = fx; f[f_stride + i] = fy; f[2 * f_stride + i] = fz; }
             e1 -= bc.qsc_eps * bc.qsc_c / isd_i;      // embedding term of atom i
         }
     }
@@ -102,13 +103,16 @@ k_sc_force(const double* __restrict__ r, BoxConst bc, const int* __restrict__ n_
 
 }  // namespace
 
-// adds the Sutton-Chen force to c->f; red[16..26) = pot1, vir[9]
-int sc_forces(capac_ctx* c) {
+// adds the Sutton-Chen force to c->f (f_sc NULL) or writes it to f_sc (3 x
+// N_m; the metal-metal CPIM launch adds it, so these launches can run while
+// the pair launches still write c->f); red[16..26) = pot1, vir[9]
+int sc_forces(capac_ctx* c, double* f_sc) {
     if (c->bc.Nm == 0) return 0;
     int nblk = (c->bc.Nm + 3) / 4;
     k_sc_dens<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, c->n_back, c->nbr, c->nb_stride, c->isd);
     k_sc_force<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, c->n_back, c->nbr, c->nb_stride, c->isd,
-                                            c->f, partial_seg(c, SEG_SC));
+                                            f_sc ? f_sc : c->f, f_sc ? c->bc.Nm : c->bc.N, f_sc ? 0 : 1,
+                                            partial_seg(c, SEG_SC));
     CHECK_HIP(hipGetLastError());
     return reduce_partials(c, SEG_SC, nblk, 10, 16);
 }
