@@ -562,7 +562,8 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
         c->seg_cap[SEG_PAIR] = pair_partial_blocks(N);
         c->seg_cap[SEG_SC] = metal_blocks;
         c->seg_cap[SEG_MW] = Nm + 1;        // one workgroup per metal atom
-        c->seg_cap[SEG_MM] = Nm + 1;
+        // the metal-metal tiles, then one slot per block of 64 metal atoms
+        c->seg_cap[SEG_MM] = (int)(tile_count(tile_blocks(Nm)) + tile_blocks(Nm));
         c->seg_cap[SEG_RATTLE] = (p->n_mols + 63) / 64 + 1;      // workgroups of 64 or more
         c->seg_cap[SEG_MOM] = (N + 255) / 256 + 1;
         c->seg_cap[SEG_BONDED] = bonded_blocks(p->n_mols) + 1;
@@ -582,6 +583,7 @@ extern "C" int capac_create(const capac_params* p, int precision, int device, ca
     }
     TRY(upload(&c->isd, (const double*)NULL, Nm));
     TRY(upload(&c->f_sc, (const double*)NULL, 3 * (size_t)Nm));
+    TRY(upload(&c->mm_part, (const double*)NULL, (size_t)tile_blocks(Nm) * tile_blocks(Nm) * 3 * kTileEdge));
     {
         double qm = (double)p->ndf * (K_BOLTZ * p->ref_temp) * p->tau_t * p->tau_t;  // mpi-main.c:552
         double h[32] = {0};
@@ -621,7 +623,7 @@ extern "C" void capac_destroy(capac_ctx* c) {
                     c->np_charge, c->cell_of, c->cell_count, c->cell_start, c->cell_atoms,
                     c->cell_tmp, c->cell_fill, c->cell_rank, c->sorted_r, c->sorted_mol,
                     c->sorted_loc, c->stencil, c->n_front, c->n_back, c->nbr, c->r_build, c->nb_ctl,
-                    c->partial, c->red, c->isd, c->f_sc, c->b, c->x, c->x_backup, c->diag, c->relay32, c->relay_part,
+                    c->partial, c->red, c->isd, c->f_sc, c->mm_part, c->b, c->x, c->x_backup, c->diag, c->relay32, c->relay_part,
                     c->relay_cnt, c->metal_u, c->x_hist[0], c->x_hist[1], c->x_hist[2], c->mol_terms, c->bond_atoms,
                     c->bond_funct, c->bond_par, c->pair14_atoms, c->pair14_par, c->angle_atoms,
                     c->angle_funct, c->angle_par, c->dihedral_atoms, c->dihedral_par, c->vsite_atoms,

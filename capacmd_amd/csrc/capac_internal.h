@@ -159,6 +159,7 @@ struct capac_ctx {
     // Sutton-Chen
     double* isd;
     double* f_sc;               // its forces on the metal atoms (3 x N_m) when the CPIM force adds them
+    double* mm_part;            // metal-metal CPIM force partials, Nb x Nb blocks of 3 x 64 (cpim.hip)
     // CPIM
     double *b, *x, *x_backup, *diag;
     int diag_set;               // diag holds the values for diag_polar / diag_capac

@@ -382,94 +382,74 @@ __device__ __forceinline__ void mw_metal_terms(const double* __restrict__ r, con
     for (int c = 0; c < 9; ++c) vals[2 + c] = v[c];
 }
 
-// Metal-metal terms, all pairs, force on i only (cpff.c:762-927), plus the
-// self terms (cpff.c:727-742) and the external field on the induced charge
-// and dipole (cpff.c:930-948); j split over 256 threads (t = 0..255).  Each
+// One metal-metal pair (cpff.c:762-927), both orders at once: r = r_j - r_i
+// in bohr, (qi, pi) and (qj, pj) the induced charges and dipoles.  The force
+// on i is -g and on j +g: swapping i and j negates r and every term of g is
+// odd in r once the pair's roles swap too, so the reference's two ordered
+// evaluations give -g and +g up to rounding.  e12, e13, e14 receive the
+// pair's energies (the two ordered halves of the reference together).  Each
 // erf comes from the Gaussian the same term needs anyway (erf_from_exp, with
-// the erfcx table staged in LDS), not from a library erf of its own.
-// vals = e9, e12, e13, e14, virial
-__device__ __forceinline__ void mm_terms(const double* __restrict__ r, const BoxConst& bc,
-                                         const double* __restrict__ x, const double* __restrict__ erfcx,
-                                         int i, int t,
-                                         double& fx, double& fy, double& fz, double vals[13]) {
-    const int N = bc.N, Nm = bc.Nm;
-    double e9 = 0.0, e12 = 0.0, e13 = 0.0, e14 = 0.0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
-    const double qi = x[3 * Nm + i];
-    const double pi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-    const double iRqq = bc.inv_R_qq, iRpq = bc.inv_R_pq, iRpp = bc.inv_R_pp;
-    const double iRpq3 = iRpq * iRpq * iRpq, iRpp3 = iRpp * iRpp * iRpp;
-    const double iRpp5 = iRpp3 * iRpp * iRpp;
-    for (int j = t; j < Nm; j += 256) {
-        if (j == i) continue;
-        double dx = r[j] - xi, dy = r[N + j] - yi, dz = r[2 * N + j] - zi;
-        pbc12(bc, dx, dy, dz);
-        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
-        double r2 = dist2_exact(dx, dy, dz);
-        double inv_r = rsqrt_pos(r2), inv_r2 = inv_r * inv_r;
-        double rij = r2 * inv_r;
-        double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
-        double qj = x[3 * Nm + j];
-        double pj[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-        double rv[3] = {dx, dy, dz};
-        double g[3] = {0.0, 0.0, 0.0};
-        // charge-charge, T0
-        double e_qq = exp_poly<false>(-(r2 * iRqq * iRqq));
-        double erf_qq = erf_from_exp(rij * iRqq, e_qq, erfcx);
-        double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * iRqq * rij * e_qq);
-        double sqq = A0 * K_FQQ_AU2 * qi * qj;
-        g[0] += sqq * dx; g[1] += sqq * dy; g[2] += sqq * dz;
-        e12 += 0.5 * K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
-        // charge-dipole, T1 both directions
-        double e_pq = exp_poly<false>(-(r2 * iRpq * iRpq));
-        double br_pq = erf_from_exp(rij * iRpq, e_pq, erfcx) - 2.0 * K_INV_SQRT_PI * iRpq * rij * e_pq;
-        double Apq0 = inv_r3 * br_pq;
-        double A1 = inv_r5 * br_pq;
-        double B1 = 4.0 * K_INV_SQRT_PI * iRpq3 * inv_r2 * e_pq;
-        // w = qj p_i - qi p_j:  f_b += sum_a (A1 (3 r_a r_b - d r^2) - B1 r_a r_b) au2 w_a
-        double w[3] = {qj * pi[0] - qi * pj[0], qj * pi[1] - qi * pj[1], qj * pi[2] - qi * pj[2]};
-        double wr = w[0] * dx + w[1] * dy + w[2] * dz;
+// the erfcx table staged in LDS).
+struct MMConst {
+    double iRqq, iRpq, iRpp, iRpq3, iRpp3, iRpp5;
+};
+
+__device__ __forceinline__ MMConst mm_const(const BoxConst& bc) {
+    MMConst k;
+    k.iRqq = bc.inv_R_qq; k.iRpq = bc.inv_R_pq; k.iRpp = bc.inv_R_pp;
+    k.iRpq3 = k.iRpq * k.iRpq * k.iRpq;
+    k.iRpp3 = k.iRpp * k.iRpp * k.iRpp;
+    k.iRpp5 = k.iRpp3 * k.iRpp * k.iRpp;
+    return k;
+}
+
+__device__ __forceinline__ void mm_pair(const MMConst& k, double dx, double dy, double dz, double qi,
+                                        const double pi[3], double qj, const double pj[3],
+                                        const double* __restrict__ erfcx, double g[3], double& e12, double& e13,
+                                        double& e14) {
+    const double r2 = dist2_exact(dx, dy, dz);
+    const double inv_r = rsqrt_pos(r2), inv_r2 = inv_r * inv_r;
+    const double rij = r2 * inv_r;
+    const double inv_r3 = inv_r2 * inv_r, inv_r5 = inv_r3 * inv_r2, inv_r7 = inv_r5 * inv_r2;
+    const double rv[3] = {dx, dy, dz};
+    // charge-charge, T0
+    const double e_qq = exp_poly<false>(-(r2 * k.iRqq * k.iRqq));
+    const double erf_qq = erf_from_exp_pos(rij * k.iRqq, e_qq, erfcx);
+    const double A0 = inv_r3 * (erf_qq - 2.0 * K_INV_SQRT_PI * k.iRqq * rij * e_qq);
+    const double sqq = A0 * K_FQQ_AU2 * qi * qj;
+    g[0] = sqq * dx; g[1] = sqq * dy; g[2] = sqq * dz;
+    e12 += K_FQQ_AU1 * qi * qj * erf_qq * inv_r;
+    // charge-dipole, T1 both directions
+    const double e_pq = exp_poly<false>(-(r2 * k.iRpq * k.iRpq));
+    const double br_pq = erf_from_exp_pos(rij * k.iRpq, e_pq, erfcx) - 2.0 * K_INV_SQRT_PI * k.iRpq * rij * e_pq;
+    const double Apq0 = inv_r3 * br_pq;
+    const double A1 = inv_r5 * br_pq;
+    const double B1 = 4.0 * K_INV_SQRT_PI * k.iRpq3 * inv_r2 * e_pq;
+    // w = qj p_i - qi p_j:  g_b += sum_a (A1 (3 r_a r_b - d r^2) - B1 r_a r_b) au2 w_a
+    const double w[3] = {qj * pi[0] - qi * pj[0], qj * pi[1] - qi * pj[1], qj * pi[2] - qi * pj[2]};
+    const double wr = w[0] * dx + w[1] * dy + w[2] * dz;
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
-            g[b] += (A1 * (3.0 * wr * rv[b] - w[b] * r2) - B1 * wr * rv[b]) * K_FQQ_AU2;
-        double pjr = pj[0] * dx + pj[1] * dy + pj[2] * dz;
-        double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
-        e13 += 0.5 * K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
-        // dipole-dipole, T2
-        double e_pp = exp_poly<false>(-(r2 * iRpp * iRpp));
-        double erf_pp = erf_from_exp(rij * iRpp, e_pp, erfcx);
-        double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * iRpp * rij * e_pp);
-        double Bpp1 = 4.0 * K_INV_SQRT_PI * iRpp3 * inv_r2 * e_pp;
-        double A2 = inv_r7 * (3.0 * erf_pp - 6.0 * K_INV_SQRT_PI * iRpp * rij * e_pp -
-                              4.0 * r2 * rij * K_INV_SQRT_PI * iRpp3 * e_pp);
-        double B2 = 8.0 * K_INV_SQRT_PI * iRpp5 * inv_r2 * e_pp;
-        double pipj = pi[0] * pj[0] + pi[1] * pj[1] + pi[2] * pj[2];
-        // sum_ab p_ia p_jb [5 r_a r_b r_c - r^2 (r_a d_bc + r_b d_ca + r_c d_ab)]
-        //   = 5 (pi.r)(pj.r) r_c - r^2 ((pi.r) pj_c + (pj.r) pi_c + (pi.pj) r_c)
+    for (int b = 0; b < 3; ++b) g[b] += (A1 * (3.0 * wr * rv[b] - w[b] * r2) - B1 * wr * rv[b]) * K_FQQ_AU2;
+    const double pjr = pj[0] * dx + pj[1] * dy + pj[2] * dz;
+    const double pir = pi[0] * dx + pi[1] * dy + pi[2] * dz;
+    e13 += K_FQQ_AU1 * Apq0 * (qj * pir - qi * pjr);
+    // dipole-dipole, T2
+    const double e_pp = exp_poly<false>(-(r2 * k.iRpp * k.iRpp));
+    const double erf_pp = erf_from_exp_pos(rij * k.iRpp, e_pp, erfcx);
+    const double App1 = inv_r5 * (erf_pp - 2.0 * K_INV_SQRT_PI * k.iRpp * rij * e_pp);
+    const double Bpp1 = 4.0 * K_INV_SQRT_PI * k.iRpp3 * inv_r2 * e_pp;
+    const double A2 = inv_r7 * (3.0 * erf_pp - 6.0 * K_INV_SQRT_PI * k.iRpp * rij * e_pp -
+                                4.0 * r2 * rij * K_INV_SQRT_PI * k.iRpp3 * e_pp);
+    const double B2 = 8.0 * K_INV_SQRT_PI * k.iRpp5 * inv_r2 * e_pp;
+    const double pipj = pi[0] * pj[0] + pi[1] * pj[1] + pi[2] * pj[2];
+    // sum_ab p_ia p_jb [5 r_a r_b r_c - r^2 (r_a d_bc + r_b d_ca + r_c d_ab)]
+    //   = 5 (pi.r)(pj.r) r_c - r^2 ((pi.r) pj_c + (pj.r) pi_c + (pi.pj) r_c)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double t5 = 5.0 * pir * pjr * rv[c] - r2 * (pir * pj[c] + pjr * pi[c] + pipj * rv[c]);
-            g[c] -= (A2 * t5 - B2 * pir * pjr * rv[c]) * K_FQQ_AU2;
-        }
-        e14 -= 0.5 * K_FQQ_AU1 * ((3.0 * pir * pjr - pipj * r2) * App1 - pir * pjr * Bpp1);
-        fx -= g[0]; fy -= g[1]; fz -= g[2];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) v[a * 3 + b] -= 0.5 * rv[a] * g[b];
+    for (int c = 0; c < 3; ++c) {
+        const double t5 = 5.0 * pir * pjr * rv[c] - r2 * (pir * pj[c] + pjr * pi[c] + pipj * rv[c]);
+        g[c] -= (A2 * t5 - B2 * pir * pjr * rv[c]) * K_FQQ_AU2;
     }
-    if (t == 0) {
-        // self terms (inv_capac, inv_polar in atomic units, FQQ in MD units)
-        e12 += 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
-        e14 += 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
-        // external field on the induced charge (force) and charge + dipole (energy)
-        fx += bc.efield[0] * qi; fy += bc.efield[1] * qi; fz += bc.efield[2] * qi;
-        e9 -= bc.efield[0] * qi * xi + bc.efield[1] * qi * yi + bc.efield[2] * qi * zi;
-        e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
-    }
-    vals[0] = e9; vals[1] = e12; vals[2] = e13; vals[3] = e14;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) vals[4 + c] = v[c];
+    e14 -= K_FQQ_AU1 * ((3.0 * pir * pjr - pipj * r2) * App1 - pir * pjr * Bpp1);
 }
 
 // atom side of the metal-water terms: non-metal atom a over the metal tail of
@@ -507,10 +487,8 @@ __device__ __forceinline__ void mw_atom_terms(const double* __restrict__ r, cons
 // few hundred metal atoms leave most SIMDs with a single wavefront, whose
 // transcendental latency nothing hides.  The force on the atom is the
 // fixed-order sum of the four wavefront sums.
-// extra (3 x Nm, or NULL): a force the same write adds (the Sutton-Chen force)
 __device__ __forceinline__ void block_force_add(double fx, double fy, double fz, double (*s_f)[3], int i, int N,
-                                                double* __restrict__ f, const double* __restrict__ extra = NULL,
-                                                int Nm = 0) {
+                                                double* __restrict__ f) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
     if (lane == 0) { s_f[wave][0] = fx; s_f[wave][1] = fy; s_f[wave][2] = fz; }
@@ -518,7 +496,6 @@ __device__ __forceinline__ void block_force_add(double fx, double fy, double fz,
     if (threadIdx.x < 3) {
         double t = 0.0;
         for (int w = 0; w < 4; ++w) t += s_f[w][threadIdx.x];
-        if (extra) t += extra[(size_t)threadIdx.x * Nm + i];
         f[(size_t)threadIdx.x * N + i] += t;
     }
 }
@@ -586,23 +563,157 @@ inline unsigned mw_atom_blocks(int n_nonmetal) {
     return (unsigned)(nb < 1 ? 1 : (nb < 2048 ? nb : 2048));
 }
 
-// metal-metal terms: one workgroup per metal atom;
-// partial = e9, e12, e13, e14, virial
-__global__ void __launch_bounds__(256)
-k_cpim_mm(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x,
-          double* __restrict__ f, double* __restrict__ partial, const double* __restrict__ f_sc) {
+// Metal-metal terms on the 64 x 64 tiles of metal blocks (relay_tile.h),
+// each unordered pair once.  One workgroup per tile (I, J), I <= J, four
+// wavefronts of sixteen diagonals: at diagonal k lane l takes the pair
+// i = I*64 + l, j = J*64 + ((l + k) & 63), adds -g to its row sum (registers)
+// and +g to its wavefront's column sum for j (LDS; within one step the 64
+// lanes meet 64 different j, so no two lanes write one slot).  A diagonal
+// tile takes k = 1..31 and half of k = 32: every unordered pair of its block
+// once.  The four wavefronts' row and column sums are added in a fixed
+// order and stored as the partials P[I][J] (forces on block I from block J)
+// and P[J][I] (3 x 64 doubles each; a diagonal tile stores rows + columns);
+// k_cpim_mm_sum adds them.  partial (per tile) = e9 (0), e12, e13, e14, virial.
+// (162 VGPRs: three workgroups of four wavefronts per compute unit hold all
+// 561 tiles of Au2057 at once)
+constexpr int kMMWaves = 4;
+constexpr int kMMSteps = kTileEdge / kMMWaves;
+constexpr int kMMBlk = 3 * kTileEdge;          // doubles of one partial
+
+__global__ void __launch_bounds__(kMMWaves * 64)
+k_cpim_mm_tiles(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int Nb,
+                double* __restrict__ P, double* __restrict__ partial) {
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
-    __shared__ double s_red[4][13];
-    __shared__ double s_f[4][3];
+    __shared__ double s_j[7][kTileEdge];                // block J: x, y, z (nm), q, p (3)
+    __shared__ double s_col[kMMWaves][3][kTileEdge];
+    __shared__ double s_row[kMMWaves][3][kTileEdge];
+    __shared__ double s_red[kMMWaves][13];
+    const int N = bc.N, Nm = bc.Nm;
+    int I, J;
+    tile_coords(blockIdx.x, Nb, I, J);
     load_erfcx_table(s_erfcx);
+    if (threadIdx.x < kTileEdge) {
+        const int j = J * kTileEdge + threadIdx.x;
+        const bool ok = j < Nm;
+        s_j[0][threadIdx.x] = ok ? r[j] : 0.0;
+        s_j[1][threadIdx.x] = ok ? r[N + j] : 0.0;
+        s_j[2][threadIdx.x] = ok ? r[2 * N + j] : 0.0;
+        s_j[3][threadIdx.x] = ok ? x[3 * Nm + j] : 0.0;
+        s_j[4][threadIdx.x] = ok ? x[3 * j] : 0.0;
+        s_j[5][threadIdx.x] = ok ? x[3 * j + 1] : 0.0;
+        s_j[6][threadIdx.x] = ok ? x[3 * j + 2] : 0.0;
+    }
+    for (int k = threadIdx.x; k < kMMWaves * 3 * kTileEdge; k += blockDim.x) (&s_col[0][0][0])[k] = 0.0;
     __syncthreads();
-    const int i = blockIdx.x;
-    double fx = 0.0, fy = 0.0, fz = 0.0, vals[13];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = I * kTileEdge + lane;
+    const bool iok = i < Nm;
+    const double xi = iok ? r[i] : 0.0, yi = iok ? r[N + i] : 0.0, zi = iok ? r[2 * N + i] : 0.0;
+    const double qi = iok ? x[3 * Nm + i] : 0.0;
+    const double pi[3] = {iok ? x[3 * i] : 0.0, iok ? x[3 * i + 1] : 0.0, iok ? x[3 * i + 2] : 0.0};
+    const MMConst mk = mm_const(bc);
+    const bool diag = I == J;
+    double fr[3] = {0.0, 0.0, 0.0}, e12 = 0.0, e13 = 0.0, e14 = 0.0;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+    for (int s = 0; s < kMMSteps; ++s) {
+        const int k = wave * kMMSteps + s;
+        const int jl = (lane + k) & (kTileEdge - 1);
+        bool valid = iok && J * kTileEdge + jl < Nm;
+        if (diag) valid = valid && k >= 1 && (k < 32 || (k == 32 && lane < 32));
+        if (!valid) continue;
+        double dx = s_j[0][jl] - xi, dy = s_j[1][jl] - yi, dz = s_j[2][jl] - zi;
+        pbc12(bc, dx, dy, dz);
+        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+        const double pj[3] = {s_j[4][jl], s_j[5][jl], s_j[6][jl]};
+        double g[3];
+        mm_pair(mk, dx, dy, dz, qi, pi, s_j[3][jl], pj, s_erfcx, g, e12, e13, e14);
+        const double rv[3] = {dx, dy, dz};
 #pragma unroll
-    for (int c = 0; c < 13; ++c) vals[c] = 0.0;
-    mm_terms(r, bc, x, s_erfcx, i, threadIdx.x, fx, fy, fz, vals);
-    block_force_add(fx, fy, fz, s_f, i, bc.N, f, f_sc, bc.Nm);
-    block_partials<13>(vals, s_red, partial);
+        for (int c = 0; c < 3; ++c) {
+            fr[c] -= g[c];
+            s_col[wave][c][jl] += g[c];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) v[a * 3 + b] -= rv[a] * g[b];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s_row[wave][c][lane] = fr[c];
+    double vals[13] = {0.0, e12, e13, e14, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+#pragma unroll
+    for (int c = 0; c < 13; ++c) {
+        const double t = wave_sum(vals[c]);
+        if (lane == 0) s_red[wave][c] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMMBlk) {
+        const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+        double rs = 0.0, cs = 0.0;
+#pragma unroll
+        for (int w = 0; w < kMMWaves; ++w) {
+            rs += s_row[w][c][l];
+            cs += s_col[w][c][l];
+        }
+        if (diag) {
+            P[((size_t)I * Nb + I) * kMMBlk + threadIdx.x] = rs + cs;
+        } else {
+            P[((size_t)I * Nb + J) * kMMBlk + threadIdx.x] = rs;
+            P[((size_t)J * Nb + I) * kMMBlk + threadIdx.x] = cs;
+        }
+    } else if (threadIdx.x < kMMBlk + 16) {
+        const int c = threadIdx.x - kMMBlk;
+        double t = 0.0;
+        if (c < 13)
+            for (int w = 0; w < kMMWaves; ++w) t += s_red[w][c];
+        partial[blockIdx.x * 16 + c] = t;
+    }
+}
+
+// The metal atoms' CPIM forces from the tiles' partials: one workgroup of 256
+// threads per block B of 64 metal atoms; thread (c, l), c < 3, sums
+// P[B][S] over S = 0 .. Nb-1 in order for atom B*64 + l, adds the external
+// field on the induced charge (cpff.c:930-948) and extra (the Sutton-Chen
+// force, 3 x N_m, or NULL), and adds the total to f.  The self terms
+// (cpff.c:727-742) and the field energies of the block's charges and dipoles
+// go to partial[slot] (e9, e12, e13 = 0, e14, virial 0).
+__global__ void __launch_bounds__(256)
+k_cpim_mm_sum(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int Nb,
+              const double* __restrict__ P, const double* __restrict__ extra, double* __restrict__ f,
+              double* __restrict__ partial, int slot0) {
+    const int N = bc.N, Nm = bc.Nm, B = blockIdx.x;
+    if (threadIdx.x < kMMBlk) {
+        const int c = threadIdx.x >> 6, l = threadIdx.x & 63, i = B * kTileEdge + l;
+        double t = 0.0;
+        for (int S = 0; S < Nb; ++S) t += P[((size_t)B * Nb + S) * kMMBlk + threadIdx.x];
+        if (i < Nm) {
+            t += bc.efield[c] * x[3 * Nm + i];
+            if (extra) t += extra[(size_t)c * Nm + i];
+            f[(size_t)c * N + i] += t;
+        }
+    }
+    // self and field energies of the block's atoms (the first wavefront)
+    if (threadIdx.x < kTileEdge) {
+        double e9 = 0.0, e12 = 0.0, e14 = 0.0;
+        const int i = B * kTileEdge + (int)threadIdx.x;
+        if (i < Nm) {
+            const double qi = x[3 * Nm + i];
+            const double pi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+            // (inv_capac, inv_polar in atomic units, FQQ in MD units)
+            e12 = 0.5 * K_FQQ * qi * qi * (bc.inv_capac * K_NM2BOHR);
+            e14 = 0.5 * K_FQQ * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2]) * (bc.inv_polar * K_NM2BOHR);
+            e9 = -(bc.efield[0] * qi * r[i] + bc.efield[1] * qi * r[N + i] + bc.efield[2] * qi * r[2 * N + i]);
+            e9 -= bc.efield[0] * pi[0] + bc.efield[1] * pi[1] + bc.efield[2] * pi[2];
+        }
+        e9 = wave_sum(e9);
+        e12 = wave_sum(e12);
+        e14 = wave_sum(e14);
+        if (threadIdx.x < 16) {
+            const int c = threadIdx.x;
+            partial[(size_t)(slot0 + B) * 16 + c] = c == 0 ? e9 : c == 1 ? e12 : c == 3 ? e14 : 0.0;
+        }
+    }
 }
 
 }  // namespace
@@ -676,6 +787,19 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
     return 0;
 }
 
+// the metal-metal terms for x: the tile launch and the per-block sums
+// (adding extra, 3 x N_m or NULL, with them); reduce: queue the energies'
+// reduction (red[48..61))
+static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce) {
+    const int Nb = tile_blocks(c->bc.Nm);
+    const int n_tiles = (int)tile_count(Nb);
+    double* part = partial_seg(c, SEG_MM);
+    k_cpim_mm_tiles<<<n_tiles, kMMWaves * 64, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, part);
+    k_cpim_mm_sum<<<Nb, 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, extra, c->f, part, n_tiles);
+    CHECK_HIP(hipGetLastError());
+    return reduce ? reduce_partials(c, SEG_MM, n_tiles + Nb, 13, 48) : 0;
+}
+
 // adds the CPIM forces for x (device) to c->f;
 // red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
@@ -691,19 +815,14 @@ int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
         k_cpim_mw_atom<<<mw_atom_blocks(N - Nm), 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
                                                                         c->nbr, c->nb_stride, c->tail_list, x,
                                                                         c->f);
-    k_cpim_mm<<<nblk, 256, 0, c->stream>>>(c->r, c->bc, x, c->f, partial_seg(c, SEG_MM), f_sc);
-    CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_MM, nblk, 13, 48);
+    return cpim_mm_launch(c, x, f_sc, true);
 }
 
 // (in a force call the Sutton-Chen force waits in f_sc)
 int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x, c->f_sc); }
 
-// the metal-metal CPIM force kernel alone, adding to c->f (for kernel timing)
+// the metal-metal CPIM force launches alone, adding to c->f (for kernel timing)
 int cpim_mm_only(capac_ctx* c) {
-    int Nm = c->bc.Nm;
-    if (Nm == 0) return 0;
-    k_cpim_mm<<<Nm, 256, 0, c->stream>>>(c->r, c->bc, c->x, c->f, partial_seg(c, SEG_MM), NULL);
-    CHECK_HIP(hipGetLastError());
-    return 0;
+    if (c->bc.Nm == 0) return 0;
+    return cpim_mm_launch(c, c->x, NULL, false);
 }

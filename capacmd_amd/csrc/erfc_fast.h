@@ -46,6 +46,20 @@ __device__ __forceinline__ double erf_from_exp(double x, double ex, const double
     return 1.0 - ex * p;
 }
 
+// erf(x) for x >= 0 (not NaN) without the library fallback: the Maclaurin
+// series below 1/2, 1 - exp(-x^2) erfcx(x) up to 6, 1 beyond
+__device__ __forceinline__ double erf_from_exp_pos(double x, double ex, const double* __restrict__ coef) {
+    if (x < 0.5) return erf_small(x);
+    if (!(x < kErfcxHi)) return 1.0;
+    const int k = (int)(x * 4.0);
+    const double t = (x - (k + 0.5) * 0.25) * 8.0;
+    const double* c = coef + k * kErfcxTerms;
+    double p = c[kErfcxTerms - 1];
+#pragma unroll
+    for (int i = kErfcxTerms - 2; i >= 0; --i) p = fma(p, t, c[i]);
+    return 1.0 - ex * p;
+}
+
 // the whole workgroup copies the table into s (kErfcxPieces * kErfcxTerms
 // doubles of LDS); the caller synchronises before use
 __device__ __forceinline__ void load_erfcx_table(double* s) {
