@@ -504,7 +504,7 @@ __device__ __forceinline__ void block_force_add(double fx, double fy, double fz,
 // of the nval values
 template <int NVAL>
 __device__ __forceinline__ void block_partials(const double (&vals)[13], double (*s_red)[13],
-                                               double* __restrict__ partial) {
+                                               double* __restrict__ partial, int slot) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < NVAL; ++c) {
@@ -516,43 +516,36 @@ __device__ __forceinline__ void block_partials(const double (&vals)[13], double 
         double s = 0.0;
         if (threadIdx.x < NVAL)
             for (int w = 0; w < 4; ++w) s += s_red[w][threadIdx.x];
-        partial[blockIdx.x * 16 + threadIdx.x] = s;
+        partial[slot * 16 + threadIdx.x] = s;
     }
 }
 
-// metal side of the metal-water terms: one workgroup per metal atom;
-// partial = e10, e11, virial
-__global__ void __launch_bounds__(256)
-k_cpim_mw_metal(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
-                const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
-                const double* __restrict__ x, double* __restrict__ f, double* __restrict__ partial) {
-    __shared__ double s_red[4][13];
-    __shared__ double s_f[4][3];
-    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
-    load_erfcx_table(s_erfcx);
-    __syncthreads();
-    const int i = blockIdx.x;
+// metal side of the metal-water terms for metal atom i (one workgroup);
+// partial[i] = e10, e11, virial
+__device__ __forceinline__ void mw_metal_block(const double* __restrict__ r, const double* __restrict__ charge,
+                                               const BoxConst& bc, const int* __restrict__ n_front,
+                                               const int* __restrict__ nbr, int stride, const double* __restrict__ x,
+                                               double* __restrict__ f, double* __restrict__ partial, int i,
+                                               const double* s_erfcx, double (*s_red)[13], double (*s_f)[3]) {
     double fx = 0.0, fy = 0.0, fz = 0.0, vals[13];
 #pragma unroll
     for (int c = 0; c < 13; ++c) vals[c] = 0.0;
     mw_metal_terms(r, charge, bc, n_front, nbr, stride, x, i, threadIdx.x, fx, fy, fz, vals, s_erfcx);
     block_force_add(fx, fy, fz, s_f, i, bc.N, f);
-    block_partials<11>(vals, s_red, partial);
+    block_partials<11>(vals, s_red, partial, i);
 }
 
-// atom side of the metal-water terms: the wavefronts stride over the
-// non-metal atoms that have metal neighbours (listed by the neighbour build;
-// most atoms have none), one atom per wavefront at a time
-__global__ void __launch_bounds__(256)
-k_cpim_mw_atom(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
-               const int* __restrict__ n_back, const int* __restrict__ nbr, int stride,
-               const int* __restrict__ tail_list, const double* __restrict__ x, double* __restrict__ f) {
-    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
-    load_erfcx_table(s_erfcx);
-    __syncthreads();
+// atom side of the metal-water terms, workgroup b of nb: the wavefronts
+// stride over the non-metal atoms that have metal neighbours (listed by the
+// neighbour build; most atoms have none), one atom per wavefront at a time
+__device__ __forceinline__ void mw_atom_block(const double* __restrict__ r, const double* __restrict__ charge,
+                                              const BoxConst& bc, const int* __restrict__ n_back,
+                                              const int* __restrict__ nbr, int stride,
+                                              const int* __restrict__ tail_list, const double* __restrict__ x,
+                                              double* __restrict__ f, int b, int nb, const double* s_erfcx) {
     const int count = tail_list[0];
-    const int nw = gridDim.x * 4;
-    for (int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)); q < count; q += nw)
+    const int nw = nb * 4;
+    for (int q = __builtin_amdgcn_readfirstlane(b * 4 + (threadIdx.x >> 6)); q < count; q += nw)
         mw_atom_terms(r, charge, bc, n_back, nbr, stride, x, tail_list[1 + q], threadIdx.x & 63, f, s_erfcx);
 }
 
@@ -580,18 +573,25 @@ constexpr int kMMWaves = 4;
 constexpr int kMMSteps = kTileEdge / kMMWaves;
 constexpr int kMMBlk = 3 * kTileEdge;          // doubles of one partial
 
-__global__ void __launch_bounds__(kMMWaves * 64)
-k_cpim_mm_tiles(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int Nb,
-                double* __restrict__ P, double* __restrict__ partial) {
-    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
-    __shared__ double s_j[7][kTileEdge];                // block J: x, y, z (nm), q, p (3)
-    __shared__ double s_col[kMMWaves][3][kTileEdge];
-    __shared__ double s_row[kMMWaves][3][kTileEdge];
-    __shared__ double s_red[kMMWaves][13];
+struct MMShared {
+    double j[7][kTileEdge];                 // block J: x, y, z (nm), q, p (3)
+    double col[kMMWaves][3][kTileEdge];
+    double row[kMMWaves][3][kTileEdge];
+    double red[kMMWaves][13];
+};
+
+// tile t (the caller has staged the erfcx table and synchronised)
+__device__ __forceinline__ void mm_tile_block(const double* __restrict__ r, const BoxConst& bc,
+                                              const double* __restrict__ x, int Nb, double* __restrict__ P,
+                                              double* __restrict__ partial, int t_slot, const double* s_erfcx,
+                                              MMShared& sh) {
+    auto& s_j = sh.j;
+    auto& s_col = sh.col;
+    auto& s_row = sh.row;
+    auto& s_red = sh.red;
     const int N = bc.N, Nm = bc.Nm;
     int I, J;
-    tile_coords(blockIdx.x, Nb, I, J);
-    load_erfcx_table(s_erfcx);
+    tile_coords(t_slot, Nb, I, J);
     if (threadIdx.x < kTileEdge) {
         const int j = J * kTileEdge + threadIdx.x;
         const bool ok = j < Nm;
@@ -667,8 +667,40 @@ k_cpim_mm_tiles(const double* __restrict__ r, BoxConst bc, const double* __restr
         double t = 0.0;
         if (c < 13)
             for (int w = 0; w < kMMWaves; ++w) t += s_red[w][c];
-        partial[blockIdx.x * 16 + c] = t;
+        partial[t_slot * 16 + c] = t;
     }
+}
+
+// The CPIM force terms in one launch: workgroups [0, n_tiles) the
+// metal-metal tiles (partials for k_cpim_mm_sum, energies to part_mm),
+// [n_tiles, n_tiles + n_metal) the metal side of the metal-water terms (one
+// per metal atom; forces added to f, energies to part_mw), then n_atom
+// workgroups for the atom side (forces added to the non-metal atoms).  The
+// three write different things (the tiles touch no force), so they share the
+// compute units instead of running one after the other.
+__global__ void __launch_bounds__(kMMWaves * 64)
+k_cpim_terms(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+             const double* __restrict__ x, int Nb, double* __restrict__ P, double* __restrict__ part_mm,
+             int n_tiles, const int* __restrict__ n_front, const int* __restrict__ n_back,
+             const int* __restrict__ nbr, int stride, const int* __restrict__ tail_list, double* __restrict__ f,
+             double* __restrict__ part_mw, int n_metal, int n_atom) {
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ MMShared sh;
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    int b = blockIdx.x;
+    if (b < n_tiles) {
+        mm_tile_block(r, bc, x, Nb, P, part_mm, b, s_erfcx, sh);
+        return;
+    }
+    b -= n_tiles;
+    if (b < n_metal) {
+        mw_metal_block(r, charge, bc, n_front, nbr, stride, x, f, part_mw, b, s_erfcx, sh.red,
+                       reinterpret_cast<double (*)[3]>(&sh.row[0][0][0]));
+        return;
+    }
+    b -= n_metal;
+    mw_atom_block(r, charge, bc, n_back, nbr, stride, tail_list, x, f, b, n_atom, s_erfcx);
 }
 
 // The metal atoms' CPIM forces from the tiles' partials: one workgroup of 256
@@ -685,8 +717,16 @@ k_cpim_mm_sum(const double* __restrict__ r, BoxConst bc, const double* __restric
     const int N = bc.N, Nm = bc.Nm, B = blockIdx.x;
     if (threadIdx.x < kMMBlk) {
         const int c = threadIdx.x >> 6, l = threadIdx.x & 63, i = B * kTileEdge + l;
+        // (the loads of a chunk issued together, the sum in order)
         double t = 0.0;
-        for (int S = 0; S < Nb; ++S) t += P[((size_t)B * Nb + S) * kMMBlk + threadIdx.x];
+        for (int S0 = 0; S0 < Nb; S0 += 16) {
+            double pv[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                pv[q] = S0 + q < Nb ? P[((size_t)B * Nb + S0 + q) * kMMBlk + threadIdx.x] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) t += pv[q];
+        }
         if (i < Nm) {
             t += bc.efield[c] * x[3 * Nm + i];
             if (extra) t += extra[(size_t)c * Nm + i];
@@ -790,32 +830,32 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 // the metal-metal terms for x: the tile launch and the per-block sums
 // (adding extra, 3 x N_m or NULL, with them); reduce: queue the energies'
 // reduction (red[48..61))
-static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce) {
-    const int Nb = tile_blocks(c->bc.Nm);
+// mw: the metal-water terms in the same launch
+static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce, bool mw) {
+    const int Nm = c->bc.Nm, N = c->bc.N;
+    const int Nb = tile_blocks(Nm);
     const int n_tiles = (int)tile_count(Nb);
+    const int n_metal = mw ? Nm : 0;
+    const int n_atom = mw && N > Nm ? (int)mw_atom_blocks(N - Nm) : 0;
     double* part = partial_seg(c, SEG_MM);
-    k_cpim_mm_tiles<<<n_tiles, kMMWaves * 64, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, part);
+    k_cpim_terms<<<n_tiles + n_metal + n_atom, kMMWaves * 64, 0, c->stream>>>(
+        c->r, c->charge, c->bc, x, Nb, c->mm_part, part, n_tiles, c->n_front, c->n_back, c->nbr, c->nb_stride,
+        c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom);
     k_cpim_mm_sum<<<Nb, 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, extra, c->f, part, n_tiles);
     CHECK_HIP(hipGetLastError());
-    return reduce ? reduce_partials(c, SEG_MM, n_tiles + Nb, 13, 48) : 0;
+    if (!reduce) return 0;
+    if (mw) {
+        int rc = reduce_partials(c, SEG_MW, Nm, 11, 32);
+        if (rc) return rc;
+    }
+    return reduce_partials(c, SEG_MM, n_tiles + Nb, 13, 48);
 }
 
 // adds the CPIM forces for x (device) to c->f;
 // red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
-    int Nm = c->bc.Nm, N = c->bc.N;
-    if (Nm == 0) return 0;
-    int nblk = Nm;                      // one workgroup per metal atom
-    k_cpim_mw_metal<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
-                                                 c->nb_stride, x, c->f, partial_seg(c, SEG_MW));
-    CHECK_HIP(hipGetLastError());
-    int rc = reduce_partials(c, SEG_MW, nblk, 11, 32);
-    if (rc) return rc;
-    if (N > Nm)
-        k_cpim_mw_atom<<<mw_atom_blocks(N - Nm), 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_back,
-                                                                        c->nbr, c->nb_stride, c->tail_list, x,
-                                                                        c->f);
-    return cpim_mm_launch(c, x, f_sc, true);
+    if (c->bc.Nm == 0) return 0;
+    return cpim_mm_launch(c, x, f_sc, true, true);
 }
 
 // (in a force call the Sutton-Chen force waits in f_sc)
@@ -824,5 +864,5 @@ int cpim_forces(capac_ctx* c) { return cpim_forces_x(c, c->x, c->f_sc); }
 // the metal-metal CPIM force launches alone, adding to c->f (for kernel timing)
 int cpim_mm_only(capac_ctx* c) {
     if (c->bc.Nm == 0) return 0;
-    return cpim_mm_launch(c, c->x, NULL, false);
+    return cpim_mm_launch(c, c->x, NULL, false, false);
 }
