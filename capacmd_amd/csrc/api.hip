@@ -943,9 +943,10 @@ static int force_compute(capac_ctx* c, bool deferred = false) {
         if (!side_after_matrix && ((rc = side_forces(c)) || (rc = sc_main()))) return rc;
         int solve_rc = 0;
         if (c->n_mat) {
-            if ((rc = cpim_rhs(c))) return rc;
+            // (one launch for both on the FP32 path's TILES layout: the RHS
+            // span then covers the relay too)
+            if ((rc = cpim_rhs_assemble(c))) return rc;
             mark(c, 4);
-            if ((rc = cpim_assemble(c))) return rc;
             mark(c, 5);
             if (side_after_matrix && ((rc = side_forces(c)) || (rc = sc_main()))) return rc;
             solve_rc = solver_run(c, c->b, c->x);

@@ -302,6 +302,7 @@ int sc_forces(capac_ctx* c, double* f_sc = NULL);
 // cpim.hip
 int cpim_rhs(capac_ctx* c);
 int cpim_assemble(capac_ctx* c);
+int cpim_rhs_assemble(capac_ctx* c);       // both, in one launch where the layout allows
 int cpim_forces(capac_ctx* c);
 // f_sc (3 x N_m, or NULL): forces the metal-metal launch adds with its own
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc = NULL);

@@ -21,19 +21,14 @@ namespace {
 // ------------------------------------------------------------------ RHS
 // One workgroup per metal atom over the non-metal front of its row (four
 // wavefronts split the row, as in the metal-side force kernels).
-__global__ void __launch_bounds__(256)
-k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
-      const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
-      const double* __restrict__ np_charge, double* __restrict__ b) {
-    main_stream_priority();
-    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
-    __shared__ double s_w[4][4];
+// (metal atom i's rows of b; the caller has staged the erfcx table)
+__device__ __forceinline__ void rhs_block(const double* __restrict__ r, const double* __restrict__ charge,
+                                          const BoxConst& bc, const int* __restrict__ n_front,
+                                          const int* __restrict__ nbr, int stride,
+                                          const double* __restrict__ np_charge, double* __restrict__ b, int i,
+                                          const double* s_erfcx, double (*s_w)[4]) {
     const int N = bc.N, Nm = bc.Nm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x;
-    load_erfcx_table(s_erfcx);
-    __syncthreads();
-    if (i >= Nm) return;
     double xi = r[i], yi = r[N + i], zi = r[2 * N + i];
     int nf = n_front[i];
     const int* row = nbr + (size_t)i * stride;
@@ -79,6 +74,19 @@ k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst 
     }
 }
 
+__global__ void __launch_bounds__(256)
+k_rhs(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+      const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+      const double* __restrict__ np_charge, double* __restrict__ b) {
+    main_stream_priority();
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ double s_w[4][4];
+    load_erfcx_table(s_erfcx);
+    __syncthreads();
+    if ((int)blockIdx.x >= bc.Nm) return;
+    rhs_block(r, charge, bc, n_front, nbr, stride, np_charge, b, blockIdx.x, s_erfcx, s_w);
+}
+
 // ------------------------------------------------------------------ relay
 // FP32-refined relay: the four coefficient planes of relay.h, T-tensors in
 // FP32 from the separation formed in FP64 with the reference's pbc_12 and
@@ -113,13 +121,12 @@ k_assemble32(const double* __restrict__ r, BoxConst bc, int ld, float* __restric
 // once, from metal_sep(i, j) with i in the tile's row block (the reversed
 // separation is its exact negative, and the coefficients depend on r^2
 // only).  Slot k = 0 of a diagonal tile (i = j) and pairs past Nm hold zeros.
-__global__ void __launch_bounds__(256)
-k_assemble32_tiles(const double* __restrict__ r, BoxConst bc, int Nb, float4* __restrict__ tiles) {
-    main_stream_priority();
+__device__ __forceinline__ void assemble_tile_block(const double* __restrict__ r, const BoxConst& bc, int Nb,
+                                                    float4* __restrict__ tiles, int t) {
     const int Nm = bc.Nm;
     int I, J;
-    tile_coords(blockIdx.x, Nb, I, J);
-    float4* dst = tiles + (size_t)blockIdx.x * kTileSlots;
+    tile_coords(t, Nb, I, J);
+    float4* dst = tiles + (size_t)t * kTileSlots;
     for (int e = threadIdx.x; e < kTileSlots; e += blockDim.x) {
         const int k = e / kTileEdge, l = e % kTileEdge;
         const int i = I * kTileEdge + l, j = J * kTileEdge + ((l + k) & (kTileEdge - 1));
@@ -136,16 +143,20 @@ k_assemble32_tiles(const double* __restrict__ r, BoxConst bc, int Nb, float4* __
     }
 }
 
+__global__ void __launch_bounds__(256)
+k_assemble32_tiles(const double* __restrict__ r, BoxConst bc, int Nb, float4* __restrict__ tiles) {
+    main_stream_priority();
+    assemble_tile_block(r, bc, Nb, tiles, blockIdx.x);
+}
+
 // Metal separations from metal atom 0, d_j = pbc_12(r_j - r_0) in bohr, and
 // whether they are compact: when every component's spread max_j d_j -
 // min_j d_j is below half the box edge, every pair's minimum-image
 // separation is d_j - d_i (pbc_12 leaves a difference inside half the box
 // alone), and the FP32 product applies its blocks from d without the
 // per-pair image test.  One workgroup; the flag lands at u[4 Nm].
-__global__ void __launch_bounds__(1024)
-k_metal_unwrap(const double* __restrict__ r, BoxConst bc, double* __restrict__ u) {
-    main_stream_priority();
-    __shared__ double s_lo[3][16], s_hi[3][16];
+__device__ __forceinline__ void metal_unwrap_block(const double* __restrict__ r, const BoxConst& bc,
+                                                   double* __restrict__ u, double (*s_lo)[16], double (*s_hi)[16]) {
     const int N = bc.N, Nm = bc.Nm;
     const double x0 = r[0], y0 = r[N], z0 = r[2 * N];
     double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
@@ -178,6 +189,38 @@ k_metal_unwrap(const double* __restrict__ r, BoxConst bc, double* __restrict__ u
             compact = compact && (b - a) < bc.half[k] * (1.0 - 1e-9);
         }
         u[4 * Nm] = compact ? 1.0 : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_metal_unwrap(const double* __restrict__ r, BoxConst bc, double* __restrict__ u) {
+    main_stream_priority();
+    __shared__ double s_lo[3][16], s_hi[3][16];
+    metal_unwrap_block(r, bc, u, s_lo, s_hi);
+}
+
+// The right-hand side and the FP32 relay's TILES layout in one launch
+// (they read the positions only and write different arrays): workgroup 0
+// the metal unwrap, then the n_tiles relay tiles (the longest-running
+// workgroups, so they start early), then one per metal atom for b.
+__global__ void __launch_bounds__(256)
+k_matrix_terms(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
+               const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
+               const double* __restrict__ np_charge, double* __restrict__ b, int Nb, int n_tiles,
+               float4* __restrict__ tiles, double* __restrict__ u) {
+    main_stream_priority();
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ double s_w[4][4];
+    __shared__ double s_lo[3][16], s_hi[3][16];
+    const int blk = blockIdx.x;
+    if (blk == 0) {
+        metal_unwrap_block(r, bc, u, s_lo, s_hi);
+    } else if (blk <= n_tiles) {
+        assemble_tile_block(r, bc, Nb, tiles, blk - 1);
+    } else {
+        load_erfcx_table(s_erfcx);
+        __syncthreads();
+        rhs_block(r, charge, bc, n_front, nbr, stride, np_charge, b, blk - 1 - n_tiles, s_erfcx, s_w);
     }
 }
 
@@ -772,6 +815,29 @@ int cpim_rhs(capac_ctx* c) {
     return 0;
 }
 
+static int assemble_finish(capac_ctx* c);
+
+// b and the relay of a force call: one launch on the FP32 path's TILES
+// layout, else cpim_rhs and cpim_assemble
+int cpim_rhs_assemble(capac_ctx* c) {
+    const int Nm = c->bc.Nm;
+    if (Nm == 0) return 0;
+    if (c->precision != CAPAC_PREC_FP32_REFINED || c->relay_layout != CAPAC_LAYOUT_TILES) {
+        int rc = cpim_rhs(c);
+        return rc ? rc : cpim_assemble(c);
+    }
+    if (c->bc.n_np > Nm) {
+        capac_set_error("more nanoparticles (%d) than metal atoms (%d)", c->bc.n_np, Nm);
+        return CAPAC_ERR_ARG;
+    }
+    const int n_tiles = (int)tile_count(c->relay_nb);
+    k_matrix_terms<<<1 + n_tiles + Nm, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
+                                                            c->nb_stride, c->np_charge, c->b, c->relay_nb, n_tiles,
+                                                            reinterpret_cast<float4*>(c->relay32), c->metal_u);
+    CHECK_HIP(hipGetLastError());
+    return assemble_finish(c);
+}
+
 int cpim_assemble(capac_ctx* c) {
     int Nm = c->bc.Nm;
     if (Nm == 0) return 0;
@@ -785,6 +851,11 @@ int cpim_assemble(capac_ctx* c) {
             k_assemble32<<<grid, 256, 0, c->stream>>>(c->r, c->bc, c->relay_ld, c->relay32);
         k_metal_unwrap<<<1, 1024, 0, c->stream>>>(c->r, c->bc, c->metal_u);
     }
+    return assemble_finish(c);
+}
+
+// the diagonal and the block preconditioner after the relay's launch
+static int assemble_finish(capac_ctx* c) {
     // the Jacobi diagonal holds constants only (1/alpha, -1/c, 1): formed once
     // and again only when those constants change (fitting parameters)
     if (!c->diag_set || c->diag_polar != c->bc.inv_polar || c->diag_capac != c->bc.inv_capac) {
