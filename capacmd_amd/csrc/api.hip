@@ -35,28 +35,9 @@ struct ReduceJobs {
 // per-block partials [nblk x 16] into red
 __global__ void k_reduce(const double* __restrict__ partial_all, ReduceJobs jobs, double* __restrict__ red) {
     __shared__ double s_w[16][16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int job = blockIdx.x;
-    const double* partial = partial_all + (size_t)jobs.part_off[job] * 16;
-    const int nblk = jobs.nblk[job], nval = jobs.nval[job];
-    double* out = red + jobs.red_off[job];
-    double acc[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) acc[v] += partial[b * 16 + v];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        double s = wave_sum(acc[v]);
-        if (lane == 0) s_w[wave][v] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < nval) {
-        double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w][threadIdx.x];
-        out[threadIdx.x] = t;
-    }
+    reduce_rows_block(partial_all + (size_t)jobs.part_off[job] * 16, jobs.nblk[job], jobs.nval[job],
+                      red + jobs.red_off[job], s_w, NULL);
 }
 
 __global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
@@ -91,6 +72,10 @@ int flush_reductions(capac_ctx* c) {
     c->n_pending = 0;
     CHECK_HIP(hipGetLastError());
     return 0;
+}
+
+int reduce_energies(capac_ctx* c, int seg, int nblk, int nval, int off) {
+    return c->need_energy ? reduce_partials(c, seg, nblk, nval, off) : 0;
 }
 
 int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int off) {

@@ -356,7 +356,7 @@ int bonded_forces(capac_ctx* c) {
                                                       c->angle_funct, c->angle_par, c->dihedral_atoms,
                                                       c->dihedral_par, c->f, partial_seg(c, SEG_BONDED));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_BONDED, (int)nblk, 14, 112);
+    return reduce_energies(c, SEG_BONDED, (int)nblk, 14, 112);
 }
 
 int vsite_build(capac_ctx* c) {

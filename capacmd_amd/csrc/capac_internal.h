@@ -287,6 +287,34 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// The fixed-order sums of nval (<= 16) values over nblk partial rows of 16
+// doubles, by one whole workgroup (blockDim.x a multiple of 64, at most
+// 1024): thread t adds rows t, t + blockDim.x, ... in turn, the wavefronts
+// sum their threads, and the wavefronts' sums are added in order.  Threads
+// t < nval write the sum to out[t] and to s_out[t] (LDS).
+__device__ __forceinline__ void reduce_rows_block(const double* __restrict__ partial, int nblk, int nval,
+                                                  double* __restrict__ out, double (*s_w)[16], double* s_out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] += partial[b * 16 + v];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        double s = wave_sum(acc[v]);
+        if (lane == 0) s_w[wave][v] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < nval) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w][threadIdx.x];
+        out[threadIdx.x] = t;
+        if (s_out) s_out[threadIdx.x] = t;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 // neighbour.hip
@@ -371,6 +399,9 @@ int md_fmax(capac_ctx* c);
 enum PartialSeg { SEG_PAIR, SEG_SC, SEG_MW, SEG_MM, SEG_RATTLE, SEG_MOM, SEG_BONDED, SEG_COUNT };
 double* partial_seg(capac_ctx* c, int seg);
 int reduce_partials(capac_ctx* c, int seg, int nblk, int nval, int red_offset);
+// the same for sums that only energies and the virial need: nothing when the
+// force call may leave them unformed (need_energy 0)
+int reduce_energies(capac_ctx* c, int seg, int nblk, int nval, int red_offset);
 int flush_reductions(capac_ctx* c);
 int zero_outputs(capac_ctx* c);
 // graph-safe clears (fill kernels, not hipMemsetAsync)

@@ -916,10 +916,10 @@ static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bo
     CHECK_HIP(hipGetLastError());
     if (!reduce) return 0;
     if (mw) {
-        int rc = reduce_partials(c, SEG_MW, Nm, 11, 32);
+        int rc = reduce_energies(c, SEG_MW, Nm, 11, 32);
         if (rc) return rc;
     }
-    return reduce_partials(c, SEG_MM, n_tiles + Nb, 13, 48);
+    return reduce_energies(c, SEG_MM, n_tiles + Nb, 13, 48);
 }
 
 // adds the CPIM forces for x (device) to c->f;

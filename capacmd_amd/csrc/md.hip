@@ -486,8 +486,8 @@ __device__ __forceinline__ void com_kinetic_from_sums(double* __restrict__ md, c
 // formed as com_kinetic_from_sums describes.  The divisions by qMass are
 // products with its reciprocal, and small exponentials use exp_small: both
 // differ from the reference's operations by rounding only.
-__global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, const double* __restrict__ sums) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void nhc_body(double* __restrict__ md, double dt, double kT, int ndf,
+                                         const double* __restrict__ sums) {
     constexpr int M = CAPAC_NHC;
     if (sums) md[0] = sums[4] + sums[5] + sums[6];
     double vQ[M], aQ[M];
@@ -517,6 +517,24 @@ __global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, co
         md[0] = eKsum * (scale * scale);
         md[1] *= scale * scale;
     }
+}
+
+__global__ void k_nhc(double* __restrict__ md, double dt, double kT, int ndf, const double* __restrict__ sums) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    nhc_body(md, dt, kT, ndf, sums);
+}
+
+// The step's second-half chain with the momentum sums' reduction in the same
+// launch (when no other reduction is queued): the workgroup forms red[96..103)
+// exactly as k_reduce does, then one thread runs the chain on them.
+__global__ void __launch_bounds__(kRedThreads)
+k_nhc_sums(double* __restrict__ md, double dt, double kT, int ndf, const double* __restrict__ partial, int nblk,
+           double* __restrict__ sums_out) {
+    __shared__ double s_w[16][16];
+    __shared__ double s_sums[16];
+    reduce_rows_block(partial, nblk, 7, sums_out, s_w, s_sums);
+    __syncthreads();
+    if (threadIdx.x == 0) nhc_body(md, dt, kT, ndf, s_sums);
 }
 
 // the same without a thermostat (scale 1)
@@ -619,7 +637,7 @@ int md_rattle2(capac_ctx* c) {
                                            c->mol_first, c->mol_last, c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
                                            c->md_status, partial_seg(c, SEG_RATTLE));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_RATTLE, nblk, 9, 64);
+    return reduce_energies(c, SEG_RATTLE, nblk, 9, 64);
 }
 
 // red[96..103) = momentum / kinetic sums of the current velocities
@@ -670,6 +688,20 @@ int md_first_half(capac_ctx* c, double kT) {
 // and one pass applying the scale and the removal.
 int md_second_half(capac_ctx* c, double kT) {
     if (!c->nvt) return md_remove_com_kinetic(c);
+    if (c->n_pending == 0) {
+        // nothing else queued (a step that forms no energies): the chain's
+        // launch reduces the sums itself
+        int nblk = grid(c->bc.N, 256);
+        if (nblk > c->seg_cap[SEG_MOM]) {
+            capac_set_error("internal: reduction of %d blocks exceeds segment %d", nblk, SEG_MOM);
+            return CAPAC_ERR_HIP;
+        }
+        k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM));
+        k_nhc_sums<<<1, kRedThreads, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, partial_seg(c, SEG_MOM), nblk,
+                                                     c->red + 96);
+        CHECK_HIP(hipGetLastError());
+        return scale_com(c, 1);
+    }
     int rc = mom_sums(c);
     if (rc) return rc;
     k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, c->red + 96);

@@ -551,14 +551,12 @@ int pair_nonbonded(capac_ctx* c, double* f_accumulate, int max_blocks, int field
             nparts += kPairRestBlocks;
         }
         CHECK_HIP(hipGetLastError());
-        // without energies red[0..12) keeps an earlier call's sums, which
-        // nothing reads
-        return c->need_energy ? reduce_partials(c, SEG_PAIR, nparts, 12, 0) : 0;
+        return reduce_energies(c, SEG_PAIR, nparts, 12, 0);
     } else {
         k_pair<<<nblk, 256, 0, c->stream>>>(c->r, c->charge, c->type, c->pair, c->bc, c->n_front,
                                             c->n_back, c->nbr, c->nb_stride, f,
                                             f_accumulate ? 1 : 0, field, partial_seg(c, SEG_PAIR));
     }
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_PAIR, nblk, 12, 0);
+    return reduce_energies(c, SEG_PAIR, nblk, 12, 0);
 }

@@ -114,5 +114,5 @@ int sc_forces(capac_ctx* c, double* f_sc) {
                                             f_sc ? f_sc : c->f, f_sc ? c->bc.Nm : c->bc.N, f_sc ? 0 : 1,
                                             partial_seg(c, SEG_SC));
     CHECK_HIP(hipGetLastError());
-    return reduce_partials(c, SEG_SC, nblk, 10, 16);
+    return reduce_energies(c, SEG_SC, nblk, 10, 16);
 }
