@@ -115,8 +115,9 @@ def pair_valu_roofline(path, deck):
     entries = path.get("pair_row_entries")
     if not ms or not entries:
         return None
-    out = {"kernels": "one force call's pair launches, alone on the full grid (k_pack_xyzq, k_pair_water, "
-                      "k_pair_rest on water decks)",
+    out = {"kernels": "one force call's pair launches, alone on the full grid (k_pack_xyzq and "
+                      "k_pair_water<true> on the Au/water decks; k_pair_rest too where a pair type is neither "
+                      "funct 1 nor 4)",
            "bound": "fp64-valu", "ms_per_launch": ms, "row_entries": entries,
            "G_entries_per_s": round(entries / (ms * 1e-3) / 1e9, 1),
            "algorithmic_bytes": 4 * entries + 60 * deck.n_atoms,
@@ -125,8 +126,8 @@ def pair_valu_roofline(path, deck):
     try:
         with open(os.path.join(PROFILES, "valu.json")) as fh:
             v = json.load(fh)
-        if int(v.get("atoms", -1)) == deck.n_atoms and "k_pair_water" in v and "k_pair_rest" in v:
-            instr = v["k_pair_water"]["valu_instr"] + v["k_pair_rest"]["valu_instr"]
+        if int(v.get("atoms", -1)) == deck.n_atoms and "k_pair_water<true>" in v:
+            instr = v["k_pair_water<true>"]["valu_instr"] + v.get("k_pair_rest", {}).get("valu_instr", 0.0)
             out["valu_instr_per_launch"] = instr
             out["achieved_G_instr_per_s"] = round(instr / (ms * 1e-3) / 1e9, 1)
             out["frac"] = round(out["achieved_G_instr_per_s"] / FP64_ISSUE_PEAK_G, 4)
