@@ -231,10 +231,12 @@ struct capac_ctx {
 
 // pbc_12 (force.c:111-121)
 __device__ __forceinline__ void pbc12(const BoxConst& b, double& x, double& y, double& z) {
-    // x < -half: x + L; x > half: x - L (one magnitude test per component)
-    if (fabs(x) > b.half[0]) x = x < 0.0 ? x + b.box[0] : x - b.box[0];
-    if (fabs(y) > b.half[1]) y = y < 0.0 ? y + b.box[1] : y - b.box[1];
-    if (fabs(z) > b.half[2]) z = z < 0.0 ? z + b.box[2] : z - b.box[2];
+    // x < -half: x + L; x > half: x - L, formed as x - copysign(L, x) (one
+    // magnitude test and one subtraction per component; x - (-L) rounds as
+    // x + L, so the results are the same bits)
+    x = fabs(x) > b.half[0] ? x - copysign(b.box[0], x) : x;
+    y = fabs(y) > b.half[1] ? y - copysign(b.box[1], y) : y;
+    z = fabs(z) > b.half[2] ? z - copysign(b.box[2], z) : z;
 }
 
 // 1/sqrt(s) for a positive finite s: the hardware estimate and one
