@@ -338,11 +338,12 @@ int cpim_forces(capac_ctx* c);
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc = NULL);
 int cpim_dense_expand(capac_ctx* c, double* dense_dev);
 // out = b - A x (b non-NULL) or A x, with the FP64 relay recomputed on the fly
-// (no stored matrix); part[g] = (out, out) over the rows of row-group g.
-// With a non-NULL guard the launch does nothing unless that solve has
-// finished with kind converged or loosened.
+// (no stored matrix); part[g] = (out, out) over the rows of row-group g,
+// *n_part of them (row-groups: metal atoms, or blocks of 64 on the TILES
+// layout, then the Lagrange rows).  With a non-NULL guard the launches do
+// nothing unless that solve has finished with kind converged or loosened.
 int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part,
-                 const SolveScalars* guard);
+                 const SolveScalars* guard, int* n_part = NULL);
 // precond.hip
 size_t bj_bytes(int Nm);                    // the inverses' size
 int bj_alloc_dev(capac_ctx* c);             // (re)allocates for bj_every (none when 0)

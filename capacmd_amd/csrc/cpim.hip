@@ -295,6 +295,148 @@ k_apply64(const double* __restrict__ r, BoxConst bc, const int* __restrict__ np_
     if (part) part[g] = acc;
 }
 
+// The same product on the TILES layout's tiles, each unordered metal pair
+// once: one workgroup per tile (I, J), four wavefronts of sixteen diagonals;
+// lane l applies block (i, j) to x_j for its row i (registers) and block
+// (j, i) -- the same coefficients at separation -r -- to x_i for row j (the
+// wavefront's column sums in LDS; the 64 lanes of a step meet 64 different
+// j).  A diagonal tile takes every ordered pair of its block for the rows
+// only (k = 1..63), as the FP32 product does.  The four wavefronts' sums go
+// to the partials P[I][J] / P[J][I] (4 x 64 doubles, the FP32 product's
+// layout), which k_apply64_sum adds.
+__global__ void __launch_bounds__(256)
+k_apply64_tiles(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int Nb,
+                double* __restrict__ P, const SolveScalars* __restrict__ guard) {
+    main_stream_priority();
+    __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
+    __shared__ double s_j[7][kTileEdge];            // block J: positions (nm) and x (4)
+    __shared__ double s_col[4][4][kTileEdge];
+    __shared__ double s_row[4][4][kTileEdge];
+    if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
+        return;
+    const int N = bc.N, Nm = bc.Nm;
+    int I, J;
+    tile_coords(blockIdx.x, Nb, I, J);
+    load_erfcx_table(s_erfcx);
+    if (threadIdx.x < kTileEdge) {
+        const int j = J * kTileEdge + threadIdx.x;
+        const bool ok = j < Nm;
+        s_j[0][threadIdx.x] = ok ? r[j] : 0.0;
+        s_j[1][threadIdx.x] = ok ? r[N + j] : 0.0;
+        s_j[2][threadIdx.x] = ok ? r[2 * N + j] : 0.0;
+        s_j[3][threadIdx.x] = ok ? x[3 * j] : 0.0;
+        s_j[4][threadIdx.x] = ok ? x[3 * j + 1] : 0.0;
+        s_j[5][threadIdx.x] = ok ? x[3 * j + 2] : 0.0;
+        s_j[6][threadIdx.x] = ok ? x[3 * Nm + j] : 0.0;
+    }
+    for (int k = threadIdx.x; k < 4 * 4 * kTileEdge; k += blockDim.x) (&s_col[0][0][0])[k] = 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = I * kTileEdge + lane;
+    const bool iok = i < Nm;
+    const double xi = iok ? r[i] : 0.0, yi = iok ? r[N + i] : 0.0, zi = iok ? r[2 * N + i] : 0.0;
+    const double vi[4] = {iok ? x[3 * i] : 0.0, iok ? x[3 * i + 1] : 0.0, iok ? x[3 * i + 2] : 0.0,
+                          iok ? x[3 * Nm + i] : 0.0};
+    const bool diag = I == J;
+    double orow[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int s = 0; s < 16; ++s) {
+        const int k = wave * 16 + s;
+        const int jl = (lane + k) & (kTileEdge - 1);
+        if (!iok || J * kTileEdge + jl >= Nm || (diag && k == 0)) continue;
+        double dx = s_j[0][jl] - xi, dy = s_j[1][jl] - yi, dz = s_j[2][jl] - zi;
+        pbc12(bc, dx, dy, dz);
+        dx *= K_NM2BOHR; dy *= K_NM2BOHR; dz *= K_NM2BOHR;
+        const double r2 = dist2_exact(dx, dy, dz);
+        const TCoef t = t_coef_fast(bc, r2, s_erfcx);
+        const double ga = t.c2 - 3.0 * t.c1, gb = t.c1 * r2;
+        block32_apply(dx, dy, dz, ga, gb, t.c3, t.cq, s_j[3][jl], s_j[4][jl], s_j[5][jl], s_j[6][jl], orow);
+        if (!diag) {
+            double oc[4] = {0.0, 0.0, 0.0, 0.0};
+            block32_apply(-dx, -dy, -dz, ga, gb, t.c3, t.cq, vi[0], vi[1], vi[2], vi[3], oc);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_col[wave][c][jl] += oc[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_row[wave][c][lane] = orow[c];
+    __syncthreads();
+    const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double rs = 0.0, cs = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        rs += s_row[w][c][l];
+        cs += s_col[w][c][l];
+    }
+    const size_t blk = 4 * kTileEdge;
+    P[((size_t)I * Nb + J) * blk + threadIdx.x] = rs;
+    if (!diag) P[((size_t)J * Nb + I) * blk + threadIdx.x] = cs;
+}
+
+// The rows' outputs from the tiles' partials: workgroup B < Nb the 4 x 64
+// rows of metal block B (thread (c, l): the Nb partials in order, the
+// diagonal term, the Lagrange column), then one workgroup per nanoparticle's
+// Lagrange row; out = b - A x (b non-NULL) or A x, part[g] = (out, out) over
+// workgroup g's rows.
+__global__ void __launch_bounds__(256)
+k_apply64_sum(BoxConst bc, const int* __restrict__ np_of, const int* __restrict__ np_first,
+              const int* __restrict__ np_last, const double* __restrict__ x, const double* __restrict__ b, int Nb,
+              const double* __restrict__ P, double* __restrict__ out, double* __restrict__ part,
+              const SolveScalars* __restrict__ guard) {
+    main_stream_priority();
+    __shared__ double s_w[4];
+    if (guard && !(guard->done && (guard->kind == CAPAC_SOLVE_CONVERGED || guard->kind == CAPAC_SOLVE_LOOSENED)))
+        return;
+    const int Nm = bc.Nm;
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc = 0.0;
+    if (g < Nb) {
+        const int c = wave, i = g * kTileEdge + lane;
+        double o = 0.0;
+        for (int S0 = 0; S0 < Nb; S0 += 16) {
+            double pv[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                pv[q] = S0 + q < Nb ? P[((size_t)g * Nb + S0 + q) * (4 * kTileEdge) + threadIdx.x] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) o += pv[q];
+        }
+        if (i < Nm) {
+            if (c < 3) {
+                o += bc.inv_polar * x[3 * i + c];
+            } else {
+                o += -bc.inv_capac * x[3 * Nm + i];
+                const int np = np_of[i];
+                if (np >= 0) o += x[4 * Nm + np];
+            }
+            const int row = c < 3 ? 3 * i + c : 3 * Nm + i;
+            const double val = b ? b[row] - o : o;
+            out[row] = val;
+            acc = val * val;
+        }
+    } else {
+        const int k = g - Nb;
+        double o3 = 0.0;
+        for (int mi = np_first[k] + (int)threadIdx.x; mi <= np_last[k]; mi += blockDim.x) o3 += x[3 * Nm + mi];
+        o3 = wave_sum(o3);
+        if (lane == 0) s_w[wave] = o3;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            o3 = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            const int row = 4 * Nm + k;
+            const double val = b ? b[row] - o3 : o3;
+            out[row] = val;
+            if (part) part[g] = val * val;
+        }
+        return;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) s_w[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0 && part) part[g] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
 __global__ void k_diag(BoxConst bc, int n_mat, double* __restrict__ diag) {
     main_stream_priority();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -889,11 +1031,22 @@ int cpim_dense_expand(capac_ctx* c, double* dense) {
 }
 
 int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, double* part,
-                 const SolveScalars* guard) {
-    const int groups = c->bc.Nm + c->bc.n_np;
-    if (c->bc.Nm == 0) return 0;
-    k_apply64<<<groups, 256, 0, c->stream>>>(c->r, c->bc, c->np_of_metal, c->np_first, c->np_last, x, b,
-                                             out, part, guard);
+                 const SolveScalars* guard, int* n_part) {
+    const int Nm = c->bc.Nm, n_np = c->bc.n_np;
+    if (n_part) *n_part = 0;
+    if (Nm == 0) return 0;
+    if (c->relay_layout == CAPAC_LAYOUT_TILES && c->relay_part) {
+        // (the FP32 product's partials are free here: no product is in flight)
+        const int Nb = tile_blocks(Nm);
+        k_apply64_tiles<<<(unsigned)tile_count(Nb), 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->relay_part, guard);
+        k_apply64_sum<<<Nb + n_np, 256, 0, c->stream>>>(c->bc, c->np_of_metal, c->np_first, c->np_last, x, b, Nb,
+                                                        c->relay_part, out, part, guard);
+        if (n_part) *n_part = Nb + n_np;
+    } else {
+        k_apply64<<<Nm + n_np, 256, 0, c->stream>>>(c->r, c->bc, c->np_of_metal, c->np_first, c->np_last, x, b,
+                                                    out, part, guard);
+        if (n_part) *n_part = Nm + n_np;
+    }
     CHECK_HIP(hipGetLastError());
     return 0;
 }

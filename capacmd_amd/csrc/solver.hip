@@ -1692,9 +1692,10 @@ static int bicgstab(capac_ctx* c, Solver* S, const double* b, double* x, int fir
         }
         launched += batch;
         if (check64) {
-            int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
+            int n_part = 0;
+            int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc, &n_part);
             if (rc) return rc;
-            k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
+            k_res_finish<<<1, 256, 0, st>>>(S->part_res, n_part, S->res, S->sc);
         }
         CHECK_HIP(hipGetLastError());
         if (S->in_red) {
@@ -1794,9 +1795,10 @@ static int gmres(capac_ctx* c, Solver* S, const double* b, double* x, int first_
                 cg_apply(c, x, S->sc, 2);
             }
             if (check64) {
-                int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc);
+                int n_part = 0;
+                int rc = cpim_apply64(c, x, b, S->rr, S->part_res, S->sc, &n_part);
                 if (rc) return rc;
-                k_res_finish<<<1, 256, 0, st>>>(S->part_res, S->n_groups, S->res, S->sc);
+                k_res_finish<<<1, 256, 0, st>>>(S->part_res, n_part, S->res, S->sc);
             }
             CHECK_HIP(hipGetLastError());
             if (S->in_red) {
@@ -1834,9 +1836,10 @@ static int inner_solve(capac_ctx* c, Solver* S, const double* b, double* x, int 
 
 // FP64 residual rr = b - A64 x with the relay recomputed on the fly; returns (rr, rr)
 static int residual64(capac_ctx* c, Solver* S, const double* b, const double* x, double* rTr) {
-    int rc = cpim_apply64(c, x, b, S->rr, S->part_res, NULL);
+    int n_part = 0;
+    int rc = cpim_apply64(c, x, b, S->rr, S->part_res, NULL, &n_part);
     if (rc) return rc;
-    k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, S->n_groups, S->res, NULL);
+    k_res_finish<<<1, 256, 0, c->stream>>>(S->part_res, n_part, S->res, NULL);
     CHECK_HIP(hipGetLastError());
     CHECK_HIP(hipMemcpyAsync(S->res_host, S->res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHECK_HIP(hipStreamSynchronize(c->stream));
