@@ -1374,7 +1374,16 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         mark(c, 9);
         if (!rc) rc = md_kick(c);
         if (!rc) rc = md_rattle2(c);
-        if (!rc) rc = md_second_half(c, kT);
+        // (an unobserved step runs the next step's first-half chain in its
+        // own chain launch: the next step's temperature from the same ramp)
+        double kT_next = 0.0;
+        if (!c->need_energy) {
+            const int s1 = step + 1;
+            kT_next = K_BOLTZ * (s1 < c->n_heating
+                                     ? c->first_temp + (c->ext_temp - c->first_temp) * s1 / c->n_heating
+                                     : c->ext_temp);
+        }
+        if (!rc) rc = md_second_half(c, kT, kT_next);
         if (!rc && save) rc = md_wrap(c);
         if (!rc && save) rc = md_fmax(c);
         mark(c, 10);

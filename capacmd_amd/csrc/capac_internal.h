@@ -216,6 +216,9 @@ struct capac_ctx {
     // 0: the force call may leave the pair energies and virial unformed (an
     // MD step whose energies nobody reads); 1 otherwise (the default)
     int need_energy;
+    // the last step's chain launch also ran the next step's first-half chain
+    // and left its centre-of-mass pass to the next step's first pass
+    int com_pending;
     double pot_dev_ofs;         // unused
     // timing
     int timing_on;
@@ -390,7 +393,9 @@ int md_kick(capac_ctx* c);
 int md_rattle1(capac_ctx* c);
 int md_rattle2(capac_ctx* c);
 int md_first_half(capac_ctx* c, double kT);    // chain, scale, half kick, drift
-int md_second_half(capac_ctx* c, double kT);   // kinetic energy, chain, scale, COM removal
+// kinetic energy, chain, scale, COM removal; kT_next > 0: an unobserved step
+// whose next step follows in the same call (see md.hip)
+int md_second_half(capac_ctx* c, double kT, double kT_next = 0.0);
 int md_remove_com_kinetic(capac_ctx* c);   // after the first force call
 int md_wrap(capac_ctx* c);
 int md_fmax(capac_ctx* c);

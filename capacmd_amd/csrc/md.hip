@@ -44,12 +44,27 @@ __global__ void k_kick(int N, int Nm, int fix_metal, double dt_2, const double* 
 // First half of a step in one pass: the thermostat's velocity scale (nvt:
 // every atom, as k_scale_v), then, for every atom that moves, the half kick
 // and the drift of k_kick and k_drift; the same operations in the same order.
+// pre_com: first the previous step's last pass (k_scale_com with its scale
+// in md[29] and the centre-of-mass velocity in md[26..28]), the same
+// operations in the same order.
 __global__ void k_scale_kick_drift(int N, int Nm, int fix_metal, int scale, double dt_2, double dt,
                                    const double* __restrict__ md, const double* __restrict__ f,
                                    const double* __restrict__ inv_mass, double* __restrict__ v,
-                                   double* __restrict__ r, double* __restrict__ old_r) {
+                                   double* __restrict__ r, double* __restrict__ old_r, int pre_com,
+                                   const double* __restrict__ mass) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (pre_com) {
+        const double s = md[29];
+        const bool has_mass = mass[i] > 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t k = (size_t)c * N + i;
+            double vk = v[k] * s;
+            if (has_mass) vk -= md[26 + c];
+            v[k] = vk;
+        }
+    }
     if (scale) {
         const double s = md[22];
 #pragma unroll
@@ -537,6 +552,25 @@ k_nhc_sums(double* __restrict__ md, double dt, double kT, int ndf, const double*
     if (threadIdx.x == 0) nhc_body(md, dt, kT, ndf, s_sums);
 }
 
+// The same, then the next step's first-half chain at kT_next on the same
+// thread (it needs only the chain and the kinetic energy this chain leaves):
+// this step's scale moves to md[29] for the next step's first pass, which
+// also applies this step's centre-of-mass removal (k_scale_kick_drift with
+// pre_com); md[22] holds the next step's scale.
+__global__ void __launch_bounds__(kRedThreads)
+k_nhc_sums_next(double* __restrict__ md, double dt, double kT, double kT_next, int ndf,
+                const double* __restrict__ partial, int nblk, double* __restrict__ sums_out) {
+    __shared__ double s_w[16][16];
+    __shared__ double s_sums[16];
+    reduce_rows_block(partial, nblk, 7, sums_out, s_w, s_sums);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        nhc_body(md, dt, kT, ndf, s_sums);
+        md[29] = md[22];
+        nhc_body(md, dt, kT_next, ndf, NULL);
+    }
+}
+
 // the same without a thermostat (scale 1)
 __global__ void k_com_finish(double* __restrict__ md, const double* __restrict__ sums, int ndf) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -668,15 +702,20 @@ int md_remove_com_kinetic(capac_ctx* c) {
 }
 
 // First half of a step (mpi-main.c:885-925): nvt chain, scale, half kick
-// and drift, in two launches (k_nhc, k_scale_kick_drift)
+// and drift, in two launches (k_nhc, k_scale_kick_drift); one when the
+// previous step's last launch already ran this chain (c->com_pending: its
+// centre-of-mass pass then runs here first)
 int md_first_half(capac_ctx* c, double kT) {
-    if (c->nvt) {
+    const int pre = c->com_pending;
+    c->com_pending = 0;
+    if (c->nvt && !pre) {
         k_nhc<<<1, 64, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, NULL);
         CHECK_HIP(hipGetLastError());
     }
     k_scale_kick_drift<<<grid(c->bc.N, 256), 256, 0, c->stream>>>(c->bc.N, c->bc.Nm, c->fix_metal, c->nvt,
                                                                   0.5 * c->dt, c->dt, c->md, c->f,
-                                                                  c->inv_mass, c->v, c->r, c->old_r);
+                                                                  c->inv_mass, c->v, c->r, c->old_r, pre,
+                                                                  c->mass);
     CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -686,17 +725,27 @@ int md_first_half(capac_ctx* c, double kT) {
 // launches after the sums: one momentum-sum pass, the chain (which also forms
 // the centre-of-mass velocity and the final kinetic energy from the sums)
 // and one pass applying the scale and the removal.
-int md_second_half(capac_ctx* c, double kT) {
+int md_second_half(capac_ctx* c, double kT, double kT_next) {
     if (!c->nvt) return md_remove_com_kinetic(c);
     if (c->n_pending == 0) {
         // nothing else queued (a step that forms no energies): the chain's
-        // launch reduces the sums itself
+        // launch reduces the sums itself.  kT_next > 0 (an unobserved step
+        // with a next step in the same call): that launch also runs the next
+        // step's first-half chain, and the next step's first pass applies
+        // this step's scale and centre-of-mass removal.
         int nblk = grid(c->bc.N, 256);
         if (nblk > c->seg_cap[SEG_MOM]) {
             capac_set_error("internal: reduction of %d blocks exceeds segment %d", nblk, SEG_MOM);
             return CAPAC_ERR_HIP;
         }
         k_mom_partial<<<nblk, 256, 0, c->stream>>>(c->bc.N, c->mass, c->v, partial_seg(c, SEG_MOM));
+        if (kT_next > 0.0) {
+            k_nhc_sums_next<<<1, kRedThreads, 0, c->stream>>>(c->md, c->dt, kT, kT_next, c->ndf,
+                                                              partial_seg(c, SEG_MOM), nblk, c->red + 96);
+            CHECK_HIP(hipGetLastError());
+            c->com_pending = 1;
+            return 0;
+        }
         k_nhc_sums<<<1, kRedThreads, 0, c->stream>>>(c->md, c->dt, kT, c->ndf, partial_seg(c, SEG_MOM), nblk,
                                                      c->red + 96);
         CHECK_HIP(hipGetLastError());
