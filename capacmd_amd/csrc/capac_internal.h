@@ -336,6 +336,9 @@ int sc_forces(capac_ctx* c, double* f_sc = NULL);
 int cpim_rhs(capac_ctx* c);
 int cpim_assemble(capac_ctx* c);
 int cpim_rhs_assemble(capac_ctx* c);       // both, in one launch where the layout allows
+// whether a force call's Sutton-Chen passes run inside its RHS/relay and CPIM
+// force launches (the FP32 path's TILES layout) instead of launches of their own
+bool sc_in_force_launches(const capac_ctx* c);
 int cpim_forces(capac_ctx* c);
 // f_sc (3 x N_m, or NULL): forces the metal-metal launch adds with its own
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc = NULL);

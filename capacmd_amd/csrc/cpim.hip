@@ -15,6 +15,7 @@
 #include "erfc_fast.h"
 #include "relay.h"
 #include "relay_tile.h"
+#include "sc_terms.h"
 
 namespace {
 
@@ -202,12 +203,14 @@ k_metal_unwrap(const double* __restrict__ r, BoxConst bc, double* __restrict__ u
 // The right-hand side and the FP32 relay's TILES layout in one launch
 // (they read the positions only and write different arrays): workgroup 0
 // the metal unwrap, then the n_tiles relay tiles (the longest-running
-// workgroups, so they start early), then one per metal atom for b.
+// workgroups, so they start early), then one per metal atom for b, then the
+// Sutton-Chen densities (N_m / 4 workgroups, when the grid has them).
 __global__ void __launch_bounds__(256)
 k_matrix_terms(const double* __restrict__ r, const double* __restrict__ charge, BoxConst bc,
                const int* __restrict__ n_front, const int* __restrict__ nbr, int stride,
                const double* __restrict__ np_charge, double* __restrict__ b, int Nb, int n_tiles,
-               float4* __restrict__ tiles, double* __restrict__ u) {
+               float4* __restrict__ tiles, double* __restrict__ u, const int* __restrict__ n_back,
+               double* __restrict__ isd) {
     main_stream_priority();
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     __shared__ double s_w[4][4];
@@ -217,6 +220,9 @@ k_matrix_terms(const double* __restrict__ r, const double* __restrict__ charge, 
         metal_unwrap_block(r, bc, u, s_lo, s_hi);
     } else if (blk <= n_tiles) {
         assemble_tile_block(r, bc, Nb, tiles, blk - 1);
+    } else if (blk > n_tiles + bc.Nm) {
+        // the Sutton-Chen densities, four metal atoms per workgroup
+        sc_dens_block(blk - 1 - n_tiles - bc.Nm, r, bc, n_back, nbr, stride, isd);
     } else {
         load_erfcx_table(s_erfcx);
         __syncthreads();
@@ -868,12 +874,19 @@ k_cpim_terms(const double* __restrict__ r, const double* __restrict__ charge, Bo
              const double* __restrict__ x, int Nb, double* __restrict__ P, double* __restrict__ part_mm,
              int n_tiles, const int* __restrict__ n_front, const int* __restrict__ n_back,
              const int* __restrict__ nbr, int stride, const int* __restrict__ tail_list, double* __restrict__ f,
-             double* __restrict__ part_mw, int n_metal, int n_atom) {
+             double* __restrict__ part_mw, int n_metal, int n_atom, int n_sc, const double* __restrict__ isd,
+             double* __restrict__ f_sc, double* __restrict__ part_sc) {
     __shared__ double s_erfcx[kErfcxPieces * kErfcxTerms];
     __shared__ MMShared sh;
+    int b = blockIdx.x;
+    if (b >= n_tiles + n_metal + n_atom) {
+        // the Sutton-Chen forces of four metal atoms, written to f_sc
+        sc_force_block(b - n_tiles - n_metal - n_atom, r, bc, n_back, nbr, stride, isd, f_sc, bc.Nm, 0, part_sc,
+                       reinterpret_cast<double (*)[10]>(&sh.red[0][0]));
+        return;
+    }
     load_erfcx_table(s_erfcx);
     __syncthreads();
-    int b = blockIdx.x;
     if (b < n_tiles) {
         mm_tile_block(r, bc, x, Nb, P, part_mm, b, s_erfcx, sh);
         return;
@@ -973,9 +986,10 @@ int cpim_rhs_assemble(capac_ctx* c) {
         return CAPAC_ERR_ARG;
     }
     const int n_tiles = (int)tile_count(c->relay_nb);
-    k_matrix_terms<<<1 + n_tiles + Nm, 256, 0, c->stream>>>(c->r, c->charge, c->bc, c->n_front, c->nbr,
-                                                            c->nb_stride, c->np_charge, c->b, c->relay_nb, n_tiles,
-                                                            reinterpret_cast<float4*>(c->relay32), c->metal_u);
+    const int n_sc = (Nm + 3) / 4;          // the Sutton-Chen densities ride along (sc_in_force_launches)
+    k_matrix_terms<<<1 + n_tiles + Nm + n_sc, 256, 0, c->stream>>>(
+        c->r, c->charge, c->bc, c->n_front, c->nbr, c->nb_stride, c->np_charge, c->b, c->relay_nb, n_tiles,
+        reinterpret_cast<float4*>(c->relay32), c->metal_u, c->n_back, c->isd);
     CHECK_HIP(hipGetLastError());
     return assemble_finish(c);
 }
@@ -1054,22 +1068,30 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 // the metal-metal terms for x: the tile launch and the per-block sums
 // (adding extra, 3 x N_m or NULL, with them); reduce: queue the energies'
 // reduction (red[48..61))
-// mw: the metal-water terms in the same launch
-static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce, bool mw) {
+// mw: the metal-water terms in the same launch; sc: the Sutton-Chen forces
+// too (into extra, which the block sums then add)
+static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce, bool mw,
+                          bool sc = false) {
     const int Nm = c->bc.Nm, N = c->bc.N;
     const int Nb = tile_blocks(Nm);
     const int n_tiles = (int)tile_count(Nb);
     const int n_metal = mw ? Nm : 0;
     const int n_atom = mw && N > Nm ? (int)mw_atom_blocks(N - Nm) : 0;
+    const int n_sc = sc && extra ? (Nm + 3) / 4 : 0;
     double* part = partial_seg(c, SEG_MM);
-    k_cpim_terms<<<n_tiles + n_metal + n_atom, kMMWaves * 64, 0, c->stream>>>(
+    k_cpim_terms<<<n_tiles + n_metal + n_atom + n_sc, kMMWaves * 64, 0, c->stream>>>(
         c->r, c->charge, c->bc, x, Nb, c->mm_part, part, n_tiles, c->n_front, c->n_back, c->nbr, c->nb_stride,
-        c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom);
+        c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom, n_sc, c->isd, const_cast<double*>(extra),
+        partial_seg(c, SEG_SC));
     k_cpim_mm_sum<<<Nb, 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, extra, c->f, part, n_tiles);
     CHECK_HIP(hipGetLastError());
     if (!reduce) return 0;
     if (mw) {
         int rc = reduce_energies(c, SEG_MW, Nm, 11, 32);
+        if (rc) return rc;
+    }
+    if (n_sc) {
+        int rc = reduce_energies(c, SEG_SC, n_sc, 10, 16);
         if (rc) return rc;
     }
     return reduce_energies(c, SEG_MM, n_tiles + Nb, 13, 48);
@@ -1079,7 +1101,14 @@ static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bo
 // red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
 int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
     if (c->bc.Nm == 0) return 0;
-    return cpim_mm_launch(c, x, f_sc, true, true);
+    return cpim_mm_launch(c, x, f_sc, true, true, sc_in_force_launches(c));
+}
+
+// the force call's Sutton-Chen passes ride in its RHS/relay launch
+// (densities) and CPIM force launch (forces): the FP32 path's TILES layout
+bool sc_in_force_launches(const capac_ctx* c) {
+    return c->bc.Nm > 0 && c->n_mat > 0 && c->precision == CAPAC_PREC_FP32_REFINED &&
+           c->relay_layout == CAPAC_LAYOUT_TILES;
 }
 
 // (in a force call the Sutton-Chen force waits in f_sc)
