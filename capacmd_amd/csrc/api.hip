@@ -1365,7 +1365,10 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         const bool save = n_save > 0 && step % n_save == 0;
         const bool last = step + 1 == first_step + n_steps;
         c->need_energy = (save || last || c->timing_on || !c->n_mat) ? 1 : 0;
+        c->hist_dst = c->n_mat ? c->x_hist[2] : NULL;   // solver_push_history's next newest buffer
+        c->hist_copied = NULL;
         rc = force_compute(c, true);
+        c->hist_dst = NULL;
         if (c->n_mat) {
             // the solve's synchronisation brought the RATTLE flags back (this
             // step's RATTLE_1 and the previous step's RATTLE_2; they stay set

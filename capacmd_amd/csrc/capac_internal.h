@@ -219,6 +219,11 @@ struct capac_ctx {
     // the last step's chain launch also ran the next step's first-half chain
     // and left its centre-of-mass pass to the next step's first pass
     int com_pending;
+    // an MD step's force call copies its solution to hist_dst (the buffer
+    // solver_push_history turns into the newest stored solution) in the CPIM
+    // force launch, and records that buffer in hist_copied
+    double* hist_dst;
+    double* hist_copied;
     double pot_dev_ofs;         // unused
     // timing
     int timing_on;

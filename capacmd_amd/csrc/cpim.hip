@@ -911,8 +911,12 @@ k_cpim_terms(const double* __restrict__ r, const double* __restrict__ charge, Bo
 __global__ void __launch_bounds__(256)
 k_cpim_mm_sum(const double* __restrict__ r, BoxConst bc, const double* __restrict__ x, int Nb,
               const double* __restrict__ P, const double* __restrict__ extra, double* __restrict__ f,
-              double* __restrict__ partial, int slot0) {
+              double* __restrict__ partial, int slot0, double* __restrict__ hist, int n_mat) {
     const int N = bc.N, Nm = bc.Nm, B = blockIdx.x;
+    // (hist non-NULL: the solution's copy for the warm-start predictor's
+    // history, in the same launch)
+    if (hist)
+        for (int k = B * blockDim.x + threadIdx.x; k < n_mat; k += gridDim.x * blockDim.x) hist[k] = x[k];
     if (threadIdx.x < kMMBlk) {
         const int c = threadIdx.x >> 6, l = threadIdx.x & 63, i = B * kTileEdge + l;
         // (the loads of a chunk issued together, the sum in order)
@@ -1083,7 +1087,14 @@ static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bo
         c->r, c->charge, c->bc, x, Nb, c->mm_part, part, n_tiles, c->n_front, c->n_back, c->nbr, c->nb_stride,
         c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom, n_sc, c->isd, const_cast<double*>(extra),
         partial_seg(c, SEG_SC));
-    k_cpim_mm_sum<<<Nb, 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, extra, c->f, part, n_tiles);
+    // a force call of an MD step also copies x into the predictor's history
+    double* hist = reduce && mw && x == c->x ? c->hist_dst : NULL;
+    if (hist) {
+        c->hist_copied = hist;          // (asked once per step)
+        c->hist_dst = NULL;
+    }
+    k_cpim_mm_sum<<<Nb, 256, 0, c->stream>>>(c->r, c->bc, x, Nb, c->mm_part, extra, c->f, part, n_tiles, hist,
+                                             c->n_mat);
     CHECK_HIP(hipGetLastError());
     if (!reduce) return 0;
     if (mw) {

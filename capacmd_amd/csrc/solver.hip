@@ -2037,7 +2037,12 @@ void solver_push_history(capac_ctx* c) {
     c->x_hist[2] = c->x_hist[1];
     c->x_hist[1] = c->x_hist[0];
     c->x_hist[0] = oldest;
-    (void)hipMemcpyAsync(oldest, c->x, sizeof(double) * c->n_mat, hipMemcpyDeviceToDevice, c->stream);
+    // (the force call's CPIM force launch has copied the solution already
+    // when it was asked to, into this same buffer)
+    if (c->hist_copied != oldest)
+        (void)hipMemcpyAsync(oldest, c->x, sizeof(double) * c->n_mat, hipMemcpyDeviceToDevice, c->stream);
+    c->hist_dst = NULL;
+    c->hist_copied = NULL;
     if (c->pred_count < 3) ++c->pred_count;
 }
 
