@@ -251,14 +251,14 @@ __device__ __forceinline__ double water_erf_vdw(const PairLJ& lj, double inv_r, 
 // no LJ with any type it meets here, so the LJ terms (exact zeros) are left
 // out.  A funct-4 pair type (lj.erf4) takes water_erf_vdw instead (no
 // Coulomb term, as the reference's compute_erf_vdw).
-template <bool Energy>
+template <bool Energy, bool Erf4>
 __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, const PairLJ& lj, double kqi, double qj,
                                            double na2, double dx, double dy, double dz, double r2, PairOut& o,
                                            const double* __restrict__ erfcx_coef) {
     const double inv_r = rsqrt_pos(r2);
     const double rij = r2 * inv_r;
     const double inv_r2 = inv_r * inv_r;
-    if (lj.erf4 != 0.0) {
+    if (Erf4 && lj.erf4 != 0.0) {
         const double fs = water_erf_vdw<Energy>(lj, inv_r, inv_r2, rij, o, erfcx_coef);
         o.fx = fma(-fs, dx, o.fx); o.fy = fma(-fs, dy, o.fy); o.fz = fma(-fs, dz, o.fz);
         if (Energy) {
@@ -310,7 +310,7 @@ __device__ __forceinline__ void water_term(bool lj_row, const BoxConst& bc, cons
 // with two rounds in flight the Au2057 launch beside the solve took 0.79 ms
 // per MD step, with three 0.69 ms).  Unrolled three times, so that each
 // round's data keep their registers.
-template <bool Energy>
+template <bool Energy, bool Erf4>
 __device__ __forceinline__ void water_segment(bool lj_row, const double2* __restrict__ xyzq,
                                               const int* __restrict__ type, const int* __restrict__ row, int nf,
                                               const int* __restrict__ back, int cnt,
@@ -324,7 +324,7 @@ __device__ __forceinline__ void water_segment(bool lj_row, const double2* __rest
         pbc12(bc, dx, dy, dz);
         const double r2 = dist2_exact(dx, dy, dz);
         if (r2 < bc.rc2)                   // rows reach rc + skin
-            water_term<Energy>(lj_row, bc, s_lj[ti + t], kqi, b.y, na2, dx, dy, dz, r2, o, s_erfcx);
+            water_term<Energy, Erf4>(lj_row, bc, s_lj[ti + t], kqi, b.y, na2, dx, dy, dz, r2, o, s_erfcx);
     };
     constexpr int W = CAPAC_WAVE;
     int k = lane;
@@ -413,8 +413,14 @@ k_pair_water(const double2* __restrict__ xyzq, const int* __restrict__ type, con
         const int* row = nbr + (size_t)i * stride;
         const int nf = n_front[i];
         const int nb = all_rows && !(i < bc.Nm && metal_inert) ? n_back[i] : 0;
-        water_segment<Energy>(lj_row, xyzq, type, row, nf, row + stride - nb, nf + nb, i, lane, bc, s_lj, ti,
-                              xi, yi, zi, K_FQQ * qi, na2, o, s_erfcx);
+        // a non-metal atom without metal neighbours meets funct-1 types only:
+        // its loop leaves out the funct-4 test
+        if (i >= bc.Nm && nb == 0)
+            water_segment<Energy, false>(lj_row, xyzq, type, row, nf, row, nf, i, lane, bc, s_lj, ti, xi, yi, zi,
+                                         K_FQQ * qi, na2, o, s_erfcx);
+        else
+            water_segment<Energy, true>(lj_row, xyzq, type, row, nf, row + stride - nb, nf + nb, i, lane, bc, s_lj,
+                                        ti, xi, yi, zi, K_FQQ * qi, na2, o, s_erfcx);
         double fx = wave_sum(o.fx), fy = wave_sum(o.fy), fz = wave_sum(o.fz);
         if (lane == 0) {
             if (field) {
