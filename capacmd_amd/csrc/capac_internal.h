@@ -346,7 +346,8 @@ int cpim_rhs_assemble(capac_ctx* c);       // both, in one launch where the layo
 bool sc_in_force_launches(const capac_ctx* c);
 int cpim_forces(capac_ctx* c);
 // f_sc (3 x N_m, or NULL): forces the metal-metal launch adds with its own
-int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc = NULL);
+// (on the FP32 path's TILES layout the same launch writes them first)
+int cpim_forces_x(capac_ctx* c, const double* x, double* f_sc = NULL);
 int cpim_dense_expand(capac_ctx* c, double* dense_dev);
 // out = b - A x (b non-NULL) or A x, with the FP64 relay recomputed on the fly
 // (no stored matrix); part[g] = (out, out) over the rows of row-group g,

@@ -1074,7 +1074,7 @@ int cpim_apply64(capac_ctx* c, const double* x, const double* b, double* out, do
 // reduction (red[48..61))
 // mw: the metal-water terms in the same launch; sc: the Sutton-Chen forces
 // too (into extra, which the block sums then add)
-static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bool reduce, bool mw,
+static int cpim_mm_launch(capac_ctx* c, const double* x, double* extra, bool reduce, bool mw,
                           bool sc = false) {
     const int Nm = c->bc.Nm, N = c->bc.N;
     const int Nb = tile_blocks(Nm);
@@ -1085,7 +1085,7 @@ static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bo
     double* part = partial_seg(c, SEG_MM);
     k_cpim_terms<<<n_tiles + n_metal + n_atom + n_sc, kMMWaves * 64, 0, c->stream>>>(
         c->r, c->charge, c->bc, x, Nb, c->mm_part, part, n_tiles, c->n_front, c->n_back, c->nbr, c->nb_stride,
-        c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom, n_sc, c->isd, const_cast<double*>(extra),
+        c->tail_list, c->f, partial_seg(c, SEG_MW), n_metal, n_atom, n_sc, c->isd, extra,
         partial_seg(c, SEG_SC));
     // a force call of an MD step also copies x into the predictor's history
     double* hist = reduce && mw && x == c->x ? c->hist_dst : NULL;
@@ -1110,7 +1110,7 @@ static int cpim_mm_launch(capac_ctx* c, const double* x, const double* extra, bo
 
 // adds the CPIM forces for x (device) to c->f;
 // red[32..43) = pot10, pot11, vir; red[48..61) = pot9, pot12, pot13, pot14, vir
-int cpim_forces_x(capac_ctx* c, const double* x, const double* f_sc) {
+int cpim_forces_x(capac_ctx* c, const double* x, double* f_sc) {
     if (c->bc.Nm == 0) return 0;
     return cpim_mm_launch(c, x, f_sc, true, true, sc_in_force_launches(c));
 }
