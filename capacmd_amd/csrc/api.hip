@@ -1384,8 +1384,8 @@ extern "C" int capac_md_run(capac_ctx* c, int first_step, int n_steps, int n_sav
         if (rc) return rc;
         solver_push_history(c);
         mark(c, 9);
-        if (!rc) rc = md_kick(c);
-        if (!rc) rc = md_rattle2(c);
+        if (!rc) rc = md_rattle2(c, true);     // the half kick, then RATTLE_2
+
         // (an unobserved step runs the next step's first-half chain in its
         // own chain launch: the next step's temperature from the same ramp)
         double kT_next = 0.0;

@@ -400,7 +400,7 @@ int fit_frame_loss_grad(capac_ctx* c, const double* f_ref, double e_ref, double 
 // md.hip
 int md_kick(capac_ctx* c);
 int md_rattle1(capac_ctx* c);
-int md_rattle2(capac_ctx* c);
+int md_rattle2(capac_ctx* c, bool with_kick = false);   // with_kick: md_kick first (fused with constraints)
 int md_first_half(capac_ctx* c, double kT);    // chain, scale, half kick, drift
 // kinetic energy, chain, scale, COM removal; kT_next > 0: an unobserved step
 // whose next step follows in the same call (see md.hip)

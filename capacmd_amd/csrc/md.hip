@@ -30,15 +30,22 @@ __device__ double block_sum1024(double v, double* s_w) {
     return t;
 }
 
-__global__ void k_kick(int N, int Nm, int fix_metal, double dt_2, const double* __restrict__ f,
-                       const double* __restrict__ inv_mass, double* __restrict__ v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+// the half kick of atom i (mpi-main.c:944-949); fixed metal atoms stay
+__device__ __forceinline__ void kick_atom(int i, int N, int Nm, int fix_metal, double dt_2,
+                                          const double* __restrict__ f, const double* __restrict__ inv_mass,
+                                          double* __restrict__ v) {
     if (fix_metal == 1 && i < Nm) return;
     double im = inv_mass[i];
     v[i] += dt_2 * f[i] * im;
     v[N + i] += dt_2 * f[N + i] * im;
     v[2 * N + i] += dt_2 * f[2 * N + i] * im;
+}
+
+__global__ void k_kick(int N, int Nm, int fix_metal, double dt_2, const double* __restrict__ f,
+                       const double* __restrict__ inv_mass, double* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    kick_atom(i, N, Nm, fix_metal, dt_2, f, inv_mass, v);
 }
 
 // First half of a step in one pass: the thermostat's velocity scale (nvt:
@@ -292,7 +299,8 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
           const int* __restrict__ mol_first, const int* __restrict__ mol_last,
           const int* __restrict__ ci, const int* __restrict__ cj, const double* __restrict__ cr,
           const double* __restrict__ inv_mass, const double* __restrict__ r, double* __restrict__ v,
-          int* __restrict__ status, double* __restrict__ partial) {
+          int* __restrict__ status, double* __restrict__ partial, int Nm, int fix_metal,
+          const double* __restrict__ f) {
     __shared__ double s_red[kRattleThreads / 64][9];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,6 +308,10 @@ k_rattle2(int n_mols, int N, BoxConst bc, double dt, const int* __restrict__ mol
     const int c0m = m < n_mols ? mol_cstr[m] : 0, c1m = m < n_mols ? mol_cstr[m + 1] : 0;
     const int f0 = m < n_mols ? mol_first[m] : 0;
     const int nat = m < n_mols ? mol_last[m] - f0 + 1 : 0, ncm = c1m - c0m;
+    // f non-NULL: the step's half kick of the molecule's atoms first (every
+    // atom is in one molecule), the same operations as k_kick
+    if (f)
+        for (int a = f0; a < f0 + nat; ++a) kick_atom(a, N, Nm, fix_metal, 0.5 * dt, f, inv_mass, v);
     if (m < n_mols && is_rigid3(f0, nat, c0m, ncm, ci, cj)) {
         const double half_dt = 0.5 * dt;
         Site s0, s1, s2;
@@ -661,15 +673,21 @@ int md_rattle1(capac_ctx* c) {
     return 0;
 }
 
-// red[64..73) = virial of the velocity constraints
-int md_rattle2(capac_ctx* c) {
+// red[64..73) = virial of the velocity constraints; with_kick: the half
+// kick first, in the same launch (with constraints; else its own launch)
+int md_rattle2(capac_ctx* c, bool with_kick) {
     if (c->n_cstr == 0) {
+        if (with_kick) {
+            int rc = md_kick(c);
+            if (rc) return rc;
+        }
         return fill_zero_d(c, c->red + 64, 16);
     }
     int nblk = grid(c->bc.n_mols, kRattleThreads);
     k_rattle2<<<nblk, kRattleThreads, 0, c->stream>>>(c->bc.n_mols, c->bc.N, c->bc, c->dt, c->mol_cstr,
                                            c->mol_first, c->mol_last, c->cstr_i, c->cstr_j, c->cstr_r, c->inv_mass, c->r, c->v,
-                                           c->md_status, partial_seg(c, SEG_RATTLE));
+                                           c->md_status, partial_seg(c, SEG_RATTLE), c->bc.Nm, c->fix_metal,
+                                           with_kick ? c->f : NULL);
     CHECK_HIP(hipGetLastError());
     return reduce_energies(c, SEG_RATTLE, nblk, 9, 64);
 }
